@@ -1,0 +1,141 @@
+/* prandtl_amd.h -- C ABI of the MI355X (gfx950) DGSEM right-hand-side and RK-stage library.
+ *
+ * This is the drop-in boundary beneath the reference's operator API: each entry
+ * point names the reference interface it replaces (paths relative to the Prandtl
+ * repository root).  Plain pointers and sizes only; no exceptions or aborts cross
+ * it.  Return value 0 is success, negative values are errors, and
+ * prandtl_last_error() describes the last error of the calling thread.
+ *
+ * Device buffers are caller-owned and use the reference's byNODES layout
+ * (src/Simulation/Simulation.hpp:64):
+ *     u[v * NE * n + e * n + node],  node = i + Np * (j + Np * k),  n = Np^dim,
+ * with v over the dim + 2 conserved variables (rho, rho u, [rho v, [rho w,]] E).
+ * The handle owns its scratch (gradient/viscous-flux buffers, RK registers).
+ * One handle is not thread-safe (like the reference's DGSEMOperator, whose Mult
+ * mutates internal state); distinct handles are independent.  Every call runs on
+ * the given HIP stream (NULL = the default stream).
+ */
+#ifndef PRANDTL_AMD_H
+#define PRANDTL_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct prandtl_op_s *prandtl_op;
+typedef void *prandtl_stream; /* a hipStream_t */
+
+enum {
+    PRANDTL_OK = 0,
+    PRANDTL_ERR_ARG = -1,         /* invalid argument */
+    PRANDTL_ERR_UNSUPPORTED = -2, /* valid request outside what this build implements */
+    PRANDTL_ERR_HIP = -3,         /* HIP runtime error */
+    PRANDTL_ERR_MESH = -4,        /* inconsistent mesh description */
+    PRANDTL_ERR_NAN = -5          /* non-finite state detected */
+};
+
+/* Surface (Riemann) flux plug-ins -- ref: NumericalFlux::ComputeFaceFlux,
+ * src/RiemannSolvers/NumericalFlux.hpp:25.  CHANDRASHEKAR is the reference's
+ * ChandrashekarFlux::ComputeFaceFlux (ChandrashekarFlux.cpp:85-161). */
+enum { PRANDTL_SURF_CHANDRASHEKAR = 0, PRANDTL_SURF_EC = 1, PRANDTL_SURF_RUSANOV = 2 };
+enum { PRANDTL_FP64 = 0, PRANDTL_MIXED = 1 };
+/* Reproduce the reference as written instead of the intended formula:
+ *   A1: the zeta-direction flux store H_inviscid(id2).SetCol(k, g) at DGSEMIntegrator.cpp:702;
+ *   A2: the viscous volume flux read at the stale node id1 (DGSEMIntegrator.cpp:620);
+ *   A3: the 3D dissipation term "rho_mean * dw + w_mean" at ChandrashekarFlux.cpp:131. */
+enum { PRANDTL_BUG_A1 = 1, PRANDTL_BUG_A2 = 2, PRANDTL_BUG_A3 = 4 };
+/* ODE solvers -- ref: the MFEM ODESolver chosen at src/Simulation/Simulation.cpp:177-206
+ * and stepped at :1010.  LSRK45 is the Carpenter-Kennedy 2N-storage RK4(3)5. */
+enum { PRANDTL_SSPRK3 = 0, PRANDTL_LSRK45 = 1 };
+
+/* Conforming affine mesh with aligned faces.
+ *   neighbors[e * 2*dim + 2*d + s]: element across local face (d, s) of e,
+ *       s = 0 on the xi_d = 0 side, s = 1 on the xi_d = 1 side;
+ *   geometry[e * (1 + dim*dim)]: det J, then adj J row-major (MFEM Weight() and
+ *       AdjugateJacobian(), DGSEMIntegrator.cpp:254-256).
+ * Node (.., Np-1 in d, ..) of e coincides with node (.., 0 in d, ..) of its
+ * +d neighbour.  Both arrays are host pointers, copied at create time. */
+typedef struct {
+    int dim;
+    int order;
+    int num_elements;
+    const int *neighbors;
+    const double *geometry;
+} prandtl_mesh_desc;
+
+/* ref: src/Physics/Physics.hpp:10-38 and NavierStokesFlux(dim, gamma, Pr, mu, ...),
+ * src/Fluxes/NavierStokesFlux.hpp:20-25 */
+typedef struct {
+    double gamma;
+    double Pr;
+    double mu;
+    double R_gas;
+    int viscous;    /* 0: Euler; 1: Navier-Stokes with BR1 lifting (PARABOLIC) */
+    int sutherland; /* 1: Sutherland viscosity (SUTHERLAND) */
+} prandtl_physics;
+
+typedef struct {
+    int surface_flux; /* PRANDTL_SURF_* */
+    int precision;    /* PRANDTL_FP64 | PRANDTL_MIXED */
+    int bug_compat;   /* PRANDTL_BUG_* bitmask */
+    int reserved;
+} prandtl_opts;
+
+/* Build an operator -- replaces the DGSEMOperator constructor,
+ * src/Operators/DGSEMOperator.hpp:87-99 (with its DGSEMIntegrator,
+ * DGSEMIntegrator.cpp:8-149, and the LiftingBR1 set-up, LiftingScheme.cpp:6-42). */
+int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys, const prandtl_opts *opts,
+                      prandtl_op *out);
+
+/* ref: DGSEMOperator::~DGSEMOperator, DGSEMOperator.cpp:52-58 */
+void prandtl_op_destroy(prandtl_op op);
+
+/* Number of doubles in a state vector -- ref: vfes->GetTrueVSize(), DGSEMOperator.cpp:18 */
+size_t prandtl_op_size(prandtl_op op);
+
+/* dudt = L(u), dudt fully overwritten -- ref: DGSEMOperator::Mult(const Vector &u,
+ * Vector &dudt) const, DGSEMOperator.cpp:537-591 (mfem::TimeDependentOperator::Mult). */
+int prandtl_op_mult(prandtl_op op, const double *u, double *dudt, prandtl_stream stream);
+
+/* Mult plus the primitive gradients the reference leaves in its dudx/dudy/dudz grid
+ * functions (DGSEMOperator.cpp:568-570).  grad holds dim state-sized blocks:
+ * grad[c * size + v * NE * n + e * n + node] = d(rho, vel, p)_v / dx_c.
+ * Navier-Stokes operators only. */
+int prandtl_op_mult_grad(prandtl_op op, const double *u, double *dudt, double *grad, prandtl_stream stream);
+
+/* Max characteristic speed max(|V| + c) of the last Mult (Euler and NS) --
+ * ref: DGSEMOperator::GetMaxCharSpeed, DGSEMOperator.hpp:106-109.  Synchronises. */
+int prandtl_op_get_max_char_speed(prandtl_op op, double *lambda_host);
+
+/* max(|V| + c) over the nodes of u (the same quantity, computed from u alone). */
+int prandtl_op_max_char_speed(prandtl_op op, const double *u, double *lambda_host, prandtl_stream stream);
+
+/* One step of an ODE solver, u advanced in place, *t += dt -- ref: ode_solver->Step(*sol,
+ * t, dt_real), src/Simulation/Simulation.cpp:1010.  The stage update is fused into the
+ * RHS kernel's epilogue. */
+int prandtl_rk_step(prandtl_op op, int scheme, double *u, double *t, double dt, prandtl_stream stream);
+
+/* nsteps fixed-dt steps (the reference's time loop without output,
+ * Simulation.cpp:995-1125).  Checks for a non-finite state every nan_check steps
+ * (0 = never), like the density scan at Simulation.cpp:1025-1035; returns
+ * PRANDTL_ERR_NAN if one is found. */
+int prandtl_rk_run(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
+                   prandtl_stream stream);
+
+/* Per-kernel timing of one RHS evaluation, with HIP events on `stream`: ms[0] = average
+ * duration of the lifting/viscous-flux pass (0 for Euler), ms[1] = the flux-differencing
+ * pass, each launched `iters` times back to back on input u.  epilogue 0 times the Mult
+ * kernels, 1 the LSRK4(5) stage kernels (writing only handle scratch).  Instrumentation
+ * in place of the reference's run timer (src/Simulation/Simulation.cpp:910-914,1162-1171). */
+int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, double *ms, prandtl_stream stream);
+
+/* Description of the last error on this thread. */
+const char *prandtl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRANDTL_AMD_H */

@@ -1,0 +1,931 @@
+// prandtl_oracle.cpp -- CPU oracle for the DGSEM right-hand side and RK stage.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a line-traceable C++17 restatement of
+// the reference solver's arithmetic (Prandtl, MFEM-based).  It is the checker
+// against which the HIP path is compared, and the CPU baseline that bench.py
+// times beside the GPU.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it; the product library never links or calls it.
+//
+// Every function cites the reference file:line it restates.  Paths are
+// relative to the reference repository root.
+//
+// MFEM-side arithmetic (the GLL rule, CalcOrtho, element Jacobians, the ODE
+// solver) is not vendored in the reference, so it is restated here from its
+// closed forms:
+//   * GLL nodes on [0,1]: 0, 1 and the roots of P'_N(2x-1); weights
+//     1/(N(N+1) P_N(2x-1)^2).
+//   * affine element x = x0 + A xi on [0,1]^d: Weight() = det A,
+//     AdjugateJacobian() = adj A, and CalcOrtho of the face xi_d = 1 is row d
+//     of adj A (the face area vector, outward).
+//   * RK3SSPSolver in Shu-Osher form; the 5-stage 2N-storage RK4(3)5 of
+//     Carpenter & Kennedy (NASA TM-109112, 1994).
+//
+// Parity pins for every function live in tests/test_oracle_*.py (closed forms,
+// invariants, brute force); see the header of each test file.
+//
+// Build: oracle/Makefile (g++ -O3 -fopenmp, portable x86-64-v3 code).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int MAXNP = 10;
+constexpr int MAXEQ = 5;
+
+// ---------------------------------------------------------------------------
+// Physics constants -- ref: src/Physics/Physics.hpp:10-38
+// ---------------------------------------------------------------------------
+struct Phys {
+    int dim, neq;
+    double gamma, gammaM1, gammaM1Inverse, Pr, PrInverse, mu, R_gas;
+    double mu_bulk = 2.0 / 3.0;   // Physics.hpp:34
+    double mu0 = 1.716e-5;        // Physics.hpp:35
+    double T0 = 273.15;           // Physics.hpp:36
+    double Ts = 110.4;            // Physics.hpp:37
+    int viscous, sutherland, surface_flux, bug;
+};
+
+// ---------------------------------------------------------------------------
+// ComputeLogMean -- ref: src/BasicOperations/BasicOperations.cpp:6-12
+// ---------------------------------------------------------------------------
+inline double LogMean(double x, double y, double eps = 1e-4) {
+    double xi = y / x;
+    double u = (xi * (xi - 2.0) + 1.0) / (xi * (xi + 2.0) + 1);
+    double ln_mean = (u < eps) ? (x + y) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + 15.0 * u)))
+                               : (y - x) / std::log(xi);
+    return ln_mean;
+}
+
+// ComputeSoundSpeed -- ref: BasicOperations.cpp:53-56 (b = 0)
+inline double SoundSpeed(double p, double rho, double gamma) {
+    return std::sqrt(gamma * p / (1.0 - 0.0 * rho) / rho);
+}
+
+// ---------------------------------------------------------------------------
+// ChandrashekarFlux::ComputeVolumeFlux -- ref: src/RiemannSolvers/ChandrashekarFlux.cpp:11-83
+// ---------------------------------------------------------------------------
+double VolumeFlux(const Phys &P, const double *state1, const double *state2,
+                  const double *metric1, const double *metric2, double *F_tilde) {
+    const int dim = P.dim, neq = P.neq;
+    double metric[3], mom[3];
+    for (int d = 0; d < dim; d++) metric[d] = (metric1[d] + metric2[d]) * 0.5;  // ComputeMean :15, BasicOperations.cpp:24-29
+
+    double rho1 = state1[0], rho2 = state2[0];
+    double rho_ln = LogMean(rho1, rho2);
+    double u1 = state1[1] / rho1, u2 = state2[1] / rho2;
+    double u_mean = 0.5 * (u1 + u2);
+    double V_sq1 = u1 * u1, V_sq2 = u2 * u2;
+    double Vn = u_mean * metric[0];
+    mom[0] = rho_ln * u_mean;
+    double h_hat = -0.25 * (u1 * u1 + u2 * u2) + u_mean * u_mean;
+    if (dim > 1) {
+        double v1 = state1[2] / rho1, v2 = state2[2] / rho2;
+        double v_mean = 0.5 * (v1 + v2);
+        V_sq1 += v1 * v1;
+        V_sq2 += v2 * v2;
+        Vn += v_mean * metric[1];
+        mom[1] = rho_ln * v_mean;
+        h_hat += -0.25 * (v1 * v1 + v2 * v2) + v_mean * v_mean;
+        if (dim > 2) {
+            double w1 = state1[3] / rho1, w2 = state2[3] / rho2;
+            double w_mean = 0.5 * (w1 + w2);
+            V_sq1 += w1 * w1;
+            V_sq2 += w2 * w2;
+            Vn += w_mean * metric[2];
+            mom[2] = rho_ln * w_mean;
+            h_hat += -0.25 * (w1 * w1 + w2 * w2) + w_mean * w_mean;
+        }
+    }
+    double p1 = P.gammaM1 * (state1[neq - 1] - 0.5 * rho1 * V_sq1);   // :58
+    double p2 = P.gammaM1 * (state2[neq - 1] - 0.5 * rho2 * V_sq2);   // :59
+    double V_mag1 = std::sqrt(V_sq1), V_mag2 = std::sqrt(V_sq2);
+    double lambda_max = V_mag1 + SoundSpeed(p1, rho1, P.gamma);
+    lambda_max = std::max(lambda_max, V_mag2 + SoundSpeed(p2, rho2, P.gamma));
+    double beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;           // :67-68
+    double beta_ln = LogMean(beta1, beta2);
+    double p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);              // :71
+    h_hat += 0.5 / beta_ln * P.gammaM1Inverse + p_hat / rho_ln;        // :73
+    F_tilde[0] = rho_ln * Vn;
+    for (int d = 0; d < dim; d++) F_tilde[1 + d] = Vn * mom[d] + p_hat * metric[d];
+    F_tilde[1 + dim] = rho_ln * Vn * h_hat;
+    return lambda_max;
+}
+
+// ---------------------------------------------------------------------------
+// ChandrashekarFlux::ComputeFaceFlux -- ref: ChandrashekarFlux.cpp:85-161
+// bug & 4 reproduces the 3D dissipation typo at :131 ("rho_mean*dw + w_mean").
+// ---------------------------------------------------------------------------
+double FaceFluxChandrashekar(const Phys &P, const double *state1, const double *state2,
+                             const double *nor, double *flux) {
+    const int dim = P.dim, neq = P.neq;
+    double mom[3];
+    double nor_mag = 0.0;
+    for (int d = 0; d < dim; d++) nor_mag += nor[d] * nor[d];
+    nor_mag = std::sqrt(nor_mag);                                      // :88 Norml2
+
+    double rho1 = state1[0], rho2 = state2[0];
+    double rho_mean = 0.5 * (rho1 + rho2);
+    double rho_ln = LogMean(rho1, rho2);
+    double drho = rho2 - rho1;
+    double u1 = state1[1] / rho1, u2 = state2[1] / rho2;
+    double u_mean = 0.5 * (u1 + u2);
+    double du = u2 - u1;
+    double V_sq1 = u1 * u1, V_sq2 = u2 * u2;
+    double Vn = u_mean * nor[0];
+    mom[0] = rho_ln * u_mean;
+    double h_hat = -0.25 * (u1 * u1 + u2 * u2) + u_mean * u_mean;
+    double diss = 0.5 * drho * u1 * u2 + rho_mean * du * u_mean;       // :105
+    if (dim > 1) {
+        double v1 = state1[2] / rho1, v2 = state2[2] / rho2;
+        double v_mean = 0.5 * (v1 + v2);
+        double dv = v2 - v1;
+        V_sq1 += v1 * v1;
+        V_sq2 += v2 * v2;
+        Vn += v_mean * nor[1];
+        mom[1] = rho_ln * v_mean;
+        h_hat += -0.25 * (v1 * v1 + v2 * v2) + v_mean * v_mean;
+        diss += 0.5 * drho * v1 * v2 + rho_mean * dv * v_mean;        // :118
+        if (dim > 2) {
+            double w1 = state1[3] / rho1, w2 = state2[3] / rho2;
+            double w_mean = 0.5 * (w1 + w2);
+            double dw = w2 - w1;
+            V_sq1 += w1 * w1;
+            V_sq2 += w2 * w2;
+            Vn += w_mean * nor[2];
+            mom[2] = rho_ln * w_mean;
+            h_hat += -0.25 * (w1 * w1 + w2 * w2) + w_mean * w_mean;
+            if (P.bug & 4)
+                diss += 0.5 * drho * w1 * w2 + rho_mean * dw + w_mean;     // :131 as written
+            else
+                diss += 0.5 * drho * w1 * w2 + rho_mean * dw * w_mean;     // :131 intended
+        }
+    }
+    double p1 = P.gammaM1 * (state1[neq - 1] - 0.5 * rho1 * V_sq1);
+    double p2 = P.gammaM1 * (state2[neq - 1] - 0.5 * rho2 * V_sq2);
+    double V_mag1 = std::sqrt(V_sq1), V_mag2 = std::sqrt(V_sq2);
+    double lambda_max = V_mag1 + SoundSpeed(p1, rho1, P.gamma);
+    lambda_max = std::max(lambda_max, V_mag2 + SoundSpeed(p2, rho2, P.gamma));
+    double beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;
+    double beta_ln = LogMean(beta1, beta2);
+    double p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);
+    h_hat += 0.5 / beta_ln * P.gammaM1Inverse + p_hat / rho_ln;
+    diss += 0.5 * drho * P.gammaM1Inverse / beta_ln +
+            0.5 * rho_mean * P.gammaM1Inverse * (1.0 / beta2 - 1.0 / beta1);   // :151
+    flux[0] = rho_ln * Vn - 0.5 * lambda_max * (rho2 - rho1) * nor_mag;       // :153
+    for (int d = 0; d < dim; d++)
+        flux[1 + d] = Vn * mom[d] + p_hat * nor[d] - 0.5 * lambda_max * (state2[1 + d] - state1[1 + d]) * nor_mag;
+    flux[1 + dim] = rho_ln * Vn * h_hat - 0.5 * lambda_max * diss * nor_mag;   // :158
+    return lambda_max;
+}
+
+// Euler physical flux dotted with n (used by the Rusanov plug-in and by the
+// consistency tests).  The reference gets it from mfem::EulerFlux
+// (NavierStokesFlux : EulerFlux, src/Fluxes/NavierStokesFlux.hpp:10); the formula is
+// the textbook one.
+double EulerFluxDotN(const Phys &P, const double *s, const double *nor, double *fn) {
+    const int dim = P.dim, neq = P.neq;
+    double rho = s[0], vel[3], vsq = 0.0, vn = 0.0;
+    for (int d = 0; d < dim; d++) {
+        vel[d] = s[1 + d] / rho;
+        vsq += vel[d] * vel[d];
+        vn += vel[d] * nor[d];
+    }
+    double p = P.gammaM1 * (s[neq - 1] - 0.5 * rho * vsq);
+    fn[0] = rho * vn;
+    for (int d = 0; d < dim; d++) fn[1 + d] = s[1 + d] * vn + p * nor[d];
+    fn[neq - 1] = (s[neq - 1] + p) * vn;
+    return std::sqrt(vsq) + SoundSpeed(p, rho, P.gamma);
+}
+
+// Surface-flux plug-ins.  kind 0 = the reference's ChandrashekarFlux::ComputeFaceFlux;
+// kind 1 = the entropy-conservative flux alone (ComputeVolumeFlux with metric = nor);
+// kind 2 = Rusanov / local Lax-Friedrichs on the central flux:
+//   f* = 1/2 (F(u1) + F(u2)).n - 1/2 lambda_max |n| (u2 - u1).
+double FaceFlux(const Phys &P, const double *s1, const double *s2, const double *nor, double *flux) {
+    if (P.surface_flux == 0) return FaceFluxChandrashekar(P, s1, s2, nor, flux);
+    if (P.surface_flux == 1) return VolumeFlux(P, s1, s2, nor, nor, flux);
+    double f1[MAXEQ], f2[MAXEQ];
+    double l1 = EulerFluxDotN(P, s1, nor, f1);
+    double l2 = EulerFluxDotN(P, s2, nor, f2);
+    double lam = std::max(l1, l2);
+    double nm = 0.0;
+    for (int d = 0; d < P.dim; d++) nm += nor[d] * nor[d];
+    nm = std::sqrt(nm);
+    for (int v = 0; v < P.neq; v++) flux[v] = 0.5 * (f1[v] + f2[v]) - 0.5 * lam * nm * (s2[v] - s1[v]);
+    return lam;
+}
+
+// ---------------------------------------------------------------------------
+// Conserv2Entropy -- ref: BasicOperations.cpp:81-104
+// ---------------------------------------------------------------------------
+void Conserv2Entropy(const Phys &P, const double *state, double *ent) {
+    const int neq = P.neq, dim = P.dim;
+    double vel[3], v_sq = 0.0;
+    for (int d = 0; d < dim; d++) vel[d] = state[1 + d] / state[0];
+    for (int d = 0; d < dim; d++) v_sq += vel[d] * vel[d];
+    v_sq = 0.5 * v_sq;
+    double p = P.gammaM1 * (state[neq - 1] - state[0] * v_sq);
+    double beta = state[0] / p;
+    double s = std::log(p * std::pow(state[0], -P.gamma));
+    ent[0] = (P.gamma - s) * P.gammaM1Inverse - beta * v_sq;
+    for (int d = 0; d < dim; d++) ent[1 + d] = beta * vel[d];
+    ent[neq - 1] = -beta;
+}
+
+// Entropy2Conserv -- ref: BasicOperations.cpp:106-124
+void Entropy2Conserv(const Phys &P, const double *ent, double *state) {
+    const int dim = P.dim;
+    const double beta = -ent[dim + 1];
+    double vel[3], vv = 0.0;
+    for (int d = 0; d < dim; d++) vel[d] = ent[1 + d] / beta;
+    for (int d = 0; d < dim; d++) vv += vel[d] * vel[d];
+    const double s = P.gamma - (ent[0] + 0.5 * beta * vv) * P.gammaM1;
+    state[0] = std::pow(std::exp(-s) / beta, P.gammaM1Inverse);
+    for (int d = 0; d < dim; d++) state[1 + d] = state[0] * vel[d];
+    state[dim + 1] = state[0] * (1.0 / beta * P.gammaM1Inverse + 0.5 * vv);
+}
+
+// EntropyGrad2PrimGrad (one node, one direction, in place) -- ref: BasicOperations.cpp:126-170
+void EntropyGrad2PrimGrad(const Phys &P, const double *state, double *g) {
+    const int dim = P.dim, neq = P.neq;
+    double vel[3], v_sq = 0.0;
+    for (int d = 0; d < dim; d++) vel[d] = state[1 + d] / state[0];
+    for (int d = 0; d < dim; d++) v_sq += vel[d] * vel[d];
+    v_sq = 0.5 * v_sq;
+    double p = P.gammaM1 * (state[neq - 1] - state[0] * v_sq);
+    double KE = v_sq * state[0];
+    double gs[MAXEQ];
+    for (int v = 0; v < neq; v++) gs[v] = g[v];
+    const double ge = gs[neq - 1];
+    double mdot = 0.0;
+    for (int d = 0; d < dim; d++) {
+        g[1 + d] = p / state[0] * (gs[1 + d] + vel[d] * ge);
+    }
+    for (int d = 0; d < dim; d++) mdot += state[1 + d] * g[1 + d];
+    g[0] = state[0] * gs[0] - ge * (KE - p * P.gammaM1Inverse) + state[0] / p * mdot;
+    g[neq - 1] = p / state[0] * (g[0] + p * ge);
+}
+
+// ---------------------------------------------------------------------------
+// NavierStokesFlux::ComputeViscousFlux -- ref: src/Fluxes/NavierStokesFlux.cpp:12-58 (3D),
+// :60-90 (2D).  Sutherland: NavierStokesFlux.hpp:32-36.  flux is neq x dim,
+// row-major (flux[v*dim + d]); row 0 stays zero (DGSEMIntegrator.cpp:106).
+// ---------------------------------------------------------------------------
+void ViscousFlux(const Phys &P, const double *state, const double *dqdx, const double *dqdy,
+                 const double *dqdz, double *flux) {
+    const int dim = P.dim, neq = P.neq;
+    double prim[MAXEQ];                                         // Conserv2Prim BasicOperations.cpp:172-186
+    prim[0] = state[0];
+    for (int d = 0; d < dim; d++) prim[1 + d] = state[1 + d] / state[0];
+    {
+        double vv = 0.0;
+        for (int d = 0; d < dim; d++) vv += prim[1 + d] * prim[1 + d];
+        prim[neq - 1] = P.gammaM1 * (state[neq - 1] - 0.5 * state[0] * vv);  // ComputePressure :31-37
+    }
+    double mu = P.mu;
+    if (P.sutherland) {
+        double T = prim[neq - 1] / (prim[0] * P.R_gas);
+        mu = P.mu0 * (P.T0 + P.Ts) / (T + P.Ts) * (T / P.T0) * std::sqrt(T / P.T0);
+    }
+    const double lambda = mu * P.gamma * P.PrInverse;
+    for (int i = 0; i < neq * dim; i++) flux[i] = 0.0;
+    auto F = [&](int r, int c) -> double & { return flux[r * dim + c]; };
+    if (dim == 3) {
+        const double drdx = dqdx[0], dudx = dqdx[1], dvdx = dqdx[2], dwdx = dqdx[3], dpdx = dqdx[4];
+        const double drdy = dqdy[0], dudy = dqdy[1], dvdy = dqdy[2], dwdy = dqdy[3], dpdy = dqdy[4];
+        const double drdz = dqdz[0], dudz = dqdz[1], dvdz = dqdz[2], dwdz = dqdz[3], dpdz = dqdz[4];
+        double div = dudx + dvdy + dwdz;
+        double cv_dTdx = P.gammaM1Inverse / prim[0] * (dpdx - prim[4] / prim[0] * drdx);
+        double cv_dTdy = P.gammaM1Inverse / prim[0] * (dpdy - prim[4] / prim[0] * drdy);
+        double cv_dTdz = P.gammaM1Inverse / prim[0] * (dpdz - prim[4] / prim[0] * drdz);
+        F(1, 0) = mu * (2.0 * dudx - P.mu_bulk * div);
+        F(2, 0) = mu * (dudy + dvdx);
+        F(3, 0) = mu * (dudz + dwdx);
+        F(4, 0) = prim[1] * F(1, 0) + prim[2] * F(2, 0) + prim[3] * F(3, 0) + lambda * cv_dTdx;
+        F(1, 1) = mu * (dvdx + dudy);
+        F(2, 1) = mu * (2.0 * dvdy - P.mu_bulk * div);
+        F(3, 1) = mu * (dvdz + dwdy);
+        F(4, 1) = prim[1] * F(1, 1) + prim[2] * F(2, 1) + prim[3] * F(3, 1) + lambda * cv_dTdy;
+        F(1, 2) = mu * (dwdx + dudz);
+        F(2, 2) = mu * (dwdy + dvdz);
+        F(3, 2) = mu * (2.0 * dwdz - P.mu_bulk * div);
+        F(4, 2) = prim[1] * F(1, 2) + prim[2] * F(2, 2) + prim[3] * F(3, 2) + lambda * cv_dTdz;
+    } else {
+        const double drdx = dqdx[0], dudx = dqdx[1], dvdx = dqdx[2], dpdx = dqdx[3];
+        const double drdy = dqdy[0], dudy = dqdy[1], dvdy = dqdy[2], dpdy = dqdy[3];
+        double div = dudx + dvdy;
+        double cv_dTdx = P.gammaM1Inverse / prim[0] * (dpdx - prim[3] / prim[0] * drdx);
+        double cv_dTdy = P.gammaM1Inverse / prim[0] * (dpdy - prim[3] / prim[0] * drdy);
+        F(1, 0) = mu * (2.0 * dudx - P.mu_bulk * div);
+        F(2, 0) = mu * (dudy + dvdx);
+        F(3, 0) = prim[1] * F(1, 0) + prim[2] * F(2, 0) + lambda * cv_dTdx;
+        F(1, 1) = mu * (dvdx + dudy);
+        F(2, 1) = mu * (2.0 * dvdy - P.mu_bulk * div);
+        F(3, 1) = prim[1] * F(1, 1) + prim[2] * F(2, 1) + lambda * cv_dTdy;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GLL rule on [0,1] (MFEM IntegrationRules(0, GaussLobatto), DGSEMIntegrator.cpp:18-37),
+// restated from its closed form: interior nodes are the roots of P'_N, found by
+// Newton's method in long double on [-1,1] then mapped to [0,1].
+// ---------------------------------------------------------------------------
+void LegendreAndDeriv(int N, long double z, long double &P, long double &dP) {
+    long double p0 = 1.0L, p1 = z;
+    if (N == 0) { P = 1.0L; dP = 0.0L; return; }
+    for (int n = 1; n < N; n++) {
+        long double p2 = ((2 * n + 1) * z * p1 - n * p0) / (n + 1);
+        p0 = p1;
+        p1 = p2;
+    }
+    P = p1;
+    dP = N * (z * p1 - p0) / (z * z - 1.0L);  // valid for |z| < 1
+}
+
+void GLL(int np, double *x, double *w) {
+    const int N = np - 1;
+    std::vector<long double> z(np);
+    z[0] = -1.0L;
+    z[N] = 1.0L;
+    for (int j = 1; j < N; j++) {
+        // Chebyshev-Gauss-Lobatto initial guess, Newton on q(z) = P_{N-1}(z) - P_{N+1}(z)
+        long double zz = -std::cos(3.14159265358979323846264338327950288L * j / N);
+        for (int it = 0; it < 100; it++) {
+            // P'_N roots: use q = (1-z^2) P'_N = N (P_{N-1} - z P_N)
+            long double P, dP;
+            LegendreAndDeriv(N, zz, P, dP);
+            // second derivative from the Legendre ODE: (1-z^2)P'' = 2zP' - N(N+1)P
+            long double d2P = (2.0L * zz * dP - N * (N + 1.0L) * P) / (1.0L - zz * zz);
+            long double dz = dP / d2P;
+            zz -= dz;
+            if (std::fabs((double)dz) < 1e-30) break;
+        }
+        z[j] = zz;
+    }
+    for (int j = 0; j < np; j++) {
+        long double P, dP;
+        if (j == 0 || j == N) {
+            P = (j == 0 && (N % 2 == 1)) ? -1.0L : 1.0L;
+        } else {
+            LegendreAndDeriv(N, z[j], P, dP);
+        }
+        long double wj = 1.0L / (N * (N + 1.0L) * P * P);  // on [0,1]: half of 2/(N(N+1)P^2)
+        x[j] = (double)(0.5L * (z[j] + 1.0L));
+        w[j] = (double)wj;
+    }
+    x[0] = 0.0;
+    x[N] = 1.0;
+}
+
+// D, Dhat, Dhat2 -- ref: DGSEMIntegrator.cpp:41-82.  Stored as the mathematical
+// matrices (D[i*np+j] = D_ij), i.e. before the reference's transposes.
+void DMatrices(int np, const double *x, const double *w, double *D, double *Dhat, double *Dhat2) {
+    double wBary[MAXNP];
+    for (int i = 0; i < np; i++) wBary[i] = 1.0;
+    for (int i = 1; i < np; i++)
+        for (int j = 0; j < i; j++) {
+            wBary[j] *= (x[j] - x[i]);
+            wBary[i] *= (x[i] - x[j]);
+        }
+    for (int i = 0; i < np; i++) wBary[i] = 1.0 / wBary[i];       // Reciprocal :57
+    for (int i = 0; i < np * np; i++) D[i] = 0.0;
+    for (int iL = 0; iL < np; iL++)
+        for (int i = 0; i < np; i++)
+            if (iL != i) {
+                D[i * np + iL] = wBary[iL] / wBary[i] / (x[i] - x[iL]);   // :65
+                D[i * np + i] -= D[i * np + iL];                          // :66
+            }
+    for (int i = 0; i < np * np; i++) {
+        Dhat[i] = D[i];
+        Dhat2[i] = 2.0 * D[i];
+    }
+    Dhat[0] += 1.0 / w[0];                                    // :72
+    Dhat[(np - 1) * np + np - 1] -= 1.0 / w[np - 1];          // :73
+    Dhat2[0] += 1.0 / w[0];                                   // :78
+    Dhat2[(np - 1) * np + np - 1] -= 1.0 / w[np - 1];         // :79
+}
+
+// ---------------------------------------------------------------------------
+// Mesh: conforming affine elements with aligned faces.  neighbors[e*2*dim + 2*d + s]
+// is the element across local face (d, s) (s = 0: xi_d = 0 side, s = 1: xi_d = 1 side).
+// geom[e*(1+dim*dim)] = det A, followed by adj A row-major.
+// ---------------------------------------------------------------------------
+struct Mesh {
+    int dim, np, nn, NE;
+    const int *nbr;
+    const double *geom;
+    double x[MAXNP], w[MAXNP], D[MAXNP * MAXNP], Dhat[MAXNP * MAXNP], Dhat2[MAXNP * MAXNP];
+    double J(int e) const { return geom[(size_t)e * (1 + dim * dim)]; }
+    double adj(int e, int r, int c) const { return geom[(size_t)e * (1 + dim * dim) + 1 + r * dim + c]; }
+    int lex(int i, int j, int k) const { return i + np * (j + np * k); }
+};
+
+inline size_t gidx(const Mesh &M, int v, int e, int node) {
+    return ((size_t)v * M.NE + e) * M.nn + node;
+}
+
+// node index of the point with index a along direction d and tangential
+// indices (t1, t2) (in increasing direction order)
+inline int line_node(const Mesh &M, int d, int a, int t1, int t2) {
+    if (M.dim == 2) return d == 0 ? M.lex(a, t1, 0) : M.lex(t1, a, 0);
+    if (d == 0) return M.lex(a, t1, t2);
+    if (d == 1) return M.lex(t1, a, t2);
+    return M.lex(t1, t2, a);
+}
+
+// ---------------------------------------------------------------------------
+// Euler element term -- ref: DGSEMIntegrator::AssembleElementVector, DGSEMIntegrator.cpp:224-433
+// (blending off; AXISYMMETRIC off).  Adds into el_dudt (n x neq, column-major as
+// the reference's DenseMatrix el_dudt_mat).  Returns the running lambda max.
+// NS 3D element -- ref: :601-750 (with `visc` = the per-node contravariant viscous
+// fluxes F_viscous/G_viscous/H_viscous, and the A1 bug under P.bug & 1).
+// ---------------------------------------------------------------------------
+double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, const double *Fvis,
+                       double *el_dudt) {
+    const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq;
+    const int Npy = dim > 1 ? np : 1, Npz = dim > 2 ? np : 1;
+    double lam = 0.0;
+    // F_inviscid(id)(v, m): DenseTensor(num_equations, Np, n) -- :120-122
+    std::vector<double> Fi((size_t)n * np * neq), Gi((size_t)n * np * neq), Hi((size_t)n * np * neq);
+    auto FI = [&](std::vector<double> &T, int id, int v, int m) -> double & {
+        return T[((size_t)id * np + m) * neq + v];
+    };
+    const double J = M.J(e);
+    double s1[MAXEQ], s2[MAXEQ], f[MAXEQ], g[MAXEQ], h[MAXEQ], m1[3], m2[3];
+    for (int k = 0; k < Npz; k++)
+        for (int j = 0; j < Npy; j++)
+            for (int i = 0; i < np; i++) {
+                const int id1 = M.lex(i, j, k);
+                for (int v = 0; v < neq; v++) s1[v] = el_u[v * n + id1];
+                for (int c = 0; c < dim; c++) m1[c] = M.adj(e, 0, c);
+                for (int v = 0; v < neq; v++) FI(Fi, id1, v, i) = 0.0;      // f = 0; SetCol(i, f)
+                for (int m = i + 1; m < np; m++) {
+                    const int id2 = M.lex(m, j, k);
+                    for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
+                    for (int c = 0; c < dim; c++) m2[c] = M.adj(e, 0, c);
+                    lam = std::max(lam, VolumeFlux(P, s1, s2, m1, m2, f));
+                    for (int v = 0; v < neq; v++) {
+                        FI(Fi, id1, v, m) = f[v];
+                        FI(Fi, id2, v, i) = f[v];
+                    }
+                }
+                if (dim > 1) {
+                    for (int c = 0; c < dim; c++) m1[c] = M.adj(e, 1, c);
+                    for (int v = 0; v < neq; v++) g[v] = 0.0;
+                    for (int v = 0; v < neq; v++) FI(Gi, id1, v, j) = 0.0;
+                    for (int m = j + 1; m < Npy; m++) {
+                        const int id2 = M.lex(i, m, k);
+                        for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
+                        for (int c = 0; c < dim; c++) m2[c] = M.adj(e, 1, c);
+                        lam = std::max(lam, VolumeFlux(P, s1, s2, m1, m2, g));
+                        for (int v = 0; v < neq; v++) {
+                            FI(Gi, id1, v, m) = g[v];
+                            FI(Gi, id2, v, j) = g[v];
+                        }
+                    }
+                    if (dim > 2) {
+                        for (int c = 0; c < dim; c++) m1[c] = M.adj(e, 2, c);
+                        for (int v = 0; v < neq; v++) FI(Hi, id1, v, k) = 0.0;
+                        for (int m = k + 1; m < Npz; m++) {
+                            const int id2 = M.lex(i, j, m);
+                            for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
+                            for (int c = 0; c < dim; c++) m2[c] = M.adj(e, 2, c);
+                            lam = std::max(lam, VolumeFlux(P, s1, s2, m1, m2, h));
+                            for (int v = 0; v < neq; v++) {
+                                FI(Hi, id1, v, m) = h[v];
+                                // :702 as written stores g (NS 3D only); :363 (Euler) stores h
+                                FI(Hi, id2, v, k) = (Fvis && (P.bug & 1)) ? g[v] : h[v];
+                            }
+                        }
+                    }
+                }
+                // D-hat-2 accumulation -- :368-390 / :707-738
+                double dUi[MAXEQ], dUv[MAXEQ];
+                for (int v = 0; v < neq; v++) {
+                    double acc = 0.0;
+                    for (int m = 0; m < np; m++) acc += FI(Fi, id1, v, m) * M.Dhat2[i * np + m];
+                    dUi[v] = acc;
+                }
+                for (int v = 0; v < neq; v++) dUv[v] = 0.0;
+                if (Fvis) {
+                    for (int l = 0; l < np; l++) {
+                        const int idl = M.lex(l, j, k);
+                        for (int v = 0; v < neq; v++) dUv[v] += Fvis[(0 * n + idl) * neq + v] * M.Dhat[i * np + l];
+                    }
+                }
+                if (dim > 1) {
+                    for (int m = 0; m < np; m++)                          // AddMult: y += A x
+                        for (int v = 0; v < neq; v++) dUi[v] += FI(Gi, id1, v, m) * M.Dhat2[j * np + m];
+                    if (Fvis)
+                        for (int l = 0; l < np; l++) {
+                            const int idl = M.lex(i, l, k);
+                            for (int v = 0; v < neq; v++) dUv[v] += Fvis[(1 * n + idl) * neq + v] * M.Dhat[j * np + l];
+                        }
+                    if (dim > 2) {
+                        for (int m = 0; m < np; m++)
+                            for (int v = 0; v < neq; v++) dUi[v] += FI(Hi, id1, v, m) * M.Dhat2[k * np + m];
+                        if (Fvis)
+                            for (int l = 0; l < np; l++) {
+                                const int idl = M.lex(i, j, l);
+                                for (int v = 0; v < neq; v++) dUv[v] += Fvis[(2 * n + idl) * neq + v] * M.Dhat[k * np + l];
+                            }
+                    }
+                }
+                for (int v = 0; v < neq; v++) {
+                    double dU = -dUi[v];                                  // Neg :392 / :740
+                    if (Fvis) dU = dU + dUv[v];                           // add :744
+                    el_dudt[v * n + id1] += dU * (1.0 / J);               // :403 / :745 (Vector /= c is *= 1/c), AddRow
+                }
+            }
+    return lam;
+}
+
+struct Rhs {
+    const Mesh &M;
+    const Phys &P;
+    int nthreads;
+};
+
+// el_u gather (GetElementVDofs + GetSubVector, byNODES): el_u[v*n + node]
+inline void gather(const Mesh &M, int neq, const double *u, int e, double *el) {
+    for (int v = 0; v < neq; v++)
+        std::memcpy(el + (size_t)v * M.nn, u + gidx(M, v, e, 0), sizeof(double) * M.nn);
+}
+
+// ---------------------------------------------------------------------------
+// BR1 lifting of the entropy variables -- ref: DGSEMOperator::Mult :551-571,
+// ComputeGlobalEntropyVector :82-94, DGSEMNonlinearForm::MultLifting :424-645,
+// LiftingBR1::AssembleLiftingElementVector (3D :156-219, 2D :222-272) and
+// AssembleLiftingFaceVector (3D :10-66, 2D :68-115), then
+// ComputeGlobalPrimitiveGradVector :96-159.  grad holds dim arrays in the
+// layout of u.
+// ---------------------------------------------------------------------------
+void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *grad) {
+    const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq, NE = M.NE;
+    const size_t size = (size_t)neq * NE * n;
+    std::vector<double> went(size);
+    // ComputeGlobalEntropyVector
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < NE; e++) {
+        double s[MAXEQ], w[MAXEQ];
+        for (int node = 0; node < n; node++) {
+            for (int v = 0; v < neq; v++) s[v] = u[gidx(M, v, e, node)];
+            Conserv2Entropy(P, s, w);
+            for (int v = 0; v < neq; v++) went[gidx(M, v, e, node)] = w[v];
+        }
+    }
+    // element lifting
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < NE; e++) {
+        const double J = M.J(e);
+        const int Npy = dim > 1 ? np : 1, Npz = dim > 2 ? np : 1;
+        for (int k = 0; k < Npz; k++)
+            for (int j = 0; j < Npy; j++)
+                for (int i = 0; i < np; i++) {
+                    const int id = M.lex(i, j, k);
+                    double gm1[MAXEQ][3];                          // grad_mat1 (neq x dim)
+                    for (int v = 0; v < neq; v++)
+                        for (int c = 0; c < 3; c++) gm1[v][c] = 0.0;
+                    for (int l = 0; l < np; l++) {
+                        for (int v = 0; v < neq; v++) {
+                            gm1[v][0] += went[gidx(M, v, e, M.lex(l, j, k))] * M.D[i * np + l];   // :192-194
+                            if (dim > 1) gm1[v][1] += went[gidx(M, v, e, M.lex(i, l, k))] * M.D[j * np + l];
+                            if (dim > 2) gm1[v][2] += went[gidx(M, v, e, M.lex(i, j, l))] * M.D[k * np + l];
+                        }
+                    }
+                    // grad_mat2 = grad_mat1 * adj; *= 1/J  -- :205-206
+                    for (int v = 0; v < neq; v++)
+                        for (int c = 0; c < dim; c++) {
+                            double acc = 0.0;
+                            for (int r = 0; r < dim; r++) acc += gm1[v][r] * M.adj(e, r, c);
+                            grad[(size_t)c * size + gidx(M, v, e, id)] = acc * (1.0 / J);
+                        }
+                }
+    }
+    // face lifting: 1/2 (w2 - w1) n_c / (w0 J) added to both sides
+    const int Nt1 = dim > 1 ? np : 1, Nt2 = dim > 2 ? np : 1;
+    for (int d = 0; d < dim; d++) {
+#pragma omp parallel for schedule(static)
+        for (int e1 = 0; e1 < NE; e1++) {
+            const int e2 = M.nbr[(size_t)e1 * 2 * dim + 2 * d + 1];
+            const double J1 = M.J(e1), J2 = M.J(e2);
+            double nor[3];
+            for (int c = 0; c < dim; c++) nor[c] = M.adj(e1, d, c);       // CalcOrtho
+            for (int t2 = 0; t2 < Nt2; t2++)
+                for (int t1 = 0; t1 < Nt1; t1++) {
+                    const int n1 = line_node(M, d, np - 1, t1, t2), n2 = line_node(M, d, 0, t1, t2);
+                    for (int v = 0; v < neq; v++) {
+                        const double f = 0.5 * (went[gidx(M, v, e2, n2)] - went[gidx(M, v, e1, n1)]);  // subtract(0.5,...) :51
+                        for (int c = 0; c < dim; c++) {
+                            const double fc = f * nor[c];
+                            grad[(size_t)c * size + gidx(M, v, e1, n1)] += 1.0 / (M.w[0] * J1) * fc;
+                            grad[(size_t)c * size + gidx(M, v, e2, n2)] += 1.0 / (M.w[0] * J2) * fc;
+                        }
+                    }
+                }
+        }
+    }
+    // ComputeGlobalPrimitiveGradVector: EntropyGrad2PrimGrad per direction
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < NE; e++) {
+        double s[MAXEQ], g[MAXEQ];
+        for (int node = 0; node < n; node++) {
+            for (int v = 0; v < neq; v++) s[v] = u[gidx(M, v, e, node)];
+            for (int c = 0; c < dim; c++) {
+                for (int v = 0; v < neq; v++) g[v] = grad[(size_t)c * size + gidx(M, v, e, node)];
+                EntropyGrad2PrimGrad(P, s, g);
+                for (int v = 0; v < neq; v++) grad[(size_t)c * size + gidx(M, v, e, node)] = g[v];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Full RHS -- ref: DGSEMOperator::Mult (src/Operators/DGSEMOperator.cpp:537-591) driving
+// DGSEMNonlinearForm::Mult (Euler :647-849, NS 3D :1302-1545, NS 2D :1070-1300):
+// zero dudt, element loop, interior-face loop.  Face term -- ref:
+// DGSEMIntegrator::AssembleFaceVector (Euler :151-222, NS 3D :435-494, NS 2D :496-549).
+// Returns lambda_max over all pair and face flux evaluations (:270,:188).
+// ---------------------------------------------------------------------------
+double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, double *grad_out) {
+    const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq, NE = M.NE;
+    const size_t size = (size_t)neq * NE * n;
+    std::vector<double> grad_local;
+    double *grad = grad_out;
+    if (P.viscous) {
+        if (!grad) {
+            grad_local.resize(size * dim);
+            grad = grad_local.data();
+        }
+        LiftingAndPrimGrad(M, P, u, grad);
+    }
+    std::fill(dudt, dudt + size, 0.0);
+    double lam = 0.0;
+    // element loop
+#pragma omp parallel for schedule(static) reduction(max : lam)
+    for (int e = 0; e < NE; e++) {
+        std::vector<double> el_u((size_t)n * neq), el_dudt((size_t)n * neq, 0.0), Fvis;
+        gather(M, neq, u, e, el_u.data());
+        if (P.viscous) {
+            // per-node viscous flux, mapped with adj -- :617-641
+            Fvis.assign((size_t)3 * n * neq, 0.0);
+            double s[MAXEQ], dq[3][MAXEQ], flux[MAXEQ * 3];
+            for (int i = 0; i < n; i++) {
+                // A2: the reference reads state1 at the stale id1 (:620); under bug_compat
+                // this is taken as node n-1 of the same element (blending on leaves id1 = n-1).
+                const int sid = (P.bug & 2) ? n - 1 : i;
+                for (int v = 0; v < neq; v++) s[v] = el_u[v * n + sid];
+                for (int c = 0; c < dim; c++)
+                    for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e, i)];
+                ViscousFlux(P, s, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, flux);
+                // MultABt(adj1, flux_mat1, flux_mat): flux_mat(r, v) = sum_c adj(r,c) flux(v,c)
+                for (int r = 0; r < dim; r++)
+                    for (int v = 0; v < neq; v++) {
+                        double acc = 0.0;
+                        for (int c = 0; c < dim; c++) acc += M.adj(e, r, c) * flux[v * dim + c];
+                        Fvis[((size_t)r * n + i) * neq + v] = acc;
+                    }
+            }
+        }
+        double l = ElementInviscid(M, P, e, el_u.data(), P.viscous ? Fvis.data() : nullptr, el_dudt.data());
+        lam = std::max(lam, l);
+        for (int v = 0; v < neq; v++)
+            for (int node = 0; node < n; node++) dudt[gidx(M, v, e, node)] += el_dudt[(size_t)v * n + node];
+    }
+    // interior-face loop.  Faces of one direction touch disjoint node sets of
+    // each element, so a direction's faces run in parallel; directions run in turn.
+    const int Nt1 = dim > 1 ? np : 1, Nt2 = dim > 2 ? np : 1;
+    for (int d = 0; d < dim; d++) {
+#pragma omp parallel for schedule(static) reduction(max : lam)
+        for (int e1 = 0; e1 < NE; e1++) {
+            const int e2 = M.nbr[(size_t)e1 * 2 * dim + 2 * d + 1];
+            const double J1 = M.J(e1), J2 = M.J(e2);
+            double nor[3], s1[MAXEQ], s2[MAXEQ], fnum[MAXEQ], fl1[MAXEQ * 3], fl2[MAXEQ * 3];
+            double dq[3][MAXEQ];
+            for (int c = 0; c < dim; c++) nor[c] = M.adj(e1, d, c);
+            for (int t2 = 0; t2 < Nt2; t2++)
+                for (int t1 = 0; t1 < Nt1; t1++) {
+                    const int n1 = line_node(M, d, np - 1, t1, t2), n2 = line_node(M, d, 0, t1, t2);
+                    for (int v = 0; v < neq; v++) {
+                        s1[v] = u[gidx(M, v, e1, n1)];
+                        s2[v] = u[gidx(M, v, e2, n2)];
+                    }
+                    lam = std::max(lam, FaceFlux(P, s1, s2, nor, fnum));
+                    double dU1[MAXEQ], dU2[MAXEQ];
+                    for (int v = 0; v < neq; v++) {
+                        dU1[v] = -fnum[v];
+                        dU2[v] = fnum[v];
+                    }
+                    if (P.viscous) {
+                        for (int c = 0; c < dim; c++)
+                            for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e1, n1)];
+                        ViscousFlux(P, s1, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, fl1);
+                        for (int c = 0; c < dim; c++)
+                            for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e2, n2)];
+                        ViscousFlux(P, s2, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, fl2);
+                        for (int v = 0; v < neq; v++) {
+                            double fv = 0.0;                   // (flux_mat1 + flux_mat2) * 0.5, Mult(nor)
+                            for (int c = 0; c < dim; c++) fv += ((fl1[v * dim + c] + fl2[v * dim + c]) * 0.5) * nor[c];
+                            dU1[v] += fv;
+                            dU2[v] -= fv;
+                        }
+                    }
+                    for (int v = 0; v < neq; v++) {
+                        dudt[gidx(M, v, e1, n1)] += 1.0 / (M.w[0] * J1) * dU1[v];   // AddMult_a_VWt :219
+                        dudt[gidx(M, v, e2, n2)] += 1.0 / (M.w[0] * J2) * dU2[v];   // :220
+                    }
+                }
+        }
+    }
+    return lam;
+}
+
+Phys MakePhys(int dim, double gamma, double Pr, double mu, double R_gas, int viscous, int sutherland,
+              int surface_flux, int bug) {
+    Phys P;
+    P.dim = dim;
+    P.neq = dim + 2;
+    P.gamma = gamma;
+    P.gammaM1 = gamma - 1.0;
+    P.gammaM1Inverse = 1.0 / P.gammaM1;
+    P.Pr = Pr;
+    P.PrInverse = 1.0 / Pr;
+    P.mu = mu;
+    P.R_gas = R_gas;
+    P.viscous = viscous;
+    P.sutherland = sutherland;
+    P.surface_flux = surface_flux;
+    P.bug = bug;
+    return P;
+}
+
+void MakeMesh(Mesh &M, int dim, int order, int NE, const int *nbr, const double *geom) {
+    M.dim = dim;
+    M.np = order + 1;
+    M.nn = 1;
+    for (int d = 0; d < dim; d++) M.nn *= M.np;
+    M.NE = NE;
+    M.nbr = nbr;
+    M.geom = geom;
+    GLL(M.np, M.x, M.w);
+    DMatrices(M.np, M.x, M.w, M.D, M.Dhat, M.Dhat2);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C interface (ctypes)
+// ===========================================================================
+extern "C" {
+
+typedef struct {
+    double gamma, Pr, mu, R_gas;
+    int viscous;       // 0 Euler, 1 Navier-Stokes (BR1)
+    int sutherland;    // 1: Sutherland viscosity
+    int surface_flux;  // 0 Chandrashekar (reference), 1 EC, 2 Rusanov
+    int bug_compat;    // bit 0: A1, bit 1: A2, bit 2: A3
+} orc_phys;
+
+static Phys phys_of(int dim, const orc_phys *p) {
+    return MakePhys(dim, p->gamma, p->Pr, p->mu, p->R_gas, p->viscous, p->sutherland, p->surface_flux,
+                    p->bug_compat);
+}
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int orc_gll(int np, double *x, double *w) {
+    if (np < 2 || np > MAXNP) return -1;
+    GLL(np, x, w);
+    return 0;
+}
+
+int orc_dmatrices(int np, double *D, double *Dhat, double *Dhat2) {
+    if (np < 2 || np > MAXNP) return -1;
+    double x[MAXNP], w[MAXNP];
+    GLL(np, x, w);
+    DMatrices(np, x, w, D, Dhat, Dhat2);
+    return 0;
+}
+
+double orc_logmean(double x, double y) { return LogMean(x, y); }
+
+double orc_volume_flux(int dim, const orc_phys *p, const double *s1, const double *s2, const double *m1,
+                       const double *m2, double *F) {
+    Phys P = phys_of(dim, p);
+    return VolumeFlux(P, s1, s2, m1, m2, F);
+}
+
+double orc_face_flux(int dim, const orc_phys *p, const double *s1, const double *s2, const double *nor,
+                     double *F) {
+    Phys P = phys_of(dim, p);
+    return FaceFlux(P, s1, s2, nor, F);
+}
+
+double orc_euler_flux_dot_n(int dim, const orc_phys *p, const double *s, const double *nor, double *F) {
+    Phys P = phys_of(dim, p);
+    return EulerFluxDotN(P, s, nor, F);
+}
+
+void orc_conserv2entropy(int dim, const orc_phys *p, const double *s, double *w) {
+    Phys P = phys_of(dim, p);
+    Conserv2Entropy(P, s, w);
+}
+
+void orc_entropy2conserv(int dim, const orc_phys *p, const double *w, double *s) {
+    Phys P = phys_of(dim, p);
+    Entropy2Conserv(P, w, s);
+}
+
+void orc_entgrad2primgrad(int dim, const orc_phys *p, const double *s, double *g) {
+    Phys P = phys_of(dim, p);
+    EntropyGrad2PrimGrad(P, s, g);
+}
+
+void orc_viscous_flux(int dim, const orc_phys *p, const double *s, const double *gx, const double *gy,
+                      const double *gz, double *flux) {
+    Phys P = phys_of(dim, p);
+    ViscousFlux(P, s, gx, gy, gz, flux);
+}
+
+// RHS of the semi-discretisation: dudt = L(u).  grad (optional, dim*size) receives
+// the primitive gradients (NS only).  lambda_max (optional) receives the max wave speed.
+int orc_rhs(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
+            const double *u, double *dudt, double *grad, double *lambda_max) {
+    if (dim < 2 || dim > 3 || order < 1 || order + 1 > MAXNP) return -1;
+    Mesh M;
+    MakeMesh(M, dim, order, NE, nbr, geom);
+    Phys P = phys_of(dim, p);
+    double lam = RhsEval(M, P, u, dudt, grad);
+    if (lambda_max) *lambda_max = lam;
+    return 0;
+}
+
+// Time integration with a fixed dt.  scheme 0: SSP-RK3 (MFEM RK3SSPSolver,
+// Shu-Osher form; the reference calls it at Simulation.cpp:1010); scheme 1:
+// LSRK4(5) 2N-storage (Carpenter & Kennedy 1994).  u is advanced in place.
+static const double CK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                               -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double CK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+
+int orc_rk_run(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int scheme,
+               int nsteps, double dt, double *u) {
+    if (dim < 2 || dim > 3 || order < 1 || order + 1 > MAXNP) return -1;
+    Mesh M;
+    MakeMesh(M, dim, order, NE, nbr, geom);
+    Phys P = phys_of(dim, p);
+    const size_t size = (size_t)P.neq * NE * M.nn;
+    std::vector<double> k(size), y(size), x0(size);
+    for (int s = 0; s < nsteps; s++) {
+        if (scheme == 0) {
+            RhsEval(M, P, u, k.data(), nullptr);
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < size; i++) y[i] = u[i] + dt * k[i];
+            RhsEval(M, P, y.data(), k.data(), nullptr);
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < size; i++) {
+                y[i] = y[i] + dt * k[i];
+                y[i] = 3. / 4 * u[i] + 1. / 4 * y[i];
+            }
+            RhsEval(M, P, y.data(), k.data(), nullptr);
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < size; i++) {
+                y[i] = y[i] + dt * k[i];
+                u[i] = 1. / 3 * u[i] + 2. / 3 * y[i];
+            }
+        } else if (scheme == 1) {
+            std::vector<double> &kk = x0;  // 2N register
+            std::fill(kk.begin(), kk.end(), 0.0);
+            for (int st = 0; st < 5; st++) {
+                RhsEval(M, P, u, k.data(), nullptr);
+#pragma omp parallel for schedule(static)
+                for (size_t i = 0; i < size; i++) {
+                    kk[i] = CK_A[st] * kk[i] + dt * k[i];
+                    u[i] = u[i] + CK_B[st] * kk[i];
+                }
+            }
+        } else {
+            return -2;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+"""ctypes binding of the C ABI in include/prandtl_amd.h (thin test/bench binding).
+
+Device memory comes from torch (ROCm) tensors; the library only sees raw device
+pointers and a HIP stream handle, exactly as a C/C++ caller would pass them.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .mesh import Mesh
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libprandtl_amd.so")
+
+PRANDTL_OK = 0
+ERRORS = {-1: "PRANDTL_ERR_ARG", -2: "PRANDTL_ERR_UNSUPPORTED", -3: "PRANDTL_ERR_HIP", -4: "PRANDTL_ERR_MESH",
+          -5: "PRANDTL_ERR_NAN"}
+SURF_CHANDRASHEKAR, SURF_EC, SURF_RUSANOV = 0, 1, 2
+FP64, MIXED = 0, 1
+BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
+SSPRK3, LSRK45 = 0, 1
+
+# every symbol the public header declares
+EXPORTED = [
+    "prandtl_op_create", "prandtl_op_destroy", "prandtl_op_size", "prandtl_op_mult", "prandtl_op_mult_grad",
+    "prandtl_op_get_max_char_speed", "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run",
+    "prandtl_op_profile", "prandtl_last_error",
+]
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("num_elements", ctypes.c_int),
+                ("neighbors", ctypes.POINTER(ctypes.c_int)), ("geometry", ctypes.POINTER(ctypes.c_double))]
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("Pr", ctypes.c_double), ("mu", ctypes.c_double),
+                ("R_gas", ctypes.c_double), ("viscous", ctypes.c_int), ("sutherland", ctypes.c_int)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("surface_flux", ctypes.c_int), ("precision", ctypes.c_int), ("bug_compat", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+class PrandtlError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libprandtl_amd.so.  Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PrandtlError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                               f"g.build()'` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+        L.prandtl_op_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(Physics), ctypes.POINTER(Opts),
+                                        ctypes.POINTER(vp)]
+        L.prandtl_op_create.restype = ctypes.c_int
+        L.prandtl_op_destroy.argtypes = [vp]
+        L.prandtl_op_destroy.restype = None
+        L.prandtl_op_size.argtypes = [vp]
+        L.prandtl_op_size.restype = ctypes.c_size_t
+        L.prandtl_op_mult.argtypes = [vp, vp, vp, vp]
+        L.prandtl_op_mult_grad.argtypes = [vp, vp, vp, vp, vp]
+        L.prandtl_op_get_max_char_speed.argtypes = [vp, dp]
+        L.prandtl_op_max_char_speed.argtypes = [vp, vp, dp, vp]
+        L.prandtl_rk_step.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, vp]
+        L.prandtl_rk_run.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
+        L.prandtl_op_profile.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, dp, vp]
+        L.prandtl_last_error.argtypes = []
+        L.prandtl_last_error.restype = ctypes.c_char_p
+        for n in ("prandtl_op_mult", "prandtl_op_mult_grad", "prandtl_op_get_max_char_speed",
+                  "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile"):
+            getattr(L, n).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != PRANDTL_OK:
+        msg = load().prandtl_last_error().decode()
+        raise PrandtlError(f"{ERRORS.get(rc, rc)}: {msg}")
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    import torch
+
+    assert isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None) -> ctypes.c_void_p:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Operator:
+    """Owns a prandtl_op handle: the DGSEMOperator of one mesh and physics."""
+
+    def __init__(self, mesh: Mesh, gamma=1.4, viscous=False, Pr=0.72, mu=0.0, R_gas=287.0, sutherland=False,
+                 surface_flux=SURF_CHANDRASHEKAR, precision=FP64, bug_compat=0):
+        L = load()
+        self.mesh = mesh
+        self._nbr = np.ascontiguousarray(mesh.neighbors, dtype=np.int32)
+        self._geom = np.ascontiguousarray(mesh.geom, dtype=np.float64)
+        md = MeshDesc(mesh.dim, mesh.order, mesh.num_elements,
+                      self._nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                      self._geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        ph = Physics(gamma, Pr, mu, R_gas, int(viscous), int(sutherland))
+        op = Opts(int(surface_flux), int(precision), int(bug_compat), 0)
+        h = ctypes.c_void_p()
+        _check(L.prandtl_op_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h)))
+        self._h = h
+        self.viscous = bool(viscous)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().prandtl_op_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size(self) -> int:
+        return int(load().prandtl_op_size(self._h))
+
+    def mult(self, u, dudt, stream=None):
+        _check(load().prandtl_op_mult(self._h, _ptr(u), _ptr(dudt), _stream(stream)))
+
+    def mult_grad(self, u, dudt, grad, stream=None):
+        _check(load().prandtl_op_mult_grad(self._h, _ptr(u), _ptr(dudt), _ptr(grad), _stream(stream)))
+
+    def get_max_char_speed(self) -> float:
+        v = ctypes.c_double(0.0)
+        _check(load().prandtl_op_get_max_char_speed(self._h, ctypes.byref(v)))
+        return v.value
+
+    def max_char_speed(self, u, stream=None) -> float:
+        v = ctypes.c_double(0.0)
+        _check(load().prandtl_op_max_char_speed(self._h, _ptr(u), ctypes.byref(v), _stream(stream)))
+        return v.value
+
+    def profile(self, u, epilogue=0, iters=20, stream=None):
+        """Average ms per launch of (lift pass, flux-differencing pass)."""
+        ms = (ctypes.c_double * 2)()
+        _check(load().prandtl_op_profile(self._h, _ptr(u), int(epilogue), int(iters), ms, _stream(stream)))
+        return float(ms[0]), float(ms[1])
+
+    def rk_step(self, scheme, u, t, dt, stream=None) -> float:
+        tt = ctypes.c_double(t)
+        _check(load().prandtl_rk_step(self._h, int(scheme), _ptr(u), ctypes.byref(tt), float(dt), _stream(stream)))
+        return tt.value
+
+    def rk_run(self, scheme, u, t, dt, nsteps, nan_check=0, stream=None) -> float:
+        tt = ctypes.c_double(t)
+        _check(load().prandtl_rk_run(self._h, int(scheme), _ptr(u), ctypes.byref(tt), float(dt), int(nsteps),
+                                     int(nan_check), _stream(stream)))
+        return tt.value

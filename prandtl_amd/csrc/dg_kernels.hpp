@@ -1,0 +1,467 @@
+// dg_kernels.hpp -- DGSEM right-hand-side kernels for gfx950 (CDNA4).
+//
+// HBM layout (the reference's byNODES layout, src/Simulation/Simulation.hpp:64):
+//   u[v][e][node], node = i + Np*(j + Np*k) (DGSEMIntegrator.cpp:250).
+// Every kernel reads and writes whole element blocks of each variable, so a wave's
+// global accesses are contiguous runs of Np^d doubles per variable.
+//
+// Work decomposition of the flux-differencing kernel (k_rhs): a workgroup owns EPB
+// elements; thread t owns GLL line (t % L) of element (t / L), L = Np^(d-1).  For each
+// direction the thread loads its line's Np node primitives from LDS, evaluates every
+// symmetric pair flux once (Np(Np-1)/2 per line: 4.5 pair fluxes per node at p=3 in
+// 3D, the reference's count, DGSEMIntegrator.cpp:260-300) and both surface fluxes at
+// the line's ends, and adds its line's contributions into an LDS accumulator that
+// only it touches for that direction.  No atomics: results are bitwise reproducible.
+//
+// Navier-Stokes runs two grid passes (BR1 needs every gradient before any viscous
+// face flux): k_lift computes entropy variables, the BR1 gradient (volume + face
+// jump), the primitive-gradient chain rule and the viscous flux per node, and stores
+// the contravariant viscous fluxes F~^d = F_v . Ja^d (d x (neq-1) values per node).
+// k_rhs then reads them for the D-hat divergence and, at faces, the neighbour's
+// F~^d, which equals F_v . n for the shared face of a conforming affine mesh.
+#pragma once
+
+#include "dg_physics.hpp"
+
+namespace prandtl {
+
+constexpr int ipow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; i++) r *= b;
+    return r;
+}
+
+// Smallest block of 64, 128 or 256 threads whose whole-element packing uses >= 90 % of it.
+constexpr int choose_block(int L) {
+    for (int B = 64; B <= 256; B *= 2)
+        if ((B / L) * L * 10 >= B * 9) return B;
+    return 256;
+}
+
+template <int DIM, int NP>
+struct LineCfg {
+    static constexpr int NN = ipow(NP, DIM);      // nodes per element
+    static constexpr int L = ipow(NP, DIM - 1);   // lines per direction
+    static constexpr int BLOCK = choose_block(L);
+    static constexpr int EPB = BLOCK / L;         // elements per workgroup
+};
+
+template <int DIM, int NP>
+struct NodeCfg {
+    static constexpr int NN = ipow(NP, DIM);
+    static constexpr int BLOCK = NN <= 32 ? 64 : (NN <= 64 ? 128 : 64 * ((NN + 63) / 64));
+    static constexpr int EPB = BLOCK / NN;
+};
+
+template <int NP>
+struct DTables {
+    double D[NP][NP];      // D_ij (DGSEMIntegrator.cpp:45-69)
+    double Dhat[NP][NP];   // D + e0 e0^T / w0 - eN eN^T / wN  (:71-74)
+    double Dhat2[NP][NP];  // 2 D + the same corrections      (:76-80)
+    double inv_w0;         // 1 / w_0
+};
+
+enum EpilogueMode { EPI_MULT = 0, EPI_LSRK = 1, EPI_SSP = 2 };
+
+template <int NP>
+struct RhsArgs {
+    DTables<NP> T;
+    PhysConst P;
+    int NE;
+    int bug3;
+    int mode;
+    int want_lambda;
+    const int *nbr;
+    const double *geom;
+    const double *u;       // stage input state
+    const double *Fv;      // NS: contravariant viscous fluxes [d][v-1][e][node]
+    double *out;           // EPI_MULT: dudt; otherwise the stage output state
+    double *k;             // EPI_LSRK: 2N register
+    const double *x;       // EPI_SSP: base state of the Shu-Osher combination
+    double ca, cb, dt;
+    unsigned long long *lam_bits;
+    int *nan_flag;
+};
+
+template <int NP>
+struct LiftArgs {
+    DTables<NP> T;
+    PhysConst P;
+    int NE;
+    const int *nbr;
+    const double *geom;
+    const double *u;
+    double *Fv;            // out: [d][v-1][e][node]
+    double *grad;          // optional out: primitive gradients [c][v][e][node]
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD walks one
+// contiguous range of elements (face neighbours then share that XCD's L2).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// node index of point a on line l of direction dir
+template <int NP>
+__device__ __forceinline__ constexpr int line_base(int l, int dir) {
+    const int st = dir == 0 ? 1 : (dir == 1 ? NP : NP * NP);
+    return (l % st) + (l / st) * st * NP;
+}
+
+template <int NP>
+__device__ __forceinline__ constexpr int dir_stride(int dir) {
+    return dir == 0 ? 1 : (dir == 1 ? NP : NP * NP);
+}
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *p, double v) {
+    // positive doubles order like their bit patterns
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs: inviscid flux differencing + surface flux (+ viscous divergence and the
+// viscous face average for NS) + a fused stage epilogue.
+// ref: DGSEMIntegrator::AssembleElementVector (Euler :224-433, NS 3D :601-750),
+//      AssembleFaceVector (Euler :151-222, NS 3D :435-494, NS 2D :496-549),
+//      DGSEMNonlinearForm::Mult (:647-849, :1302-1545).
+// ---------------------------------------------------------------------------
+template <int DIM, int NP, bool VISC, int SURF>
+__global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs<NP> A) {
+    using C = LineCfg<DIM, NP>;
+    constexpr int NEQ = DIM + 2, NN = C::NN, L = C::L, EPB = C::EPB, NQ = DIM + 3;
+    __shared__ double sP[NQ][EPB * NN];   // rho, vel[DIM], p, beta
+    __shared__ double sD[NEQ][EPB * NN];  // dudt accumulator
+
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int e0 = blk * EPB;
+    const int tid = threadIdx.x;
+    const int NE = A.NE;
+    const size_t vs = (size_t)NE * NN;  // variable stride
+    const size_t off0 = (size_t)e0 * NN;
+    const int nvalid = min(EPB, NE - e0) * NN;
+
+    double lam = 0.0;
+    for (int idx = tid; idx < nvalid; idx += C::BLOCK) {
+        double s[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + off0 + idx];
+        const NodePrim<DIM, double> q = make_prim<DIM, double>(s, A.P.gammaM1);
+        sP[0][idx] = q.rho;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) sP[1 + d][idx] = q.vel[d];
+        sP[DIM + 1][idx] = q.p;
+        sP[DIM + 2][idx] = q.beta;
+        if (A.want_lambda) lam = fmax(lam, wave_speed<DIM, double>(q, A.P.gamma));
+    }
+    __syncthreads();
+
+    const int eb = tid / L, l = tid - (tid / L) * L;
+    const int e = e0 + eb;
+    const bool active = (eb < EPB) && (e < NE);
+    double invJ = 0.0, adj[DIM][DIM];
+    if (active) {
+        const double *g = A.geom + (size_t)e * (1 + DIM * DIM);
+        invJ = 1.0 / g[0];
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) adj[r][c] = g[1 + r * DIM + c];
+    }
+    const double sw = A.T.inv_w0 * invJ;
+
+#pragma unroll
+    for (int dir = 0; dir < DIM; dir++) {
+        if (active) {
+            const int base = line_base<NP>(l, dir);
+            const int st = dir_stride<NP>(dir);
+            const int lb = eb * NN + base;
+            NodePrim<DIM, double> q[NP];
+#pragma unroll
+            for (int a = 0; a < NP; a++) {
+                const int li = lb + a * st;
+                q[a].rho = sP[0][li];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) q[a].vel[d] = sP[1 + d][li];
+                q[a].p = sP[DIM + 1][li];
+                q[a].beta = sP[DIM + 2][li];
+            }
+            double acc[NP][NEQ];
+#pragma unroll
+            for (int a = 0; a < NP; a++)
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) acc[a][v] = 0.0;
+            const double *m = adj[dir];
+#pragma unroll
+            for (int a = 0; a < NP; a++)
+#pragma unroll
+                for (int b = a + 1; b < NP; b++) {
+                    double F[NEQ];
+                    volume_flux<DIM, double>(q[a], q[b], m, A.P.gammaM1Inverse, F);
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) {
+                        acc[a][v] += A.T.Dhat2[a][b] * F[v];
+                        acc[b][v] += A.T.Dhat2[b][a] * F[v];
+                    }
+                }
+#pragma unroll
+            for (int a = 0; a < NP; a++)
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) acc[a][v] = -acc[a][v] * invJ;
+
+            double fv[NP][NEQ - 1];
+            if constexpr (VISC) {
+                const double *Fd = A.Fv + (size_t)dir * (NEQ - 1) * vs;
+#pragma unroll
+                for (int a = 0; a < NP; a++)
+#pragma unroll
+                    for (int v = 0; v < NEQ - 1; v++) fv[a][v] = Fd[v * vs + (size_t)e * NN + base + a * st];
+#pragma unroll
+                for (int a = 0; a < NP; a++)
+#pragma unroll
+                    for (int v = 0; v < NEQ - 1; v++) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int b = 0; b < NP; b++) t += A.T.Dhat[a][b] * fv[b][v];
+                        acc[a][1 + v] += t * invJ;
+                    }
+            }
+            // surface terms at both ends of the line
+            {
+                // + side: this element is el1, neighbour el2 (outward normal Ja^dir)
+                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + 1];
+                const int nown = base + (NP - 1) * st, nnb = base;
+                double s1[NEQ], s2[NEQ], f[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    s1[v] = A.u[v * vs + (size_t)e * NN + nown];
+                    s2[v] = A.u[v * vs + (size_t)en * NN + nnb];
+                }
+                face_flux<DIM, SURF>(s1, s2, m, A.P, A.bug3 != 0, f);
+                if constexpr (VISC) {
+                    const double *Fd = A.Fv + (size_t)dir * (NEQ - 1) * vs;
+#pragma unroll
+                    for (int v = 0; v < NEQ - 1; v++)
+                        f[1 + v] -= 0.5 * (fv[NP - 1][v] + Fd[v * vs + (size_t)en * NN + nnb]);
+                }
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) acc[NP - 1][v] -= f[v] * sw;
+            }
+            {
+                // - side: the neighbour is el1, this element el2
+                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir];
+                const int nown = base, nnb = base + (NP - 1) * st;
+                double s1[NEQ], s2[NEQ], f[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    s1[v] = A.u[v * vs + (size_t)en * NN + nnb];
+                    s2[v] = A.u[v * vs + (size_t)e * NN + nown];
+                }
+                face_flux<DIM, SURF>(s1, s2, m, A.P, A.bug3 != 0, f);
+                if constexpr (VISC) {
+                    const double *Fd = A.Fv + (size_t)dir * (NEQ - 1) * vs;
+#pragma unroll
+                    for (int v = 0; v < NEQ - 1; v++)
+                        f[1 + v] -= 0.5 * (Fd[v * vs + (size_t)en * NN + nnb] + fv[0][v]);
+                }
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) acc[0][v] += f[v] * sw;
+            }
+#pragma unroll
+            for (int a = 0; a < NP; a++) {
+                const int li = lb + a * st;
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    if (dir == 0)
+                        sD[v][li] = acc[a][v];
+                    else
+                        sD[v][li] += acc[a][v];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: one thread per node, coalesced per variable
+    int bad = 0;
+    for (int idx = tid; idx < nvalid; idx += C::BLOCK) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            const size_t gi = v * vs + off0 + idx;
+            const double du = sD[v][idx];
+            if (A.mode == EPI_MULT) {
+                A.out[gi] = du;
+            } else if (A.mode == EPI_LSRK) {
+                const double kk = A.ca * A.k[gi] + A.dt * du;
+                A.k[gi] = kk;
+                const double un = A.u[gi] + A.cb * kk;
+                A.out[gi] = un;
+                bad |= !(un == un);
+            } else {
+                const double un = A.ca * A.x[gi] + A.cb * (A.u[gi] + A.dt * du);
+                A.out[gi] = un;
+                bad |= !(un == un);
+            }
+        }
+    }
+    if (A.want_lambda) {
+        // wave-level max, then one atomic per wave
+        for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
+        if ((tid & 63) == 0 && lam > 0.0) atomic_max_pos(A.lam_bits, lam);
+    }
+    if (A.nan_flag && __any(bad)) {
+        if ((tid & 63) == 0) atomicOr(A.nan_flag, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_lift: BR1 lifting of the entropy variables + chain rule + viscous flux.
+// ref: DGSEMOperator::Mult (:551-571), ComputeGlobalEntropyVector (:82-94),
+//      LiftingBR1::AssembleLiftingElementVector (3D :156-219, 2D :222-272),
+//      LiftingBR1::AssembleLiftingFaceVector (3D :10-66, 2D :68-115),
+//      ComputeGlobalPrimitiveGradVector (:96-159), NavierStokesFlux::ComputeViscousFlux,
+//      and the adj(J) mapping of the viscous flux at DGSEMIntegrator.cpp:632-640.
+// One thread per node.
+// ---------------------------------------------------------------------------
+template <int DIM, int NP>
+__global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftArgs<NP> A) {
+    using C = NodeCfg<DIM, NP>;
+    constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
+    __shared__ double sW[NEQ][EPB * NN];
+
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    const int eb = tid / NN, node = tid - (tid / NN) * NN;
+    const int e = blk * EPB + eb;
+    const int NE = A.NE;
+    const bool active = (eb < EPB) && (e < NE);
+    const size_t vs = (size_t)NE * NN;
+
+    double s[NEQ], w[NEQ];
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
+        conserv_to_entropy<DIM>(s, A.P, w);
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sW[v][eb * NN + node] = w[v];
+    }
+    __syncthreads();
+    if (!active) return;
+
+    const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
+    const double invJ = 1.0 / gm[0];
+    double adj[DIM][DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; r++)
+#pragma unroll
+        for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
+
+    int ai[3];
+    ai[0] = node % NP;
+    ai[1] = (node / NP) % NP;
+    ai[2] = node / (NP * NP);
+
+    double gref[DIM][NEQ];
+#pragma unroll
+    for (int dir = 0; dir < DIM; dir++) {
+        const int st = dir_stride<NP>(dir);
+        const int a = ai[dir];
+        const int lb = eb * NN + node - a * st;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < NP; b++) t += A.T.D[a][b] * sW[v][lb + b * st];
+            gref[dir][v] = t;
+        }
+        if (a == NP - 1 || a == 0) {
+            // face jump 1/2 (w+ - w-) / w0, the same sign on both sides (LiftingBR1.cpp:51-65)
+            const bool plus = (a == NP - 1);
+            const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + (plus ? 1 : 0)];
+            const int nn = node + (plus ? -(NP - 1) * st : (NP - 1) * st);
+            double sn[NEQ], wn[NEQ];
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nn];
+            conserv_to_entropy<DIM>(sn, A.P, wn);
+#pragma unroll
+            for (int v = 0; v < NEQ; v++)
+                gref[dir][v] += 0.5 * (plus ? (wn[v] - w[v]) : (w[v] - wn[v])) * A.T.inv_w0;
+        }
+    }
+    // physical gradient of w, then primitive gradients
+    double g[DIM][NEQ];
+#pragma unroll
+    for (int c = 0; c < DIM; c++)
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            double t = 0.0;
+#pragma unroll
+            for (int r = 0; r < DIM; r++) t += gref[r][v] * adj[r][c];
+            g[c][v] = t * invJ;
+        }
+#pragma unroll
+    for (int c = 0; c < DIM; c++) entgrad_to_primgrad<DIM>(s, A.P, g[c]);
+    if (A.grad) {
+#pragma unroll
+        for (int c = 0; c < DIM; c++)
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) A.grad[((size_t)c * NEQ + v) * vs + (size_t)e * NN + node] = g[c][v];
+    }
+    double flux[DIM + 1][DIM];
+    viscous_flux<DIM>(s, g, A.P, flux);
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+#pragma unroll
+        for (int v = 0; v < NEQ - 1; v++) {
+            double t = 0.0;
+#pragma unroll
+            for (int c = 0; c < DIM; c++) t += adj[d][c] * flux[v][c];
+            A.Fv[((size_t)d * (NEQ - 1) + v) * vs + (size_t)e * NN + node] = t;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// small utility kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i32(int *p, int v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, double v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = v;
+}
+
+__global__ void k_copy_f64(double *__restrict__ dst, const double *__restrict__ src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) dst[i] = src[i];
+}
+
+// max over nodes of |V| + c (the reference's max_char_speed, ChandrashekarFlux.cpp:61-65)
+template <int DIM>
+__global__ void k_max_speed(const double *u, size_t nnodes, double gamma, unsigned long long *lam_bits) {
+    constexpr int NEQ = DIM + 2;
+    double lam = 0.0;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < nnodes; i += st) {
+        double s[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s[v] = u[v * nnodes + i];
+        const NodePrim<DIM, double> q = make_prim<DIM, double>(s, gamma - 1.0);
+        lam = fmax(lam, wave_speed<DIM, double>(q, gamma));
+    }
+    for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
+    if ((threadIdx.x & 63) == 0 && lam > 0.0) atomic_max_pos(lam_bits, lam);
+}
+
+}  // namespace prandtl

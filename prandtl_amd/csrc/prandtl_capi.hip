@@ -1,0 +1,531 @@
+// prandtl_capi.hip -- C ABI of the gfx950 DGSEM library: operator handle, mesh
+// validation, kernel dispatch and the fused RK steppers.
+//
+// Reference interfaces replaced are cited per entry point in include/prandtl_amd.h.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/prandtl_amd.h"
+#include "dg_kernels.hpp"
+
+using namespace prandtl;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(PRANDTL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));    \
+    } while (0)
+
+constexpr int MAX_NP = 5;
+
+// GLL rule on [0,1] from its closed form (0, 1 and the roots of P'_N(2x-1), weights
+// 1/(N(N+1) P_N^2)), Newton's method in long double; the reference takes it from
+// MFEM's IntegrationRules(0, GaussLobatto) (DGSEMIntegrator.cpp:18-37).
+void gll_rule(int np, double *x, double *w) {
+    const int N = np - 1;
+    auto legendre = [N](long double z, long double &P, long double &dP) {
+        long double p0 = 1.0L, p1 = z;
+        for (int n = 1; n < N; n++) {
+            long double p2 = ((2 * n + 1) * z * p1 - n * p0) / (n + 1);
+            p0 = p1;
+            p1 = p2;
+        }
+        P = p1;
+        dP = N * (z * p1 - p0) / (z * z - 1.0L);
+    };
+    std::vector<long double> z(np);
+    z[0] = -1.0L;
+    z[N] = 1.0L;
+    for (int j = 1; j < N; j++) {
+        long double zz = -std::cos(3.14159265358979323846264338327950288L * j / N);
+        for (int it = 0; it < 100; it++) {
+            long double P, dP;
+            legendre(zz, P, dP);
+            const long double d2P = (2.0L * zz * dP - N * (N + 1.0L) * P) / (1.0L - zz * zz);
+            const long double dz = dP / d2P;
+            zz -= dz;
+            if (std::fabs((double)dz) < 1e-30) break;
+        }
+        z[j] = zz;
+    }
+    for (int j = 0; j < np; j++) {
+        long double P = 1.0L, dP;
+        if (j == 0)
+            P = (N % 2 == 1) ? -1.0L : 1.0L;
+        else if (j < N)
+            legendre(z[j], P, dP);
+        w[j] = (double)(1.0L / (N * (N + 1.0L) * P * P));
+        x[j] = (double)(0.5L * (z[j] + 1.0L));
+    }
+    x[0] = 0.0;
+    x[N] = 1.0;
+}
+
+// Barycentric differentiation matrix and its surface-corrected forms
+// (DGSEMIntegrator.cpp:45-80), stored as mathematical matrices M[i][j].
+void diff_matrices(int np, const double *x, const double *w, double *D, double *Dh, double *Dh2) {
+    double lam[MAX_NP + 1];
+    for (int i = 0; i < np; i++) lam[i] = 1.0;
+    for (int i = 1; i < np; i++)
+        for (int j = 0; j < i; j++) {
+            lam[j] *= (x[j] - x[i]);
+            lam[i] *= (x[i] - x[j]);
+        }
+    for (int i = 0; i < np; i++) lam[i] = 1.0 / lam[i];
+    for (int i = 0; i < np * np; i++) D[i] = 0.0;
+    for (int j = 0; j < np; j++)
+        for (int i = 0; i < np; i++)
+            if (i != j) {
+                D[i * np + j] = lam[j] / lam[i] / (x[i] - x[j]);
+                D[i * np + i] -= D[i * np + j];
+            }
+    for (int i = 0; i < np * np; i++) {
+        Dh[i] = D[i];
+        Dh2[i] = 2.0 * D[i];
+    }
+    Dh[0] += 1.0 / w[0];
+    Dh[np * np - 1] -= 1.0 / w[np - 1];
+    Dh2[0] += 1.0 / w[0];
+    Dh2[np * np - 1] -= 1.0 / w[np - 1];
+}
+
+}  // namespace
+
+struct prandtl_op_s {
+    int dim = 0, order = 0, np = 0, nn = 0, NE = 0, neq = 0;
+    size_t size = 0;       // doubles per state vector
+    prandtl_physics phys{};
+    prandtl_opts opts{};
+    PhysConst P{};
+    double D[MAX_NP * MAX_NP], Dh[MAX_NP * MAX_NP], Dh2[MAX_NP * MAX_NP], inv_w0 = 0.0;
+    int *d_nbr = nullptr;
+    double *d_geom = nullptr;
+    double *d_Fv = nullptr;       // NS: dim x (neq-1) state-sized blocks
+    double *d_k = nullptr;        // RK register
+    double *d_ub = nullptr;       // RK alternate state buffer
+    double *d_y2 = nullptr;       // SSP-RK3 second stage state
+    unsigned long long *d_lam = nullptr;
+    int *d_nan = nullptr;
+    bool lam_valid = false;
+};
+
+namespace {
+
+template <int NP>
+DTables<NP> make_tables(const prandtl_op_s *op) {
+    DTables<NP> T;
+    for (int i = 0; i < NP; i++)
+        for (int j = 0; j < NP; j++) {
+            T.D[i][j] = op->D[i * NP + j];
+            T.Dhat[i][j] = op->Dh[i * NP + j];
+            T.Dhat2[i][j] = op->Dh2[i * NP + j];
+        }
+    T.inv_w0 = op->inv_w0;
+    return T;
+}
+
+struct StageSpec {
+    int mode;            // EPI_*
+    const double *u;     // stage input
+    double *out;
+    double *k;
+    const double *x;
+    double ca, cb, dt;
+    double *grad;        // optional primitive-gradient export (NS)
+    bool want_lambda;
+    bool nan_check;
+    int phases = 3;      // bit 0: k_lift (NS), bit 1: k_rhs
+};
+
+template <int DIM, int NP, bool VISC, int SURF>
+int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
+    if constexpr (VISC) {
+      if (S.phases & 1) {
+        using NC = NodeCfg<DIM, NP>;
+        LiftArgs<NP> la;
+        la.T = make_tables<NP>(op);
+        la.P = op->P;
+        la.NE = op->NE;
+        la.nbr = op->d_nbr;
+        la.geom = op->d_geom;
+        la.u = S.u;
+        la.Fv = op->d_Fv;
+        la.grad = S.grad;
+        const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
+        hipLaunchKernelGGL((k_lift<DIM, NP>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+        HIP_TRY(hipGetLastError());
+      }
+    }
+    if (!(S.phases & 2)) return PRANDTL_OK;
+    using LC = LineCfg<DIM, NP>;
+    RhsArgs<NP> ra;
+    ra.T = make_tables<NP>(op);
+    ra.P = op->P;
+    ra.NE = op->NE;
+    ra.bug3 = (op->opts.bug_compat & PRANDTL_BUG_A3) ? 1 : 0;
+    ra.mode = S.mode;
+    ra.want_lambda = S.want_lambda ? 1 : 0;
+    ra.nbr = op->d_nbr;
+    ra.geom = op->d_geom;
+    ra.u = S.u;
+    ra.Fv = op->d_Fv;
+    ra.out = S.out;
+    ra.k = S.k;
+    ra.x = S.x;
+    ra.ca = S.ca;
+    ra.cb = S.cb;
+    ra.dt = S.dt;
+    ra.lam_bits = op->d_lam;
+    ra.nan_flag = S.nan_check ? op->d_nan : nullptr;
+    const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+    hipLaunchKernelGGL((k_rhs<DIM, NP, VISC, SURF>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+    HIP_TRY(hipGetLastError());
+    return PRANDTL_OK;
+}
+
+template <int DIM, int NP, bool VISC>
+int dispatch_surf(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
+    switch (op->opts.surface_flux) {
+        case PRANDTL_SURF_CHANDRASHEKAR: return launch_stage<DIM, NP, VISC, SURF_CHANDRASHEKAR>(op, S, st);
+        case PRANDTL_SURF_EC: return launch_stage<DIM, NP, VISC, SURF_EC>(op, S, st);
+        case PRANDTL_SURF_RUSANOV: return launch_stage<DIM, NP, VISC, SURF_RUSANOV>(op, S, st);
+    }
+    return fail(PRANDTL_ERR_ARG, "unknown surface flux");
+}
+
+template <int DIM, int NP>
+int dispatch_visc(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
+    return op->phys.viscous ? dispatch_surf<DIM, NP, true>(op, S, st) : dispatch_surf<DIM, NP, false>(op, S, st);
+}
+
+template <int DIM>
+int dispatch_np(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
+    switch (op->np) {
+        case 2: return dispatch_visc<DIM, 2>(op, S, st);
+        case 3: return dispatch_visc<DIM, 3>(op, S, st);
+        case 4: return dispatch_visc<DIM, 4>(op, S, st);
+        case 5: return dispatch_visc<DIM, 5>(op, S, st);
+    }
+    return fail(PRANDTL_ERR_UNSUPPORTED, "order out of range");
+}
+
+int run_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
+    if (S.want_lambda) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, op->d_lam, 0ull, 1);
+        HIP_TRY(hipGetLastError());
+    }
+    return op->dim == 2 ? dispatch_np<2>(op, S, st) : dispatch_np<3>(op, S, st);
+}
+
+int ensure_rk_buffers(prandtl_op_s *op, int scheme) {
+    const size_t bytes = op->size * sizeof(double);
+    if (!op->d_ub) {
+        HIP_TRY(hipMalloc(&op->d_ub, bytes));
+    }
+    if (scheme == PRANDTL_LSRK45 && !op->d_k) {
+        HIP_TRY(hipMalloc(&op->d_k, bytes));
+        hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, 0, op->d_k, 0.0, op->size);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    if (scheme == PRANDTL_SSPRK3 && !op->d_y2) {
+        HIP_TRY(hipMalloc(&op->d_y2, bytes));
+    }
+    return PRANDTL_OK;
+}
+
+// Carpenter & Kennedy (1994) 2N-storage RK4(3)5 coefficients.
+const double CK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                        -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double CK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                        1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                        2277821191437.0 / 14882151754819.0};
+
+// Advance u by nsteps; the state ping-pongs between u and the handle's alternate buffer
+// (a stage reads its neighbours' face nodes, so it cannot update in place).
+int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, int nan_check, hipStream_t st) {
+    int rc = ensure_rk_buffers(op, scheme);
+    if (rc) return rc;
+    if (nan_check > 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, op->d_nan, 0, 1);
+        HIP_TRY(hipGetLastError());
+    }
+    double *cur = u;
+    for (int s = 0; s < nsteps; s++) {
+        const bool chk = nan_check > 0 && ((s + 1) % nan_check == 0 || s + 1 == nsteps);
+        if (scheme == PRANDTL_LSRK45) {
+            for (int stg = 0; stg < 5; stg++) {
+                double *nxt = (cur == u) ? op->d_ub : u;
+                StageSpec S{EPI_LSRK, cur, nxt, op->d_k, nullptr, CK_A[stg], CK_B[stg], dt, nullptr, false,
+                            chk && stg == 4};
+                rc = run_stage(op, S, st);
+                if (rc) return rc;
+                cur = nxt;
+            }
+        } else {
+            // MFEM RK3SSPSolver (Shu-Osher): y = x + dt L(x); y2 = 3/4 x + 1/4 (y + dt L(y));
+            // x = 1/3 x + 2/3 (y2 + dt L(y2)).  x stays in `cur`.
+            double *y = (cur == u) ? op->d_ub : u;
+            StageSpec S1{EPI_SSP, cur, y, nullptr, cur, 0.0, 1.0, dt, nullptr, false, false};
+            rc = run_stage(op, S1, st);
+            if (rc) return rc;
+            StageSpec S2{EPI_SSP, y, op->d_y2, nullptr, cur, 3.0 / 4.0, 1.0 / 4.0, dt, nullptr, false, false};
+            rc = run_stage(op, S2, st);
+            if (rc) return rc;
+            StageSpec S3{EPI_SSP, op->d_y2, cur, nullptr, cur, 1.0 / 3.0, 2.0 / 3.0, dt, nullptr, false, chk};
+            rc = run_stage(op, S3, st);
+            if (rc) return rc;
+        }
+        if (chk) {
+            int flag = 0;
+            HIP_TRY(hipMemcpyAsync(&flag, op->d_nan, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (flag) {
+                if (cur != u) {
+                    hipLaunchKernelGGL(k_copy_f64, dim3(2048), dim3(256), 0, st, u, cur, op->size);
+                    HIP_TRY(hipGetLastError());
+                }
+                return fail(PRANDTL_ERR_NAN, "non-finite state after step " + std::to_string(s + 1));
+            }
+        }
+    }
+    if (cur != u) {
+        hipLaunchKernelGGL(k_copy_f64, dim3(2048), dim3(256), 0, st, u, cur, op->size);
+        HIP_TRY(hipGetLastError());
+    }
+    return PRANDTL_OK;
+}
+
+int validate_mesh(const prandtl_mesh_desc *m) {
+    const int dim = m->dim, NE = m->num_elements, nf = 2 * dim, ng = 1 + dim * dim;
+    for (int e = 0; e < NE; e++) {
+        const double J = m->geometry[(size_t)e * ng];
+        if (!(J > 0.0) || !std::isfinite(J))
+            return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) + ": det J must be positive");
+        for (int d = 0; d < dim; d++) {
+            const int ep = m->neighbors[(size_t)e * nf + 2 * d + 1];
+            const int em = m->neighbors[(size_t)e * nf + 2 * d];
+            if (ep < 0 || ep >= NE || em < 0 || em >= NE)
+                return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) +
+                                                  ": every face needs a neighbour (boundary faces are not supported)");
+            if (m->neighbors[(size_t)ep * nf + 2 * d] != e)
+                return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) + ": neighbour table is not reciprocal");
+            // the shared face's area vector must agree from both sides (conforming affine faces)
+            double nrm = 0.0, diff = 0.0;
+            for (int c = 0; c < dim; c++) {
+                const double a = m->geometry[(size_t)e * ng + 1 + d * dim + c];
+                const double b = m->geometry[(size_t)ep * ng + 1 + d * dim + c];
+                nrm += a * a;
+                diff += (a - b) * (a - b);
+            }
+            if (diff > 1e-24 * nrm)
+                return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) +
+                                                  ": face area vectors of neighbours disagree (non-conforming face)");
+        }
+    }
+    return PRANDTL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *prandtl_last_error(void) { return g_last_error.c_str(); }
+
+int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys, const prandtl_opts *opts,
+                      prandtl_op *out) {
+    if (!mesh || !phys || !out) return fail(PRANDTL_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (mesh->dim < 2 || mesh->dim > 3) return fail(PRANDTL_ERR_UNSUPPORTED, "dim must be 2 or 3");
+    if (mesh->order < 1 || mesh->order + 1 > MAX_NP) return fail(PRANDTL_ERR_UNSUPPORTED, "order must be 1..4");
+    if (mesh->num_elements <= 0 || !mesh->neighbors || !mesh->geometry) return fail(PRANDTL_ERR_ARG, "empty mesh");
+    if (!(phys->gamma > 1.0)) return fail(PRANDTL_ERR_ARG, "gamma must exceed 1");
+    prandtl_opts o{};
+    if (opts) o = *opts;
+    if (o.surface_flux < 0 || o.surface_flux > 2) return fail(PRANDTL_ERR_ARG, "unknown surface flux");
+    if (o.precision != PRANDTL_FP64) return fail(PRANDTL_ERR_UNSUPPORTED, "precision mode not built");
+    if (phys->viscous && (o.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
+        return fail(PRANDTL_ERR_UNSUPPORTED, "bug_compat A1/A2 are reproduced by the CPU oracle only");
+    if (phys->viscous && (!(phys->Pr > 0.0) || phys->mu < 0.0)) return fail(PRANDTL_ERR_ARG, "bad Pr or mu");
+    int rc = validate_mesh(mesh);
+    if (rc) return rc;
+
+    prandtl_op_s *op = new prandtl_op_s();
+    op->dim = mesh->dim;
+    op->order = mesh->order;
+    op->np = mesh->order + 1;
+    op->nn = 1;
+    for (int d = 0; d < op->dim; d++) op->nn *= op->np;
+    op->NE = mesh->num_elements;
+    op->neq = op->dim + 2;
+    op->size = (size_t)op->neq * op->NE * op->nn;
+    op->phys = *phys;
+    op->opts = o;
+    op->P.gamma = phys->gamma;
+    op->P.gammaM1 = phys->gamma - 1.0;
+    op->P.gammaM1Inverse = 1.0 / (phys->gamma - 1.0);
+    op->P.mu = phys->mu;
+    op->P.PrInverse = phys->viscous ? 1.0 / phys->Pr : 0.0;
+    op->P.R_gas = phys->R_gas;
+    op->P.mu_bulk = 2.0 / 3.0;   // Physics.hpp:34
+    op->P.mu0 = 1.716e-5;        // Physics.hpp:35
+    op->P.T0 = 273.15;           // Physics.hpp:36
+    op->P.Ts = 110.4;            // Physics.hpp:37
+    op->P.sutherland = phys->sutherland;
+    double x[MAX_NP], w[MAX_NP];
+    gll_rule(op->np, x, w);
+    diff_matrices(op->np, x, w, op->D, op->Dh, op->Dh2);
+    op->inv_w0 = 1.0 / w[0];
+
+    auto cleanup = [&](int code) {
+        prandtl_op_destroy(op);
+        return code;
+    };
+    const size_t nb = (size_t)op->NE * 2 * op->dim * sizeof(int);
+    const size_t gb = (size_t)op->NE * (1 + op->dim * op->dim) * sizeof(double);
+    hipError_t e;
+    if ((e = hipMalloc(&op->d_nbr, nb)) != hipSuccess ||
+        (e = hipMemcpy(op->d_nbr, mesh->neighbors, nb, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMalloc(&op->d_geom, gb)) != hipSuccess ||
+        (e = hipMemcpy(op->d_geom, mesh->geometry, gb, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMalloc(&op->d_lam, sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&op->d_nan, sizeof(int))) != hipSuccess)
+        return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+    if (phys->viscous) {
+        if ((e = hipMalloc(&op->d_Fv, (size_t)op->dim * (op->neq - 1) * op->NE * op->nn * sizeof(double))) !=
+            hipSuccess)
+            return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+    }
+    *out = op;
+    return PRANDTL_OK;
+}
+
+void prandtl_op_destroy(prandtl_op op) {
+    if (!op) return;
+    (void)hipFree(op->d_nbr);
+    (void)hipFree(op->d_geom);
+    (void)hipFree(op->d_Fv);
+    (void)hipFree(op->d_k);
+    (void)hipFree(op->d_ub);
+    (void)hipFree(op->d_y2);
+    (void)hipFree(op->d_lam);
+    (void)hipFree(op->d_nan);
+    delete op;
+}
+
+size_t prandtl_op_size(prandtl_op op) { return op ? op->size : 0; }
+
+int prandtl_op_mult(prandtl_op op, const double *u, double *dudt, prandtl_stream stream) {
+    if (!op || !u || !dudt) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (u == dudt) return fail(PRANDTL_ERR_ARG, "u and dudt must not alias");
+    StageSpec S{EPI_MULT, u, dudt, nullptr, nullptr, 0.0, 0.0, 0.0, nullptr, true, false};
+    int rc = run_stage(op, S, (hipStream_t)stream);
+    op->lam_valid = (rc == PRANDTL_OK);
+    return rc;
+}
+
+int prandtl_op_mult_grad(prandtl_op op, const double *u, double *dudt, double *grad, prandtl_stream stream) {
+    if (!op || !u || !dudt || !grad) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (!op->phys.viscous) return fail(PRANDTL_ERR_ARG, "gradients exist only for Navier-Stokes operators");
+    StageSpec S{EPI_MULT, u, dudt, nullptr, nullptr, 0.0, 0.0, 0.0, grad, true, false};
+    int rc = run_stage(op, S, (hipStream_t)stream);
+    op->lam_valid = (rc == PRANDTL_OK);
+    return rc;
+}
+
+int prandtl_op_get_max_char_speed(prandtl_op op, double *lambda_host) {
+    if (!op || !lambda_host) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (!op->lam_valid) return fail(PRANDTL_ERR_ARG, "no Mult has run on this operator");
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpy(&bits, op->d_lam, sizeof(bits), hipMemcpyDeviceToHost));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *lambda_host = v;
+    return PRANDTL_OK;
+}
+
+int prandtl_op_max_char_speed(prandtl_op op, const double *u, double *lambda_host, prandtl_stream stream) {
+    if (!op || !u || !lambda_host) return fail(PRANDTL_ERR_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, op->d_lam, 0ull, 1);
+    HIP_TRY(hipGetLastError());
+    const size_t nnodes = (size_t)op->NE * op->nn;
+    const int nblk = (int)std::min<size_t>(2048, (nnodes + 255) / 256);
+    if (op->dim == 2)
+        hipLaunchKernelGGL(k_max_speed<2>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam);
+    else
+        hipLaunchKernelGGL(k_max_speed<3>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam);
+    HIP_TRY(hipGetLastError());
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, op->d_lam, sizeof(bits), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *lambda_host = v;
+    return PRANDTL_OK;
+}
+
+int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, double *ms, prandtl_stream stream) {
+    if (!op || !u || !ms || iters <= 0) return fail(PRANDTL_ERR_ARG, "bad argument");
+    if (epilogue != EPI_MULT && epilogue != EPI_LSRK) return fail(PRANDTL_ERR_ARG, "epilogue must be 0 or 1");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = ensure_rk_buffers(op, PRANDTL_LSRK45);
+    if (rc) return rc;
+    StageSpec S{epilogue, u, op->d_ub, op->d_k, nullptr, 0.5, 0.25, 1e-6, nullptr, false, false};
+    // the lift pass must have produced the viscous fluxes once before timing the RHS pass alone
+    rc = run_stage(op, S, st);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int ph = 0; ph < 2; ph++) {
+        ms[ph] = 0.0;
+        if (ph == 0 && !op->phys.viscous) continue;
+        S.phases = 1 << ph;
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int i = 0; i < iters; i++) {
+            rc = run_stage(op, S, st);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        ms[ph] = (double)t / iters;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return PRANDTL_OK;
+}
+
+int prandtl_rk_step(prandtl_op op, int scheme, double *u, double *t, double dt, prandtl_stream stream) {
+    return prandtl_rk_run(op, scheme, u, t, dt, 1, 0, stream);
+}
+
+int prandtl_rk_run(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
+                   prandtl_stream stream) {
+    if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (scheme != PRANDTL_SSPRK3 && scheme != PRANDTL_LSRK45) return fail(PRANDTL_ERR_ARG, "unknown ODE solver");
+    if (nsteps < 0 || !(dt >= 0.0)) return fail(PRANDTL_ERR_ARG, "bad step count or dt");
+    int rc = rk_advance(op, scheme, u, dt, nsteps, nan_check, (hipStream_t)stream);
+    if (rc == PRANDTL_OK && t) *t += dt * nsteps;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the public
+header declares, and rejects invalid meshes and arguments before touching the device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from prandtl_amd import capi, mesh as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "prandtl_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(prandtl_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert header_symbols() == sorted(capi.EXPORTED)
+
+
+def test_library_exports_header_symbols():
+    lib = capi.load()
+    for name in header_symbols():
+        assert hasattr(lib, name), name
+
+
+def _create(m, **kw):
+    lib = capi.load()
+    nbr = np.ascontiguousarray(m.neighbors, dtype=np.int32)
+    geom = np.ascontiguousarray(m.geom, dtype=np.float64)
+    md = capi.MeshDesc(m.dim, m.order, m.num_elements, nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    ph = capi.Physics(kw.get("gamma", 1.4), kw.get("Pr", 0.72), kw.get("mu", 0.0), 287.0, kw.get("viscous", 0), 0)
+    op = capi.Opts(kw.get("surface_flux", 0), kw.get("precision", 0), kw.get("bug_compat", 0), 0)
+    h = ctypes.c_void_p()
+    rc = lib.prandtl_op_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
+    return rc, lib.prandtl_last_error().decode(), h
+
+
+def test_rejects_non_reciprocal_neighbours():
+    m = M.periodic_lattice(2, 3, (4, 4), lengths=(1, 1))
+    m.neighbors[5, 1] = 7
+    rc, msg, h = _create(m)
+    assert rc == -4 and "reciprocal" in msg and not h.value
+
+
+def test_rejects_missing_neighbour():
+    m = M.periodic_lattice(3, 2, (2, 2, 2), lengths=(1, 1, 1))
+    m.neighbors[0, 0] = -1
+    rc, msg, _ = _create(m)
+    assert rc == -4 and "neighbour" in msg
+
+
+def test_rejects_nonconforming_faces_and_bad_jacobian():
+    m = M.periodic_lattice(2, 3, (4, 4), lengths=(1, 1))
+    m.geom[3, 1] *= 1.5
+    rc, msg, _ = _create(m)
+    assert rc == -4 and "non-conforming" in msg
+    m = M.periodic_lattice(2, 3, (4, 4), lengths=(1, 1))
+    m.geom[2, 0] = -1.0
+    rc, msg, _ = _create(m)
+    assert rc == -4 and "det J" in msg
+
+
+@pytest.mark.parametrize("kw,code", [({"gamma": 1.0}, -1), ({"surface_flux": 9}, -1), ({"precision": 7}, -2),
+                                     ({"viscous": 1, "bug_compat": 1}, -2), ({"viscous": 1, "Pr": 0.0}, -1)])
+def test_rejects_bad_options(kw, code):
+    m = M.periodic_lattice(2, 3, (2, 2), lengths=(1, 1))
+    rc, msg, _ = _create(m, **kw)
+    assert rc == code, msg
+
+
+def test_rejects_unsupported_order():
+    m = M.periodic_lattice(2, 7, (2, 2), lengths=(1, 1))
+    rc, msg, _ = _create(m)
+    assert rc == -2 and "order" in msg
+
+
+def test_null_arguments():
+    lib = capi.load()
+    assert lib.prandtl_op_mult(None, None, None, None) == -1
+    assert lib.prandtl_rk_run(None, 0, None, None, 0.1, 1, 0, None) == -1
+    assert lib.prandtl_op_size(None) == 0
+    lib.prandtl_op_destroy(None)

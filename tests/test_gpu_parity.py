@@ -1,0 +1,224 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bar: relative max-norm difference <= 1e-12 of the RHS (and of RK states after a few
+steps) on the same seeded inputs, for every dimension, order, surface flux and the
+Euler/Navier-Stokes paths; at the full benchmark size (TGV 32^3, p=3) through
+size-independent properties: discrete conservation, free-stream preservation, bitwise
+determinism, entropy conservation with the EC flux.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from prandtl_amd import capi, cases, mesh as M
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def sheared_mesh(dim, order, n):
+    if dim == 2:
+        A = np.array([[0.5, 0.1], [0.05, 0.45]])
+    else:
+        A = np.array([[0.5, 0.1, 0.0], [0.05, 0.45, 0.08], [0.02, -0.03, 0.55]])
+    return M.periodic_lattice(dim, order, n, A=A)
+
+
+def perturbed_state(mesh, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    X = mesh.node_coords()
+    rho = 1.0 + 0.2 * np.sin(X[0]) + amp * rng.uniform(-1, 1, X[0].shape)
+    vel = [0.3 * np.cos(X[(d + 1) % mesh.dim]) + amp * rng.uniform(-1, 1, X[0].shape) for d in range(mesh.dim)]
+    p = 1.0 + 0.1 * np.cos(X[0] + X[1]) + amp * rng.uniform(-1, 1, X[0].shape)
+    return cases.prim_to_conserv(rho, vel, p)
+
+
+def gpu_rhs(mesh, u, device, grad=False, **kw):
+    import torch
+
+    op = capi.Operator(mesh, **kw)
+    ud = torch.from_numpy(np.ascontiguousarray(u)).to(device)
+    du = torch.empty_like(ud)
+    g = None
+    if grad:
+        g = torch.empty((mesh.dim,) + tuple(ud.shape), dtype=torch.float64, device=device)
+        op.mult_grad(ud, du, g)
+    else:
+        op.mult(ud, du)
+    torch.cuda.synchronize()
+    lam = op.get_max_char_speed()
+    out = du.cpu().numpy(), lam, (g.cpu().numpy() if grad else None)
+    op.close()
+    return out
+
+
+SURF = [oracle.SURF_CHANDRASHEKAR, oracle.SURF_EC, oracle.SURF_RUSANOV]
+
+
+@pytest.mark.parametrize("surf", SURF)
+def test_vortex_2d_euler(device, surf):
+    _say(f"vortex 16x16 p=3 Euler surf={surf}")
+    m = cases.vortex_mesh(n=16, order=3)
+    u = cases.vortex_state(m)
+    # element-wise jitter so surface fluxes see real jumps
+    u = u * (1.0 + 1e-3 * np.random.default_rng(1).uniform(-1, 1, u.shape))
+    d, lam, _ = gpu_rhs(m, u, device, surface_flux=surf)
+    ref, lref = oracle.rhs(m, oracle.phys(surface_flux=surf), u)
+    assert relerr(d, ref) <= TOL
+    assert abs(lam - lref) <= 1e-14 * lref
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_sheared_all_orders(device, dim, order, viscous):
+    _say(f"sheared dim={dim} p={order} viscous={viscous}")
+    n = (5, 4) if dim == 2 else (3, 2, 3)
+    m = sheared_mesh(dim, order, n)
+    u = perturbed_state(m, 10 * dim + order)
+    kw = dict(viscous=viscous, mu=0.02, Pr=0.71)
+    d, lam, _ = gpu_rhs(m, u, device, **kw)
+    ref, lref = oracle.rhs(m, oracle.phys(**kw), u)
+    assert relerr(d, ref) <= TOL
+    assert abs(lam - lref) <= 1e-14 * lref
+
+
+@pytest.mark.parametrize("surf", SURF)
+def test_tgv_ns_3d(device, surf):
+    _say(f"TGV 6^3 p=3 NS surf={surf}")
+    m = cases.tgv_mesh(n=6, order=3)
+    u = cases.tgv_state(m)
+    u = u * (1.0 + 1e-4 * np.random.default_rng(2).uniform(-1, 1, u.shape))
+    ph = cases.tgv_physics()
+    d, lam, g = gpu_rhs(m, u, device, grad=True, surface_flux=surf, **ph)
+    ref, lref, gref = oracle.rhs(m, oracle.phys(surface_flux=surf, gamma=ph["gamma"], Pr=ph["Pr"], mu=ph["mu"],
+                                                viscous=True), u, want_grad=True)
+    assert relerr(d, ref) <= TOL
+    # primitive gradients (the reference's dudx/dudy/dudz side effect)
+    assert relerr(g, gref) <= 1e-11
+    assert abs(lam - lref) <= 1e-14 * lref
+
+
+def test_bug_a3_3d(device):
+    _say("bug_compat A3, 3D Euler and NS")
+    m = sheared_mesh(3, 3, (2, 3, 2))
+    u = perturbed_state(m, 7)
+    for visc in (False, True):
+        kw = dict(viscous=visc, mu=0.01, Pr=0.7)
+        d, _, _ = gpu_rhs(m, u, device, bug_compat=capi.BUG_A3, **kw)
+        ref, _ = oracle.rhs(m, oracle.phys(bug_compat=oracle.BUG_A3, **kw), u)
+        assert relerr(d, ref) <= TOL
+
+
+@pytest.mark.parametrize("scheme", [capi.SSPRK3, capi.LSRK45])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_rk_steps(device, scheme, viscous):
+    import torch
+
+    _say(f"RK scheme={scheme} viscous={viscous}")
+    if viscous:
+        m = cases.tgv_mesh(n=4, order=3)
+        u = cases.tgv_state(m)
+        ph = cases.tgv_physics()
+        kw = dict(gamma=ph["gamma"], Pr=ph["Pr"], mu=ph["mu"], viscous=True)
+        dt = 2e-3
+    else:
+        m = cases.vortex_mesh(n=8, order=3)
+        u = cases.vortex_state(m)
+        kw = {}
+        dt = 5e-3
+    op = capi.Operator(m, **kw)
+    ud = torch.from_numpy(u).to(device)
+    t = op.rk_run(scheme, ud, 0.0, dt, 3)
+    torch.cuda.synchronize()
+    assert abs(t - 3 * dt) < 1e-15
+    ref = oracle.rk_run(m, oracle.phys(**kw), u, scheme, 3, dt)
+    assert relerr(ud.cpu().numpy(), ref) <= TOL
+    # single steps agree with a run
+    ud2 = torch.from_numpy(u).to(device)
+    for _ in range(3):
+        op.rk_step(scheme, ud2, 0.0, dt)
+    torch.cuda.synchronize()
+    assert torch.equal(ud, ud2)
+    op.close()
+
+
+def test_full_size_tgv_invariants(device):
+    """TGV 32^3 p=3 (2.1M nodes): conservation, determinism, free stream."""
+    import torch
+
+    _say("TGV 32^3 p=3 NS full-size invariants")
+    m = cases.tgv_mesh(n=32, order=3)
+    u = cases.tgv_state(m)
+    op = capi.Operator(m, **cases.tgv_physics())
+    ud = torch.from_numpy(u).to(device)
+    d1 = torch.empty_like(ud)
+    d2 = torch.empty_like(ud)
+    op.mult(ud, d1)
+    op.mult(ud, d2)
+    torch.cuda.synchronize()
+    assert torch.equal(d1, d2), "RHS is not bitwise reproducible"
+    W = torch.from_numpy(m.quadrature_weights()).to(device)
+    for v in range(5):
+        s = (W * d1[v]).sum().item()
+        a = (W * d1[v].abs()).sum().item()
+        assert abs(s) <= 1e-12 * a, (v, s, a)
+    assert torch.isfinite(d1).all()
+    lam = op.get_max_char_speed()
+    assert abs(lam - cases.max_wave_speed(u)) < 1e-13 * lam
+    # free stream on the same mesh
+    u0 = np.zeros(m.state_shape)
+    u0[:] = cases.prim_to_conserv(np.ones(1), [np.full(1, 0.3), np.full(1, -0.1), np.full(1, 0.2)],
+                                  np.full(1, 71.4))[:, :, None]
+    ud0 = torch.from_numpy(u0).to(device)
+    op.mult(ud0, d1)
+    torch.cuda.synchronize()
+    assert d1.abs().max().item() < 1e-10
+    op.close()
+
+
+def test_full_size_entropy_conservation(device):
+    """Euler + EC surface flux on a jittered 32^3 TGV state: sum J w_i w(u).du = 0."""
+    import torch
+
+    _say("TGV 32^3 p=3 Euler EC entropy conservation")
+    m = cases.tgv_mesh(n=32, order=3)
+    u = cases.tgv_state(m) * (1.0 + 1e-3 * np.random.default_rng(9).uniform(-1, 1, m.state_shape))
+    op = capi.Operator(m, surface_flux=capi.SURF_EC)
+    ud = torch.from_numpy(u).to(device)
+    du = torch.empty_like(ud)
+    op.mult(ud, du)
+    torch.cuda.synchronize()
+    d = du.cpu().numpy()
+    rho, vel, p = cases.conserv_to_prim(u)
+    s = np.log(p) - 1.4 * np.log(rho)
+    b = rho / p
+    w = np.stack([(1.4 - s) / 0.4 - 0.5 * b * sum(v * v for v in vel)] + [b * v for v in vel] + [-b])
+    W = m.quadrature_weights()
+    rate = (W * (w * d).sum(0)).sum()
+    scale = (W * np.abs(w * d).sum(0)).sum()
+    assert abs(rate) <= 1e-12 * scale
+    op.close()
+
+
+def test_nan_detection(device):
+    import torch
+
+    _say("NaN detection in rk_run")
+    m = cases.vortex_mesh(n=4, order=3)
+    u = cases.vortex_state(m)
+    u[0, 3, 5] = np.nan
+    op = capi.Operator(m)
+    ud = torch.from_numpy(u).to(device)
+    with pytest.raises(capi.PrandtlError, match="NAN"):
+        op.rk_run(capi.LSRK45, ud, 0.0, 1e-3, 4, nan_check=2)
+    op.close()
