@@ -15,7 +15,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_FMA_PATH = os.path.join(_HERE, "liboracle_fma.so")
 _lib = None
+_lib_fma = None
 
 SURF_CHANDRASHEKAR, SURF_EC, SURF_RUSANOV = 0, 1, 2
 BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
@@ -35,46 +37,52 @@ class OrcPhys(ctypes.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with make (g++)."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-B" if force else "liboracle.so"], cwd=_HERE, check=True)
+    """Compile liboracle.so and liboracle_fma.so with make (g++)."""
+    if force or not (os.path.exists(_LIB_PATH) and os.path.exists(_FMA_PATH)):
+        args = ["make", "-s"] + (["-B"] if force else []) + ["all"]
+        subprocess.run(args, cwd=_HERE, check=True)
     return _LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def _configure(L):
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    pp = ctypes.POINTER(OrcPhys)
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_set_threads.restype = ctypes.c_int
+    L.orc_gll.argtypes = [ctypes.c_int, dp, dp]
+    L.orc_dmatrices.argtypes = [ctypes.c_int, dp, dp, dp]
+    L.orc_logmean.argtypes = [ctypes.c_double, ctypes.c_double]
+    L.orc_logmean.restype = ctypes.c_double
+    for name in ("orc_volume_flux",):
+        getattr(L, name).argtypes = [ctypes.c_int, pp, dp, dp, dp, dp, dp]
+        getattr(L, name).restype = ctypes.c_double
+    L.orc_face_flux.argtypes = [ctypes.c_int, pp, dp, dp, dp, dp]
+    L.orc_face_flux.restype = ctypes.c_double
+    L.orc_euler_flux_dot_n.argtypes = [ctypes.c_int, pp, dp, dp, dp]
+    L.orc_euler_flux_dot_n.restype = ctypes.c_double
+    for name in ("orc_conserv2entropy", "orc_entropy2conserv", "orc_entgrad2primgrad"):
+        getattr(L, name).argtypes = [ctypes.c_int, pp, dp, dp]
+        getattr(L, name).restype = None
+    L.orc_viscous_flux.argtypes = [ctypes.c_int, pp, dp, dp, dp, dp, dp]
+    L.orc_viscous_flux.restype = None
+    L.orc_rhs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, dp, dp, dp, dp]
+    L.orc_rhs.restype = ctypes.c_int
+    L.orc_rk_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_double, dp]
+    L.orc_rk_run.restype = ctypes.c_int
+    return L
+
+
+def lib(fma: bool = False):
+    """The oracle library; fma=True gives the FMA-contracted build of the same source."""
+    global _lib, _lib_fma
+    if _lib is None or _lib_fma is None:
+        if not (os.path.exists(_LIB_PATH) and os.path.exists(_FMA_PATH)):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        dp = ctypes.POINTER(ctypes.c_double)
-        ip = ctypes.POINTER(ctypes.c_int)
-        pp = ctypes.POINTER(OrcPhys)
-        L.orc_set_threads.argtypes = [ctypes.c_int]
-        L.orc_set_threads.restype = ctypes.c_int
-        L.orc_gll.argtypes = [ctypes.c_int, dp, dp]
-        L.orc_dmatrices.argtypes = [ctypes.c_int, dp, dp, dp]
-        L.orc_logmean.argtypes = [ctypes.c_double, ctypes.c_double]
-        L.orc_logmean.restype = ctypes.c_double
-        for name in ("orc_volume_flux",):
-            getattr(L, name).argtypes = [ctypes.c_int, pp, dp, dp, dp, dp, dp]
-            getattr(L, name).restype = ctypes.c_double
-        L.orc_face_flux.argtypes = [ctypes.c_int, pp, dp, dp, dp, dp]
-        L.orc_face_flux.restype = ctypes.c_double
-        L.orc_euler_flux_dot_n.argtypes = [ctypes.c_int, pp, dp, dp, dp]
-        L.orc_euler_flux_dot_n.restype = ctypes.c_double
-        for name in ("orc_conserv2entropy", "orc_entropy2conserv", "orc_entgrad2primgrad"):
-            getattr(L, name).argtypes = [ctypes.c_int, pp, dp, dp]
-            getattr(L, name).restype = None
-        L.orc_viscous_flux.argtypes = [ctypes.c_int, pp, dp, dp, dp, dp, dp]
-        L.orc_viscous_flux.restype = None
-        L.orc_rhs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, dp, dp, dp, dp]
-        L.orc_rhs.restype = ctypes.c_int
-        L.orc_rk_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
-                                 ctypes.c_int, ctypes.c_double, dp]
-        L.orc_rk_run.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        _lib = _configure(ctypes.CDLL(_LIB_PATH))
+        _lib_fma = _configure(ctypes.CDLL(_FMA_PATH))
+    return _lib_fma if fma else _lib
 
 
 def _dp(a):
@@ -161,13 +169,13 @@ def viscous_flux(dim, ph, s, gx, gy, gz=None):
     return flux
 
 
-def rhs(mesh, ph, u, want_grad=False):
+def rhs(mesh, ph, u, want_grad=False, fma=False):
     """dudt = L(u) on ``mesh`` (a prandtl_amd.mesh.Mesh); returns (dudt, lambda_max[, grad])."""
     u = np.ascontiguousarray(u, dtype=np.float64)
     dudt = np.zeros_like(u)
     grad = np.zeros((mesh.dim,) + u.shape) if want_grad else None
     lam = ctypes.c_double(0.0)
-    rc = lib().orc_rhs(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib(fma).orc_rhs(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
                        ctypes.byref(ph), _dp(u), _dp(dudt), _dp(grad), ctypes.byref(lam))
     assert rc == 0, rc
     if want_grad:
@@ -175,10 +183,15 @@ def rhs(mesh, ph, u, want_grad=False):
     return dudt, lam.value
 
 
-def rk_run(mesh, ph, u, scheme: int, nsteps: int, dt: float):
+def rk_run(mesh, ph, u, scheme: int, nsteps: int, dt: float, fma=False):
     """Advance u (copy) by nsteps fixed steps; scheme 0 = SSP-RK3, 1 = LSRK4(5)."""
     u = np.array(u, dtype=np.float64, copy=True)
-    rc = lib().orc_rk_run(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib(fma).orc_rk_run(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
                           ctypes.byref(ph), int(scheme), int(nsteps), float(dt), _dp(u))
     assert rc == 0, rc
     return u
+
+
+def rounding_floor(a, b) -> float:
+    """Relative max-norm difference of two oracle builds' results (their rounding spread)."""
+    return float(np.abs(a - b).max() / np.abs(a).max())

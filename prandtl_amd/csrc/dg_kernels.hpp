@@ -15,13 +15,22 @@
 //
 // Navier-Stokes runs two grid passes (BR1 needs every gradient before any viscous
 // face flux): k_lift computes entropy variables, the BR1 gradient (volume + face
-// jump), the primitive-gradient chain rule and the viscous flux per node, and stores
-// the contravariant viscous fluxes F~^d = F_v . Ja^d (d x (neq-1) values per node).
-// k_rhs then reads them for the D-hat divergence and, at faces, the neighbour's
-// F~^d, which equals F_v . n for the shared face of a conforming affine mesh.
+// jump), the primitive-gradient chain rule and the viscous flux per node, stages the
+// contravariant viscous fluxes F~^d = F_v . Ja^d in LDS, and writes (a) the viscous
+// volume term (1/J) sum_d D-hat F~^d (neq-1 values per node) and (b) F~^d at the
+// element's face points in a face-trace layout [d][side][v][e][t] (t = the point's
+// index on the face), so k_rhs reads both sides of every face coalesced.  On a
+// conforming affine mesh F~^d at a face point is F_v . n for that face.
 #pragma once
 
 #include "dg_physics.hpp"
+
+#ifndef PRANDTL_LINE_WAVES
+#define PRANDTL_LINE_WAVES 2
+#endif
+#ifndef PRANDTL_NODE_WAVES
+#define PRANDTL_NODE_WAVES 4
+#endif
 
 namespace prandtl {
 
@@ -74,7 +83,8 @@ struct RhsArgs {
     const int *nbr;
     const double *geom;
     const double *u;       // stage input state
-    const double *Fv;      // NS: contravariant viscous fluxes [d][v-1][e][node]
+    const double *dv;      // NS: viscous volume term [v-1][e][node]
+    const double *Ff;      // NS: viscous face traces F~^d [d][side][v-1][e][t]
     double *out;           // EPI_MULT: dudt; otherwise the stage output state
     double *k;             // EPI_LSRK: 2N register
     const double *x;       // EPI_SSP: base state of the Shu-Osher combination
@@ -91,7 +101,8 @@ struct LiftArgs {
     const int *nbr;
     const double *geom;
     const double *u;
-    double *Fv;            // out: [d][v-1][e][node]
+    double *dv;            // out: viscous volume term [v-1][e][node]
+    double *Ff;            // out: viscous face traces [d][side][v-1][e][t]
     double *grad;          // optional out: primitive gradients [c][v][e][node]
 };
 
@@ -115,23 +126,66 @@ __device__ __forceinline__ constexpr int dir_stride(int dir) {
     return dir == 0 ? 1 : (dir == 1 ? NP : NP * NP);
 }
 
+// index t of the line (of direction dir) through a node: the inverse of line_base
+template <int NP>
+__device__ __forceinline__ int line_index(int node, int dir) {
+    const int st = dir_stride<NP>(dir);
+    return (node % st) + (node / (st * NP)) * st;
+}
+
+// offset of face-trace value (dir, side, v, e, t)
+template <int DIM, int NP>
+__device__ __forceinline__ size_t ftrace(int dir, int side, int v, int e, int t, int NE) {
+    constexpr int L = ipow(NP, DIM - 1);
+    return ((((size_t)(dir * 2 + side) * (DIM + 1) + v) * NE + e) * L) + t;
+}
+
 __device__ __forceinline__ void atomic_max_pos(unsigned long long *p, double v) {
     // positive doubles order like their bit patterns
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
+// Stage epilogue: writes the RHS (Mult) or applies the fused RK stage update to one value.
+template <int NP>
+__device__ __forceinline__ void stage_update(const RhsArgs<NP> &A, size_t gi, double u_in, double du, int &bad) {
+    if (A.mode == EPI_MULT) {
+        A.out[gi] = du;
+    } else if (A.mode == EPI_LSRK) {
+        const double kk = A.ca * A.k[gi] + A.dt * du;
+        A.k[gi] = kk;
+        const double un = u_in + A.cb * kk;
+        A.out[gi] = un;
+        bad |= !(un == un);
+    } else {
+        const double un = A.ca * A.x[gi] + A.cb * (u_in + A.dt * du);
+        A.out[gi] = un;
+        bad |= !(un == un);
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ void finish_block(const RhsArgs<NP> &A, double lam, int bad) {
+    if (A.want_lambda) {
+        for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
+        if ((threadIdx.x & 63) == 0 && lam > 0.0) atomic_max_pos(A.lam_bits, lam);
+    }
+    if (A.nan_flag && __any(bad)) {
+        if ((threadIdx.x & 63) == 0) atomicOr(A.nan_flag, 1);
+    }
+}
+
 // ---------------------------------------------------------------------------
-// k_rhs: inviscid flux differencing + surface flux (+ viscous divergence and the
-// viscous face average for NS) + a fused stage epilogue.
+// k_rhs (line variant): inviscid flux differencing + surface flux (+ viscous divergence
+// and the viscous face average for NS) + a fused stage epilogue.
 // ref: DGSEMIntegrator::AssembleElementVector (Euler :224-433, NS 3D :601-750),
 //      AssembleFaceVector (Euler :151-222, NS 3D :435-494, NS 2D :496-549),
 //      DGSEMNonlinearForm::Mult (:647-849, :1302-1545).
 // ---------------------------------------------------------------------------
 template <int DIM, int NP, bool VISC, int SURF>
-__global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs<NP> A) {
+__global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES) k_rhs(const RhsArgs<NP> A) {
     using C = LineCfg<DIM, NP>;
-    constexpr int NEQ = DIM + 2, NN = C::NN, L = C::L, EPB = C::EPB, NQ = DIM + 3;
-    __shared__ double sP[NQ][EPB * NN];   // rho, vel[DIM], p, beta
+    constexpr int NEQ = DIM + 2, NN = C::NN, L = C::L, EPB = C::EPB, NQ = DIM + 2;
+    __shared__ double sP[NQ][EPB * NN];   // rho, vel[DIM], beta
     __shared__ double sD[NEQ][EPB * NN];  // dudt accumulator
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -147,13 +201,12 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs
         double s[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + off0 + idx];
-        const NodePrim<DIM, double> q = make_prim<DIM, double>(s, A.P.gammaM1);
+        const SideState<DIM> q = make_side<DIM>(s, A.P);
         sP[0][idx] = q.rho;
 #pragma unroll
         for (int d = 0; d < DIM; d++) sP[1 + d][idx] = q.vel[d];
-        sP[DIM + 1][idx] = q.p;
-        sP[DIM + 2][idx] = q.beta;
-        if (A.want_lambda) lam = fmax(lam, wave_speed<DIM, double>(q, A.P.gamma));
+        sP[DIM + 1][idx] = q.beta;
+        lam = fmax(lam, q.lam);
     }
     __syncthreads();
 
@@ -163,7 +216,7 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs
     double invJ = 0.0, adj[DIM][DIM];
     if (active) {
         const double *g = A.geom + (size_t)e * (1 + DIM * DIM);
-        invJ = 1.0 / g[0];
+        invJ = rcp(g[0]);
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
@@ -177,15 +230,14 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs
             const int base = line_base<NP>(l, dir);
             const int st = dir_stride<NP>(dir);
             const int lb = eb * NN + base;
-            NodePrim<DIM, double> q[NP];
+            NodePrim<DIM> q[NP];
 #pragma unroll
             for (int a = 0; a < NP; a++) {
                 const int li = lb + a * st;
                 q[a].rho = sP[0][li];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) q[a].vel[d] = sP[1 + d][li];
-                q[a].p = sP[DIM + 1][li];
-                q[a].beta = sP[DIM + 2][li];
+                q[a].beta = sP[DIM + 1][li];
             }
             double acc[NP][NEQ];
 #pragma unroll
@@ -198,7 +250,7 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs
 #pragma unroll
                 for (int b = a + 1; b < NP; b++) {
                     double F[NEQ];
-                    volume_flux<DIM, double>(q[a], q[b], m, A.P.gammaM1Inverse, F);
+                    volume_flux<DIM>(q[a], q[b], m, A.P.gammaM1Inverse, F);
 #pragma unroll
                     for (int v = 0; v < NEQ; v++) {
                         acc[a][v] += A.T.Dhat2[a][b] * F[v];
@@ -210,63 +262,39 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) acc[a][v] = -acc[a][v] * invJ;
 
-            double fv[NP][NEQ - 1];
-            if constexpr (VISC) {
-                const double *Fd = A.Fv + (size_t)dir * (NEQ - 1) * vs;
+            double nmag = m[0] * m[0];
 #pragma unroll
-                for (int a = 0; a < NP; a++)
-#pragma unroll
-                    for (int v = 0; v < NEQ - 1; v++) fv[a][v] = Fd[v * vs + (size_t)e * NN + base + a * st];
-#pragma unroll
-                for (int a = 0; a < NP; a++)
-#pragma unroll
-                    for (int v = 0; v < NEQ - 1; v++) {
-                        double t = 0.0;
-#pragma unroll
-                        for (int b = 0; b < NP; b++) t += A.T.Dhat[a][b] * fv[b][v];
-                        acc[a][1 + v] += t * invJ;
-                    }
-            }
+            for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
+            nmag = sqrt(nmag);
             // surface terms at both ends of the line
-            {
-                // + side: this element is el1, neighbour el2 (outward normal Ja^dir)
-                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + 1];
-                const int nown = base + (NP - 1) * st, nnb = base;
-                double s1[NEQ], s2[NEQ], f[NEQ];
+#pragma unroll
+            for (int side = 1; side >= 0; side--) {
+                // side 1: this element is el1 (outward normal Ja^dir), the neighbour el2;
+                // side 0: the neighbour is el1, this element el2
+                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + side];
+                const int nown = base + (side ? (NP - 1) * st : 0);
+                const int nnb = base + (side ? 0 : (NP - 1) * st);
+                double so[NEQ], sn[NEQ], f[NEQ];
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) {
-                    s1[v] = A.u[v * vs + (size_t)e * NN + nown];
-                    s2[v] = A.u[v * vs + (size_t)en * NN + nnb];
+                    so[v] = A.u[v * vs + (size_t)e * NN + nown];
+                    sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
                 }
-                face_flux<DIM, SURF>(s1, s2, m, A.P, A.bug3 != 0, f);
+                const SideState<DIM> qo = make_side<DIM>(so, A.P);
+                const SideState<DIM> qn = make_side<DIM>(sn, A.P);
+                if (side)
+                    face_flux<DIM, SURF>(so, qo, sn, qn, m, nmag, A.P, A.bug3 != 0, f);
+                else
+                    face_flux<DIM, SURF>(sn, qn, so, qo, m, nmag, A.P, A.bug3 != 0, f);
                 if constexpr (VISC) {
-                    const double *Fd = A.Fv + (size_t)dir * (NEQ - 1) * vs;
 #pragma unroll
                     for (int v = 0; v < NEQ - 1; v++)
-                        f[1 + v] -= 0.5 * (fv[NP - 1][v] + Fd[v * vs + (size_t)en * NN + nnb]);
+                        f[1 + v] -= 0.5 * (A.Ff[ftrace<DIM, NP>(dir, side, v, e, l, NE)] +
+                                           A.Ff[ftrace<DIM, NP>(dir, 1 - side, v, en, l, NE)]);
                 }
+                const double sg = side ? -sw : sw;
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) acc[NP - 1][v] -= f[v] * sw;
-            }
-            {
-                // - side: the neighbour is el1, this element el2
-                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir];
-                const int nown = base, nnb = base + (NP - 1) * st;
-                double s1[NEQ], s2[NEQ], f[NEQ];
-#pragma unroll
-                for (int v = 0; v < NEQ; v++) {
-                    s1[v] = A.u[v * vs + (size_t)en * NN + nnb];
-                    s2[v] = A.u[v * vs + (size_t)e * NN + nown];
-                }
-                face_flux<DIM, SURF>(s1, s2, m, A.P, A.bug3 != 0, f);
-                if constexpr (VISC) {
-                    const double *Fd = A.Fv + (size_t)dir * (NEQ - 1) * vs;
-#pragma unroll
-                    for (int v = 0; v < NEQ - 1; v++)
-                        f[1 + v] -= 0.5 * (Fd[v * vs + (size_t)en * NN + nnb] + fv[0][v]);
-                }
-#pragma unroll
-                for (int v = 0; v < NEQ; v++) acc[0][v] += f[v] * sw;
+                for (int v = 0; v < NEQ; v++) acc[side ? NP - 1 : 0][v] += f[v] * sg;
             }
 #pragma unroll
             for (int a = 0; a < NP; a++) {
@@ -289,30 +317,126 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK)) k_rhs(const RhsArgs
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
             const size_t gi = v * vs + off0 + idx;
-            const double du = sD[v][idx];
-            if (A.mode == EPI_MULT) {
-                A.out[gi] = du;
-            } else if (A.mode == EPI_LSRK) {
-                const double kk = A.ca * A.k[gi] + A.dt * du;
-                A.k[gi] = kk;
-                const double un = A.u[gi] + A.cb * kk;
-                A.out[gi] = un;
-                bad |= !(un == un);
-            } else {
-                const double un = A.ca * A.x[gi] + A.cb * (A.u[gi] + A.dt * du);
-                A.out[gi] = un;
-                bad |= !(un == un);
-            }
+            double du = sD[v][idx];
+            if (VISC && v > 0) du += A.dv[(v - 1) * vs + off0 + idx];
+            stage_update<NP>(A, gi, A.mode == EPI_MULT ? 0.0 : A.u[gi], du, bad);
         }
     }
-    if (A.want_lambda) {
-        // wave-level max, then one atomic per wave
-        for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
-        if ((tid & 63) == 0 && lam > 0.0) atomic_max_pos(A.lam_bits, lam);
+    finish_block<NP>(A, lam, bad);
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs_node (node variant): the same operator with one thread per node.  Each node
+// evaluates the two-point flux with its Np-1 line partners in every direction (the
+// symmetric pair is evaluated by both of its nodes), which needs no LDS accumulator
+// and keeps the node's own state in registers through the fused epilogue.
+// ---------------------------------------------------------------------------
+template <int DIM, int NP, bool VISC, int SURF>
+__global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES) k_rhs_node(const RhsArgs<NP> A) {
+    using C = NodeCfg<DIM, NP>;
+    constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB, NQ = DIM + 2;
+    __shared__ double sP[NQ][EPB * NN];
+    __shared__ double sT[1][NP * NP];  // Dhat2
+
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    const int eb = tid / NN, node = tid - (tid / NN) * NN;
+    const int e = blk * EPB + eb;
+    const int NE = A.NE;
+    const bool active = (eb < EPB) && (e < NE);
+    const size_t vs = (size_t)NE * NN;
+    if (tid < NP * NP) {
+        sT[0][tid] = A.T.Dhat2[tid / NP][tid % NP];
     }
-    if (A.nan_flag && __any(bad)) {
-        if ((tid & 63) == 0) atomicOr(A.nan_flag, 1);
+    double s[NEQ];
+    SideState<DIM> qs;
+    double lam = 0.0;
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
+        qs = make_side<DIM>(s, A.P);
+        sP[0][tid] = qs.rho;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) sP[1 + d][tid] = qs.vel[d];
+        sP[DIM + 1][tid] = qs.beta;
+        lam = qs.lam;
     }
+    __syncthreads();
+    int bad = 0;
+    if (active) {
+        const NodePrim<DIM> qo = to_prim(qs);
+        const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
+        const double invJ = rcp(gm[0]);
+        double adj[DIM][DIM];
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
+        int ai[3];
+        ai[0] = node % NP;
+        ai[1] = (node / NP) % NP;
+        ai[2] = node / (NP * NP);
+        double acc[NEQ], accs[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = 0.0;
+#pragma unroll
+        for (int dir = 0; dir < DIM; dir++) {
+            const int st = dir_stride<NP>(dir);
+            const int a = ai[dir];
+            const int lb = eb * NN + node - a * st;
+            const double *m = adj[dir];
+#pragma unroll 1
+            for (int k = 1; k < NP; k++) {
+                const int b = (a + k) % NP;
+                NodePrim<DIM> qb;
+                const int li = lb + b * st;
+                qb.rho = sP[0][li];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) qb.vel[d] = sP[1 + d][li];
+                qb.beta = sP[DIM + 1][li];
+                double F[NEQ];
+                volume_flux<DIM>(qo, qb, m, A.P.gammaM1Inverse, F);
+                const double dd = sT[0][a * NP + b];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+            }
+            if (a == 0 || a == NP - 1) {
+                const int side = (a == NP - 1) ? 1 : 0;
+                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + side];
+                const int nnb = node + (side ? -(NP - 1) * st : (NP - 1) * st);
+                double sn[NEQ], f[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
+                const SideState<DIM> qn = make_side<DIM>(sn, A.P);
+                double nmag = m[0] * m[0];
+#pragma unroll
+                for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
+                nmag = sqrt(nmag);
+                if (side)
+                    face_flux<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, f);
+                else
+                    face_flux<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, f);
+                if constexpr (VISC) {
+                    const int t = line_index<NP>(node, dir);
+#pragma unroll
+                    for (int v = 0; v < NEQ - 1; v++)
+                        f[1 + v] -= 0.5 * (A.Ff[ftrace<DIM, NP>(dir, side, v, e, t, NE)] +
+                                           A.Ff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
+                }
+                const double sg = side ? -1.0 : 1.0;
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) accs[v] += sg * f[v];
+            }
+        }
+        const double sw = A.T.inv_w0 * invJ;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            double du = -acc[v] * invJ + accs[v] * sw;
+            if (VISC && v > 0) du += A.dv[(v - 1) * vs + (size_t)e * NN + node];
+            stage_update<NP>(A, v * vs + (size_t)e * NN + node, s[v], du, bad);
+        }
+    }
+    finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -329,6 +453,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
     using C = NodeCfg<DIM, NP>;
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
     __shared__ double sW[NEQ][EPB * NN];
+    __shared__ double sF[DIM][NEQ - 1][EPB * NN];
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -344,81 +469,107 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
         for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
         conserv_to_entropy<DIM>(s, A.P, w);
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) sW[v][eb * NN + node] = w[v];
+        for (int v = 0; v < NEQ; v++) sW[v][tid] = w[v];
     }
     __syncthreads();
-    if (!active) return;
-
-    const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
-    const double invJ = 1.0 / gm[0];
-    double adj[DIM][DIM];
-#pragma unroll
-    for (int r = 0; r < DIM; r++)
-#pragma unroll
-        for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
 
     int ai[3];
     ai[0] = node % NP;
     ai[1] = (node / NP) % NP;
     ai[2] = node / (NP * NP);
+    double invJ = 0.0, adj[DIM][DIM];
+    if (active) {
+        const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
+        invJ = 1.0 / gm[0];
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
 
-    double gref[DIM][NEQ];
+        double gref[DIM][NEQ];
 #pragma unroll
-    for (int dir = 0; dir < DIM; dir++) {
-        const int st = dir_stride<NP>(dir);
-        const int a = ai[dir];
-        const int lb = eb * NN + node - a * st;
+        for (int dir = 0; dir < DIM; dir++) {
+            const int st = dir_stride<NP>(dir);
+            const int a = ai[dir];
+            const int lb = tid - a * st;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-            double t = 0.0;
+            for (int v = 0; v < NEQ; v++) {
+                double t = 0.0;
 #pragma unroll
-            for (int b = 0; b < NP; b++) t += A.T.D[a][b] * sW[v][lb + b * st];
-            gref[dir][v] = t;
+                for (int b = 0; b < NP; b++) t += A.T.D[a][b] * sW[v][lb + b * st];
+                gref[dir][v] = t;
+            }
+            if (a == NP - 1 || a == 0) {
+                // face jump 1/2 (w+ - w-) / w0, the same sign on both sides (LiftingBR1.cpp:51-65)
+                const bool plus = (a == NP - 1);
+                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + (plus ? 1 : 0)];
+                const int nn = node + (plus ? -(NP - 1) * st : (NP - 1) * st);
+                double sn[NEQ], wn[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nn];
+                conserv_to_entropy<DIM>(sn, A.P, wn);
+#pragma unroll
+                for (int v = 0; v < NEQ; v++)
+                    gref[dir][v] += 0.5 * (plus ? (wn[v] - w[v]) : (w[v] - wn[v])) * A.T.inv_w0;
+            }
         }
-        if (a == NP - 1 || a == 0) {
-            // face jump 1/2 (w+ - w-) / w0, the same sign on both sides (LiftingBR1.cpp:51-65)
-            const bool plus = (a == NP - 1);
-            const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + (plus ? 1 : 0)];
-            const int nn = node + (plus ? -(NP - 1) * st : (NP - 1) * st);
-            double sn[NEQ], wn[NEQ];
-#pragma unroll
-            for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nn];
-            conserv_to_entropy<DIM>(sn, A.P, wn);
-#pragma unroll
-            for (int v = 0; v < NEQ; v++)
-                gref[dir][v] += 0.5 * (plus ? (wn[v] - w[v]) : (w[v] - wn[v])) * A.T.inv_w0;
-        }
-    }
-    // physical gradient of w, then primitive gradients
-    double g[DIM][NEQ];
-#pragma unroll
-    for (int c = 0; c < DIM; c++)
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-            double t = 0.0;
-#pragma unroll
-            for (int r = 0; r < DIM; r++) t += gref[r][v] * adj[r][c];
-            g[c][v] = t * invJ;
-        }
-#pragma unroll
-    for (int c = 0; c < DIM; c++) entgrad_to_primgrad<DIM>(s, A.P, g[c]);
-    if (A.grad) {
+        // physical gradient of w, then primitive gradients
+        double g[DIM][NEQ];
 #pragma unroll
         for (int c = 0; c < DIM; c++)
 #pragma unroll
-            for (int v = 0; v < NEQ; v++) A.grad[((size_t)c * NEQ + v) * vs + (size_t)e * NN + node] = g[c][v];
-    }
-    double flux[DIM + 1][DIM];
-    viscous_flux<DIM>(s, g, A.P, flux);
+            for (int v = 0; v < NEQ; v++) {
+                double t = 0.0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++)
+                for (int r = 0; r < DIM; r++) t += gref[r][v] * adj[r][c];
+                g[c][v] = t * invJ;
+            }
+#pragma unroll
+        for (int c = 0; c < DIM; c++) entgrad_to_primgrad<DIM>(s, A.P, g[c]);
+        if (A.grad) {
+#pragma unroll
+            for (int c = 0; c < DIM; c++)
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) A.grad[((size_t)c * NEQ + v) * vs + (size_t)e * NN + node] = g[c][v];
+        }
+        double flux[DIM + 1][DIM];
+        viscous_flux<DIM>(s, g, A.P, flux);
+        // contravariant fluxes (MultABt(adj1, flux_mat1, flux_mat), DGSEMIntegrator.cpp:632) to
+        // LDS, and to the face traces at face points
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const int a = ai[d];
+            const int t = line_index<NP>(node, d);
+#pragma unroll
+            for (int v = 0; v < NEQ - 1; v++) {
+                double f = 0.0;
+#pragma unroll
+                for (int c = 0; c < DIM; c++) f += adj[d][c] * flux[v][c];
+                sF[d][v][tid] = f;
+                if (a == 0) A.Ff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
+                if (a == NP - 1) A.Ff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // viscous volume term (1/J) sum_d sum_l Dhat[a_d][l] F~^d_l  (DGSEMIntegrator.cpp:705-738)
+    double dvv[NEQ - 1];
+#pragma unroll
+    for (int v = 0; v < NEQ - 1; v++) dvv[v] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const int st = dir_stride<NP>(d);
+        const int a = ai[d];
+        const int lb = tid - a * st;
 #pragma unroll
         for (int v = 0; v < NEQ - 1; v++) {
-            double t = 0.0;
 #pragma unroll
-            for (int c = 0; c < DIM; c++) t += adj[d][c] * flux[v][c];
-            A.Fv[((size_t)d * (NEQ - 1) + v) * vs + (size_t)e * NN + node] = t;
+            for (int b = 0; b < NP; b++) dvv[v] += A.T.Dhat[a][b] * sF[d][v][lb + b * st];
         }
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ - 1; v++) A.dv[(size_t)v * vs + (size_t)e * NN + node] = dvv[v] * invJ;
 }
 
 // ---------------------------------------------------------------------------
@@ -457,8 +608,10 @@ __global__ void k_max_speed(const double *u, size_t nnodes, double gamma, unsign
         double s[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) s[v] = u[v * nnodes + i];
-        const NodePrim<DIM, double> q = make_prim<DIM, double>(s, gamma - 1.0);
-        lam = fmax(lam, wave_speed<DIM, double>(q, gamma));
+        PhysConst P{};
+        P.gamma = gamma;
+        P.gammaM1 = gamma - 1.0;
+        lam = fmax(lam, make_side<DIM>(s, P).lam);
     }
     for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
     if ((threadIdx.x & 63) == 0 && lam > 0.0) atomic_max_pos(lam_bits, lam);

@@ -17,162 +17,171 @@ struct PhysConst {
     int sutherland;
 };
 
-// Primitive state of one node as the two-point flux consumes it.  Every field is
-// computed with the reference's own per-call expressions (ChandrashekarFlux.cpp:17-68),
-// so evaluating them once per node gives the same bits as once per pair.
-template <int DIM, typename T>
-struct NodePrim {
-    T rho;
-    T vel[DIM];
-    T p;
-    T beta;  // 0.5 rho / p
-};
-
-// ComputeLogMean -- ref: src/BasicOperations/BasicOperations.cpp:6-12
-template <typename T>
-__device__ __forceinline__ T log_mean(T x, T y) {
-    const T xi = y / x;
-    const T u = (xi * (xi - T(2)) + T(1)) / (xi * (xi + T(2)) + T(1));
-    T r;
-    if (u < T(1e-4)) {
-        r = (x + y) * T(52.5) / (T(105) + u * (T(35) + u * (T(21) + T(15) * u)));
-    } else {
-        r = (y - x) / log(xi);
-    }
-    return r;
+// ---------------------------------------------------------------------------
+// Arithmetic helpers.  Divisions in the flux kernels are reciprocals: v_rcp_f64 seed
+// refined by two Newton-Raphson steps (within ~1 ulp; every argument is a positive,
+// normal density, pressure or sum of them).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
-template <int DIM, typename T>
-__device__ __forceinline__ NodePrim<DIM, T> make_prim(const T *s, T gammaM1) {
-    NodePrim<DIM, T> q;
+// Primitive state of one node as the two-point flux consumes it.
+template <int DIM>
+struct NodePrim {
+    double rho;
+    double vel[DIM];
+    double beta;  // rho / (2 p)  (ChandrashekarFlux.cpp:67)
+};
+
+// Primitive state plus what a surface flux needs of one side.
+template <int DIM>
+struct SideState {
+    double rho, irho, vel[DIM], vsq, p, beta, lam;
+};
+
+template <int DIM>
+__device__ __forceinline__ SideState<DIM> make_side(const double *s, const PhysConst &P) {
+    SideState<DIM> q;
     q.rho = s[0];
-    T vsq = T(0);
+    q.irho = rcp(q.rho);
+    q.vsq = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        q.vel[d] = s[1 + d] / q.rho;
+        q.vel[d] = s[1 + d] * q.irho;
+        q.vsq += q.vel[d] * q.vel[d];
     }
-    vsq = q.vel[0] * q.vel[0];
-#pragma unroll
-    for (int d = 1; d < DIM; d++) vsq += q.vel[d] * q.vel[d];
-    q.p = gammaM1 * (s[DIM + 1] - T(0.5) * q.rho * vsq);   // ChandrashekarFlux.cpp:58
-    q.beta = T(0.5) * q.rho / q.p;                          // :67
+    q.p = P.gammaM1 * (s[DIM + 1] - 0.5 * q.rho * q.vsq);       // ChandrashekarFlux.cpp:58
+    const double por = q.p * q.irho;
+    q.beta = 0.5 * rcp(por);                                   // :67  rho / (2 p)
+    q.lam = sqrt(q.vsq) + sqrt(P.gamma * por);                 // :61-65
     return q;
 }
 
-// sqrt(|V|^2) + c -- ChandrashekarFlux.cpp:61-65, ComputeSoundSpeed BasicOperations.cpp:53-56
-template <int DIM, typename T>
-__device__ __forceinline__ T wave_speed(const NodePrim<DIM, T> &q, T gamma) {
-    T vsq = q.vel[0] * q.vel[0];
+template <int DIM>
+__device__ __forceinline__ NodePrim<DIM> to_prim(const SideState<DIM> &q) {
+    NodePrim<DIM> r;
+    r.rho = q.rho;
 #pragma unroll
-    for (int d = 1; d < DIM; d++) vsq += q.vel[d] * q.vel[d];
-    return sqrt(vsq) + sqrt(gamma * q.p / q.rho);
+    for (int d = 0; d < DIM; d++) r.vel[d] = q.vel[d];
+    r.beta = q.beta;
+    return r;
+}
+
+// 1 / L(x, y), L the logarithmic mean, given r = 1/(x+y).
+// ref: ComputeLogMean, src/BasicOperations/BasicOperations.cpp:6-12.  With f = (y-x)/(y+x)
+// and u = f^2 (= the reference's (xi-1)^2/(xi+1)^2, xi = y/x), ln(y/x) = 2 artanh f, so
+// L = (x+y)/2 / (1 + u/3 + u^2/5 + u^3/7 + ...); the reference's series branch
+// (x+y) 52.5/(105 + u(35 + u(21 + 15u))) is that expansion to u^3.  Same branch point u < 1e-4;
+// the reciprocal form lets one 1/(x+y) serve the flux's other terms.
+__device__ __forceinline__ double inv_log_mean(double x, double y, double r) {
+    const double f = (y - x) * r;
+    const double u = f * f;
+    double il;
+    if (u < 1e-4) {
+        il = 2.0 * r * (1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0))));
+    } else {
+        il = log(y * rcp(x)) * rcp(y - x);
+    }
+    return il;
 }
 
 // ChandrashekarFlux::ComputeVolumeFlux -- ref: src/RiemannSolvers/ChandrashekarFlux.cpp:11-83.
 // m is the (already averaged) metric row Ja^dir; F receives DIM+2 values.
-template <int DIM, typename T>
-__device__ __forceinline__ void volume_flux(const NodePrim<DIM, T> &a, const NodePrim<DIM, T> &b, const T *m,
-                                            T gammaM1Inverse, T *F) {
-    const T rho_ln = log_mean(a.rho, b.rho);
-    T mean[DIM];
-    T h_hat = T(0), Vn = T(0);
+template <int DIM>
+__device__ __forceinline__ void volume_flux(const NodePrim<DIM> &a, const NodePrim<DIM> &b, const double *m,
+                                            double gammaM1Inverse, double *F) {
+    const double rsum = a.rho + b.rho;
+    const double r_rho = rcp(rsum);
+    const double r_b = rcp(a.beta + b.beta);
+    const double il_rho = inv_log_mean(a.rho, b.rho, r_rho);
+    const double il_b = inv_log_mean(a.beta, b.beta, r_b);
+    const double rho_ln = rcp(il_rho);
+    double mean[DIM];
+    double h_hat = 0.0, Vn = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        mean[d] = T(0.5) * (a.vel[d] + b.vel[d]);
-        const T hh = T(-0.25) * (a.vel[d] * a.vel[d] + b.vel[d] * b.vel[d]) + mean[d] * mean[d];
-        h_hat = (d == 0) ? hh : h_hat + hh;
-        Vn = (d == 0) ? mean[0] * m[0] : Vn + mean[d] * m[d];
+        mean[d] = 0.5 * (a.vel[d] + b.vel[d]);
+        h_hat += -0.25 * (a.vel[d] * a.vel[d] + b.vel[d] * b.vel[d]) + mean[d] * mean[d];
+        Vn += mean[d] * m[d];
     }
-    const T beta_ln = log_mean(a.beta, b.beta);
-    const T p_hat = T(0.5) * (a.rho + b.rho) / (a.beta + b.beta);          // :71
-    h_hat += T(0.5) / beta_ln * gammaM1Inverse + p_hat / rho_ln;            // :73
-    F[0] = rho_ln * Vn;
+    const double p_hat = 0.5 * rsum * r_b;                                // :71
+    h_hat += 0.5 * il_b * gammaM1Inverse + p_hat * il_rho;                 // :73
+    const double mflux = rho_ln * Vn;
+    F[0] = mflux;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) F[1 + d] = Vn * (rho_ln * mean[d]) + p_hat * m[d];
-    F[DIM + 1] = rho_ln * Vn * h_hat;
+    for (int d = 0; d < DIM; d++) F[1 + d] = mflux * mean[d] + p_hat * m[d];
+    F[DIM + 1] = mflux * h_hat;
 }
 
 // Euler flux . n (textbook; the reference takes it from mfem::EulerFlux)
 template <int DIM>
-__device__ __forceinline__ double euler_flux_n(const double *s, const double *n, double gammaM1, double gamma,
-                                               double *fn) {
-    const double rho = s[0];
-    double vel[DIM], vsq = 0.0, vn = 0.0;
+__device__ __forceinline__ void euler_flux_n(const double *s, const SideState<DIM> &q, const double *n, double *fn) {
+    double vn = 0.0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        vel[d] = s[1 + d] / rho;
-        vsq += vel[d] * vel[d];
-        vn += vel[d] * n[d];
-    }
-    const double p = gammaM1 * (s[DIM + 1] - 0.5 * rho * vsq);
-    fn[0] = rho * vn;
+    for (int d = 0; d < DIM; d++) vn += q.vel[d] * n[d];
+    fn[0] = q.rho * vn;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) fn[1 + d] = s[1 + d] * vn + p * n[d];
-    fn[DIM + 1] = (s[DIM + 1] + p) * vn;
-    return sqrt(vsq) + sqrt(gamma * p / rho);
+    for (int d = 0; d < DIM; d++) fn[1 + d] = s[1 + d] * vn + q.p * n[d];
+    fn[DIM + 1] = (s[DIM + 1] + q.p) * vn;
 }
 
 enum SurfaceKind { SURF_CHANDRASHEKAR = 0, SURF_EC = 1, SURF_RUSANOV = 2 };
 
 // Surface flux plug-ins.  SURF_CHANDRASHEKAR -- ref: ChandrashekarFlux::ComputeFaceFlux,
-// ChandrashekarFlux.cpp:85-161 (BUG3 reproduces the :131 typo "rho_mean*dw + w_mean");
+// ChandrashekarFlux.cpp:85-161 (bug3 reproduces the :131 typo "rho_mean*dw + w_mean");
 // SURF_EC -- the volume flux with metric = n; SURF_RUSANOV -- local Lax-Friedrichs on the
-// central flux.  s1/s2 are conserved states, n is the face area vector outward from s1.
+// central flux.  s1/s2 are conserved states with their SideStates, n the face area vector
+// outward from side 1, nmag = |n|.
 template <int DIM, int SURF>
-__device__ __forceinline__ void face_flux(const double *s1, const double *s2, const double *n, const PhysConst &P,
-                                          bool bug3, double *flux) {
+__device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM> &q1, const double *s2,
+                                          const SideState<DIM> &q2, const double *n, double nmag,
+                                          const PhysConst &P, bool bug3, double *flux) {
     if constexpr (SURF == SURF_CHANDRASHEKAR) {
-        double nor_mag = n[0] * n[0];
-#pragma unroll
-        for (int d = 1; d < DIM; d++) nor_mag += n[d] * n[d];
-        nor_mag = sqrt(nor_mag);
-        const double rho1 = s1[0], rho2 = s2[0];
-        const double rho_mean = 0.5 * (rho1 + rho2);
-        const double rho_ln = log_mean(rho1, rho2);
-        const double drho = rho2 - rho1;
-        double v1[DIM], v2[DIM], vm[DIM];
-        double Vsq1 = 0.0, Vsq2 = 0.0, Vn = 0.0, h_hat = 0.0, diss = 0.0;
+        const double rsum = q1.rho + q2.rho;
+        const double r_rho = rcp(rsum);
+        const double r_b = rcp(q1.beta + q2.beta);
+        const double il_rho = inv_log_mean(q1.rho, q2.rho, r_rho);
+        const double il_b = inv_log_mean(q1.beta, q2.beta, r_b);
+        const double rho_ln = rcp(il_rho);
+        const double rho_mean = 0.5 * rsum;
+        const double drho = q2.rho - q1.rho;
+        double vm[DIM];
+        double Vn = 0.0, h_hat = 0.0, diss = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            v1[d] = s1[1 + d] / rho1;
-            v2[d] = s2[1 + d] / rho2;
-            vm[d] = 0.5 * (v1[d] + v2[d]);
-            const double dv = v2[d] - v1[d];
-            Vsq1 += v1[d] * v1[d];
-            Vsq2 += v2[d] * v2[d];
+            vm[d] = 0.5 * (q1.vel[d] + q2.vel[d]);
+            const double dv = q2.vel[d] - q1.vel[d];
             Vn += vm[d] * n[d];
-            h_hat += -0.25 * (v1[d] * v1[d] + v2[d] * v2[d]) + vm[d] * vm[d];
+            h_hat += -0.25 * (q1.vel[d] * q1.vel[d] + q2.vel[d] * q2.vel[d]) + vm[d] * vm[d];
             if (DIM == 3 && d == 2 && bug3)
-                diss += 0.5 * drho * v1[d] * v2[d] + rho_mean * dv + vm[d];        // :131 as written
+                diss += 0.5 * drho * q1.vel[d] * q2.vel[d] + rho_mean * dv + vm[d];     // :131 as written
             else
-                diss += 0.5 * drho * v1[d] * v2[d] + rho_mean * dv * vm[d];        // :105,:118,:131
+                diss += 0.5 * drho * q1.vel[d] * q2.vel[d] + rho_mean * dv * vm[d];     // :105,:118,:131
         }
-        const double p1 = P.gammaM1 * (s1[DIM + 1] - 0.5 * rho1 * Vsq1);
-        const double p2 = P.gammaM1 * (s2[DIM + 1] - 0.5 * rho2 * Vsq2);
-        const double lam = fmax(sqrt(Vsq1) + sqrt(P.gamma * p1 / rho1), sqrt(Vsq2) + sqrt(P.gamma * p2 / rho2));
-        const double beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;
-        const double beta_ln = log_mean(beta1, beta2);
-        const double p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);
-        h_hat += 0.5 / beta_ln * P.gammaM1Inverse + p_hat / rho_ln;
-        diss += 0.5 * drho * P.gammaM1Inverse / beta_ln +
-                0.5 * rho_mean * P.gammaM1Inverse * (1.0 / beta2 - 1.0 / beta1);           // :151
-        const double hl = 0.5 * lam * nor_mag;
-        flux[0] = rho_ln * Vn - hl * (rho2 - rho1);
+        const double lam = fmax(q1.lam, q2.lam);
+        const double p_hat = rho_mean * r_b;
+        h_hat += 0.5 * il_b * P.gammaM1Inverse + p_hat * il_rho;
+        // 1/beta = 2 p / rho
+        diss += 0.5 * drho * P.gammaM1Inverse * il_b +
+                rho_mean * P.gammaM1Inverse * (q2.p * q2.irho - q1.p * q1.irho);        // :151
+        const double hl = 0.5 * lam * nmag;
+        const double mflux = rho_ln * Vn;
+        flux[0] = mflux - hl * drho;
 #pragma unroll
-        for (int d = 0; d < DIM; d++) flux[1 + d] = Vn * (rho_ln * vm[d]) + p_hat * n[d] - hl * (s2[1 + d] - s1[1 + d]);
-        flux[DIM + 1] = rho_ln * Vn * h_hat - hl * diss;
+        for (int d = 0; d < DIM; d++) flux[1 + d] = mflux * vm[d] + p_hat * n[d] - hl * (s2[1 + d] - s1[1 + d]);
+        flux[DIM + 1] = mflux * h_hat - hl * diss;
     } else if constexpr (SURF == SURF_EC) {
-        const NodePrim<DIM, double> a = make_prim<DIM, double>(s1, P.gammaM1);
-        const NodePrim<DIM, double> b = make_prim<DIM, double>(s2, P.gammaM1);
-        volume_flux<DIM, double>(a, b, n, P.gammaM1Inverse, flux);
+        volume_flux<DIM>(to_prim(q1), to_prim(q2), n, P.gammaM1Inverse, flux);
     } else {
         double f1[DIM + 2], f2[DIM + 2];
-        const double l1 = euler_flux_n<DIM>(s1, n, P.gammaM1, P.gamma, f1);
-        const double l2 = euler_flux_n<DIM>(s2, n, P.gammaM1, P.gamma, f2);
-        double nm = n[0] * n[0];
-#pragma unroll
-        for (int d = 1; d < DIM; d++) nm += n[d] * n[d];
-        const double hl = 0.5 * fmax(l1, l2) * sqrt(nm);
+        euler_flux_n<DIM>(s1, q1, n, f1);
+        euler_flux_n<DIM>(s2, q2, n, f2);
+        const double hl = 0.5 * fmax(q1.lam, q2.lam) * nmag;
 #pragma unroll
         for (int v = 0; v < DIM + 2; v++) flux[v] = 0.5 * (f1[v] + f2[v]) - hl * (s2[v] - s1[v]);
     }
