@@ -68,6 +68,8 @@ def _configure(L):
     L.orc_viscous_flux.restype = None
     L.orc_rhs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, dp, dp, dp, dp]
     L.orc_rhs.restype = ctypes.c_int
+    L.orc_rhs_mag.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, dp, dp, dp, dp, dp, dp]
+    L.orc_rhs_mag.restype = ctypes.c_int
     L.orc_rk_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
                              ctypes.c_int, ctypes.c_double, dp]
     L.orc_rk_run.restype = ctypes.c_int
@@ -181,6 +183,29 @@ def rhs(mesh, ph, u, want_grad=False, fma=False):
     if want_grad:
         return dudt, lam.value, grad
     return dudt, lam.value
+
+
+def rhs_mag(mesh, ph, u, want_grad=False):
+    """RHS with the rounding scale of each entry: (dudt, lambda, mag[, grad, gmag]).
+    mag is the same assembly with every term in absolute value (see orc_rhs_mag)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    dudt = np.zeros_like(u)
+    mag = np.zeros_like(u)
+    grad = np.zeros((mesh.dim,) + u.shape) if want_grad else None
+    gmag = np.zeros((mesh.dim,) + u.shape) if want_grad else None
+    lam = ctypes.c_double(0.0)
+    rc = lib().orc_rhs_mag(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+                           ctypes.byref(ph), _dp(u), _dp(dudt), _dp(grad), ctypes.byref(lam), _dp(mag), _dp(gmag))
+    assert rc == 0, rc
+    if want_grad:
+        return dudt, lam.value, mag, grad, gmag
+    return dudt, lam.value, mag
+
+
+def scaled_error(a, ref, mag) -> float:
+    """max |a - ref| / (eps * mag): the difference in units of the entry's rounding scale."""
+    eps = np.finfo(np.float64).eps
+    return float(np.max(np.abs(a - ref) / (eps * mag + np.finfo(np.float64).tiny)))
 
 
 def rk_run(mesh, ph, u, scheme: int, nsteps: int, dt: float, fma=False):

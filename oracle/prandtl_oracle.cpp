@@ -333,6 +333,41 @@ void ViscousFlux(const Phys &P, const double *state, const double *dqdx, const d
     }
 }
 
+// Rounding scale of ViscousFlux: the same formula with every term in absolute value and
+// the gradients replaced by their rounding scales (gmag >= |grad| entrywise).
+void ViscousFluxMag(const Phys &P, const double *state, const double *ax, const double *ay, const double *az,
+                    double *flux) {
+    const int dim = P.dim, neq = P.neq;
+    const double *g[3] = {ax, ay, az};
+    const double rho = state[0];
+    double vel[3], vsq = 0.0;
+    for (int d = 0; d < dim; d++) {
+        vel[d] = state[1 + d] / rho;
+        vsq += vel[d] * vel[d];
+    }
+    const double p = P.gammaM1 * (state[neq - 1] - 0.5 * rho * vsq);
+    double mu = P.mu;
+    if (P.sutherland) {
+        double T = p / (rho * P.R_gas);
+        mu = P.mu0 * (P.T0 + P.Ts) / (T + P.Ts) * (T / P.T0) * std::sqrt(T / P.T0);
+    }
+    mu = std::fabs(mu);
+    const double lambda = mu * P.gamma * P.PrInverse;
+    double div = 0.0;
+    for (int d = 0; d < dim; d++) div += g[d][1 + d];
+    for (int i = 0; i < neq * dim; i++) flux[i] = 0.0;
+    for (int c = 0; c < dim; c++) {
+        double e = 0.0;
+        for (int r = 0; r < dim; r++) {
+            const double t = (r == c) ? mu * (2.0 * g[c][1 + r] + P.mu_bulk * div) : mu * (g[c][1 + r] + g[r][1 + c]);
+            flux[(1 + r) * dim + c] = t;
+            e += std::fabs(vel[r]) * t;
+        }
+        const double cv = P.gammaM1Inverse / rho * (g[c][neq - 1] + std::fabs(p / rho) * g[c][0]);
+        flux[(neq - 1) * dim + c] = e + lambda * cv;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // GLL rule on [0,1] (MFEM IntegrationRules(0, GaussLobatto), DGSEMIntegrator.cpp:18-37),
 // restated from its closed form: interior nodes are the roots of P'_N, found by
@@ -449,7 +484,7 @@ inline int line_node(const Mesh &M, int d, int a, int t1, int t2) {
 // fluxes F_viscous/G_viscous/H_viscous, and the A1 bug under P.bug & 1).
 // ---------------------------------------------------------------------------
 double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, const double *Fvis,
-                       double *el_dudt) {
+                       double *el_dudt, double *el_mag, const double *FvisMag) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq;
     const int Npy = dim > 1 ? np : 1, Npz = dim > 2 ? np : 1;
     double lam = 0.0;
@@ -544,6 +579,23 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
                     if (Fvis) dU = dU + dUv[v];                           // add :744
                     el_dudt[v * n + id1] += dU * (1.0 / J);               // :403 / :745 (Vector /= c is *= 1/c), AddRow
                 }
+                if (el_mag) {
+                    // rounding scale: the same sums with every term in absolute value
+                    for (int v = 0; v < neq; v++) {
+                        double a = 0.0;
+                        for (int m = 0; m < np; m++) {
+                            a += std::fabs(FI(Fi, id1, v, m) * M.Dhat2[i * np + m]);
+                            if (dim > 1) a += std::fabs(FI(Gi, id1, v, m) * M.Dhat2[j * np + m]);
+                            if (dim > 2) a += std::fabs(FI(Hi, id1, v, m) * M.Dhat2[k * np + m]);
+                            if (FvisMag) {
+                                a += FvisMag[(0 * n + M.lex(m, j, k)) * neq + v] * std::fabs(M.Dhat[i * np + m]);
+                                if (dim > 1) a += FvisMag[(1 * n + M.lex(i, m, k)) * neq + v] * std::fabs(M.Dhat[j * np + m]);
+                                if (dim > 2) a += FvisMag[(2 * n + M.lex(i, j, m)) * neq + v] * std::fabs(M.Dhat[k * np + m]);
+                            }
+                        }
+                        el_mag[v * n + id1] += a / std::fabs(J);
+                    }
+                }
             }
     return lam;
 }
@@ -568,7 +620,7 @@ inline void gather(const Mesh &M, int neq, const double *u, int e, double *el) {
 // ComputeGlobalPrimitiveGradVector :96-159.  grad holds dim arrays in the
 // layout of u.
 // ---------------------------------------------------------------------------
-void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *grad) {
+void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *grad, double *gmag) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq, NE = M.NE;
     const size_t size = (size_t)neq * NE * n;
     std::vector<double> went(size);
@@ -608,6 +660,23 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                             for (int r = 0; r < dim; r++) acc += gm1[v][r] * M.adj(e, r, c);
                             grad[(size_t)c * size + gidx(M, v, e, id)] = acc * (1.0 / J);
                         }
+                    if (gmag) {
+                        double am[MAXEQ][3];
+                        for (int v = 0; v < neq; v++)
+                            for (int r = 0; r < 3; r++) am[v][r] = 0.0;
+                        for (int l = 0; l < np; l++)
+                            for (int v = 0; v < neq; v++) {
+                                am[v][0] += std::fabs(went[gidx(M, v, e, M.lex(l, j, k))] * M.D[i * np + l]);
+                                if (dim > 1) am[v][1] += std::fabs(went[gidx(M, v, e, M.lex(i, l, k))] * M.D[j * np + l]);
+                                if (dim > 2) am[v][2] += std::fabs(went[gidx(M, v, e, M.lex(i, j, l))] * M.D[k * np + l]);
+                            }
+                        for (int v = 0; v < neq; v++)
+                            for (int c = 0; c < dim; c++) {
+                                double acc = 0.0;
+                                for (int r = 0; r < dim; r++) acc += am[v][r] * std::fabs(M.adj(e, r, c));
+                                gmag[(size_t)c * size + gidx(M, v, e, id)] = acc / std::fabs(J);
+                            }
+                    }
                 }
     }
     // face lifting: 1/2 (w2 - w1) n_c / (w0 J) added to both sides
@@ -628,6 +697,12 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                             const double fc = f * nor[c];
                             grad[(size_t)c * size + gidx(M, v, e1, n1)] += 1.0 / (M.w[0] * J1) * fc;
                             grad[(size_t)c * size + gidx(M, v, e2, n2)] += 1.0 / (M.w[0] * J2) * fc;
+                            if (gmag) {
+                                const double a = 0.5 * (std::fabs(went[gidx(M, v, e2, n2)]) +
+                                                        std::fabs(went[gidx(M, v, e1, n1)])) * std::fabs(nor[c]);
+                                gmag[(size_t)c * size + gidx(M, v, e1, n1)] += a / (M.w[0] * std::fabs(J1));
+                                gmag[(size_t)c * size + gidx(M, v, e2, n2)] += a / (M.w[0] * std::fabs(J2));
+                            }
                         }
                     }
                 }
@@ -643,6 +718,28 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                 for (int v = 0; v < neq; v++) g[v] = grad[(size_t)c * size + gidx(M, v, e, node)];
                 EntropyGrad2PrimGrad(P, s, g);
                 for (int v = 0; v < neq; v++) grad[(size_t)c * size + gidx(M, v, e, node)] = g[v];
+                if (gmag) {
+                    // the chain rule applied to the magnitudes, term by term in absolute value
+                    double a[MAXEQ], o[MAXEQ];
+                    for (int v = 0; v < neq; v++) a[v] = gmag[(size_t)c * size + gidx(M, v, e, node)];
+                    const double rho = s[0];
+                    double vel[3], vsq = 0.0;
+                    for (int d = 0; d < dim; d++) {
+                        vel[d] = s[1 + d] / rho;
+                        vsq += vel[d] * vel[d];
+                    }
+                    const double p = P.gammaM1 * (s[neq - 1] - 0.5 * rho * vsq);
+                    const double por = std::fabs(p / rho), ae = a[neq - 1];
+                    double mdot = 0.0;
+                    for (int d = 0; d < dim; d++) {
+                        o[1 + d] = por * (a[1 + d] + std::fabs(vel[d]) * ae);
+                        mdot += std::fabs(s[1 + d]) * o[1 + d];
+                    }
+                    o[0] = rho * a[0] + ae * (0.5 * rho * vsq + std::fabs(p) * P.gammaM1Inverse) +
+                           std::fabs(rho / p) * mdot;
+                    o[neq - 1] = por * (o[0] + std::fabs(p) * ae);
+                    for (int v = 0; v < neq; v++) gmag[(size_t)c * size + gidx(M, v, e, node)] = o[v];
+                }
             }
         }
     }
@@ -655,28 +752,37 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
 // DGSEMIntegrator::AssembleFaceVector (Euler :151-222, NS 3D :435-494, NS 2D :496-549).
 // Returns lambda_max over all pair and face flux evaluations (:270,:188).
 // ---------------------------------------------------------------------------
-double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, double *grad_out) {
+double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, double *grad_out, double *mag = nullptr,
+               double *gmag = nullptr) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq, NE = M.NE;
     const size_t size = (size_t)neq * NE * n;
-    std::vector<double> grad_local;
+    std::vector<double> grad_local, gmag_local;
     double *grad = grad_out;
     if (P.viscous) {
         if (!grad) {
             grad_local.resize(size * dim);
             grad = grad_local.data();
         }
-        LiftingAndPrimGrad(M, P, u, grad);
+        if (mag && !gmag) {
+            gmag_local.resize(size * dim);
+            gmag = gmag_local.data();
+        }
+        LiftingAndPrimGrad(M, P, u, grad, gmag);
     }
     std::fill(dudt, dudt + size, 0.0);
+    if (mag) std::fill(mag, mag + size, 0.0);
     double lam = 0.0;
     // element loop
 #pragma omp parallel for schedule(static) reduction(max : lam)
     for (int e = 0; e < NE; e++) {
-        std::vector<double> el_u((size_t)n * neq), el_dudt((size_t)n * neq, 0.0), Fvis;
+        std::vector<double> el_u((size_t)n * neq), el_dudt((size_t)n * neq, 0.0), el_mag, Fvis;
+        if (mag) el_mag.assign((size_t)n * neq, 0.0);
         gather(M, neq, u, e, el_u.data());
+        std::vector<double> FvisMag;
         if (P.viscous) {
             // per-node viscous flux, mapped with adj -- :617-641
             Fvis.assign((size_t)3 * n * neq, 0.0);
+            if (mag) FvisMag.assign((size_t)3 * n * neq, 0.0);
             double s[MAXEQ], dq[3][MAXEQ], flux[MAXEQ * 3];
             for (int i = 0; i < n; i++) {
                 // A2: the reference reads state1 at the stale id1 (:620); under bug_compat
@@ -693,12 +799,28 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                         for (int c = 0; c < dim; c++) acc += M.adj(e, r, c) * flux[v * dim + c];
                         Fvis[((size_t)r * n + i) * neq + v] = acc;
                     }
+                if (mag) {
+                    double gq[3][MAXEQ], fm[MAXEQ * 3];
+                    for (int c = 0; c < dim; c++)
+                        for (int v = 0; v < neq; v++) gq[c][v] = gmag[(size_t)c * size + gidx(M, v, e, i)];
+                    ViscousFluxMag(P, s, gq[0], gq[1], dim > 2 ? gq[2] : nullptr, fm);
+                    for (int r = 0; r < dim; r++)
+                        for (int v = 0; v < neq; v++) {
+                            double acc = 0.0;
+                            for (int c = 0; c < dim; c++) acc += std::fabs(M.adj(e, r, c)) * fm[v * dim + c];
+                            FvisMag[((size_t)r * n + i) * neq + v] = acc;
+                        }
+                }
             }
         }
-        double l = ElementInviscid(M, P, e, el_u.data(), P.viscous ? Fvis.data() : nullptr, el_dudt.data());
+        double l = ElementInviscid(M, P, e, el_u.data(), P.viscous ? Fvis.data() : nullptr, el_dudt.data(),
+                                   mag ? el_mag.data() : nullptr, FvisMag.empty() ? nullptr : FvisMag.data());
         lam = std::max(lam, l);
         for (int v = 0; v < neq; v++)
             for (int node = 0; node < n; node++) dudt[gidx(M, v, e, node)] += el_dudt[(size_t)v * n + node];
+        if (mag)
+            for (int v = 0; v < neq; v++)
+                for (int node = 0; node < n; node++) mag[gidx(M, v, e, node)] = el_mag[(size_t)v * n + node];
     }
     // interior-face loop.  Faces of one direction touch disjoint node sets of
     // each element, so a direction's faces run in parallel; directions run in turn.
@@ -719,10 +841,11 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                         s2[v] = u[gidx(M, v, e2, n2)];
                     }
                     lam = std::max(lam, FaceFlux(P, s1, s2, nor, fnum));
-                    double dU1[MAXEQ], dU2[MAXEQ];
+                    double dU1[MAXEQ], dU2[MAXEQ], aU[MAXEQ];
                     for (int v = 0; v < neq; v++) {
                         dU1[v] = -fnum[v];
                         dU2[v] = fnum[v];
+                        aU[v] = std::fabs(fnum[v]);
                     }
                     if (P.viscous) {
                         for (int c = 0; c < dim; c++)
@@ -731,16 +854,34 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                         for (int c = 0; c < dim; c++)
                             for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e2, n2)];
                         ViscousFlux(P, s2, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, fl2);
+                        double fm1[MAXEQ * 3], fm2[MAXEQ * 3];
+                        if (mag) {
+                            for (int c = 0; c < dim; c++)
+                                for (int v = 0; v < neq; v++) dq[c][v] = gmag[(size_t)c * size + gidx(M, v, e1, n1)];
+                            ViscousFluxMag(P, s1, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, fm1);
+                            for (int c = 0; c < dim; c++)
+                                for (int v = 0; v < neq; v++) dq[c][v] = gmag[(size_t)c * size + gidx(M, v, e2, n2)];
+                            ViscousFluxMag(P, s2, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, fm2);
+                        }
                         for (int v = 0; v < neq; v++) {
                             double fv = 0.0;                   // (flux_mat1 + flux_mat2) * 0.5, Mult(nor)
-                            for (int c = 0; c < dim; c++) fv += ((fl1[v * dim + c] + fl2[v * dim + c]) * 0.5) * nor[c];
+                            double af = 0.0;
+                            for (int c = 0; c < dim; c++) {
+                                fv += ((fl1[v * dim + c] + fl2[v * dim + c]) * 0.5) * nor[c];
+                                if (mag) af += 0.5 * (fm1[v * dim + c] + fm2[v * dim + c]) * std::fabs(nor[c]);
+                            }
                             dU1[v] += fv;
                             dU2[v] -= fv;
+                            aU[v] += af;
                         }
                     }
                     for (int v = 0; v < neq; v++) {
                         dudt[gidx(M, v, e1, n1)] += 1.0 / (M.w[0] * J1) * dU1[v];   // AddMult_a_VWt :219
                         dudt[gidx(M, v, e2, n2)] += 1.0 / (M.w[0] * J2) * dU2[v];   // :220
+                        if (mag) {
+                            mag[gidx(M, v, e1, n1)] += aU[v] / (M.w[0] * std::fabs(J1));
+                            mag[gidx(M, v, e2, n2)] += aU[v] / (M.w[0] * std::fabs(J2));
+                        }
                     }
                 }
         }
@@ -865,13 +1006,27 @@ void orc_viscous_flux(int dim, const orc_phys *p, const double *s, const double 
 
 // RHS of the semi-discretisation: dudt = L(u).  grad (optional, dim*size) receives
 // the primitive gradients (NS only).  lambda_max (optional) receives the max wave speed.
+int orc_rhs_mag(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
+                const double *u, double *dudt, double *grad, double *lambda_max, double *mag, double *gmag);
+
 int orc_rhs(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
             const double *u, double *dudt, double *grad, double *lambda_max) {
+    return orc_rhs_mag(dim, order, NE, nbr, geom, p, u, dudt, grad, lambda_max, nullptr, nullptr);
+}
+
+// As orc_rhs, plus the rounding scale of every entry: mag (size of dudt) holds the same
+// assembly with every term in absolute value, gmag (size of grad) the same for the lifted
+// gradients and their chain rule.  |a - b| <= C eps mag bounds the difference of two
+// evaluations of the same sums in different order or contraction.
+int orc_rhs_mag(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
+                const double *u, double *dudt, double *grad, double *lambda_max, double *mag, double *gmag) {
     if (dim < 2 || dim > 3 || order < 1 || order + 1 > MAXNP) return -1;
     Mesh M;
     MakeMesh(M, dim, order, NE, nbr, geom);
     Phys P = phys_of(dim, p);
-    double lam = RhsEval(M, P, u, dudt, grad);
+    std::vector<double> gm_local;
+    if (gmag && !grad) return -3;
+    double lam = RhsEval(M, P, u, dudt, grad, mag, gmag);
     if (lambda_max) *lambda_max = lam;
     return 0;
 }

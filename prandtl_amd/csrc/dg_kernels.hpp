@@ -32,6 +32,7 @@
 #define PRANDTL_NODE_WAVES 4
 #endif
 
+
 namespace prandtl {
 
 constexpr int ipow(int b, int e) {
@@ -445,21 +446,36 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 //      LiftingBR1::AssembleLiftingElementVector (3D :156-219, 2D :222-272),
 //      LiftingBR1::AssembleLiftingFaceVector (3D :10-66, 2D :68-115),
 //      ComputeGlobalPrimitiveGradVector (:96-159), NavierStokesFlux::ComputeViscousFlux,
-//      and the adj(J) mapping of the viscous flux at DGSEMIntegrator.cpp:632-640.
-// One thread per node.
+//      the adj(J) mapping of the viscous flux (DGSEMIntegrator.cpp:632-640) and its
+//      D-hat divergence (:705-738).
+// One thread per node.  The entropy variables of the face neighbours (2*dim*Np^(dim-1)
+// face points per element) are evaluated as cooperative items shared by all threads of
+// the workgroup (full lane occupancy) and staged in LDS.
 // ---------------------------------------------------------------------------
 template <int DIM, int NP>
-__global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftArgs<NP> A) {
-    using C = NodeCfg<DIM, NP>;
-    constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
+struct LiftCfg {
+    static constexpr int NN = ipow(NP, DIM);
+    static constexpr int L = ipow(NP, DIM - 1);
+    static constexpr int FITEMS = 2 * DIM * L;   // face points per element
+    static constexpr int BLOCK = NN <= 32 ? 64 : (NN <= 64 ? 128 : 64 * ((NN + 63) / 64));
+    static constexpr int EPB = BLOCK / NN;
+};
+
+template <int DIM, int NP>
+__global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftArgs<NP> A) {
+    using C = LiftCfg<DIM, NP>;
+    constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB, L = C::L;
+    constexpr int NW = NEQ * EPB * C::FITEMS, NFV = DIM * (NEQ - 1) * EPB * NN;
     __shared__ double sW[NEQ][EPB * NN];
-    __shared__ double sF[DIM][NEQ - 1][EPB * NN];
+    __shared__ double sU[NW > NFV ? NW : NFV];  // neighbour w at face points, then F~^d
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     const int eb = tid / NN, node = tid - (tid / NN) * NN;
-    const int e = blk * EPB + eb;
+    const int e0 = blk * EPB;
+    const int e = e0 + eb;
     const int NE = A.NE;
+    const int nel = min(EPB, NE - e0);
     const bool active = (eb < EPB) && (e < NE);
     const size_t vs = (size_t)NE * NN;
 
@@ -471,21 +487,33 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
 #pragma unroll
         for (int v = 0; v < NEQ; v++) sW[v][tid] = w[v];
     }
+    for (int it = tid; it < nel * C::FITEMS; it += C::BLOCK) {
+        const int fe = it / C::FITEMS, i = it - fe * C::FITEMS;
+        const int dir = i / (2 * L), side = (i / L) & 1, t = i % L;
+        const int st = dir_stride<NP>(dir);
+        const int en = A.nbr[(size_t)(e0 + fe) * 2 * DIM + 2 * dir + side];
+        const int nnb = line_base<NP>(t, dir) + (side ? 0 : (NP - 1) * st);
+        double sn[NEQ], wn[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
+        conserv_to_entropy<DIM>(sn, A.P, wn);
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sU[(v * EPB + fe) * C::FITEMS + i] = wn[v];
+    }
     __syncthreads();
 
     int ai[3];
     ai[0] = node % NP;
     ai[1] = (node / NP) % NP;
     ai[2] = node / (NP * NP);
-    double invJ = 0.0, adj[DIM][DIM];
+    double invJ = 0.0, adj[DIM][DIM], Fc[DIM][NEQ - 1];
     if (active) {
         const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
-        invJ = 1.0 / gm[0];
+        invJ = rcp(gm[0]);
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
             for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
-
         double gref[DIM][NEQ];
 #pragma unroll
         for (int dir = 0; dir < DIM; dir++) {
@@ -502,18 +530,14 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
             if (a == NP - 1 || a == 0) {
                 // face jump 1/2 (w+ - w-) / w0, the same sign on both sides (LiftingBR1.cpp:51-65)
                 const bool plus = (a == NP - 1);
-                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + (plus ? 1 : 0)];
-                const int nn = node + (plus ? -(NP - 1) * st : (NP - 1) * st);
-                double sn[NEQ], wn[NEQ];
+                const int i = dir * 2 * L + (plus ? L : 0) + line_index<NP>(node, dir);
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nn];
-                conserv_to_entropy<DIM>(sn, A.P, wn);
-#pragma unroll
-                for (int v = 0; v < NEQ; v++)
-                    gref[dir][v] += 0.5 * (plus ? (wn[v] - w[v]) : (w[v] - wn[v])) * A.T.inv_w0;
+                for (int v = 0; v < NEQ; v++) {
+                    const double wn = sU[(v * EPB + eb) * C::FITEMS + i];
+                    gref[dir][v] += 0.5 * (plus ? (wn - w[v]) : (w[v] - wn)) * A.T.inv_w0;
+                }
             }
         }
-        // physical gradient of w, then primitive gradients
         double g[DIM][NEQ];
 #pragma unroll
         for (int c = 0; c < DIM; c++)
@@ -534,8 +558,6 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
         }
         double flux[DIM + 1][DIM];
         viscous_flux<DIM>(s, g, A.P, flux);
-        // contravariant fluxes (MultABt(adj1, flux_mat1, flux_mat), DGSEMIntegrator.cpp:632) to
-        // LDS, and to the face traces at face points
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             const int a = ai[d];
@@ -545,15 +567,21 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
                 double f = 0.0;
 #pragma unroll
                 for (int c = 0; c < DIM; c++) f += adj[d][c] * flux[v][c];
-                sF[d][v][tid] = f;
+                Fc[d][v] = f;
                 if (a == 0) A.Ff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
                 if (a == NP - 1) A.Ff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;
             }
         }
     }
+    __syncthreads();  // sU (neighbour w) dead: reuse it for F~^d
+    if (active) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+#pragma unroll
+            for (int v = 0; v < NEQ - 1; v++) sU[(d * (NEQ - 1) + v) * EPB * NN + tid] = Fc[d][v];
+    }
     __syncthreads();
     if (!active) return;
-    // viscous volume term (1/J) sum_d sum_l Dhat[a_d][l] F~^d_l  (DGSEMIntegrator.cpp:705-738)
     double dvv[NEQ - 1];
 #pragma unroll
     for (int v = 0; v < NEQ - 1; v++) dvv[v] = 0.0;
@@ -563,10 +591,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
         const int a = ai[d];
         const int lb = tid - a * st;
 #pragma unroll
-        for (int v = 0; v < NEQ - 1; v++) {
+        for (int v = 0; v < NEQ - 1; v++)
 #pragma unroll
-            for (int b = 0; b < NP; b++) dvv[v] += A.T.Dhat[a][b] * sF[d][v][lb + b * st];
-        }
+            for (int b = 0; b < NP; b++) dvv[v] += A.T.Dhat[a][b] * sU[(d * (NEQ - 1) + v) * EPB * NN + lb + b * st];
     }
 #pragma unroll
     for (int v = 0; v < NEQ - 1; v++) A.dv[(size_t)v * vs + (size_t)e * NN + node] = dvv[v] * invJ;

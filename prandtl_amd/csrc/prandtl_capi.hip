@@ -159,7 +159,7 @@ template <int DIM, int NP, bool VISC, int SURF>
 int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     if constexpr (VISC) {
       if (S.phases & 1) {
-        using NC = NodeCfg<DIM, NP>;
+        using NC = LiftCfg<DIM, NP>;
         LiftArgs<NP> la;
         la.T = make_tables<NP>(op);
         la.P = op->P;

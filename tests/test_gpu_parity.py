@@ -1,16 +1,20 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
-Bar: relative max-norm difference of the RHS (and of RK states after a few steps) on
-the same seeded inputs, for every dimension, order, surface flux and the Euler /
-Navier-Stokes paths, of at most
+Bar, on the same seeded inputs, for every dimension, order, surface flux and the Euler /
+Navier-Stokes paths, at every entry of the RHS (and of the primitive gradients):
 
-    tol = max(1e-12, 2 * floor),
+    |gpu - oracle| <= C eps mag,   C = 64,
 
-where floor is the relative max-norm difference between two builds of the oracle
-itself (without and with FMA contraction) on the same input: the rounding spread of
-the restated algorithm.  On pressure-dominated states (Taylor-Green, p ~ 71, RHS O(1))
-the momentum fluxes cancel to three digits and floor reaches ~1e-12; elsewhere the
-bar is 1e-12.  At the full benchmark size (TGV 32^3, p=3) parity is shown through
+where mag is the oracle's rounding scale of that entry: the same assembly with every
+term in absolute value (oracle.rhs_mag).  The relative max-norm difference
+||gpu - oracle||_inf / ||oracle||_inf is printed beside it, with the same quantity for two
+builds of the oracle itself (without and with FMA contraction): on pressure-dominated
+states (Taylor-Green, p ~ 71, RHS O(1)) the momentum and energy sums cancel by three to
+five digits and that spread alone reaches 7e-13 at 6^3 and 4e-11 at 32^3, so a fixed
+1e-12 relative bar cannot separate rounding from error there; the entrywise bound does.
+On well-conditioned states the relative difference is printed and also held to 1e-12.
+RK steps (a few steps from a smooth state) are held to 1e-12 relative.  At the full
+benchmark size (TGV 32^3, p=3) the RHS is compared entrywise as well, plus
 size-independent properties: discrete conservation, free-stream preservation, bitwise
 determinism, entropy conservation with the EC flux.
 """
@@ -33,12 +37,24 @@ def relerr(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-def assert_parity(gpu, ref, ref_fma, what=""):
+C_ROUND = 64.0
+
+
+def assert_parity(gpu, ref, ref_fma, what="", mag=None):
+    """Entrywise |gpu - ref| <= C eps mag when mag is given; otherwise relative <= 1e-12."""
     floor = oracle.rounding_floor(ref, ref_fma)
-    tol = max(TOL, 2.0 * floor)
     err = relerr(gpu, ref)
-    print(f"[parity] {what} rel.err={err:.3e} oracle floor={floor:.3e} tol={tol:.3e}", flush=True)
-    assert err <= tol, (err, floor)
+    if mag is not None:
+        sc = oracle.scaled_error(gpu, ref, mag)
+        scf = oracle.scaled_error(ref_fma, ref, mag)
+        print(f"[parity] {what} rel.err={err:.3e} (oracle fma spread {floor:.3e}); "
+              f"entrywise {sc:.2f} eps*mag (oracle fma spread {scf:.2f}, bound {C_ROUND:.0f})", flush=True)
+        assert sc <= C_ROUND, (sc, err, floor)
+        if floor <= 1e-13:
+            assert err <= TOL, (err, floor)
+    else:
+        print(f"[parity] {what} rel.err={err:.3e} (oracle fma spread {floor:.3e}, bound {TOL:.0e})", flush=True)
+        assert err <= TOL, (err, floor)
 
 
 def sheared_mesh(dim, order, n):
@@ -88,9 +104,9 @@ def test_vortex_2d_euler(device, surf):
     # element-wise jitter so surface fluxes see real jumps
     u = u * (1.0 + 1e-3 * np.random.default_rng(1).uniform(-1, 1, u.shape))
     d, lam, _ = gpu_rhs(m, u, device, surface_flux=surf)
-    ref, lref = oracle.rhs(m, oracle.phys(surface_flux=surf), u)
+    ref, lref, mag = oracle.rhs_mag(m, oracle.phys(surface_flux=surf), u)
     reff, _ = oracle.rhs(m, oracle.phys(surface_flux=surf), u, fma=True)
-    assert_parity(d, ref, reff, "vortex")
+    assert_parity(d, ref, reff, "vortex", mag)
     assert abs(lam - lref) <= 1e-14 * lref
 
 
@@ -104,9 +120,9 @@ def test_sheared_all_orders(device, dim, order, viscous):
     u = perturbed_state(m, 10 * dim + order)
     kw = dict(viscous=viscous, mu=0.02, Pr=0.71)
     d, lam, _ = gpu_rhs(m, u, device, **kw)
-    ref, lref = oracle.rhs(m, oracle.phys(**kw), u)
+    ref, lref, mag = oracle.rhs_mag(m, oracle.phys(**kw), u)
     reff, _ = oracle.rhs(m, oracle.phys(**kw), u, fma=True)
-    assert_parity(d, ref, reff, "sheared")
+    assert_parity(d, ref, reff, "sheared", mag)
     assert abs(lam - lref) <= 1e-14 * lref
 
 
@@ -119,11 +135,11 @@ def test_tgv_ns_3d(device, surf):
     ph = cases.tgv_physics()
     d, lam, g = gpu_rhs(m, u, device, grad=True, surface_flux=surf, **ph)
     oph = oracle.phys(surface_flux=surf, gamma=ph["gamma"], Pr=ph["Pr"], mu=ph["mu"], viscous=True)
-    ref, lref, gref = oracle.rhs(m, oph, u, want_grad=True)
+    ref, lref, mag, gref, gmag = oracle.rhs_mag(m, oph, u, want_grad=True)
     reff, _, greff = oracle.rhs(m, oph, u, want_grad=True, fma=True)
-    assert_parity(d, ref, reff, "tgv ns")
+    assert_parity(d, ref, reff, "tgv ns", mag)
     # primitive gradients (the reference's dudx/dudy/dudz side effect)
-    assert_parity(g, gref, greff, "tgv ns gradients")
+    assert_parity(g, gref, greff, "tgv ns gradients", gmag)
     assert abs(lam - lref) <= 1e-14 * lref
 
 
@@ -134,9 +150,9 @@ def test_bug_a3_3d(device):
     for visc in (False, True):
         kw = dict(viscous=visc, mu=0.01, Pr=0.7)
         d, _, _ = gpu_rhs(m, u, device, bug_compat=capi.BUG_A3, **kw)
-        ref, _ = oracle.rhs(m, oracle.phys(bug_compat=oracle.BUG_A3, **kw), u)
+        ref, _, mag = oracle.rhs_mag(m, oracle.phys(bug_compat=oracle.BUG_A3, **kw), u)
         reff, _ = oracle.rhs(m, oracle.phys(bug_compat=oracle.BUG_A3, **kw), u, fma=True)
-        assert_parity(d, ref, reff, "bug A3")
+        assert_parity(d, ref, reff, "bug A3", mag)
 
 
 @pytest.mark.parametrize("scheme", [capi.SSPRK3, capi.LSRK45])
@@ -170,6 +186,27 @@ def test_rk_steps(device, scheme, viscous):
         op.rk_step(scheme, ud2, 0.0, dt)
     torch.cuda.synchronize()
     assert torch.equal(ud, ud2)
+    op.close()
+
+
+def test_full_size_tgv_parity(device):
+    """TGV 32^3 p=3 NS (the benchmark workload): RHS against the oracle at full size."""
+    import torch
+
+    _say("TGV 32^3 p=3 NS full-size parity vs oracle")
+    oracle.set_threads(16)
+    m = cases.tgv_mesh(n=32, order=3)
+    u = cases.tgv_state(m)
+    ph = cases.tgv_physics()
+    op = capi.Operator(m, **ph)
+    ud = torch.from_numpy(u).to(device)
+    du = torch.empty_like(ud)
+    op.mult(ud, du)
+    torch.cuda.synchronize()
+    oph = oracle.phys(gamma=ph["gamma"], Pr=ph["Pr"], mu=ph["mu"], viscous=True)
+    ref, _, mag = oracle.rhs_mag(m, oph, u)
+    reff, _ = oracle.rhs(m, oph, u, fma=True)
+    assert_parity(du.cpu().numpy(), ref, reff, "tgv32 ns", mag)
     op.close()
 
 

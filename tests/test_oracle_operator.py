@@ -163,3 +163,24 @@ def test_bug_compat_switches():
     de, _ = oracle.rhs(m, oracle.phys(), u)
     de2, _ = oracle.rhs(m, oracle.phys(bug_compat=oracle.BUG_A1 | oracle.BUG_A2), u)
     assert np.array_equal(de, de2)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_rounding_scale_bounds_build_spread(dim, viscous):
+    """mag bounds the difference between two compilations of the same oracle (with and
+    without FMA contraction): the entrywise yardstick the GPU parity tests use."""
+    m = sheared_mesh(dim, 3, (3,) * dim)
+    u = perturbed_state(m, 8)
+    ph = oracle.phys(viscous=viscous, mu=0.02, Pr=0.7)
+    r = oracle.rhs_mag(m, ph, u, want_grad=viscous)
+    rf = oracle.rhs(m, ph, u, want_grad=viscous, fma=True)
+    assert np.array_equal(r[0], oracle.rhs(m, ph, u)[0])
+    assert np.all(r[2] >= np.abs(r[0]) * (1 - 1e-12))
+    assert oracle.scaled_error(rf[0], r[0], r[2]) < 16.0
+    if viscous:
+        assert oracle.scaled_error(rf[2], r[3], r[4]) < 16.0
+    # a planted mistake in a single term is far outside the bound
+    bad = r[0].copy()
+    bad[1, 0, 0] += 1e-9 * max(1.0, abs(bad[1, 0, 0]))
+    assert oracle.scaled_error(bad, r[0], r[2]) > 100.0
