@@ -31,12 +31,14 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic HBM bytes per node per launch (compulsory traffic; face-neighbour trace
-# re-reads excluded) -- FP64, 3D, 5 conserved variables:
-#   k_lift : read u (40) + write F~v, 3 directions x 4 values (96)           = 136
-#   k_rhs  : Mult epilogue: read u (40) + F~v (96) + write dudt (40)         = 176
-#            LSRK epilogue: read u (40) + F~v (96) + k (40), write k, u (80)  = 256
-BYTES_PER_NODE = {"lift": 136.0, "rhs_mult": 176.0, "rhs_lsrk": 256.0}
+# Algorithmic HBM bytes per node per launch (compulsory traffic: every array read or
+# written once; the second read of a face trace by the neighbouring element excluded)
+# -- FP64, 3D, p=3, 5 conserved variables:
+#   k_lift : read u (40); write the viscous volume term dv (4 x 8 = 32) and the face
+#            traces Ff (6 faces x 16 points x 4 values x 8 B / 64 nodes = 48)      = 120
+#   k_rhs  : Mult epilogue: read u (40) + dv (32) + Ff (48), write dudt (40)        = 160
+#            LSRK epilogue: read u, dv, Ff, k (160), write k and u' (80)            = 240
+BYTES_PER_NODE = {"lift": 120.0, "rhs_mult": 160.0, "rhs_lsrk": 240.0}
 
 
 def log(msg):
@@ -70,6 +72,75 @@ def cpu_baseline(mesh, u, phys, threads, budget_s=15.0):
     }
 
 
+def ke_terms(u, du, W):
+    """Volume-averaged kinetic energy, its rate from the RHS, and the total entropy S = sum W rho s."""
+    vel = [u[1 + k] / u[0] for k in range(3)]
+    v2 = sum(v * v for v in vel)
+    ke = (W * 0.5 * u[0] * v2).sum() / W.sum()
+    rate = (W * (sum(vel[k] * du[1 + k] for k in range(3)) - 0.5 * v2 * du[0])).sum() / W.sum()
+    p = 0.4 * (u[4] - 0.5 * u[0] * v2)
+    ent = (W * u[0] * (p.log() - 1.4 * u[0].log())).sum() / W.sum()
+    return float(ke), float(rate), float(ent)
+
+
+def compare_mixed(args, dev):
+    """FP64 and FP32-mixed TGV runs side by side: throughput of each and the mixed run's errors
+    against the FP64 run (density L2, kinetic-energy dissipation rate, entropy drift)."""
+    import torch
+
+    from prandtl_amd import capi, cases
+
+    mesh = cases.tgv_mesh(n=args.n, order=args.order)
+    phys = cases.tgv_physics()
+    u0 = torch.from_numpy(cases.tgv_state(mesh)).to(dev)
+    W = torch.from_numpy(mesh.quadrature_weights()).to(dev)
+    dt = cases.cfl_dt(mesh, 11.0, args.cfl)
+    nsteps = int(math.ceil(args.t_final / dt))
+    dt = args.t_final / nsteps
+    nout = 10
+    chunk = [nsteps // nout + (1 if i < nsteps % nout else 0) for i in range(nout)]
+    runs = {}
+    for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+        op = capi.Operator(mesh, precision=prec, **phys)
+        u = u0.clone()
+        du = torch.empty_like(u)
+        hist = []
+        op.mult(u, du)
+        hist.append((0.0,) + ke_terms(u, du, W))
+        t, ms = 0.0, 0.0
+        for c in chunk:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            t = op.rk_run(capi.LSRK45, u, t, dt, c)
+            e1.record()
+            torch.cuda.synchronize()
+            ms += e0.elapsed_time(e1)
+            op.mult(u, du)
+            hist.append((t,) + ke_terms(u, du, W))
+            log(f"{name}: t={t:.4f} K={hist[-1][1]:.8f} -dK/dt={-hist[-1][2]:.6e}")
+        runs[name] = dict(u=u, hist=hist, ms=ms, value=mesh.num_nodes * 5 * nsteps / (ms * 1e-3))
+        op.close()
+    h64, hmx = runs["fp64"]["hist"], runs["mixed"]["hist"]
+    eps_err = max(abs(b[2] - a[2]) / abs(a[2]) for a, b in zip(h64, hmx))
+    rho_l2 = float(((W * (runs["mixed"]["u"][0] - runs["fp64"]["u"][0]) ** 2).sum() /
+                    (W * runs["fp64"]["u"][0] ** 2).sum()).sqrt())
+    ent_drift = {k: (r["hist"][-1][3] - r["hist"][0][3]) / abs(r["hist"][0][3]) for k, r in runs.items()}
+    out = {
+        "metric": "DOF-RHS evals/s, 3D TGV p=3 NS: FP32-mixed vs FP64 (LSRK4(5), stage update fused)",
+        "config": {"workload": f"TGV Navier-Stokes {args.n}^3 hexes p={args.order} ({mesh.num_nodes} nodes), "
+                               f"Ma=0.1 Re=1600, to t={args.t_final} in {nsteps} steps (dt={dt:.4e})"},
+        "fp64_value": runs["fp64"]["value"],
+        "mixed_value": runs["mixed"]["value"],
+        "mixed_speedup": runs["mixed"]["value"] / runs["fp64"]["value"],
+        "ke_dissipation_rate_max_rel_err": eps_err,
+        "density_l2_rel_diff_at_t_final": rho_l2,
+        "entropy_drift_rel": ent_drift,
+        "history": {k: r["hist"] for k, r in runs.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +151,10 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--compare-mixed", action="store_true",
+                    help="run the FP32-mixed TGV comparison (FP64 and mixed side by side to --t-final) "
+                         "instead of the default benchmark")
+    ap.add_argument("--t-final", type=float, default=2.0)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -95,6 +170,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+    if args.compare_mixed:
+        compare_mixed(args, dev)
+        return
 
     mesh = cases.tgv_mesh(n=args.n, order=args.order)
     phys = cases.tgv_physics()

@@ -23,6 +23,8 @@
 // conforming affine mesh F~^d at a face point is F_v . n for that face.
 #pragma once
 
+#include <type_traits>
+
 #include "dg_physics.hpp"
 
 #ifndef PRANDTL_LINE_WAVES
@@ -86,6 +88,8 @@ struct RhsArgs {
     const double *u;       // stage input state
     const double *dv;      // NS: viscous volume term [v-1][e][node]
     const double *Ff;      // NS: viscous face traces F~^d [d][side][v-1][e][t]
+    const float *dvf;      // the same, FP32 (mixed precision)
+    const float *Fff;
     double *out;           // EPI_MULT: dudt; otherwise the stage output state
     double *k;             // EPI_LSRK: 2N register
     const double *x;       // EPI_SSP: base state of the Shu-Osher combination
@@ -104,6 +108,8 @@ struct LiftArgs {
     const double *u;
     double *dv;            // out: viscous volume term [v-1][e][node]
     double *Ff;            // out: viscous face traces [d][side][v-1][e][t]
+    float *dvf;            // out, FP32 (mixed precision)
+    float *Fff;
     double *grad;          // optional out: primitive gradients [c][v][e][node]
 };
 
@@ -332,12 +338,15 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
 // symmetric pair is evaluated by both of its nodes), which needs no LDS accumulator
 // and keeps the node's own state in registers through the fused epilogue.
 // ---------------------------------------------------------------------------
-template <int DIM, int NP, bool VISC, int SURF>
+template <int DIM, int NP, bool VISC, int SURF, bool MIX>
 __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES) k_rhs_node(const RhsArgs<NP> A) {
     using C = NodeCfg<DIM, NP>;
-    constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB, NQ = DIM + 2;
-    __shared__ double sP[NQ][EPB * NN];
-    __shared__ double sT[1][NP * NP];  // Dhat2
+    using R = typename std::conditional<MIX, float, double>::type;  // flux arithmetic
+    constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
+    __shared__ double sPd[2][EPB * NN];   // rho, beta
+    __shared__ R sPv[DIM][EPB * NN];      // velocity
+    __shared__ R sT[NP * NP];             // Dhat2
+    __shared__ double sPref[EPB];         // element reference pressure (mixed precision)
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -346,9 +355,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     const int NE = A.NE;
     const bool active = (eb < EPB) && (e < NE);
     const size_t vs = (size_t)NE * NN;
-    if (tid < NP * NP) {
-        sT[0][tid] = A.T.Dhat2[tid / NP][tid % NP];
-    }
+    if (tid < NP * NP) sT[tid] = (R)A.T.Dhat2[tid / NP][tid % NP];
     double s[NEQ];
     SideState<DIM> qs;
     double lam = 0.0;
@@ -356,48 +363,64 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
         for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
         qs = make_side<DIM>(s, A.P);
-        sP[0][tid] = qs.rho;
+        sPd[0][tid] = qs.rho;
+        sPd[1][tid] = qs.beta;
 #pragma unroll
-        for (int d = 0; d < DIM; d++) sP[1 + d][tid] = qs.vel[d];
-        sP[DIM + 1][tid] = qs.beta;
+        for (int d = 0; d < DIM; d++) sPv[d][tid] = (R)qs.vel[d];
+        if (MIX && node == 0) sPref[eb] = qs.p;
         lam = qs.lam;
     }
     __syncthreads();
     int bad = 0;
     if (active) {
-        const NodePrim<DIM> qo = to_prim(qs);
         const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
         const double invJ = rcp(gm[0]);
-        double adj[DIM][DIM];
+        R adj[DIM][DIM];
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
-            for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
+            for (int c = 0; c < DIM; c++) adj[r][c] = (R)gm[1 + r * DIM + c];
+        const double p_ref = MIX ? sPref[eb] : 0.0;
         int ai[3];
         ai[0] = node % NP;
         ai[1] = (node / NP) % NP;
         ai[2] = node / (NP * NP);
-        double acc[NEQ], accs[NEQ];
+        R acc[NEQ], accs[NEQ];
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = 0.0;
+        for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
 #pragma unroll
         for (int dir = 0; dir < DIM; dir++) {
             const int st = dir_stride<NP>(dir);
             const int a = ai[dir];
             const int lb = eb * NN + node - a * st;
-            const double *m = adj[dir];
+            const R *m = adj[dir];
 #pragma unroll 1
             for (int k = 1; k < NP; k++) {
                 const int b = (a + k) % NP;
-                NodePrim<DIM> qb;
                 const int li = lb + b * st;
-                qb.rho = sP[0][li];
+                R F[NEQ];
+                if constexpr (MIX) {
+                    NodePrimMx<DIM> qo, qb;
+                    qo.rho = qs.rho;
+                    qo.beta = qs.beta;
+                    qb.rho = sPd[0][li];
+                    qb.beta = sPd[1][li];
 #pragma unroll
-                for (int d = 0; d < DIM; d++) qb.vel[d] = sP[1 + d][li];
-                qb.beta = sP[DIM + 1][li];
-                double F[NEQ];
-                volume_flux<DIM>(qo, qb, m, A.P.gammaM1Inverse, F);
-                const double dd = sT[0][a * NP + b];
+                    for (int d = 0; d < DIM; d++) {
+                        qo.vel[d] = (float)qs.vel[d];
+                        qb.vel[d] = sPv[d][li];
+                    }
+                    volume_flux_mx<DIM>(qo, qb, m, (float)A.P.gammaM1Inverse, p_ref, F);
+                } else {
+                    const NodePrim<DIM> qo = to_prim(qs);
+                    NodePrim<DIM> qb;
+                    qb.rho = sPd[0][li];
+                    qb.beta = sPd[1][li];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
+                    volume_flux<DIM>(qo, qb, m, A.P.gammaM1Inverse, F);
+                }
+                const R dd = sT[a * NP + b];
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
             }
@@ -405,36 +428,58 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 const int side = (a == NP - 1) ? 1 : 0;
                 const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + side];
                 const int nnb = node + (side ? -(NP - 1) * st : (NP - 1) * st);
-                double sn[NEQ], f[NEQ];
+                double sn[NEQ];
+                R f[NEQ];
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
                 const SideState<DIM> qn = make_side<DIM>(sn, A.P);
-                double nmag = m[0] * m[0];
+                R nmag = m[0] * m[0];
 #pragma unroll
                 for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
                 nmag = sqrt(nmag);
-                if (side)
-                    face_flux<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, f);
-                else
-                    face_flux<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, f);
+                if constexpr (MIX) {
+                    if (side)
+                        face_flux_mx<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, p_ref, f);
+                    else
+                        face_flux_mx<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, p_ref, f);
+                } else {
+                    if (side)
+                        face_flux<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, f);
+                    else
+                        face_flux<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, f);
+                }
                 if constexpr (VISC) {
                     const int t = line_index<NP>(node, dir);
 #pragma unroll
-                    for (int v = 0; v < NEQ - 1; v++)
-                        f[1 + v] -= 0.5 * (A.Ff[ftrace<DIM, NP>(dir, side, v, e, t, NE)] +
-                                           A.Ff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
+                    for (int v = 0; v < NEQ - 1; v++) {
+                        if constexpr (MIX)
+                            f[1 + v] -= 0.5f * (A.Fff[ftrace<DIM, NP>(dir, side, v, e, t, NE)] +
+                                                A.Fff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
+                        else
+                            f[1 + v] -= 0.5 * (A.Ff[ftrace<DIM, NP>(dir, side, v, e, t, NE)] +
+                                               A.Ff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
+                    }
                 }
-                const double sg = side ? -1.0 : 1.0;
+                const R sg = side ? R(-1) : R(1);
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) accs[v] += sg * f[v];
             }
         }
-        const double sw = A.T.inv_w0 * invJ;
+        const R sw = (R)(A.T.inv_w0 * invJ);
+        const R rinvJ = (R)invJ;
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
-            double du = -acc[v] * invJ + accs[v] * sw;
-            if (VISC && v > 0) du += A.dv[(v - 1) * vs + (size_t)e * NN + node];
-            stage_update<NP>(A, v * vs + (size_t)e * NN + node, s[v], du, bad);
+            R du = -acc[v] * rinvJ + accs[v] * sw;
+            if constexpr (VISC) {
+                if (v > 0) {
+                    if constexpr (MIX)
+                        du += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
+                    else
+                        du += A.dv[(v - 1) * vs + (size_t)e * NN + node];
+                }
+            }
+            // the RK accumulation is FP64 in both modes
+            stage_update<NP>(A, v * vs + (size_t)e * NN + node, s[v], (double)du, bad);
         }
     }
     finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
@@ -461,13 +506,18 @@ struct LiftCfg {
     static constexpr int EPB = BLOCK / NN;
 };
 
-template <int DIM, int NP>
+template <int DIM, int NP, bool MIX>
 __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftArgs<NP> A) {
     using C = LiftCfg<DIM, NP>;
+    // derivative contractions, chain rule and viscous flux in R; entropy variables in FP64.
+    // Mixed precision contracts w - w_ref (w_ref = the element's node-0 value; D annihilates
+    // constants) so the FP32 values carry the variation, not the ~1e1 offset of w.
+    using R = typename std::conditional<MIX, float, double>::type;
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB, L = C::L;
     constexpr int NW = NEQ * EPB * C::FITEMS, NFV = DIM * (NEQ - 1) * EPB * NN;
-    __shared__ double sW[NEQ][EPB * NN];
-    __shared__ double sU[NW > NFV ? NW : NFV];  // neighbour w at face points, then F~^d
+    __shared__ R sW[NEQ][EPB * NN];
+    __shared__ R sU[NW > NFV ? NW : NFV];  // neighbour w at face points, then F~^d
+    __shared__ double sWref[EPB][NEQ];
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -484,8 +534,18 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
 #pragma unroll
         for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
         conserv_to_entropy<DIM>(s, A.P, w);
+        if (MIX && node == 0) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) sW[v][tid] = w[v];
+            for (int v = 0; v < NEQ; v++) sWref[eb][v] = w[v];
+        }
+    }
+    if (MIX) __syncthreads();
+    double wref[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) wref[v] = (MIX && active) ? sWref[eb][v] : 0.0;
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sW[v][tid] = (R)(w[v] - wref[v]);
     }
     for (int it = tid; it < nel * C::FITEMS; it += C::BLOCK) {
         const int fe = it / C::FITEMS, i = it - fe * C::FITEMS;
@@ -498,7 +558,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
         for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
         conserv_to_entropy<DIM>(sn, A.P, wn);
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) sU[(v * EPB + fe) * C::FITEMS + i] = wn[v];
+        for (int v = 0; v < NEQ; v++) sU[(v * EPB + fe) * C::FITEMS + i] = (R)(wn[v] - (MIX ? sWref[fe][v] : 0.0));
     }
     __syncthreads();
 
@@ -506,15 +566,18 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
     ai[0] = node % NP;
     ai[1] = (node / NP) % NP;
     ai[2] = node / (NP * NP);
-    double invJ = 0.0, adj[DIM][DIM], Fc[DIM][NEQ - 1];
+    R invJ = R(0), adj[DIM][DIM], Fc[DIM][NEQ - 1];
     if (active) {
         const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
-        invJ = rcp(gm[0]);
+        invJ = (R)rcp(gm[0]);
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
-            for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
-        double gref[DIM][NEQ];
+            for (int c = 0; c < DIM; c++) adj[r][c] = (R)gm[1 + r * DIM + c];
+        R wrel[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) wrel[v] = (R)(w[v] - wref[v]);
+        R gref[DIM][NEQ];
 #pragma unroll
         for (int dir = 0; dir < DIM; dir++) {
             const int st = dir_stride<NP>(dir);
@@ -522,9 +585,9 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
             const int lb = tid - a * st;
 #pragma unroll
             for (int v = 0; v < NEQ; v++) {
-                double t = 0.0;
+                R t = R(0);
 #pragma unroll
-                for (int b = 0; b < NP; b++) t += A.T.D[a][b] * sW[v][lb + b * st];
+                for (int b = 0; b < NP; b++) t += (R)A.T.D[a][b] * sW[v][lb + b * st];
                 gref[dir][v] = t;
             }
             if (a == NP - 1 || a == 0) {
@@ -533,43 +596,52 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
                 const int i = dir * 2 * L + (plus ? L : 0) + line_index<NP>(node, dir);
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) {
-                    const double wn = sU[(v * EPB + eb) * C::FITEMS + i];
-                    gref[dir][v] += 0.5 * (plus ? (wn - w[v]) : (w[v] - wn)) * A.T.inv_w0;
+                    const R wn = sU[(v * EPB + eb) * C::FITEMS + i];
+                    gref[dir][v] += R(0.5) * (plus ? (wn - wrel[v]) : (wrel[v] - wn)) * (R)A.T.inv_w0;
                 }
             }
         }
-        double g[DIM][NEQ];
+        R g[DIM][NEQ];
 #pragma unroll
         for (int c = 0; c < DIM; c++)
 #pragma unroll
             for (int v = 0; v < NEQ; v++) {
-                double t = 0.0;
+                R t = R(0);
 #pragma unroll
                 for (int r = 0; r < DIM; r++) t += gref[r][v] * adj[r][c];
                 g[c][v] = t * invJ;
             }
+        R sr[NEQ];
 #pragma unroll
-        for (int c = 0; c < DIM; c++) entgrad_to_primgrad<DIM>(s, A.P, g[c]);
+        for (int v = 0; v < NEQ; v++) sr[v] = (R)s[v];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) entgrad_to_primgrad<DIM, R>(sr, A.P, g[c]);
         if (A.grad) {
 #pragma unroll
             for (int c = 0; c < DIM; c++)
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) A.grad[((size_t)c * NEQ + v) * vs + (size_t)e * NN + node] = g[c][v];
+                for (int v = 0; v < NEQ; v++)
+                    A.grad[((size_t)c * NEQ + v) * vs + (size_t)e * NN + node] = (double)g[c][v];
         }
-        double flux[DIM + 1][DIM];
-        viscous_flux<DIM>(s, g, A.P, flux);
+        R flux[DIM + 1][DIM];
+        viscous_flux<DIM, R>(sr, g, A.P, flux);
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             const int a = ai[d];
             const int t = line_index<NP>(node, d);
 #pragma unroll
             for (int v = 0; v < NEQ - 1; v++) {
-                double f = 0.0;
+                R f = R(0);
 #pragma unroll
                 for (int c = 0; c < DIM; c++) f += adj[d][c] * flux[v][c];
                 Fc[d][v] = f;
-                if (a == 0) A.Ff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
-                if (a == NP - 1) A.Ff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;
+                if constexpr (MIX) {
+                    if (a == 0) A.Fff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
+                    if (a == NP - 1) A.Fff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;
+                } else {
+                    if (a == 0) A.Ff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
+                    if (a == NP - 1) A.Ff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;
+                }
             }
         }
     }
@@ -582,9 +654,9 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
     }
     __syncthreads();
     if (!active) return;
-    double dvv[NEQ - 1];
+    R dvv[NEQ - 1];
 #pragma unroll
-    for (int v = 0; v < NEQ - 1; v++) dvv[v] = 0.0;
+    for (int v = 0; v < NEQ - 1; v++) dvv[v] = R(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         const int st = dir_stride<NP>(d);
@@ -593,10 +665,16 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
 #pragma unroll
         for (int v = 0; v < NEQ - 1; v++)
 #pragma unroll
-            for (int b = 0; b < NP; b++) dvv[v] += A.T.Dhat[a][b] * sU[(d * (NEQ - 1) + v) * EPB * NN + lb + b * st];
+            for (int b = 0; b < NP; b++)
+                dvv[v] += (R)A.T.Dhat[a][b] * sU[(d * (NEQ - 1) + v) * EPB * NN + lb + b * st];
     }
 #pragma unroll
-    for (int v = 0; v < NEQ - 1; v++) A.dv[(size_t)v * vs + (size_t)e * NN + node] = dvv[v] * invJ;
+    for (int v = 0; v < NEQ - 1; v++) {
+        if constexpr (MIX)
+            A.dvf[(size_t)v * vs + (size_t)e * NN + node] = dvv[v] * invJ;
+        else
+            A.dv[(size_t)v * vs + (size_t)e * NN + node] = dvv[v] * invJ;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -30,6 +30,8 @@ __device__ __forceinline__ double rcp(double x) {
     return fma(r, e, r);
 }
 
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Primitive state of one node as the two-point flux consumes it.
 template <int DIM>
 struct NodePrim {
@@ -187,6 +189,117 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision (PRANDTL_MIXED).  The logarithmic means, the reciprocals of the sums
+// they share and p-hat stay FP64; the rest of the two-point and surface fluxes is FP32.
+// The momentum fluxes carry p-hat - p_ref with p_ref a constant of the element whose
+// equation is being assembled: a constant pressure contributes nothing to the assembled
+// RHS on an affine element (the volume sum of D-hat-2 rows and the surface terms cancel,
+// i.e. free-stream preservation), and subtracting it in FP64 keeps the FP32 momentum
+// sums from cancelling three digits of a ~1/(gamma Ma^2) pressure.
+// ---------------------------------------------------------------------------
+template <int DIM>
+struct NodePrimMx {
+    double rho, beta;
+    float vel[DIM];
+};
+
+template <int DIM>
+__device__ __forceinline__ void volume_flux_mx(const NodePrimMx<DIM> &a, const NodePrimMx<DIM> &b, const float *m,
+                                               float gammaM1Inverse, double p_ref, float *F) {
+    const double rsum = a.rho + b.rho;
+    const double r_rho = rcp(rsum);
+    const double r_b = rcp(a.beta + b.beta);
+    const double il_rho = inv_log_mean(a.rho, b.rho, r_rho);
+    const double il_b = inv_log_mean(a.beta, b.beta, r_b);
+    const double p_hat = 0.5 * rsum * r_b;
+    const float ilr = (float)il_rho, ilb = (float)il_b;
+    const float rho_ln = rcp(ilr);
+    float mean[DIM];
+    float h_hat = 0.f, Vn = 0.f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        mean[d] = 0.5f * (a.vel[d] + b.vel[d]);
+        h_hat += -0.25f * (a.vel[d] * a.vel[d] + b.vel[d] * b.vel[d]) + mean[d] * mean[d];
+        Vn += mean[d] * m[d];
+    }
+    h_hat += 0.5f * ilb * gammaM1Inverse + (float)p_hat * ilr;
+    const float pd = (float)(p_hat - p_ref);
+    const float mflux = rho_ln * Vn;
+    F[0] = mflux;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) F[1 + d] = mflux * mean[d] + pd * m[d];
+    F[DIM + 1] = mflux * h_hat;
+}
+
+template <int DIM, int SURF>
+__device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<DIM> &q1, const double *s2,
+                                             const SideState<DIM> &q2, const float *n, float nmag, const PhysConst &P,
+                                             bool bug3, double p_ref, float *flux) {
+    if constexpr (SURF == SURF_CHANDRASHEKAR) {
+        const double rsum = q1.rho + q2.rho;
+        const double r_rho = rcp(rsum);
+        const double r_b = rcp(q1.beta + q2.beta);
+        const double il_rho = inv_log_mean(q1.rho, q2.rho, r_rho);
+        const double il_b = inv_log_mean(q1.beta, q2.beta, r_b);
+        const double p_hat = 0.5 * rsum * r_b;
+        const float ilr = (float)il_rho, ilb = (float)il_b;
+        const float rho_ln = rcp(ilr);
+        const float rho_mean = (float)(0.5 * rsum);
+        const float drho = (float)(q2.rho - q1.rho);
+        const float g1 = (float)P.gammaM1Inverse;
+        float vm[DIM];
+        float Vn = 0.f, h_hat = 0.f, diss = 0.f;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const float v1 = (float)q1.vel[d], v2 = (float)q2.vel[d];
+            vm[d] = 0.5f * (v1 + v2);
+            const float dv = (float)(q2.vel[d] - q1.vel[d]);
+            Vn += vm[d] * n[d];
+            h_hat += -0.25f * (v1 * v1 + v2 * v2) + vm[d] * vm[d];
+            if (DIM == 3 && d == 2 && bug3)
+                diss += 0.5f * drho * v1 * v2 + rho_mean * dv + vm[d];
+            else
+                diss += 0.5f * drho * v1 * v2 + rho_mean * dv * vm[d];
+        }
+        const float lam = (float)fmax(q1.lam, q2.lam);
+        h_hat += 0.5f * ilb * g1 + (float)p_hat * ilr;
+        diss += 0.5f * drho * g1 * ilb + rho_mean * g1 * (float)(q2.p * q2.irho - q1.p * q1.irho);
+        const float hl = 0.5f * lam * nmag;
+        const float mflux = rho_ln * Vn;
+        const float pd = (float)(p_hat - p_ref);
+        flux[0] = mflux - hl * drho;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) flux[1 + d] = mflux * vm[d] + pd * n[d] - hl * (float)(s2[1 + d] - s1[1 + d]);
+        flux[DIM + 1] = mflux * h_hat - hl * diss;
+    } else if constexpr (SURF == SURF_EC) {
+        NodePrimMx<DIM> a, b;
+        a.rho = q1.rho;
+        a.beta = q1.beta;
+        b.rho = q2.rho;
+        b.beta = q2.beta;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            a.vel[d] = (float)q1.vel[d];
+            b.vel[d] = (float)q2.vel[d];
+        }
+        volume_flux_mx<DIM>(a, b, n, (float)P.gammaM1Inverse, p_ref, flux);
+    } else {
+        double nd[DIM], f1[DIM + 2], f2[DIM + 2];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) nd[d] = n[d];
+        euler_flux_n<DIM>(s1, q1, nd, f1);
+        euler_flux_n<DIM>(s2, q2, nd, f2);
+        const double hl = 0.5 * fmax(q1.lam, q2.lam) * nmag;
+#pragma unroll
+        for (int v = 0; v < DIM + 2; v++) {
+            double fv = 0.5 * (f1[v] + f2[v]) - hl * (s2[v] - s1[v]);
+            if (v >= 1 && v <= DIM) fv -= p_ref * nd[v - 1];
+            flux[v] = (float)fv;
+        }
+    }
+}
+
 // Conserv2Entropy -- ref: BasicOperations.cpp:81-104.  s = ln(p rho^-gamma) is evaluated as
 // ln p - gamma ln rho (same value to rounding, two logs instead of pow + log).
 template <int DIM>
@@ -209,61 +322,63 @@ __device__ __forceinline__ void conserv_to_entropy(const double *s, const PhysCo
 }
 
 // EntropyGrad2PrimGrad for one direction, in place -- ref: BasicOperations.cpp:126-170
-template <int DIM>
-__device__ __forceinline__ void entgrad_to_primgrad(const double *s, const PhysConst &P, double *g) {
-    const double rho = s[0];
-    double vel[DIM], v_sq = 0.0;
+template <int DIM, typename T = double>
+__device__ __forceinline__ void entgrad_to_primgrad(const T *s, const PhysConst &P, T *g) {
+    const T rho = s[0];
+    const T irho = rcp(rho);
+    T vel[DIM], v_sq = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        vel[d] = s[1 + d] / rho;
+        vel[d] = s[1 + d] * irho;
         v_sq += vel[d] * vel[d];
     }
-    v_sq = 0.5 * v_sq;
-    const double p = P.gammaM1 * (s[DIM + 1] - rho * v_sq);
-    const double KE = v_sq * rho;
-    const double ge = g[DIM + 1];
-    const double por = p / rho;
-    double mdot = 0.0;
+    v_sq = T(0.5) * v_sq;
+    const T p = T(P.gammaM1) * (s[DIM + 1] - rho * v_sq);
+    const T KE = v_sq * rho;
+    const T ge = g[DIM + 1];
+    const T por = p * irho;
+    T mdot = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         g[1 + d] = por * (g[1 + d] + vel[d] * ge);
         mdot += s[1 + d] * g[1 + d];
     }
-    g[0] = rho * g[0] - ge * (KE - p * P.gammaM1Inverse) + rho / p * mdot;
+    g[0] = rho * g[0] - ge * (KE - p * T(P.gammaM1Inverse)) + rcp(por) * mdot;
     g[DIM + 1] = por * (g[0] + p * ge);
 }
 
 // NavierStokesFlux::ComputeViscousFlux -- ref: src/Fluxes/NavierStokesFlux.cpp:12-58 (3D),
 // :60-90 (2D); Sutherland NavierStokesFlux.hpp:32-36.  g[c][v] are the primitive
 // gradients (rho, vel, p) along x_c.  flux[v-1][c] for v = 1..DIM+1 (row 0 is zero).
-template <int DIM>
-__device__ __forceinline__ void viscous_flux(const double *s, const double (*g)[DIM + 2], const PhysConst &P,
-                                             double (*flux)[DIM]) {
-    const double rho = s[0];
-    double vel[DIM], vsq = 0.0;
+template <int DIM, typename T = double>
+__device__ __forceinline__ void viscous_flux(const T *s, const T (*g)[DIM + 2], const PhysConst &P, T (*flux)[DIM]) {
+    const T rho = s[0];
+    const T irho = rcp(rho);
+    T vel[DIM], vsq = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        vel[d] = s[1 + d] / rho;
+        vel[d] = s[1 + d] * irho;
         vsq += vel[d] * vel[d];
     }
-    const double p = P.gammaM1 * (s[DIM + 1] - 0.5 * rho * vsq);
-    double mu = P.mu;
+    const T p = T(P.gammaM1) * (s[DIM + 1] - T(0.5) * rho * vsq);
+    T mu = T(P.mu);
     if (P.sutherland) {
-        const double T = p / (rho * P.R_gas);
-        mu = P.mu0 * (P.T0 + P.Ts) / (T + P.Ts) * (T / P.T0) * sqrt(T / P.T0);
+        const double Tk = double(p) / (double(rho) * P.R_gas);
+        mu = T(P.mu0 * (P.T0 + P.Ts) / (Tk + P.Ts) * (Tk / P.T0) * sqrt(Tk / P.T0));
     }
-    const double lambda = mu * P.gamma * P.PrInverse;
-    double div = g[0][1];
+    const T lambda = mu * T(P.gamma * P.PrInverse);
+    const T por = p * irho;
+    T div = g[0][1];
 #pragma unroll
     for (int d = 1; d < DIM; d++) div += g[d][1 + d];
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
-        const double cv_dT = P.gammaM1Inverse / rho * (g[c][DIM + 1] - p / rho * g[c][0]);
-        double e = 0.0;
+        const T cv_dT = T(P.gammaM1Inverse) * irho * (g[c][DIM + 1] - por * g[c][0]);
+        T e = T(0);
 #pragma unroll
         for (int r = 0; r < DIM; r++) {
             // tau_rc = mu (d vel_r / dx_c + d vel_c / dx_r) - mu mu_bulk div delta_rc
-            double t = (r == c) ? mu * (2.0 * g[c][1 + r] - P.mu_bulk * div) : mu * (g[c][1 + r] + g[r][1 + c]);
+            T t = (r == c) ? mu * (T(2) * g[c][1 + r] - T(P.mu_bulk) * div) : mu * (g[c][1 + r] + g[r][1 + c]);
             flux[r][c] = t;
             e += vel[r] * t;
         }

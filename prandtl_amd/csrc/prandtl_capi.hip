@@ -155,7 +155,7 @@ struct StageSpec {
     int phases = 3;      // bit 0: k_lift (NS), bit 1: k_rhs
 };
 
-template <int DIM, int NP, bool VISC, int SURF>
+template <int DIM, int NP, bool VISC, int SURF, bool MIX>
 int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     if constexpr (VISC) {
       if (S.phases & 1) {
@@ -169,9 +169,11 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
         la.u = S.u;
         la.dv = op->d_dv;
         la.Ff = op->d_Ff;
+        la.dvf = reinterpret_cast<float *>(op->d_dv);
+        la.Fff = reinterpret_cast<float *>(op->d_Ff);
         la.grad = S.grad;
         const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
-        hipLaunchKernelGGL((k_lift<DIM, NP>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+        hipLaunchKernelGGL((k_lift<DIM, NP, MIX>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
         HIP_TRY(hipGetLastError());
       }
     }
@@ -188,6 +190,8 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.u = S.u;
     ra.dv = op->d_dv;
     ra.Ff = op->d_Ff;
+    ra.dvf = reinterpret_cast<const float *>(op->d_dv);
+    ra.Fff = reinterpret_cast<const float *>(op->d_Ff);
     ra.out = S.out;
     ra.k = S.k;
     ra.x = S.x;
@@ -199,10 +203,10 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     // Navier-Stokes: one thread per node (the node's state stays in registers through the
     // fused epilogue, no LDS accumulator: 4 waves/SIMD); Euler: one thread per GLL line
     // (each symmetric pair flux evaluated once).  Chosen by measurement at TGV 32^3, p=3.
-    if constexpr (VISC) {
+    if constexpr (VISC || MIX) {
         using LC = NodeCfg<DIM, NP>;
         const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
-        hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+        hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, MIX>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
     } else {
         using LC = LineCfg<DIM, NP>;
         const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
@@ -212,19 +216,23 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     return PRANDTL_OK;
 }
 
-template <int DIM, int NP, bool VISC>
+template <int DIM, int NP, bool VISC, bool MIX>
 int dispatch_surf(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     switch (op->opts.surface_flux) {
-        case PRANDTL_SURF_CHANDRASHEKAR: return launch_stage<DIM, NP, VISC, SURF_CHANDRASHEKAR>(op, S, st);
-        case PRANDTL_SURF_EC: return launch_stage<DIM, NP, VISC, SURF_EC>(op, S, st);
-        case PRANDTL_SURF_RUSANOV: return launch_stage<DIM, NP, VISC, SURF_RUSANOV>(op, S, st);
+        case PRANDTL_SURF_CHANDRASHEKAR: return launch_stage<DIM, NP, VISC, SURF_CHANDRASHEKAR, MIX>(op, S, st);
+        case PRANDTL_SURF_EC: return launch_stage<DIM, NP, VISC, SURF_EC, MIX>(op, S, st);
+        case PRANDTL_SURF_RUSANOV: return launch_stage<DIM, NP, VISC, SURF_RUSANOV, MIX>(op, S, st);
     }
     return fail(PRANDTL_ERR_ARG, "unknown surface flux");
 }
 
 template <int DIM, int NP>
 int dispatch_visc(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
-    return op->phys.viscous ? dispatch_surf<DIM, NP, true>(op, S, st) : dispatch_surf<DIM, NP, false>(op, S, st);
+    if (op->opts.precision == PRANDTL_MIXED)
+        return op->phys.viscous ? dispatch_surf<DIM, NP, true, true>(op, S, st)
+                                : dispatch_surf<DIM, NP, false, true>(op, S, st);
+    return op->phys.viscous ? dispatch_surf<DIM, NP, true, false>(op, S, st)
+                            : dispatch_surf<DIM, NP, false, false>(op, S, st);
 }
 
 template <int DIM>
@@ -372,7 +380,7 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     prandtl_opts o{};
     if (opts) o = *opts;
     if (o.surface_flux < 0 || o.surface_flux > 2) return fail(PRANDTL_ERR_ARG, "unknown surface flux");
-    if (o.precision != PRANDTL_FP64) return fail(PRANDTL_ERR_UNSUPPORTED, "precision mode not built");
+    if (o.precision != PRANDTL_FP64 && o.precision != PRANDTL_MIXED) return fail(PRANDTL_ERR_ARG, "unknown precision");
     if (phys->viscous && (o.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
         return fail(PRANDTL_ERR_UNSUPPORTED, "bug_compat A1/A2 are reproduced by the CPU oracle only");
     if (phys->viscous && (!(phys->Pr > 0.0) || phys->mu < 0.0)) return fail(PRANDTL_ERR_ARG, "bad Pr or mu");

@@ -66,7 +66,7 @@ def test_rejects_nonconforming_faces_and_bad_jacobian():
     assert rc == -4 and "det J" in msg
 
 
-@pytest.mark.parametrize("kw,code", [({"gamma": 1.0}, -1), ({"surface_flux": 9}, -1), ({"precision": 7}, -2),
+@pytest.mark.parametrize("kw,code", [({"gamma": 1.0}, -1), ({"surface_flux": 9}, -1), ({"precision": 7}, -1),
                                      ({"viscous": 1, "bug_compat": 1}, -2), ({"viscous": 1, "Pr": 0.0}, -1)])
 def test_rejects_bad_options(kw, code):
     m = M.periodic_lattice(2, 3, (2, 2), lengths=(1, 1))

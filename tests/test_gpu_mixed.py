@@ -1,0 +1,107 @@
+"""Mixed precision (PRANDTL_MIXED): FP32 flux differencing and derivative contractions,
+FP64 logarithmic means, RK accumulation and CFL reduction.  Its results are held against
+the FP64 GPU path, whose parity with the oracle is established in test_gpu_parity.py:
+
+* RHS on the Taylor-Green state (16^3, p=3, Navier-Stokes): relative max-norm difference
+  <= 2e-5 per conserved variable scale;
+* discrete conservation (sum J w du = 0) to FP32 rounding: <= 1e-6 of sum |J w du|;
+* free-stream preservation: |du| <= 1e-5 x the flux scale p |Ja| / J;
+* 100 LSRK4(5) steps at 16^3: kinetic-energy dissipation rate within 1e-5 relative, density
+  L2 difference and entropy drift reported.
+"""
+import numpy as np
+import pytest
+
+from prandtl_amd import capi, cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _ke_rate(mesh, u, du, W):
+    vel = [u[1 + k] / u[0] for k in range(3)]
+    r = sum(vel[k] * du[1 + k] for k in range(3)) - 0.5 * sum(v * v for v in vel) * du[0]
+    return float((W * r).sum() / W.sum())
+
+
+def test_mixed_rhs_vs_fp64(device):
+    import torch
+
+    _say("mixed vs FP64 RHS, TGV 16^3 p=3 NS")
+    m = cases.tgv_mesh(n=16, order=3)
+    u = cases.tgv_state(m)
+    ph = cases.tgv_physics()
+    ud = torch.from_numpy(u).to(device)
+    d64 = torch.empty_like(ud)
+    d32 = torch.empty_like(ud)
+    op64 = capi.Operator(m, **ph)
+    opmx = capi.Operator(m, precision=capi.MIXED, **ph)
+    op64.mult(ud, d64)
+    opmx.mult(ud, d32)
+    torch.cuda.synchronize()
+    a, b = d64.cpu().numpy(), d32.cpu().numpy()
+    for v in range(5):
+        rel = np.abs(a[v] - b[v]).max() / np.abs(a[v]).max()
+        print(f"[mixed] var {v} rel. max-norm diff vs FP64 = {rel:.3e}", flush=True)
+        assert rel <= 2e-5, (v, rel)
+    W = m.quadrature_weights()
+    for v in range(5):
+        s, t = (W * b[v]).sum(), (W * np.abs(b[v])).sum()
+        assert abs(s) <= 1e-6 * t, (v, s, t)
+    r64 = _ke_rate(m, u, a, W)
+    rmx = _ke_rate(m, u, b, W)
+    print(f"[mixed] dK/dt FP64 {r64:.10e} mixed {rmx:.10e} rel {abs(rmx - r64) / abs(r64):.3e}", flush=True)
+    assert abs(rmx - r64) <= 1e-5 * abs(r64)
+    op64.close()
+    opmx.close()
+
+
+def test_mixed_free_stream(device):
+    import torch
+
+    _say("mixed free stream, TGV mesh 8^3")
+    m = cases.tgv_mesh(n=8, order=3)
+    u0 = np.zeros(m.state_shape)
+    u0[:] = cases.prim_to_conserv(np.ones(1), [np.full(1, 0.3), np.full(1, -0.1), np.full(1, 0.2)],
+                                  np.full(1, 71.4))[:, :, None]
+    op = capi.Operator(m, precision=capi.MIXED, **cases.tgv_physics())
+    ud = torch.from_numpy(u0).to(device)
+    du = torch.empty_like(ud)
+    op.mult(ud, du)
+    torch.cuda.synchronize()
+    h = m.A[0, 0]
+    scale = 71.4 * h * h / h**3
+    assert du.abs().max().item() <= 1e-5 * scale
+    op.close()
+
+
+def test_mixed_time_integration(device):
+    import torch
+
+    _say("mixed vs FP64, 100 LSRK4(5) steps, TGV 16^3 p=3 NS")
+    m = cases.tgv_mesh(n=16, order=3)
+    u = cases.tgv_state(m)
+    ph = cases.tgv_physics()
+    W = m.quadrature_weights()
+    dt = cases.cfl_dt(m, 11.0, 0.5)
+    res = {}
+    for prec in (capi.FP64, capi.MIXED):
+        op = capi.Operator(m, precision=prec, **ph)
+        ud = torch.from_numpy(u).to(device)
+        op.rk_run(capi.LSRK45, ud, 0.0, dt, 100)
+        du = torch.empty_like(ud)
+        op.mult(ud, du)
+        torch.cuda.synchronize()
+        un, dn = ud.cpu().numpy(), du.cpu().numpy()
+        res[prec] = (un, _ke_rate(m, un, dn, W))
+        op.close()
+    (u64, e64), (umx, emx) = res[capi.FP64], res[capi.MIXED]
+    rho_l2 = np.sqrt((W * (umx[0] - u64[0]) ** 2).sum() / (W * u64[0] ** 2).sum())
+    rel = abs(emx - e64) / abs(e64)
+    print(f"[mixed] after 100 steps: density L2 rel diff {rho_l2:.3e}, KE dissipation rate rel err {rel:.3e}",
+          flush=True)
+    assert rel <= 1e-5
+    assert rho_l2 <= 1e-7
