@@ -199,6 +199,23 @@ def main():
     value = nodes * rhs_evals / (ms / 1e3)
     log(f"{args.steps} LSRK4(5) steps: {ms:.2f} ms, {value / 1e9:.3f} G DOF-RHS/s")
 
+    # ---- the same steps in FP32-mixed precision ---------------------------------
+    opm = capi.Operator(mesh, precision=capi.MIXED, **phys)
+    um = torch.from_numpy(u0).to(dev)
+    opm.rk_run(capi.LSRK45, um, 0.0, dt, args.warmup)
+    torch.cuda.synchronize()
+    e2 = torch.cuda.Event(enable_timing=True)
+    e3 = torch.cuda.Event(enable_timing=True)
+    e2.record(stream)
+    opm.rk_run(capi.LSRK45, um, 0.0, dt, args.steps)
+    e3.record(stream)
+    torch.cuda.synchronize()
+    mixed_ms = e2.elapsed_time(e3)
+    mixed_value = nodes * rhs_evals / (mixed_ms / 1e3)
+    log(f"mixed precision: {mixed_ms:.2f} ms, {mixed_value / 1e9:.3f} G DOF-RHS/s")
+    opm.close()
+    del um
+
     # ---- bare Mult (DGSEMOperator::Mult) ---------------------------------------
     du = torch.empty_like(ud)
     for _ in range(5):
@@ -265,6 +282,8 @@ def main():
             "rhs_evals_per_step": 5,
             "parallelism": "single GPU",
         },
+        "mixed_value": mixed_value,
+        "mixed_speedup": mixed_value / value,
         "mult_dof_rhs_per_s": nodes / (mult_med * 1e-3),
         "mult_ms": mult_med,
         "t_final": t,

@@ -2,8 +2,9 @@
 FP64 logarithmic means, RK accumulation and CFL reduction.  Its results are held against
 the FP64 GPU path, whose parity with the oracle is established in test_gpu_parity.py:
 
-* RHS on the Taylor-Green state (16^3, p=3, Navier-Stokes): relative max-norm difference
-  <= 2e-5 per conserved variable scale;
+* RHS on the Taylor-Green state (16^3, p=3, Navier-Stokes): max-norm difference <= 2e-5 of
+  the RHS max-norm over all variables (the density RHS of the divergence-free initial state
+  is itself only discretisation error, so a per-variable relative bar is meaningless there);
 * discrete conservation (sum J w du = 0) to FP32 rounding: <= 1e-6 of sum |J w du|;
 * free-stream preservation: |du| <= 1e-5 x the flux scale p |Ja| / J;
 * 100 LSRK4(5) steps at 16^3: kinetic-energy dissipation rate within 1e-5 relative, density
@@ -43,9 +44,10 @@ def test_mixed_rhs_vs_fp64(device):
     opmx.mult(ud, d32)
     torch.cuda.synchronize()
     a, b = d64.cpu().numpy(), d32.cpu().numpy()
+    scale = np.abs(a).max()
     for v in range(5):
-        rel = np.abs(a[v] - b[v]).max() / np.abs(a[v]).max()
-        print(f"[mixed] var {v} rel. max-norm diff vs FP64 = {rel:.3e}", flush=True)
+        rel = np.abs(a[v] - b[v]).max() / scale
+        print(f"[mixed] var {v} max-norm diff vs FP64 / ||du||_inf = {rel:.3e}", flush=True)
         assert rel <= 2e-5, (v, rel)
     W = m.quadrature_weights()
     for v in range(5):
