@@ -141,6 +141,83 @@ def compare_mixed(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def adjoint_case(dev):
+    """Gradients of J = integrated kinetic energy after 40 LSRK4(5) steps (200 stages) of the
+    3D Euler TGV (16^3 hexes, p=4, Ma=0.1, + 64 seeded Fourier velocity modes of amplitude 1e-3)
+    with respect to the Mach number and the 64 mode amplitudes, by the discrete adjoint with a
+    checkpoint every 20 steps; checked against central finite differences (second order at
+    eps = 1e-6 relative, and sixth order at a larger step); adjoint/forward cost ratio."""
+    import torch
+
+    from prandtl_amd import capi, cases
+    from prandtl_amd import torch_adjoint as TA
+
+    mesh = cases.tgv_mesh(n=16, order=4)
+    modes = cases.fourier_modes(count=64, kmax=4, seed=4242)
+    W = torch.from_numpy(mesh.quadrature_weights()).to(dev)
+    op = capi.Operator(mesh)
+    nsteps, ckpt = 40, 20
+    dt = cases.cfl_dt(mesh, 11.0, 0.5)
+
+    def J_of(mach, amps):
+        u0 = TA.tgv_modes_state(mesh, mach, amps, modes)
+        return TA.kinetic_energy(TA.RKSolve.apply(u0, op, dt, nsteps, ckpt), W)
+
+    mach = torch.tensor(0.1, dtype=torch.float64, device=dev, requires_grad=True)
+    amps = torch.full((64,), 1e-3, dtype=torch.float64, device=dev, requires_grad=True)
+    # timing: plain forward, taped forward, adjoint sweep
+    with torch.no_grad():
+        u0 = TA.tgv_modes_state(mesh, mach, amps, modes)
+    uu = u0.clone()
+    op.rk_run(capi.LSRK45, uu, 0.0, dt, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    uu = u0.clone()
+    op.rk_run(capi.LSRK45, uu, 0.0, dt, nsteps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    J = J_of(mach, amps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gM, gA = torch.autograd.grad(J, [mach, amps])
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t1
+    # finite differences: the second-order central difference at eps = 1e-6 (relative to the
+    # parameter), and the sixth-order Richardson combination at a step where its truncation and
+    # rounding errors are both far below 1e-6 relative (the reference value of the check)
+    checks = []
+    with torch.no_grad():
+        def fd_entry(name, g, f, x, h2, h6):
+            fd2 = (f(x + h2) - f(x - h2)) / (2 * h2)
+            d4, d42, fd6 = TA.central_fd(f, x, h6)
+            checks.append({"param": name, "adjoint": g, "fd6": fd6, "fd4_h": d4, "fd4_2h": d42,
+                           "fd2_eps1e-6": fd2, "rel_err": abs(g - fd6) / max(abs(fd6), 1e-300),
+                           "rel_err_fd2": abs(g - fd2) / max(abs(fd2), 1e-300)})
+
+        fd_entry("Ma", gM.item(), lambda x: J_of(x, amps).item(), mach, 1e-6 * 0.1, 1e-2 * 0.1)
+        for i in (0, 17, 42, 63):
+            e = torch.zeros_like(amps)
+            e[i] = 1.0
+            fd_entry(f"a_{i}", gA[i].item(), lambda x: J_of(mach, amps + (x - amps[i]) * e).item(), amps[i],
+                     1e-6 * 1e-3, 1e-4)
+    for c in checks:
+        log(f"adjoint {c['param']}: {c['adjoint']:.12e} FD6 {c['fd6']:.12e} rel {c['rel_err']:.2e} "
+            f"(FD2 eps=1e-6 {c['fd2_eps1e-6']:.12e}, rel {c['rel_err_fd2']:.2e})")
+    out = {
+        "metric": "adjoint/forward cost, 3D Euler TGV 16^3 p=4 (200 LSRK4(5) stages, checkpoint every 20 steps)",
+        "config": {"workload": f"TGV Euler 16^3 hexes p=4 ({mesh.num_nodes} nodes), Ma=0.1, 64 Fourier modes "
+                               f"(|k|<=4, amplitude 1e-3, seed 4242), J = integrated kinetic energy, dt={dt:.4e}"},
+        "forward_s": t_fwd,
+        "adjoint_s": t_adj,
+        "adjoint_over_forward": t_adj / t_fwd,
+        "forward_dof_rhs_per_s": mesh.num_nodes * 5 * nsteps / t_fwd,
+        "gradient_checks": checks,
+        "max_rel_err": max(c["rel_err"] for c in checks),
+    }
+    op.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +232,9 @@ def main():
                     help="run the FP32-mixed TGV comparison (FP64 and mixed side by side to --t-final) "
                          "instead of the default benchmark")
     ap.add_argument("--t-final", type=float, default=2.0)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="run the discrete-adjoint case (TGV Euler 16^3 p=4 + 64 Fourier modes, 200 RK stages, "
+                         "checkpoints every 20 steps) instead of the default benchmark")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -172,6 +252,9 @@ def main():
     stream = torch.cuda.current_stream()
     if args.compare_mixed:
         compare_mixed(args, dev)
+        return
+    if args.adjoint:
+        adjoint_case(dev)
         return
 
     mesh = cases.tgv_mesh(n=args.n, order=args.order)

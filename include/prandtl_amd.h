@@ -124,6 +124,23 @@ int prandtl_rk_step(prandtl_op op, int scheme, double *u, double *t, double dt, 
 int prandtl_rk_run(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
                    prandtl_stream stream);
 
+/* Discrete adjoint (no counterpart in the reference, which has no derivatives).
+ * out = (dL/du)^T lam at state u: the vector-Jacobian product of the semi-discrete operator
+ * (volume flux differencing and surface flux; Euler operators, FP64, Chandrashekar or EC
+ * surface flux).  out must not alias u or lam. */
+int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *out, prandtl_stream stream);
+
+/* prandtl_rk_run (LSRK4(5)) that also keeps the state at the start of every ckpt_every-step
+ * segment in the handle, for prandtl_rk_adjoint. */
+int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
+                        prandtl_stream stream);
+
+/* Reverse sweep through the last taped run: ubar holds dJ/du at the final state on entry
+ * and dJ/du at the initial state on return.  Each segment is recomputed forward from its
+ * checkpoint keeping its stage inputs (5 x ckpt_every states of scratch), then the LSRK4(5)
+ * stage adjoints run backwards, fused into the operator-adjoint kernel.  Synchronises. */
+int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream);
+
 /* Per-kernel timing of one RHS evaluation, with HIP events on `stream`: ms[0] = average
  * duration of the lifting/viscous-flux pass (0 for Euler), ms[1] = the flux-differencing
  * pass, each launched `iters` times back to back on input u.  epilogue 0 times the Mult

@@ -27,7 +27,7 @@ SSPRK3, LSRK45 = 0, 1
 EXPORTED = [
     "prandtl_op_create", "prandtl_op_destroy", "prandtl_op_size", "prandtl_op_mult", "prandtl_op_mult_grad",
     "prandtl_op_get_max_char_speed", "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run",
-    "prandtl_op_profile", "prandtl_last_error",
+    "prandtl_op_profile", "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_last_error",
 ]
 
 
@@ -76,10 +76,14 @@ def load():
         L.prandtl_rk_step.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, vp]
         L.prandtl_rk_run.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.prandtl_op_profile.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, dp, vp]
+        L.prandtl_op_vjp.argtypes = [vp, vp, vp, vp, vp]
+        L.prandtl_rk_run_tape.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
+        L.prandtl_rk_adjoint.argtypes = [vp, vp, vp]
         L.prandtl_last_error.argtypes = []
         L.prandtl_last_error.restype = ctypes.c_char_p
         for n in ("prandtl_op_mult", "prandtl_op_mult_grad", "prandtl_op_get_max_char_speed",
-                  "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile"):
+                  "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile",
+                  "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -171,3 +175,17 @@ class Operator:
         _check(load().prandtl_rk_run(self._h, int(scheme), _ptr(u), ctypes.byref(tt), float(dt), int(nsteps),
                                      int(nan_check), _stream(stream)))
         return tt.value
+
+    def vjp(self, u, lam, out, stream=None):
+        """out = (dL/du)^T lam at u."""
+        _check(load().prandtl_op_vjp(self._h, _ptr(u), _ptr(lam), _ptr(out), _stream(stream)))
+
+    def rk_run_tape(self, u, t, dt, nsteps, ckpt_every, stream=None) -> float:
+        tt = ctypes.c_double(t)
+        _check(load().prandtl_rk_run_tape(self._h, LSRK45, _ptr(u), ctypes.byref(tt), float(dt), int(nsteps),
+                                          int(ckpt_every), _stream(stream)))
+        return tt.value
+
+    def rk_adjoint(self, ubar, stream=None):
+        """ubar: dJ/du_final on entry, dJ/du_initial on return (for the last taped run)."""
+        _check(load().prandtl_rk_adjoint(self._h, _ptr(ubar), _stream(stream)))

@@ -1,0 +1,278 @@
+// dg_adjoint.hpp -- reverse-mode derivatives (vector-Jacobian products) of the pointwise
+// Euler fluxes, for the discrete adjoint of the DGSEM operator.
+//
+// Each *_vjp function recomputes the forward quantities of its flux from the conserved
+// states and accumulates  s_b += (dF/ds)^T F_b  for the given cotangent F_b.  The volume
+// kernel splits the same steps: node_q once per node, ec_core_back once per pair (adjoints of
+// density, velocity and beta), prim_back once per node.  The forward
+// formulas are those of dg_physics.hpp (the reference's ChandrashekarFlux.cpp:11-161 and
+// ComputeLogMean, BasicOperations.cpp:6-12); the derivatives are of the same mathematics,
+// with the logarithmic mean differentiated through the same two branches (series for
+// u = ((y-x)/(y+x))^2 < 1e-4, closed form otherwise).  The wave speed max(|V|+c) of the
+// surface dissipation is differentiated through the branch the forward takes (both halves at
+// a tie).
+//
+// The functions are host+device so the same code is checked against finite differences on
+// the CPU (tests/native/adjoint_vjp_host.cpp).
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#else  // host-only build of the same functions (the finite-difference check in tests/native)
+#define __host__
+#define __device__
+#endif
+
+#include <cmath>
+
+namespace prandtl {
+namespace adj {
+
+__host__ __device__ inline double sq(double x) { return x * x; }
+
+// reciprocal: on the device the v_rcp_f64 seed with two Newton steps (within an ulp or two of
+// the division); on the host the division
+__host__ __device__ inline double inv(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
+// per-node quantities the volume adjoint reads for every pair: density, velocity, beta =
+// rho/(2p), their reciprocals and logarithms (computed once per node, not once per pair)
+template <int DIM>
+struct NodeQ {
+    double rho, v[DIM], beta, irho, ibeta, lrho, lbeta;
+};
+
+template <int DIM>
+__host__ __device__ inline NodeQ<DIM> node_q(const double *s, double gm1) {
+    NodeQ<DIM> q;
+    q.rho = s[0];
+    q.irho = inv(q.rho);
+    double vsq = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        q.v[d] = s[1 + d] * q.irho;
+        vsq += q.v[d] * q.v[d];
+    }
+    const double p = gm1 * (s[DIM + 1] - 0.5 * q.rho * vsq);
+    q.beta = 0.5 * q.rho * inv(p);
+    q.ibeta = inv(q.beta);
+    q.lrho = log(q.rho);
+    q.lbeta = log(q.beta);
+    return q;
+}
+
+// logarithmic mean of (x, y) and its partials, given 1/x, 1/y, ln x, ln y; the two branches
+// of the forward (series in u = ((y-x)/(y+x))^2 below 1e-4, closed form otherwise)
+__host__ __device__ inline void log_mean_dq(double x, double y, double ix, double iy, double lx, double ly, double &L,
+                                            double &Lx, double &Ly) {
+    const double s = x + y, r = inv(s), f = (y - x) * r, u = f * f;
+    if (u < 1e-4) {
+        const double P = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
+        const double dP = 1.0 / 3.0 + u * (2.0 / 5.0 + u * (3.0 / 7.0));
+        const double iP = inv(P);
+        const double A = 0.5 * s;
+        L = A * iP;
+        const double c = A * dP * iP * iP * 4.0 * f * r * r;
+        Lx = 0.5 * iP + c * y;
+        Ly = 0.5 * iP - c * x;
+    } else {
+        const double lg = ly - lx, ilg = inv(lg), d = y - x;
+        L = d * ilg;
+        Lx = (d * ix - lg) * ilg * ilg;
+        Ly = (lg - d * iy) * ilg * ilg;
+    }
+}
+
+// adjoints of a node's (rho, v, beta)
+template <int DIM>
+struct PrimB {
+    double rho, v[DIM], beta;
+};
+
+template <int DIM>
+__host__ __device__ inline void primb_zero(PrimB<DIM> &g) {
+    g.rho = g.beta = 0.0;
+    for (int d = 0; d < DIM; d++) g.v[d] = 0.0;
+}
+
+// core of the entropy-conservative two-point flux
+//   F = (rho_ln Vn, rho_ln Vn vbar + p_hat m, rho_ln Vn h_hat),
+//   Vn = vbar . m, p_hat = (rho_1 + rho_2) / (2 (beta_1 + beta_2)),
+//   h_hat = 1/(2 beta_ln (gamma-1)) - |v_1|^2/4 - |v_2|^2/4 + |vbar|^2 + p_hat / rho_ln,
+// reversed: accumulates the adjoints of (rho, v, beta) of both nodes for the cotangent Fb.
+template <int DIM>
+__host__ __device__ inline void ec_core_back(const NodeQ<DIM> &a, const NodeQ<DIM> &b, const double *m,
+                                             double gm1inv, const double *Fb, PrimB<DIM> &ga, PrimB<DIM> &gb) {
+    double Lr, Lr1, Lr2, Lb, Lb1, Lb2;
+    log_mean_dq(a.rho, b.rho, a.irho, b.irho, a.lrho, b.lrho, Lr, Lr1, Lr2);
+    log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, Lb1, Lb2);
+    const double iLr = inv(Lr), iLb = inv(Lb);
+    double vbar[DIM], Vn = 0.0, h = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        vbar[d] = 0.5 * (a.v[d] + b.v[d]);
+        Vn += vbar[d] * m[d];
+        h += -0.25 * (a.v[d] * a.v[d] + b.v[d] * b.v[d]) + vbar[d] * vbar[d];
+    }
+    const double ibs = inv(a.beta + b.beta);
+    const double p_hat = 0.5 * (a.rho + b.rho) * ibs;
+    h += 0.5 * iLb * gm1inv + p_hat * iLr;
+    const double mflux = Lr * Vn;
+    // reverse
+    double mflux_b = Fb[0] + Fb[DIM + 1] * h;
+    double vbar_b[DIM], p_hat_b = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        mflux_b += Fb[1 + d] * vbar[d];
+        vbar_b[d] = Fb[1 + d] * mflux;
+        p_hat_b += Fb[1 + d] * m[d];
+    }
+    const double h_b = Fb[DIM + 1] * mflux;
+    const double Lb_b = h_b * (-0.5 * gm1inv * iLb * iLb);
+    p_hat_b += h_b * iLr;
+    const double Lr_b = h_b * (-p_hat * iLr * iLr) + mflux_b * Vn;
+    const double Vn_b = mflux_b * Lr;
+    for (int d = 0; d < DIM; d++) {
+        const double vb = h_b * 2.0 * vbar[d] + Vn_b * m[d] + vbar_b[d];
+        ga.v[d] += h_b * (-0.5 * a.v[d]) + 0.5 * vb;
+        gb.v[d] += h_b * (-0.5 * b.v[d]) + 0.5 * vb;
+    }
+    const double rb = p_hat_b * 0.5 * ibs, bb = -p_hat_b * p_hat * ibs;
+    ga.rho += rb + Lr_b * Lr1;
+    gb.rho += rb + Lr_b * Lr2;
+    ga.beta += bb + Lb_b * Lb1;
+    gb.beta += bb + Lb_b * Lb2;
+}
+
+// adjoints of a node's (rho, v, beta) -> adjoint of its conserved state (accumulated):
+// v = m/rho, p = (gamma-1)(E - rho|v|^2/2), beta = rho/(2p)
+template <int DIM>
+__host__ __device__ inline void prim_back(const NodeQ<DIM> &q, double gm1, const PrimB<DIM> &g, double *s_b) {
+    // beta = rho/(2p): d/drho = 1/(2p) = beta/rho, d/dp = -beta/p = -2 beta^2/rho
+    double rho_b = g.rho + g.beta * q.beta * q.irho;
+    const double pg = -g.beta * 2.0 * q.beta * q.beta * q.irho * gm1;   // adjoint of p, times gm1
+    s_b[DIM + 1] += pg;
+    double vsq = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        vsq += q.v[d] * q.v[d];
+        const double vb = g.v[d] - pg * q.rho * q.v[d];
+        s_b[1 + d] += vb * q.irho;
+        rho_b -= vb * q.v[d] * q.irho;
+    }
+    rho_b -= 0.5 * pg * vsq;
+    s_b[0] += rho_b;
+}
+
+// (dF#/ds1)^T Fb and (dF#/ds2)^T Fb of the two-point volume flux with metric m
+template <int DIM>
+__host__ __device__ inline void volume_flux_vjp(const double *s1, const double *s2, const double *m, double gm1,
+                                                const double *Fb, double *s1b, double *s2b) {
+    const NodeQ<DIM> a = node_q<DIM>(s1, gm1), b = node_q<DIM>(s2, gm1);
+    PrimB<DIM> ga, gb;
+    primb_zero<DIM>(ga);
+    primb_zero<DIM>(gb);
+    ec_core_back<DIM>(a, b, m, 1.0 / gm1, Fb, ga, gb);
+    prim_back<DIM>(a, gm1, ga, s1b);
+    prim_back<DIM>(b, gm1, gb, s2b);
+}
+
+// (df*/ds1)^T fb and (df*/ds2)^T fb of the reference's Chandrashekar surface flux
+// (ChandrashekarFlux.cpp:85-161); bug3 follows the :131 typo.  kind 1 = the EC flux.
+template <int DIM>
+__host__ __device__ inline void face_flux_vjp(int kind, const double *s1, const double *s2, const double *n,
+                                              double gamma, bool bug3, const double *fb, double *s1b, double *s2b) {
+    const double gm1 = gamma - 1.0, gm1inv = 1.0 / gm1;
+    const NodeQ<DIM> a = node_q<DIM>(s1, gm1), b = node_q<DIM>(s2, gm1);
+    PrimB<DIM> ga, gb;
+    primb_zero<DIM>(ga);
+    primb_zero<DIM>(gb);
+    ec_core_back<DIM>(a, b, n, gm1inv, fb, ga, gb);
+    if (kind == 0) {
+        // dissipation  -1/2 lam |n| [drho, dm, diss]
+        double nmag = 0.0;
+        for (int d = 0; d < DIM; d++) nmag += n[d] * n[d];
+        nmag = sqrt(nmag);
+        const double hl = 0.5 * nmag;
+        double Lb, Lb1, Lb2;
+        log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, Lb1, Lb2);
+        const double iLb = inv(Lb);
+        const double rho_mean = 0.5 * (a.rho + b.rho), drho = b.rho - a.rho;
+        double diss = 0.0, vsq1 = 0.0, vsq2 = 0.0;
+        for (int d = 0; d < DIM; d++) {
+            const double vm = 0.5 * (a.v[d] + b.v[d]), dv = b.v[d] - a.v[d];
+            if (DIM == 3 && d == 2 && bug3)
+                diss += 0.5 * drho * a.v[d] * b.v[d] + rho_mean * dv + vm;
+            else
+                diss += 0.5 * drho * a.v[d] * b.v[d] + rho_mean * dv * vm;
+            vsq1 += a.v[d] * a.v[d];
+            vsq2 += b.v[d] * b.v[d];
+        }
+        diss += 0.5 * drho * gm1inv * iLb + 0.5 * rho_mean * gm1inv * (b.ibeta - a.ibeta);
+        // p = rho / (2 beta), c = sqrt(gamma p / rho) = sqrt(gamma / (2 beta))
+        const double c1 = sqrt(0.5 * gamma * a.ibeta), c2 = sqrt(0.5 * gamma * b.ibeta);
+        const double vn1 = sqrt(vsq1), vn2 = sqrt(vsq2);
+        const double l1 = vn1 + c1, l2 = vn2 + c2;
+        const double lam = l1 >= l2 ? l1 : l2;
+        // reverse of the dissipation terms
+        double lam_b = -hl * fb[0] * drho;
+        double drho_b = -hl * lam * fb[0];
+        for (int d = 0; d < DIM; d++) {
+            const double dm = s2[1 + d] - s1[1 + d];
+            lam_b += -hl * fb[1 + d] * dm;
+            s2b[1 + d] += -hl * lam * fb[1 + d];
+            s1b[1 + d] += hl * lam * fb[1 + d];
+        }
+        lam_b += -hl * fb[DIM + 1] * diss;
+        const double diss_b = -hl * lam * fb[DIM + 1];
+        double rho_mean_b = 0.0;
+        for (int d = 0; d < DIM; d++) {
+            const double vm = 0.5 * (a.v[d] + b.v[d]), dv = b.v[d] - a.v[d];
+            drho_b += diss_b * 0.5 * a.v[d] * b.v[d];
+            ga.v[d] += diss_b * 0.5 * drho * b.v[d];
+            gb.v[d] += diss_b * 0.5 * drho * a.v[d];
+            double dv_b, vm_b;
+            if (DIM == 3 && d == 2 && bug3) {
+                rho_mean_b += diss_b * dv;
+                dv_b = diss_b * rho_mean;
+                vm_b = diss_b;
+            } else {
+                rho_mean_b += diss_b * dv * vm;
+                dv_b = diss_b * rho_mean * vm;
+                vm_b = diss_b * rho_mean * dv;
+            }
+            gb.v[d] += dv_b + 0.5 * vm_b;
+            ga.v[d] += -dv_b + 0.5 * vm_b;
+        }
+        drho_b += diss_b * 0.5 * gm1inv * iLb;
+        const double Lb_b = diss_b * (-0.5 * drho * gm1inv * iLb * iLb);
+        ga.beta += Lb_b * Lb1;
+        gb.beta += Lb_b * Lb2;
+        rho_mean_b += diss_b * 0.5 * gm1inv * (b.ibeta - a.ibeta);
+        gb.beta += diss_b * 0.5 * rho_mean * gm1inv * (-b.ibeta * b.ibeta);
+        ga.beta += diss_b * 0.5 * rho_mean * gm1inv * (a.ibeta * a.ibeta);
+        ga.rho += 0.5 * rho_mean_b - drho_b;
+        gb.rho += 0.5 * rho_mean_b + drho_b;
+        // lam = max(|V_1| + c_1, |V_2| + c_2) through the branch taken; at a tie the mean of the
+        // two one-sided derivatives (what central differences of the forward measure)
+        const double l1b = l1 > l2 ? lam_b : (l1 < l2 ? 0.0 : 0.5 * lam_b);
+        const double l2b = lam_b - l1b;
+        if (vn1 > 0.0)
+            for (int d = 0; d < DIM; d++) ga.v[d] += l1b * a.v[d] / vn1;
+        if (vn2 > 0.0)
+            for (int d = 0; d < DIM; d++) gb.v[d] += l2b * b.v[d] / vn2;
+        // dc/dbeta = -c / (2 beta)
+        ga.beta += -l1b * c1 * 0.5 * a.ibeta;
+        gb.beta += -l2b * c2 * 0.5 * b.ibeta;
+    }
+    prim_back<DIM>(a, gm1, ga, s1b);
+    prim_back<DIM>(b, gm1, gb, s2b);
+}
+
+}  // namespace adj
+}  // namespace prandtl

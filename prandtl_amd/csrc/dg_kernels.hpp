@@ -25,6 +25,7 @@
 
 #include <type_traits>
 
+#include "dg_adjoint.hpp"
 #include "dg_physics.hpp"
 
 #ifndef PRANDTL_LINE_WAVES
@@ -347,6 +348,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     __shared__ R sPv[DIM][EPB * NN];      // velocity
     __shared__ R sT[NP * NP];             // Dhat2
     __shared__ double sPref[EPB];         // element reference pressure (mixed precision)
+    __shared__ double sHref[EPB];         // element reference enthalpy gamma p / ((gamma-1) rho)
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -367,7 +369,10 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         sPd[1][tid] = qs.beta;
 #pragma unroll
         for (int d = 0; d < DIM; d++) sPv[d][tid] = (R)qs.vel[d];
-        if (MIX && node == 0) sPref[eb] = qs.p;
+        if (MIX && node == 0) {
+            sPref[eb] = qs.p;
+            sHref[eb] = A.P.gamma * A.P.gammaM1Inverse * qs.p * qs.irho;
+        }
         lam = qs.lam;
     }
     __syncthreads();
@@ -381,6 +386,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
             for (int c = 0; c < DIM; c++) adj[r][c] = (R)gm[1 + r * DIM + c];
         const double p_ref = MIX ? sPref[eb] : 0.0;
+        const double h_ref = MIX ? sHref[eb] : 0.0;
         int ai[3];
         ai[0] = node % NP;
         ai[1] = (node / NP) % NP;
@@ -410,7 +416,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         qo.vel[d] = (float)qs.vel[d];
                         qb.vel[d] = sPv[d][li];
                     }
-                    volume_flux_mx<DIM>(qo, qb, m, (float)A.P.gammaM1Inverse, p_ref, F);
+                    volume_flux_mx<DIM>(qo, qb, m, A.P.gammaM1Inverse, p_ref, h_ref, F);
                 } else {
                     const NodePrim<DIM> qo = to_prim(qs);
                     NodePrim<DIM> qb;
@@ -439,9 +445,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 nmag = sqrt(nmag);
                 if constexpr (MIX) {
                     if (side)
-                        face_flux_mx<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, p_ref, f);
+                        face_flux_mx<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, p_ref, h_ref, f);
                     else
-                        face_flux_mx<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, p_ref, f);
+                        face_flux_mx<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, p_ref, h_ref, f);
                 } else {
                     if (side)
                         face_flux<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, f);
@@ -467,19 +473,28 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         }
         const R sw = (R)(A.T.inv_w0 * invJ);
         const R rinvJ = (R)invJ;
+        double du0 = 0.0;
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
             R du = -acc[v] * rinvJ + accs[v] * sw;
+            double dud;
+            if constexpr (MIX) {
+                // the energy sums carried F_E - h_ref F_rho: add h_ref times the inviscid mass RHS
+                if (v == 0) du0 = (double)du;
+                dud = (v == NEQ - 1) ? (double)du + h_ref * du0 : (double)du;
+            } else {
+                dud = du;
+            }
             if constexpr (VISC) {
                 if (v > 0) {
                     if constexpr (MIX)
-                        du += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
+                        dud += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
                     else
-                        du += A.dv[(v - 1) * vs + (size_t)e * NN + node];
+                        dud += A.dv[(v - 1) * vs + (size_t)e * NN + node];
                 }
             }
             // the RK accumulation is FP64 in both modes
-            stage_update<NP>(A, v * vs + (size_t)e * NN + node, s[v], (double)du, bad);
+            stage_update<NP>(A, v * vs + (size_t)e * NN + node, s[v], dud, bad);
         }
     }
     finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
@@ -674,6 +689,215 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
             A.dvf[(size_t)v * vs + (size_t)e * NN + node] = dvv[v] * invJ;
         else
             A.dv[(size_t)v * vs + (size_t)e * NN + node] = dvv[v] * invJ;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_vjp: the discrete adjoint of the Euler operator, ubar = (dL/du)^T lambda, with an
+// optional fused LSRK4(5) stage adjoint.  Forward stage s (2N storage):
+//     k_s = A_s k_{s-1} + dt L(u_{s-1}),   u_s = u_{s-1} + B_s k_s;
+// reverse, given the adjoints (ub, kb) of (u_s, k_s):
+//     lambda = kb + B_s ub,   ub_{s-1} = ub + dt (dL/du(u_{s-1}))^T lambda,   kb_{s-1} = A_s lambda.
+// k_vjp_prep evaluates, once per node, the quantities every pair reads (density, velocity,
+// beta, their reciprocals and logarithms; SoA, NQ = DIM + 6 arrays) and lambda.  k_vjp then
+// works as k_rhs (one thread per GLL line, each symmetric pair once): a pair's cotangent is
+// Fb = -(1/J)(Dhat2[a][b] lambda_a + Dhat2[b][a] lambda_b), its reverse accumulates the
+// adjoints of (rho, v, beta) of the line's nodes in registers, and those are mapped to
+// conserved adjoints once per node and line.  A face point's cotangent is
+// fb = -lambda_1/(w0 J1) + lambda_2/(w0 J2).  Each element accumulates only its own nodes (face
+// adjoints are evaluated by both neighbours), so no atomics.
+// ---------------------------------------------------------------------------
+enum VjpMode { VJP_PLAIN = 0, VJP_LSRK = 1 };
+
+template <int NP>
+struct VjpArgs {
+    DTables<NP> T;
+    PhysConst P;
+    int NE, bug3, mode;
+    const int *nbr;
+    const double *geom;
+    const double *u;      // state at which the Jacobian is taken (the stage input)
+    const double *lam;    // VJP_PLAIN: the cotangent
+    const double *ub;     // VJP_LSRK: adjoint of u_s
+    const double *kb;     // VJP_LSRK: adjoint of k_s
+    double *q;            // per-node quantities, NQ x NE x Np^DIM (written by k_vjp_prep)
+    double *lamw;         // VJP_LSRK: lambda = kb + B_s ub (written by k_vjp_prep)
+    double *out;          // VJP_PLAIN: (dL/du)^T lambda; VJP_LSRK: adjoint of u_{s-1}
+    double *kb_out;       // VJP_LSRK: adjoint of k_{s-1}
+    double ca, cb, dt;    // A_s, B_s, dt
+};
+
+template <int DIM, int NP>
+__global__ void __launch_bounds__(256) k_vjp_prep(const VjpArgs<NP> A) {
+    constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
+    const size_t nodes = (size_t)A.NE * NN;
+    for (size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gi < nodes; gi += (size_t)gridDim.x * blockDim.x) {
+        double s[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s[v] = A.u[v * nodes + gi];
+        const adj::NodeQ<DIM> q = adj::node_q<DIM>(s, A.P.gammaM1);
+        A.q[gi] = q.rho;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) A.q[(1 + d) * nodes + gi] = q.v[d];
+        A.q[(DIM + 1) * nodes + gi] = q.beta;
+        A.q[(DIM + 2) * nodes + gi] = q.irho;
+        A.q[(DIM + 3) * nodes + gi] = q.ibeta;
+        A.q[(DIM + 4) * nodes + gi] = q.lrho;
+        A.q[(DIM + 5) * nodes + gi] = q.lbeta;
+        if (A.mode == VJP_LSRK) {
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) A.lamw[v * nodes + gi] = A.kb[v * nodes + gi] + A.cb * A.ub[v * nodes + gi];
+        }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ adj::NodeQ<DIM> load_q(const double *Q, size_t nodes, size_t gi) {
+    adj::NodeQ<DIM> q;
+    q.rho = Q[gi];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) q.v[d] = Q[(1 + d) * nodes + gi];
+    q.beta = Q[(DIM + 1) * nodes + gi];
+    q.irho = Q[(DIM + 2) * nodes + gi];
+    q.ibeta = Q[(DIM + 3) * nodes + gi];
+    q.lrho = Q[(DIM + 4) * nodes + gi];
+    q.lbeta = Q[(DIM + 5) * nodes + gi];
+    return q;
+}
+
+template <int DIM, int NP, int SURF>
+__global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES) k_vjp(const VjpArgs<NP> A) {
+    using C = LineCfg<DIM, NP>;
+    constexpr int NEQ = DIM + 2, NN = C::NN, L = C::L, EPB = C::EPB;
+    __shared__ double sB[NEQ][EPB * NN];
+
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int e0 = blk * EPB;
+    const int tid = threadIdx.x;
+    const int NE = A.NE;
+    const size_t vs = (size_t)NE * NN;
+    const size_t off0 = (size_t)e0 * NN;
+    const int nvalid = min(EPB, NE - e0) * NN;
+    const int eb = tid / L, l = tid - (tid / L) * L;
+    const int e = e0 + eb;
+    const bool active = (eb < EPB) && (e < NE);
+    const double *lamv = A.mode == VJP_PLAIN ? A.lam : A.lamw;
+    double invJ = 0.0, adjm[DIM][DIM];
+    if (active) {
+        const double *g = A.geom + (size_t)e * (1 + DIM * DIM);
+        invJ = 1.0 / g[0];
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) adjm[r][c] = g[1 + r * DIM + c];
+    }
+    const double gm1 = A.P.gammaM1, gm1inv = A.P.gammaM1Inverse;
+#pragma unroll
+    for (int dir = 0; dir < DIM; dir++) {
+        if (active) {
+            const int base = line_base<NP>(l, dir);
+            const int st = dir_stride<NP>(dir);
+            const size_t ebase = (size_t)e * NN + base;
+            const double *m = adjm[dir];
+            adj::PrimB<DIM> acc[NP];
+#pragma unroll
+            for (int a = 0; a < NP; a++) adj::primb_zero<DIM>(acc[a]);
+#pragma unroll 1
+            for (int a = 0; a < NP - 1; a++) {
+                const size_t ga = ebase + a * st;
+                const adj::NodeQ<DIM> qa = load_q<DIM>(A.q, vs, ga);
+                double la[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) la[v] = lamv[v * vs + ga];
+#pragma unroll 1
+                for (int b = a + 1; b < NP; b++) {
+                    const size_t gb = ebase + b * st;
+                    const adj::NodeQ<DIM> qb = load_q<DIM>(A.q, vs, gb);
+                    const double dab = A.T.Dhat2[a][b], dba = A.T.Dhat2[b][a];
+                    double Fb[NEQ];
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) Fb[v] = -invJ * (dab * la[v] + dba * lamv[v * vs + gb]);
+                    adj::PrimB<DIM> pa, pb;
+                    adj::primb_zero<DIM>(pa);
+                    adj::primb_zero<DIM>(pb);
+                    adj::ec_core_back<DIM>(qa, qb, m, gm1inv, Fb, pa, pb);
+                    // dynamic register indices avoided: add through a switch on the line position
+#pragma unroll
+                    for (int r = 0; r < NP; r++) {
+                        if (r == a) {
+                            acc[r].rho += pa.rho;
+                            acc[r].beta += pa.beta;
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) acc[r].v[d] += pa.v[d];
+                        }
+                        if (r == b) {
+                            acc[r].rho += pb.rho;
+                            acc[r].beta += pb.beta;
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) acc[r].v[d] += pb.v[d];
+                        }
+                    }
+                }
+            }
+            // conserved adjoints of the line's nodes
+#pragma unroll
+            for (int a = 0; a < NP; a++) {
+                const size_t ga = ebase + a * st;
+                const adj::NodeQ<DIM> qa = load_q<DIM>(A.q, vs, ga);
+                double sb[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) sb[v] = 0.0;
+                adj::prim_back<DIM>(qa, gm1, acc[a], sb);
+                const int li = eb * NN + base + a * st;
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    if (dir == 0)
+                        sB[v][li] = sb[v];
+                    else
+                        sB[v][li] += sb[v];
+                }
+            }
+            // surface terms at both ends of the line
+#pragma unroll
+            for (int side = 1; side >= 0; side--) {
+                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + side];
+                const int nown = base + (side ? (NP - 1) * st : 0);
+                const int nnb = base + (side ? 0 : (NP - 1) * st);
+                const double swo = A.T.inv_w0 * invJ;
+                const double swn = A.T.inv_w0 / A.geom[(size_t)en * (1 + DIM * DIM)];
+                double so[NEQ], sn[NEQ], fb[NEQ], b1[NEQ], b2[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    const size_t go = v * vs + (size_t)e * NN + nown, gn = v * vs + (size_t)en * NN + nnb;
+                    so[v] = A.u[go];
+                    sn[v] = A.u[gn];
+                    // side 1: own = el1 (gets -f/(w0 J)), neighbour = el2; side 0: the reverse
+                    fb[v] = side ? (-swo * lamv[go] + swn * lamv[gn]) : (-swn * lamv[gn] + swo * lamv[go]);
+                    b1[v] = b2[v] = 0.0;
+                }
+                if (side)
+                    adj::face_flux_vjp<DIM>(SURF, so, sn, m, A.P.gamma, A.bug3 != 0, fb, b1, b2);
+                else
+                    adj::face_flux_vjp<DIM>(SURF, sn, so, m, A.P.gamma, A.bug3 != 0, fb, b1, b2);
+                const int li = eb * NN + nown;
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) sB[v][li] += side ? b1[v] : b2[v];
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nvalid; idx += C::BLOCK) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            const size_t gi = v * vs + off0 + idx;
+            const double g = sB[v][idx];
+            if (A.mode == VJP_PLAIN) {
+                A.out[gi] = g;
+            } else {
+                A.out[gi] = A.ub[gi] + A.dt * g;
+                A.kb_out[gi] = A.ca * A.lamw[gi];
+            }
+        }
     }
 }
 

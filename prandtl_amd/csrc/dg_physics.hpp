@@ -196,7 +196,10 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
 // equation is being assembled: a constant pressure contributes nothing to the assembled
 // RHS on an affine element (the volume sum of D-hat-2 rows and the surface terms cancel,
 // i.e. free-stream preservation), and subtracting it in FP64 keeps the FP32 momentum
-// sums from cancelling three digits of a ~1/(gamma Ma^2) pressure.
+// sums from cancelling three digits of a ~1/(gamma Ma^2) pressure.  Likewise the energy
+// fluxes carry F_E - h_ref F_rho (h_ref a constant enthalpy of the element); the caller adds
+// h_ref times the assembled mass RHS back in FP64, which restores the energy RHS exactly in
+// exact arithmetic while the FP32 sums see only the enthalpy's variation.
 // ---------------------------------------------------------------------------
 template <int DIM>
 struct NodePrimMx {
@@ -206,7 +209,7 @@ struct NodePrimMx {
 
 template <int DIM>
 __device__ __forceinline__ void volume_flux_mx(const NodePrimMx<DIM> &a, const NodePrimMx<DIM> &b, const float *m,
-                                               float gammaM1Inverse, double p_ref, float *F) {
+                                               double gammaM1Inverse, double p_ref, double h_ref, float *F) {
     const double rsum = a.rho + b.rho;
     const double r_rho = rcp(rsum);
     const double r_b = rcp(a.beta + b.beta);
@@ -223,7 +226,7 @@ __device__ __forceinline__ void volume_flux_mx(const NodePrimMx<DIM> &a, const N
         h_hat += -0.25f * (a.vel[d] * a.vel[d] + b.vel[d] * b.vel[d]) + mean[d] * mean[d];
         Vn += mean[d] * m[d];
     }
-    h_hat += 0.5f * ilb * gammaM1Inverse + (float)p_hat * ilr;
+    h_hat += (float)(0.5 * il_b * gammaM1Inverse + p_hat * il_rho - h_ref);
     const float pd = (float)(p_hat - p_ref);
     const float mflux = rho_ln * Vn;
     F[0] = mflux;
@@ -235,7 +238,7 @@ __device__ __forceinline__ void volume_flux_mx(const NodePrimMx<DIM> &a, const N
 template <int DIM, int SURF>
 __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<DIM> &q1, const double *s2,
                                              const SideState<DIM> &q2, const float *n, float nmag, const PhysConst &P,
-                                             bool bug3, double p_ref, float *flux) {
+                                             bool bug3, double p_ref, double h_ref, float *flux) {
     if constexpr (SURF == SURF_CHANDRASHEKAR) {
         const double rsum = q1.rho + q2.rho;
         const double r_rho = rcp(rsum);
@@ -263,8 +266,11 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
                 diss += 0.5f * drho * v1 * v2 + rho_mean * dv * vm[d];
         }
         const float lam = (float)fmax(q1.lam, q2.lam);
-        h_hat += 0.5f * ilb * g1 + (float)p_hat * ilr;
-        diss += 0.5f * drho * g1 * ilb + rho_mean * g1 * (float)(q2.p * q2.irho - q1.p * q1.irho);
+        h_hat += (float)(0.5 * il_b * P.gammaM1Inverse + p_hat * il_rho - h_ref);
+        // diss - h_ref drho: the energy jump measured from h_ref times the density jump
+        const double drho64 = q2.rho - q1.rho;
+        diss += (float)(drho64 * (0.5 * P.gammaM1Inverse * il_b - h_ref)) +
+                rho_mean * g1 * (float)(q2.p * q2.irho - q1.p * q1.irho);
         const float hl = 0.5f * lam * nmag;
         const float mflux = rho_ln * Vn;
         const float pd = (float)(p_hat - p_ref);
@@ -283,7 +289,7 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
             a.vel[d] = (float)q1.vel[d];
             b.vel[d] = (float)q2.vel[d];
         }
-        volume_flux_mx<DIM>(a, b, n, (float)P.gammaM1Inverse, p_ref, flux);
+        volume_flux_mx<DIM>(a, b, n, P.gammaM1Inverse, p_ref, h_ref, flux);
     } else {
         double nd[DIM], f1[DIM + 2], f2[DIM + 2];
 #pragma unroll
@@ -297,6 +303,8 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
             if (v >= 1 && v <= DIM) fv -= p_ref * nd[v - 1];
             flux[v] = (float)fv;
         }
+        flux[DIM + 1] = (float)(0.5 * (f1[DIM + 1] + f2[DIM + 1]) - hl * (s2[DIM + 1] - s1[DIM + 1]) -
+                                h_ref * (0.5 * (f1[0] + f2[0]) - hl * (s2[0] - s1[0])));
     }
 }
 

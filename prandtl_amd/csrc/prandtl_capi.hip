@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -122,9 +123,19 @@ struct prandtl_op_s {
     double *d_k = nullptr;        // RK register
     double *d_ub = nullptr;       // RK alternate state buffer
     double *d_y2 = nullptr;       // SSP-RK3 second stage state
+    double *d_vq = nullptr;       // adjoint: per-node quantities, (dim+6) x NE x n
+    double *d_vlam = nullptr;     // adjoint: stage cotangent lambda
     unsigned long long *d_lam = nullptr;
     int *d_nan = nullptr;
     bool lam_valid = false;
+    // taped forward run for the discrete adjoint: the state at the start of every segment
+    int tape_steps = 0, tape_every = 0;
+    double tape_dt = 0.0;
+    std::vector<double *> tape_ckpt;
+    // adjoint sweep storage, kept between calls: the stage inputs of one segment (5 per step;
+    // slot 0 points at the segment's checkpoint) and the double-buffered adjoint registers
+    std::vector<double *> adj_stage;
+    double *adj_ub2 = nullptr, *adj_kb = nullptr, *adj_kb2 = nullptr;
 };
 
 namespace {
@@ -269,6 +280,82 @@ int ensure_rk_buffers(prandtl_op_s *op, int scheme) {
         HIP_TRY(hipMalloc(&op->d_y2, bytes));
     }
     return PRANDTL_OK;
+}
+
+void release_adjoint_storage(prandtl_op_s *op) {
+    for (size_t i = 1; i < op->adj_stage.size(); i++) (void)hipFree(op->adj_stage[i]);
+    op->adj_stage.clear();
+    (void)hipFree(op->adj_ub2);
+    (void)hipFree(op->adj_kb);
+    (void)hipFree(op->adj_kb2);
+    op->adj_ub2 = op->adj_kb = op->adj_kb2 = nullptr;
+}
+
+struct VjpSpec {
+    int mode;
+    const double *u, *lam, *ub, *kb;
+    double *out, *kb_out;
+    double ca, cb, dt;
+};
+
+template <int DIM, int NP>
+int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
+    const size_t nodes = (size_t)op->NE * op->nn;
+    if (!op->d_vq) HIP_TRY(hipMalloc(&op->d_vq, (size_t)(DIM + 6) * nodes * sizeof(double)));
+    if (!op->d_vlam) HIP_TRY(hipMalloc(&op->d_vlam, op->size * sizeof(double)));
+    VjpArgs<NP> a;
+    a.T = make_tables<NP>(op);
+    a.P = op->P;
+    a.NE = op->NE;
+    a.bug3 = (op->opts.bug_compat & PRANDTL_BUG_A3) ? 1 : 0;
+    a.mode = S.mode;
+    a.nbr = op->d_nbr;
+    a.geom = op->d_geom;
+    a.u = S.u;
+    a.lam = S.lam;
+    a.ub = S.ub;
+    a.kb = S.kb;
+    a.q = op->d_vq;
+    a.lamw = op->d_vlam;
+    a.out = S.out;
+    a.kb_out = S.kb_out;
+    a.ca = S.ca;
+    a.cb = S.cb;
+    a.dt = S.dt;
+    const int pblk = (int)std::min<size_t>((nodes + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_vjp_prep<DIM, NP>), dim3(pblk), dim3(256), 0, st, a);
+    using LC = LineCfg<DIM, NP>;
+    const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+    if (op->opts.surface_flux == PRANDTL_SURF_CHANDRASHEKAR)
+        hipLaunchKernelGGL((k_vjp<DIM, NP, 0>), dim3(nblk), dim3(LC::BLOCK), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_vjp<DIM, NP, 1>), dim3(nblk), dim3(LC::BLOCK), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    return PRANDTL_OK;
+}
+
+int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
+    if (op->phys.viscous)
+        return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers the Euler operator (Navier-Stokes not built)");
+    if (op->opts.surface_flux == PRANDTL_SURF_RUSANOV)
+        return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers the Chandrashekar and EC surface fluxes");
+    if (op->opts.precision != PRANDTL_FP64) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint is FP64");
+    if (op->dim == 2) {
+        switch (op->np) {
+            case 2: return launch_vjp<2, 2>(op, S, st);
+            case 3: return launch_vjp<2, 3>(op, S, st);
+            case 4: return launch_vjp<2, 4>(op, S, st);
+            case 5: return launch_vjp<2, 5>(op, S, st);
+        }
+    } else {
+        switch (op->np) {
+            case 2: return launch_vjp<3, 2>(op, S, st);
+            case 3: return launch_vjp<3, 3>(op, S, st);
+            case 4: return launch_vjp<3, 4>(op, S, st);
+            case 5: return launch_vjp<3, 5>(op, S, st);
+        }
+    }
+    return fail(PRANDTL_ERR_UNSUPPORTED, "order out of range");
 }
 
 // Carpenter & Kennedy (1994) 2N-storage RK4(3)5 coefficients.
@@ -448,8 +535,12 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_k);
     (void)hipFree(op->d_ub);
     (void)hipFree(op->d_y2);
+    (void)hipFree(op->d_vq);
+    (void)hipFree(op->d_vlam);
     (void)hipFree(op->d_lam);
     (void)hipFree(op->d_nan);
+    for (double *p : op->tape_ckpt) (void)hipFree(p);
+    release_adjoint_storage(op);
     delete op;
 }
 
@@ -535,6 +626,99 @@ int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return PRANDTL_OK;
+}
+
+int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *out, prandtl_stream stream) {
+    if (!op || !u || !lam || !out) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (out == u || out == lam) return fail(PRANDTL_ERR_ARG, "out must not alias u or lam");
+    VjpSpec S{VJP_PLAIN, u, lam, nullptr, nullptr, out, nullptr, 0.0, 0.0, 0.0};
+    return run_vjp(op, S, (hipStream_t)stream);
+}
+
+int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
+                        prandtl_stream stream) {
+    if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (scheme != PRANDTL_LSRK45) return fail(PRANDTL_ERR_UNSUPPORTED, "the taped run supports LSRK4(5)");
+    if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
+    hipStream_t st = (hipStream_t)stream;
+    const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+    if ((int)op->tape_ckpt.size() != nseg) {   // checkpoint storage is kept for runs of the same length
+        release_adjoint_storage(op);            // its slot 0 points at a checkpoint
+        for (double *p : op->tape_ckpt) (void)hipFree(p);
+        op->tape_ckpt.assign(nseg, nullptr);
+        for (int k = 0; k < nseg; k++)
+            if (hipMalloc(&op->tape_ckpt[k], op->size * sizeof(double)) != hipSuccess) {
+                for (double *p : op->tape_ckpt) (void)hipFree(p);
+                op->tape_ckpt.clear();
+                return fail(PRANDTL_ERR_HIP, "checkpoint storage allocation failed");
+            }
+    }
+    op->tape_steps = nsteps;
+    op->tape_every = ckpt_every;
+    op->tape_dt = dt;
+    for (int k = 0; k < nseg; k++) {
+        HIP_TRY(hipMemcpyAsync(op->tape_ckpt[k], u, op->size * sizeof(double), hipMemcpyDeviceToDevice, st));
+        const int n = std::min(ckpt_every, nsteps - k * ckpt_every);
+        int rc = rk_advance(op, scheme, u, dt, n, 0, st);
+        if (rc) return rc;
+    }
+    if (t) *t += dt * nsteps;
+    return PRANDTL_OK;
+}
+
+int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
+    if (!op || !ubar) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (op->tape_ckpt.empty()) return fail(PRANDTL_ERR_ARG, "no taped run on this operator");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t bytes = op->size * sizeof(double);
+    const int C = op->tape_every, nsteps = op->tape_steps, nseg = (int)op->tape_ckpt.size();
+    int rc = ensure_rk_buffers(op, PRANDTL_LSRK45);
+    if (rc) return rc;
+    // stage inputs of one segment (5 per step; slot 0 is the segment's checkpoint itself), and
+    // the adjoint registers (ub, kb double-buffered: a stage reads its neighbours' adjoints)
+    if (op->adj_stage.size() != (size_t)C * 5) {
+        release_adjoint_storage(op);
+        op->adj_stage.assign((size_t)C * 5, nullptr);
+        for (size_t i = 1; i < op->adj_stage.size(); i++)
+            if (hipMalloc(&op->adj_stage[i], bytes) != hipSuccess) {
+                release_adjoint_storage(op);
+                return fail(PRANDTL_ERR_HIP, "adjoint stage storage allocation failed");
+            }
+        if (hipMalloc(&op->adj_ub2, bytes) != hipSuccess || hipMalloc(&op->adj_kb, bytes) != hipSuccess ||
+            hipMalloc(&op->adj_kb2, bytes) != hipSuccess) {
+            release_adjoint_storage(op);
+            return fail(PRANDTL_ERR_HIP, "adjoint register allocation failed");
+        }
+    }
+    std::vector<double *> &stage = op->adj_stage;
+    double *ub2 = op->adj_ub2, *kb = op->adj_kb, *kb2 = op->adj_kb2;
+    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, kb, 0.0, op->size);
+    double *ub_cur = ubar, *ub_nxt = ub2, *kb_cur = kb, *kb_nxt = kb2;
+    for (int seg = nseg - 1; seg >= 0 && rc == PRANDTL_OK; seg--) {
+        const int n = std::min(C, nsteps - seg * C);
+        // recompute the segment forward from its checkpoint, keeping every stage input (the
+        // last stage's output, the segment's end state, is not needed)
+        stage[0] = op->tape_ckpt[seg];
+        for (int j = 0; j < 5 * n - 1 && rc == PRANDTL_OK; j++) {
+            const int stg = j % 5;
+            StageSpec S{EPI_LSRK, stage[j], stage[j + 1], op->d_k, nullptr, CK_A[stg], CK_B[stg], op->tape_dt,
+                        nullptr, false, false};
+            rc = run_stage(op, S, st);
+        }
+        // reverse through the segment's stages
+        for (int j = 5 * n - 1; j >= 0 && rc == PRANDTL_OK; j--) {
+            const int stg = j % 5;
+            VjpSpec V{VJP_LSRK, stage[j], nullptr, ub_cur, kb_cur, ub_nxt, kb_nxt, CK_A[stg], CK_B[stg], op->tape_dt};
+            rc = run_vjp(op, V, st);
+            std::swap(ub_cur, ub_nxt);
+            std::swap(kb_cur, kb_nxt);
+        }
+    }
+    if (rc == PRANDTL_OK && ub_cur != ubar) (void)hipMemcpyAsync(ubar, ub_cur, bytes, hipMemcpyDeviceToDevice, st);
+    hipError_t e = hipStreamSynchronize(st);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(PRANDTL_ERR_HIP, hipGetErrorString(e));
     return PRANDTL_OK;
 }
 

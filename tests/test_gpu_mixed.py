@@ -2,9 +2,11 @@
 FP64 logarithmic means, RK accumulation and CFL reduction.  Its results are held against
 the FP64 GPU path, whose parity with the oracle is established in test_gpu_parity.py:
 
-* RHS on the Taylor-Green state (16^3, p=3, Navier-Stokes): max-norm difference <= 2e-5 of
-  the RHS max-norm over all variables (the density RHS of the divergence-free initial state
-  is itself only discretisation error, so a per-variable relative bar is meaningless there);
+* RHS on the Taylor-Green state (16^3, p=3, Navier-Stokes): every entry within
+  8 eps_32 x its rounding scale of the oracle (the oracle's assembly with every term in
+  absolute value, as the FP64 parity bound uses with eps_64).  The energy entries carry the
+  largest errors: the energy RHS contains h = gamma p / ((gamma-1) rho) ~ 250 (Ma = 0.1) times
+  the FP32 mass-flux divergence, whose rounding the FP64 h cannot remove;
 * discrete conservation (sum J w du = 0) to FP32 rounding: <= 1e-6 of sum |J w du|;
 * free-stream preservation: |du| <= 1e-5 x the flux scale p |Ja| / J;
 * 100 LSRK4(5) steps at 16^3: kinetic-energy dissipation rate within 1e-5 relative, density
@@ -31,6 +33,8 @@ def _ke_rate(mesh, u, du, W):
 def test_mixed_rhs_vs_fp64(device):
     import torch
 
+    import oracle
+
     _say("mixed vs FP64 RHS, TGV 16^3 p=3 NS")
     m = cases.tgv_mesh(n=16, order=3)
     u = cases.tgv_state(m)
@@ -44,11 +48,16 @@ def test_mixed_rhs_vs_fp64(device):
     opmx.mult(ud, d32)
     torch.cuda.synchronize()
     a, b = d64.cpu().numpy(), d32.cpu().numpy()
-    scale = np.abs(a).max()
+    oph = oracle.phys(gamma=ph["gamma"], Pr=ph["Pr"], mu=ph["mu"], R_gas=ph["R_gas"], viscous=True)
+    ref, _, mag = oracle.rhs_mag(m, oph, u)
+    eps32 = float(np.finfo(np.float32).eps)
+    scale = np.abs(ref).max()
     for v in range(5):
-        rel = np.abs(a[v] - b[v]).max() / scale
-        print(f"[mixed] var {v} max-norm diff vs FP64 / ||du||_inf = {rel:.3e}", flush=True)
-        assert rel <= 2e-5, (v, rel)
+        sc = float(np.max(np.abs(b[v] - ref[v]) / (eps32 * mag[v] + 1e-300)))
+        rel = np.abs(b[v] - a[v]).max() / scale
+        print(f"[mixed] var {v}: max |mixed - oracle| = {sc:.2f} eps_32 x rounding scale; "
+              f"max-norm diff vs FP64 / ||du||_inf = {rel:.3e}", flush=True)
+        assert sc <= 8.0, (v, sc)
     W = m.quadrature_weights()
     for v in range(5):
         s, t = (W * b[v]).sum(), (W * np.abs(b[v])).sum()
