@@ -76,11 +76,17 @@ typedef struct {
     int sutherland; /* 1: Sutherland viscosity (SUTHERLAND) */
 } prandtl_physics;
 
+/* Stage hooks the steppers (prandtl_rk_step, prandtl_rk_run) apply after every RK stage:
+ * the entropy filter (bounds from the state at the start of each step), then the
+ * positivity limiter -- the Filter and Limiter hooks of src/Filters/Filter.hpp:11-30 and
+ * src/Limiters/Limiter.hpp:10-16. */
+enum { PRANDTL_HOOK_FILTER = 1, PRANDTL_HOOK_LIMITER = 2 };
+
 typedef struct {
     int surface_flux; /* PRANDTL_SURF_* */
     int precision;    /* PRANDTL_FP64 | PRANDTL_MIXED */
     int bug_compat;   /* PRANDTL_BUG_* bitmask */
-    int reserved;
+    int stage_hooks;  /* PRANDTL_HOOK_* bitmask */
 } prandtl_opts;
 
 /* Build an operator -- replaces the DGSEMOperator constructor,
@@ -123,6 +129,28 @@ int prandtl_rk_step(prandtl_op op, int scheme, double *u, double *t, double dt, 
  * PRANDTL_ERR_NAN if one is found. */
 int prandtl_rk_run(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
                    prandtl_stream stream);
+
+/* Entropy-filter bounds from u: per element, the minimum nodal entropy rho ln(p rho^-gamma)
+ * over the element and its face neighbours -- ref: Filter::GetFilterConstraints(const
+ * Vector&), src/Filters/Filter.hpp:27; EntropyFilter::GetEntropyConstraints,
+ * src/Filters/EntropyFilter.cpp:81-158 (whose empty ComputeElementMinima, :275-277, is
+ * implemented as intended). */
+int prandtl_filter_constraints(prandtl_op op, const double *u, prandtl_stream stream);
+
+/* Entropy filter of u in place, against the last constraints -- ref: Filter::FilterSolution
+ * (Vector&), Filter.hpp:21; EntropyFilter::FilterSolution, EntropyFilter.cpp:171-242 (the
+ * root-find against the element's own bound, :338).  Elements with rho, p >= 1e-8 and
+ * entropy >= bound - 1e-4 at every node are untouched. */
+int prandtl_filter_apply(prandtl_op op, double *u, prandtl_stream stream);
+
+/* Positivity limiter of u in place -- ref: Limiter::LimitSolution(Vector&),
+ * src/Limiters/Limiter.hpp:14 (the reference has the interface only): the Zhang-Shu scaling
+ * toward the element's GLL mean so that rho, p >= 1e-8 at every node. */
+int prandtl_limiter_apply(prandtl_op op, double *u, prandtl_stream stream);
+
+/* Elements changed by the filter and by the limiter since the previous call (resets the
+ * counters; synchronises the device). */
+int prandtl_hook_counts(prandtl_op op, long long *filtered, long long *limited);
 
 /* Discrete adjoint (no counterpart in the reference, which has no derivatives).
  * out = (dL/du)^T lam at state u: the vector-Jacobian product of the semi-discrete operator

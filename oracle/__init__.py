@@ -73,6 +73,20 @@ def _configure(L):
     L.orc_rk_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
                              ctypes.c_int, ctypes.c_double, dp]
     L.orc_rk_run.restype = ctypes.c_int
+    # stage hooks (prandtl_oracle_hooks.cpp)
+    L.orc_modal_basis.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ip]
+    L.orc_modal_basis.restype = ctypes.c_int
+    L.orc_legendre01.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+    L.orc_legendre01.restype = None
+    L.orc_filter_constraints.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, ctypes.c_double, dp]
+    L.orc_filter_constraints.restype = ctypes.c_int
+    L.orc_filter_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, dp, dp]
+    L.orc_filter_apply.restype = ctypes.c_int
+    L.orc_limiter_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]
+    L.orc_limiter_apply.restype = ctypes.c_int
+    L.orc_rk_run_hooked.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, ip]
+    L.orc_rk_run_hooked.restype = ctypes.c_int
     return L
 
 
@@ -220,3 +234,66 @@ def rk_run(mesh, ph, u, scheme: int, nsteps: int, dt: float, fma=False):
 def rounding_floor(a, b) -> float:
     """Relative max-norm difference of two oracle builds' results (their rounding spread)."""
     return float(np.abs(a - b).max() / np.abs(a).max())
+
+
+# ---------------------------------------------------------------------------
+# stage hooks: entropy filter and positivity limiter (prandtl_oracle_hooks.cpp)
+# ---------------------------------------------------------------------------
+HOOK_FILTER, HOOK_LIMITER = 1, 2
+LIMITER_EPS = 1e-8
+
+
+def legendre01(p: int, x: float):
+    """P_n(2x - 1), n = 0..p (MFEM Poly_1D::CalcLegendre)."""
+    L = np.zeros(p + 1)
+    lib().orc_legendre01(int(p), float(x), _dp(L))
+    return L
+
+
+def modal_basis(dim: int, order: int):
+    """(V, V^-1, maxdeg) of the ModalBasis on the tensor GLL nodes."""
+    n = (order + 1) ** dim
+    V, Vi = np.zeros((n, n)), np.zeros((n, n))
+    md = np.zeros(n, dtype=np.int32)
+    assert lib().orc_modal_basis(dim, order, _dp(V), _dp(Vi), _ip(md)) == 0
+    return V, Vi, md
+
+
+def filter_constraints(mesh, u, gamma=1.4):
+    """Per-element entropy bounds: min nodal rho ln(p rho^-gamma) over the element and its face
+    neighbours (EntropyFilter::GetEntropyConstraints with the intended element minima)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    b = np.zeros(mesh.num_elements)
+    assert lib().orc_filter_constraints(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(u),
+                                        float(gamma), _dp(b)) == 0
+    return b
+
+
+def filter_apply(mesh, u, bound, gamma=1.4):
+    """EntropyFilter::FilterSolution on a copy of u: (filtered u, zeta per element, count)."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    z = np.zeros(mesh.num_elements)
+    bound = np.ascontiguousarray(bound, dtype=np.float64)
+    c = lib().orc_filter_apply(mesh.dim, mesh.order, mesh.num_elements, _dp(u), float(gamma), _dp(bound), _dp(z))
+    assert c >= 0, c
+    return u, z, c
+
+
+def limiter_apply(mesh, u, gamma=1.4, eps=LIMITER_EPS):
+    """Zhang-Shu positivity limiter on a copy of u: (limited u, count)."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    c = lib().orc_limiter_apply(mesh.dim, mesh.order, mesh.num_elements, _dp(u), float(gamma), float(eps))
+    assert c >= 0, c
+    return u, c
+
+
+def rk_run_hooked(mesh, ph, u, scheme: int, nsteps: int, dt: float, hooks: int):
+    """rk_run with the stage hooks (filter constraints from the step's initial state; after every
+    stage the filter, then the limiter).  Returns (u, [elements filtered, elements limited])."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    counts = np.zeros(2, dtype=np.int32)
+    rc = lib().orc_rk_run_hooked(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+                                 ctypes.byref(ph), int(scheme), int(nsteps), float(dt), int(hooks), _dp(u),
+                                 _ip(counts))
+    assert rc == 0, rc
+    return u, counts

@@ -31,6 +31,8 @@
 #include <cstring>
 #include <limits>
 #include <vector>
+
+#include "prandtl_oracle.h"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -927,13 +929,6 @@ void MakeMesh(Mesh &M, int dim, int order, int NE, const int *nbr, const double 
 // ===========================================================================
 extern "C" {
 
-typedef struct {
-    double gamma, Pr, mu, R_gas;
-    int viscous;       // 0 Euler, 1 Navier-Stokes (BR1)
-    int sutherland;    // 1: Sutherland viscosity
-    int surface_flux;  // 0 Chandrashekar (reference), 1 EC, 2 Rusanov
-    int bug_compat;    // bit 0: A1, bit 1: A2, bit 2: A3
-} orc_phys;
 
 static Phys phys_of(int dim, const orc_phys *p) {
     return MakePhys(dim, p->gamma, p->Pr, p->mu, p->R_gas, p->viscous, p->sutherland, p->surface_flux,

@@ -1,0 +1,22 @@
+// prandtl_oracle.h -- shared C declarations of the CPU oracle (test infrastructure only).
+#pragma once
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double gamma, Pr, mu, R_gas;
+    int viscous;       // 0 Euler, 1 Navier-Stokes (BR1)
+    int sutherland;    // 1: Sutherland viscosity
+    int surface_flux;  // 0 Chandrashekar (reference), 1 EC, 2 Rusanov
+    int bug_compat;    // bit 0: A1, bit 1: A2, bit 2: A3
+} orc_phys;
+
+int orc_gll(int np, double *x, double *w);
+int orc_rhs(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, const double *u,
+            double *dudt, double *grad, double *lambda_max);
+
+#ifdef __cplusplus
+}
+#endif

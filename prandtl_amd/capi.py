@@ -22,12 +22,14 @@ SURF_CHANDRASHEKAR, SURF_EC, SURF_RUSANOV = 0, 1, 2
 FP64, MIXED = 0, 1
 BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
 SSPRK3, LSRK45 = 0, 1
+HOOK_FILTER, HOOK_LIMITER = 1, 2
 
 # every symbol the public header declares
 EXPORTED = [
     "prandtl_op_create", "prandtl_op_destroy", "prandtl_op_size", "prandtl_op_mult", "prandtl_op_mult_grad",
     "prandtl_op_get_max_char_speed", "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run",
     "prandtl_op_profile", "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_last_error",
+    "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
 ]
 
 
@@ -43,7 +45,7 @@ class Physics(ctypes.Structure):
 
 class Opts(ctypes.Structure):
     _fields_ = [("surface_flux", ctypes.c_int), ("precision", ctypes.c_int), ("bug_compat", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("stage_hooks", ctypes.c_int)]
 
 
 class PrandtlError(RuntimeError):
@@ -79,11 +81,16 @@ def load():
         L.prandtl_op_vjp.argtypes = [vp, vp, vp, vp, vp]
         L.prandtl_rk_run_tape.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.prandtl_rk_adjoint.argtypes = [vp, vp, vp]
+        L.prandtl_filter_constraints.argtypes = [vp, vp, vp]
+        L.prandtl_filter_apply.argtypes = [vp, vp, vp]
+        L.prandtl_limiter_apply.argtypes = [vp, vp, vp]
+        L.prandtl_hook_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
         L.prandtl_last_error.argtypes = []
         L.prandtl_last_error.restype = ctypes.c_char_p
         for n in ("prandtl_op_mult", "prandtl_op_mult_grad", "prandtl_op_get_max_char_speed",
                   "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile",
-                  "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint"):
+                  "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_filter_constraints",
+                  "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -113,7 +120,7 @@ class Operator:
     """Owns a prandtl_op handle: the DGSEMOperator of one mesh and physics."""
 
     def __init__(self, mesh: Mesh, gamma=1.4, viscous=False, Pr=0.72, mu=0.0, R_gas=287.0, sutherland=False,
-                 surface_flux=SURF_CHANDRASHEKAR, precision=FP64, bug_compat=0):
+                 surface_flux=SURF_CHANDRASHEKAR, precision=FP64, bug_compat=0, stage_hooks=0):
         L = load()
         self.mesh = mesh
         self._nbr = np.ascontiguousarray(mesh.neighbors, dtype=np.int32)
@@ -122,7 +129,7 @@ class Operator:
                       self._nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                       self._geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         ph = Physics(gamma, Pr, mu, R_gas, int(viscous), int(sutherland))
-        op = Opts(int(surface_flux), int(precision), int(bug_compat), 0)
+        op = Opts(int(surface_flux), int(precision), int(bug_compat), int(stage_hooks))
         h = ctypes.c_void_p()
         _check(L.prandtl_op_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h)))
         self._h = h
@@ -189,3 +196,19 @@ class Operator:
     def rk_adjoint(self, ubar, stream=None):
         """ubar: dJ/du_final on entry, dJ/du_initial on return (for the last taped run)."""
         _check(load().prandtl_rk_adjoint(self._h, _ptr(ubar), _stream(stream)))
+
+    # stage hooks -- Filter::GetFilterConstraints / FilterSolution, Limiter::LimitSolution
+    def filter_constraints(self, u, stream=None):
+        _check(load().prandtl_filter_constraints(self._h, _ptr(u), _stream(stream)))
+
+    def filter_apply(self, u, stream=None):
+        _check(load().prandtl_filter_apply(self._h, _ptr(u), _stream(stream)))
+
+    def limiter_apply(self, u, stream=None):
+        _check(load().prandtl_limiter_apply(self._h, _ptr(u), _stream(stream)))
+
+    def hook_counts(self):
+        """(elements filtered, elements limited) since the previous call."""
+        f, l_ = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        _check(load().prandtl_hook_counts(self._h, ctypes.byref(f), ctypes.byref(l_)))
+        return f.value, l_.value

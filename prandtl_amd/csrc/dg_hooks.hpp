@@ -1,0 +1,349 @@
+// dg_hooks.hpp -- stage hooks on the GPU: the entropy filter and the positivity limiter.
+//
+// Entropy filter -- ref: src/Filters/EntropyFilter.cpp (GetEntropyConstraints :81-158,
+// FilterSolution :171-242, ComputeHierarchStates :244-260, ComputeRoot :325-349) and
+// src/ModalBasis/ModalBasis.cpp, with the reference's defects replaced by the intended
+// behaviour: the entropy bound of an element is the minimum nodal entropy over the element
+// and its face neighbours (ComputeElementMinima, :275-277, is empty in the reference), the
+// root-find compares with the element's own bound (the reference reads the first element's,
+// :338), and the entropy rho ln(p rho^-gamma) is used for Euler and Navier-Stokes states.
+//
+// The filtered state of node i is  u_i(zeta) = sum_j exp(-zeta j^2) H_j(i),  with H_j the part
+// of the element's Legendre expansion whose modes have maximum degree j.  The reference forms
+// H_j from V^-1 u and V; here H_j = S_j - S_{j-1} with S_j = (P_j x P_j [x P_j]) u, P_j the 1D
+// projection onto Legendre degrees <= j on the GLL nodes (a tensor product, applied one
+// direction at a time through LDS): the same polynomial, O(n Np) work per element instead of
+// O(n^2).  The root-find is the reference's node-by-node scan: nodes are visited in order at
+// the current zeta; the first inadmissible one runs the 20-step bisection on [zeta,
+// -ln 1e-10] and raises zeta to the bracket's admissible end; the scan resumes after it.  A
+// workgroup (one element, one thread per node) evaluates all remaining nodes at once and
+// picks the first inadmissible one with an LDS atomicMin, which visits exactly the nodes the
+// sequential scan visits, in its order, at its zeta.  Root-finding and the constraint tests
+// are FP64.
+//
+// Positivity limiter -- the reference declares only the Limiter interface
+// (src/Limiters/Limiter.hpp:10-16).  This is the scaling limiter of Zhang & Shu (J. Comput.
+// Phys. 229 (2010) 8918-8934): with the GLL-mean state ubar of an element, rho is scaled
+// toward rhobar by theta_1 = min(1, (rhobar - eps)/(rhobar - rho_min)), then the state toward
+// ubar by theta_2 = min_i t_i, t_i the admissible end of a bisection for
+// p(ubar + t (u_i - ubar)) = eps on [0, 1] down to a bracket of 2^-52.
+#pragma once
+
+#include "dg_kernels.hpp"
+
+namespace prandtl {
+
+constexpr double kFilterRhoMin = 1e-8;   // EntropyFilter.hpp:59
+constexpr double kFilterPMin = 1e-8;     // :60
+constexpr double kFilterEntTol = 1e-4;   // :61
+constexpr double kFilterEpsilon = 1e-10; // :62 (zeta_max = -ln epsilon)
+constexpr int kFilterNIter = 20;         // :63
+constexpr double kLimiterEps = 1e-8;
+
+template <int NP>
+struct HookTables {
+    double P[NP][NP][NP];   // P[j]: 1D projection onto Legendre degrees <= j at the GLL nodes
+    double w[NP];           // GLL weights on [0,1]
+};
+
+template <int NP>
+struct HookArgs {
+    HookTables<NP> T;
+    int NE;
+    double gamma, gammaM1;
+    const int *nbr;
+    double *u;                     // state (filtered / limited in place)
+    double *emin;                  // per-element minimum nodal entropy
+    double *bound;                 // per-element entropy bound
+    int *flag;                     // per-element: the element needs the hook
+    unsigned long long *count;     // elements changed: [0] filter, [1] limiter
+};
+
+template <int DIM>
+__device__ __forceinline__ double hook_pressure(const double *s, double gammaM1) {
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 1; d <= DIM; d++) v2 += (s[d] / s[0]) * (s[d] / s[0]);
+    return gammaM1 * (s[DIM + 1] - 0.5 * s[0] * v2);
+}
+
+// rho ln(p rho^-gamma) for rho, p > 0, else +inf (BasicOperations.cpp:39-46)
+__device__ __forceinline__ double hook_entropy(double rho, double p, double gamma) {
+    return (rho > 0.0 && p > 0.0) ? rho * (log(p) - gamma * log(rho)) : __builtin_inf();
+}
+
+template <int DIM>
+__device__ __forceinline__ bool filter_violates(const double *s, double gamma, double gammaM1, double eb) {
+    const double p = hook_pressure<DIM>(s, gammaM1);
+    return s[0] < kFilterRhoMin || p < kFilterPMin || hook_entropy(s[0], p, gamma) < eb - kFilterEntTol;
+}
+
+template <int DIM, int NP>
+struct HookCfg {
+    static constexpr int NN = ipow(NP, DIM);
+    static constexpr int EPB = 256 / NN > 0 ? 256 / NN : 1;   // elements per 256-thread block
+};
+
+// Per-element pass over the nodes, one thread per node, EPB elements per block:
+//   MODE 0: emin[e] = minimum nodal entropy (GetEntropyConstraints, EntropyFilter.cpp:113-157);
+//   MODE 1: flag[e] = the element fails the filter's detection (:183-191);
+//   MODE 2: flag[e] = some node has rho < eps or p < eps (the limiter's trigger).
+template <int DIM, int NP, int MODE>
+__global__ void __launch_bounds__(256) k_hook_scan(const HookArgs<NP> A) {
+    using C = HookCfg<DIM, NP>;
+    constexpr int NN = C::NN, EPB = C::EPB;
+    __shared__ double sm[EPB * NN];
+    const int tid = threadIdx.x, eb = tid / NN, node = tid - eb * NN;
+    const int e = blockIdx.x * EPB + eb;
+    const size_t vs = (size_t)A.NE * NN;
+    const bool active = eb < EPB && e < A.NE;
+    if (active) {
+        double s[DIM + 2];
+#pragma unroll
+        for (int v = 0; v < DIM + 2; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
+        const double p = hook_pressure<DIM>(s, A.gammaM1);
+        if (MODE == 0)
+            sm[tid] = hook_entropy(s[0], p, A.gamma);
+        else if (MODE == 1)
+            sm[tid] = filter_violates<DIM>(s, A.gamma, A.gammaM1, A.bound[e]) ? 1.0 : 0.0;
+        else
+            sm[tid] = (s[0] < kLimiterEps || p < kLimiterEps) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (active && node == 0) {
+        if (MODE == 0) {
+            double m = 1.7976931348623157e308;   // el_min_e = DBL_MAX (EntropyFilter.cpp:120)
+            for (int i = 0; i < NN; i++) m = fmin(m, sm[eb * NN + i]);
+            A.emin[e] = m;
+        } else {
+            double f = 0.0;
+            for (int i = 0; i < NN; i++) f = fmax(f, sm[eb * NN + i]);
+            A.flag[e] = f > 0.0 ? 1 : 0;
+        }
+    }
+}
+
+// entropy bound: minimum over the element and its 2 dim face neighbours
+template <int DIM, int NP>
+__global__ void __launch_bounds__(256) k_filter_bound(const HookArgs<NP> A) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.NE) return;
+    double m = A.emin[e];
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; f++) m = fmin(m, A.emin[A.nbr[(size_t)e * 2 * DIM + f]]);
+    A.bound[e] = m;
+}
+
+template <int NP>
+__device__ __forceinline__ double filter_sigma(double zeta, int j) {
+    return j == 0 ? 1.0 : exp(-zeta * (double)(j * j));
+}
+
+// FilterSolution: one element per workgroup, one thread per node
+template <int DIM, int NP>
+__global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
+    constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
+    static_assert(NN <= 128, "one element per workgroup of at most 128 threads");
+    __shared__ double sU[NEQ][NN], sT[NEQ][NN];
+    __shared__ int s_first;
+    __shared__ double s_zeta;
+    const int e = blockIdx.x;
+    if (!A.flag[e]) return;   // detection (EntropyFilter.cpp:183-191) by k_hook_scan<.., 1>
+    const int i = threadIdx.x;
+    const bool node_ok = i < NN;
+    const size_t vs = (size_t)A.NE * NN;
+    const double eb = A.bound[e];
+    if (node_ok) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sU[v][i] = A.u[v * vs + (size_t)e * NN + i];
+    }
+    __syncthreads();
+    const int i0 = i % NP, i1 = (i / NP) % NP, i2 = i / (NP * NP);
+    // H_j (registers) from S_j = tensor projection of u
+    double H[NP][NEQ], prev[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) prev[v] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        double t[NEQ];
+        // direction 0
+        if (node_ok) {
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                double a = 0.0;
+#pragma unroll
+                for (int k = 0; k < NP; k++) a += A.T.P[j][i0][k] * sU[v][i - i0 + k];
+                t[v] = a;
+            }
+        }
+        __syncthreads();
+        if (node_ok)
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) sT[v][i] = t[v];
+        __syncthreads();
+        // direction 1
+        if (node_ok) {
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                double a = 0.0;
+#pragma unroll
+                for (int k = 0; k < NP; k++) a += A.T.P[j][i1][k] * sT[v][i + (k - i1) * NP];
+                t[v] = a;
+            }
+        }
+        if constexpr (DIM == 3) {
+            __syncthreads();
+            if (node_ok)
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) sT[v][i] = t[v];
+            __syncthreads();
+            if (node_ok) {
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int k = 0; k < NP; k++) a += A.T.P[j][i2][k] * sT[v][i + (k - i2) * NP * NP];
+                    t[v] = a;
+                }
+            }
+        }
+        __syncthreads();   // sT is rewritten by the next j
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            H[j][v] = t[v] - prev[v];
+            prev[v] = t[v];
+        }
+    }
+    // node-by-node scan (EntropyFilter.cpp:200-212) with ComputeRoot (:325-349)
+    const double zmax = -log(kFilterEpsilon);
+    double zeta = 0.0;
+    int start = 0;
+    for (int guard = 0; guard <= NN; guard++) {
+        if (i == 0) s_first = NN;
+        __syncthreads();
+        if (node_ok && i >= start) {
+            double st[NEQ];
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                double a = H[0][v];
+#pragma unroll
+                for (int j = 1; j < NP; j++) a += H[j][v] * filter_sigma<NP>(zeta, j);
+                st[v] = a;
+            }
+            if (filter_violates<DIM>(st, A.gamma, A.gammaM1, eb)) atomicMin(&s_first, i);
+        }
+        __syncthreads();
+        const int first = s_first;
+        if (first >= NN) break;
+        if (i == first) {
+            double z1 = zeta, z2 = zmax;
+            for (int it = 0; it < kFilterNIter; it++) {
+                const double z3 = 0.5 * (z1 + z2);
+                double st[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    double a = H[0][v];
+#pragma unroll
+                    for (int j = 1; j < NP; j++) a += H[j][v] * filter_sigma<NP>(z3, j);
+                    st[v] = a;
+                }
+                if (filter_violates<DIM>(st, A.gamma, A.gammaM1, eb))
+                    z1 = z3;
+                else
+                    z2 = z3;
+            }
+            s_zeta = z2;
+        }
+        __syncthreads();
+        zeta = s_zeta;
+        start = first + 1;
+        __syncthreads();   // s_first is reset at the top of the next pass
+    }
+    // reconstruction (EntropyFilter.cpp:214-238)
+    if (node_ok) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            double a = H[0][v];
+#pragma unroll
+            for (int j = 1; j < NP; j++) a += H[j][v] * filter_sigma<NP>(zeta, j);
+            A.u[v * vs + (size_t)e * NN + i] = a;
+        }
+    }
+    if (i == 0) atomicAdd(A.count, 1ull);
+}
+
+// Zhang-Shu limiter: one element per workgroup, one thread per node
+template <int DIM, int NP>
+__global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
+    constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
+    static_assert(NN <= 128, "one element per workgroup of at most 128 threads");
+    __shared__ double sR[NEQ][NN];
+    __shared__ double sMean[NEQ];
+    __shared__ double s_rmin, s_th2;
+    const int e = blockIdx.x;
+    if (!A.flag[e]) return;   // admissible element (k_hook_scan<.., 2>): untouched
+    const int i = threadIdx.x;
+    const bool node_ok = i < NN;
+    const size_t vs = (size_t)A.NE * NN;
+    const double eps = kLimiterEps;
+    double s[NEQ];
+    if (node_ok) {
+        const double w = A.T.w[i % NP] * A.T.w[(i / NP) % NP] * (DIM == 3 ? A.T.w[i / (NP * NP)] : 1.0);
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            s[v] = A.u[v * vs + (size_t)e * NN + i];
+            sR[v][i] = w * s[v];
+        }
+    }
+    __syncthreads();
+    if (i < NEQ) {
+        double m = 0.0;
+        for (int k = 0; k < NN; k++) m += sR[i][k];
+        sMean[i] = m;
+    }
+    __syncthreads();
+    if (node_ok) sR[0][i] = s[0];
+    __syncthreads();
+    if (i == 0) {
+        double r = sR[0][0];
+        for (int k = 1; k < NN; k++) r = fmin(r, sR[0][k]);
+        s_rmin = r;
+        s_th2 = 1.0;
+    }
+    __syncthreads();
+    double mean[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) mean[v] = sMean[v];
+    // density (theta_1)
+    if (s_rmin < eps && node_ok) {
+        const double th = fmax(0.0, fmin(1.0, (mean[0] - eps) / (mean[0] - s_rmin)));
+        s[0] = mean[0] + th * (s[0] - mean[0]);
+    }
+    // pressure (theta_2 = min_i t_i)
+    if (node_ok && hook_pressure<DIM>(s, A.gammaM1) < eps) {
+        double lo = 0.0, hi = 1.0, st[NEQ];
+        while (hi - lo > 0x1p-52) {
+            const double mid = 0.5 * (lo + hi);
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) st[v] = mean[v] + mid * (s[v] - mean[v]);
+            if (hook_pressure<DIM>(st, A.gammaM1) >= eps)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        // fmin over the workgroup through the ordered bit pattern of a non-negative double
+        atomicMin(reinterpret_cast<unsigned long long *>(&s_th2), (unsigned long long)__double_as_longlong(lo));
+    }
+    __syncthreads();
+    const double th2 = s_th2;
+    if (node_ok) {
+        if (th2 < 1.0)
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) s[v] = mean[v] + th2 * (s[v] - mean[v]);
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) A.u[v * vs + (size_t)e * NN + i] = s[v];
+    }
+    if (i == 0) atomicAdd(A.count + 1, 1ull);
+}
+
+}  // namespace prandtl

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/prandtl_amd.h"
+#include "dg_hooks.hpp"
 #include "dg_kernels.hpp"
 
 using namespace prandtl;
@@ -123,6 +124,11 @@ struct prandtl_op_s {
     double *d_k = nullptr;        // RK register
     double *d_ub = nullptr;       // RK alternate state buffer
     double *d_y2 = nullptr;       // SSP-RK3 second stage state
+    double *d_emin = nullptr;     // stage hooks: per-element minimum nodal entropy
+    double *d_fbound = nullptr;   // stage hooks: per-element entropy bound of the filter
+    int *d_hflag = nullptr;       // stage hooks: per-element trigger flags
+    unsigned long long *d_hcount = nullptr;   // stage hooks: elements filtered, limited
+    bool have_bounds = false;     // filter constraints computed
     double *d_vq = nullptr;       // adjoint: per-node quantities, (dim+6) x NE x n
     double *d_vlam = nullptr;     // adjoint: stage cotangent lambda
     unsigned long long *d_lam = nullptr;
@@ -358,6 +364,124 @@ int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     return fail(PRANDTL_ERR_UNSUPPORTED, "order out of range");
 }
 
+// ---------------------------------------------------------------------------
+// stage hooks: entropy filter and positivity limiter (dg_hooks.hpp)
+// ---------------------------------------------------------------------------
+// P[j] = V1[:, :j+1] V1^-1[:j+1, :], V1[i][k] = P_k(2 x_i - 1) on the GLL nodes x_i of [0,1]
+template <int NP>
+HookTables<NP> make_hook_tables() {
+    HookTables<NP> T;
+    double x[NP], w[NP], V[NP][NP], A[NP][NP], Vi[NP][NP];
+    gll_rule(NP, x, w);
+    for (int i = 0; i < NP; i++) {
+        const double z = 2.0 * x[i] - 1.0;
+        V[i][0] = 1.0;
+        if (NP > 1) V[i][1] = z;
+        for (int n = 1; n + 1 < NP; n++) V[i][n + 1] = ((2.0 * n + 1.0) * z * V[i][n] - n * V[i][n - 1]) / (n + 1.0);
+        T.w[i] = w[i];
+    }
+    // Gauss-Jordan inverse with partial pivoting
+    for (int i = 0; i < NP; i++)
+        for (int k = 0; k < NP; k++) A[i][k] = V[i][k], Vi[i][k] = (i == k) ? 1.0 : 0.0;
+    for (int c = 0; c < NP; c++) {
+        int piv = c;
+        for (int r = c + 1; r < NP; r++)
+            if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+        for (int k = 0; k < NP; k++) std::swap(A[c][k], A[piv][k]), std::swap(Vi[c][k], Vi[piv][k]);
+        const double d = 1.0 / A[c][c];
+        for (int k = 0; k < NP; k++) A[c][k] *= d, Vi[c][k] *= d;
+        for (int r = 0; r < NP; r++) {
+            if (r == c) continue;
+            const double f = A[r][c];
+            for (int k = 0; k < NP; k++) A[r][k] -= f * A[c][k], Vi[r][k] -= f * Vi[c][k];
+        }
+    }
+    for (int j = 0; j < NP; j++)
+        for (int i = 0; i < NP; i++)
+            for (int a = 0; a < NP; a++) {
+                double t = 0.0;
+                for (int k = 0; k <= j; k++) t += V[i][k] * Vi[k][a];
+                T.P[j][i][a] = t;
+            }
+    return T;
+}
+
+int ensure_hook_buffers(prandtl_op_s *op) {
+    if (!op->d_emin) {
+        HIP_TRY(hipMalloc(&op->d_emin, (size_t)op->NE * sizeof(double)));
+        HIP_TRY(hipMalloc(&op->d_fbound, (size_t)op->NE * sizeof(double)));
+        HIP_TRY(hipMalloc(&op->d_hflag, (size_t)op->NE * sizeof(int)));
+        HIP_TRY(hipMalloc(&op->d_hcount, 2 * sizeof(unsigned long long)));
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 2);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    return PRANDTL_OK;
+}
+
+enum HookOp { HOOK_CONSTRAINTS = 0, HOOK_FILTER = 1, HOOK_LIMITER = 2 };
+
+template <int DIM, int NP>
+int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
+    HookArgs<NP> a;
+    a.T = make_hook_tables<NP>();
+    a.NE = op->NE;
+    a.gamma = op->phys.gamma;
+    a.gammaM1 = op->phys.gamma - 1.0;
+    a.nbr = op->d_nbr;
+    a.u = u;
+    a.emin = op->d_emin;
+    a.bound = op->d_fbound;
+    a.flag = op->d_hflag;
+    a.count = op->d_hcount;
+    using C = HookCfg<DIM, NP>;
+    const int sblk = (op->NE + C::EPB - 1) / C::EPB;
+    constexpr int ebl = C::NN <= 64 ? 64 : 128;   // one element per workgroup
+    if (what == HOOK_CONSTRAINTS) {
+        hipLaunchKernelGGL((k_hook_scan<DIM, NP, 0>), dim3(sblk), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_filter_bound<DIM, NP>), dim3((op->NE + 255) / 256), dim3(256), 0, st, a);
+    } else if (what == HOOK_FILTER) {
+        hipLaunchKernelGGL((k_hook_scan<DIM, NP, 1>), dim3(sblk), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_filter<DIM, NP>), dim3(op->NE), dim3(ebl), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_hook_scan<DIM, NP, 2>), dim3(sblk), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_limiter<DIM, NP>), dim3(op->NE), dim3(ebl), 0, st, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return PRANDTL_OK;
+}
+
+int run_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
+    int rc = ensure_hook_buffers(op);
+    if (rc) return rc;
+    if (what == HOOK_FILTER && !op->have_bounds)
+        return fail(PRANDTL_ERR_ARG, "no filter constraints: call prandtl_filter_constraints first");
+    if (op->dim == 2) {
+        switch (op->np) {
+            case 2: return launch_hook<2, 2>(op, what, u, st);
+            case 3: return launch_hook<2, 3>(op, what, u, st);
+            case 4: return launch_hook<2, 4>(op, what, u, st);
+            case 5: return launch_hook<2, 5>(op, what, u, st);
+        }
+    } else {
+        switch (op->np) {
+            case 2: return launch_hook<3, 2>(op, what, u, st);
+            case 3: return launch_hook<3, 3>(op, what, u, st);
+            case 4: return launch_hook<3, 4>(op, what, u, st);
+            case 5: return launch_hook<3, 5>(op, what, u, st);
+        }
+    }
+    return fail(PRANDTL_ERR_UNSUPPORTED, "order out of range");
+}
+
+// the hooks the stepper applies after a stage: filter, then limiter
+int stage_hooks(prandtl_op_s *op, double *u, hipStream_t st) {
+    int rc = PRANDTL_OK;
+    if (op->opts.stage_hooks & PRANDTL_HOOK_FILTER) rc = run_hook(op, HOOK_FILTER, u, st);
+    if (!rc && (op->opts.stage_hooks & PRANDTL_HOOK_LIMITER)) rc = run_hook(op, HOOK_LIMITER, u, st);
+    return rc;
+}
+
 // Carpenter & Kennedy (1994) 2N-storage RK4(3)5 coefficients.
 const double CK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
                         -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
@@ -375,14 +499,22 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
         HIP_TRY(hipGetLastError());
     }
     double *cur = u;
+    const bool hooks = op->opts.stage_hooks != 0;
     for (int s = 0; s < nsteps; s++) {
         const bool chk = nan_check > 0 && ((s + 1) % nan_check == 0 || s + 1 == nsteps);
+        if (op->opts.stage_hooks & PRANDTL_HOOK_FILTER) {
+            // GetFilterConstraints on the state at the start of the step
+            rc = run_hook(op, HOOK_CONSTRAINTS, cur, st);
+            if (rc) return rc;
+            op->have_bounds = true;
+        }
         if (scheme == PRANDTL_LSRK45) {
             for (int stg = 0; stg < 5; stg++) {
                 double *nxt = (cur == u) ? op->d_ub : u;
                 StageSpec S{EPI_LSRK, cur, nxt, op->d_k, nullptr, CK_A[stg], CK_B[stg], dt, nullptr, false,
                             chk && stg == 4};
                 rc = run_stage(op, S, st);
+                if (!rc && hooks) rc = stage_hooks(op, nxt, st);
                 if (rc) return rc;
                 cur = nxt;
             }
@@ -392,12 +524,15 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
             double *y = (cur == u) ? op->d_ub : u;
             StageSpec S1{EPI_SSP, cur, y, nullptr, cur, 0.0, 1.0, dt, nullptr, false, false};
             rc = run_stage(op, S1, st);
+            if (!rc && hooks) rc = stage_hooks(op, y, st);
             if (rc) return rc;
             StageSpec S2{EPI_SSP, y, op->d_y2, nullptr, cur, 3.0 / 4.0, 1.0 / 4.0, dt, nullptr, false, false};
             rc = run_stage(op, S2, st);
+            if (!rc && hooks) rc = stage_hooks(op, op->d_y2, st);
             if (rc) return rc;
             StageSpec S3{EPI_SSP, op->d_y2, cur, nullptr, cur, 1.0 / 3.0, 2.0 / 3.0, dt, nullptr, false, chk};
             rc = run_stage(op, S3, st);
+            if (!rc && hooks) rc = stage_hooks(op, cur, st);
             if (rc) return rc;
         }
         if (chk) {
@@ -468,6 +603,7 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     if (opts) o = *opts;
     if (o.surface_flux < 0 || o.surface_flux > 2) return fail(PRANDTL_ERR_ARG, "unknown surface flux");
     if (o.precision != PRANDTL_FP64 && o.precision != PRANDTL_MIXED) return fail(PRANDTL_ERR_ARG, "unknown precision");
+    if (o.stage_hooks & ~(PRANDTL_HOOK_FILTER | PRANDTL_HOOK_LIMITER)) return fail(PRANDTL_ERR_ARG, "unknown stage hook");
     if (phys->viscous && (o.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
         return fail(PRANDTL_ERR_UNSUPPORTED, "bug_compat A1/A2 are reproduced by the CPU oracle only");
     if (phys->viscous && (!(phys->Pr > 0.0) || phys->mu < 0.0)) return fail(PRANDTL_ERR_ARG, "bad Pr or mu");
@@ -537,6 +673,10 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_y2);
     (void)hipFree(op->d_vq);
     (void)hipFree(op->d_vlam);
+    (void)hipFree(op->d_emin);
+    (void)hipFree(op->d_fbound);
+    (void)hipFree(op->d_hflag);
+    (void)hipFree(op->d_hcount);
     (void)hipFree(op->d_lam);
     (void)hipFree(op->d_nan);
     for (double *p : op->tape_ckpt) (void)hipFree(p);
@@ -629,6 +769,39 @@ int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, 
     return PRANDTL_OK;
 }
 
+int prandtl_filter_constraints(prandtl_op op, const double *u, prandtl_stream stream) {
+    if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
+    int rc = run_hook(op, HOOK_CONSTRAINTS, const_cast<double *>(u), (hipStream_t)stream);   // reads u only
+    if (rc) return rc;
+    op->have_bounds = true;
+    return PRANDTL_OK;
+}
+
+int prandtl_filter_apply(prandtl_op op, double *u, prandtl_stream stream) {
+    if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
+    return run_hook(op, HOOK_FILTER, u, (hipStream_t)stream);
+}
+
+int prandtl_limiter_apply(prandtl_op op, double *u, prandtl_stream stream) {
+    if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
+    return run_hook(op, HOOK_LIMITER, u, (hipStream_t)stream);
+}
+
+int prandtl_hook_counts(prandtl_op op, long long *filtered, long long *limited) {
+    if (!op) return fail(PRANDTL_ERR_ARG, "null argument");
+    int rc = ensure_hook_buffers(op);
+    if (rc) return rc;
+    unsigned long long c[2] = {0, 0};
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(c, op->d_hcount, sizeof(c), hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 2);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    if (filtered) *filtered = (long long)c[0];
+    if (limited) *limited = (long long)c[1];
+    return PRANDTL_OK;
+}
+
 int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *out, prandtl_stream stream) {
     if (!op || !u || !lam || !out) return fail(PRANDTL_ERR_ARG, "null argument");
     if (out == u || out == lam) return fail(PRANDTL_ERR_ARG, "out must not alias u or lam");
@@ -640,6 +813,8 @@ int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double 
                         prandtl_stream stream) {
     if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
     if (scheme != PRANDTL_LSRK45) return fail(PRANDTL_ERR_UNSUPPORTED, "the taped run supports LSRK4(5)");
+    if (op->opts.stage_hooks)
+        return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the filter or limiter hooks");
     if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
     hipStream_t st = (hipStream_t)stream;
     const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;

@@ -35,7 +35,8 @@ def _create(m, **kw):
     md = capi.MeshDesc(m.dim, m.order, m.num_elements, nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                        geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     ph = capi.Physics(kw.get("gamma", 1.4), kw.get("Pr", 0.72), kw.get("mu", 0.0), 287.0, kw.get("viscous", 0), 0)
-    op = capi.Opts(kw.get("surface_flux", 0), kw.get("precision", 0), kw.get("bug_compat", 0), 0)
+    op = capi.Opts(kw.get("surface_flux", 0), kw.get("precision", 0), kw.get("bug_compat", 0),
+                   kw.get("stage_hooks", 0))
     h = ctypes.c_void_p()
     rc = lib.prandtl_op_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
     return rc, lib.prandtl_last_error().decode(), h
@@ -67,7 +68,8 @@ def test_rejects_nonconforming_faces_and_bad_jacobian():
 
 
 @pytest.mark.parametrize("kw,code", [({"gamma": 1.0}, -1), ({"surface_flux": 9}, -1), ({"precision": 7}, -1),
-                                     ({"viscous": 1, "bug_compat": 1}, -2), ({"viscous": 1, "Pr": 0.0}, -1)])
+                                     ({"viscous": 1, "bug_compat": 1}, -2), ({"viscous": 1, "Pr": 0.0}, -1),
+                                     ({"stage_hooks": 4}, -1)])
 def test_rejects_bad_options(kw, code):
     m = M.periodic_lattice(2, 3, (2, 2), lengths=(1, 1))
     rc, msg, _ = _create(m, **kw)
