@@ -176,6 +176,15 @@ int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream);
  * in place of the reference's run timer (src/Simulation/Simulation.cpp:910-914,1162-1171). */
 int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, double *ms, prandtl_stream stream);
 
+/* Device memory and copies for host programs that do not use HIP themselves (the C++ mirror
+ * in prandtl_amd.hpp).  No counterpart in the reference, whose vectors live in host memory.
+ * prandtl_memcpy is synchronous; kind is one of PRANDTL_COPY_*. */
+enum { PRANDTL_COPY_H2D = 0, PRANDTL_COPY_D2H = 1, PRANDTL_COPY_D2D = 2 };
+int prandtl_device_alloc(size_t bytes, void **ptr);
+void prandtl_device_free(void *ptr);
+int prandtl_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int prandtl_stream_synchronize(prandtl_stream stream);
+
 /* Description of the last error on this thread. */
 const char *prandtl_last_error(void);
 

@@ -1,0 +1,275 @@
+// prandtl_amd.hpp -- C++ host-side mirror of the reference's operator, solver, filter and
+// limiter classes over the C ABI of prandtl_amd.h.  Header-only; a host program compiles with
+// any C++17 compiler and links libprandtl_amd.so, with no HIP headers of its own.
+//
+// Reference interfaces mirrored (paths relative to the Prandtl repository root):
+//   mfem::TimeDependentOperator / DGSEMOperator   src/Operators/DGSEMOperator.hpp:15-115
+//   NumericalFlux (surface flux plug-in)          src/RiemannSolvers/NumericalFlux.hpp:21-29
+//   mfem::ODESolver Init / Step                   src/Simulation/Simulation.cpp:177-206, 807, 1010
+//   Filter / EntropyFilter                        src/Filters/Filter.hpp:11-30, EntropyFilter.hpp
+//   Limiter                                       src/Limiters/Limiter.hpp:10-16
+// The virtual per-point flux calls of the reference cannot cross to the device; the surface
+// flux is chosen at construction (SurfaceFlux) and compiled into the kernels.
+//
+// Errors: the reference aborts through mfem_error; here a failing call throws prandtl::Error
+// carrying prandtl_last_error(), which a caller that wants the reference's behaviour lets
+// escape.  Device vectors live in HBM in the reference's byNODES layout.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "prandtl_amd.h"
+
+namespace prandtl {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string &msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+inline void check(int rc) {
+    if (rc != PRANDTL_OK) throw Error(rc, prandtl_last_error());
+}
+
+// A device-resident FP64 vector (the role of mfem::Vector with its device memory).
+class Vector {
+public:
+    Vector() = default;
+    explicit Vector(size_t n) : n_(n) {
+        void *p = nullptr;
+        check(prandtl_device_alloc(n * sizeof(double), &p));
+        d_ = static_cast<double *>(p);
+    }
+    Vector(const Vector &) = delete;
+    Vector &operator=(const Vector &) = delete;
+    Vector(Vector &&o) noexcept : d_(o.d_), n_(o.n_) { o.d_ = nullptr, o.n_ = 0; }
+    Vector &operator=(Vector &&o) noexcept {
+        std::swap(d_, o.d_);
+        std::swap(n_, o.n_);
+        return *this;
+    }
+    ~Vector() { prandtl_device_free(d_); }
+
+    size_t Size() const { return n_; }
+    const double *Read() const { return d_; }
+    double *Write() { return d_; }
+    double *ReadWrite() { return d_; }
+    void CopyFromHost(const double *h) { check(prandtl_memcpy(d_, h, n_ * sizeof(double), PRANDTL_COPY_H2D)); }
+    void CopyToHost(double *h) const { check(prandtl_memcpy(h, d_, n_ * sizeof(double), PRANDTL_COPY_D2H)); }
+    std::vector<double> HostCopy() const {
+        std::vector<double> h(n_);
+        CopyToHost(h.data());
+        return h;
+    }
+
+private:
+    double *d_ = nullptr;
+    size_t n_ = 0;
+};
+
+// The surface flux plug-in (NumericalFlux::ComputeFaceFlux).  The two-point volume flux is
+// the reference's Chandrashekar flux (ChandrashekarFlux::ComputeVolumeFlux) in every case.
+enum class SurfaceFlux { Chandrashekar = PRANDTL_SURF_CHANDRASHEKAR, EC = PRANDTL_SURF_EC, Rusanov = PRANDTL_SURF_RUSANOV };
+enum class Precision { FP64 = PRANDTL_FP64, Mixed = PRANDTL_MIXED };
+
+// Conforming affine mesh: per element its 2 dim face neighbours and (det J, adj J).
+struct Mesh {
+    int dim = 0, order = 0, num_elements = 0;
+    std::vector<int> neighbors;     // [e][2 d + s]
+    std::vector<double> geometry;   // [e][1 + dim^2]
+
+    // Periodic lattice of n[0] x n[1] (x n[2]) axis-aligned boxes of the domain
+    // [origin, origin + length] (the generated meshes of the benchmark cases).
+    static Mesh PeriodicLattice(int dim, int order, const std::array<int, 3> &n, const std::array<double, 3> &length) {
+        Mesh m;
+        m.dim = dim;
+        m.order = order;
+        const int nx = n[0], ny = n[1], nz = dim == 3 ? n[2] : 1;
+        m.num_elements = nx * ny * nz;
+        m.neighbors.resize((size_t)m.num_elements * 2 * dim);
+        m.geometry.assign((size_t)m.num_elements * (1 + dim * dim), 0.0);
+        double h[3] = {length[0] / nx, length[1] / ny, dim == 3 ? length[2] / nz : 1.0};
+        double det = 1.0;
+        for (int d = 0; d < dim; d++) det *= h[d];
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    const int e = i + nx * (j + ny * k);
+                    const int c[3] = {i, j, k}, nn[3] = {nx, ny, nz};
+                    for (int d = 0; d < dim; d++)
+                        for (int s = 0; s < 2; s++) {
+                            int cc[3] = {c[0], c[1], c[2]};
+                            cc[d] = (cc[d] + (s ? 1 : nn[d] - 1)) % nn[d];
+                            m.neighbors[(size_t)e * 2 * dim + 2 * d + s] = cc[0] + nx * (cc[1] + ny * cc[2]);
+                        }
+                    double *g = &m.geometry[(size_t)e * (1 + dim * dim)];
+                    g[0] = det;
+                    for (int d = 0; d < dim; d++) g[1 + d * dim + d] = det / h[d];   // adj of diag(h)
+                }
+        return m;
+    }
+    int NodesPerElement() const {
+        int n = 1;
+        for (int d = 0; d < dim; d++) n *= order + 1;
+        return n;
+    }
+    size_t StateSize() const { return (size_t)(dim + 2) * num_elements * NodesPerElement(); }
+};
+
+struct Physics {
+    double gamma = 1.4, Pr = 0.72, mu = 0.0, R_gas = 287.0;
+    bool viscous = false, sutherland = false;
+};
+
+struct Options {
+    SurfaceFlux surface_flux = SurfaceFlux::Chandrashekar;
+    Precision precision = Precision::FP64;
+    int bug_compat = 0;     // PRANDTL_BUG_* bitmask
+    int stage_hooks = 0;    // PRANDTL_HOOK_*: filter / limiter inside the fused steppers
+};
+
+// mfem::TimeDependentOperator's surface as the reference uses it.
+class TimeDependentOperator {
+public:
+    explicit TimeDependentOperator(int height) : height_(height) {}
+    virtual ~TimeDependentOperator() = default;
+    virtual void Mult(const Vector &u, Vector &dudt) const = 0;
+    virtual void SetTime(double t) { t_ = t; }
+    double GetTime() const { return t_; }
+    double &GetTimeRef() { return t_; }
+    int Height() const { return height_; }
+
+protected:
+    int height_;
+    double t_ = 0.0;
+};
+
+// DGSEMOperator -- DGSEMOperator.hpp:87-115 over prandtl_op_*.
+class DGSEMOperator : public TimeDependentOperator {
+public:
+    DGSEMOperator(const Mesh &mesh, const Physics &phys, const Options &opts = {})
+        : TimeDependentOperator((int)mesh.StateSize()) {
+        prandtl_mesh_desc md{mesh.dim, mesh.order, mesh.num_elements, mesh.neighbors.data(), mesh.geometry.data()};
+        prandtl_physics ph{phys.gamma, phys.Pr, phys.mu, phys.R_gas, phys.viscous ? 1 : 0, phys.sutherland ? 1 : 0};
+        prandtl_opts o{(int)opts.surface_flux, (int)opts.precision, opts.bug_compat, opts.stage_hooks};
+        check(prandtl_op_create(&md, &ph, &o, &op_));
+    }
+    DGSEMOperator(const DGSEMOperator &) = delete;
+    DGSEMOperator &operator=(const DGSEMOperator &) = delete;
+    ~DGSEMOperator() override { prandtl_op_destroy(op_); }
+
+    // dudt = L(u), fully overwritten (DGSEMOperator.cpp:537-591)
+    void Mult(const Vector &u, Vector &dudt) const override {
+        check(prandtl_op_mult(op_, u.Read(), dudt.Write(), stream_));
+    }
+    // max(|V| + c) of the last Mult (DGSEMOperator.hpp:106-109)
+    double GetMaxCharSpeed() const {
+        double l = 0.0;
+        check(prandtl_op_get_max_char_speed(op_, &l));
+        return l;
+    }
+    // the same quantity from a state alone
+    double MaxCharSpeed(const Vector &u) const {
+        double l = 0.0;
+        check(prandtl_op_max_char_speed(op_, u.Read(), &l, stream_));
+        return l;
+    }
+    void SetStream(prandtl_stream s) { stream_ = s; }
+    prandtl_stream Stream() const { return stream_; }
+    prandtl_op Handle() const { return op_; }
+
+private:
+    prandtl_op op_ = nullptr;
+    prandtl_stream stream_ = nullptr;
+};
+
+// mfem::ODESolver's Init / Step.  The steppers run the fused stage kernels of the library, so
+// they step a DGSEMOperator (Init throws for any other operator).
+class ODESolver {
+public:
+    virtual ~ODESolver() = default;
+    virtual void Init(TimeDependentOperator &f) {
+        op_ = dynamic_cast<DGSEMOperator *>(&f);
+        if (!op_) throw Error(PRANDTL_ERR_UNSUPPORTED, "the fused steppers advance a prandtl::DGSEMOperator");
+    }
+    // x advanced by one step in place; t += dt (dt unchanged: fixed-step solvers)
+    virtual void Step(Vector &x, double &t, double &dt) {
+        if (!op_) throw Error(PRANDTL_ERR_ARG, "ODESolver::Step before Init");
+        check(prandtl_rk_step(op_->Handle(), scheme(), x.ReadWrite(), &t, dt, op_->Stream()));
+        op_->SetTime(t);
+    }
+    // nsteps steps at once (the time loop without output), NaN scan every nan_check steps
+    void Run(Vector &x, double &t, double dt, int nsteps, int nan_check = 0) {
+        if (!op_) throw Error(PRANDTL_ERR_ARG, "ODESolver::Run before Init");
+        check(prandtl_rk_run(op_->Handle(), scheme(), x.ReadWrite(), &t, dt, nsteps, nan_check, op_->Stream()));
+        op_->SetTime(t);
+    }
+
+protected:
+    virtual int scheme() const = 0;
+    DGSEMOperator *op_ = nullptr;
+};
+
+// MFEM RK3SSPSolver (Shu-Osher form)
+class SSPRK3Solver : public ODESolver {
+protected:
+    int scheme() const override { return PRANDTL_SSPRK3; }
+};
+
+// Carpenter-Kennedy 2N-storage RK4(3)5
+class LSRK45Solver : public ODESolver {
+protected:
+    int scheme() const override { return PRANDTL_LSRK45; }
+};
+
+// Filter -- Filter.hpp:11-30
+class Filter {
+public:
+    virtual ~Filter() = default;
+    virtual void FilterSolution(Vector &sol) = 0;
+    virtual void GetFilterConstraints(const Vector &x) = 0;
+};
+
+// EntropyFilter -- EntropyFilter.hpp / EntropyFilter.cpp, on the operator's mesh
+class EntropyFilter : public Filter {
+public:
+    explicit EntropyFilter(const DGSEMOperator &op) : op_(op) {}
+    void GetFilterConstraints(const Vector &x) override {
+        check(prandtl_filter_constraints(op_.Handle(), x.Read(), op_.Stream()));
+    }
+    void FilterSolution(Vector &sol) override { check(prandtl_filter_apply(op_.Handle(), sol.ReadWrite(), op_.Stream())); }
+
+private:
+    const DGSEMOperator &op_;
+};
+
+// Limiter -- Limiter.hpp:10-16
+class Limiter {
+public:
+    virtual ~Limiter() = default;
+    virtual void LimitSolution(Vector &sol) = 0;
+};
+
+// Zhang-Shu positivity limiter (the reference has only the interface)
+class PositivityLimiter : public Limiter {
+public:
+    explicit PositivityLimiter(const DGSEMOperator &op) : op_(op) {}
+    void LimitSolution(Vector &sol) override { check(prandtl_limiter_apply(op_.Handle(), sol.ReadWrite(), op_.Stream())); }
+
+private:
+    const DGSEMOperator &op_;
+};
+
+// Completes the work queued on the operator's stream.
+inline void Synchronize(const DGSEMOperator &op) { check(prandtl_stream_synchronize(op.Stream())); }
+
+}  // namespace prandtl

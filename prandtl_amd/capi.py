@@ -30,6 +30,7 @@ EXPORTED = [
     "prandtl_op_get_max_char_speed", "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run",
     "prandtl_op_profile", "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_last_error",
     "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
+    "prandtl_device_alloc", "prandtl_device_free", "prandtl_memcpy", "prandtl_stream_synchronize",
 ]
 
 

@@ -802,6 +802,36 @@ int prandtl_hook_counts(prandtl_op op, long long *filtered, long long *limited) 
     return PRANDTL_OK;
 }
 
+int prandtl_device_alloc(size_t bytes, void **ptr) {
+    if (!ptr) return fail(PRANDTL_ERR_ARG, "null argument");
+    *ptr = nullptr;
+    if (bytes == 0) return PRANDTL_OK;
+    HIP_TRY(hipMalloc(ptr, bytes));
+    return PRANDTL_OK;
+}
+
+void prandtl_device_free(void *ptr) {
+    if (ptr) (void)hipFree(ptr);
+}
+
+int prandtl_memcpy(void *dst, const void *src, size_t bytes, int kind) {
+    if ((!dst || !src) && bytes) return fail(PRANDTL_ERR_ARG, "null argument");
+    hipMemcpyKind k;
+    switch (kind) {
+        case PRANDTL_COPY_H2D: k = hipMemcpyHostToDevice; break;
+        case PRANDTL_COPY_D2H: k = hipMemcpyDeviceToHost; break;
+        case PRANDTL_COPY_D2D: k = hipMemcpyDeviceToDevice; break;
+        default: return fail(PRANDTL_ERR_ARG, "unknown copy kind");
+    }
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, k));
+    return PRANDTL_OK;
+}
+
+int prandtl_stream_synchronize(prandtl_stream stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return PRANDTL_OK;
+}
+
 int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *out, prandtl_stream stream) {
     if (!op || !u || !lam || !out) return fail(PRANDTL_ERR_ARG, "null argument");
     if (out == u || out == lam) return fail(PRANDTL_ERR_ARG, "out must not alias u or lam");

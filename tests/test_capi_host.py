@@ -88,3 +88,13 @@ def test_null_arguments():
     assert lib.prandtl_rk_run(None, 0, None, None, 0.1, 1, 0, None) == -1
     assert lib.prandtl_op_size(None) == 0
     lib.prandtl_op_destroy(None)
+
+
+def test_cpp_mirror_compiles_and_links(tmp_path):
+    """The C++ mirror header compiles with g++ alone and the example links against the library."""
+    import subprocess
+
+    from tests.test_gpu_host_mirror import build_example
+
+    exe = build_example(str(tmp_path))
+    assert os.path.exists(exe)
