@@ -533,6 +533,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
     __shared__ R sW[NEQ][EPB * NN];
     __shared__ R sU[NW > NFV ? NW : NFV];  // neighbour w at face points, then F~^d
     __shared__ double sWref[EPB][NEQ];
+    __shared__ LogRef sLR[EPB];           // reference state of the logarithms (the element's node 0)
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -548,7 +549,11 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
     if (active) {
 #pragma unroll
         for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
-        conserv_to_entropy<DIM>(s, A.P, w);
+        if (node == 0) sLR[eb] = make_log_ref<DIM>(s, A.P);
+    }
+    __syncthreads();
+    if (active) {
+        conserv_to_entropy_ref<DIM>(s, A.P, sLR[eb], w);
         if (MIX && node == 0) {
 #pragma unroll
             for (int v = 0; v < NEQ; v++) sWref[eb][v] = w[v];
@@ -571,7 +576,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
         double sn[NEQ], wn[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
-        conserv_to_entropy<DIM>(sn, A.P, wn);
+        conserv_to_entropy_ref<DIM>(sn, A.P, sLR[fe], wn);
 #pragma unroll
         for (int v = 0; v < NEQ; v++) sU[(v * EPB + fe) * C::FITEMS + i] = (R)(wn[v] - (MIX ? sWref[fe][v] : 0.0));
     }

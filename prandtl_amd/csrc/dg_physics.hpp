@@ -308,21 +308,60 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
     }
 }
 
-// Conserv2Entropy -- ref: BasicOperations.cpp:81-104.  s = ln(p rho^-gamma) is evaluated as
-// ln p - gamma ln rho (same value to rounding, two logs instead of pow + log).
+// ln x near a reference x0 whose logarithm lx0 and reciprocal ix0 are known:
+// ln x = ln x0 + 2 artanh(t / (2 + t)), t = x/x0 - 1, for |t| <= 1/32 by the artanh series to
+// z^11 (|z| < 1/63, so the first omitted term z^13/13 is below 2^-80 relative to ln x); the
+// library log otherwise.  x - x0 is exact there (Sterbenz), so the result is within about
+// half an ulp of ln x as the library log is.
+__device__ __forceinline__ double log_near(double x, double x0, double lx0, double ix0) {
+    const double t = (x - x0) * ix0;
+    if (fabs(t) <= 0.03125) {
+        const double z = t * rcp(2.0 + t);
+        const double z2 = z * z;
+        const double q = 1.0 / 3.0 + z2 * (1.0 / 5.0 + z2 * (1.0 / 7.0 + z2 * (1.0 / 9.0 + z2 * (1.0 / 11.0))));
+        return lx0 + 2.0 * fma(z * z2, q, z);
+    }
+    return log(x);
+}
+
+// Reference state of an element for log_near: p0, rho0 with their logarithms and reciprocals.
+struct LogRef {
+    double p0, lp0, ip0, r0, lr0, ir0;
+};
+
 template <int DIM>
-__device__ __forceinline__ void conserv_to_entropy(const double *s, const PhysConst &P, double *w) {
+__device__ __forceinline__ LogRef make_log_ref(const double *s, const PhysConst &P) {
+    LogRef R;
+    R.r0 = s[0];
+    R.ir0 = rcp(R.r0);
+    double v_sq = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) v_sq += s[1 + d] * s[1 + d];
+    R.p0 = P.gammaM1 * (s[DIM + 1] - 0.5 * v_sq * R.ir0);
+    R.ip0 = rcp(R.p0);
+    R.lp0 = log(R.p0);
+    R.lr0 = log(R.r0);
+    return R;
+}
+
+// Conserv2Entropy -- ref: BasicOperations.cpp:81-104.  s = ln(p rho^-gamma) is evaluated as
+// ln p - gamma ln rho (the same value to rounding), each logarithm near the element's
+// reference state (log_near), the divisions as reciprocals.
+template <int DIM>
+__device__ __forceinline__ void conserv_to_entropy_ref(const double *s, const PhysConst &P, const LogRef &R,
+                                                       double *w) {
     const double rho = s[0];
+    const double irho = rcp(rho);
     double vel[DIM], v_sq = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        vel[d] = s[1 + d] / rho;
+        vel[d] = s[1 + d] * irho;
         v_sq += vel[d] * vel[d];
     }
     v_sq = 0.5 * v_sq;
     const double p = P.gammaM1 * (s[DIM + 1] - rho * v_sq);
-    const double beta = rho / p;
-    const double ent = log(p) - P.gamma * log(rho);
+    const double beta = rho * rcp(p);
+    const double ent = log_near(p, R.p0, R.lp0, R.ip0) - P.gamma * log_near(rho, R.r0, R.lr0, R.ir0);
     w[0] = (P.gamma - ent) * P.gammaM1Inverse - beta * v_sq;
 #pragma unroll
     for (int d = 0; d < DIM; d++) w[1 + d] = beta * vel[d];
