@@ -274,5 +274,161 @@ __host__ __device__ inline void face_flux_vjp(int kind, const double *s1, const 
     prim_back<DIM>(b, gm1, gb, s2b);
 }
 
+// ---------------------------------------------------------------------------
+// Navier-Stokes pointwise adjoints (constant viscosity).  Forward formulas: conserv_to_entropy
+// (BasicOperations.cpp:81-104), entgrad_to_primgrad (:126-170), viscous_flux
+// (NavierStokesFlux.cpp:12-90), as in dg_physics.hpp.  Each accumulates s_b (and g_b).
+// ---------------------------------------------------------------------------
+
+// w = (dU/du)^-1: w0 = (gamma - s)/(gamma-1) - beta q, w_d = beta v_d, w_E = -beta,
+// s = ln p - gamma ln rho, beta = rho/p, q = |v|^2/2.  Given w_b, accumulate s_b.
+template <int DIM>
+__host__ __device__ inline void entropy_vars_vjp(const double *s, double gamma, const double *w_b, double *s_b) {
+    const double gm1 = gamma - 1.0, rho = s[0], irho = inv(rho);
+    double v[DIM], q = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        v[d] = s[1 + d] * irho;
+        q += v[d] * v[d];
+    }
+    q *= 0.5;
+    const double p = gm1 * (s[DIM + 1] - rho * q), ip = inv(p);
+    const double beta = rho * ip;
+    double beta_b = -w_b[DIM + 1] - w_b[0] * q, v_b[DIM];
+    for (int d = 0; d < DIM; d++) {
+        beta_b += w_b[1 + d] * v[d];
+        v_b[d] = w_b[1 + d] * beta;
+    }
+    double q_b = -w_b[0] * beta;
+    const double ent_b = -w_b[0] / gm1;
+    double p_b = ent_b * ip - beta_b * beta * ip;
+    double rho_b = -gamma * ent_b * irho + beta_b * ip;
+    // p = (gamma-1)(E - rho q)
+    s_b[DIM + 1] += gm1 * p_b;
+    rho_b += -gm1 * p_b * q;
+    q_b += -gm1 * p_b * rho;
+    for (int d = 0; d < DIM; d++) {
+        v_b[d] += q_b * v[d];
+        s_b[1 + d] += v_b[d] * irho;
+        rho_b += -v_b[d] * v[d] * irho;
+    }
+    s_b[0] += rho_b;
+}
+
+// EntropyGrad2PrimGrad of one direction: gp = T(s, g) (g: gradient of w, gp: gradient of
+// (rho, v, p)); given gp_b, accumulate s_b and g_b.
+template <int DIM>
+__host__ __device__ inline void entgrad_vjp(const double *s, const double *g, double gamma, const double *gp_b,
+                                            double *s_b, double *g_b) {
+    const double gm1 = gamma - 1.0, gm1inv = 1.0 / gm1;
+    const double rho = s[0], irho = inv(rho);
+    double vel[DIM], vsq = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        vel[d] = s[1 + d] * irho;
+        vsq += vel[d] * vel[d];
+    }
+    vsq *= 0.5;   // v_sq of the forward
+    const double p = gm1 * (s[DIM + 1] - rho * vsq);
+    const double KE = vsq * rho, ge = g[DIM + 1], por = p * irho, ipor = inv(por);
+    double outd[DIM], mdot = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        outd[d] = por * (g[1 + d] + vel[d] * ge);
+        mdot += s[1 + d] * outd[d];
+    }
+    const double out0 = rho * g[0] - ge * (KE - p * gm1inv) + ipor * mdot;
+    // reverse
+    const double outE_b = gp_b[DIM + 1];
+    double por_b = outE_b * (out0 + p * ge);
+    const double out0_b = gp_b[0] + outE_b * por;
+    double p_b = outE_b * por * ge;
+    double ge_b = outE_b * por * p;
+    double rho_b = out0_b * g[0];
+    g_b[0] += out0_b * rho;
+    ge_b += -out0_b * (KE - p * gm1inv);
+    const double KE_b = -out0_b * ge;
+    p_b += out0_b * ge * gm1inv;
+    const double ipor_b = out0_b * mdot;
+    const double mdot_b = out0_b * ipor;
+    double vel_b[DIM], m_b[DIM];
+    for (int d = 0; d < DIM; d++) {
+        m_b[d] = mdot_b * outd[d];
+        const double od_b = gp_b[1 + d] + mdot_b * s[1 + d];
+        por_b += od_b * (g[1 + d] + vel[d] * ge);
+        g_b[1 + d] += od_b * por;
+        vel_b[d] = od_b * por * ge;
+        ge_b += od_b * por * vel[d];
+    }
+    por_b += -ipor_b * ipor * ipor;
+    p_b += por_b * irho;
+    rho_b += -por_b * p * irho * irho;
+    double vsq_b = KE_b * rho;
+    rho_b += KE_b * vsq;
+    s_b[DIM + 1] += gm1 * p_b;
+    rho_b += -gm1 * p_b * vsq;
+    vsq_b += -gm1 * p_b * rho;
+    g_b[DIM + 1] += ge_b;
+    for (int d = 0; d < DIM; d++) {
+        vel_b[d] += vsq_b * vel[d];
+        m_b[d] += vel_b[d] * irho;
+        rho_b += -vel_b[d] * vel[d] * irho;
+        s_b[1 + d] += m_b[d];
+    }
+    s_b[0] += rho_b;
+}
+
+// NavierStokesFlux::ComputeViscousFlux with constant mu: flux[r][c] (r < DIM momentum, r = DIM
+// energy) from the state and the primitive gradients g[c][(rho, v, p)].  Given flux_b,
+// accumulate s_b and g_b.
+template <int DIM>
+__host__ __device__ inline void viscous_flux_vjp(const double *s, const double (*g)[DIM + 2], double gamma, double mu,
+                                                 double Pr_inv, double mu_bulk, const double (*flux_b)[DIM],
+                                                 double *s_b, double (*g_b)[DIM + 2]) {
+    const double gm1 = gamma - 1.0, gm1inv = 1.0 / gm1;
+    const double rho = s[0], irho = inv(rho);
+    double vel[DIM], vsq = 0.0;
+    for (int d = 0; d < DIM; d++) {
+        vel[d] = s[1 + d] * irho;
+        vsq += vel[d] * vel[d];
+    }
+    const double p = gm1 * (s[DIM + 1] - 0.5 * rho * vsq);
+    const double lambda = mu * gamma * Pr_inv, por = p * irho;
+    double div = g[0][1];
+    for (int d = 1; d < DIM; d++) div += g[d][1 + d];
+    double vel_b[DIM], div_b = 0.0, por_b = 0.0, irho_b = 0.0;
+    for (int d = 0; d < DIM; d++) vel_b[d] = 0.0;
+    for (int c = 0; c < DIM; c++) {
+        const double e_b = flux_b[DIM][c];
+        const double cv_b = lambda * e_b;
+        for (int r = 0; r < DIM; r++) {
+            const double t = (r == c) ? mu * (2.0 * g[c][1 + r] - mu_bulk * div) : mu * (g[c][1 + r] + g[r][1 + c]);
+            const double t_b = flux_b[r][c] + e_b * vel[r];
+            vel_b[r] += e_b * t;
+            if (r == c) {
+                g_b[c][1 + r] += 2.0 * mu * t_b;
+                div_b += -mu * mu_bulk * t_b;
+            } else {
+                g_b[c][1 + r] += mu * t_b;
+                g_b[r][1 + c] += mu * t_b;
+            }
+        }
+        // cv_dT = (gamma-1)^-1 irho (g[c][p] - por g[c][rho])
+        g_b[c][DIM + 1] += cv_b * gm1inv * irho;
+        g_b[c][0] += -cv_b * gm1inv * irho * por;
+        por_b += -cv_b * gm1inv * irho * g[c][0];
+        irho_b += cv_b * gm1inv * (g[c][DIM + 1] - por * g[c][0]);
+    }
+    for (int d = 0; d < DIM; d++) g_b[d][1 + d] += div_b;
+    const double p_b = por_b * irho;
+    irho_b += por_b * p;
+    s_b[DIM + 1] += gm1 * p_b;
+    double rho_b = -gm1 * 0.5 * vsq * p_b;
+    for (int d = 0; d < DIM; d++) {
+        vel_b[d] += -gm1 * rho * vel[d] * p_b;
+        s_b[1 + d] += vel_b[d] * irho;
+        irho_b += vel_b[d] * s[1 + d];
+    }
+    rho_b += -irho_b * irho * irho;
+    s_b[0] += rho_b;
+}
+
 }  // namespace adj
 }  // namespace prandtl

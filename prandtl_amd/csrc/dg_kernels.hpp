@@ -907,6 +907,264 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
 }
 
 // ---------------------------------------------------------------------------
+// The viscous part of the Navier-Stokes operator adjoint (constant viscosity), in two passes
+// around k_vjp.  Forward (k_lift and the viscous terms of the RHS): w = w(u); BR1 lifting
+// g_ref[r] = D w along r-lines + 1/2 (w+ - w-)/w0 at faces; g_w[c] = (1/J) sum_r adj[r][c]
+// g_ref[r]; g_p[c] = EntropyGrad2PrimGrad(u, g_w[c]); Fv = viscous_flux(u, g_p); Fc[d] =
+// adj[d] . Fv; du += (1/J) sum_d Dhat Fc[d] + the face average -/+ 1/2 (Fc- + Fc+)/(w0 J).
+// k_vjp_visc1 (element per workgroup, thread per node) recomputes the forward lifting and
+// reverses everything except the neighbour side of the lifting jump: it writes
+// the adjoint of g_ref at the element's face points (Gf, for the neighbours), and per node the
+// partial adjoints of w (WB) and of u (SB).  k_vjp_visc2 (thread per node) adds the
+// neighbours' jump contributions to WB, reverses w(u), and adds the result to the output of
+// k_vjp (times dt for the LSRK stage adjoint).
+// ---------------------------------------------------------------------------
+template <int NP>
+struct VjpViscArgs {
+    DTables<NP> T;
+    PhysConst P;
+    int NE, mode;
+    double dt;
+    const int *nbr;
+    const double *geom;
+    const double *u;
+    const double *lam;    // the cotangent lambda (VJP_PLAIN) or kb + B_s ub (VJP_LSRK)
+    double *Gf;           // adjoint of g_ref at face points [r][side][v][e][t]
+    double *WB, *SB;      // partial adjoints of w and u [v][e][node]
+    double *out;          // k_vjp's output, completed by k_vjp_visc2
+};
+
+template <int DIM, int NP>
+__device__ __forceinline__ size_t gslot(int dir, int side, int v, int e, int t, int NE) {
+    constexpr int L = ipow(NP, DIM - 1);
+    return ((((size_t)(dir * 2 + side) * (DIM + 2) + v) * NE + e) * L) + t;
+}
+
+template <int DIM, int NP>
+__global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_vjp_visc1(const VjpViscArgs<NP> A) {
+    using C = LiftCfg<DIM, NP>;
+    constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB, L = C::L;
+    __shared__ double sW[NEQ][EPB * NN];               // w, then lambda, then adjoint of g_ref
+    __shared__ double sU[NEQ][EPB * C::FITEMS];        // neighbour w at face points
+    __shared__ double sG[DIM][NEQ][EPB * NN];          // adjoint of g_ref
+    __shared__ LogRef sLR[EPB];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    const int eb = tid / NN, node = tid - (tid / NN) * NN;
+    const int e0 = blk * EPB, e = e0 + eb;
+    const int NE = A.NE;
+    const int nel = min(EPB, NE - e0);
+    const bool active = (eb < EPB) && (e < NE);
+    const size_t vs = (size_t)NE * NN;
+    double s[NEQ];
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
+        if (node == 0) sLR[eb] = make_log_ref<DIM>(s, A.P);
+    }
+    __syncthreads();
+    double w[NEQ];
+    if (active) {
+        conserv_to_entropy_ref<DIM>(s, A.P, sLR[eb], w);
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sW[v][tid] = w[v];
+    }
+    for (int it = tid; it < nel * C::FITEMS; it += C::BLOCK) {
+        const int fe = it / C::FITEMS, i = it - fe * C::FITEMS;
+        const int dir = i / (2 * L), side = (i / L) & 1, t = i % L;
+        const int st = dir_stride<NP>(dir);
+        const int en = A.nbr[(size_t)(e0 + fe) * 2 * DIM + 2 * dir + side];
+        const int nnb = line_base<NP>(t, dir) + (side ? 0 : (NP - 1) * st);
+        double sn[NEQ], wn[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
+        conserv_to_entropy_ref<DIM>(sn, A.P, sLR[fe], wn);
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sU[v][fe * C::FITEMS + i] = wn[v];
+    }
+    __syncthreads();
+    int ai[3];
+    ai[0] = node % NP;
+    ai[1] = (node / NP) % NP;
+    ai[2] = node / (NP * NP);
+    double invJ = 0.0, adj[DIM][DIM], gw[DIM][NEQ], gp[DIM][NEQ];
+    if (active) {
+        const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
+        invJ = rcp(gm[0]);
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) adj[r][c] = gm[1 + r * DIM + c];
+        double gref[DIM][NEQ];
+#pragma unroll
+        for (int r = 0; r < DIM; r++) {
+            const int st = dir_stride<NP>(r);
+            const int a = ai[r];
+            const int lb = tid - a * st;
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < NP; b++) t += A.T.D[a][b] * sW[v][lb + b * st];
+                gref[r][v] = t;
+            }
+            if (a == NP - 1 || a == 0) {
+                const bool plus = (a == NP - 1);
+                const int i = r * 2 * L + (plus ? L : 0) + line_index<NP>(node, r);
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    const double wn = sU[v][eb * C::FITEMS + i];
+                    gref[r][v] += 0.5 * (plus ? (wn - w[v]) : (w[v] - wn)) * A.T.inv_w0;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                double t = 0.0;
+#pragma unroll
+                for (int r = 0; r < DIM; r++) t += gref[r][v] * adj[r][c];
+                gw[c][v] = t * invJ;
+                gp[c][v] = gw[c][v];
+            }
+            entgrad_to_primgrad<DIM, double>(s, A.P, gp[c]);
+        }
+    }
+    __syncthreads();   // sW (w) dead: load lambda
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sW[v][tid] = A.lam[v * vs + (size_t)e * NN + node];
+    }
+    __syncthreads();
+    double sb[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) sb[v] = 0.0;
+    if (active) {
+        const double sw = A.T.inv_w0 * invJ;
+        double Fcb[DIM][NEQ - 1];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const int st = dir_stride<NP>(d);
+            const int b = ai[d];
+            const int lb = tid - b * st;
+#pragma unroll
+            for (int v = 0; v < NEQ - 1; v++) {
+                double t = 0.0;
+#pragma unroll
+                for (int a = 0; a < NP; a++) t += A.T.Dhat[a][b] * sW[1 + v][lb + a * st];
+                Fcb[d][v] = t * invJ;
+            }
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                if (b != (side ? NP - 1 : 0)) continue;
+                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * d + side];
+                const double swn = A.T.inv_w0 * rcp(A.geom[(size_t)en * (1 + DIM * DIM)]);
+                const int jn = node + (side ? -(NP - 1) : (NP - 1)) * st;
+#pragma unroll
+                for (int v = 0; v < NEQ - 1; v++) {
+                    const double ln = A.lam[(1 + v) * vs + (size_t)en * NN + jn];
+                    Fcb[d][v] += 0.5 * (side ? 1.0 : -1.0) * (sw * sW[1 + v][tid] - swn * ln);
+                }
+            }
+        }
+        double fb[DIM + 1][DIM];
+#pragma unroll
+        for (int v = 0; v < NEQ - 1; v++)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+                double t = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) t += adj[d][c] * Fcb[d][v];
+                fb[v][c] = t;
+            }
+        double gpb[DIM][NEQ], gwb[DIM][NEQ];
+#pragma unroll
+        for (int c = 0; c < DIM; c++)
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) gpb[c][v] = gwb[c][v] = 0.0;
+        adj::viscous_flux_vjp<DIM>(s, gp, A.P.gamma, A.P.mu, A.P.PrInverse, A.P.mu_bulk, fb, sb, gpb);
+#pragma unroll
+        for (int c = 0; c < DIM; c++) adj::entgrad_vjp<DIM>(s, gw[c], A.P.gamma, gpb[c], sb, gwb[c]);
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                double t = 0.0;
+#pragma unroll
+                for (int c = 0; c < DIM; c++) t += adj[r][c] * gwb[c][v];
+                t *= invJ;
+                sG[r][v][tid] = t;
+                const int a = ai[r];
+                if (a == 0) A.Gf[gslot<DIM, NP>(r, 0, v, e, line_index<NP>(node, r), NE)] = t;
+                if (a == NP - 1) A.Gf[gslot<DIM, NP>(r, 1, v, e, line_index<NP>(node, r), NE)] = t;
+            }
+    }
+    __syncthreads();
+    if (!active) return;
+    double wb[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) wb[v] = 0.0;
+#pragma unroll
+    for (int r = 0; r < DIM; r++) {
+        const int st = dir_stride<NP>(r);
+        const int b = ai[r];
+        const int lb = tid - b * st;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < NP; a++) t += A.T.D[a][b] * sG[r][v][lb + a * st];
+            wb[v] += t;
+        }
+        if (b == 0 || b == NP - 1) {
+            const double c = (b == NP - 1 ? -0.5 : 0.5) * A.T.inv_w0;
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) wb[v] += c * sG[r][v][tid];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+        A.WB[v * vs + (size_t)e * NN + node] = wb[v];
+        A.SB[v * vs + (size_t)e * NN + node] = sb[v];
+    }
+}
+
+template <int DIM, int NP>
+__global__ void __launch_bounds__(256) k_vjp_visc2(const VjpViscArgs<NP> A) {
+    constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
+    const size_t vs = (size_t)A.NE * NN;
+    const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= vs) return;
+    const int e = (int)(gi / NN), node = (int)(gi - (size_t)e * NN);
+    double s[NEQ], wb[NEQ], sb[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+        s[v] = A.u[v * vs + gi];
+        wb[v] = A.WB[v * vs + gi];
+        sb[v] = A.SB[v * vs + gi];
+    }
+    // the neighbours' lifting jumps use this node's w: -/+ 1/2 / w0 times their g_ref adjoint
+#pragma unroll
+    for (int r = 0; r < DIM; r++) {
+        const int b = (node / dir_stride<NP>(r)) % NP;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            if (b != (side ? NP - 1 : 0)) continue;
+            const int en = A.nbr[(size_t)e * 2 * DIM + 2 * r + side];
+            const double c = (side ? -0.5 : 0.5) * A.T.inv_w0;
+            const int t = line_index<NP>(node, r);
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) wb[v] += c * A.Gf[gslot<DIM, NP>(r, 1 - side, v, en, t, A.NE)];
+        }
+    }
+    adj::entropy_vars_vjp<DIM>(s, A.P.gamma, wb, sb);
+    const double f = A.mode == VJP_LSRK ? A.dt : 1.0;
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) A.out[v * vs + gi] += f * sb[v];
+}
+
+// ---------------------------------------------------------------------------
 // small utility kernels
 // ---------------------------------------------------------------------------
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n) {

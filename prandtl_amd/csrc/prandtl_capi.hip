@@ -130,6 +130,9 @@ struct prandtl_op_s {
     unsigned long long *d_hcount = nullptr;   // stage hooks: elements filtered, limited
     bool have_bounds = false;     // filter constraints computed
     double *d_vq = nullptr;       // adjoint: per-node quantities, (dim+6) x NE x n
+    double *d_vgf = nullptr;      // NS adjoint: lifting adjoint at face points, dim x 2 x neq x NE x Np^(dim-1)
+    double *d_vwb = nullptr;      // NS adjoint: partial adjoint of the entropy variables
+    double *d_vsb = nullptr;      // NS adjoint: partial adjoint of the state
     double *d_vlam = nullptr;     // adjoint: stage cotangent lambda
     unsigned long long *d_lam = nullptr;
     int *d_nan = nullptr;
@@ -330,19 +333,46 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     a.dt = S.dt;
     const int pblk = (int)std::min<size_t>((nodes + 255) / 256, 8192);
     hipLaunchKernelGGL((k_vjp_prep<DIM, NP>), dim3(pblk), dim3(256), 0, st, a);
+    VjpViscArgs<NP> va;
+    if (op->phys.viscous) {
+        // the viscous part: recompute the lifting and reverse it (k_vjp_visc1) before the
+        // inviscid adjoint writes the output, and complete it afterwards (k_vjp_visc2)
+        if (!op->d_vgf) {
+            HIP_TRY(hipMalloc(&op->d_vgf, (size_t)DIM * 2 * op->neq * op->NE * (op->nn / op->np) * sizeof(double)));
+            HIP_TRY(hipMalloc(&op->d_vwb, op->size * sizeof(double)));
+            HIP_TRY(hipMalloc(&op->d_vsb, op->size * sizeof(double)));
+        }
+        va.T = a.T;
+        va.P = op->P;
+        va.NE = op->NE;
+        va.mode = S.mode;
+        va.dt = S.dt;
+        va.nbr = op->d_nbr;
+        va.geom = op->d_geom;
+        va.u = S.u;
+        va.lam = S.mode == VJP_PLAIN ? S.lam : op->d_vlam;
+        va.Gf = op->d_vgf;
+        va.WB = op->d_vwb;
+        va.SB = op->d_vsb;
+        va.out = S.out;
+        using VC = LiftCfg<DIM, NP>;
+        hipLaunchKernelGGL((k_vjp_visc1<DIM, NP>), dim3((op->NE + VC::EPB - 1) / VC::EPB), dim3(VC::BLOCK), 0, st, va);
+    }
     using LC = LineCfg<DIM, NP>;
     const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
     if (op->opts.surface_flux == PRANDTL_SURF_CHANDRASHEKAR)
         hipLaunchKernelGGL((k_vjp<DIM, NP, 0>), dim3(nblk), dim3(LC::BLOCK), 0, st, a);
     else
         hipLaunchKernelGGL((k_vjp<DIM, NP, 1>), dim3(nblk), dim3(LC::BLOCK), 0, st, a);
+    if (op->phys.viscous)
+        hipLaunchKernelGGL((k_vjp_visc2<DIM, NP>), dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, va);
     HIP_TRY(hipGetLastError());
     return PRANDTL_OK;
 }
 
 int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
-    if (op->phys.viscous)
-        return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers the Euler operator (Navier-Stokes not built)");
+    if (op->phys.viscous && op->phys.sutherland)
+        return fail(PRANDTL_ERR_UNSUPPORTED, "the Navier-Stokes adjoint covers constant viscosity");
     if (op->opts.surface_flux == PRANDTL_SURF_RUSANOV)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers the Chandrashekar and EC surface fluxes");
     if (op->opts.precision != PRANDTL_FP64) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint is FP64");
@@ -673,6 +703,9 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_y2);
     (void)hipFree(op->d_vq);
     (void)hipFree(op->d_vlam);
+    (void)hipFree(op->d_vgf);
+    (void)hipFree(op->d_vwb);
+    (void)hipFree(op->d_vsb);
     (void)hipFree(op->d_emin);
     (void)hipFree(op->d_fbound);
     (void)hipFree(op->d_hflag);

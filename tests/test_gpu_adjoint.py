@@ -204,3 +204,125 @@ def test_operator_vjp_vs_host_assembly(device, dim, surf):
     assert frel <= 1e-12
     assert rel <= 1e-11
     op.close()
+
+
+# ---------------------------------------------------------------------------
+# Navier-Stokes (constant viscosity): the viscous part of the adjoint (BR1 lifting of the
+# entropy variables, EntropyGrad2PrimGrad, viscous flux, its divergence and face average)
+# ---------------------------------------------------------------------------
+NS = dict(gamma=1.4, mu=0.05, Pr=0.71, R_gas=1.0, viscous=True)
+
+
+def _ns_state(dim, rng):
+    if dim == 2:
+        m = M.periodic_lattice(2, 3, (5, 4), A=np.array([[0.5, 0.1], [0.05, 0.45]]))
+        X = m.node_coords()
+        u = cases.prim_to_conserv(1.0 + 0.2 * np.sin(X[0]), [0.3 * np.cos(X[1]), 0.2 * np.sin(X[0])],
+                                  1.0 + 0.1 * np.cos(X[0] + X[1]))
+    else:
+        m = M.periodic_lattice(3, 3, (3, 3, 3), A=np.array([[0.5, 0.1, 0.0], [0.05, 0.45, 0.08],
+                                                            [0.02, -0.03, 0.55]]))
+        X = m.node_coords()
+        u = cases.prim_to_conserv(1.0 + 0.2 * np.sin(X[0]) * np.cos(X[2]),
+                                  [0.3 * np.cos(X[1]), 0.2 * np.sin(X[0]), 0.1 * np.cos(X[2])],
+                                  1.0 + 0.1 * np.cos(X[0] + X[1]))
+    return m, u * (1.0 + 5e-2 * rng.uniform(-1, 1, u.shape))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("surf", [capi.SURF_CHANDRASHEKAR, capi.SURF_EC])
+def test_operator_vjp_ns(device, dim, surf):
+    """NS operator VJP: the dot-product identity against central differences of the GPU NS RHS
+    (relative 1e-7), and entrywise agreement (1e-11 of the largest entry) with the host
+    assembly of the same adjoint (tests/native/adjoint_op_host.cpp), itself checked against
+    the oracle's NS RHS in tests/test_adjoint_flux.py."""
+    import ctypes
+    import os
+    import subprocess
+    import tempfile
+
+    import torch
+
+    import oracle
+
+    _say(f"NS operator VJP dim={dim} surf={surf}")
+    rng = np.random.default_rng(40 + dim + 10 * surf)
+    m, u = _ns_state(dim, rng)
+    op = capi.Operator(m, surface_flux=surf, **NS)
+    ud = torch.from_numpy(u).to(device)
+    lam_np = rng.normal(size=u.shape)
+    lam = torch.from_numpy(lam_np).to(device)
+    v = torch.from_numpy(rng.normal(size=u.shape) * np.abs(u)).to(device)
+    g = torch.empty_like(ud)
+    op.vjp(ud, lam, g)
+    eps = 1e-6
+    dp, dm = torch.empty_like(ud), torch.empty_like(ud)
+    op.mult(ud + eps * v, dp)
+    op.mult(ud - eps * v, dm)
+    torch.cuda.synchronize()
+    lhs = (lam * (dp - dm) / (2 * eps)).sum().item()
+    rhs = (g * v).sum().item()
+    print(f"[adjoint] NS <lam, Jv> = {lhs:.12e}, <J^T lam, v> = {rhs:.12e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-7 * max(abs(lhs), abs(rhs))
+    # host assembly
+    here = os.path.dirname(os.path.abspath(__file__))
+    so = os.path.join(tempfile.mkdtemp(), "libadjop.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", so,
+                    os.path.join(here, "native", "adjoint_op_host.cpp")], check=True)
+    L = ctypes.CDLL(so)
+    dpp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.op_vjp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dpp, dpp, ctypes.c_double, ctypes.c_double,
+                         ctypes.c_int, ctypes.c_int, dpp, dpp, dpp]
+    L.op_vjp_visc.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dpp, dpp, dpp, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_double, dpp, dpp, dpp]
+    D, Dh, Dh2 = oracle.dmatrices(m.order + 1)
+    _, w = oracle.gll(m.order + 1)
+    nbr = np.ascontiguousarray(m.neighbors, dtype=np.int32)
+    href = np.zeros_like(u)
+    args = (m.dim, m.order + 1, m.num_elements, nbr.ctypes.data_as(ip), m.geom.ctypes.data_as(dpp))
+    L.op_vjp(*args, Dh2.ctypes.data_as(dpp), 1.0 / w[0], 1.4, int(surf), 0, u.ctypes.data_as(dpp),
+             lam_np.ctypes.data_as(dpp), href.ctypes.data_as(dpp))
+    L.op_vjp_visc(*args, D.ctypes.data_as(dpp), Dh.ctypes.data_as(dpp), 1.0 / w[0], 1.4, NS["mu"], NS["Pr"],
+                  u.ctypes.data_as(dpp), lam_np.ctypes.data_as(dpp), href.ctypes.data_as(dpp))
+    rel = np.abs(g.cpu().numpy() - href).max() / np.abs(href).max()
+    print(f"[adjoint] NS GPU VJP vs host assembly: {rel:.3e}", flush=True)
+    assert rel <= 1e-11
+    op.close()
+
+
+def test_rk_adjoint_ns(device):
+    """RK adjoint through the NS operator (TGV 3^3 p=3 + 8 modes, mu = 0.02, 10 LSRK4(5) steps,
+    checkpoints every 4): dJ/da_i and dJ/dMa against sixth-order central differences
+    (1e-6 relative), and the dot-product identity for a smooth initial perturbation."""
+    import torch
+
+    from prandtl_amd import torch_adjoint as TA
+
+    _say("NS RK adjoint: TGV 3^3 p=3 + 8 modes, mu=0.02, 10 LSRK4(5) steps")
+    m = cases.tgv_mesh(n=3, order=3)
+    modes = cases.fourier_modes(count=8, kmax=2, seed=13)
+    W = torch.from_numpy(m.quadrature_weights()).to(device)
+    op = capi.Operator(m, gamma=1.4, mu=0.02, Pr=0.71, R_gas=1.0, viscous=True)
+    dt, nsteps, ckpt = 2e-3, 10, 4
+
+    def J_of(mach, amps):
+        u0 = TA.tgv_modes_state(m, mach, amps, modes)
+        return TA.kinetic_energy(TA.RKSolve.apply(u0, op, dt, nsteps, ckpt), W)
+
+    mach = torch.tensor(0.3, dtype=torch.float64, device=device, requires_grad=True)
+    amps = torch.full((8,), 1e-2, dtype=torch.float64, device=device, requires_grad=True)
+    J = J_of(mach, amps)
+    gM, gA = torch.autograd.grad(J, [mach, amps])
+    with torch.no_grad():
+        d4, d42, fd = TA.central_fd(lambda x: J_of(x, amps).item(), mach, 1e-2 * 0.3)
+        print(f"[adjoint] NS dJ/dMa adjoint {gM.item():.12e} FD6 {fd:.12e} (FD4 h {d4:.12e}, 2h {d42:.12e})",
+              flush=True)
+        assert abs(gM.item() - fd) <= 1e-6 * abs(fd)
+        for i in (0, 5):
+            e = torch.zeros_like(amps)
+            e[i] = 1.0
+            d4, d42, fd = TA.central_fd(lambda x: J_of(mach, amps + (x - amps[i]) * e).item(), amps[i], 1e-3)
+            print(f"[adjoint] NS dJ/da_{i} adjoint {gA[i].item():.12e} FD6 {fd:.12e}", flush=True)
+            assert abs(gA[i].item() - fd) <= 1e-6 * abs(fd)
+    op.close()
