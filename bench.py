@@ -40,6 +40,12 @@ FP64_PEAK_TFLOPS = 78.6
 #            LSRK epilogue: read u, dv, Ff, k (160), write k and u' (80)            = 240
 BYTES_PER_NODE = {"lift": 120.0, "rhs_mult": 160.0, "rhs_lsrk": 240.0}
 
+# HBM bytes per launch of the dominant kernel (k_rhs_node FP64, LSRK stage, TGV 32^3 p=3),
+# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (two runs; counter KiB x 1024, the
+# factor a plain copy of known size reads back): 210.7 MB + 167.8 MB.  Recorded in
+# profiles/pmc_tgv32_p3_fp64_mixed.md; PMC counters cannot be read inside this process.
+RHS_LSRK_TRAFFIC_TGV32_P3 = 378.4e6
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -394,7 +400,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": (RHS_LSRK_TRAFFIC_TGV32_P3 if (dom == "k_rhs" and args.n == 32 and args.order == 3) else None),
+        "traffic_source": "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, profiles/pmc_tgv32_p3_fp64_mixed.md",
         "kernel_ms": {"k_lift": round(lift_ms, 4), "k_rhs_lsrk": round(rhs_ms, 4),
                       "k_rhs_mult": round(rhs_mult_ms, 4)},
         "bytes_per_node": BYTES_PER_NODE,
