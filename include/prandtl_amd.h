@@ -154,19 +154,21 @@ int prandtl_hook_counts(prandtl_op op, long long *filtered, long long *limited);
 
 /* Discrete adjoint (no counterpart in the reference, which has no derivatives).
  * out = (dL/du)^T lam at state u: the vector-Jacobian product of the semi-discrete operator
- * (volume flux differencing and surface flux; Euler operators, FP64, Chandrashekar or EC
- * surface flux).  out must not alias u or lam. */
+ * (volume flux differencing, surface flux, and for Navier-Stokes with constant viscosity the
+ * BR1 lifting, viscous flux and viscous face terms; FP64, Chandrashekar or EC surface flux).
+ * out must not alias u or lam. */
 int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *out, prandtl_stream stream);
 
-/* prandtl_rk_run (LSRK4(5)) that also keeps the state at the start of every ckpt_every-step
- * segment in the handle, for prandtl_rk_adjoint. */
+/* prandtl_rk_run (PRANDTL_LSRK45 or PRANDTL_SSPRK3; no stage hooks) that also keeps the state
+ * at the start of every ckpt_every-step segment in the handle, for prandtl_rk_adjoint. */
 int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
                         prandtl_stream stream);
 
 /* Reverse sweep through the last taped run: ubar holds dJ/du at the final state on entry
  * and dJ/du at the initial state on return.  Each segment is recomputed forward from its
- * checkpoint keeping its stage inputs (5 x ckpt_every states of scratch), then the LSRK4(5)
- * stage adjoints run backwards, fused into the operator-adjoint kernel.  Synchronises. */
+ * checkpoint keeping its stage inputs (5 x ckpt_every states of scratch), then the stage
+ * adjoints (LSRK4(5) or SSP-RK3) run backwards, fused into the operator-adjoint kernel.
+ * Euler and constant-viscosity Navier-Stokes operators, FP64.  Synchronises. */
 int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream);
 
 /* Per-kernel timing of one RHS evaluation, with HIP events on `stream`: ms[0] = average

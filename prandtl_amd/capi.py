@@ -188,9 +188,9 @@ class Operator:
         """out = (dL/du)^T lam at u."""
         _check(load().prandtl_op_vjp(self._h, _ptr(u), _ptr(lam), _ptr(out), _stream(stream)))
 
-    def rk_run_tape(self, u, t, dt, nsteps, ckpt_every, stream=None) -> float:
+    def rk_run_tape(self, u, t, dt, nsteps, ckpt_every, stream=None, scheme=LSRK45) -> float:
         tt = ctypes.c_double(t)
-        _check(load().prandtl_rk_run_tape(self._h, LSRK45, _ptr(u), ctypes.byref(tt), float(dt), int(nsteps),
+        _check(load().prandtl_rk_run_tape(self._h, int(scheme), _ptr(u), ctypes.byref(tt), float(dt), int(nsteps),
                                           int(ckpt_every), _stream(stream)))
         return tt.value
 

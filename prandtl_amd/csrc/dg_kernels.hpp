@@ -712,7 +712,9 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
 // fb = -lambda_1/(w0 J1) + lambda_2/(w0 J2).  Each element accumulates only its own nodes (face
 // adjoints are evaluated by both neighbours), so no atomics.
 // ---------------------------------------------------------------------------
-enum VjpMode { VJP_PLAIN = 0, VJP_LSRK = 1 };
+// VJP_SSP: the SSP-RK3 stage adjoint in the form out = x + B (lambda + dt (dL/du)^T lambda)
+// (x optional), lambda the plain cotangent.
+enum VjpMode { VJP_PLAIN = 0, VJP_LSRK = 1, VJP_SSP = 2 };
 
 template <int NP>
 struct VjpArgs {
@@ -722,7 +724,8 @@ struct VjpArgs {
     const int *nbr;
     const double *geom;
     const double *u;      // state at which the Jacobian is taken (the stage input)
-    const double *lam;    // VJP_PLAIN: the cotangent
+    const double *lam;    // VJP_PLAIN, VJP_SSP: the cotangent
+    const double *x;      // VJP_SSP: the additive term (or null)
     const double *ub;     // VJP_LSRK: adjoint of u_s
     const double *kb;     // VJP_LSRK: adjoint of k_s
     double *q;            // per-node quantities, NQ x NE x Np^DIM (written by k_vjp_prep)
@@ -786,7 +789,7 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
     const int eb = tid / L, l = tid - (tid / L) * L;
     const int e = e0 + eb;
     const bool active = (eb < EPB) && (e < NE);
-    const double *lamv = A.mode == VJP_PLAIN ? A.lam : A.lamw;
+    const double *lamv = A.mode == VJP_LSRK ? A.lamw : A.lam;
     double invJ = 0.0, adjm[DIM][DIM];
     if (active) {
         const double *g = A.geom + (size_t)e * (1 + DIM * DIM);
@@ -898,9 +901,11 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
             const double g = sB[v][idx];
             if (A.mode == VJP_PLAIN) {
                 A.out[gi] = g;
-            } else {
+            } else if (A.mode == VJP_LSRK) {
                 A.out[gi] = A.ub[gi] + A.dt * g;
                 A.kb_out[gi] = A.ca * A.lamw[gi];
+            } else {
+                A.out[gi] = (A.x ? A.x[gi] : 0.0) + A.cb * (A.lam[gi] + A.dt * g);
             }
         }
     }
@@ -924,11 +929,11 @@ struct VjpViscArgs {
     DTables<NP> T;
     PhysConst P;
     int NE, mode;
-    double dt;
+    double dt, cb;
     const int *nbr;
     const double *geom;
     const double *u;
-    const double *lam;    // the cotangent lambda (VJP_PLAIN) or kb + B_s ub (VJP_LSRK)
+    const double *lam;    // the cotangent lambda (VJP_PLAIN, VJP_SSP) or kb + B_s ub (VJP_LSRK)
     double *Gf;           // adjoint of g_ref at face points [r][side][v][e][t]
     double *WB, *SB;      // partial adjoints of w and u [v][e][node]
     double *out;          // k_vjp's output, completed by k_vjp_visc2
@@ -1159,7 +1164,7 @@ __global__ void __launch_bounds__(256) k_vjp_visc2(const VjpViscArgs<NP> A) {
         }
     }
     adj::entropy_vars_vjp<DIM>(s, A.P.gamma, wb, sb);
-    const double f = A.mode == VJP_LSRK ? A.dt : 1.0;
+    const double f = A.mode == VJP_LSRK ? A.dt : (A.mode == VJP_SSP ? A.cb * A.dt : 1.0);
 #pragma unroll
     for (int v = 0; v < NEQ; v++) A.out[v * vs + gi] += f * sb[v];
 }
@@ -1167,6 +1172,12 @@ __global__ void __launch_bounds__(256) k_vjp_visc2(const VjpViscArgs<NP> A) {
 // ---------------------------------------------------------------------------
 // small utility kernels
 // ---------------------------------------------------------------------------
+// y = a x1 + b x2 (the adjoint sweep's combinations)
+__global__ void k_axpby(double *y, double a, const double *x1, double b, const double *x2, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = a * x1[i] + b * x2[i];
+}
+
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

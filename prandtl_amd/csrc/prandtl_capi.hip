@@ -138,7 +138,7 @@ struct prandtl_op_s {
     int *d_nan = nullptr;
     bool lam_valid = false;
     // taped forward run for the discrete adjoint: the state at the start of every segment
-    int tape_steps = 0, tape_every = 0;
+    int tape_steps = 0, tape_every = 0, tape_scheme = PRANDTL_LSRK45;
     double tape_dt = 0.0;
     std::vector<double *> tape_ckpt;
     // adjoint sweep storage, kept between calls: the stage inputs of one segment (5 per step;
@@ -305,6 +305,7 @@ struct VjpSpec {
     const double *u, *lam, *ub, *kb;
     double *out, *kb_out;
     double ca, cb, dt;
+    const double *x = nullptr;   // VJP_SSP additive term
 };
 
 template <int DIM, int NP>
@@ -322,6 +323,7 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     a.geom = op->d_geom;
     a.u = S.u;
     a.lam = S.lam;
+    a.x = S.x;
     a.ub = S.ub;
     a.kb = S.kb;
     a.q = op->d_vq;
@@ -347,10 +349,11 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
         va.NE = op->NE;
         va.mode = S.mode;
         va.dt = S.dt;
+        va.cb = S.cb;
         va.nbr = op->d_nbr;
         va.geom = op->d_geom;
         va.u = S.u;
-        va.lam = S.mode == VJP_PLAIN ? S.lam : op->d_vlam;
+        va.lam = S.mode == VJP_LSRK ? op->d_vlam : S.lam;
         va.Gf = op->d_vgf;
         va.WB = op->d_vwb;
         va.SB = op->d_vsb;
@@ -875,7 +878,7 @@ int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *ou
 int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
                         prandtl_stream stream) {
     if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
-    if (scheme != PRANDTL_LSRK45) return fail(PRANDTL_ERR_UNSUPPORTED, "the taped run supports LSRK4(5)");
+    if (scheme != PRANDTL_LSRK45 && scheme != PRANDTL_SSPRK3) return fail(PRANDTL_ERR_ARG, "unknown scheme");
     if (op->opts.stage_hooks)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the filter or limiter hooks");
     if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
@@ -895,6 +898,7 @@ int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double 
     op->tape_steps = nsteps;
     op->tape_every = ckpt_every;
     op->tape_dt = dt;
+    op->tape_scheme = scheme;
     for (int k = 0; k < nseg; k++) {
         HIP_TRY(hipMemcpyAsync(op->tape_ckpt[k], u, op->size * sizeof(double), hipMemcpyDeviceToDevice, st));
         const int n = std::min(ckpt_every, nsteps - k * ckpt_every);
@@ -931,6 +935,52 @@ int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
     }
     std::vector<double *> &stage = op->adj_stage;
     double *ub2 = op->adj_ub2, *kb = op->adj_kb, *kb2 = op->adj_kb2;
+    if (op->tape_scheme == PRANDTL_SSPRK3) {
+        // SSP-RK3 (Shu-Osher): y1 = u + dt L(u), y2 = 3/4 u + 1/4 (y1 + dt L(y1)),
+        // u' = 1/3 u + 2/3 (y2 + dt L(y2)).  Reverse, given ub = adjoint of u':
+        //   yb2 = 2/3 (ub + dt J(y2)^T ub),  yb1 = 1/4 (yb2 + dt J(y1)^T yb2),
+        //   ub <- 1/3 ub + 3/4 yb2 + yb1 + dt J(u)^T yb1.
+        rc = ensure_rk_buffers(op, PRANDTL_SSPRK3);
+        if (rc) return rc;
+        const double dt = op->tape_dt;
+        double *U = ubar, *Y2 = kb, *Y1 = kb2, *X = ub2;
+        for (int seg = nseg - 1; seg >= 0 && rc == PRANDTL_OK; seg--) {
+            const int n = std::min(C, nsteps - seg * C);
+            // recompute the segment: stage[3j] = u^j, stage[3j+1] = y1, stage[3j+2] = y2
+            stage[0] = op->tape_ckpt[seg];
+            for (int j = 0; j < n && rc == PRANDTL_OK; j++) {
+                double *u0 = stage[3 * j];
+                StageSpec S1{EPI_SSP, u0, stage[3 * j + 1], nullptr, u0, 0.0, 1.0, dt, nullptr, false, false};
+                rc = run_stage(op, S1, st);
+                StageSpec S2{EPI_SSP, stage[3 * j + 1], stage[3 * j + 2], nullptr, u0, 0.75, 0.25, dt, nullptr,
+                             false, false};
+                if (!rc) rc = run_stage(op, S2, st);
+                if (!rc && j + 1 < n) {
+                    StageSpec S3{EPI_SSP, stage[3 * j + 2], stage[3 * j + 3], nullptr, u0, 1.0 / 3.0, 2.0 / 3.0, dt,
+                                 nullptr, false, false};
+                    rc = run_stage(op, S3, st);
+                }
+            }
+            for (int j = n - 1; j >= 0 && rc == PRANDTL_OK; j--) {
+                VjpSpec Va{VJP_SSP, stage[3 * j + 2], U, nullptr, nullptr, Y2, nullptr, 0.0, 2.0 / 3.0, dt, nullptr};
+                rc = run_vjp(op, Va, st);
+                VjpSpec Vb{VJP_SSP, stage[3 * j + 1], Y2, nullptr, nullptr, Y1, nullptr, 0.0, 0.25, dt, nullptr};
+                if (!rc) rc = run_vjp(op, Vb, st);
+                if (!rc) {
+                    hipLaunchKernelGGL(k_axpby, dim3(2048), dim3(256), 0, st, X, 1.0 / 3.0, (const double *)U, 0.75,
+                                       (const double *)Y2, op->size);
+                    VjpSpec Vc{VJP_SSP, stage[3 * j], Y1, nullptr, nullptr, X, nullptr, 0.0, 1.0, dt, X};
+                    rc = run_vjp(op, Vc, st);
+                }
+                std::swap(U, X);
+            }
+        }
+        if (rc == PRANDTL_OK && U != ubar) (void)hipMemcpyAsync(ubar, U, bytes, hipMemcpyDeviceToDevice, st);
+        hipError_t e = hipStreamSynchronize(st);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(PRANDTL_ERR_HIP, hipGetErrorString(e));
+        return PRANDTL_OK;
+    }
     hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, kb, 0.0, op->size);
     double *ub_cur = ubar, *ub_nxt = ub2, *kb_cur = kb, *kb_nxt = kb2;
     for (int seg = nseg - 1; seg >= 0 && rc == PRANDTL_OK; seg--) {

@@ -1,9 +1,9 @@
 """Thin torch binding of the discrete adjoint (tests and the adjoint benchmark).
 
-RKSolve.apply(u0, op, dt, nsteps, ckpt_every) advances u0 by nsteps LSRK4(5) steps through
-the C ABI (prandtl_rk_run_tape) and, on backward, returns dJ/du0 = (du_T/du0)^T dJ/du_T from
-prandtl_rk_adjoint: the hand-written operator adjoint and the reverse LSRK stages, with the
-stage inputs recomputed segment by segment from the checkpoints.
+RKSolve.apply(u0, op, dt, nsteps, ckpt_every[, scheme]) advances u0 by nsteps LSRK4(5) (or
+SSP-RK3) steps through the C ABI (prandtl_rk_run_tape) and, on backward, returns
+dJ/du0 = (du_T/du0)^T dJ/du_T from prandtl_rk_adjoint: the hand-written operator adjoint and the
+reverse RK stages, with the stage inputs recomputed segment by segment from the checkpoints.
 """
 from __future__ import annotations
 
@@ -12,9 +12,9 @@ import torch
 
 class RKSolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, op, dt, nsteps, ckpt_every):
+    def forward(ctx, u0, op, dt, nsteps, ckpt_every, scheme=1):
         u = u0.detach().clone().contiguous()
-        op.rk_run_tape(u, 0.0, dt, nsteps, ckpt_every)
+        op.rk_run_tape(u, 0.0, dt, nsteps, ckpt_every, scheme=scheme)
         ctx.op = op
         return u
 
@@ -22,7 +22,7 @@ class RKSolve(torch.autograd.Function):
     def backward(ctx, ubar):
         g = ubar.detach().clone().contiguous()
         ctx.op.rk_adjoint(g)
-        return g, None, None, None, None
+        return g, None, None, None, None, None
 
 
 def central_fd(f, x, h):
