@@ -82,11 +82,21 @@ typedef struct {
  * src/Limiters/Limiter.hpp:10-16. */
 enum { PRANDTL_HOOK_FILTER = 1, PRANDTL_HOOK_LIMITER = 2 };
 
+/* Subcell finite-volume blending (the reference built with SUBCELL_FV_BLENDING): every RHS
+ * evaluation computes the per-element coefficient alpha from its input state (Persson-Peraire
+ * indicator on rho p) and blends the inviscid volume term as (1 - alpha) DG + alpha FV --
+ * ref: DGSEMOperator::Mult, DGSEMOperator.cpp:547-549; DGSEMIntegrator.cpp:240-241, 394-396,
+ * 972-1177.  FP64 only.  The reference's defaults: alpha_max 0.5 (runtime "alpha_max",
+ * Simulation.cpp:477-484), alpha_min 0.001 (DGSEMOperator.hpp:99); they are taken as given
+ * here (a zero-initialised struct has alpha_max = 0, which blends nothing). */
 typedef struct {
     int surface_flux; /* PRANDTL_SURF_* */
     int precision;    /* PRANDTL_FP64 | PRANDTL_MIXED */
     int bug_compat;   /* PRANDTL_BUG_* bitmask */
     int stage_hooks;  /* PRANDTL_HOOK_* bitmask */
+    int blending;     /* 1: subcell finite-volume blending */
+    double alpha_max; /* in [0, 1] */
+    double alpha_min; /* in [0, 0.5) */
 } prandtl_opts;
 
 /* Build an operator -- replaces the DGSEMOperator constructor,
@@ -129,6 +139,13 @@ int prandtl_rk_step(prandtl_op op, int scheme, double *u, double *t, double dt, 
  * PRANDTL_ERR_NAN if one is found. */
 int prandtl_rk_run(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
                    prandtl_stream stream);
+
+/* Persson-Peraire indicator eta and blending coefficient alpha of every element of u (device
+ * arrays of num_elements doubles; either may be NULL) -- ref: DGSEMOperator::
+ * ComputeBlendingCoefficient, DGSEMOperator.cpp:60-80, and PerssonPeraireIndicator::
+ * CheckSmoothness, src/Indicators/PerssonPeraireIndicator.cpp:21-58 (the reference's alpha and
+ * eta grid functions).  Uses the operator's alpha_max and alpha_min. */
+int prandtl_blend_coefficients(prandtl_op op, const double *u, double *alpha, double *eta, prandtl_stream stream);
 
 /* Entropy-filter bounds from u: per element, the minimum nodal entropy rho ln(p rho^-gamma)
  * over the element and its face neighbours -- ref: Filter::GetFilterConstraints(const

@@ -8,6 +8,8 @@
 //   mfem::ODESolver Init / Step                   src/Simulation/Simulation.cpp:177-206, 807, 1010
 //   Filter / EntropyFilter                        src/Filters/Filter.hpp:11-30, EntropyFilter.hpp
 //   Limiter                                       src/Limiters/Limiter.hpp:10-16
+//   blending coefficient (Indicator / alpha)      src/Operators/DGSEMOperator.cpp:60-80,
+//                                                 src/Indicators/PerssonPeraireIndicator.cpp:21-58
 // The virtual per-point flux calls of the reference cannot cross to the device; the surface
 // flux is chosen at construction (SurfaceFlux) and compiled into the kernels.
 //
@@ -135,6 +137,9 @@ struct Options {
     Precision precision = Precision::FP64;
     int bug_compat = 0;     // PRANDTL_BUG_* bitmask
     int stage_hooks = 0;    // PRANDTL_HOOK_*: filter / limiter inside the fused steppers
+    bool blending = false;  // subcell finite-volume blending (SUBCELL_FV_BLENDING), FP64
+    double alpha_max = 0.5; // Simulation.cpp:477-484
+    double alpha_min = 0.001;  // DGSEMOperator.hpp:99
 };
 
 // mfem::TimeDependentOperator's surface as the reference uses it.
@@ -160,8 +165,10 @@ public:
         : TimeDependentOperator((int)mesh.StateSize()) {
         prandtl_mesh_desc md{mesh.dim, mesh.order, mesh.num_elements, mesh.neighbors.data(), mesh.geometry.data()};
         prandtl_physics ph{phys.gamma, phys.Pr, phys.mu, phys.R_gas, phys.viscous ? 1 : 0, phys.sutherland ? 1 : 0};
-        prandtl_opts o{(int)opts.surface_flux, (int)opts.precision, opts.bug_compat, opts.stage_hooks};
+        prandtl_opts o{(int)opts.surface_flux, (int)opts.precision, opts.bug_compat, opts.stage_hooks,
+                       opts.blending ? 1 : 0, opts.alpha_max, opts.alpha_min};
         check(prandtl_op_create(&md, &ph, &o, &op_));
+        num_elements_ = mesh.num_elements;
     }
     DGSEMOperator(const DGSEMOperator &) = delete;
     DGSEMOperator &operator=(const DGSEMOperator &) = delete;
@@ -183,6 +190,14 @@ public:
         check(prandtl_op_max_char_speed(op_, u.Read(), &l, stream_));
         return l;
     }
+    // the blending coefficient alpha and indicator eta of every element of u (the reference's
+    // alpha and eta grid functions, DGSEMOperator::ComputeBlendingCoefficient); each Vector has
+    // num_elements entries
+    void ComputeBlendingCoefficient(const Vector &u, Vector &alpha, Vector &eta) const {
+        if (alpha.Size() != (size_t)num_elements_ || eta.Size() != (size_t)num_elements_)
+            throw Error(PRANDTL_ERR_ARG, "alpha and eta hold one value per element");
+        check(prandtl_blend_coefficients(op_, u.Read(), alpha.Write(), eta.Write(), stream_));
+    }
     void SetStream(prandtl_stream s) { stream_ = s; }
     prandtl_stream Stream() const { return stream_; }
     prandtl_op Handle() const { return op_; }
@@ -190,6 +205,7 @@ public:
 private:
     prandtl_op op_ = nullptr;
     prandtl_stream stream_ = nullptr;
+    int num_elements_ = 0;
 };
 
 // mfem::ODESolver's Init / Step.  The steppers run the fused stage kernels of the library, so

@@ -87,6 +87,15 @@ def _configure(L):
     L.orc_rk_run_hooked.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, ip]
     L.orc_rk_run_hooked.restype = ctypes.c_int
+    L.orc_blend_coef.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, dp, dp]
+    L.orc_blend_coef.restype = ctypes.c_int
+    L.orc_rhs_blend_mag.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, dp, dp, dp, dp, dp, dp, dp]
+    L.orc_rhs_blend_mag.restype = ctypes.c_int
+    L.orc_rk_run_blend.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   dp, ip]
+    L.orc_rk_run_blend.restype = ctypes.c_int
     return L
 
 
@@ -295,5 +304,53 @@ def rk_run_hooked(mesh, ph, u, scheme: int, nsteps: int, dt: float, hooks: int):
     rc = lib().orc_rk_run_hooked(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
                                  ctypes.byref(ph), int(scheme), int(nsteps), float(dt), int(hooks), _dp(u),
                                  _ip(counts))
+    assert rc == 0, rc
+    return u, counts
+
+
+# ---------------------------------------------------------------------------
+# subcell finite-volume blending (prandtl_oracle.cpp, prandtl_oracle_hooks.cpp)
+ALPHA_MAX = 0.5     # Simulation.cpp:477-484 (runtime "alpha_max", default 0.5)
+ALPHA_MIN = 0.001   # DGSEMOperator.hpp:99
+
+
+def blend_coef(mesh, u, gamma=1.4, alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN):
+    """Persson-Peraire indicator and blending coefficient per element: (alpha, eta)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    a = np.zeros(mesh.num_elements)
+    eta = np.zeros(mesh.num_elements)
+    assert lib().orc_blend_coef(mesh.dim, mesh.order, mesh.num_elements, _dp(u), float(gamma), float(alpha_max),
+                                float(alpha_min), _dp(a), _dp(eta)) == 0
+    return a, eta
+
+
+def rhs_blend(mesh, ph, u, alpha, want_grad=False, fma=False):
+    """Blended RHS with the given per-element coefficients: (dudt, lambda, mag[, grad, gmag])."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    assert alpha.shape == (mesh.num_elements,)
+    dudt = np.zeros_like(u)
+    mag = np.zeros_like(u)
+    grad = np.zeros((mesh.dim,) + u.shape) if want_grad else None
+    gmag = np.zeros((mesh.dim,) + u.shape) if want_grad else None
+    lam = ctypes.c_double(0.0)
+    rc = lib(fma).orc_rhs_blend_mag(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+                                    ctypes.byref(ph), _dp(u), _dp(alpha), _dp(dudt), _dp(grad), ctypes.byref(lam),
+                                    _dp(mag), _dp(gmag))
+    assert rc == 0, rc
+    if want_grad:
+        return dudt, lam.value, mag, grad, gmag
+    return dudt, lam.value, mag
+
+
+def rk_run_blend(mesh, ph, u, scheme: int, nsteps: int, dt: float, hooks: int = 0,
+                 alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN):
+    """rk_run with blending in every RHS (coefficients from the stage state), and the stage hooks.
+    Returns (u, [elements filtered, elements limited])."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    counts = np.zeros(2, dtype=np.int32)
+    rc = lib().orc_rk_run_blend(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+                                ctypes.byref(ph), int(scheme), int(nsteps), float(dt), int(hooks), 1,
+                                float(alpha_max), float(alpha_min), _dp(u), _ip(counts))
     assert rc == 0, rc
     return u, counts

@@ -486,7 +486,7 @@ inline int line_node(const Mesh &M, int d, int a, int t1, int t2) {
 // fluxes F_viscous/G_viscous/H_viscous, and the A1 bug under P.bug & 1).
 // ---------------------------------------------------------------------------
 double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, const double *Fvis,
-                       double *el_dudt, double *el_mag, const double *FvisMag) {
+                       double *el_dudt, double *el_mag, const double *FvisMag, const double *alpha) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq;
     const int Npy = dim > 1 ? np : 1, Npz = dim > 2 ? np : 1;
     double lam = 0.0;
@@ -578,17 +578,19 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
                 }
                 for (int v = 0; v < neq; v++) {
                     double dU = -dUi[v];                                  // Neg :392 / :740
+                    if (alpha) dU *= (1.0 - *alpha);                      // :395 / :742
                     if (Fvis) dU = dU + dUv[v];                           // add :744
                     el_dudt[v * n + id1] += dU * (1.0 / J);               // :403 / :745 (Vector /= c is *= 1/c), AddRow
                 }
                 if (el_mag) {
                     // rounding scale: the same sums with every term in absolute value
+                    const double sa = alpha ? std::fabs(1.0 - *alpha) : 1.0;
                     for (int v = 0; v < neq; v++) {
                         double a = 0.0;
                         for (int m = 0; m < np; m++) {
-                            a += std::fabs(FI(Fi, id1, v, m) * M.Dhat2[i * np + m]);
-                            if (dim > 1) a += std::fabs(FI(Gi, id1, v, m) * M.Dhat2[j * np + m]);
-                            if (dim > 2) a += std::fabs(FI(Hi, id1, v, m) * M.Dhat2[k * np + m]);
+                            a += sa * std::fabs(FI(Fi, id1, v, m) * M.Dhat2[i * np + m]);
+                            if (dim > 1) a += sa * std::fabs(FI(Gi, id1, v, m) * M.Dhat2[j * np + m]);
+                            if (dim > 2) a += sa * std::fabs(FI(Hi, id1, v, m) * M.Dhat2[k * np + m]);
                             if (FvisMag) {
                                 a += FvisMag[(0 * n + M.lex(m, j, k)) * neq + v] * std::fabs(M.Dhat[i * np + m]);
                                 if (dim > 1) a += FvisMag[(1 * n + M.lex(i, m, k)) * neq + v] * std::fabs(M.Dhat[j * np + m]);
@@ -599,6 +601,71 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
                     }
                 }
             }
+    return lam;
+}
+
+// ---------------------------------------------------------------------------
+// Subcell finite-volume term of the blended scheme -- ref: DGSEMIntegrator::ComputeFVFluxes
+// (DGSEMIntegrator.cpp:972-1177) with the subcell metrics of ComputeSubcellMetrics
+// (:1180-1304); AXISYMMETRIC off.  Along every line in direction d the interface between
+// nodes i and i+1 carries the surface flux f*(u_i, u_{i+1}; n_{i+1}) (the surface-flux
+// plug-in, :1011); node i receives (f_{i-1/2} - f_{i+1/2}) / (J w_i) (:1042-1057), the two
+// outer interfaces of the line excluded (the DG surface term carries them).  The element's
+// sum is scaled by alpha (:1177).  Subcell normal n_i = adj_d(x_first) + sum_{l<i} w_l
+// sum_m D_lm adj_d(x_m) (:1194-1211; D_T holds D^T, so its column l is row l of D), evaluated
+// as written: on the affine elements here adj is constant and the sums vanish to rounding.
+// ---------------------------------------------------------------------------
+double ElementSubcellFV(const Mesh &M, const Phys &P, int e, const double *el_u, double alpha, double *el_dudt,
+                        double *el_mag) {
+    const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq;
+    const int Nt1 = dim > 1 ? np : 1, Nt2 = dim > 2 ? np : 1;
+    const double J = M.J(e);
+    double lam = 0.0;
+    std::vector<double> fv((size_t)n * neq, 0.0), fm((size_t)n * neq, 0.0);
+    for (int d = 0; d < dim; d++) {
+        // subcell normals of the element's lines in direction d (the same for every line: affine)
+        double nor[MAXNP][3];
+        for (int i = 0; i < np; i++) {
+            for (int c = 0; c < dim; c++) nor[i][c] = M.adj(e, d, c);       // nor = metric1
+            for (int l = 0; l < i; l++) {
+                double tmp[3] = {0.0, 0.0, 0.0};
+                for (int m = 0; m < np; m++)
+                    for (int c = 0; c < dim; c++) tmp[c] += M.adj(e, d, c) * M.D[l * np + m];
+                for (int c = 0; c < dim; c++) nor[i][c] += tmp[c] * M.w[l];
+            }
+        }
+        for (int t2 = 0; t2 < Nt2; t2++)
+            for (int t1 = 0; t1 < Nt1; t1++) {
+                double dU[MAXEQ], aU[MAXEQ], s1[MAXEQ], s2[MAXEQ], f[MAXEQ];
+                for (int v = 0; v < neq; v++) dU[v] = aU[v] = 0.0;
+                int id1 = line_node(M, d, 0, t1, t2);
+                for (int v = 0; v < neq; v++) s1[v] = el_u[v * n + id1];
+                for (int i = 0; i < np - 1; i++) {
+                    const int id2 = line_node(M, d, i + 1, t1, t2);
+                    for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
+                    lam = std::max(lam, FaceFlux(P, s1, s2, nor[i + 1], f));
+                    const double sc = 1.0 / (J * M.w[i]);                  // Vector /= c multiplies by 1/c
+                    for (int v = 0; v < neq; v++) {
+                        dU[v] -= f[v];
+                        fv[(size_t)v * n + id1] += dU[v] * sc;             // SetRow (xi) / AddRow (eta, zeta)
+                        fm[(size_t)v * n + id1] += (aU[v] + std::fabs(f[v])) / std::fabs(J * M.w[i]);
+                        dU[v] = f[v];
+                        aU[v] = std::fabs(f[v]);
+                    }
+                    for (int v = 0; v < neq; v++) s1[v] = s2[v];
+                    id1 = id2;
+                }
+                const double sc = 1.0 / (J * M.w[np - 1]);
+                for (int v = 0; v < neq; v++) {
+                    fv[(size_t)v * n + id1] += dU[v] * sc;
+                    fm[(size_t)v * n + id1] += aU[v] / std::fabs(J * M.w[np - 1]);
+                }
+            }
+    }
+    for (size_t i = 0; i < (size_t)n * neq; i++) {
+        el_dudt[i] += fv[i] * alpha;                                        // el_dudt_mat *= alpha
+        if (el_mag) el_mag[i] += fm[i] * std::fabs(alpha);
+    }
     return lam;
 }
 
@@ -755,7 +822,7 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
 // Returns lambda_max over all pair and face flux evaluations (:270,:188).
 // ---------------------------------------------------------------------------
 double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, double *grad_out, double *mag = nullptr,
-               double *gmag = nullptr) {
+               double *gmag = nullptr, const double *alpha = nullptr) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq, NE = M.NE;
     const size_t size = (size_t)neq * NE * n;
     std::vector<double> grad_local, gmag_local;
@@ -815,8 +882,15 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                 }
             }
         }
+        if (alpha) {
+            // SUBCELL_FV_BLENDING: the FV term first, into the zeroed element vector (:240-241)
+            const double l = ElementSubcellFV(M, P, e, el_u.data(), alpha[e], el_dudt.data(),
+                                              mag ? el_mag.data() : nullptr);
+            lam = std::max(lam, l);
+        }
         double l = ElementInviscid(M, P, e, el_u.data(), P.viscous ? Fvis.data() : nullptr, el_dudt.data(),
-                                   mag ? el_mag.data() : nullptr, FvisMag.empty() ? nullptr : FvisMag.data());
+                                   mag ? el_mag.data() : nullptr, FvisMag.empty() ? nullptr : FvisMag.data(),
+                                   alpha ? alpha + e : nullptr);
         lam = std::max(lam, l);
         for (int v = 0; v < neq; v++)
             for (int node = 0; node < n; node++) dudt[gidx(M, v, e, node)] += el_dudt[(size_t)v * n + node];
@@ -1022,6 +1096,23 @@ int orc_rhs_mag(int dim, int order, int NE, const int *nbr, const double *geom, 
     std::vector<double> gm_local;
     if (gmag && !grad) return -3;
     double lam = RhsEval(M, P, u, dudt, grad, mag, gmag);
+    if (lambda_max) *lambda_max = lam;
+    return 0;
+}
+
+// As orc_rhs_mag with subcell finite-volume blending (the reference built with
+// SUBCELL_FV_BLENDING): alpha holds one blending coefficient per element, as
+// DGSEMOperator::ComputeBlendingCoefficient leaves it in the alpha grid function
+// (orc_blend_coef computes it).
+int orc_rhs_blend_mag(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
+                      const double *u, const double *alpha, double *dudt, double *grad, double *lambda_max,
+                      double *mag, double *gmag) {
+    if (dim < 2 || dim > 3 || order < 1 || order + 1 > MAXNP || !alpha) return -1;
+    Mesh M;
+    MakeMesh(M, dim, order, NE, nbr, geom);
+    Phys P = phys_of(dim, p);
+    if (gmag && !grad) return -3;
+    double lam = RhsEval(M, P, u, dudt, grad, mag, gmag, alpha);
     if (lambda_max) *lambda_max = lam;
     return 0;
 }

@@ -16,6 +16,10 @@ typedef struct {
 int orc_gll(int np, double *x, double *w);
 int orc_rhs(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, const double *u,
             double *dudt, double *grad, double *lambda_max);
+// RHS with subcell finite-volume blending; alpha: one blending coefficient per element
+int orc_rhs_blend_mag(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
+                      const double *u, const double *alpha, double *dudt, double *grad, double *lambda_max,
+                      double *mag, double *gmag);
 
 #ifdef __cplusplus
 }

@@ -313,9 +313,70 @@ int LimiterApply(int dim, int order, int NE, double *u, double gamma, double eps
     return count;
 }
 
+// ---------------------------------------------------------------------------
+// Blending coefficient of the subcell finite-volume blending -- ref:
+// PerssonPeraireIndicator::CheckSmoothness (src/Indicators/PerssonPeraireIndicator.cpp:21-58)
+// and DGSEMOperator::ComputeBlendingCoefficient (src/Operators/DGSEMOperator.cpp:60-80), with
+// modalThreshold T = 0.5 10^(-1.8 p^(1/4)) (:25) and sharpness_fac 9.21024
+// (DGSEMOperator.hpp:50).  Per element: q = rho p at the nodes (ComputePressure,
+// BasicOperations.cpp:31-37), its modes V^-1 q (ModalBasis::ComputeModes, :146-150); modesM1
+// drops the modes with a degree above p - 1, modesM2 those above p - 2;
+// eta = max(1 - |M1|^2/|M|^2, 1 - |M2|^2/|M1|^2);  alpha = 1/(1 + exp(-9.21024 (eta - T)/T)),
+// set to 0 below alpha_min, to 1 above 1 - alpha_min, then min(alpha, alpha_max).
+// At p = 1 every mode has a degree above p - 2, so |M2| = 0 and eta = 1 (as in the reference).
+// ---------------------------------------------------------------------------
+void BlendingCoefficient(int dim, int order, int NE, const double *u, double gamma, double alpha_max,
+                         double alpha_min, double *alpha, double *eta) {
+    const ModalBasis B = MakeModalBasis(dim, order);
+    const int n = B.n, neq = dim + 2;
+    const size_t vs = (size_t)NE * n;
+    const double gammaM1 = gamma - 1.0;
+    const double T = 0.5 * std::pow(10.0, -1.8 * std::pow((double)order, 0.25));
+    const double sharpness = 9.21024;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < NE; e++) {
+        std::vector<double> q(n), modes(n, 0.0);
+        for (int i = 0; i < n; i++) {
+            double s[5];
+            for (int v = 0; v < neq; v++) s[v] = u[v * vs + (size_t)e * n + i];
+            const double inv = 1.0 / s[0];                 // V /= state(0)
+            double vv = 0.0;
+            for (int d = 1; d < neq - 1; d++) vv += (s[d] * inv) * (s[d] * inv);
+            q[i] = s[0] * (gammaM1 * (s[neq - 1] - 0.5 * s[0] * vv));
+        }
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) modes[i] += B.Vinv[(size_t)i * n + j] * q[j];
+        double mm = 0.0, m1 = 0.0, m2 = 0.0;              // modes*modes, modesM1*modesM1, modesM2*modesM2
+        for (int i = 0; i < n; i++) {
+            const double a = modes[i];
+            mm += a * a;
+            if (B.maxdeg[i] <= order - 1) m1 += a * a;
+            if (B.maxdeg[i] <= order - 2) m2 += a * a;
+        }
+        double ind = 1.0 - m1 / mm;
+        ind = std::max(ind, 1.0 - m2 / m1);
+        double a = 1.0 / (1.0 + std::exp(-sharpness * (ind - T) / T));
+        if (a < alpha_min)
+            a = 0.0;
+        else if (a > (1.0 - alpha_min))
+            a = 1.0;
+        a = std::min(a, alpha_max);
+        if (alpha) alpha[e] = a;
+        if (eta) eta[e] = ind;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// Persson-Peraire indicator eta and blending coefficient alpha of every element
+int orc_blend_coef(int dim, int order, int NE, const double *u, double gamma, double alpha_max, double alpha_min,
+                   double *alpha, double *eta) {
+    if ((dim != 2 && dim != 3) || order < 1 || order > 4) return -1;
+    BlendingCoefficient(dim, order, NE, u, gamma, alpha_max, alpha_min, alpha, eta);
+    return 0;
+}
 
 // V (n x n, row-major) and the max degree of every mode (ModalBasis + EntropyFilter.cpp:59-73)
 int orc_modal_basis(int dim, int order, double *V, double *Vinv, int *maxdeg) {
@@ -349,18 +410,35 @@ int orc_limiter_apply(int dim, int order, int NE, double *u, double gamma, doubl
 // filter's entropy bounds from the state at the start of each step (GetFilterConstraints),
 // then after every stage the filter (FilterSolution) and the limiter (LimitSolution).
 // hooks: bit 0 filter, bit 1 limiter.  counts[0], counts[1] accumulate the elements changed.
+// blend != 0: every RHS evaluation with subcell finite-volume blending, its coefficients from
+// the stage state (DGSEMOperator::Mult, DGSEMOperator.cpp:547-549).
+int orc_rk_run_blend(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
+                     int scheme, int nsteps, double dt, int hooks, int blend, double alpha_max, double alpha_min,
+                     double *u, int *counts);
+
 int orc_rk_run_hooked(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
                       int scheme, int nsteps, double dt, int hooks, double *u, int *counts) {
+    return orc_rk_run_blend(dim, order, NE, nbr, geom, p, scheme, nsteps, dt, hooks, 0, 0.0, 0.0, u, counts);
+}
+
+int orc_rk_run_blend(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
+                     int scheme, int nsteps, double dt, int hooks, int blend, double alpha_max, double alpha_min,
+                     double *u, int *counts) {
     if ((dim != 2 && dim != 3) || order < 1 || order > 4) return -1;
     const double gamma = p->gamma;
     const int np = order + 1, n = dim == 2 ? np * np : np * np * np;
     const size_t size = (size_t)(dim + 2) * NE * n;
-    std::vector<double> k(size), y(size), kk(size), bound(NE);
+    std::vector<double> k(size), y(size), kk(size), bound(NE), alpha(NE);
     auto hook = [&](double *x) {
         if (hooks & 1) counts[0] += FilterApply(dim, order, NE, x, gamma, bound.data(), nullptr);
         if (hooks & 2) counts[1] += LimiterApply(dim, order, NE, x, gamma, 1e-8);
     };
-    auto rhs = [&](const double *x) { return orc_rhs(dim, order, NE, nbr, geom, p, x, k.data(), nullptr, nullptr); };
+    auto rhs = [&](const double *x) {
+        if (!blend) return orc_rhs(dim, order, NE, nbr, geom, p, x, k.data(), nullptr, nullptr);
+        BlendingCoefficient(dim, order, NE, x, gamma, alpha_max, alpha_min, alpha.data(), nullptr);
+        return orc_rhs_blend_mag(dim, order, NE, nbr, geom, p, x, alpha.data(), k.data(), nullptr, nullptr, nullptr,
+                                 nullptr);
+    };
     for (int s = 0; s < nsteps; s++) {
         if (hooks & 1) FilterConstraints(dim, order, NE, nbr, u, gamma, bound.data());
         if (scheme == 0) {

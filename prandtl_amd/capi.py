@@ -31,7 +31,9 @@ EXPORTED = [
     "prandtl_op_profile", "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_last_error",
     "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
     "prandtl_device_alloc", "prandtl_device_free", "prandtl_memcpy", "prandtl_stream_synchronize",
+    "prandtl_blend_coefficients",
 ]
+ALPHA_MAX, ALPHA_MIN = 0.5, 0.001   # the reference's blending defaults
 
 
 class MeshDesc(ctypes.Structure):
@@ -46,7 +48,8 @@ class Physics(ctypes.Structure):
 
 class Opts(ctypes.Structure):
     _fields_ = [("surface_flux", ctypes.c_int), ("precision", ctypes.c_int), ("bug_compat", ctypes.c_int),
-                ("stage_hooks", ctypes.c_int)]
+                ("stage_hooks", ctypes.c_int), ("blending", ctypes.c_int), ("alpha_max", ctypes.c_double),
+                ("alpha_min", ctypes.c_double)]
 
 
 class PrandtlError(RuntimeError):
@@ -86,12 +89,14 @@ def load():
         L.prandtl_filter_apply.argtypes = [vp, vp, vp]
         L.prandtl_limiter_apply.argtypes = [vp, vp, vp]
         L.prandtl_hook_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+        L.prandtl_blend_coefficients.argtypes = [vp, vp, vp, vp, vp]
         L.prandtl_last_error.argtypes = []
         L.prandtl_last_error.restype = ctypes.c_char_p
         for n in ("prandtl_op_mult", "prandtl_op_mult_grad", "prandtl_op_get_max_char_speed",
                   "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile",
                   "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_filter_constraints",
-                  "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts"):
+                  "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
+                  "prandtl_blend_coefficients"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -121,7 +126,8 @@ class Operator:
     """Owns a prandtl_op handle: the DGSEMOperator of one mesh and physics."""
 
     def __init__(self, mesh: Mesh, gamma=1.4, viscous=False, Pr=0.72, mu=0.0, R_gas=287.0, sutherland=False,
-                 surface_flux=SURF_CHANDRASHEKAR, precision=FP64, bug_compat=0, stage_hooks=0):
+                 surface_flux=SURF_CHANDRASHEKAR, precision=FP64, bug_compat=0, stage_hooks=0, blending=False,
+                 alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN):
         L = load()
         self.mesh = mesh
         self._nbr = np.ascontiguousarray(mesh.neighbors, dtype=np.int32)
@@ -130,7 +136,8 @@ class Operator:
                       self._nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                       self._geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         ph = Physics(gamma, Pr, mu, R_gas, int(viscous), int(sutherland))
-        op = Opts(int(surface_flux), int(precision), int(bug_compat), int(stage_hooks))
+        op = Opts(int(surface_flux), int(precision), int(bug_compat), int(stage_hooks), int(blending),
+                  float(alpha_max), float(alpha_min))
         h = ctypes.c_void_p()
         _check(L.prandtl_op_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h)))
         self._h = h
@@ -197,6 +204,16 @@ class Operator:
     def rk_adjoint(self, ubar, stream=None):
         """ubar: dJ/du_final on entry, dJ/du_initial on return (for the last taped run)."""
         _check(load().prandtl_rk_adjoint(self._h, _ptr(ubar), _stream(stream)))
+
+    # subcell blending -- DGSEMOperator::ComputeBlendingCoefficient (the alpha and eta grid functions)
+    def blend_coefficients(self, u, stream=None):
+        """(alpha, eta) per element of u, device tensors."""
+        import torch
+
+        alpha = torch.empty(self.mesh.num_elements, dtype=torch.float64, device=u.device)
+        eta = torch.empty_like(alpha)
+        _check(load().prandtl_blend_coefficients(self._h, _ptr(u), _ptr(alpha), _ptr(eta), _stream(stream)))
+        return alpha, eta
 
     # stage hooks -- Filter::GetFilterConstraints / FilterSolution, Limiter::LimitSolution
     def filter_constraints(self, u, stream=None):

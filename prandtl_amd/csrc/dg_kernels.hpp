@@ -72,6 +72,7 @@ struct DTables {
     double Dhat[NP][NP];   // D + e0 e0^T / w0 - eN eN^T / wN  (:71-74)
     double Dhat2[NP][NP];  // 2 D + the same corrections      (:76-80)
     double inv_w0;         // 1 / w_0
+    double inv_w[NP];      // 1 / w_a (subcell finite-volume term)
 };
 
 enum EpilogueMode { EPI_MULT = 0, EPI_LSRK = 1, EPI_SSP = 2 };
@@ -97,6 +98,7 @@ struct RhsArgs {
     double ca, cb, dt;
     unsigned long long *lam_bits;
     int *nan_flag;
+    const double *alpha;   // blending coefficient per element (k_rhs_node<..., BLEND = true>)
 };
 
 template <int NP>
@@ -337,18 +339,41 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
 // k_rhs_node (node variant): the same operator with one thread per node.  Each node
 // evaluates the two-point flux with its Np-1 line partners in every direction (the
 // symmetric pair is evaluated by both of its nodes), which needs no LDS accumulator
-// and keeps the node's own state in registers through the fused epilogue.
+// and keeps the node's own state in registers through the fused epilogue.  The surface
+// fluxes of the element's 2 dim Np^(dim-1) face points are work items spread over all
+// threads of the workgroup (a face node's thread would otherwise run them while the
+// interior nodes of its wave idle), with the element's own traces and metric from LDS and
+// the neighbour table read once per workgroup; the face nodes read their fluxes from LDS.
+//
+// BLEND (subcell finite-volume blending, FP64): the inviscid volume term becomes
+// (1 - alpha) DG + alpha FV (ref: DGSEMIntegrator.cpp:240-241, 394-396, 741-742, and
+// ComputeFVFluxes :972-1177), alpha per element from k_blend_coef.  FV: along every line the
+// interface between nodes i and i+1 carries the surface flux f*(u_i, u_{i+1}; adj_d) (on an
+// affine element the subcell metric of ComputeSubcellMetrics, :1180-1304, is the adj row);
+// node i receives (f_{i-1/2} - f_{i+1/2}) / (J w_i), the line's outer interfaces excluded.
+// The interface fluxes are work items spread over the workgroup one direction at a time,
+// staged in the surface-flux LDS buffer after the DG accumulation has read it; workgroups
+// whose elements all have alpha = 0 skip them.
 // ---------------------------------------------------------------------------
-template <int DIM, int NP, bool VISC, int SURF, bool MIX>
+template <int DIM, int NP, bool VISC, int SURF, bool MIX, bool BLEND = false>
 __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES) k_rhs_node(const RhsArgs<NP> A) {
     using C = NodeCfg<DIM, NP>;
     using R = typename std::conditional<MIX, float, double>::type;  // flux arithmetic
+    static_assert(!(MIX && BLEND), "subcell blending is FP64 only");
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
     __shared__ double sPd[2][EPB * NN];   // rho, beta
     __shared__ R sPv[DIM][EPB * NN];      // velocity
     __shared__ R sT[NP * NP];             // Dhat2
     __shared__ double sPref[EPB];         // element reference pressure (mixed precision)
     __shared__ double sHref[EPB];         // element reference enthalpy gamma p / ((gamma-1) rho)
+    __shared__ int sNbr[EPB][2 * DIM];    // face neighbours of the workgroup's elements
+    constexpr int L = ipow(NP, DIM - 1), FITEMS = 2 * DIM * L;
+    __shared__ double sS[NEQ][EPB * NN];  // conserved state
+    __shared__ R sM[EPB][DIM][DIM];       // adj J rows
+    __shared__ R sF[EPB * FITEMS][NEQ];   // surface flux (with the viscous face average) per face point
+    constexpr int SITEMS = (NP - 1) * L;  // subcell interfaces per element and direction
+    static_assert(SITEMS <= FITEMS, "subcell fluxes are staged in the surface-flux buffer");
+    __shared__ double sAlpha[BLEND ? EPB : 1];
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -358,13 +383,30 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     const bool active = (eb < EPB) && (e < NE);
     const size_t vs = (size_t)NE * NN;
     if (tid < NP * NP) sT[tid] = (R)A.T.Dhat2[tid / NP][tid % NP];
-    double s[NEQ];
-    SideState<DIM> qs;
+    // the neighbour table rows and metric rows of the workgroup's elements, read once here
+    // rather than by every face item (EPB (2 dim + dim^2) entries can exceed the block size)
+    for (int i = tid; i < EPB * 2 * DIM; i += C::BLOCK) {
+        const int ib = i / (2 * DIM), f = i - ib * 2 * DIM;
+        const int ee = min(blk * EPB + ib, NE - 1);
+        sNbr[ib][f] = A.nbr[(size_t)ee * 2 * DIM + f];
+    }
+    for (int i = tid; i < EPB * DIM * DIM; i += C::BLOCK) {
+        const int ib = i / (DIM * DIM), r = i - ib * DIM * DIM;
+        const int ee = min(blk * EPB + ib, NE - 1);
+        sM[ib][r / DIM][r % DIM] = (R)A.geom[(size_t)ee * (1 + DIM * DIM) + 1 + r];
+    }
+    if constexpr (BLEND) {
+        for (int i = tid; i < EPB; i += C::BLOCK) sAlpha[i] = (blk * EPB + i < NE) ? A.alpha[blk * EPB + i] : 0.0;
+    }
     double lam = 0.0;
     if (active) {
+        double s[NEQ];
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
-        qs = make_side<DIM>(s, A.P);
+        for (int v = 0; v < NEQ; v++) {
+            s[v] = A.u[v * vs + (size_t)e * NN + node];
+            sS[v][tid] = s[v];
+        }
+        const SideState<DIM> qs = make_side<DIM>(s, A.P);
         sPd[0][tid] = qs.rho;
         sPd[1][tid] = qs.beta;
 #pragma unroll
@@ -376,10 +418,76 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         lam = qs.lam;
     }
     __syncthreads();
+    // surface fluxes: work items (element ib, direction, side, line t) over the whole workgroup,
+    // the element's own traces from LDS, the neighbour's from HBM
+    for (int it = tid; it < EPB * FITEMS; it += C::BLOCK) {
+        const int ib = it / FITEMS, r = it - ib * FITEMS;
+        const int ee = blk * EPB + ib;
+        if (ee >= NE) continue;
+        const int dir = r / (2 * L), side = (r / L) & 1, t = r - (r / L) * L;
+        const int st = dir_stride<NP>(dir);
+        const int base = line_base<NP>(t, dir);
+        const int nown = base + (side ? (NP - 1) * st : 0);
+        const int nnb = base + (side ? 0 : (NP - 1) * st);
+        const int en = sNbr[ib][2 * dir + side];
+        double so[NEQ], sn[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+            so[v] = sS[v][ib * NN + nown];
+            sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
+        }
+        R fv[NEQ - 1];
+        if constexpr (VISC) {
+#pragma unroll
+            for (int v = 0; v < NEQ - 1; v++) {
+                if constexpr (MIX)
+                    fv[v] = 0.5f * (A.Fff[ftrace<DIM, NP>(dir, side, v, ee, t, NE)] +
+                                    A.Fff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
+                else
+                    fv[v] = 0.5 * (A.Ff[ftrace<DIM, NP>(dir, side, v, ee, t, NE)] +
+                                   A.Ff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
+            }
+        }
+        const SideState<DIM> qo = make_side<DIM>(so, A.P);
+        const SideState<DIM> qn = make_side<DIM>(sn, A.P);
+        const R *m = sM[ib][dir];
+        R nmag = m[0] * m[0];
+#pragma unroll
+        for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
+        nmag = sqrt(nmag);
+        R f[NEQ];
+        if constexpr (MIX) {
+            const double p_ref = sPref[ib], h_ref = sHref[ib];
+            if (side)
+                face_flux_mx<DIM, SURF>(so, qo, sn, qn, m, nmag, A.P, A.bug3 != 0, p_ref, h_ref, f);
+            else
+                face_flux_mx<DIM, SURF>(sn, qn, so, qo, m, nmag, A.P, A.bug3 != 0, p_ref, h_ref, f);
+        } else {
+            if (side)
+                face_flux<DIM, SURF>(so, qo, sn, qn, m, nmag, A.P, A.bug3 != 0, f);
+            else
+                face_flux<DIM, SURF>(sn, qn, so, qo, m, nmag, A.P, A.bug3 != 0, f);
+        }
+        if constexpr (VISC) {
+#pragma unroll
+            for (int v = 0; v < NEQ - 1; v++) f[1 + v] -= fv[v];
+        }
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
+    }
+    __syncthreads();
     int bad = 0;
+    int ai[3];
+    ai[0] = node % NP;
+    ai[1] = (node / NP) % NP;
+    ai[2] = node / (NP * NP);
+    R acc[NEQ], accs[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
+    double invJ = 0.0;
     if (active) {
         const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
-        const double invJ = rcp(gm[0]);
+        invJ = rcp(gm[0]);
         R adj[DIM][DIM];
 #pragma unroll
         for (int r = 0; r < DIM; r++)
@@ -387,13 +495,6 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             for (int c = 0; c < DIM; c++) adj[r][c] = (R)gm[1 + r * DIM + c];
         const double p_ref = MIX ? sPref[eb] : 0.0;
         const double h_ref = MIX ? sHref[eb] : 0.0;
-        int ai[3];
-        ai[0] = node % NP;
-        ai[1] = (node / NP) % NP;
-        ai[2] = node / (NP * NP);
-        R acc[NEQ], accs[NEQ];
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
 #pragma unroll
         for (int dir = 0; dir < DIM; dir++) {
             const int st = dir_stride<NP>(dir);
@@ -407,19 +508,22 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 R F[NEQ];
                 if constexpr (MIX) {
                     NodePrimMx<DIM> qo, qb;
-                    qo.rho = qs.rho;
-                    qo.beta = qs.beta;
+                    qo.rho = sPd[0][tid];
+                    qo.beta = sPd[1][tid];
                     qb.rho = sPd[0][li];
                     qb.beta = sPd[1][li];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
-                        qo.vel[d] = (float)qs.vel[d];
+                        qo.vel[d] = sPv[d][tid];
                         qb.vel[d] = sPv[d][li];
                     }
                     volume_flux_mx<DIM>(qo, qb, m, A.P.gammaM1Inverse, p_ref, h_ref, F);
                 } else {
-                    const NodePrim<DIM> qo = to_prim(qs);
-                    NodePrim<DIM> qb;
+                    NodePrim<DIM> qo, qb;
+                    qo.rho = sPd[0][tid];
+                    qo.beta = sPd[1][tid];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) qo.vel[d] = sPv[d][tid];
                     qb.rho = sPd[0][li];
                     qb.beta = sPd[1][li];
 #pragma unroll
@@ -432,51 +536,78 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             }
             if (a == 0 || a == NP - 1) {
                 const int side = (a == NP - 1) ? 1 : 0;
-                const int en = A.nbr[(size_t)e * 2 * DIM + 2 * dir + side];
-                const int nnb = node + (side ? -(NP - 1) * st : (NP - 1) * st);
-                double sn[NEQ];
-                R f[NEQ];
-#pragma unroll
-                for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
-                const SideState<DIM> qn = make_side<DIM>(sn, A.P);
-                R nmag = m[0] * m[0];
-#pragma unroll
-                for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
-                nmag = sqrt(nmag);
-                if constexpr (MIX) {
-                    if (side)
-                        face_flux_mx<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, p_ref, h_ref, f);
-                    else
-                        face_flux_mx<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, p_ref, h_ref, f);
-                } else {
-                    if (side)
-                        face_flux<DIM, SURF>(s, qs, sn, qn, m, nmag, A.P, A.bug3 != 0, f);
-                    else
-                        face_flux<DIM, SURF>(sn, qn, s, qs, m, nmag, A.P, A.bug3 != 0, f);
-                }
-                if constexpr (VISC) {
-                    const int t = line_index<NP>(node, dir);
-#pragma unroll
-                    for (int v = 0; v < NEQ - 1; v++) {
-                        if constexpr (MIX)
-                            f[1 + v] -= 0.5f * (A.Fff[ftrace<DIM, NP>(dir, side, v, e, t, NE)] +
-                                                A.Fff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
-                        else
-                            f[1 + v] -= 0.5 * (A.Ff[ftrace<DIM, NP>(dir, side, v, e, t, NE)] +
-                                               A.Ff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
-                    }
-                }
+                const int it = eb * FITEMS + (dir * 2 + side) * L + line_index<NP>(node, dir);
                 const R sg = side ? R(-1) : R(1);
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) accs[v] += sg * f[v];
+                for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
             }
         }
+    }
+    double fvs[BLEND ? NEQ : 1];
+    if constexpr (BLEND) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) fvs[v] = 0.0;
+        bool any = false;
+#pragma unroll
+        for (int ib = 0; ib < EPB; ib++) any |= sAlpha[ib] != 0.0;
+        if (any) {   // the same value in every thread of the workgroup
+#pragma unroll 1
+            for (int dir = 0; dir < DIM; dir++) {
+                __syncthreads();   // the previous readers of sF are done
+                const int st = dir_stride<NP>(dir);
+                for (int it = tid; it < EPB * SITEMS; it += C::BLOCK) {
+                    const int ib = it / SITEMS, r = it - ib * SITEMS;
+                    if (blk * EPB + ib >= NE || sAlpha[ib] == 0.0) continue;
+                    const int i = r / L, t = r - i * L;
+                    const int nl = ib * NN + line_base<NP>(t, dir) + i * st;
+                    double sl[NEQ], sr[NEQ];
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) {
+                        sl[v] = sS[v][nl];
+                        sr[v] = sS[v][nl + st];
+                    }
+                    const SideState<DIM> ql = make_side<DIM>(sl, A.P);
+                    const SideState<DIM> qr = make_side<DIM>(sr, A.P);
+                    const R *m = sM[ib][dir];
+                    R nmag = m[0] * m[0];
+#pragma unroll
+                    for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
+                    nmag = sqrt(nmag);
+                    R f[NEQ];
+                    face_flux<DIM, SURF>(sl, ql, sr, qr, m, nmag, A.P, A.bug3 != 0, f);
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
+                }
+                __syncthreads();
+                if (active && sAlpha[eb] != 0.0) {
+                    const int a = ai[dir];
+                    const int it = eb * SITEMS + line_index<NP>(node, dir);
+                    const double iw = A.T.inv_w[a];
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) {
+                        double g = 0.0;
+                        if (a > 0) g += sF[it + (a - 1) * L][v];
+                        if (a < NP - 1) g -= sF[it + a * L][v];
+                        fvs[v] += g * iw;
+                    }
+                }
+            }
+        }
+    }
+    if (active) {
         const R sw = (R)(A.T.inv_w0 * invJ);
         const R rinvJ = (R)invJ;
+        const double h_ref = MIX ? sHref[eb] : 0.0;
         double du0 = 0.0;
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
-            R du = -acc[v] * rinvJ + accs[v] * sw;
+            R du;
+            if constexpr (BLEND) {
+                const double al = sAlpha[eb];
+                du = (-acc[v] * rinvJ) * (1.0 - al) + al * (fvs[v] * invJ) + accs[v] * sw;
+            } else {
+                du = -acc[v] * rinvJ + accs[v] * sw;
+            }
             double dud;
             if constexpr (MIX) {
                 // the energy sums carried F_E - h_ref F_rho: add h_ref times the inviscid mass RHS
@@ -494,7 +625,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 }
             }
             // the RK accumulation is FP64 in both modes
-            stage_update<NP>(A, v * vs + (size_t)e * NN + node, s[v], dud, bad);
+            stage_update<NP>(A, v * vs + (size_t)e * NN + node, sS[v][tid], dud, bad);
         }
     }
     finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
@@ -534,6 +665,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
     __shared__ R sU[NW > NFV ? NW : NFV];  // neighbour w at face points, then F~^d
     __shared__ double sWref[EPB][NEQ];
     __shared__ LogRef sLR[EPB];           // reference state of the logarithms (the element's node 0)
+    __shared__ int sNbr[EPB][2 * DIM];    // face neighbours of the workgroup's elements
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -545,6 +677,10 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
     const bool active = (eb < EPB) && (e < NE);
     const size_t vs = (size_t)NE * NN;
 
+    for (int i = tid; i < EPB * 2 * DIM; i += C::BLOCK) {
+        const int ib = i / (2 * DIM), f = i - ib * 2 * DIM;
+        sNbr[ib][f] = A.nbr[(size_t)min(e0 + ib, NE - 1) * 2 * DIM + f];
+    }
     double s[NEQ], w[NEQ];
     if (active) {
 #pragma unroll
@@ -571,7 +707,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftAr
         const int fe = it / C::FITEMS, i = it - fe * C::FITEMS;
         const int dir = i / (2 * L), side = (i / L) & 1, t = i % L;
         const int st = dir_stride<NP>(dir);
-        const int en = A.nbr[(size_t)(e0 + fe) * 2 * DIM + 2 * dir + side];
+        const int en = sNbr[fe][2 * dir + side];
         const int nnb = line_base<NP>(t, dir) + (side ? 0 : (NP - 1) * st);
         double sn[NEQ], wn[NEQ];
 #pragma unroll

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/prandtl_amd.h"
+#include "dg_blend.hpp"
 #include "dg_hooks.hpp"
 #include "dg_kernels.hpp"
 
@@ -117,6 +118,8 @@ struct prandtl_op_s {
     prandtl_opts opts{};
     PhysConst P{};
     double D[MAX_NP * MAX_NP], Dh[MAX_NP * MAX_NP], Dh2[MAX_NP * MAX_NP], inv_w0 = 0.0;
+    double w[MAX_NP] = {};        // GLL weights on [0,1]
+    double *d_alpha = nullptr;    // blending: per-element coefficient of the current RHS
     int *d_nbr = nullptr;
     double *d_geom = nullptr;
     double *d_dv = nullptr;       // NS: viscous volume term, (neq-1) x NE x n
@@ -159,7 +162,49 @@ DTables<NP> make_tables(const prandtl_op_s *op) {
             T.Dhat2[i][j] = op->Dh2[i * NP + j];
         }
     T.inv_w0 = op->inv_w0;
+    for (int i = 0; i < NP; i++) T.inv_w[i] = 1.0 / op->w[i];
     return T;
+}
+
+// 1D inverse Vandermonde of the Legendre basis on the GLL nodes: Vi[k][a] = w_a P_k(x_a) / gamma_k,
+// gamma_k = sum_b w_b P_k(x_b)^2 (the GLL rule integrates P_k P_l exactly for k + l <= 2 Np - 3 and
+// the discrete products vanish for k != l <= Np - 1, so Vi V1 = I).
+template <int NP>
+BlendArgs<NP> make_blend_args(const prandtl_op_s *op) {
+    BlendArgs<NP> B{};
+    double x[NP], w[NP], V[NP][NP];
+    gll_rule(NP, x, w);
+    for (int i = 0; i < NP; i++) {
+        const double z = 2.0 * x[i] - 1.0;
+        V[i][0] = 1.0;
+        if (NP > 1) V[i][1] = z;
+        for (int n = 1; n + 1 < NP; n++) V[i][n + 1] = ((2.0 * n + 1.0) * z * V[i][n] - n * V[i][n - 1]) / (n + 1.0);
+    }
+    for (int k = 0; k < NP; k++) {
+        double g = 0.0;
+        for (int b = 0; b < NP; b++) g += w[b] * V[b][k] * V[b][k];
+        for (int a = 0; a < NP; a++) B.Vi[k][a] = w[a] * V[a][k] / g;
+    }
+    B.NE = op->NE;
+    B.gammaM1 = op->phys.gamma - 1.0;
+    B.T = 0.5 * std::pow(10.0, -1.8 * std::pow((double)op->order, 0.25));   // DGSEMOperator.cpp:25
+    B.sharpness = 9.21024;                                                    // DGSEMOperator.hpp:50
+    B.alpha_max = op->opts.alpha_max;
+    B.alpha_min = op->opts.alpha_min;
+    return B;
+}
+
+template <int DIM, int NP>
+int launch_blend_coef(prandtl_op_s *op, const double *u, double *alpha, double *eta, hipStream_t st) {
+    BlendArgs<NP> B = make_blend_args<NP>(op);
+    B.u = u;
+    B.alpha = alpha;
+    B.eta = eta;
+    using C = NodeCfg<DIM, NP>;
+    const int nblk = (op->NE + C::EPB - 1) / C::EPB;
+    hipLaunchKernelGGL((k_blend_coef<DIM, NP>), dim3(nblk), dim3(C::BLOCK), 0, st, B);
+    HIP_TRY(hipGetLastError());
+    return PRANDTL_OK;
 }
 
 struct StageSpec {
@@ -220,6 +265,19 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.dt = S.dt;
     ra.lam_bits = op->d_lam;
     ra.nan_flag = S.nan_check ? op->d_nan : nullptr;
+    ra.alpha = op->d_alpha;
+    if constexpr (!MIX) {
+        if (op->opts.blending) {
+            // the coefficients of this RHS evaluation from its input state (DGSEMOperator.cpp:547-549)
+            int rc = launch_blend_coef<DIM, NP>(op, S.u, op->d_alpha, nullptr, st);
+            if (rc) return rc;
+            using LC = NodeCfg<DIM, NP>;
+            const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+            hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            HIP_TRY(hipGetLastError());
+            return PRANDTL_OK;
+        }
+    }
     // Navier-Stokes: one thread per node (the node's state stays in registers through the
     // fused epilogue, no LDS accumulator: 4 waves/SIMD); Euler: one thread per GLL line
     // (each symmetric pair flux evaluated once).  Chosen by measurement at TGV 32^3, p=3.
@@ -374,6 +432,7 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
 }
 
 int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
+    if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
     if (op->phys.viscous && op->phys.sutherland)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the Navier-Stokes adjoint covers constant viscosity");
     if (op->opts.surface_flux == PRANDTL_SURF_RUSANOV)
@@ -637,6 +696,11 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     if (o.surface_flux < 0 || o.surface_flux > 2) return fail(PRANDTL_ERR_ARG, "unknown surface flux");
     if (o.precision != PRANDTL_FP64 && o.precision != PRANDTL_MIXED) return fail(PRANDTL_ERR_ARG, "unknown precision");
     if (o.stage_hooks & ~(PRANDTL_HOOK_FILTER | PRANDTL_HOOK_LIMITER)) return fail(PRANDTL_ERR_ARG, "unknown stage hook");
+    if (o.blending != 0 && o.blending != 1) return fail(PRANDTL_ERR_ARG, "blending must be 0 or 1");
+    if (!(o.alpha_min >= 0.0 && o.alpha_min < 0.5) || !(o.alpha_max >= 0.0 && o.alpha_max <= 1.0))
+        return fail(PRANDTL_ERR_ARG, "alpha_min must lie in [0, 0.5) and alpha_max in [0, 1]");
+    if (o.blending && o.precision != PRANDTL_FP64)
+        return fail(PRANDTL_ERR_UNSUPPORTED, "subcell blending runs in FP64");
     if (phys->viscous && (o.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
         return fail(PRANDTL_ERR_UNSUPPORTED, "bug_compat A1/A2 are reproduced by the CPU oracle only");
     if (phys->viscous && (!(phys->Pr > 0.0) || phys->mu < 0.0)) return fail(PRANDTL_ERR_ARG, "bad Pr or mu");
@@ -669,6 +733,7 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     gll_rule(op->np, x, w);
     diff_matrices(op->np, x, w, op->D, op->Dh, op->Dh2);
     op->inv_w0 = 1.0 / w[0];
+    for (int i = 0; i < op->np; i++) op->w[i] = w[i];
 
     auto cleanup = [&](int code) {
         prandtl_op_destroy(op);
@@ -682,7 +747,8 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         (e = hipMalloc(&op->d_geom, gb)) != hipSuccess ||
         (e = hipMemcpy(op->d_geom, mesh->geometry, gb, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMalloc(&op->d_lam, sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMalloc(&op->d_nan, sizeof(int))) != hipSuccess)
+        (e = hipMalloc(&op->d_nan, sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&op->d_alpha, (size_t)op->NE * sizeof(double))) != hipSuccess)
         return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
     if (phys->viscous) {
         const size_t nface = op->nn / op->np;  // points per face
@@ -715,6 +781,7 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_hcount);
     (void)hipFree(op->d_lam);
     (void)hipFree(op->d_nan);
+    (void)hipFree(op->d_alpha);
     for (double *p : op->tape_ckpt) (void)hipFree(p);
     release_adjoint_storage(op);
     delete op;
@@ -805,6 +872,27 @@ int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, 
     return PRANDTL_OK;
 }
 
+int prandtl_blend_coefficients(prandtl_op op, const double *u, double *alpha, double *eta, prandtl_stream stream) {
+    if (!op || !u || (!alpha && !eta)) return fail(PRANDTL_ERR_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (op->dim == 2) {
+        switch (op->np) {
+            case 2: return launch_blend_coef<2, 2>(op, u, alpha, eta, st);
+            case 3: return launch_blend_coef<2, 3>(op, u, alpha, eta, st);
+            case 4: return launch_blend_coef<2, 4>(op, u, alpha, eta, st);
+            case 5: return launch_blend_coef<2, 5>(op, u, alpha, eta, st);
+        }
+    } else {
+        switch (op->np) {
+            case 2: return launch_blend_coef<3, 2>(op, u, alpha, eta, st);
+            case 3: return launch_blend_coef<3, 3>(op, u, alpha, eta, st);
+            case 4: return launch_blend_coef<3, 4>(op, u, alpha, eta, st);
+            case 5: return launch_blend_coef<3, 5>(op, u, alpha, eta, st);
+        }
+    }
+    return fail(PRANDTL_ERR_UNSUPPORTED, "order out of range");
+}
+
 int prandtl_filter_constraints(prandtl_op op, const double *u, prandtl_stream stream) {
     if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
     int rc = run_hook(op, HOOK_CONSTRAINTS, const_cast<double *>(u), (hipStream_t)stream);   // reads u only
@@ -881,6 +969,7 @@ int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double 
     if (scheme != PRANDTL_LSRK45 && scheme != PRANDTL_SSPRK3) return fail(PRANDTL_ERR_ARG, "unknown scheme");
     if (op->opts.stage_hooks)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the filter or limiter hooks");
+    if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
     if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
     hipStream_t st = (hipStream_t)stream;
     const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;

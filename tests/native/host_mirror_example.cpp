@@ -11,6 +11,8 @@
 //          <dir>/ssp.bin     u0 after 10 SSPRK3Solver steps of dt
 //          <dir>/lsrk.bin    u0 after 4 LSRK45Solver steps of dt (one Run call)
 //          <dir>/hooks.bin   bad after EntropyFilter (constraints from u0) and PositivityLimiter
+//          <dir>/blend.bin   Mult(u0) of an operator with subcell finite-volume blending
+//          <dir>/alpha.bin   its blending coefficients (one per element)
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -75,6 +77,17 @@ int main(int argc, char **argv) {
         limiter.LimitSolution(w);
         prandtl::Synchronize(op);
         save(dir + "/hooks.bin", w.HostCopy());
+
+        // subcell finite-volume blending (the reference's SUBCELL_FV_BLENDING build)
+        prandtl::Options bo;
+        bo.blending = true;
+        prandtl::DGSEMOperator bop(mesh, phys, bo);
+        prandtl::Vector alpha((size_t)mesh.num_elements), eta((size_t)mesh.num_elements);
+        w.CopyFromHost(load(dir + "/u0.bin", n).data());
+        bop.Mult(w, dudt);
+        bop.ComputeBlendingCoefficient(w, alpha, eta);
+        save(dir + "/blend.bin", dudt.HostCopy());
+        save(dir + "/alpha.bin", alpha.HostCopy());
 
         // errors surface as exceptions carrying the library's message
         try {

@@ -1,8 +1,9 @@
 """The C++ host mirror (include/prandtl_amd.hpp) driving the library from a plain g++ program
 (tests/native/host_mirror_example.cpp): DGSEMOperator::Mult, SSPRK3Solver Init/Step,
-LSRK45Solver Run, EntropyFilter and PositivityLimiter, each result against the oracle
-(RHS 64 eps x rounding scale entrywise; 10 SSP-RK3 and 4 LSRK4(5) steps and the hooks to
-1e-12 relative)."""
+LSRK45Solver Run, EntropyFilter and PositivityLimiter, and an operator with subcell
+finite-volume blending, each result against the oracle (RHS 64 eps x rounding scale
+entrywise; 10 SSP-RK3 and 4 LSRK4(5) steps and the hooks to 1e-12 relative; the blending
+coefficients to 1e-9 and the blended RHS, fed them, 64 eps x rounding scale)."""
 import os
 import subprocess
 import tempfile
@@ -58,3 +59,10 @@ def test_host_mirror_program(device):
     err = np.abs(rd("hooks.bin") - hl).max() / np.abs(hl).max()
     print(f"[mirror] hooks ({cnt} elements filtered): {err:.3e}", flush=True)
     assert cnt == 5 and err <= 1e-12
+    a = np.fromfile(os.path.join(d, "alpha.bin"))
+    a_ref, _ = oracle.blend_coef(m, u0)
+    print(f"[mirror] blending: alpha > 0 in {int((a > 0).sum())} elements, max diff {np.abs(a - a_ref).max():.3e}",
+          flush=True)
+    assert (a > 0).any() and np.abs(a - a_ref).max() <= 1e-9
+    ref, _, mag = oracle.rhs_blend(m, ph, u0, a)
+    assert oracle.scaled_error(rd("blend.bin"), ref, mag) <= 64
