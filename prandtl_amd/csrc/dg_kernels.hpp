@@ -34,6 +34,9 @@
 #ifndef PRANDTL_NODE_WAVES
 #define PRANDTL_NODE_WAVES 4
 #endif
+#ifndef PRANDTL_LIFT_WAVES
+#define PRANDTL_LIFT_WAVES 4   // 128 VGPRs: at 3D p=3 FP64, 4 waves/SIMD beat 3 without the cap
+#endif
 
 
 namespace prandtl {
@@ -486,13 +489,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
     double invJ = 0.0;
     if (active) {
-        const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
-        invJ = rcp(gm[0]);
-        R adj[DIM][DIM];
-#pragma unroll
-        for (int r = 0; r < DIM; r++)
-#pragma unroll
-            for (int c = 0; c < DIM; c++) adj[r][c] = (R)gm[1 + r * DIM + c];
+        invJ = rcp(A.geom[(size_t)e * (1 + DIM * DIM)]);
         const double p_ref = MIX ? sPref[eb] : 0.0;
         const double h_ref = MIX ? sHref[eb] : 0.0;
 #pragma unroll
@@ -500,7 +497,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             const int st = dir_stride<NP>(dir);
             const int a = ai[dir];
             const int lb = eb * NN + node - a * st;
-            const R *m = adj[dir];
+            const R *m = sM[eb][dir];
 #pragma unroll 1
             for (int k = 1; k < NP; k++) {
                 const int b = (a + k) % NP;
@@ -528,7 +525,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     qb.beta = sPd[1][li];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
-                    volume_flux<DIM>(qo, qb, m, A.P.gammaM1Inverse, F);
+                    volume_flux<DIM, true>(qo, qb, m, A.P.gammaM1Inverse, F);
                 }
                 const R dd = sT[a * NP + b];
 #pragma unroll
@@ -653,7 +650,7 @@ struct LiftCfg {
 };
 
 template <int DIM, int NP, bool MIX>
-__global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK)) k_lift(const LiftArgs<NP> A) {
+__global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES) k_lift(const LiftArgs<NP> A) {
     using C = LiftCfg<DIM, NP>;
     // derivative contractions, chain rule and viscous flux in R; entropy variables in FP64.
     // Mixed precision contracts w - w_ref (w_ref = the element's node-0 value; D annihilates

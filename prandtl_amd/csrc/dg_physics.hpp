@@ -92,17 +92,40 @@ __device__ __forceinline__ double inv_log_mean(double x, double y, double r) {
     return il;
 }
 
+// 1 / L(x, y) and L(x, y) together, given r = 1/(x+y).  On the series branch
+// L = (x+y)/2 / S(u), S = 1 + u/3 + u^2/5 + u^3/7, and 1/S is itself a series,
+// 1 - u/3 - 4u^2/45 - 44u^3/945 + O(u^4) (the omitted term is below 1e-17 relative for
+// u < 1e-4), so L needs no reciprocal there.
+__device__ __forceinline__ void log_mean_both(double x, double y, double r, double &il, double &l) {
+    const double f = (y - x) * r;
+    const double u = f * f;
+    if (u < 1e-4) {
+        il = 2.0 * r * (1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0))));
+        l = 0.5 * (x + y) * (1.0 - u * (1.0 / 3.0 + u * (4.0 / 45.0 + u * (44.0 / 945.0))));
+    } else {
+        il = log(y * rcp(x)) * rcp(y - x);
+        l = rcp(il);
+    }
+}
+
 // ChandrashekarFlux::ComputeVolumeFlux -- ref: src/RiemannSolvers/ChandrashekarFlux.cpp:11-83.
-// m is the (already averaged) metric row Ja^dir; F receives DIM+2 values.
-template <int DIM>
+// m is the (already averaged) metric row Ja^dir; F receives DIM+2 values.  LN_SERIES: the
+// logarithmic mean of the density from log_mean_both (one reciprocal fewer; the line kernel,
+// whose threads hold whole lines, keeps the reciprocal to stay within its registers).
+template <int DIM, bool LN_SERIES = false>
 __device__ __forceinline__ void volume_flux(const NodePrim<DIM> &a, const NodePrim<DIM> &b, const double *m,
                                             double gammaM1Inverse, double *F) {
     const double rsum = a.rho + b.rho;
     const double r_rho = rcp(rsum);
     const double r_b = rcp(a.beta + b.beta);
-    const double il_rho = inv_log_mean(a.rho, b.rho, r_rho);
+    double il_rho, rho_ln;
+    if constexpr (LN_SERIES) {
+        log_mean_both(a.rho, b.rho, r_rho, il_rho, rho_ln);
+    } else {
+        il_rho = inv_log_mean(a.rho, b.rho, r_rho);
+        rho_ln = rcp(il_rho);
+    }
     const double il_b = inv_log_mean(a.beta, b.beta, r_b);
-    const double rho_ln = rcp(il_rho);
     double mean[DIM];
     double h_hat = 0.0, Vn = 0.0;
 #pragma unroll
@@ -147,9 +170,9 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
         const double rsum = q1.rho + q2.rho;
         const double r_rho = rcp(rsum);
         const double r_b = rcp(q1.beta + q2.beta);
-        const double il_rho = inv_log_mean(q1.rho, q2.rho, r_rho);
+        double il_rho, rho_ln;
+        log_mean_both(q1.rho, q2.rho, r_rho, il_rho, rho_ln);
         const double il_b = inv_log_mean(q1.beta, q2.beta, r_b);
-        const double rho_ln = rcp(il_rho);
         const double rho_mean = 0.5 * rsum;
         const double drho = q2.rho - q1.rho;
         double vm[DIM];
