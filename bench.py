@@ -44,7 +44,7 @@ BYTES_PER_NODE = {"lift": 120.0, "rhs_mult": 160.0, "rhs_lsrk": 240.0}
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (two runs; counter KiB x 1024, the
 # factor a plain copy of known size reads back): 210.7 MB + 167.8 MB.  Recorded in
 # profiles/pmc_tgv32_p3_fp64_mixed.md; PMC counters cannot be read inside this process.
-RHS_LSRK_TRAFFIC_TGV32_P3 = 378.4e6
+RHS_LSRK_TRAFFIC_TGV32_P3 = 378.5e6
 
 
 def log(msg):
@@ -389,18 +389,18 @@ def main():
     # ---- per-kernel timing (HIP events on the launch stream) -------------------
     lift_ms, rhs_ms = op.profile(ud, epilogue=1, iters=20)
     lift_mult_ms, rhs_mult_ms = op.profile(ud, epilogue=0, iters=20)
-    dom = "k_rhs" if rhs_ms >= lift_ms else "k_lift"
+    dom = "k_rhs_node" if rhs_ms >= lift_ms else "k_lift"
     dom_ms = max(rhs_ms, lift_ms)
-    dom_bytes = nodes * (BYTES_PER_NODE["rhs_lsrk"] if dom == "k_rhs" else BYTES_PER_NODE["lift"])
+    dom_bytes = nodes * (BYTES_PER_NODE["rhs_lsrk"] if dom == "k_rhs_node" else BYTES_PER_NODE["lift"])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": f"{dom} (LSRK4(5) stage epilogue)" if dom == "k_rhs" else dom,
+        "kernel": f"{dom} (LSRK4(5) stage epilogue)" if dom == "k_rhs_node" else dom,
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": (RHS_LSRK_TRAFFIC_TGV32_P3 if (dom == "k_rhs" and args.n == 32 and args.order == 3) else None),
+        "traffic": (RHS_LSRK_TRAFFIC_TGV32_P3 if (dom == "k_rhs_node" and args.n == 32 and args.order == 3) else None),
         "traffic_source": "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, profiles/pmc_tgv32_p3_fp64_mixed.md",
         "kernel_ms": {"k_lift": round(lift_ms, 4), "k_rhs_lsrk": round(rhs_ms, 4),
                       "k_rhs_mult": round(rhs_mult_ms, 4)},
