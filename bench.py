@@ -283,6 +283,97 @@ def khi_case(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def blend_case(args, dev):
+    """Subcell finite-volume blending (the reference's SUBCELL_FV_BLENDING build):
+    (a) the default TGV 32^3 p=3 NS workload with blending on: every RHS evaluation computes the
+        Persson-Peraire coefficients; on this smooth state alpha = 0 in every element, so the
+        FV work is skipped and the figure is the coefficient kernel's overhead;
+    (b) a periodic 2D Sod shock tube, Euler, [0,1] x [0,1/8], 512 x 64 quads p=3 (524,288 nodes):
+        rho, p = 1, 1 for |x - 1/2| < 1/4 and 0.125, 0.1 outside, at rest; SSP-RK3 at CFL 0.3 to
+        t = 0.1 with blending, the two shocks and rarefactions running outward.  Reported: the
+        RHS rate with blending, the troubled-element fraction, positivity and conservation."""
+    import numpy as np
+    import torch
+
+    from prandtl_amd import capi, cases
+
+    out = {"metric": "DOF-RHS evals/s with subcell finite-volume blending (Persson-Peraire alpha every RHS)"}
+    # (a) TGV with blending on
+    mesh = cases.tgv_mesh(n=32, order=3)
+    u0 = cases.tgv_state(mesh)
+    ud = torch.from_numpy(u0).to(dev)
+    lam = cases.max_wave_speed(u0)
+    dt = cases.cfl_dt(mesh, lam, 0.5)
+    rates = {}
+    for blend in (False, True):
+        op = capi.Operator(mesh, blending=blend, **cases.tgv_physics())
+        uu = ud.clone()
+        op.rk_run(capi.LSRK45, uu, 0.0, dt, 5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.rk_run(capi.LSRK45, uu, 0.0, dt, 50)
+        torch.cuda.synchronize()
+        rates[blend] = mesh.num_nodes * 5 * 50 / (time.perf_counter() - t0)
+        if blend:
+            a, _ = op.blend_coefficients(uu)
+            out["tgv_alpha_positive_elements"] = int((a > 0).sum().item())
+        op.close()
+    out["tgv32_p3_ns_blend_off"] = rates[False]
+    out["tgv32_p3_ns_blend_on"] = rates[True]
+    out["tgv32_blend_overhead"] = rates[False] / rates[True] - 1.0
+    log(f"TGV 32^3 NS: blending off {rates[False] / 1e9:.3f}, on {rates[True] / 1e9:.3f} G DOF-RHS/s")
+    # (b) 2D Sod shock tube
+    from prandtl_amd import mesh as M
+
+    m = M.periodic_lattice(2, 3, (512, 64), lengths=[1.0, 0.125])
+    X = m.node_coords()
+    inside = np.abs(X[0] - 0.5) < 0.25
+    rho = np.where(inside, 1.0, 0.125)
+    p = np.where(inside, 1.0, 0.1)
+    u0 = cases.prim_to_conserv(rho, [np.zeros_like(rho), np.zeros_like(rho)], p)
+    op = capi.Operator(m, blending=True)
+    ud = torch.from_numpy(u0).to(dev)
+    t, steps, t_final = 0.0, 0, 0.1
+    rhs_time = 0.0
+    alpha_pos = []
+    while t < t_final - 1e-12:
+        lam = op.max_char_speed(ud)
+        dt = cases.cfl_dt(m, lam, 0.3)
+        k = 10
+        if t + k * dt > t_final:
+            k = max(1, int(math.ceil((t_final - t) / dt)))
+            dt = (t_final - t) / k
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.rk_run(capi.SSPRK3, ud, t, dt, k, k)
+        torch.cuda.synchronize()
+        rhs_time += time.perf_counter() - t0
+        t += k * dt
+        steps += k
+        a, _ = op.blend_coefficients(ud)
+        alpha_pos.append(float((a > 0).float().mean().item()))
+    u = ud.cpu().numpy()
+    W = m.quadrature_weights()
+    rho, vel, p = cases.conserv_to_prim(u)
+    mscale = float((W * np.sqrt(u[1] ** 2 + u[2] ** 2)).sum())
+    out["sod2d"] = {
+        "workload": f"periodic 2D Sod tube [0,1]x[0,1/8], 512x64 quads p=3 ({m.num_nodes} nodes), SSP-RK3 CFL 0.3 "
+                    f"to t={t_final}",
+        "steps": steps, "dof_rhs_per_s": m.num_nodes * 3 * steps / rhs_time,
+        "troubled_fraction_mean": float(np.mean(alpha_pos)), "troubled_fraction_max": float(np.max(alpha_pos)),
+        "rho_min": float(rho.min()), "p_min": float(p.min()), "finite": bool(np.isfinite(u).all()),
+        # change of total mass, momentum and energy over the run: mass and energy relative to their
+        # totals, both momentum components relative to the final |momentum| total (the tube starts
+        # at rest, and the y-momentum stays zero by symmetry)
+        "conservation_rel": [float(abs((W * (u[v] - u0[v])).sum()) / scale) for v, scale in
+                             enumerate([(W * u0[0]).sum(), mscale, mscale, (W * u0[3]).sum()])],
+    }
+    log(f"Sod 2D: {steps} steps, {out['sod2d']['dof_rhs_per_s'] / 1e9:.3f} G DOF-RHS/s, "
+        f"troubled {out['sod2d']['troubled_fraction_mean']:.3f}")
+    op.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +391,9 @@ def main():
     ap.add_argument("--khi", action="store_true",
                     help="run the Kelvin-Helmholtz case (2D Euler 256^2 p=3, entropy filter + positivity limiter, "
                          "SSP-RK3 to t=3) instead of the default benchmark")
+    ap.add_argument("--blend", action="store_true",
+                    help="run the subcell-blending cases (TGV 32^3 NS with blending on; periodic 2D Sod tube) "
+                         "instead of the default benchmark")
     ap.add_argument("--adjoint", action="store_true",
                     help="run the discrete-adjoint case (TGV Euler 16^3 p=4 + 64 Fourier modes, 200 RK stages, "
                          "checkpoints every 20 steps) instead of the default benchmark")
@@ -326,6 +420,9 @@ def main():
         return
     if args.khi:
         khi_case(args, dev)
+        return
+    if args.blend:
+        blend_case(args, dev)
         return
 
     mesh = cases.tgv_mesh(n=args.n, order=args.order)

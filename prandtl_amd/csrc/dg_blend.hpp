@@ -13,32 +13,112 @@
 // of maximum degree <= p - 1) and |M2|^2 (<= p - 2) are summed per element in a fixed order, so
 // the result is reproducible; then eta = max(1 - |M1|^2/|M|^2, 1 - |M2|^2/|M1|^2) and
 // alpha = 1/(1 + exp(-s (eta - T)/T)), clipped as the reference does.
+//
+// Included by dg_kernels.hpp after its layout helpers (NodeCfg, dir_stride, xcd_block): the
+// blended RHS kernel computes the coefficients of its own elements with blend_coef_block, and
+// k_blend_coef serves prandtl_blend_coefficients.
 #pragma once
-
-#include "dg_kernels.hpp"
 
 namespace prandtl {
 
 template <int NP>
 struct BlendArgs {
-    double Vi[NP][NP];     // V1^-1: Vi[k][a] = w_a P_k(x_a) / sum_b w_b P_k(x_b)^2
+    BlendParams<NP> B;
     const double *u;
     double *alpha;         // out (may be null)
     double *eta;           // out (may be null)
     int NE;
-    double gammaM1;
-    double T;              // modalThreshold 0.5 10^(-1.8 p^(1/4))
-    double sharpness;      // 9.21024
-    double alpha_max, alpha_min;
+    PhysConst P;
 };
+
+// rho p of one state (ComputePressure, BasicOperations.cpp:31-37: V = m / rho,
+// p = (gamma-1)(E - rho V.V / 2))
+template <int DIM>
+__device__ __forceinline__ double rho_p(const double *s, const PhysConst &P) {
+    const double gammaM1 = P.gammaM1;
+    const double inv = 1.0 / s[0];
+    double vv = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const double vd = s[1 + d] * inv;
+        vv += vd * vd;
+    }
+    return s[0] * (gammaM1 * (s[DIM + 1] - 0.5 * s[0] * vv));
+}
+
+// The workgroup's elements (EPB of NN = Np^dim nodes, thread tid = eb NN + node) from q = rho p
+// at the nodes: modes by three sum-factorised passes (double-buffered in sA / sB, one barrier
+// each), the three modal energies summed per element in a fixed order, eta and alpha into
+// sRes[eb] = {alpha, eta}.  Every thread of the workgroup calls it (it contains barriers);
+// sA, sB and sC hold EPB NN doubles each.  sRes is valid after the caller's next barrier.
+template <int DIM, int NP, int EPB>
+__device__ __forceinline__ void blend_coef_block(double q, bool active, int tid, int eb, int node,
+                                                 const BlendParams<NP> &B, double *sA, double *sB, double *sC,
+                                                 double (*sSum)[3], double (*sRes)[2]) {
+    constexpr int NN = ipow(NP, DIM);
+    int ai[3];
+    ai[0] = node % NP;
+    ai[1] = (node / NP) % NP;
+    ai[2] = node / (NP * NP);
+    if (active) sA[tid] = q;
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const double *src = (d & 1) ? sB : sA;
+        double *dst = (d & 1) ? sA : sB;
+        if (active) {
+            const int st = dir_stride<NP>(d);
+            const int a = ai[d];
+            const int lb = tid - a * st;
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < NP; b++) t += B.Vi[a][b] * src[lb + b * st];
+            q = t;
+            if (d + 1 < DIM) dst[tid] = t;
+        }
+        __syncthreads();   // the last pass's reads are done before sA / sB are reused below
+    }
+    // q: the mode of degrees (ai[0], ai[1], ai[2])
+    if (active) {
+        int md = ai[0];
+#pragma unroll
+        for (int d = 1; d < DIM; d++) md = max(md, ai[d]);
+        const double q2 = q * q;
+        sC[tid] = q2;
+        sB[tid] = md <= NP - 2 ? q2 : 0.0;   // modesM1: no degree above p - 1
+        sA[tid] = md <= NP - 3 ? q2 : 0.0;   // modesM2: no degree above p - 2
+    }
+    __syncthreads();
+    if (active && node < 3) {
+        const double *src = node == 0 ? sC : (node == 1 ? sB : sA);
+        double acc = 0.0;
+        for (int i = 0; i < NN; i++) acc += src[eb * NN + i];
+        sSum[eb][node] = acc;
+    }
+    __syncthreads();
+    if (active && node == 0) {
+        const double mm = sSum[eb][0], m1 = sSum[eb][1], m2 = sSum[eb][2];
+        double ind = 1.0 - m1 / mm;
+        const double ind2 = 1.0 - m2 / m1;
+        ind = ind2 > ind ? ind2 : ind;                       // std::max(ind, ind2)
+        double a = 1.0 / (1.0 + exp(-B.sharpness * (ind - B.T) / B.T));
+        if (a < B.alpha_min)
+            a = 0.0;
+        else if (a > (1.0 - B.alpha_min))
+            a = 1.0;
+        a = (B.alpha_max < a) ? B.alpha_max : a;             // std::min(a, alpha_max)
+        sRes[eb][0] = a;
+        sRes[eb][1] = ind;
+    }
+}
 
 template <int DIM, int NP>
 __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_blend_coef(const BlendArgs<NP> A) {
     using C = NodeCfg<DIM, NP>;
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
-    __shared__ double sQ[EPB * NN];
-    __shared__ double sE[3][EPB * NN];
+    __shared__ double sA[EPB * NN], sB[EPB * NN], sC[EPB * NN];
     __shared__ double sSum[EPB][3];
+    __shared__ double sRes[EPB][2];
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -52,64 +132,12 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_blend_coef(const 
         double s[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
-        // ComputePressure (BasicOperations.cpp:31-37): V = m / rho, p = (gamma-1)(E - rho V.V / 2)
-        const double inv = 1.0 / s[0];
-        double vv = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            const double vd = s[1 + d] * inv;
-            vv += vd * vd;
-        }
-        q = s[0] * (A.gammaM1 * (s[NEQ - 1] - 0.5 * s[0] * vv));
+        q = rho_p<DIM>(s, A.P);
     }
-    int ai[3];
-    ai[0] = node % NP;
-    ai[1] = (node / NP) % NP;
-    ai[2] = node / (NP * NP);
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        if (active) sQ[tid] = q;
-        __syncthreads();
-        if (active) {
-            const int st = dir_stride<NP>(d);
-            const int a = ai[d];
-            const int lb = tid - a * st;
-            double t = 0.0;
-#pragma unroll
-            for (int b = 0; b < NP; b++) t += A.Vi[a][b] * sQ[lb + b * st];
-            q = t;
-        }
-        __syncthreads();
-    }
-    if (active) {
-        int md = ai[0];
-#pragma unroll
-        for (int d = 1; d < DIM; d++) md = max(md, ai[d]);
-        const double q2 = q * q;
-        sE[0][tid] = q2;
-        sE[1][tid] = md <= NP - 2 ? q2 : 0.0;   // modesM1: no degree above p - 1
-        sE[2][tid] = md <= NP - 3 ? q2 : 0.0;   // modesM2: no degree above p - 2
-    }
-    __syncthreads();
-    if (active && node < 3) {
-        double acc = 0.0;
-        for (int i = 0; i < NN; i++) acc += sE[node][eb * NN + i];
-        sSum[eb][node] = acc;
-    }
-    __syncthreads();
+    blend_coef_block<DIM, NP, EPB>(q, active, tid, eb, node, A.B, sA, sB, sC, sSum, sRes);
     if (active && node == 0) {
-        const double mm = sSum[eb][0], m1 = sSum[eb][1], m2 = sSum[eb][2];
-        double ind = 1.0 - m1 / mm;
-        const double ind2 = 1.0 - m2 / m1;
-        ind = ind2 > ind ? ind2 : ind;                       // std::max(ind, ind2)
-        double a = 1.0 / (1.0 + exp(-A.sharpness * (ind - A.T) / A.T));
-        if (a < A.alpha_min)
-            a = 0.0;
-        else if (a > (1.0 - A.alpha_min))
-            a = 1.0;
-        a = (A.alpha_max < a) ? A.alpha_max : a;             // std::min(a, alpha_max)
-        if (A.alpha) A.alpha[e] = a;
-        if (A.eta) A.eta[e] = ind;
+        if (A.alpha) A.alpha[e] = sRes[eb][0];
+        if (A.eta) A.eta[e] = sRes[eb][1];
     }
 }
 

@@ -70,6 +70,14 @@ struct NodeCfg {
 };
 
 template <int NP>
+struct BlendParams {
+    double Vi[NP][NP];     // V1^-1: Vi[k][a] = w_a P_k(x_a) / sum_b w_b P_k(x_b)^2
+    double T;              // modalThreshold 0.5 10^(-1.8 p^(1/4))
+    double sharpness;      // 9.21024
+    double alpha_max, alpha_min;
+};
+
+template <int NP>
 struct DTables {
     double D[NP][NP];      // D_ij (DGSEMIntegrator.cpp:45-69)
     double Dhat[NP][NP];   // D + e0 e0^T / w0 - eN eN^T / wN  (:71-74)
@@ -101,7 +109,8 @@ struct RhsArgs {
     double ca, cb, dt;
     unsigned long long *lam_bits;
     int *nan_flag;
-    const double *alpha;   // blending coefficient per element (k_rhs_node<..., BLEND = true>)
+    double *alpha;         // k_rhs_node<..., BLEND = true>: the coefficients it computed (may be null)
+    BlendParams<NP> blend; // the same: Persson-Peraire indicator and blending constants
 };
 
 template <int NP>
@@ -186,6 +195,12 @@ __device__ __forceinline__ void finish_block(const RhsArgs<NP> &A, double lam, i
         if ((threadIdx.x & 63) == 0) atomicOr(A.nan_flag, 1);
     }
 }
+
+}  // namespace prandtl
+
+#include "dg_blend.hpp"
+
+namespace prandtl {
 
 // ---------------------------------------------------------------------------
 // k_rhs (line variant): inviscid flux differencing + surface flux (+ viscous divergence
@@ -376,7 +391,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     __shared__ R sF[EPB * FITEMS][NEQ];   // surface flux (with the viscous face average) per face point
     constexpr int SITEMS = (NP - 1) * L;  // subcell interfaces per element and direction
     static_assert(SITEMS <= FITEMS, "subcell fluxes are staged in the surface-flux buffer");
-    __shared__ double sAlpha[BLEND ? EPB : 1];
+    static_assert(!BLEND || EPB * FITEMS * NEQ >= 3 * EPB * NN, "the coefficient scratch fits the surface-flux buffer");
+    __shared__ double sBSum[BLEND ? EPB : 1][3];
+    __shared__ double sAlpha[BLEND ? EPB : 1][2];   // alpha, eta
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -398,10 +415,8 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         const int ee = min(blk * EPB + ib, NE - 1);
         sM[ib][r / DIM][r % DIM] = (R)A.geom[(size_t)ee * (1 + DIM * DIM) + 1 + r];
     }
-    if constexpr (BLEND) {
-        for (int i = tid; i < EPB; i += C::BLOCK) sAlpha[i] = (blk * EPB + i < NE) ? A.alpha[blk * EPB + i] : 0.0;
-    }
     double lam = 0.0;
+    double qrp = 0.0;   // rho p at the node (BLEND)
     if (active) {
         double s[NEQ];
 #pragma unroll
@@ -419,8 +434,18 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             sHref[eb] = A.P.gamma * A.P.gammaM1Inverse * qs.p * qs.irho;
         }
         lam = qs.lam;
+        if constexpr (BLEND) qrp = rho_p<DIM>(s, A.P);
     }
     __syncthreads();
+    if constexpr (BLEND) {
+        // the elements' blending coefficients (DGSEMOperator.cpp:547-549), the surface-flux
+        // buffer as scratch before the face items fill it
+        double *scr = &sF[0][0];
+        blend_coef_block<DIM, NP, EPB>(qrp, active, tid, eb, node, A.blend, scr, scr + EPB * NN, scr + 2 * EPB * NN,
+                                       sBSum, sAlpha);
+        __syncthreads();
+        if (active && node == 0 && A.alpha) A.alpha[e] = sAlpha[eb][0];
+    }
     // surface fluxes: work items (element ib, direction, side, line t) over the whole workgroup,
     // the element's own traces from LDS, the neighbour's from HBM
     for (int it = tid; it < EPB * FITEMS; it += C::BLOCK) {
@@ -540,13 +565,20 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             }
         }
     }
-    double fvs[BLEND ? NEQ : 1];
     if constexpr (BLEND) {
+        if (active) {
+            // fold the DG terms now, (1 - alpha) volume + surface, to free registers for the FV
+            // items; acc then collects the RHS and the FV term is added into it
+            const double al = sAlpha[eb][0], sw = A.T.inv_w0 * invJ;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) fvs[v] = 0.0;
+            for (int v = 0; v < NEQ; v++) {
+                acc[v] = (-acc[v] * invJ) * (1.0 - al) + accs[v] * sw;
+                accs[v] = 0.0;
+            }
+        }
         bool any = false;
 #pragma unroll
-        for (int ib = 0; ib < EPB; ib++) any |= sAlpha[ib] != 0.0;
+        for (int ib = 0; ib < EPB; ib++) any |= (blk * EPB + ib < NE) && sAlpha[ib][0] != 0.0;
         if (any) {   // the same value in every thread of the workgroup
 #pragma unroll 1
             for (int dir = 0; dir < DIM; dir++) {
@@ -554,7 +586,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 const int st = dir_stride<NP>(dir);
                 for (int it = tid; it < EPB * SITEMS; it += C::BLOCK) {
                     const int ib = it / SITEMS, r = it - ib * SITEMS;
-                    if (blk * EPB + ib >= NE || sAlpha[ib] == 0.0) continue;
+                    if (blk * EPB + ib >= NE || sAlpha[ib][0] == 0.0) continue;
                     const int i = r / L, t = r - i * L;
                     const int nl = ib * NN + line_base<NP>(t, dir) + i * st;
                     double sl[NEQ], sr[NEQ];
@@ -576,16 +608,16 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
                 }
                 __syncthreads();
-                if (active && sAlpha[eb] != 0.0) {
-                    const int a = ai[dir];
+                if (active && sAlpha[eb][0] != 0.0) {
+                    const int a = (node / st) % NP;   // not ai[dir]: dir is not unrolled here
                     const int it = eb * SITEMS + line_index<NP>(node, dir);
-                    const double iw = A.T.inv_w[a];
+                    const double sc = sAlpha[eb][0] * invJ * A.T.inv_w[a];   // alpha / (J w_a)
 #pragma unroll
                     for (int v = 0; v < NEQ; v++) {
                         double g = 0.0;
                         if (a > 0) g += sF[it + (a - 1) * L][v];
                         if (a < NP - 1) g -= sF[it + a * L][v];
-                        fvs[v] += g * iw;
+                        acc[v] += g * sc;
                     }
                 }
             }
@@ -600,8 +632,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         for (int v = 0; v < NEQ; v++) {
             R du;
             if constexpr (BLEND) {
-                const double al = sAlpha[eb];
-                du = (-acc[v] * rinvJ) * (1.0 - al) + al * (fvs[v] * invJ) + accs[v] * sw;
+                du = acc[v];
             } else {
                 du = -acc[v] * rinvJ + accs[v] * sw;
             }

@@ -13,7 +13,6 @@
 #include <vector>
 
 #include "../../include/prandtl_amd.h"
-#include "dg_blend.hpp"
 #include "dg_hooks.hpp"
 #include "dg_kernels.hpp"
 
@@ -170,8 +169,8 @@ DTables<NP> make_tables(const prandtl_op_s *op) {
 // gamma_k = sum_b w_b P_k(x_b)^2 (the GLL rule integrates P_k P_l exactly for k + l <= 2 Np - 3 and
 // the discrete products vanish for k != l <= Np - 1, so Vi V1 = I).
 template <int NP>
-BlendArgs<NP> make_blend_args(const prandtl_op_s *op) {
-    BlendArgs<NP> B{};
+BlendParams<NP> make_blend_params(const prandtl_op_s *op) {
+    BlendParams<NP> B{};
     double x[NP], w[NP], V[NP][NP];
     gll_rule(NP, x, w);
     for (int i = 0; i < NP; i++) {
@@ -185,8 +184,6 @@ BlendArgs<NP> make_blend_args(const prandtl_op_s *op) {
         for (int b = 0; b < NP; b++) g += w[b] * V[b][k] * V[b][k];
         for (int a = 0; a < NP; a++) B.Vi[k][a] = w[a] * V[a][k] / g;
     }
-    B.NE = op->NE;
-    B.gammaM1 = op->phys.gamma - 1.0;
     B.T = 0.5 * std::pow(10.0, -1.8 * std::pow((double)op->order, 0.25));   // DGSEMOperator.cpp:25
     B.sharpness = 9.21024;                                                    // DGSEMOperator.hpp:50
     B.alpha_max = op->opts.alpha_max;
@@ -196,10 +193,13 @@ BlendArgs<NP> make_blend_args(const prandtl_op_s *op) {
 
 template <int DIM, int NP>
 int launch_blend_coef(prandtl_op_s *op, const double *u, double *alpha, double *eta, hipStream_t st) {
-    BlendArgs<NP> B = make_blend_args<NP>(op);
+    BlendArgs<NP> B{};
+    B.B = make_blend_params<NP>(op);
     B.u = u;
     B.alpha = alpha;
     B.eta = eta;
+    B.NE = op->NE;
+    B.P = op->P;
     using C = NodeCfg<DIM, NP>;
     const int nblk = (op->NE + C::EPB - 1) / C::EPB;
     hipLaunchKernelGGL((k_blend_coef<DIM, NP>), dim3(nblk), dim3(C::BLOCK), 0, st, B);
@@ -268,9 +268,9 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.alpha = op->d_alpha;
     if constexpr (!MIX) {
         if (op->opts.blending) {
-            // the coefficients of this RHS evaluation from its input state (DGSEMOperator.cpp:547-549)
-            int rc = launch_blend_coef<DIM, NP>(op, S.u, op->d_alpha, nullptr, st);
-            if (rc) return rc;
+            // the kernel computes this evaluation's coefficients from its input state
+            // (DGSEMOperator.cpp:547-549) and blends
+            ra.blend = make_blend_params<NP>(op);
             using LC = NodeCfg<DIM, NP>;
             const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
             hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
