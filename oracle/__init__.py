@@ -23,6 +23,31 @@ SURF_CHANDRASHEKAR, SURF_EC, SURF_RUSANOV = 0, 1, 2
 BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
 
 
+# boundary-condition types (BdrFaceIntegrator subclasses); q as listed in include/prandtl_amd.h
+BC_SLIP_WALL, BC_SYMMETRY, BC_SUPERSONIC_INFLOW, BC_SUPERSONIC_OUTFLOW, BC_SPECIFIED_STATE = 1, 2, 3, 4, 5
+BC_RIEMANN_INVARIANT, BC_SUBSONIC_INFLOW_RV, BC_SUBSONIC_INFLOW_PTTT, BC_SUBSONIC_OUTFLOW_P = 6, 7, 8, 9
+BC_NOSLIP_ADIABATIC, BC_NOSLIP_ISOTHERMAL = 10, 11
+
+
+class OrcBc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("q", ctypes.c_double * 8)]
+
+
+def bc(type_: int, *q) -> OrcBc:
+    b = OrcBc()
+    b.type = int(type_)
+    for i, v in enumerate(q):
+        b.q[i] = float(v)
+    return b
+
+
+def _bcs(bcs):
+    if not bcs:
+        return 0, None
+    arr = (OrcBc * len(bcs))(*bcs)
+    return len(bcs), arr
+
+
 class OrcPhys(ctypes.Structure):
     _fields_ = [
         ("gamma", ctypes.c_double),
@@ -87,6 +112,23 @@ def _configure(L):
     L.orc_rk_run_hooked.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, ip]
     L.orc_rk_run_hooked.restype = ctypes.c_int
+    bp = ctypes.POINTER(OrcBc)
+    L.orc_rhs_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int, bp, dp, dp, dp, dp,
+                             dp, dp, dp]
+    L.orc_rhs_ex.restype = ctypes.c_int
+    L.orc_rk_run_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int, bp, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_double, dp, ip]
+    L.orc_rk_run_ex.restype = ctypes.c_int
+    L.orc_bc_outer_state.argtypes = [ctypes.c_int, pp, bp, dp, dp, dp]
+    L.orc_bc_outer_state.restype = ctypes.c_int
+    L.orc_bc_inviscid_flux.argtypes = [ctypes.c_int, pp, bp, dp, dp, dp]
+    L.orc_bc_inviscid_flux.restype = ctypes.c_double
+    for name in ("orc_bc_viscous_flux",):
+        getattr(L, name).argtypes = [ctypes.c_int, pp, bp, dp, dp, dp, dp]
+        getattr(L, name).restype = None
+    L.orc_bc_lifting_flux.argtypes = [ctypes.c_int, pp, bp, dp, dp, dp]
+    L.orc_bc_lifting_flux.restype = None
     L.orc_blend_coef.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, dp, dp]
     L.orc_blend_coef.restype = ctypes.c_int
@@ -354,3 +396,70 @@ def rk_run_blend(mesh, ph, u, scheme: int, nsteps: int, dt: float, hooks: int = 
                                 float(alpha_max), float(alpha_min), _dp(u), _ip(counts))
     assert rc == 0, rc
     return u, counts
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions (prandtl_oracle.cpp)
+def rhs_ex(mesh, ph, u, bcs=None, alpha=None, want_grad=False, fma=False):
+    """The general RHS with boundary conditions (and blending): (dudt, lambda, mag[, grad, gmag])."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    dudt = np.zeros_like(u)
+    mag = np.zeros_like(u)
+    grad = np.zeros((mesh.dim,) + u.shape) if want_grad else None
+    gmag = np.zeros((mesh.dim,) + u.shape) if want_grad else None
+    if alpha is not None:
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    lam = ctypes.c_double(0.0)
+    nbc, arr = _bcs(bcs)
+    rc = lib(fma).orc_rhs_ex(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+                             ctypes.byref(ph), nbc, arr, _dp(u), _dp(alpha), _dp(dudt), _dp(grad), ctypes.byref(lam),
+                             _dp(mag), _dp(gmag))
+    assert rc == 0, rc
+    if want_grad:
+        return dudt, lam.value, mag, grad, gmag
+    return dudt, lam.value, mag
+
+
+def rk_run_ex(mesh, ph, u, scheme: int, nsteps: int, dt: float, bcs=None, hooks: int = 0, blend=False,
+              alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN):
+    """The general stepper (boundary conditions, blending, stage hooks): (u, counts)."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    counts = np.zeros(2, dtype=np.int32)
+    nbc, arr = _bcs(bcs)
+    rc = lib().orc_rk_run_ex(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+                             ctypes.byref(ph), nbc, arr, int(scheme), int(nsteps), float(dt), int(hooks), int(blend),
+                             float(alpha_max), float(alpha_min), _dp(u), _ip(counts))
+    assert rc == 0, rc
+    return u, counts
+
+
+def bc_outer_state(dim, ph, b, s1, nor):
+    """The BC's outer (ghost) state for the trace s1 and outward normal nor."""
+    s1, nor = (np.ascontiguousarray(a, dtype=np.float64) for a in (s1, nor))
+    s2 = np.zeros(dim + 2)
+    assert lib().orc_bc_outer_state(dim, ctypes.byref(ph), ctypes.byref(b), _dp(s1), _dp(nor), _dp(s2)) == 0
+    return s2
+
+
+def bc_inviscid_flux(dim, ph, b, s1, nor):
+    """(f, wave speed) of the BC's inviscid flux at one face point (outward normal nor)."""
+    s1, nor = (np.ascontiguousarray(a, dtype=np.float64) for a in (s1, nor))
+    f = np.zeros(dim + 2)
+    lam = lib().orc_bc_inviscid_flux(dim, ctypes.byref(ph), ctypes.byref(b), _dp(s1), _dp(nor), _dp(f))
+    return f, lam
+
+
+def bc_viscous_flux(dim, ph, b, s1, dq, nor):
+    """F_v . nor of the BC at one face point; dq: (dim, neq) primitive gradients."""
+    s1, dq, nor = (np.ascontiguousarray(a, dtype=np.float64) for a in (s1, dq, nor))
+    fv = np.zeros(dim + 2)
+    lib().orc_bc_viscous_flux(dim, ctypes.byref(ph), ctypes.byref(b), _dp(s1), _dp(dq), _dp(nor), _dp(fv))
+    return fv
+
+
+def bc_lifting_flux(dim, ph, b, w1, nor):
+    """The BC's BR1 lifting flux at one face point (w1: the element's entropy variables)."""
+    w1, nor = (np.ascontiguousarray(a, dtype=np.float64) for a in (w1, nor))
+    fl = np.zeros(dim + 2)
+    lib().orc_bc_lifting_flux(dim, ctypes.byref(ph), ctypes.byref(b), _dp(w1), _dp(nor), _dp(fl))
+    return fl

@@ -451,14 +451,270 @@ void DMatrices(int np, const double *x, const double *w, double *D, double *Dhat
 }
 
 // ---------------------------------------------------------------------------
+// Boundary conditions -- ref: src/Integrators/BoundaryConditions/ (the BdrFaceIntegrator
+// hierarchy, one integrator per boundary attribute, Simulation.cpp:505-800), with constant
+// boundary values.  nor is the element's outward face normal (CalcOrtho, scaled by the face
+// area), state1 the element's trace.  Each BC supplies
+//   ComputeBdrFaceInviscidFlux  -- the surface flux f(state1, state2; nor) and a wave speed;
+//   ComputeBdrFaceViscousFlux   -- F_v . nor at the face (Navier-Stokes);
+//   ComputeBdrFaceLiftingFlux   -- the BR1 face term (entropy variables), default
+//                                  1/2 (w(state2) - w1) (BdrFaceIntegrator.cpp:275-282).
+// The slip wall and symmetry BCs override only the inviscid flux; their viscous and lifting
+// fluxes reach the base class's mfem_error (Euler only in the reference).
+// ---------------------------------------------------------------------------
+enum BcType {
+    BC_SLIP_WALL = 1, BC_SYMMETRY = 2, BC_SUPERSONIC_INFLOW = 3, BC_SUPERSONIC_OUTFLOW = 4,
+    BC_SPECIFIED_STATE = 5, BC_RIEMANN_INVARIANT = 6, BC_SUBSONIC_INFLOW_RV = 7,
+    BC_SUBSONIC_INFLOW_PTTT = 8, BC_SUBSONIC_OUTFLOW_P = 9, BC_NOSLIP_ADIABATIC = 10,
+    BC_NOSLIP_ISOTHERMAL = 11
+};
+
+// ComputePressure -- BasicOperations.cpp:31-37 (V = m/rho as Vector /= rho, i.e. times 1/rho)
+double PressureOf(const Phys &P, const double *s) {
+    const double inv = 1.0 / s[0];
+    double vv = 0.0;
+    for (int d = 0; d < P.dim; d++) vv += (s[1 + d] * inv) * (s[1 + d] * inv);
+    return P.gammaM1 * (s[P.neq - 1] - 0.5 * s[0] * vv);
+}
+
+void NormalizeVec(int dim, double *v) {   // Normalize, BasicOperations.hpp:39-42
+    double n = 0.0;
+    for (int d = 0; d < dim; d++) n += v[d] * v[d];
+    n = std::sqrt(n);
+    for (int d = 0; d < dim; d++) v[d] /= n;
+}
+
+// Normal (BasicOperations.cpp:221-235), Cross (:211-219), RotateState (:238-259): the momentum
+// in the frame (n, t1[, n x t1]).
+void RotateStateRef(int dim, double *state, const double *n) {
+    double t1[3];
+    if (dim == 2) {
+        t1[0] = -n[1];
+        t1[1] = n[0];
+    } else {
+        const double r = n[1] / n[0];
+        t1[0] = std::abs(n[0]) > 1e-10 ? -r / std::sqrt(1.0 + std::pow(r, 2)) : -1.0;
+        t1[1] = std::abs(n[0]) > 1e-10 ? 1.0 / std::sqrt(1.0 + std::pow(r, 2)) : 0.0;
+        t1[2] = 0.0;
+    }
+    double *m = state + 1;
+    double mu = 0.0, mv = 0.0;
+    for (int d = 0; d < dim; d++) {
+        mu += m[d] * n[d];
+        mv += m[d] * t1[d];
+    }
+    if (dim == 3) {
+        const double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+        m[2] = m[0] * t2[0] + m[1] * t2[1] + m[2] * t2[2];
+    }
+    m[0] = mu;
+    m[1] = mv;
+}
+
+// ComputeOuterInviscidState of the BCs that define an outer state.
+void OuterState(const Phys &P, const orc_bc &bc, const double *s1, const double *nor, double *s2) {
+    const int dim = P.dim, neq = P.neq;
+    switch (bc.type) {
+        case BC_SUPERSONIC_INFLOW:    // SupersonicInflowBdrFaceIntegrator.cpp:34-48
+        case BC_SPECIFIED_STATE:      // SpecifiedStateBdrFaceIntegrator.cpp:26-40
+            for (int v = 0; v < neq; v++) s2[v] = bc.q[v];
+            break;
+        case BC_SUPERSONIC_OUTFLOW:   // SupersonicOutflowBdrFaceIntegrator.cpp:12-15
+            for (int v = 0; v < neq; v++) s2[v] = s1[v];
+            break;
+        case BC_RIEMANN_INVARIANT: {  // RiemannInvariantBdrFaceIntegrator.cpp:44-117
+            double un[3], Vo[3], Vi[3];
+            for (int d = 0; d < dim; d++) un[d] = nor[d];
+            NormalizeVec(dim, un);
+            const double rho_o = bc.q[0], p_o = bc.q[neq - 1];
+            for (int d = 0; d < dim; d++) Vo[d] = bc.q[1 + d];
+            const double inv = 1.0 / s1[0];
+            double Vn_i = 0.0, Vn_o = 0.0;
+            for (int d = 0; d < dim; d++) {
+                Vi[d] = s1[1 + d] * inv;
+                Vn_i += Vi[d] * un[d];
+                Vn_o += Vo[d] * un[d];
+            }
+            const double p_i = PressureOf(P, s1);
+            const double a_o = SoundSpeed(p_o, rho_o, P.gamma), a_i = SoundSpeed(p_i, s1[0], P.gamma);
+            const double gI = P.gammaM1Inverse;
+            const double R_minus = (std::abs(Vn_o) >= a_o && Vn_i >= 0.0) ? Vn_i - 2.0 * a_i * gI : Vn_o - 2.0 * a_o * gI;
+            const double R_plus = (std::abs(Vn_o) >= a_o && Vn_i < 0.0) ? Vn_o + 2.0 * a_o * gI : Vn_i + 2.0 * a_i * gI;
+            const double Vn_b = 0.5 * (R_plus + R_minus);
+            const double a_b = 0.25 * P.gammaM1 * (R_plus - R_minus);
+            const double dVn_i = Vn_b - Vn_i, dVn_o = Vn_b - Vn_o;
+            const double gInv = 1.0 / P.gamma;
+            s2[0] = (Vn_i < 0.0) ? std::pow(a_b * a_b * gInv * rho_o / p_o, gI) * rho_o
+                                 : std::pow(a_b * a_b * gInv * s1[0] / p_i, gI) * s1[0];
+            double e = 0.0;
+            for (int d = 0; d < dim; d++) {
+                s2[1 + d] = (Vn_i >= 0.0) ? s2[0] * (Vi[d] + dVn_i * un[d]) : s2[0] * (Vo[d] + dVn_o * un[d]);
+                e += s2[1 + d] * s2[1 + d];
+            }
+            const double p_b = a_b * a_b * s2[0] * gInv;
+            s2[neq - 1] = e * (0.5 / s2[0]) + p_b * gI;
+            break;
+        }
+        case BC_SUBSONIC_INFLOW_RV: { // SubsonicInflowRVBdrFaceIntegrator.cpp:22-52
+            const double r = bc.q[0];
+            double dke = 0.0, uu = 0.0;
+            s2[0] = r;
+            s2[neq - 1] = s1[neq - 1];
+            for (int d = 0; d < dim; d++) {
+                s2[1 + d] = r * bc.q[1 + d];
+                dke -= s1[1 + d] * s1[1 + d];
+                uu += bc.q[1 + d] * bc.q[1 + d];
+            }
+            dke *= 0.5 / s1[0];
+            dke += 0.5 * uu * r;
+            s2[neq - 1] += dke;
+            break;
+        }
+        case BC_SUBSONIC_INFLOW_PTTT: {  // SubsonicInflowPtTtAngBdrFaceIntegrator.cpp:48-73
+            const double pt = bc.q[0], Tt = bc.q[1], theta = bc.q[2], phi = bc.q[3], cp = bc.q[4];
+            double Vc[3] = {std::cos(theta * M_PI / 180.0), std::sin(theta * M_PI / 180.0), 0.0};
+            if (dim > 2) {
+                // the reference writes cos(phi) to V_comps(3), outside the 3-vector (:19, :39); the
+                // intended z component is V_comps(2)
+                Vc[0] *= std::sin(phi * M_PI / 180.0);
+                Vc[1] *= std::sin(phi * M_PI / 180.0);
+                Vc[2] = std::cos(phi * M_PI / 180.0);
+            }
+            const double p = PressureOf(P, s1);
+            double V_sq = 2.0 * cp * Tt * (1.0 - std::pow(p / pt, P.gammaM1 / P.gamma));
+            V_sq = std::max(0.0, V_sq);
+            s2[0] = (P.gamma * P.gammaM1Inverse) * p / (cp * Tt - 0.5 * V_sq);
+            for (int d = 0; d < dim; d++) s2[1 + d] = s2[0] * std::sqrt(V_sq) * Vc[d];
+            s2[neq - 1] = p * P.gammaM1Inverse + 0.5 * s2[0] * V_sq;
+            break;
+        }
+        case BC_SUBSONIC_OUTFLOW_P: {  // SubsonicOutflowPBdrFaceIntegrator.cpp:17-28
+            const double inv = 1.0 / s1[0];
+            double vv = 0.0;
+            for (int d = 0; d < dim; d++) vv += (s1[1 + d] * inv) * (s1[1 + d] * inv);
+            for (int v = 0; v < neq; v++) s2[v] = s1[v];
+            s2[neq - 1] = bc.q[0] * P.gammaM1Inverse + 0.5 * s1[0] * vv;
+            break;
+        }
+        default:
+            for (int v = 0; v < neq; v++) s2[v] = s1[v];
+    }
+}
+
+// ComputeBdrFaceInviscidFlux: the slip wall's (SlipWallBdrFaceIntegrator.cpp:15-57, inherited by
+// the no-slip walls), the symmetry BC's (SymmetryBdrFaceIntegrator.cpp:10-32), otherwise the
+// surface flux with the outer state (BdrFaceIntegrator.cpp:245-253).
+double BdrInviscidFlux(const Phys &P, const orc_bc &bc, const double *s1, const double *nor, double *flux) {
+    const int dim = P.dim, neq = P.neq;
+    double s2[MAXEQ];
+    if (bc.type == BC_SLIP_WALL || bc.type == BC_NOSLIP_ADIABATIC || bc.type == BC_NOSLIP_ISOTHERMAL) {
+        double un[3];
+        for (int d = 0; d < dim; d++) un[d] = nor[d];
+        NormalizeVec(dim, un);
+        for (int v = 0; v < neq; v++) s2[v] = s1[v];
+        RotateStateRef(dim, s2, un);
+        // Conserv2Prim (BasicOperations.cpp:172-186)
+        double prim[MAXEQ];
+        prim[0] = s2[0];
+        for (int d = 0; d < dim; d++) prim[1 + d] = s2[1 + d] / s2[0];
+        prim[neq - 1] = PressureOf(P, s2);
+        double vv = 0.0;
+        for (int d = 0; d < dim; d++) vv += prim[1 + d] * prim[1 + d];
+        const double an = SoundSpeed(prim[neq - 1], prim[0], P.gamma);
+        const double gP1 = 1.0 + P.gamma, gP1I = 1.0 / gP1;
+        double p_star;
+        if (prim[1] > 0.0)
+            p_star = prim[neq - 1] + 0.25 * prim[1] * gP1 * prim[0] *
+                                         (prim[1] + std::sqrt(std::pow(prim[1], 2) +
+                                                              8.0 * gP1I / prim[0] * prim[neq - 1] * (P.gammaM1 * gP1I + 1.0)));
+        else
+            p_star = prim[neq - 1] *
+                     std::pow(std::max(1.0 + 0.5 * P.gammaM1 * prim[1] / an, 0.0001), 2.0 * P.gamma * P.gammaM1Inverse);
+        for (int v = 0; v < neq; v++) flux[v] = 0.0;
+        for (int d = 0; d < dim; d++) flux[1 + d] = p_star * nor[d];
+        return std::sqrt(vv) + an;
+    }
+    if (bc.type == BC_SYMMETRY) {
+        double un[3], mn = 0.0;
+        for (int d = 0; d < dim; d++) un[d] = nor[d];
+        NormalizeVec(dim, un);
+        for (int v = 0; v < neq; v++) s2[v] = s1[v];
+        for (int d = 0; d < dim; d++) mn += s2[1 + d] * un[d];
+        for (int d = 0; d < dim; d++) s2[1 + d] += -2.0 * mn * un[d];
+        return FaceFlux(P, s1, s2, nor, flux);
+    }
+    OuterState(P, bc, s1, nor, s2);
+    return FaceFlux(P, s1, s2, nor, flux);
+}
+
+// ComputeBdrFaceViscousFlux: F_v . nor (row 0 zero); dq[c] the element's primitive gradients.
+void BdrViscousFlux(const Phys &P, const orc_bc &bc, const double *s1, const double (*dq)[MAXEQ], const double *nor,
+                    double *fn) {
+    const int dim = P.dim, neq = P.neq;
+    double s2[MAXEQ], zero[3][MAXEQ] = {}, flux[MAXEQ * 3];
+    const double(*g)[MAXEQ] = dq;
+    const double *sv = s1;
+    switch (bc.type) {
+        case BC_SUPERSONIC_OUTFLOW:    // state2 = state1, the element's gradients
+        case BC_NOSLIP_ADIABATIC:
+        case BC_NOSLIP_ISOTHERMAL:
+            break;
+        case BC_SUBSONIC_INFLOW_PTTT:  // the outer state with the element's gradients (:75-99)
+            OuterState(P, bc, s1, nor, s2);
+            sv = s2;
+            break;
+        default:                       // the outer state with zero gradients
+            OuterState(P, bc, s1, nor, s2);
+            sv = s2;
+            g = zero;
+    }
+    ViscousFlux(P, sv, g[0], g[1], dim > 2 ? g[2] : nullptr, flux);
+    for (int v = 0; v < neq; v++) {
+        double a = 0.0;
+        for (int c = 0; c < dim; c++) a += flux[v * dim + c] * nor[c];   // flux_mat.Mult(nor)
+        fn[v] = a;
+    }
+    if (bc.type == BC_NOSLIP_ADIABATIC) {   // NoSlipAdiabWallBdrFaceIntegrator.cpp:30-44
+        double nn = 0.0;
+        for (int c = 0; c < dim; c++) nn += nor[c] * nor[c];
+        const double qn = bc.q[dim] * std::sqrt(nn);
+        double e = 0.0;
+        for (int d = 0; d < dim; d++) e += bc.q[d] * fn[1 + d];
+        fn[neq - 1] = e + qn;
+    }
+}
+
+// ComputeBdrFaceLiftingFlux: w1 the element's entropy variables at the face point.
+void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const double *w1, const double *nor, double *fn) {
+    const int dim = P.dim, neq = P.neq;
+    if (bc.type == BC_NOSLIP_ADIABATIC || bc.type == BC_NOSLIP_ISOTHERMAL) {
+        // NoSlipAdiabWallBdrFaceIntegrator.cpp:74-95, NoSlipIsothWallBdrFaceIntegrator.cpp:41-66
+        const double v = bc.type == BC_NOSLIP_ADIABATIC ? -w1[neq - 1] : 1.0 / (P.R_gas * bc.q[dim]);
+        fn[0] = w1[0];
+        for (int d = 0; d < dim; d++) fn[1 + d] = bc.q[d] * v;
+        fn[neq - 1] = -v;
+        for (int k = 0; k < neq; k++) fn[k] -= w1[k];
+        return;
+    }
+    // BdrFaceIntegrator.cpp:275-282
+    double s1[MAXEQ], s2[MAXEQ], w2[MAXEQ];
+    Entropy2Conserv(P, w1, s1);
+    OuterState(P, bc, s1, nor, s2);
+    Conserv2Entropy(P, s2, w2);
+    for (int k = 0; k < neq; k++) fn[k] = (w2[k] - w1[k]) * 0.5;
+}
+
+// ---------------------------------------------------------------------------
 // Mesh: conforming affine elements with aligned faces.  neighbors[e*2*dim + 2*d + s]
 // is the element across local face (d, s) (s = 0: xi_d = 0 side, s = 1: xi_d = 1 side).
 // geom[e*(1+dim*dim)] = det A, followed by adj A row-major.
 // ---------------------------------------------------------------------------
 struct Mesh {
     int dim, np, nn, NE;
-    const int *nbr;
+    const int *nbr;                 // neighbour, or -(k+1) for a boundary face with condition bc[k]
     const double *geom;
+    const orc_bc *bc = nullptr;
+    int nbc = 0;
     double x[MAXNP], w[MAXNP], D[MAXNP * MAXNP], Dhat[MAXNP * MAXNP], Dhat2[MAXNP * MAXNP];
     double J(int e) const { return geom[(size_t)e * (1 + dim * dim)]; }
     double adj(int e, int r, int c) const { return geom[(size_t)e * (1 + dim * dim) + 1 + r * dim + c]; }
@@ -754,6 +1010,7 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
 #pragma omp parallel for schedule(static)
         for (int e1 = 0; e1 < NE; e1++) {
             const int e2 = M.nbr[(size_t)e1 * 2 * dim + 2 * d + 1];
+            if (e2 < 0) continue;   // a boundary face (below)
             const double J1 = M.J(e1), J2 = M.J(e2);
             double nor[3];
             for (int c = 0; c < dim; c++) nor[c] = M.adj(e1, d, c);       // CalcOrtho
@@ -774,6 +1031,37 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                             }
                         }
                     }
+                }
+        }
+    }
+    // boundary faces -- LiftingBR1::AssembleLiftingBdrFaceVector (LiftingBR1.cpp:311-407): the
+    // BC's lifting flux f times the outward normal, f n_c / (w0 J), to the element's face point.
+    // Each face belongs to one element, so the elements run in parallel.
+    if (M.bc) {
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < NE; e++) {
+            const double J = M.J(e);
+            for (int d = 0; d < dim; d++)
+                for (int side = 0; side < 2; side++) {
+                    const int nb = M.nbr[(size_t)e * 2 * dim + 2 * d + side];
+                    if (nb >= 0) continue;
+                    const orc_bc &bc = M.bc[-nb - 1];
+                    double nor[3];
+                    for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adj(e, d, c);   // outward
+                    for (int t2 = 0; t2 < Nt2; t2++)
+                        for (int t1 = 0; t1 < Nt1; t1++) {
+                            const int n1 = line_node(M, d, side ? np - 1 : 0, t1, t2);
+                            double w1[MAXEQ], f[MAXEQ];
+                            for (int v = 0; v < neq; v++) w1[v] = went[gidx(M, v, e, n1)];
+                            BdrLiftingFlux(P, bc, w1, nor, f);
+                            for (int v = 0; v < neq; v++)
+                                for (int c = 0; c < dim; c++) {
+                                    grad[(size_t)c * size + gidx(M, v, e, n1)] += 1.0 / (M.w[0] * J) * (f[v] * nor[c]);
+                                    if (gmag)
+                                        gmag[(size_t)c * size + gidx(M, v, e, n1)] +=
+                                            (std::fabs(f[v]) + std::fabs(w1[v])) * std::fabs(nor[c]) / (M.w[0] * std::fabs(J));
+                                }
+                        }
                 }
         }
     }
@@ -905,6 +1193,7 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
 #pragma omp parallel for schedule(static) reduction(max : lam)
         for (int e1 = 0; e1 < NE; e1++) {
             const int e2 = M.nbr[(size_t)e1 * 2 * dim + 2 * d + 1];
+            if (e2 < 0) continue;   // a boundary face (below)
             const double J1 = M.J(e1), J2 = M.J(e2);
             double nor[3], s1[MAXEQ], s2[MAXEQ], fnum[MAXEQ], fl1[MAXEQ * 3], fl2[MAXEQ * 3];
             double dq[3][MAXEQ];
@@ -959,6 +1248,57 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                             mag[gidx(M, v, e2, n2)] += aU[v] / (M.w[0] * std::fabs(J2));
                         }
                     }
+                }
+        }
+    }
+    // boundary faces -- BdrFaceIntegrator::AssembleFaceVector (Euler BdrFaceIntegrator.cpp:68-139,
+    // NS :148-231): dU = -f(state1, nor) [+ F_v . nor], added as dU / (w0 J) at the face point.
+    if (M.bc) {
+#pragma omp parallel for schedule(static) reduction(max : lam)
+        for (int e = 0; e < NE; e++) {
+            const double J = M.J(e);
+            for (int d = 0; d < dim; d++)
+                for (int side = 0; side < 2; side++) {
+                    const int nb = M.nbr[(size_t)e * 2 * dim + 2 * d + side];
+                    if (nb >= 0) continue;
+                    const orc_bc &bc = M.bc[-nb - 1];
+                    double nor[3];
+                    for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adj(e, d, c);
+                    for (int t2 = 0; t2 < Nt2; t2++)
+                        for (int t1 = 0; t1 < Nt1; t1++) {
+                            const int n1 = line_node(M, d, side ? np - 1 : 0, t1, t2);
+                            double s1[MAXEQ], f[MAXEQ], dU[MAXEQ], aU[MAXEQ];
+                            for (int v = 0; v < neq; v++) s1[v] = u[gidx(M, v, e, n1)];
+                            lam = std::max(lam, BdrInviscidFlux(P, bc, s1, nor, f));
+                            for (int v = 0; v < neq; v++) {
+                                dU[v] = -f[v];
+                                aU[v] = std::fabs(f[v]);
+                            }
+                            if (P.viscous) {
+                                double dq[3][MAXEQ], fv[MAXEQ];
+                                for (int c = 0; c < dim; c++)
+                                    for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e, n1)];
+                                BdrViscousFlux(P, bc, s1, dq, nor, fv);
+                                for (int v = 0; v < neq; v++) {
+                                    dU[v] += fv[v];
+                                    if (mag) {
+                                        // rounding scale: the element-side viscous flux's scale (a bound for
+                                        // each BC's flux of the trace and its gradients)
+                                        double gq[3][MAXEQ], fm[MAXEQ * 3];
+                                        for (int c = 0; c < dim; c++)
+                                            for (int k = 0; k < neq; k++) gq[c][k] = gmag[(size_t)c * size + gidx(M, k, e, n1)];
+                                        ViscousFluxMag(P, s1, gq[0], gq[1], dim > 2 ? gq[2] : nullptr, fm);
+                                        double a = std::fabs(fv[v]);
+                                        for (int c = 0; c < dim; c++) a += fm[v * dim + c] * std::fabs(nor[c]);
+                                        aU[v] += a;
+                                    }
+                                }
+                            }
+                            for (int v = 0; v < neq; v++) {
+                                dudt[gidx(M, v, e, n1)] += 1.0 / (M.w[0] * J) * dU[v];
+                                if (mag) mag[gidx(M, v, e, n1)] += aU[v] / (M.w[0] * std::fabs(J));
+                            }
+                        }
                 }
         }
     }
@@ -1096,6 +1436,55 @@ int orc_rhs_mag(int dim, int order, int NE, const int *nbr, const double *geom, 
     std::vector<double> gm_local;
     if (gmag && !grad) return -3;
     double lam = RhsEval(M, P, u, dudt, grad, mag, gmag);
+    if (lambda_max) *lambda_max = lam;
+    return 0;
+}
+
+// One boundary face point: the outer state, the inviscid flux (returns the wave speed), the
+// viscous flux (dq: dim x neq primitive gradients) and the lifting flux (w1: entropy variables).
+int orc_bc_outer_state(int dim, const orc_phys *p, const orc_bc *bc, const double *s1, const double *nor,
+                       double *s2) {
+    Phys P = phys_of(dim, p);
+    OuterState(P, *bc, s1, nor, s2);
+    return 0;
+}
+
+double orc_bc_inviscid_flux(int dim, const orc_phys *p, const orc_bc *bc, const double *s1, const double *nor,
+                            double *f) {
+    Phys P = phys_of(dim, p);
+    return BdrInviscidFlux(P, *bc, s1, nor, f);
+}
+
+void orc_bc_viscous_flux(int dim, const orc_phys *p, const orc_bc *bc, const double *s1, const double *dq,
+                         const double *nor, double *fv) {
+    Phys P = phys_of(dim, p);
+    double g[3][MAXEQ] = {};
+    for (int c = 0; c < dim; c++)
+        for (int v = 0; v < P.neq; v++) g[c][v] = dq[c * P.neq + v];
+    BdrViscousFlux(P, *bc, s1, g, nor, fv);
+}
+
+void orc_bc_lifting_flux(int dim, const orc_phys *p, const orc_bc *bc, const double *w1, const double *nor,
+                         double *fl) {
+    Phys P = phys_of(dim, p);
+    BdrLiftingFlux(P, *bc, w1, nor, fl);
+}
+
+// The general RHS: boundary conditions bcs[0..nbc-1] for the faces whose neighbour entry is
+// -(k+1), blending when alpha is non-null, rounding scales when mag (gmag) are non-null.
+int orc_rhs_ex(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int nbc,
+               const orc_bc *bcs, const double *u, const double *alpha, double *dudt, double *grad,
+               double *lambda_max, double *mag, double *gmag) {
+    if (dim < 2 || dim > 3 || order < 1 || order + 1 > MAXNP) return -1;
+    Mesh M;
+    MakeMesh(M, dim, order, NE, nbr, geom);
+    Phys P = phys_of(dim, p);
+    for (size_t i = 0; i < (size_t)NE * 2 * dim; i++)
+        if (nbr[i] < 0 && (-nbr[i] - 1 >= nbc || !bcs)) return -4;   // a boundary face without its condition
+    M.bc = nbc > 0 ? bcs : nullptr;
+    M.nbc = nbc;
+    if (gmag && !grad) return -3;
+    double lam = RhsEval(M, P, u, dudt, grad, mag, gmag, alpha);
     if (lambda_max) *lambda_max = lam;
     return 0;
 }

@@ -13,9 +13,22 @@ typedef struct {
     int bug_compat;    // bit 0: A1, bit 1: A2, bit 2: A3
 } orc_phys;
 
+// A boundary condition (constant values); the meaning of q by type is listed in
+// include/prandtl_amd.h (prandtl_bc).  A mesh marks a boundary face with neighbour -(k+1), k
+// the index of its condition.
+typedef struct {
+    int type;
+    double q[8];
+} orc_bc;
+
 int orc_gll(int np, double *x, double *w);
 int orc_rhs(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, const double *u,
             double *dudt, double *grad, double *lambda_max);
+// The general RHS: boundary conditions (nbc, bcs), blending (alpha, may be null), rounding
+// scales (mag, gmag, may be null)
+int orc_rhs_ex(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int nbc,
+               const orc_bc *bcs, const double *u, const double *alpha, double *dudt, double *grad,
+               double *lambda_max, double *mag, double *gmag);
 // RHS with subcell finite-volume blending; alpha: one blending coefficient per element
 int orc_rhs_blend_mag(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
                       const double *u, const double *alpha, double *dudt, double *grad, double *lambda_max,

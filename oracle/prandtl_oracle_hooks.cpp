@@ -165,7 +165,10 @@ void FilterConstraints(int dim, int order, int NE, const int *nbr, const double 
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < NE; e++) {
         double m = emin[e];
-        for (int f = 0; f < 2 * dim; f++) m = std::min(m, emin[nbr[(size_t)e * 2 * dim + f]]);
+        for (int f = 0; f < 2 * dim; f++) {
+            const int nb = nbr[(size_t)e * 2 * dim + f];
+            if (nb >= 0) m = std::min(m, emin[nb]);   // boundary faces have no neighbour
+        }
         bound[e] = m;   // entropy->SetSubVector(ent_indices, el_min_e) (:153)
     }
 }
@@ -421,9 +424,21 @@ int orc_rk_run_hooked(int dim, int order, int NE, const int *nbr, const double *
     return orc_rk_run_blend(dim, order, NE, nbr, geom, p, scheme, nsteps, dt, hooks, 0, 0.0, 0.0, u, counts);
 }
 
+int orc_rk_run_ex(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int nbc,
+                  const orc_bc *bcs, int scheme, int nsteps, double dt, int hooks, int blend, double alpha_max,
+                  double alpha_min, double *u, int *counts);
+
 int orc_rk_run_blend(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
                      int scheme, int nsteps, double dt, int hooks, int blend, double alpha_max, double alpha_min,
                      double *u, int *counts) {
+    return orc_rk_run_ex(dim, order, NE, nbr, geom, p, 0, nullptr, scheme, nsteps, dt, hooks, blend, alpha_max,
+                         alpha_min, u, counts);
+}
+
+// The general stepper: boundary conditions, blending, stage hooks.
+int orc_rk_run_ex(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int nbc,
+                  const orc_bc *bcs, int scheme, int nsteps, double dt, int hooks, int blend, double alpha_max,
+                  double alpha_min, double *u, int *counts) {
     if ((dim != 2 && dim != 3) || order < 1 || order > 4) return -1;
     const double gamma = p->gamma;
     const int np = order + 1, n = dim == 2 ? np * np : np * np * np;
@@ -434,10 +449,9 @@ int orc_rk_run_blend(int dim, int order, int NE, const int *nbr, const double *g
         if (hooks & 2) counts[1] += LimiterApply(dim, order, NE, x, gamma, 1e-8);
     };
     auto rhs = [&](const double *x) {
-        if (!blend) return orc_rhs(dim, order, NE, nbr, geom, p, x, k.data(), nullptr, nullptr);
-        BlendingCoefficient(dim, order, NE, x, gamma, alpha_max, alpha_min, alpha.data(), nullptr);
-        return orc_rhs_blend_mag(dim, order, NE, nbr, geom, p, x, alpha.data(), k.data(), nullptr, nullptr, nullptr,
-                                 nullptr);
+        if (blend) BlendingCoefficient(dim, order, NE, x, gamma, alpha_max, alpha_min, alpha.data(), nullptr);
+        return orc_rhs_ex(dim, order, NE, nbr, geom, p, nbc, bcs, x, blend ? alpha.data() : nullptr, k.data(),
+                          nullptr, nullptr, nullptr, nullptr);
     };
     for (int s = 0; s < nsteps; s++) {
         if (hooks & 1) FilterConstraints(dim, order, NE, nbr, u, gamma, bound.data());

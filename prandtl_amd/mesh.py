@@ -115,10 +115,14 @@ class Mesh:
         return float(np.linalg.svd(self.A, compute_uv=False).min())
 
 
-def periodic_lattice(dim: int, order: int, n, lengths=None, A=None, origin=None) -> Mesh:
-    """Periodic lattice of identical affine elements.  With ``lengths`` the
-    elements are axis-aligned boxes (Cartesian mesh of [origin, origin+L]);
-    ``A`` gives a general (sheared) element map instead."""
+def periodic_lattice(dim: int, order: int, n, lengths=None, A=None, origin=None, periodic=None,
+                     boundary=None) -> Mesh:
+    """Lattice of identical affine elements.  With ``lengths`` the elements are
+    axis-aligned boxes (Cartesian mesh of [origin, origin+L]); ``A`` gives a general
+    (sheared) element map instead.  Every direction is periodic unless ``periodic``
+    (one flag per direction) says otherwise; the faces on the lattice's side s of a
+    non-periodic direction d get the neighbour entry -(k+1), k = boundary[(d, s)] the
+    index of their boundary condition."""
     n = tuple(int(v) for v in n)
     assert len(n) == dim
     if A is None:
@@ -135,12 +139,18 @@ def periodic_lattice(dim: int, order: int, n, lengths=None, A=None, origin=None)
         rem //= n[k]
     coords = np.array(coords)
     strides = np.cumprod((1,) + n[:-1])
+    periodic = tuple(True for _ in range(dim)) if periodic is None else tuple(bool(p) for p in periodic)
+    assert len(periodic) == dim
     nbr = np.zeros((NE, 2 * dim), dtype=np.int32)
     for d in range(dim):
         for s, off in ((0, -1), (1, +1)):
             c = coords.copy()
             c[d] = (c[d] + off) % n[d]
             nbr[:, 2 * d + s] = (c * strides[:, None]).sum(axis=0)
+            if not periodic[d]:
+                k = boundary[(d, s)]
+                edge = coords[d] == (0 if s == 0 else n[d] - 1)
+                nbr[edge, 2 * d + s] = -(k + 1)
     det = float(np.linalg.det(A))
     adj = adjugate(A)
     geom = np.zeros((NE, 1 + dim * dim))
