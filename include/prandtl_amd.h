@@ -50,19 +50,50 @@ enum { PRANDTL_BUG_A1 = 1, PRANDTL_BUG_A2 = 2, PRANDTL_BUG_A3 = 4 };
  * and stepped at :1010.  LSRK45 is the Carpenter-Kennedy 2N-storage RK4(3)5. */
 enum { PRANDTL_SSPRK3 = 0, PRANDTL_LSRK45 = 1 };
 
+/* Boundary conditions -- the reference's BdrFaceIntegrator subclasses
+ * (src/Integrators/BoundaryConditions/, chosen per boundary attribute at
+ * src/Simulation/Simulation.cpp:505-800), with constant values q:
+ *   SLIP_WALL          "slip"               (none; Euler only, as in the reference)
+ *   SYMMETRY           "symmetry" / "axis"  (none; Euler only)
+ *   SUPERSONIC_INFLOW  q[0..neq-1] conserved state
+ *   SUPERSONIC_OUTFLOW (none)
+ *   SPECIFIED_STATE    q[0..neq-1] conserved state
+ *   RIEMANN_INVARIANT  q[0..neq-1] primitive far-field state (rho, velocity, p)
+ *   SUBSONIC_INFLOW_RV q[0] rho, q[1..dim] velocity
+ *   SUBSONIC_INFLOW_PTTT q[0] total pressure, q[1] total temperature, q[2] theta, q[3] phi
+ *                      (degrees), q[4] cp
+ *   SUBSONIC_OUTFLOW_P q[0] pressure
+ *   NOSLIP_ADIABATIC   q[0..dim-1] wall velocity, q[dim] normal heat flux
+ *   NOSLIP_ISOTHERMAL  q[0..dim-1] wall velocity, q[dim] wall temperature (with R_gas) */
+enum {
+    PRANDTL_BC_SLIP_WALL = 1, PRANDTL_BC_SYMMETRY = 2, PRANDTL_BC_SUPERSONIC_INFLOW = 3,
+    PRANDTL_BC_SUPERSONIC_OUTFLOW = 4, PRANDTL_BC_SPECIFIED_STATE = 5, PRANDTL_BC_RIEMANN_INVARIANT = 6,
+    PRANDTL_BC_SUBSONIC_INFLOW_RV = 7, PRANDTL_BC_SUBSONIC_INFLOW_PTTT = 8, PRANDTL_BC_SUBSONIC_OUTFLOW_P = 9,
+    PRANDTL_BC_NOSLIP_ADIABATIC = 10, PRANDTL_BC_NOSLIP_ISOTHERMAL = 11
+};
+typedef struct {
+    int type;     /* PRANDTL_BC_* */
+    double q[8];
+} prandtl_bc;
+
 /* Conforming affine mesh with aligned faces.
  *   neighbors[e * 2*dim + 2*d + s]: element across local face (d, s) of e,
- *       s = 0 on the xi_d = 0 side, s = 1 on the xi_d = 1 side;
+ *       s = 0 on the xi_d = 0 side, s = 1 on the xi_d = 1 side; -(k+1) marks a
+ *       boundary face with condition bc[k] (the reference's boundary attributes);
  *   geometry[e * (1 + dim*dim)]: det J, then adj J row-major (MFEM Weight() and
- *       AdjugateJacobian(), DGSEMIntegrator.cpp:254-256).
+ *       AdjugateJacobian(), DGSEMIntegrator.cpp:254-256);
+ *   bc[0..num_bc-1]: the boundary conditions (num_bc = 0 for a periodic mesh).
  * Node (.., Np-1 in d, ..) of e coincides with node (.., 0 in d, ..) of its
- * +d neighbour.  Both arrays are host pointers, copied at create time. */
+ * +d neighbour.  All arrays are host pointers, copied at create time.  Boundary
+ * faces run in FP64 (mixed precision and the adjoint refuse them). */
 typedef struct {
     int dim;
     int order;
     int num_elements;
     const int *neighbors;
     const double *geometry;
+    int num_bc;
+    const prandtl_bc *bc;
 } prandtl_mesh_desc;
 
 /* ref: src/Physics/Physics.hpp:10-38 and NavierStokesFlux(dim, gamma, Pr, mu, ...),

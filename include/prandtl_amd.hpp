@@ -86,8 +86,9 @@ enum class Precision { FP64 = PRANDTL_FP64, Mixed = PRANDTL_MIXED };
 // Conforming affine mesh: per element its 2 dim face neighbours and (det J, adj J).
 struct Mesh {
     int dim = 0, order = 0, num_elements = 0;
-    std::vector<int> neighbors;     // [e][2 d + s]
+    std::vector<int> neighbors;     // [e][2 d + s]; -(k+1): a boundary face with condition bc[k]
     std::vector<double> geometry;   // [e][1 + dim^2]
+    std::vector<prandtl_bc> bc;     // boundary conditions (empty for a periodic mesh)
 
     // Periodic lattice of n[0] x n[1] (x n[2]) axis-aligned boxes of the domain
     // [origin, origin + length] (the generated meshes of the benchmark cases).
@@ -163,7 +164,8 @@ class DGSEMOperator : public TimeDependentOperator {
 public:
     DGSEMOperator(const Mesh &mesh, const Physics &phys, const Options &opts = {})
         : TimeDependentOperator((int)mesh.StateSize()) {
-        prandtl_mesh_desc md{mesh.dim, mesh.order, mesh.num_elements, mesh.neighbors.data(), mesh.geometry.data()};
+        prandtl_mesh_desc md{mesh.dim, mesh.order, mesh.num_elements, mesh.neighbors.data(), mesh.geometry.data(),
+                             (int)mesh.bc.size(), mesh.bc.empty() ? nullptr : mesh.bc.data()};
         prandtl_physics ph{phys.gamma, phys.Pr, phys.mu, phys.R_gas, phys.viscous ? 1 : 0, phys.sutherland ? 1 : 0};
         prandtl_opts o{(int)opts.surface_flux, (int)opts.precision, opts.bug_compat, opts.stage_hooks,
                        opts.blending ? 1 : 0, opts.alpha_max, opts.alpha_min};

@@ -36,9 +36,28 @@ EXPORTED = [
 ALPHA_MAX, ALPHA_MIN = 0.5, 0.001   # the reference's blending defaults
 
 
+# boundary conditions (prandtl_bc); q as listed in include/prandtl_amd.h
+BC_SLIP_WALL, BC_SYMMETRY, BC_SUPERSONIC_INFLOW, BC_SUPERSONIC_OUTFLOW, BC_SPECIFIED_STATE = 1, 2, 3, 4, 5
+BC_RIEMANN_INVARIANT, BC_SUBSONIC_INFLOW_RV, BC_SUBSONIC_INFLOW_PTTT, BC_SUBSONIC_OUTFLOW_P = 6, 7, 8, 9
+BC_NOSLIP_ADIABATIC, BC_NOSLIP_ISOTHERMAL = 10, 11
+
+
+class Bc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("q", ctypes.c_double * 8)]
+
+
+def bc(type_: int, *q) -> Bc:
+    b = Bc()
+    b.type = int(type_)
+    for i, v in enumerate(q):
+        b.q[i] = float(v)
+    return b
+
+
 class MeshDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("num_elements", ctypes.c_int),
-                ("neighbors", ctypes.POINTER(ctypes.c_int)), ("geometry", ctypes.POINTER(ctypes.c_double))]
+                ("neighbors", ctypes.POINTER(ctypes.c_int)), ("geometry", ctypes.POINTER(ctypes.c_double)),
+                ("num_bc", ctypes.c_int), ("bc", ctypes.POINTER(Bc))]
 
 
 class Physics(ctypes.Structure):
@@ -127,14 +146,16 @@ class Operator:
 
     def __init__(self, mesh: Mesh, gamma=1.4, viscous=False, Pr=0.72, mu=0.0, R_gas=287.0, sutherland=False,
                  surface_flux=SURF_CHANDRASHEKAR, precision=FP64, bug_compat=0, stage_hooks=0, blending=False,
-                 alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN):
+                 alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN, bcs=None):
         L = load()
         self.mesh = mesh
         self._nbr = np.ascontiguousarray(mesh.neighbors, dtype=np.int32)
         self._geom = np.ascontiguousarray(mesh.geom, dtype=np.float64)
+        self._bcs = (Bc * len(bcs))(*bcs) if bcs else None
         md = MeshDesc(mesh.dim, mesh.order, mesh.num_elements,
                       self._nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                      self._geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+                      self._geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      len(bcs) if bcs else 0, self._bcs)
         ph = Physics(gamma, Pr, mu, R_gas, int(viscous), int(sutherland))
         op = Opts(int(surface_flux), int(precision), int(bug_compat), int(stage_hooks), int(blending),
                   float(alpha_max), float(alpha_min))

@@ -130,7 +130,10 @@ __global__ void __launch_bounds__(256) k_filter_bound(const HookArgs<NP> A) {
     if (e >= A.NE) return;
     double m = A.emin[e];
 #pragma unroll
-    for (int f = 0; f < 2 * DIM; f++) m = fmin(m, A.emin[A.nbr[(size_t)e * 2 * DIM + f]]);
+    for (int f = 0; f < 2 * DIM; f++) {
+        const int nb = A.nbr[(size_t)e * 2 * DIM + f];
+        if (nb >= 0) m = fmin(m, A.emin[nb]);   // boundary faces have no neighbour
+    }
     A.bound[e] = m;
 }
 

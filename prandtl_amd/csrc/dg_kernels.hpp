@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "dg_adjoint.hpp"
+#include "dg_bc.hpp"
 #include "dg_physics.hpp"
 
 #ifndef PRANDTL_LINE_WAVES
@@ -109,6 +110,7 @@ struct RhsArgs {
     double ca, cb, dt;
     unsigned long long *lam_bits;
     int *nan_flag;
+    const BcDev *bc;       // boundary conditions (neighbour -(k+1) -> bc[k]; k_rhs_node<..., BND = true>)
     double *alpha;         // k_rhs_node<..., BLEND = true>: the coefficients it computed (may be null)
     BlendParams<NP> blend; // the same: Persson-Peraire indicator and blending constants
 };
@@ -126,6 +128,7 @@ struct LiftArgs {
     float *dvf;            // out, FP32 (mixed precision)
     float *Fff;
     double *grad;          // optional out: primitive gradients [c][v][e][node]
+    const BcDev *bc;       // boundary conditions (neighbour -(k+1) -> bc[k])
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD walks one
@@ -373,11 +376,12 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
 // staged in the surface-flux LDS buffer after the DG accumulation has read it; workgroups
 // whose elements all have alpha = 0 skip them.
 // ---------------------------------------------------------------------------
-template <int DIM, int NP, bool VISC, int SURF, bool MIX, bool BLEND = false>
+template <int DIM, int NP, bool VISC, int SURF, bool MIX, bool BLEND = false, bool BND = false>
 __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES) k_rhs_node(const RhsArgs<NP> A) {
     using C = NodeCfg<DIM, NP>;
     using R = typename std::conditional<MIX, float, double>::type;  // flux arithmetic
     static_assert(!(MIX && BLEND), "subcell blending is FP64 only");
+    static_assert(!(MIX && BND), "boundary faces run in FP64");
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
     __shared__ double sPd[2][EPB * NN];   // rho, beta
     __shared__ R sPv[DIM][EPB * NN];      // velocity
@@ -458,6 +462,29 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         const int nown = base + (side ? (NP - 1) * st : 0);
         const int nnb = base + (side ? 0 : (NP - 1) * st);
         const int en = sNbr[ib][2 * dir + side];
+        if constexpr (BND) {
+            if (en < 0) {
+                // boundary face (BdrFaceIntegrator::AssembleFaceVector, Euler :68-139, NS :148-231):
+                // f = side ? fb : -fb along +adj_d, fb the condition's inviscid flux with the
+                // outward normal, minus the viscous trace the lift kernel stored for it
+                double so[NEQ], nor[DIM], fb[NEQ];
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) so[v] = sS[v][ib * NN + nown];
+                const SideState<DIM> qo = make_side<DIM>(so, A.P);
+                const R *m = sM[ib][dir];
+#pragma unroll
+                for (int c = 0; c < DIM; c++) nor[c] = side ? m[c] : -m[c];
+                lam = fmax(lam, bc_inviscid_flux<DIM, SURF>(A.bc[-en - 1], A.P, A.bug3 != 0, so, qo, nor, fb));
+                const double sg = side ? 1.0 : -1.0;
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    double fv = 0.0;
+                    if (VISC && v > 0) fv = A.Ff[ftrace<DIM, NP>(dir, side, v - 1, ee, t, NE)];
+                    sF[it][v] = sg * fb[v] - fv;
+                }
+                continue;
+            }
+        }
         double so[NEQ], sn[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
@@ -680,8 +707,9 @@ struct LiftCfg {
     static constexpr int EPB = BLOCK / NN;
 };
 
-template <int DIM, int NP, bool MIX>
+template <int DIM, int NP, bool MIX, bool BND = false>
 __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES) k_lift(const LiftArgs<NP> A) {
+    static_assert(!(MIX && BND), "boundary faces run in FP64");
     using C = LiftCfg<DIM, NP>;
     // derivative contractions, chain rule and viscous flux in R; entropy variables in FP64.
     // Mixed precision contracts w - w_ref (w_ref = the element's node-0 value; D annihilates
@@ -738,9 +766,25 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
         const int en = sNbr[fe][2 * dir + side];
         const int nnb = line_base<NP>(t, dir) + (side ? 0 : (NP - 1) * st);
         double sn[NEQ], wn[NEQ];
+        if (BND && en < 0) {
+            // boundary face (LiftingBR1.cpp:311-407): the condition's lifting flux f enters the
+            // node's face term 1/2 (wn - w) as wn = w + 2 f, w the element's own trace
+            const int nown = line_base<NP>(t, dir) + (side ? (NP - 1) * st : 0);
+            double so[NEQ], wo[NEQ], nor[DIM], fl[NEQ];
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
-        conserv_to_entropy_ref<DIM>(sn, A.P, sLR[fe], wn);
+            for (int v = 0; v < NEQ; v++) so[v] = A.u[v * vs + (size_t)(e0 + fe) * NN + nown];
+            conserv_to_entropy_ref<DIM>(so, A.P, sLR[fe], wo);
+            const double *gm = A.geom + (size_t)(e0 + fe) * (1 + DIM * DIM) + 1 + dir * DIM;
+#pragma unroll
+            for (int c = 0; c < DIM; c++) nor[c] = side ? gm[c] : -gm[c];
+            bc_lifting_flux<DIM>(A.bc[-en - 1], A.P, wo, so, nor, fl);
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) wn[v] = wo[v] + 2.0 * fl[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
+            conserv_to_entropy_ref<DIM>(sn, A.P, sLR[fe], wn);
+        }
 #pragma unroll
         for (int v = 0; v < NEQ; v++) sU[(v * EPB + fe) * C::FITEMS + i] = (R)(wn[v] - (MIX ? sWref[fe][v] : 0.0));
     }
@@ -822,9 +866,29 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
                 if constexpr (MIX) {
                     if (a == 0) A.Fff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
                     if (a == NP - 1) A.Fff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;
-                } else {
+                } else if constexpr (!BND) {
                     if (a == 0) A.Ff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
                     if (a == NP - 1) A.Ff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;
+                }
+            }
+            if constexpr (BND) {
+                // a face point on a boundary stores the condition's viscous flux (along +adj_d)
+#pragma unroll
+                for (int side = 0; side < 2; side++) {
+                    if (a != (side ? NP - 1 : 0)) continue;
+                    const int nb = sNbr[eb][2 * d + side];
+                    double tr[NEQ - 1];
+                    if (nb < 0) {
+                        double adjd[DIM];
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) adjd[c] = adj[d][c];
+                        bc_viscous_trace<DIM>(A.bc[-nb - 1], A.P, sr, g, adjd, side, Fc[d], tr);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NEQ - 1; v++) tr[v] = Fc[d][v];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NEQ - 1; v++) A.Ff[ftrace<DIM, NP>(d, side, v, e, t, NE)] = tr[v];
                 }
             }
         }

@@ -118,6 +118,8 @@ struct prandtl_op_s {
     PhysConst P{};
     double D[MAX_NP * MAX_NP], Dh[MAX_NP * MAX_NP], Dh2[MAX_NP * MAX_NP], inv_w0 = 0.0;
     double w[MAX_NP] = {};        // GLL weights on [0,1]
+    BcDev *d_bc = nullptr;        // boundary conditions (neighbour -(k+1) -> d_bc[k])
+    int nbc = 0;
     double *d_alpha = nullptr;    // blending: per-element coefficient of the current RHS
     int *d_nbr = nullptr;
     double *d_geom = nullptr;
@@ -237,8 +239,16 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
         la.dvf = reinterpret_cast<float *>(op->d_dv);
         la.Fff = reinterpret_cast<float *>(op->d_Ff);
         la.grad = S.grad;
+        la.bc = op->d_bc;
         const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
-        hipLaunchKernelGGL((k_lift<DIM, NP, MIX>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+        if constexpr (!MIX) {
+            if (op->nbc)
+                hipLaunchKernelGGL((k_lift<DIM, NP, false, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+            else
+                hipLaunchKernelGGL((k_lift<DIM, NP, false>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+        } else {
+            hipLaunchKernelGGL((k_lift<DIM, NP, MIX>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+        }
         HIP_TRY(hipGetLastError());
       }
     }
@@ -266,14 +276,21 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.lam_bits = op->d_lam;
     ra.nan_flag = S.nan_check ? op->d_nan : nullptr;
     ra.alpha = op->d_alpha;
+    ra.bc = op->d_bc;
     if constexpr (!MIX) {
-        if (op->opts.blending) {
-            // the kernel computes this evaluation's coefficients from its input state
-            // (DGSEMOperator.cpp:547-549) and blends
-            ra.blend = make_blend_params<NP>(op);
+        if (op->opts.blending || op->nbc) {
+            // blending: the kernel computes this evaluation's coefficients from its input state
+            // (DGSEMOperator.cpp:547-549) and blends; boundary faces: the conditions' fluxes.
+            // Both run the node kernel (for Euler too).
+            if (op->opts.blending) ra.blend = make_blend_params<NP>(op);
             using LC = NodeCfg<DIM, NP>;
             const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
-            hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            if (op->opts.blending && op->nbc)
+                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            else if (op->opts.blending)
+                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, false>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            else
+                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
             HIP_TRY(hipGetLastError());
             return PRANDTL_OK;
         }
@@ -433,6 +450,7 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
 
 int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
+    if (op->nbc) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic meshes");
     if (op->phys.viscous && op->phys.sutherland)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the Navier-Stokes adjoint covers constant viscosity");
     if (op->opts.surface_flux == PRANDTL_SURF_RUSANOV)
@@ -656,9 +674,14 @@ int validate_mesh(const prandtl_mesh_desc *m) {
         for (int d = 0; d < dim; d++) {
             const int ep = m->neighbors[(size_t)e * nf + 2 * d + 1];
             const int em = m->neighbors[(size_t)e * nf + 2 * d];
-            if (ep < 0 || ep >= NE || em < 0 || em >= NE)
-                return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) +
-                                                  ": every face needs a neighbour (boundary faces are not supported)");
+            for (int nb : {ep, em}) {
+                if (nb >= NE || (nb < 0 && -nb - 1 >= m->num_bc))
+                    return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) +
+                                                      ": a neighbour entry is neither an element nor a boundary condition");
+            }
+            if (em >= 0 && m->neighbors[(size_t)em * nf + 2 * d + 1] != e)
+                return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) + ": neighbour table is not reciprocal");
+            if (ep < 0) continue;   // a boundary face
             if (m->neighbors[(size_t)ep * nf + 2 * d] != e)
                 return fail(PRANDTL_ERR_MESH, "element " + std::to_string(e) + ": neighbour table is not reciprocal");
             // the shared face's area vector must agree from both sides (conforming affine faces)
@@ -704,6 +727,17 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     if (phys->viscous && (o.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
         return fail(PRANDTL_ERR_UNSUPPORTED, "bug_compat A1/A2 are reproduced by the CPU oracle only");
     if (phys->viscous && (!(phys->Pr > 0.0) || phys->mu < 0.0)) return fail(PRANDTL_ERR_ARG, "bad Pr or mu");
+    if (mesh->num_bc < 0 || (mesh->num_bc > 0 && !mesh->bc)) return fail(PRANDTL_ERR_ARG, "bad boundary-condition table");
+    for (int k = 0; k < mesh->num_bc; k++) {
+        const int ty = mesh->bc[k].type;
+        if (ty < PRANDTL_BC_SLIP_WALL || ty > PRANDTL_BC_NOSLIP_ISOTHERMAL)
+            return fail(PRANDTL_ERR_ARG, "unknown boundary-condition type " + std::to_string(ty));
+        if (phys->viscous && (ty == PRANDTL_BC_SLIP_WALL || ty == PRANDTL_BC_SYMMETRY))
+            return fail(PRANDTL_ERR_UNSUPPORTED, "slip and symmetry walls define no viscous or lifting flux "
+                                                 "(the reference's base class aborts for them)");
+    }
+    if (mesh->num_bc > 0 && o.precision != PRANDTL_FP64)
+        return fail(PRANDTL_ERR_UNSUPPORTED, "boundary faces run in FP64");
     int rc = validate_mesh(mesh);
     if (rc) return rc;
 
@@ -750,6 +784,13 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         (e = hipMalloc(&op->d_nan, sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&op->d_alpha, (size_t)op->NE * sizeof(double))) != hipSuccess)
         return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+    if (mesh->num_bc > 0) {
+        static_assert(sizeof(BcDev) == sizeof(prandtl_bc), "BcDev mirrors prandtl_bc");
+        op->nbc = mesh->num_bc;
+        if ((e = hipMalloc(&op->d_bc, (size_t)op->nbc * sizeof(BcDev))) != hipSuccess ||
+            (e = hipMemcpy(op->d_bc, mesh->bc, (size_t)op->nbc * sizeof(BcDev), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+    }
     if (phys->viscous) {
         const size_t nface = op->nn / op->np;  // points per face
         if ((e = hipMalloc(&op->d_dv, (size_t)(op->neq - 1) * op->NE * op->nn * sizeof(double))) != hipSuccess ||
@@ -782,6 +823,7 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_lam);
     (void)hipFree(op->d_nan);
     (void)hipFree(op->d_alpha);
+    (void)hipFree(op->d_bc);
     for (double *p : op->tape_ckpt) (void)hipFree(p);
     release_adjoint_storage(op);
     delete op;
@@ -970,6 +1012,7 @@ int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double 
     if (op->opts.stage_hooks)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the filter or limiter hooks");
     if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
+    if (op->nbc) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic meshes");
     if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
     hipStream_t st = (hipStream_t)stream;
     const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;

@@ -98,3 +98,34 @@ def test_cpp_mirror_compiles_and_links(tmp_path):
 
     exe = build_example(str(tmp_path))
     assert os.path.exists(exe)
+
+
+def _create_bc(m, bcs, **kw):
+    lib = capi.load()
+    nbr = np.ascontiguousarray(m.neighbors, dtype=np.int32)
+    geom = np.ascontiguousarray(m.geom, dtype=np.float64)
+    arr = (capi.Bc * len(bcs))(*bcs)
+    md = capi.MeshDesc(m.dim, m.order, m.num_elements, nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), kw.pop("num_bc", len(bcs)), arr)
+    ph = capi.Physics(1.4, 0.72, kw.get("mu", 0.0), 287.0, kw.get("viscous", 0), 0)
+    op = capi.Opts(0, kw.get("precision", 0), 0, 0)
+    h = ctypes.c_void_p()
+    rc = lib.prandtl_op_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
+    return rc, lib.prandtl_last_error().decode(), h
+
+
+def test_boundary_condition_validation():
+    m = M.periodic_lattice(2, 3, (4, 3), lengths=(1, 1), periodic=(True, False), boundary={(1, 0): 0, (1, 1): 1})
+    wall = capi.bc(capi.BC_SLIP_WALL)
+    # a face that names a condition the table does not hold
+    rc, msg, h = _create_bc(m, [wall], num_bc=1)
+    assert rc == -4 and "boundary condition" in msg and not h.value
+    # an unknown type
+    rc, msg, h = _create_bc(m, [wall, capi.bc(99)])
+    assert rc == -1 and "unknown boundary-condition type" in msg
+    # Navier-Stokes on slip walls (the reference's base class aborts there)
+    rc, msg, h = _create_bc(m, [wall, wall], viscous=1, mu=1e-3)
+    assert rc == -2 and "slip and symmetry" in msg
+    # mixed precision
+    rc, msg, h = _create_bc(m, [wall, wall], precision=capi.MIXED)
+    assert rc == -2 and "FP64" in msg
