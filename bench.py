@@ -374,6 +374,87 @@ def blend_case(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def cavity_case(args, dev):
+    """Lid-driven cavity (the reference's TestCases/NavierStokes/2D/LidDrivenCavity): [0,2]^2,
+    16 x 16 quads p=3, rho = 1 at rest, p = 1/(gamma Ma^2) with Ma = 0.1, mu = 0.02 (Re = 100 on
+    the lid speed 1 and the side 2), Pr = 0.72, R = 287.05, no-slip adiabatic walls, the lid
+    moving at (1, 0), blending on, SSP-RK3 at dt = 1e-4 to t = 3 (30,000 steps) -- and the same
+    physics on 256 x 256 quads (1,048,576 nodes) for throughput, against the periodic mesh of
+    that size."""
+    import numpy as np
+    import torch
+
+    from prandtl_amd import capi, cases
+    from prandtl_amd import mesh as M
+
+    gamma, mach, mu, Pr, R = 1.4, 0.1, 0.02, 0.72, 287.05
+    kw = dict(viscous=True, mu=mu, Pr=Pr, R_gas=R, gamma=gamma)
+    walls = [capi.bc(capi.BC_NOSLIP_ADIABATIC, 0.0, 0.0, 0.0), capi.bc(capi.BC_NOSLIP_ADIABATIC, 1.0, 0.0, 0.0)]
+
+    def make(n, periodic=False):
+        if periodic:
+            return M.periodic_lattice(2, 3, (n, n), lengths=[2.0, 2.0])
+        return M.periodic_lattice(2, 3, (n, n), lengths=[2.0, 2.0], periodic=(False, False),
+                                  boundary={(0, 0): 0, (0, 1): 0, (1, 0): 0, (1, 1): 1})
+
+    def rest(m):
+        one = np.ones((m.num_elements, m.nodes_per_element))
+        return cases.prim_to_conserv(one, [0 * one, 0 * one], one / (gamma * mach * mach), gamma=gamma)
+
+    out = {"metric": "lid-driven cavity, NS with no-slip adiabatic walls (boundary conditions on the GPU)"}
+    # (a) the reference case
+    m = make(16)
+    u0 = rest(m)
+    op = capi.Operator(m, bcs=walls, blending=True, **kw)
+    ud = torch.from_numpy(u0).to(dev)
+    dt, t_final = 1e-4, (args.t_final if args.t_final != 2.0 else 3.0)
+    nsteps = int(round(t_final / dt))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < nsteps:
+        k = min(1000, nsteps - done)
+        op.rk_run(capi.SSPRK3, ud, done * dt, dt, k, k)
+        done += k
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    u = ud.cpu().numpy()
+    rho, vel, p = cases.conserv_to_prim(u, gamma=gamma)
+    X = m.node_coords()
+    W = m.quadrature_weights()
+    centre = np.abs(X[0] - 1.0) < 1e-9          # the vertical centreline x = 1 (element faces)
+    ys, us = X[1][centre], vel[0][centre]
+    out["reference_case"] = {
+        "workload": f"[0,2]^2 16x16 quads p=3 ({m.num_nodes} nodes), Ma 0.1, Re 100, blending on, "
+                    f"SSP-RK3 dt=1e-4 to t={t_final}",
+        "steps": nsteps, "wall_s": wall, "steps_per_s": nsteps / wall,
+        "kinetic_energy": float((W * 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2)).sum()),
+        "mass_change_rel": float(abs((W * (u[0] - u0[0])).sum()) / (W * u0[0]).sum()),
+        "centreline_u_min": float(us.min()), "centreline_y_at_min": float(ys[np.argmin(us)]),
+        "finite": bool(np.isfinite(u).all()),
+    }
+    op.close()
+    log(f"cavity 16x16: {nsteps} steps in {wall:.2f} s, centreline u_min {us.min():.4f} at y={ys[np.argmin(us)]:.3f}")
+    # (b) throughput at 256 x 256, boundaries against periodic
+    for periodic in (False, True):
+        m = make(256, periodic)
+        op = capi.Operator(m, bcs=None if periodic else walls, **kw)
+        ud = torch.from_numpy(rest(m)).to(dev)
+        op.rk_run(capi.SSPRK3, ud, 0.0, 1e-5, 5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.rk_run(capi.SSPRK3, ud, 0.0, 1e-5, 100)
+        torch.cuda.synchronize()
+        rate = m.num_nodes * 3 * 100 / (time.perf_counter() - t0)
+        out["cavity256_p3_ns" + ("_periodic" if periodic else "_walls") + "_dof_rhs_per_s"] = rate
+        op.close()
+    out["boundary_overhead"] = (out["cavity256_p3_ns_periodic_dof_rhs_per_s"] /
+                                out["cavity256_p3_ns_walls_dof_rhs_per_s"] - 1)
+    log(f"256^2 p=3 NS: walls {out['cavity256_p3_ns_walls_dof_rhs_per_s'] / 1e9:.3f}, periodic "
+        f"{out['cavity256_p3_ns_periodic_dof_rhs_per_s'] / 1e9:.3f} G DOF-RHS/s")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,6 +475,9 @@ def main():
     ap.add_argument("--blend", action="store_true",
                     help="run the subcell-blending cases (TGV 32^3 NS with blending on; periodic 2D Sod tube) "
                          "instead of the default benchmark")
+    ap.add_argument("--cavity", action="store_true",
+                    help="run the lid-driven cavity (NS, no-slip walls; the reference's 16x16 case and a 256^2 "
+                         "throughput run) instead of the default benchmark")
     ap.add_argument("--adjoint", action="store_true",
                     help="run the discrete-adjoint case (TGV Euler 16^3 p=4 + 64 Fourier modes, 200 RK stages, "
                          "checkpoints every 20 steps) instead of the default benchmark")
@@ -423,6 +507,9 @@ def main():
         return
     if args.blend:
         blend_case(args, dev)
+        return
+    if args.cavity:
+        cavity_case(args, dev)
         return
 
     mesh = cases.tgv_mesh(n=args.n, order=args.order)
