@@ -120,6 +120,13 @@ def _configure(L):
                                 ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_double, dp, ip]
     L.orc_rk_run_ex.restype = ctypes.c_int
+    L.orc_rhs_gen.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, dp, ip, pp, ctypes.c_int, bp, dp, dp,
+                              dp, dp, dp, dp, dp]
+    L.orc_rhs_gen.restype = ctypes.c_int
+    L.orc_rk_run_gen.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, dp, ip, pp, ctypes.c_int, bp,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, dp, ip]
+    L.orc_rk_run_gen.restype = ctypes.c_int
     L.orc_bc_outer_state.argtypes = [ctypes.c_int, pp, bp, dp, dp, dp]
     L.orc_bc_outer_state.restype = ctypes.c_int
     L.orc_bc_inviscid_flux.argtypes = [ctypes.c_int, pp, bp, dp, dp, dp]
@@ -157,7 +164,21 @@ def _dp(a):
 
 
 def _ip(a):
-    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if a is not None else None
+
+
+def _lattice(mesh):
+    # entry points without the general-mesh arguments take aligned affine lattices only
+    assert not getattr(mesh, "general", False), "a general mesh needs rhs_ex / rk_run_ex"
+    return mesh
+
+
+def _gen(mesh):
+    ng = getattr(mesh, "node_geom", None)
+    fo = getattr(mesh, "face_orient", None)
+    ng = None if ng is None else np.ascontiguousarray(ng, dtype=np.float64)
+    fo = None if fo is None else np.ascontiguousarray(fo, dtype=np.int32)
+    return ng, fo
 
 
 def phys(gamma=1.4, Pr=0.72, mu=0.0, R_gas=287.0, viscous=False, sutherland=False,
@@ -242,7 +263,7 @@ def rhs(mesh, ph, u, want_grad=False, fma=False):
     dudt = np.zeros_like(u)
     grad = np.zeros((mesh.dim,) + u.shape) if want_grad else None
     lam = ctypes.c_double(0.0)
-    rc = lib(fma).orc_rhs(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib(fma).orc_rhs(mesh.dim, mesh.order, mesh.num_elements, _ip(_lattice(mesh).neighbors), _dp(mesh.geom),
                        ctypes.byref(ph), _dp(u), _dp(dudt), _dp(grad), ctypes.byref(lam))
     assert rc == 0, rc
     if want_grad:
@@ -259,7 +280,7 @@ def rhs_mag(mesh, ph, u, want_grad=False):
     grad = np.zeros((mesh.dim,) + u.shape) if want_grad else None
     gmag = np.zeros((mesh.dim,) + u.shape) if want_grad else None
     lam = ctypes.c_double(0.0)
-    rc = lib().orc_rhs_mag(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib().orc_rhs_mag(mesh.dim, mesh.order, mesh.num_elements, _ip(_lattice(mesh).neighbors), _dp(mesh.geom),
                            ctypes.byref(ph), _dp(u), _dp(dudt), _dp(grad), ctypes.byref(lam), _dp(mag), _dp(gmag))
     assert rc == 0, rc
     if want_grad:
@@ -276,7 +297,7 @@ def scaled_error(a, ref, mag) -> float:
 def rk_run(mesh, ph, u, scheme: int, nsteps: int, dt: float, fma=False):
     """Advance u (copy) by nsteps fixed steps; scheme 0 = SSP-RK3, 1 = LSRK4(5)."""
     u = np.array(u, dtype=np.float64, copy=True)
-    rc = lib(fma).orc_rk_run(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib(fma).orc_rk_run(mesh.dim, mesh.order, mesh.num_elements, _ip(_lattice(mesh).neighbors), _dp(mesh.geom),
                           ctypes.byref(ph), int(scheme), int(nsteps), float(dt), _dp(u))
     assert rc == 0, rc
     return u
@@ -343,7 +364,7 @@ def rk_run_hooked(mesh, ph, u, scheme: int, nsteps: int, dt: float, hooks: int):
     stage the filter, then the limiter).  Returns (u, [elements filtered, elements limited])."""
     u = np.array(u, dtype=np.float64, copy=True)
     counts = np.zeros(2, dtype=np.int32)
-    rc = lib().orc_rk_run_hooked(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib().orc_rk_run_hooked(mesh.dim, mesh.order, mesh.num_elements, _ip(_lattice(mesh).neighbors), _dp(mesh.geom),
                                  ctypes.byref(ph), int(scheme), int(nsteps), float(dt), int(hooks), _dp(u),
                                  _ip(counts))
     assert rc == 0, rc
@@ -376,7 +397,7 @@ def rhs_blend(mesh, ph, u, alpha, want_grad=False, fma=False):
     grad = np.zeros((mesh.dim,) + u.shape) if want_grad else None
     gmag = np.zeros((mesh.dim,) + u.shape) if want_grad else None
     lam = ctypes.c_double(0.0)
-    rc = lib(fma).orc_rhs_blend_mag(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib(fma).orc_rhs_blend_mag(mesh.dim, mesh.order, mesh.num_elements, _ip(_lattice(mesh).neighbors), _dp(mesh.geom),
                                     ctypes.byref(ph), _dp(u), _dp(alpha), _dp(dudt), _dp(grad), ctypes.byref(lam),
                                     _dp(mag), _dp(gmag))
     assert rc == 0, rc
@@ -391,7 +412,7 @@ def rk_run_blend(mesh, ph, u, scheme: int, nsteps: int, dt: float, hooks: int = 
     Returns (u, [elements filtered, elements limited])."""
     u = np.array(u, dtype=np.float64, copy=True)
     counts = np.zeros(2, dtype=np.int32)
-    rc = lib().orc_rk_run_blend(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
+    rc = lib().orc_rk_run_blend(mesh.dim, mesh.order, mesh.num_elements, _ip(_lattice(mesh).neighbors), _dp(mesh.geom),
                                 ctypes.byref(ph), int(scheme), int(nsteps), float(dt), int(hooks), 1,
                                 float(alpha_max), float(alpha_min), _dp(u), _ip(counts))
     assert rc == 0, rc
@@ -411,9 +432,10 @@ def rhs_ex(mesh, ph, u, bcs=None, alpha=None, want_grad=False, fma=False):
         alpha = np.ascontiguousarray(alpha, dtype=np.float64)
     lam = ctypes.c_double(0.0)
     nbc, arr = _bcs(bcs)
-    rc = lib(fma).orc_rhs_ex(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
-                             ctypes.byref(ph), nbc, arr, _dp(u), _dp(alpha), _dp(dudt), _dp(grad), ctypes.byref(lam),
-                             _dp(mag), _dp(gmag))
+    ng, fo = _gen(mesh)
+    rc = lib(fma).orc_rhs_gen(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom), _dp(ng),
+                              _ip(fo), ctypes.byref(ph), nbc, arr, _dp(u), _dp(alpha), _dp(dudt), _dp(grad),
+                              ctypes.byref(lam), _dp(mag), _dp(gmag))
     assert rc == 0, rc
     if want_grad:
         return dudt, lam.value, mag, grad, gmag
@@ -426,9 +448,10 @@ def rk_run_ex(mesh, ph, u, scheme: int, nsteps: int, dt: float, bcs=None, hooks:
     u = np.array(u, dtype=np.float64, copy=True)
     counts = np.zeros(2, dtype=np.int32)
     nbc, arr = _bcs(bcs)
-    rc = lib().orc_rk_run_ex(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom),
-                             ctypes.byref(ph), nbc, arr, int(scheme), int(nsteps), float(dt), int(hooks), int(blend),
-                             float(alpha_max), float(alpha_min), _dp(u), _ip(counts))
+    ng, fo = _gen(mesh)
+    rc = lib().orc_rk_run_gen(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom), _dp(ng),
+                              _ip(fo), ctypes.byref(ph), nbc, arr, int(scheme), int(nsteps), float(dt), int(hooks),
+                              int(blend), float(alpha_max), float(alpha_min), _dp(u), _ip(counts))
     assert rc == 0, rc
     return u, counts
 

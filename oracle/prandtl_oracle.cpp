@@ -719,6 +719,28 @@ struct Mesh {
     double J(int e) const { return geom[(size_t)e * (1 + dim * dim)]; }
     double adj(int e, int r, int c) const { return geom[(size_t)e * (1 + dim * dim) + 1 + r * dim + c]; }
     int lex(int i, int j, int k) const { return i + np * (j + np * k); }
+    // general geometry: per-node (det J, adj J) -- the reference evaluates Tr.Weight() and
+    // Tr.AdjugateJacobian() at every node (DGSEMIntegrator.cpp:254-276) -- and per-face
+    // orientation codes (neighbour's local face 2 d' + s', bit 3 flips the first face
+    // coordinate, bit 4 the second, bit 5 swaps them); null: affine elements, aligned faces.
+    const double *ngeom = nullptr;
+    const int *forient = nullptr;
+    double Jn(int e, int node) const {
+        return ngeom ? ngeom[((size_t)e * nn + node) * (1 + dim * dim)] : J(e);
+    }
+    double adjn(int e, int node, int r, int c) const {
+        return ngeom ? ngeom[((size_t)e * nn + node) * (1 + dim * dim) + 1 + r * dim + c] : adj(e, r, c);
+    }
+    int nbface(int e, int f) const { return forient ? (forient[(size_t)e * 2 * dim + f] & 7) : (f ^ 1); }
+    // the neighbour's face coordinates of the point (t1, t2) of face f of e
+    void nbpoint(int e, int f, int t1, int t2, int &u1, int &u2) const {
+        const int code = forient ? forient[(size_t)e * 2 * dim + f] : 0;
+        int a = t1, b = t2;
+        if (code & 32) std::swap(a, b);
+        u1 = (code & 8) ? np - 1 - a : a;
+        u2 = (code & 16) ? np - 1 - b : b;
+        if (dim == 2) u2 = 0;
+    }
 };
 
 inline size_t gidx(const Mesh &M, int v, int e, int node) {
@@ -751,19 +773,19 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
     auto FI = [&](std::vector<double> &T, int id, int v, int m) -> double & {
         return T[((size_t)id * np + m) * neq + v];
     };
-    const double J = M.J(e);
     double s1[MAXEQ], s2[MAXEQ], f[MAXEQ], g[MAXEQ], h[MAXEQ], m1[3], m2[3];
     for (int k = 0; k < Npz; k++)
         for (int j = 0; j < Npy; j++)
             for (int i = 0; i < np; i++) {
                 const int id1 = M.lex(i, j, k);
+                const double J = M.Jn(e, id1);                             // Tr.Weight() at id1 (:254)
                 for (int v = 0; v < neq; v++) s1[v] = el_u[v * n + id1];
-                for (int c = 0; c < dim; c++) m1[c] = M.adj(e, 0, c);
+                for (int c = 0; c < dim; c++) m1[c] = M.adjn(e, id1, 0, c);
                 for (int v = 0; v < neq; v++) FI(Fi, id1, v, i) = 0.0;      // f = 0; SetCol(i, f)
                 for (int m = i + 1; m < np; m++) {
                     const int id2 = M.lex(m, j, k);
                     for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
-                    for (int c = 0; c < dim; c++) m2[c] = M.adj(e, 0, c);
+                    for (int c = 0; c < dim; c++) m2[c] = M.adjn(e, id2, 0, c);
                     lam = std::max(lam, VolumeFlux(P, s1, s2, m1, m2, f));
                     for (int v = 0; v < neq; v++) {
                         FI(Fi, id1, v, m) = f[v];
@@ -771,13 +793,13 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
                     }
                 }
                 if (dim > 1) {
-                    for (int c = 0; c < dim; c++) m1[c] = M.adj(e, 1, c);
+                    for (int c = 0; c < dim; c++) m1[c] = M.adjn(e, id1, 1, c);
                     for (int v = 0; v < neq; v++) g[v] = 0.0;
                     for (int v = 0; v < neq; v++) FI(Gi, id1, v, j) = 0.0;
                     for (int m = j + 1; m < Npy; m++) {
                         const int id2 = M.lex(i, m, k);
                         for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
-                        for (int c = 0; c < dim; c++) m2[c] = M.adj(e, 1, c);
+                        for (int c = 0; c < dim; c++) m2[c] = M.adjn(e, id2, 1, c);
                         lam = std::max(lam, VolumeFlux(P, s1, s2, m1, m2, g));
                         for (int v = 0; v < neq; v++) {
                             FI(Gi, id1, v, m) = g[v];
@@ -785,12 +807,12 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
                         }
                     }
                     if (dim > 2) {
-                        for (int c = 0; c < dim; c++) m1[c] = M.adj(e, 2, c);
+                        for (int c = 0; c < dim; c++) m1[c] = M.adjn(e, id1, 2, c);
                         for (int v = 0; v < neq; v++) FI(Hi, id1, v, k) = 0.0;
                         for (int m = k + 1; m < Npz; m++) {
                             const int id2 = M.lex(i, j, m);
                             for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
-                            for (int c = 0; c < dim; c++) m2[c] = M.adj(e, 2, c);
+                            for (int c = 0; c < dim; c++) m2[c] = M.adjn(e, id2, 2, c);
                             lam = std::max(lam, VolumeFlux(P, s1, s2, m1, m2, h));
                             for (int v = 0; v < neq; v++) {
                                 FI(Hi, id1, v, m) = h[v];
@@ -875,23 +897,23 @@ double ElementSubcellFV(const Mesh &M, const Phys &P, int e, const double *el_u,
                         double *el_mag) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq;
     const int Nt1 = dim > 1 ? np : 1, Nt2 = dim > 2 ? np : 1;
-    const double J = M.J(e);
     double lam = 0.0;
     std::vector<double> fv((size_t)n * neq, 0.0), fm((size_t)n * neq, 0.0);
     for (int d = 0; d < dim; d++) {
-        // subcell normals of the element's lines in direction d (the same for every line: affine)
-        double nor[MAXNP][3];
-        for (int i = 0; i < np; i++) {
-            for (int c = 0; c < dim; c++) nor[i][c] = M.adj(e, d, c);       // nor = metric1
-            for (int l = 0; l < i; l++) {
-                double tmp[3] = {0.0, 0.0, 0.0};
-                for (int m = 0; m < np; m++)
-                    for (int c = 0; c < dim; c++) tmp[c] += M.adj(e, d, c) * M.D[l * np + m];
-                for (int c = 0; c < dim; c++) nor[i][c] += tmp[c] * M.w[l];
-            }
-        }
         for (int t2 = 0; t2 < Nt2; t2++)
             for (int t1 = 0; t1 < Nt1; t1++) {
+                // subcell normals of this line (ComputeSubcellMetrics :1194-1211): metric1 at the
+                // line's first node plus the weighted D-sums of the line's adj rows
+                double nor[MAXNP][3];
+                for (int i = 0; i < np; i++) {
+                    for (int c = 0; c < dim; c++) nor[i][c] = M.adjn(e, line_node(M, d, 0, t1, t2), d, c);
+                    for (int l = 0; l < i; l++) {
+                        double tmp[3] = {0.0, 0.0, 0.0};
+                        for (int m = 0; m < np; m++)
+                            for (int c = 0; c < dim; c++) tmp[c] += M.adjn(e, line_node(M, d, m, t1, t2), d, c) * M.D[l * np + m];
+                        for (int c = 0; c < dim; c++) nor[i][c] += tmp[c] * M.w[l];
+                    }
+                }
                 double dU[MAXEQ], aU[MAXEQ], s1[MAXEQ], s2[MAXEQ], f[MAXEQ];
                 for (int v = 0; v < neq; v++) dU[v] = aU[v] = 0.0;
                 int id1 = line_node(M, d, 0, t1, t2);
@@ -900,6 +922,7 @@ double ElementSubcellFV(const Mesh &M, const Phys &P, int e, const double *el_u,
                     const int id2 = line_node(M, d, i + 1, t1, t2);
                     for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
                     lam = std::max(lam, FaceFlux(P, s1, s2, nor[i + 1], f));
+                    const double J = M.Jn(e, id1);                           // Tr.Weight() at id1
                     const double sc = 1.0 / (J * M.w[i]);                  // Vector /= c multiplies by 1/c
                     for (int v = 0; v < neq; v++) {
                         dU[v] -= f[v];
@@ -911,6 +934,7 @@ double ElementSubcellFV(const Mesh &M, const Phys &P, int e, const double *el_u,
                     for (int v = 0; v < neq; v++) s1[v] = s2[v];
                     id1 = id2;
                 }
+                const double J = M.Jn(e, id1);
                 const double sc = 1.0 / (J * M.w[np - 1]);
                 for (int v = 0; v < neq; v++) {
                     fv[(size_t)v * n + id1] += dU[v] * sc;
@@ -962,12 +986,12 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
     // element lifting
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < NE; e++) {
-        const double J = M.J(e);
         const int Npy = dim > 1 ? np : 1, Npz = dim > 2 ? np : 1;
         for (int k = 0; k < Npz; k++)
             for (int j = 0; j < Npy; j++)
                 for (int i = 0; i < np; i++) {
                     const int id = M.lex(i, j, k);
+                    const double J = M.Jn(e, id);
                     double gm1[MAXEQ][3];                          // grad_mat1 (neq x dim)
                     for (int v = 0; v < neq; v++)
                         for (int c = 0; c < 3; c++) gm1[v][c] = 0.0;
@@ -982,7 +1006,7 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                     for (int v = 0; v < neq; v++)
                         for (int c = 0; c < dim; c++) {
                             double acc = 0.0;
-                            for (int r = 0; r < dim; r++) acc += gm1[v][r] * M.adj(e, r, c);
+                            for (int r = 0; r < dim; r++) acc += gm1[v][r] * M.adjn(e, id, r, c);
                             grad[(size_t)c * size + gidx(M, v, e, id)] = acc * (1.0 / J);
                         }
                     if (gmag) {
@@ -998,7 +1022,7 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                         for (int v = 0; v < neq; v++)
                             for (int c = 0; c < dim; c++) {
                                 double acc = 0.0;
-                                for (int r = 0; r < dim; r++) acc += am[v][r] * std::fabs(M.adj(e, r, c));
+                                for (int r = 0; r < dim; r++) acc += am[v][r] * std::fabs(M.adjn(e, id, r, c));
                                 gmag[(size_t)c * size + gidx(M, v, e, id)] = acc / std::fabs(J);
                             }
                     }
@@ -1006,17 +1030,51 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
     }
     // face lifting: 1/2 (w2 - w1) n_c / (w0 J) added to both sides
     const int Nt1 = dim > 1 ? np : 1, Nt2 = dim > 2 ? np : 1;
-    for (int d = 0; d < dim; d++) {
+    if (M.forient) {
+        // general connectivity: every interior face once, from its lower-numbered side (serial:
+        // a face may meet the neighbour in another direction)
+        for (int e1 = 0; e1 < NE; e1++)
+            for (int f1 = 0; f1 < 2 * dim; f1++) {
+                const int e2 = M.nbr[(size_t)e1 * 2 * dim + f1];
+                const int f2 = M.nbface(e1, f1);
+                if (e2 < 0 || e2 < e1 || (e2 == e1 && f2 < f1)) continue;
+                const int d1 = f1 >> 1, s1 = f1 & 1, d2 = f2 >> 1, s2 = f2 & 1;
+                for (int t2 = 0; t2 < Nt2; t2++)
+                    for (int t1 = 0; t1 < Nt1; t1++) {
+                        int u1, u2;
+                        M.nbpoint(e1, f1, t1, t2, u1, u2);
+                        const int n1 = line_node(M, d1, s1 ? np - 1 : 0, t1, t2), n2 = line_node(M, d2, s2 ? np - 1 : 0, u1, u2);
+                        const double J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
+                        double nor[3];
+                        for (int c = 0; c < dim; c++) nor[c] = (s1 ? 1.0 : -1.0) * M.adjn(e1, n1, d1, c);   // outward from e1
+                        for (int v = 0; v < neq; v++) {
+                            const double f = 0.5 * (went[gidx(M, v, e2, n2)] - went[gidx(M, v, e1, n1)]);
+                            for (int c = 0; c < dim; c++) {
+                                const double fc = f * nor[c];
+                                grad[(size_t)c * size + gidx(M, v, e1, n1)] += 1.0 / (M.w[0] * J1) * fc;
+                                grad[(size_t)c * size + gidx(M, v, e2, n2)] += 1.0 / (M.w[0] * J2) * fc;
+                                if (gmag) {
+                                    const double a = 0.5 * (std::fabs(went[gidx(M, v, e2, n2)]) +
+                                                            std::fabs(went[gidx(M, v, e1, n1)])) * std::fabs(nor[c]);
+                                    gmag[(size_t)c * size + gidx(M, v, e1, n1)] += a / (M.w[0] * std::fabs(J1));
+                                    gmag[(size_t)c * size + gidx(M, v, e2, n2)] += a / (M.w[0] * std::fabs(J2));
+                                }
+                            }
+                        }
+                    }
+            }
+    }
+    for (int d = 0; d < dim && !M.forient; d++) {
 #pragma omp parallel for schedule(static)
         for (int e1 = 0; e1 < NE; e1++) {
             const int e2 = M.nbr[(size_t)e1 * 2 * dim + 2 * d + 1];
             if (e2 < 0) continue;   // a boundary face (below)
-            const double J1 = M.J(e1), J2 = M.J(e2);
             double nor[3];
-            for (int c = 0; c < dim; c++) nor[c] = M.adj(e1, d, c);       // CalcOrtho
             for (int t2 = 0; t2 < Nt2; t2++)
                 for (int t1 = 0; t1 < Nt1; t1++) {
                     const int n1 = line_node(M, d, np - 1, t1, t2), n2 = line_node(M, d, 0, t1, t2);
+                    const double J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
+                    for (int c = 0; c < dim; c++) nor[c] = M.adjn(e1, n1, d, c);   // CalcOrtho
                     for (int v = 0; v < neq; v++) {
                         const double f = 0.5 * (went[gidx(M, v, e2, n2)] - went[gidx(M, v, e1, n1)]);  // subtract(0.5,...) :51
                         for (int c = 0; c < dim; c++) {
@@ -1040,17 +1098,17 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
     if (M.bc) {
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < NE; e++) {
-            const double J = M.J(e);
             for (int d = 0; d < dim; d++)
                 for (int side = 0; side < 2; side++) {
                     const int nb = M.nbr[(size_t)e * 2 * dim + 2 * d + side];
                     if (nb >= 0) continue;
                     const orc_bc &bc = M.bc[-nb - 1];
-                    double nor[3];
-                    for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adj(e, d, c);   // outward
                     for (int t2 = 0; t2 < Nt2; t2++)
                         for (int t1 = 0; t1 < Nt1; t1++) {
                             const int n1 = line_node(M, d, side ? np - 1 : 0, t1, t2);
+                            const double J = M.Jn(e, n1);
+                            double nor[3];
+                            for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adjn(e, n1, d, c);   // outward
                             double w1[MAXEQ], f[MAXEQ];
                             for (int v = 0; v < neq; v++) w1[v] = went[gidx(M, v, e, n1)];
                             BdrLiftingFlux(P, bc, w1, nor, f);
@@ -1153,7 +1211,7 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                 for (int r = 0; r < dim; r++)
                     for (int v = 0; v < neq; v++) {
                         double acc = 0.0;
-                        for (int c = 0; c < dim; c++) acc += M.adj(e, r, c) * flux[v * dim + c];
+                        for (int c = 0; c < dim; c++) acc += M.adjn(e, i, r, c) * flux[v * dim + c];
                         Fvis[((size_t)r * n + i) * neq + v] = acc;
                     }
                 if (mag) {
@@ -1164,7 +1222,7 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                     for (int r = 0; r < dim; r++)
                         for (int v = 0; v < neq; v++) {
                             double acc = 0.0;
-                            for (int c = 0; c < dim; c++) acc += std::fabs(M.adj(e, r, c)) * fm[v * dim + c];
+                            for (int c = 0; c < dim; c++) acc += std::fabs(M.adjn(e, i, r, c)) * fm[v * dim + c];
                             FvisMag[((size_t)r * n + i) * neq + v] = acc;
                         }
                 }
@@ -1186,21 +1244,34 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
             for (int v = 0; v < neq; v++)
                 for (int node = 0; node < n; node++) mag[gidx(M, v, e, node)] = el_mag[(size_t)v * n + node];
     }
-    // interior-face loop.  Faces of one direction touch disjoint node sets of
-    // each element, so a direction's faces run in parallel; directions run in turn.
+    // interior-face loop.  Aligned lattice: faces of one direction touch disjoint node sets
+    // of each element, so a direction's faces run in parallel; directions run in turn.  General
+    // connectivity: every face once, from its lower-numbered side (e1 < e2, or e1 == e2 and the
+    // lower local face), in one serial pass (a face may meet its neighbour in any direction).
     const int Nt1 = dim > 1 ? np : 1, Nt2 = dim > 2 ? np : 1;
-    for (int d = 0; d < dim; d++) {
-#pragma omp parallel for schedule(static) reduction(max : lam)
-        for (int e1 = 0; e1 < NE; e1++) {
-            const int e2 = M.nbr[(size_t)e1 * 2 * dim + 2 * d + 1];
+    const int npass = M.forient ? 1 : dim;
+    for (int d = 0; d < npass; d++) {
+        const int nf = M.forient ? 2 * dim : 1;
+#pragma omp parallel for schedule(static) reduction(max : lam) if (!M.forient)
+        for (int e1 = 0; e1 < NE; e1++)
+            for (int fi = 0; fi < nf; fi++) {
+            const int f1 = M.forient ? fi : 2 * d + 1;
+            const int e2 = M.nbr[(size_t)e1 * 2 * dim + f1];
             if (e2 < 0) continue;   // a boundary face (below)
-            const double J1 = M.J(e1), J2 = M.J(e2);
+            const int f2 = M.nbface(e1, f1);
+            if (M.forient && (e2 < e1 || (e2 == e1 && f2 < f1))) continue;
+            const int d1 = f1 >> 1, sd1 = f1 & 1, d2 = f2 >> 1, sd2 = f2 & 1;
             double nor[3], s1[MAXEQ], s2[MAXEQ], fnum[MAXEQ], fl1[MAXEQ * 3], fl2[MAXEQ * 3];
             double dq[3][MAXEQ];
-            for (int c = 0; c < dim; c++) nor[c] = M.adj(e1, d, c);
             for (int t2 = 0; t2 < Nt2; t2++)
                 for (int t1 = 0; t1 < Nt1; t1++) {
-                    const int n1 = line_node(M, d, np - 1, t1, t2), n2 = line_node(M, d, 0, t1, t2);
+                    int u1, u2;
+                    M.nbpoint(e1, f1, t1, t2, u1, u2);
+                    const int n1 = line_node(M, d1, sd1 ? np - 1 : 0, t1, t2);
+                    const int n2 = line_node(M, d2, sd2 ? np - 1 : 0, u1, u2);
+                    const double J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
+                    // CalcOrtho of the face Jacobian: e1's adj row d1 at the face point, outward
+                    for (int c = 0; c < dim; c++) nor[c] = (sd1 ? 1.0 : -1.0) * M.adjn(e1, n1, d1, c);
                     for (int v = 0; v < neq; v++) {
                         s1[v] = u[gidx(M, v, e1, n1)];
                         s2[v] = u[gidx(M, v, e2, n2)];
@@ -1249,24 +1320,24 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                         }
                     }
                 }
-        }
+            }
     }
     // boundary faces -- BdrFaceIntegrator::AssembleFaceVector (Euler BdrFaceIntegrator.cpp:68-139,
     // NS :148-231): dU = -f(state1, nor) [+ F_v . nor], added as dU / (w0 J) at the face point.
     if (M.bc) {
 #pragma omp parallel for schedule(static) reduction(max : lam)
         for (int e = 0; e < NE; e++) {
-            const double J = M.J(e);
             for (int d = 0; d < dim; d++)
                 for (int side = 0; side < 2; side++) {
                     const int nb = M.nbr[(size_t)e * 2 * dim + 2 * d + side];
                     if (nb >= 0) continue;
                     const orc_bc &bc = M.bc[-nb - 1];
-                    double nor[3];
-                    for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adj(e, d, c);
                     for (int t2 = 0; t2 < Nt2; t2++)
                         for (int t1 = 0; t1 < Nt1; t1++) {
                             const int n1 = line_node(M, d, side ? np - 1 : 0, t1, t2);
+                            const double J = M.Jn(e, n1);
+                            double nor[3];
+                            for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adjn(e, n1, d, c);
                             double s1[MAXEQ], f[MAXEQ], dU[MAXEQ], aU[MAXEQ];
                             for (int v = 0; v < neq; v++) s1[v] = u[gidx(M, v, e, n1)];
                             lam = std::max(lam, BdrInviscidFlux(P, bc, s1, nor, f));
@@ -1472,12 +1543,30 @@ void orc_bc_lifting_flux(int dim, const orc_phys *p, const orc_bc *bc, const dou
 
 // The general RHS: boundary conditions bcs[0..nbc-1] for the faces whose neighbour entry is
 // -(k+1), blending when alpha is non-null, rounding scales when mag (gmag) are non-null.
+int orc_rhs_gen(int dim, int order, int NE, const int *nbr, const double *geom, const double *ngeom,
+                const int *forient, const orc_phys *p, int nbc, const orc_bc *bcs, const double *u,
+                const double *alpha, double *dudt, double *grad, double *lambda_max, double *mag, double *gmag);
+
 int orc_rhs_ex(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int nbc,
                const orc_bc *bcs, const double *u, const double *alpha, double *dudt, double *grad,
                double *lambda_max, double *mag, double *gmag) {
+    return orc_rhs_gen(dim, order, NE, nbr, geom, nullptr, nullptr, p, nbc, bcs, u, alpha, dudt, grad, lambda_max,
+                       mag, gmag);
+}
+
+// orc_rhs_ex on a general mesh: ngeom (per node (det J, adj J), [e][node][1 + dim^2]) replaces
+// the per-element geom when non-null; forient (per face [e][2 dim]: the neighbour's local face
+// 2 d + s in bits 0-2, bit 3 flips the first face coordinate, bit 4 the second, bit 5 swaps
+// them) gives the face matching when non-null, else faces are those of an aligned lattice.
+int orc_rhs_gen(int dim, int order, int NE, const int *nbr, const double *geom, const double *ngeom,
+                const int *forient, const orc_phys *p, int nbc, const orc_bc *bcs, const double *u,
+                const double *alpha, double *dudt, double *grad, double *lambda_max, double *mag, double *gmag) {
     if (dim < 2 || dim > 3 || order < 1 || order + 1 > MAXNP) return -1;
+    if (!geom && !ngeom) return -1;
     Mesh M;
     MakeMesh(M, dim, order, NE, nbr, geom);
+    M.ngeom = ngeom;
+    M.forient = forient;
     Phys P = phys_of(dim, p);
     for (size_t i = 0; i < (size_t)NE * 2 * dim; i++)
         if (nbr[i] < 0 && (-nbr[i] - 1 >= nbc || !bcs)) return -4;   // a boundary face without its condition

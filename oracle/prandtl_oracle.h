@@ -29,6 +29,11 @@ int orc_rhs(int dim, int order, int NE, const int *nbr, const double *geom, cons
 int orc_rhs_ex(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int nbc,
                const orc_bc *bcs, const double *u, const double *alpha, double *dudt, double *grad,
                double *lambda_max, double *mag, double *gmag);
+// orc_rhs_ex on a general mesh: per-node geometry ngeom [e][node][1 + dim^2] (replaces geom
+// when non-null) and face orientation codes forient [e][2 dim] (null: an aligned lattice)
+int orc_rhs_gen(int dim, int order, int NE, const int *nbr, const double *geom, const double *ngeom,
+                const int *forient, const orc_phys *p, int nbc, const orc_bc *bcs, const double *u,
+                const double *alpha, double *dudt, double *grad, double *lambda_max, double *mag, double *gmag);
 // RHS with subcell finite-volume blending; alpha: one blending coefficient per element
 int orc_rhs_blend_mag(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p,
                       const double *u, const double *alpha, double *dudt, double *grad, double *lambda_max,

@@ -435,10 +435,22 @@ int orc_rk_run_blend(int dim, int order, int NE, const int *nbr, const double *g
                          alpha_min, u, counts);
 }
 
-// The general stepper: boundary conditions, blending, stage hooks.
+int orc_rk_run_gen(int dim, int order, int NE, const int *nbr, const double *geom, const double *ngeom,
+                   const int *forient, const orc_phys *p, int nbc, const orc_bc *bcs, int scheme, int nsteps,
+                   double dt, int hooks, int blend, double alpha_max, double alpha_min, double *u, int *counts);
+
 int orc_rk_run_ex(int dim, int order, int NE, const int *nbr, const double *geom, const orc_phys *p, int nbc,
                   const orc_bc *bcs, int scheme, int nsteps, double dt, int hooks, int blend, double alpha_max,
                   double alpha_min, double *u, int *counts) {
+    return orc_rk_run_gen(dim, order, NE, nbr, geom, nullptr, nullptr, p, nbc, bcs, scheme, nsteps, dt, hooks, blend,
+                          alpha_max, alpha_min, u, counts);
+}
+
+// The general stepper: boundary conditions, blending, stage hooks, general meshes (ngeom,
+// forient as in orc_rhs_gen).
+int orc_rk_run_gen(int dim, int order, int NE, const int *nbr, const double *geom, const double *ngeom,
+                   const int *forient, const orc_phys *p, int nbc, const orc_bc *bcs, int scheme, int nsteps,
+                   double dt, int hooks, int blend, double alpha_max, double alpha_min, double *u, int *counts) {
     if ((dim != 2 && dim != 3) || order < 1 || order > 4) return -1;
     const double gamma = p->gamma;
     const int np = order + 1, n = dim == 2 ? np * np : np * np * np;
@@ -450,8 +462,8 @@ int orc_rk_run_ex(int dim, int order, int NE, const int *nbr, const double *geom
     };
     auto rhs = [&](const double *x) {
         if (blend) BlendingCoefficient(dim, order, NE, x, gamma, alpha_max, alpha_min, alpha.data(), nullptr);
-        return orc_rhs_ex(dim, order, NE, nbr, geom, p, nbc, bcs, x, blend ? alpha.data() : nullptr, k.data(),
-                          nullptr, nullptr, nullptr, nullptr);
+        return orc_rhs_gen(dim, order, NE, nbr, geom, ngeom, forient, p, nbc, bcs, x, blend ? alpha.data() : nullptr,
+                           k.data(), nullptr, nullptr, nullptr, nullptr);
     };
     for (int s = 0; s < nsteps; s++) {
         if (hooks & 1) FilterConstraints(dim, order, NE, nbr, u, gamma, bound.data());

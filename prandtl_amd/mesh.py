@@ -1,22 +1,33 @@
-"""Conforming affine element lattices with periodic connectivity.
+"""Element meshes: conforming affine lattices, and general (curved, unstructured) meshes.
 
-The operator library takes a mesh as three arrays (see ``include/prandtl_amd.h``):
+The operator library takes a mesh as arrays (see ``include/prandtl_amd.h``):
 
 * ``neighbors[e, 2*d + s]`` -- element across local face (d, s) of element e,
-  s = 0 on the xi_d = 0 side, s = 1 on the xi_d = 1 side;
+  s = 0 on the xi_d = 0 side, s = 1 on the xi_d = 1 side; -(k+1) on a boundary face
+  with boundary condition k;
 * ``geom[e] = [det A, adj(A) row-major]`` -- the affine map x = x0 + A xi of the
   reference element [0,1]^d (MFEM's ``Weight()`` and ``AdjugateJacobian()``);
-* the aligned-face convention: node (.., Np-1 in d, ..) of e coincides with node
-  (.., 0 in d, ..) of ``neighbors[e, 2*d+1]``.
+* on a lattice, the aligned-face convention: node (.., Np-1 in d, ..) of e coincides
+  with node (.., 0 in d, ..) of ``neighbors[e, 2*d+1]``.
 
-Element numbering is x-fastest (e = ex + nx*(ey + ny*ez)), node numbering is the
-reference's lexicographic GLL order lex = i + Np*(j + Np*k)
+A general mesh adds ``node_geom[e, node] = [det J, adj(J) row-major]`` at every node
+(the reference evaluates Tr.Weight() and Tr.AdjugateJacobian() per node,
+src/Integrators/DGSEMIntegrator/DGSEMIntegrator.cpp:254-276) and
+``face_orient[e, f]``: the neighbour's local face 2 d' + s' (bits 0-2), bit 3 flips
+the first face coordinate, bit 4 the second, bit 5 swaps them -- the point (t1, t2)
+of face f of e is the neighbour's point (u1, u2) with (a, b) = swap ? (t2, t1) :
+(t1, t2), u1 = flip1 ? Np-1-a : a, u2 = flip2 ? Np-1-b : b (face coordinates are the
+two other reference directions in increasing order).
+
+Lattice element numbering is x-fastest (e = ex + nx*(ey + ny*ez)), node numbering is
+the reference's lexicographic GLL order lex = i + Np*(j + Np*k)
 (src/Integrators/DGSEMIntegrator/DGSEMIntegrator.cpp:250), and a state vector is
 stored byNODES: u[v, e, node] (src/Simulation/Simulation.hpp:64).
 """
 from __future__ import annotations
 
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 
@@ -54,11 +65,19 @@ def adjugate(A: np.ndarray) -> np.ndarray:
 class Mesh:
     dim: int
     order: int
-    n: tuple
-    A: np.ndarray
-    origin: np.ndarray
-    neighbors: np.ndarray  # int32 (NE, 2*dim)
-    geom: np.ndarray       # float64 (NE, 1+dim*dim)
+    n: Optional[tuple]              # lattice shape (None: general mesh)
+    A: Optional[np.ndarray]         # lattice element map (None: general mesh)
+    origin: Optional[np.ndarray]
+    neighbors: np.ndarray           # int32 (NE, 2*dim)
+    geom: np.ndarray                # float64 (NE, 1+dim*dim)
+    node_geom: Optional[np.ndarray] = None    # float64 (NE, n, 1+dim*dim): per-node (det J, adj J)
+    face_orient: Optional[np.ndarray] = None  # int32 (NE, 2*dim): face matching codes
+    coords: Optional[np.ndarray] = None       # float64 (dim, NE, n): node coordinates
+    hmin: Optional[float] = None
+
+    @property
+    def general(self) -> bool:
+        return self.node_geom is not None or self.face_orient is not None
 
     @property
     def np1(self) -> int:
@@ -70,7 +89,7 @@ class Mesh:
 
     @property
     def num_elements(self) -> int:
-        return int(np.prod(self.n))
+        return int(self.neighbors.shape[0])
 
     @property
     def num_nodes(self) -> int:
@@ -86,6 +105,8 @@ class Mesh:
 
     def node_coords(self) -> np.ndarray:
         """Physical node coordinates, shape (dim, NE, n)."""
+        if self.coords is not None:
+            return self.coords
         xg, _ = gll_nodes(self.np1)
         Np, d = self.np1, self.dim
         # reference-coordinate index of each node
@@ -108,10 +129,15 @@ class Mesh:
         for _ in range(self.dim - 1):
             W = np.multiply.outer(wg, W).reshape(-1)  # slower index outer
         W = W.reshape(-1)
+        if self.node_geom is not None:
+            return self.node_geom[:, :, 0] * W[None, :]
         return self.geom[:, 0][:, None] * W[None, :]
 
     def h_min(self) -> float:
-        """Smallest singular value of the element map (MFEM GetElementSize(e, 1))."""
+        """Smallest singular value of the element map (MFEM GetElementSize(e, 1)); on a
+        general mesh, of the Jacobians at the element centres."""
+        if self.hmin is not None:
+            return self.hmin
         return float(np.linalg.svd(self.A, compute_uv=False).min())
 
 
@@ -157,3 +183,303 @@ def periodic_lattice(dim: int, order: int, n, lengths=None, A=None, origin=None,
     geom[:, 0] = det
     geom[:, 1:] = adj.reshape(-1)[None, :]
     return Mesh(dim, order, n, A, origin, np.ascontiguousarray(nbr), np.ascontiguousarray(geom))
+
+
+# ---------------------------------------------------------------------------
+# General meshes: per-node metrics from node coordinates, faces matched by their corners.
+
+def gll_derivative(np_: int) -> np.ndarray:
+    """D[i, j] = l_j'(x_i) of the Lagrange basis on the GLL nodes of [0,1]
+    (barycentric form; each row sums to zero)."""
+    x, _ = gll_nodes(np_)
+    c = np.array([np.prod([x[i] - x[j] for j in range(np_) if j != i]) for i in range(np_)])
+    D = np.zeros((np_, np_))
+    for i in range(np_):
+        for j in range(np_):
+            if i != j:
+                D[i, j] = c[i] / (c[j] * (x[i] - x[j]))
+        D[i, i] = -D[i].sum()
+    return D
+
+
+def reference_nodes(dim: int, np_: int) -> np.ndarray:
+    """Reference coordinates of the element's nodes in lexicographic order, shape (dim, n)."""
+    xg, _ = gll_nodes(np_)
+    idx = np.indices((np_,) * dim).reshape(dim, -1)[::-1]   # lex: i fastest
+    return xg[idx]
+
+
+def _ref_derivatives(X: np.ndarray, np_: int) -> np.ndarray:
+    """dX_r/dxi_c at the nodes by the GLL derivative matrix: (dim, dim, NE, n) [r][c]."""
+    dim, NE, n = X.shape
+    D = gll_derivative(np_)
+    Xt = X.reshape((dim, NE) + (np_,) * dim)        # axes (.., k, j, i): reversed reference order
+    J = np.zeros((dim, dim, NE, n))
+    for c in range(dim):
+        ax = 2 + (dim - 1 - c)
+        Jc = np.moveaxis(np.tensordot(D, np.moveaxis(Xt, ax, 0), axes=(1, 0)), 0, ax)
+        J[:, c] = Jc.reshape(dim, NE, n)
+    return J
+
+
+def node_geometry(X: np.ndarray, np_: int) -> np.ndarray:
+    """Per-node (det J, adj J row-major), shape (NE, n, 1 + dim^2), of the element maps whose
+    node coordinates are X (dim, NE, n): J[r][c] = dx_r/dxi_c of the degree-(Np-1) polynomial
+    through the nodes (MFEM's Tr.Weight() and Tr.AdjugateJacobian() on a nodal mesh)."""
+    dim, NE, n = X.shape
+    J = np.moveaxis(_ref_derivatives(X, np_), (0, 1), (2, 3))   # (NE, n, dim, dim)
+    if dim == 2:
+        det = J[..., 0, 0] * J[..., 1, 1] - J[..., 0, 1] * J[..., 1, 0]
+        adj = np.empty_like(J)
+        adj[..., 0, 0] = J[..., 1, 1]
+        adj[..., 0, 1] = -J[..., 0, 1]
+        adj[..., 1, 0] = -J[..., 1, 0]
+        adj[..., 1, 1] = J[..., 0, 0]
+    else:
+        adj = np.empty_like(J)
+        for r in range(3):
+            for c in range(3):
+                r1, r2 = [i for i in range(3) if i != c]
+                c1, c2 = [i for i in range(3) if i != r]
+                adj[..., r, c] = (-1) ** (r + c) * (J[..., r1, c1] * J[..., r2, c2] - J[..., r1, c2] * J[..., r2, c1])
+        det = (J[..., 0, 0] * adj[..., 0, 0] + J[..., 0, 1] * adj[..., 1, 0] + J[..., 0, 2] * adj[..., 2, 0])
+    G = np.empty((NE, n, 1 + dim * dim))
+    G[..., 0] = det
+    G[..., 1:] = adj.reshape(NE, n, dim * dim)
+    return G
+
+
+def face_nodes(dim: int, np_: int, f: int) -> np.ndarray:
+    """Node indices of local face f = 2 d + s, shape (Nt2, Nt1): entry [t2, t1] is the face
+    point (t1, t2), the face coordinates being the other reference directions in order."""
+    d, s = divmod(f, 2)
+    a = np_ - 1 if s else 0
+    t = np.arange(np_)
+    if dim == 2:
+        i, j = (np.full(np_, a), t) if d == 0 else (t, np.full(np_, a))
+        return (i + np_ * j)[None, :]
+    t1, t2 = np.meshgrid(t, t, indexing="xy")       # [t2, t1]
+    full = np.full_like(t1, a)
+    ijk = {0: (full, t1, t2), 1: (t1, full, t2), 2: (t1, t2, full)}[d]
+    return ijk[0] + np_ * (ijk[1] + np_ * ijk[2])
+
+
+def _orient_map(dim: int, np_: int, code: int):
+    """The neighbour's face point (u1, u2) of each point (t1, t2): arrays [t2, t1]."""
+    Nt2 = np_ if dim == 3 else 1
+    t2, t1 = np.meshgrid(np.arange(Nt2), np.arange(np_), indexing="ij")
+    a, b = (t2, t1) if code & 32 else (t1, t2)
+    u1 = np_ - 1 - a if code & 8 else a
+    u2 = np_ - 1 - b if code & 16 else b
+    if dim == 2:
+        u2 = np.zeros_like(u1)
+    return u1, u2
+
+
+def _point_keys(P: np.ndarray, periods, tol: float):
+    """Integer keys of points P (..., dim) equal for points equal up to tol, modulo periods."""
+    keys = []
+    for c in range(P.shape[-1]):
+        x = P[..., c]
+        L = None if periods is None else periods[c]
+        if L:
+            M = int(round(L / tol))
+            keys.append(np.mod(np.round(np.mod(x, L) / tol).astype(np.int64), M))
+        else:
+            keys.append(np.round(x / tol).astype(np.int64))
+    return np.stack(keys, axis=-1)
+
+
+def general_mesh(dim: int, order: int, X: np.ndarray, periods=None, boundary=None, tol=None) -> Mesh:
+    """A conforming mesh from the node coordinates X (dim, NE, n) of its elements (each a
+    degree-``order`` polynomial map of [0,1]^dim with det J > 0).  Faces are matched by their
+    corner points -- modulo ``periods`` (one length or None per direction) for periodic
+    meshes -- and the orientation code of each matched face is the one that maps every face
+    point onto its neighbour's.  An unmatched face is a boundary face; ``boundary(corners)``
+    (corners: (2^(dim-1), dim) points) gives the index k of its condition (neighbour entry
+    -(k+1)).  Per-node metrics come from node_geometry."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    np_ = order + 1
+    dimX, NE, n = X.shape
+    assert dimX == dim and n == np_ ** dim
+    G = node_geometry(X, np_)
+    if (G[..., 0] <= 0).any():
+        raise ValueError("element map with det J <= 0 at a node")
+    if tol is None:
+        span = float(np.max(X.max(axis=(1, 2)) - X.min(axis=(1, 2))))
+        tol = 1e-9 * max(span, 1e-300)
+    nf = 2 * dim
+    FN = [face_nodes(dim, np_, f) for f in range(nf)]
+    cidx = [0, np_ - 1]
+    corners = [(FN[f][[0] if dim == 2 else cidx][:, cidx]) for f in range(nf)]   # [c2, c1] node ids
+    P = np.moveaxis(X, 0, -1)                                                    # (NE, n, dim)
+    table = {}
+    ckeys = {}
+    for f in range(nf):
+        K = _point_keys(P[:, corners[f].reshape(-1)], periods, tol)             # (NE, ncorner, dim)
+        ckeys[f] = K
+        for e in range(NE):
+            key = tuple(sorted(map(tuple, K[e])))
+            table.setdefault(key, []).append((e, f))
+    nbr = np.zeros((NE, nf), dtype=np.int32)
+    code = np.zeros((NE, nf), dtype=np.int32)
+    for key, lst in table.items():
+        if len(lst) > 2:
+            raise ValueError(f"{len(lst)} faces share the corners {key} (a periodic direction needs at least "
+                             "3 elements for faces to be told apart by their corners)")
+        if len(lst) == 1:
+            e, f = lst[0]
+            if boundary is None:
+                raise ValueError(f"face {f} of element {e} has no neighbour and no boundary condition")
+            k = int(boundary(P[e, corners[f].reshape(-1)]))
+            nbr[e, f] = -(k + 1)
+            code[e, f] = f ^ 1
+            continue
+        for (e, f), (e2, f2) in (lst, lst[::-1]):
+            pts = _point_keys(P[e, FN[f]], periods, tol)                          # (Nt2, Nt1, dim)
+            found = None
+            for c in (0, 8, 16, 24, 32, 40, 48, 56) if dim == 3 else (0, 8):
+                u1, u2 = _orient_map(dim, np_, c)
+                nb = FN[f2][u2, u1]
+                if np.array_equal(pts, _point_keys(P[e2, nb], periods, tol)):
+                    found = c
+                    break
+            if found is None:
+                raise ValueError(f"faces ({e}, {f}) and ({e2}, {f2}) share corners but not their points")
+            nbr[e, f] = e2
+            code[e, f] = f2 | found
+    # element sizes: smallest singular value of J at the element centre
+    D = gll_derivative(np_)
+    xg, _ = gll_nodes(np_)
+    lc = np.array([np.prod([(0.5 - xg[j]) / (xg[i] - xg[j]) for j in range(np_) if j != i]) for i in range(np_)])
+    Jn = np.moveaxis(_ref_derivatives(X, np_), (0, 1), (2, 3))                   # (NE, n, dim, dim)
+    W = lc
+    for _ in range(dim - 1):
+        W = np.multiply.outer(lc, W).reshape(-1)
+    Jc = np.einsum("n,enrc->erc", W, Jn)
+    hmin = float(np.linalg.svd(Jc, compute_uv=False).min())
+    geom = np.ascontiguousarray(G[:, 0, :])          # node 0's values: the per-element summary
+    return Mesh(dim, order, None, None, None, np.ascontiguousarray(nbr), geom, np.ascontiguousarray(G),
+                np.ascontiguousarray(code), X, hmin)
+
+
+def lattice_coords(dim: int, order: int, n, lengths=None, origin=None) -> np.ndarray:
+    """Node coordinates (dim, NE, n) of the Cartesian lattice of periodic_lattice."""
+    return periodic_lattice(dim, order, n, lengths=lengths, origin=origin).node_coords()
+
+
+def multilinear_coords(dim: int, order: int, vertices: np.ndarray, cells: np.ndarray) -> np.ndarray:
+    """Node coordinates (dim, NE, n) of bilinear quadrilaterals / trilinear hexahedra: cells
+    (NE, 2^dim) lists each element's vertices in MFEM's (and Gmsh's) order -- counter-clockwise
+    from (0,0) in 2D; the bottom face then the top face in 3D."""
+    xi = reference_nodes(dim, order + 1)                    # (dim, n)
+    V = np.asarray(vertices, dtype=np.float64)[:, :dim]
+    if dim == 2:
+        x, y = xi
+        N = np.stack([(1 - x) * (1 - y), x * (1 - y), x * y, (1 - x) * y])                     # (4, n)
+    else:
+        x, y, z = xi
+        N = np.stack([(1 - x) * (1 - y) * (1 - z), x * (1 - y) * (1 - z), x * y * (1 - z), (1 - x) * y * (1 - z),
+                      (1 - x) * (1 - y) * z, x * (1 - y) * z, x * y * z, (1 - x) * y * z])
+    C = V[np.asarray(cells)]                                # (NE, 2^dim, dim)
+    return np.einsum("vn,evd->den", N, C)
+
+
+# ---------------------------------------------------------------------------
+# Gmsh 2.2 ASCII meshes (the reference reads them through MFEM, Simulation.cpp:272)
+
+def read_gmsh(path: str) -> dict:
+    """Parse a Gmsh 2.2 ASCII file: {'vertices': (nv, 3), 'cells': {type: (tags, vertex ids)},
+    'names': {physical tag: name}}; vertex ids are 0-based."""
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh]
+    i = 0
+    verts, names, ids = None, {}, {}
+    cells = {}
+    while i < len(lines):
+        ln = lines[i]
+        if ln == "$MeshFormat":
+            ver = lines[i + 1].split()
+            if not ver[0].startswith("2.") or ver[1] != "0":
+                raise ValueError(f"{path}: only the Gmsh 2.x ASCII format is read (found {ver})")
+            i += 3
+        elif ln == "$PhysicalNames":
+            cnt = int(lines[i + 1])
+            for ln2 in lines[i + 2:i + 2 + cnt]:
+                dim_, tag, name = ln2.split(maxsplit=2)
+                names[int(tag)] = name.strip('"')
+            i += cnt + 3
+        elif ln == "$Nodes":
+            cnt = int(lines[i + 1])
+            arr = np.array([ln2.split() for ln2 in lines[i + 2:i + 2 + cnt]], dtype=np.float64)
+            ids = {int(t): k for k, t in enumerate(arr[:, 0])}
+            verts = arr[:, 1:4]
+            i += cnt + 3
+        elif ln == "$Elements":
+            cnt = int(lines[i + 1])
+            for ln2 in lines[i + 2:i + 2 + cnt]:
+                f = [int(v) for v in ln2.split()]
+                etype, ntags = f[1], f[2]
+                tags = f[3:3 + ntags]
+                vs = [ids[v] for v in f[3 + ntags:]]
+                cells.setdefault(etype, ([], []))
+                cells[etype][0].append(tags[0] if tags else 0)
+                cells[etype][1].append(vs)
+            i += cnt + 3
+        else:
+            i += 1
+    if verts is None:
+        raise ValueError(f"{path}: no $Nodes section")
+    return {"vertices": verts, "names": names,
+            "cells": {t: (np.array(a, dtype=np.int64), np.array(b, dtype=np.int64)) for t, (a, b) in cells.items()}}
+
+
+def gmsh_mesh(path: str, order: int, bc_index=None, curve=None) -> Mesh:
+    """A general mesh of the linear quadrilaterals (Gmsh type 3; 2D) or hexahedra (type 5; 3D)
+    in a Gmsh 2.2 file.  Boundary faces take the physical group of the line (type 1) or
+    quadrilateral (type 3, 3D) boundary element on them: ``bc_index`` maps the group's name
+    (or tag) to the condition index k.  Elements whose vertex order gives det J < 0 are
+    reflected (vertices 1 and 3 exchanged; MFEM's Mesh::CheckElementOrientation(true) does the
+    same).  ``curve`` (optional) maps node coordinates (dim, m) to curved ones."""
+    g = read_gmsh(path)
+    cells = g["cells"]
+    if 5 in cells:
+        dim, vol, bnd = 3, 5, 3
+    elif 3 in cells:
+        dim, vol, bnd = 2, 3, 1
+    else:
+        raise ValueError(f"{path}: no quadrilateral or hexahedral elements")
+    V = g["vertices"][:, :dim]
+    elems = cells[vol][1].copy()
+    if dim == 2:
+        a = V[elems[:, 1]] - V[elems[:, 0]]
+        b = V[elems[:, 3]] - V[elems[:, 0]]
+        neg = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0] < 0
+        elems[neg, 1], elems[neg, 3] = elems[neg, 3].copy(), elems[neg, 1].copy()
+    X = multilinear_coords(dim, order, V, elems)
+    if curve is not None:
+        X = np.asarray(curve(X.reshape(dim, -1)), dtype=np.float64).reshape(X.shape)
+    span = float(np.max(V.max(axis=0) - V.min(axis=0)))
+    tol = 1e-9 * span
+    bfaces = {}
+    if bnd in cells:
+        tags, vs = cells[bnd]
+        for tag, vv in zip(tags, vs):
+            P = V[vv]
+            if curve is not None:
+                P = np.asarray(curve(P.T), dtype=np.float64).T
+            key = tuple(sorted(map(tuple, _point_keys(P, None, tol))))
+            name = g["names"].get(int(tag), int(tag))
+            bfaces[key] = name
+    bc_index = bc_index or {}
+
+    def boundary(corners):
+        key = tuple(sorted(map(tuple, _point_keys(corners, None, tol))))
+        if key not in bfaces:
+            raise ValueError(f"{path}: a boundary face at {corners.tolist()} has no boundary element")
+        name = bfaces[key]
+        if name not in bc_index:
+            raise ValueError(f"{path}: no boundary condition for the physical group {name!r}")
+        return bc_index[name]
+
+    return general_mesh(dim, order, X, boundary=boundary, tol=tol)
