@@ -1397,55 +1397,4 @@ __global__ void __launch_bounds__(256) k_vjp_visc2(const VjpViscArgs<NP> A) {
     for (int v = 0; v < NEQ; v++) A.out[v * vs + gi] += f * sb[v];
 }
 
-// ---------------------------------------------------------------------------
-// small utility kernels
-// ---------------------------------------------------------------------------
-// y = a x1 + b x2 (the adjoint sweep's combinations)
-__global__ void k_axpby(double *y, double a, const double *x1, double b, const double *x2, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        y[i] = a * x1[i] + b * x2[i];
-}
-
-__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ void k_fill_i32(int *p, int v, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ void k_fill_f64(double *p, double v, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t st = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += st) p[i] = v;
-}
-
-__global__ void k_copy_f64(double *__restrict__ dst, const double *__restrict__ src, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t st = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += st) dst[i] = src[i];
-}
-
-// max over nodes of |V| + c (the reference's max_char_speed, ChandrashekarFlux.cpp:61-65)
-template <int DIM>
-__global__ void k_max_speed(const double *u, size_t nnodes, double gamma, unsigned long long *lam_bits) {
-    constexpr int NEQ = DIM + 2;
-    double lam = 0.0;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t st = (size_t)gridDim.x * blockDim.x;
-    for (; i < nnodes; i += st) {
-        double s[NEQ];
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) s[v] = u[v * nnodes + i];
-        PhysConst P{};
-        P.gamma = gamma;
-        P.gammaM1 = gamma - 1.0;
-        lam = fmax(lam, make_side<DIM>(s, P).lam);
-    }
-    for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
-    if ((threadIdx.x & 63) == 0 && lam > 0.0) atomic_max_pos(lam_bits, lam);
-}
-
 }  // namespace prandtl

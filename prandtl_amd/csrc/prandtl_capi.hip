@@ -3,39 +3,14 @@
 //
 // Reference interfaces replaced are cited per entry point in include/prandtl_amd.h.
 
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdio>
-#include <algorithm>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "../../include/prandtl_amd.h"
-#include "dg_hooks.hpp"
-#include "dg_kernels.hpp"
+#include "op_internal.hpp"
+#include "dg_util.hpp"
 
 using namespace prandtl;
 
-
-namespace {
+namespace prandtl {
 
 thread_local std::string g_last_error;
-
-int fail(int code, const std::string &msg) {
-    g_last_error = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t _e = (expr);                                                                \
-        if (_e != hipSuccess)                                                                  \
-            return fail(PRANDTL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));    \
-    } while (0)
-
-constexpr int MAX_NP = 5;
 
 // GLL rule on [0,1] from its closed form (0, 1 and the roots of P'_N(2x-1), weights
 // 1/(N(N+1) P_N^2)), Newton's method in long double; the reference takes it from
@@ -80,6 +55,10 @@ void gll_rule(int np, double *x, double *w) {
     x[N] = 1.0;
 }
 
+}  // namespace prandtl
+
+namespace {
+
 // Barycentric differentiation matrix and its surface-corrected forms
 // (DGSEMIntegrator.cpp:45-80), stored as mathematical matrices M[i][j].
 void diff_matrices(int np, const double *x, const double *w, double *D, double *Dh, double *Dh2) {
@@ -108,91 +87,6 @@ void diff_matrices(int np, const double *x, const double *w, double *D, double *
     Dh2[np * np - 1] -= 1.0 / w[np - 1];
 }
 
-}  // namespace
-
-struct prandtl_op_s {
-    int dim = 0, order = 0, np = 0, nn = 0, NE = 0, neq = 0;
-    size_t size = 0;       // doubles per state vector
-    prandtl_physics phys{};
-    prandtl_opts opts{};
-    PhysConst P{};
-    double D[MAX_NP * MAX_NP], Dh[MAX_NP * MAX_NP], Dh2[MAX_NP * MAX_NP], inv_w0 = 0.0;
-    double w[MAX_NP] = {};        // GLL weights on [0,1]
-    BcDev *d_bc = nullptr;        // boundary conditions (neighbour -(k+1) -> d_bc[k])
-    int nbc = 0;
-    double *d_alpha = nullptr;    // blending: per-element coefficient of the current RHS
-    int *d_nbr = nullptr;
-    double *d_geom = nullptr;
-    double *d_dv = nullptr;       // NS: viscous volume term, (neq-1) x NE x n
-    double *d_Ff = nullptr;       // NS: viscous face traces, dim x 2 x (neq-1) x NE x Np^(dim-1)
-    double *d_k = nullptr;        // RK register
-    double *d_ub = nullptr;       // RK alternate state buffer
-    double *d_y2 = nullptr;       // SSP-RK3 second stage state
-    double *d_emin = nullptr;     // stage hooks: per-element minimum nodal entropy
-    double *d_fbound = nullptr;   // stage hooks: per-element entropy bound of the filter
-    int *d_hflag = nullptr;       // stage hooks: per-element trigger flags
-    unsigned long long *d_hcount = nullptr;   // stage hooks: elements filtered, limited
-    bool have_bounds = false;     // filter constraints computed
-    double *d_vq = nullptr;       // adjoint: per-node quantities, (dim+6) x NE x n
-    double *d_vgf = nullptr;      // NS adjoint: lifting adjoint at face points, dim x 2 x neq x NE x Np^(dim-1)
-    double *d_vwb = nullptr;      // NS adjoint: partial adjoint of the entropy variables
-    double *d_vsb = nullptr;      // NS adjoint: partial adjoint of the state
-    double *d_vlam = nullptr;     // adjoint: stage cotangent lambda
-    unsigned long long *d_lam = nullptr;
-    int *d_nan = nullptr;
-    bool lam_valid = false;
-    // taped forward run for the discrete adjoint: the state at the start of every segment
-    int tape_steps = 0, tape_every = 0, tape_scheme = PRANDTL_LSRK45;
-    double tape_dt = 0.0;
-    std::vector<double *> tape_ckpt;
-    // adjoint sweep storage, kept between calls: the stage inputs of one segment (5 per step;
-    // slot 0 points at the segment's checkpoint) and the double-buffered adjoint registers
-    std::vector<double *> adj_stage;
-    double *adj_ub2 = nullptr, *adj_kb = nullptr, *adj_kb2 = nullptr;
-};
-
-namespace {
-
-template <int NP>
-DTables<NP> make_tables(const prandtl_op_s *op) {
-    DTables<NP> T;
-    for (int i = 0; i < NP; i++)
-        for (int j = 0; j < NP; j++) {
-            T.D[i][j] = op->D[i * NP + j];
-            T.Dhat[i][j] = op->Dh[i * NP + j];
-            T.Dhat2[i][j] = op->Dh2[i * NP + j];
-        }
-    T.inv_w0 = op->inv_w0;
-    for (int i = 0; i < NP; i++) T.inv_w[i] = 1.0 / op->w[i];
-    return T;
-}
-
-// 1D inverse Vandermonde of the Legendre basis on the GLL nodes: Vi[k][a] = w_a P_k(x_a) / gamma_k,
-// gamma_k = sum_b w_b P_k(x_b)^2 (the GLL rule integrates P_k P_l exactly for k + l <= 2 Np - 3 and
-// the discrete products vanish for k != l <= Np - 1, so Vi V1 = I).
-template <int NP>
-BlendParams<NP> make_blend_params(const prandtl_op_s *op) {
-    BlendParams<NP> B{};
-    double x[NP], w[NP], V[NP][NP];
-    gll_rule(NP, x, w);
-    for (int i = 0; i < NP; i++) {
-        const double z = 2.0 * x[i] - 1.0;
-        V[i][0] = 1.0;
-        if (NP > 1) V[i][1] = z;
-        for (int n = 1; n + 1 < NP; n++) V[i][n + 1] = ((2.0 * n + 1.0) * z * V[i][n] - n * V[i][n - 1]) / (n + 1.0);
-    }
-    for (int k = 0; k < NP; k++) {
-        double g = 0.0;
-        for (int b = 0; b < NP; b++) g += w[b] * V[b][k] * V[b][k];
-        for (int a = 0; a < NP; a++) B.Vi[k][a] = w[a] * V[a][k] / g;
-    }
-    B.T = 0.5 * std::pow(10.0, -1.8 * std::pow((double)op->order, 0.25));   // DGSEMOperator.cpp:25
-    B.sharpness = 9.21024;                                                    // DGSEMOperator.hpp:50
-    B.alpha_max = op->opts.alpha_max;
-    B.alpha_min = op->opts.alpha_min;
-    return B;
-}
-
 template <int DIM, int NP>
 int launch_blend_coef(prandtl_op_s *op, const double *u, double *alpha, double *eta, hipStream_t st) {
     BlendArgs<NP> B{};
@@ -207,127 +101,6 @@ int launch_blend_coef(prandtl_op_s *op, const double *u, double *alpha, double *
     hipLaunchKernelGGL((k_blend_coef<DIM, NP>), dim3(nblk), dim3(C::BLOCK), 0, st, B);
     HIP_TRY(hipGetLastError());
     return PRANDTL_OK;
-}
-
-struct StageSpec {
-    int mode;            // EPI_*
-    const double *u;     // stage input
-    double *out;
-    double *k;
-    const double *x;
-    double ca, cb, dt;
-    double *grad;        // optional primitive-gradient export (NS)
-    bool want_lambda;
-    bool nan_check;
-    int phases = 3;      // bit 0: k_lift (NS), bit 1: k_rhs
-};
-
-template <int DIM, int NP, bool VISC, int SURF, bool MIX>
-int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
-    if constexpr (VISC) {
-      if (S.phases & 1) {
-        using NC = LiftCfg<DIM, NP>;
-        LiftArgs<NP> la;
-        la.T = make_tables<NP>(op);
-        la.P = op->P;
-        la.NE = op->NE;
-        la.nbr = op->d_nbr;
-        la.geom = op->d_geom;
-        la.u = S.u;
-        la.dv = op->d_dv;
-        la.Ff = op->d_Ff;
-        la.dvf = reinterpret_cast<float *>(op->d_dv);
-        la.Fff = reinterpret_cast<float *>(op->d_Ff);
-        la.grad = S.grad;
-        la.bc = op->d_bc;
-        const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
-        if constexpr (!MIX) {
-            if (op->nbc)
-                hipLaunchKernelGGL((k_lift<DIM, NP, false, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
-            else
-                hipLaunchKernelGGL((k_lift<DIM, NP, false>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
-        } else {
-            hipLaunchKernelGGL((k_lift<DIM, NP, MIX>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
-        }
-        HIP_TRY(hipGetLastError());
-      }
-    }
-    if (!(S.phases & 2)) return PRANDTL_OK;
-    RhsArgs<NP> ra;
-    ra.T = make_tables<NP>(op);
-    ra.P = op->P;
-    ra.NE = op->NE;
-    ra.bug3 = (op->opts.bug_compat & PRANDTL_BUG_A3) ? 1 : 0;
-    ra.mode = S.mode;
-    ra.want_lambda = S.want_lambda ? 1 : 0;
-    ra.nbr = op->d_nbr;
-    ra.geom = op->d_geom;
-    ra.u = S.u;
-    ra.dv = op->d_dv;
-    ra.Ff = op->d_Ff;
-    ra.dvf = reinterpret_cast<const float *>(op->d_dv);
-    ra.Fff = reinterpret_cast<const float *>(op->d_Ff);
-    ra.out = S.out;
-    ra.k = S.k;
-    ra.x = S.x;
-    ra.ca = S.ca;
-    ra.cb = S.cb;
-    ra.dt = S.dt;
-    ra.lam_bits = op->d_lam;
-    ra.nan_flag = S.nan_check ? op->d_nan : nullptr;
-    ra.alpha = op->d_alpha;
-    ra.bc = op->d_bc;
-    if constexpr (!MIX) {
-        if (op->opts.blending || op->nbc) {
-            // blending: the kernel computes this evaluation's coefficients from its input state
-            // (DGSEMOperator.cpp:547-549) and blends; boundary faces: the conditions' fluxes.
-            // Both run the node kernel (for Euler too).
-            if (op->opts.blending) ra.blend = make_blend_params<NP>(op);
-            using LC = NodeCfg<DIM, NP>;
-            const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
-            if (op->opts.blending && op->nbc)
-                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
-            else if (op->opts.blending)
-                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, false>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
-            else
-                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
-            HIP_TRY(hipGetLastError());
-            return PRANDTL_OK;
-        }
-    }
-    // Navier-Stokes: one thread per node (the node's state stays in registers through the
-    // fused epilogue, no LDS accumulator: 4 waves/SIMD); Euler: one thread per GLL line
-    // (each symmetric pair flux evaluated once).  Chosen by measurement at TGV 32^3, p=3.
-    if constexpr (VISC || MIX) {
-        using LC = NodeCfg<DIM, NP>;
-        const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
-        hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, MIX>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
-    } else {
-        using LC = LineCfg<DIM, NP>;
-        const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
-        hipLaunchKernelGGL((k_rhs<DIM, NP, VISC, SURF>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
-    }
-    HIP_TRY(hipGetLastError());
-    return PRANDTL_OK;
-}
-
-template <int DIM, int NP, bool VISC, bool MIX>
-int dispatch_surf(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
-    switch (op->opts.surface_flux) {
-        case PRANDTL_SURF_CHANDRASHEKAR: return launch_stage<DIM, NP, VISC, SURF_CHANDRASHEKAR, MIX>(op, S, st);
-        case PRANDTL_SURF_EC: return launch_stage<DIM, NP, VISC, SURF_EC, MIX>(op, S, st);
-        case PRANDTL_SURF_RUSANOV: return launch_stage<DIM, NP, VISC, SURF_RUSANOV, MIX>(op, S, st);
-    }
-    return fail(PRANDTL_ERR_ARG, "unknown surface flux");
-}
-
-template <int DIM, int NP>
-int dispatch_visc(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
-    if (op->opts.precision == PRANDTL_MIXED)
-        return op->phys.viscous ? dispatch_surf<DIM, NP, true, true>(op, S, st)
-                                : dispatch_surf<DIM, NP, false, true>(op, S, st);
-    return op->phys.viscous ? dispatch_surf<DIM, NP, true, false>(op, S, st)
-                            : dispatch_surf<DIM, NP, false, false>(op, S, st);
 }
 
 template <int DIM>
