@@ -76,16 +76,28 @@ typedef struct {
     double q[8];
 } prandtl_bc;
 
-/* Conforming affine mesh with aligned faces.
+/* Conforming mesh -- the reference's ParMesh and its finite-element space as the
+ * integrators see them (src/Simulation/Simulation.cpp:272-330, DGSEMIntegrator.cpp:8-149).
  *   neighbors[e * 2*dim + 2*d + s]: element across local face (d, s) of e,
  *       s = 0 on the xi_d = 0 side, s = 1 on the xi_d = 1 side; -(k+1) marks a
  *       boundary face with condition bc[k] (the reference's boundary attributes);
  *   geometry[e * (1 + dim*dim)]: det J, then adj J row-major (MFEM Weight() and
- *       AdjugateJacobian(), DGSEMIntegrator.cpp:254-256);
+ *       AdjugateJacobian(), DGSEMIntegrator.cpp:254-256) of an affine element;
  *   bc[0..num_bc-1]: the boundary conditions (num_bc = 0 for a periodic mesh).
- * Node (.., Np-1 in d, ..) of e coincides with node (.., 0 in d, ..) of its
- * +d neighbour.  All arrays are host pointers, copied at create time.  Boundary
- * faces run in FP64 (mixed precision and the adjoint refuse them). */
+ * Affine lattice (node_geometry and face_orientation NULL): node (.., Np-1 in d, ..)
+ * of e coincides with node (.., 0 in d, ..) of its +d neighbour.
+ * General mesh (curved or unstructured; either array non-NULL):
+ *   node_geometry[(e * n + node) * (1 + dim*dim)]: det J and adj J at every node
+ *       (the reference evaluates them per node, DGSEMIntegrator.cpp:254-276); NULL:
+ *       geometry at every node (geometry may be NULL when this is given);
+ *   face_orientation[e * 2*dim + f]: the neighbour's local face f' = 2 d' + s' in
+ *       bits 0-2; bit 3 flips the first face coordinate, bit 4 the second, bit 5 swaps
+ *       them: point (t1, t2) of face f (t1, t2 the indices along the other reference
+ *       directions, in increasing order) is the neighbour's point (u1, u2) of face f'
+ *       with (a, b) = swap ? (t2, t1) : (t1, t2), u1 = flip1 ? Np-1-a : a,
+ *       u2 = flip2 ? Np-1-b : b; NULL: the aligned-lattice matching (f' = f ^ 1).
+ * All arrays are host pointers, copied at create time.  Boundary faces and general
+ * meshes run in FP64 (mixed precision and the adjoint refuse them). */
 typedef struct {
     int dim;
     int order;
@@ -94,6 +106,8 @@ typedef struct {
     const double *geometry;
     int num_bc;
     const prandtl_bc *bc;
+    const double *node_geometry;
+    const int *face_orientation;
 } prandtl_mesh_desc;
 
 /* ref: src/Physics/Physics.hpp:10-38 and NavierStokesFlux(dim, gamma, Pr, mu, ...),

@@ -83,12 +83,16 @@ private:
 enum class SurfaceFlux { Chandrashekar = PRANDTL_SURF_CHANDRASHEKAR, EC = PRANDTL_SURF_EC, Rusanov = PRANDTL_SURF_RUSANOV };
 enum class Precision { FP64 = PRANDTL_FP64, Mixed = PRANDTL_MIXED };
 
-// Conforming affine mesh: per element its 2 dim face neighbours and (det J, adj J).
+// Conforming mesh: per element its 2 dim face neighbours and (det J, adj J); a general
+// (curved or unstructured) mesh adds the per-node geometry and the face orientation codes
+// (their layout: prandtl_mesh_desc).
 struct Mesh {
     int dim = 0, order = 0, num_elements = 0;
-    std::vector<int> neighbors;     // [e][2 d + s]; -(k+1): a boundary face with condition bc[k]
-    std::vector<double> geometry;   // [e][1 + dim^2]
-    std::vector<prandtl_bc> bc;     // boundary conditions (empty for a periodic mesh)
+    std::vector<int> neighbors;         // [e][2 d + s]; -(k+1): a boundary face with condition bc[k]
+    std::vector<double> geometry;       // [e][1 + dim^2] (may be empty when node_geometry is given)
+    std::vector<prandtl_bc> bc;         // boundary conditions (empty for a periodic mesh)
+    std::vector<double> node_geometry;  // general mesh: [e][node][1 + dim^2] (empty: affine elements)
+    std::vector<int> face_orientation;  // general mesh: [e][2 dim] (empty: aligned lattice faces)
 
     // Periodic lattice of n[0] x n[1] (x n[2]) axis-aligned boxes of the domain
     // [origin, origin + length] (the generated meshes of the benchmark cases).
@@ -164,8 +168,11 @@ class DGSEMOperator : public TimeDependentOperator {
 public:
     DGSEMOperator(const Mesh &mesh, const Physics &phys, const Options &opts = {})
         : TimeDependentOperator((int)mesh.StateSize()) {
-        prandtl_mesh_desc md{mesh.dim, mesh.order, mesh.num_elements, mesh.neighbors.data(), mesh.geometry.data(),
-                             (int)mesh.bc.size(), mesh.bc.empty() ? nullptr : mesh.bc.data()};
+        prandtl_mesh_desc md{mesh.dim, mesh.order, mesh.num_elements, mesh.neighbors.data(),
+                             mesh.geometry.empty() ? nullptr : mesh.geometry.data(),
+                             (int)mesh.bc.size(), mesh.bc.empty() ? nullptr : mesh.bc.data(),
+                             mesh.node_geometry.empty() ? nullptr : mesh.node_geometry.data(),
+                             mesh.face_orientation.empty() ? nullptr : mesh.face_orientation.data()};
         prandtl_physics ph{phys.gamma, phys.Pr, phys.mu, phys.R_gas, phys.viscous ? 1 : 0, phys.sutherland ? 1 : 0};
         prandtl_opts o{(int)opts.surface_flux, (int)opts.precision, opts.bug_compat, opts.stage_hooks,
                        opts.blending ? 1 : 0, opts.alpha_max, opts.alpha_min};

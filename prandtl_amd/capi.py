@@ -57,7 +57,8 @@ def bc(type_: int, *q) -> Bc:
 class MeshDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("num_elements", ctypes.c_int),
                 ("neighbors", ctypes.POINTER(ctypes.c_int)), ("geometry", ctypes.POINTER(ctypes.c_double)),
-                ("num_bc", ctypes.c_int), ("bc", ctypes.POINTER(Bc))]
+                ("num_bc", ctypes.c_int), ("bc", ctypes.POINTER(Bc)),
+                ("node_geometry", ctypes.POINTER(ctypes.c_double)), ("face_orientation", ctypes.POINTER(ctypes.c_int))]
 
 
 class Physics(ctypes.Structure):
@@ -152,10 +153,16 @@ class Operator:
         self._nbr = np.ascontiguousarray(mesh.neighbors, dtype=np.int32)
         self._geom = np.ascontiguousarray(mesh.geom, dtype=np.float64)
         self._bcs = (Bc * len(bcs))(*bcs) if bcs else None
+        ng = getattr(mesh, "node_geom", None)
+        fo = getattr(mesh, "face_orient", None)
+        self._ngeom = None if ng is None else np.ascontiguousarray(ng, dtype=np.float64)
+        self._forient = None if fo is None else np.ascontiguousarray(fo, dtype=np.int32)
         md = MeshDesc(mesh.dim, mesh.order, mesh.num_elements,
                       self._nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                       self._geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                      len(bcs) if bcs else 0, self._bcs)
+                      len(bcs) if bcs else 0, self._bcs,
+                      None if self._ngeom is None else self._ngeom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      None if self._forient is None else self._forient.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         ph = Physics(gamma, Pr, mu, R_gas, int(viscous), int(sutherland))
         op = Opts(int(surface_flux), int(precision), int(bug_compat), int(stage_hooks), int(blending),
                   float(alpha_max), float(alpha_min))

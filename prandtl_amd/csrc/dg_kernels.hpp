@@ -85,6 +85,10 @@ struct DTables {
     double Dhat2[NP][NP];  // 2 D + the same corrections      (:76-80)
     double inv_w0;         // 1 / w_0
     double inv_w[NP];      // 1 / w_a (subcell finite-volume term)
+    // subcell metric of interface i+1 of a line (between its nodes i and i+1) on a general
+    // element: sum_m sub[i][m] adj_d(node m), sub[i][m] = delta_m0 + sum_{l<=i} w_l D_lm
+    // (ComputeSubcellMetrics, DGSEMIntegrator.cpp:1199-1304); rows 0..Np-2
+    double sub[NP][NP];
 };
 
 enum EpilogueMode { EPI_MULT = 0, EPI_LSRK = 1, EPI_SSP = 2 };
@@ -113,6 +117,8 @@ struct RhsArgs {
     const BcDev *bc;       // boundary conditions (neighbour -(k+1) -> bc[k]; k_rhs_node<..., BND = true>)
     double *alpha;         // k_rhs_node<..., BLEND = true>: the coefficients it computed (may be null)
     BlendParams<NP> blend; // the same: Persson-Peraire indicator and blending constants
+    const double *ngeom;   // general mesh (k_rhs_node<..., GEN = true>): per-node (det J, adj J) [e][node][1+dim^2]
+    const int *forient;    // general mesh: face orientation codes [e][2 dim]
 };
 
 template <int NP>
@@ -129,6 +135,8 @@ struct LiftArgs {
     float *Fff;
     double *grad;          // optional out: primitive gradients [c][v][e][node]
     const BcDev *bc;       // boundary conditions (neighbour -(k+1) -> bc[k])
+    const double *ngeom;   // general mesh (k_lift<..., GEN = true>): per-node (det J, adj J)
+    const int *forient;    // general mesh: face orientation codes
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD walks one
@@ -163,6 +171,19 @@ template <int DIM, int NP>
 __device__ __forceinline__ size_t ftrace(int dir, int side, int v, int e, int t, int NE) {
     constexpr int L = ipow(NP, DIM - 1);
     return ((((size_t)(dir * 2 + side) * (DIM + 1) + v) * NE + e) * L) + t;
+}
+
+// General mesh: the neighbour's face point tn (face coordinates u1 + Np u2) of the point
+// t = t1 + Np t2 of a face with orientation code `code` (bits 0-2 the neighbour's local face,
+// bit 3 flips the first face coordinate, bit 4 the second, bit 5 swaps them).
+template <int DIM, int NP>
+__device__ __forceinline__ int face_point_map(int code, int t) {
+    const int t1 = t % NP, t2 = t / NP;
+    const bool sw = (code & 32) != 0;
+    const int a = sw ? t2 : t1, b = sw ? t1 : t2;
+    const int u1 = (code & 8) ? NP - 1 - a : a;
+    const int u2 = (code & 16) ? NP - 1 - b : b;
+    return DIM == 2 ? u1 : u1 + NP * u2;
 }
 
 __device__ __forceinline__ void atomic_max_pos(unsigned long long *p, double v) {
@@ -375,13 +396,24 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
 // The interface fluxes are work items spread over the workgroup one direction at a time,
 // staged in the surface-flux LDS buffer after the DG accumulation has read it; workgroups
 // whose elements all have alpha = 0 skip them.
+//
+// GEN (general meshes, FP64): per-node metrics (det J, adj J at every node, as the reference
+// evaluates Tr.Weight() and Tr.AdjugateJacobian() per node, DGSEMIntegrator.cpp:254-276) and
+// faces that meet their neighbour's face in any direction, flipped or swapped (an orientation
+// code per face).  A pair flux takes the averaged metric 1/2 (adj_d(i) + adj_d(j)) (ComputeMean,
+// ChandrashekarFlux.cpp:15); a face flux is evaluated as the reference's face integrator does,
+// once per face with the normal of its lower-numbered element (CalcOrtho: that element's adj row
+// at the face point, outward), by both elements' work items; the FV subcell metrics come from
+// the line's per-node adj rows (the DTables sub weights).
 // ---------------------------------------------------------------------------
-template <int DIM, int NP, bool VISC, int SURF, bool MIX, bool BLEND = false, bool BND = false>
+template <int DIM, int NP, bool VISC, int SURF, bool MIX, bool BLEND = false, bool BND = false, bool GEN = false>
 __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES) k_rhs_node(const RhsArgs<NP> A) {
     using C = NodeCfg<DIM, NP>;
     using R = typename std::conditional<MIX, float, double>::type;  // flux arithmetic
     static_assert(!(MIX && BLEND), "subcell blending is FP64 only");
     static_assert(!(MIX && BND), "boundary faces run in FP64");
+    static_assert(!(MIX && GEN), "general meshes run in FP64");
+    static_assert(!GEN || BND, "general meshes take the boundary-face path");
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
     __shared__ double sPd[2][EPB * NN];   // rho, beta
     __shared__ R sPv[DIM][EPB * NN];      // velocity
@@ -391,7 +423,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     __shared__ int sNbr[EPB][2 * DIM];    // face neighbours of the workgroup's elements
     constexpr int L = ipow(NP, DIM - 1), FITEMS = 2 * DIM * L;
     __shared__ double sS[NEQ][EPB * NN];  // conserved state
-    __shared__ R sM[EPB][DIM][DIM];       // adj J rows
+    __shared__ R sM[GEN ? 1 : EPB][DIM][DIM];            // adj J rows (affine elements)
+    __shared__ double sMn[GEN ? EPB * NN : 1][DIM][DIM];   // GEN: adj J rows at every node
+    __shared__ int sOr[GEN ? EPB : 1][2 * DIM];            // GEN: face orientation codes
     __shared__ R sF[EPB * FITEMS][NEQ];   // surface flux (with the viscous face average) per face point
     constexpr int SITEMS = (NP - 1) * L;  // subcell interfaces per element and direction
     static_assert(SITEMS <= FITEMS, "subcell fluxes are staged in the surface-flux buffer");
@@ -414,13 +448,32 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         const int ee = min(blk * EPB + ib, NE - 1);
         sNbr[ib][f] = A.nbr[(size_t)ee * 2 * DIM + f];
     }
-    for (int i = tid; i < EPB * DIM * DIM; i += C::BLOCK) {
-        const int ib = i / (DIM * DIM), r = i - ib * DIM * DIM;
-        const int ee = min(blk * EPB + ib, NE - 1);
-        sM[ib][r / DIM][r % DIM] = (R)A.geom[(size_t)ee * (1 + DIM * DIM) + 1 + r];
+    if constexpr (GEN) {
+        for (int i = tid; i < EPB * 2 * DIM; i += C::BLOCK) {
+            const int ib = i / (2 * DIM), f = i - ib * 2 * DIM;
+            const int ee = min(blk * EPB + ib, NE - 1);
+            sOr[ib][f] = A.forient[(size_t)ee * 2 * DIM + f];
+        }
+    } else {
+        for (int i = tid; i < EPB * DIM * DIM; i += C::BLOCK) {
+            const int ib = i / (DIM * DIM), r = i - ib * DIM * DIM;
+            const int ee = min(blk * EPB + ib, NE - 1);
+            sM[ib][r / DIM][r % DIM] = (R)A.geom[(size_t)ee * (1 + DIM * DIM) + 1 + r];
+        }
     }
     double lam = 0.0;
     double qrp = 0.0;   // rho p at the node (BLEND)
+    double detJ = 0.0;  // det J at the node (GEN)
+    if constexpr (GEN) {
+        if (active) {
+            const double *g = A.ngeom + ((size_t)e * NN + node) * (1 + DIM * DIM);
+            detJ = g[0];
+#pragma unroll
+            for (int r = 0; r < DIM; r++)
+#pragma unroll
+                for (int c = 0; c < DIM; c++) sMn[tid][r][c] = g[1 + r * DIM + c];
+        }
+    }
     if (active) {
         double s[NEQ];
 #pragma unroll
@@ -471,7 +524,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) so[v] = sS[v][ib * NN + nown];
                 const SideState<DIM> qo = make_side<DIM>(so, A.P);
-                const R *m = sM[ib][dir];
+                const R *m = GEN ? sMn[ib * NN + nown][dir] : sM[ib][dir];
 #pragma unroll
                 for (int c = 0; c < DIM; c++) nor[c] = side ? m[c] : -m[c];
                 lam = fmax(lam, bc_inviscid_flux<DIM, SURF>(A.bc[-en - 1], A.P, A.bug3 != 0, so, qo, nor, fb));
@@ -484,6 +537,57 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 }
                 continue;
             }
+        }
+        if constexpr (GEN) {
+            // the neighbour's face (d2, s2) and point tn; the flux once per face, with the normal
+            // of the lower-numbered element (ties: the lower local face), stored along +adj_dir
+            const int code = sOr[ib][2 * dir + side];
+            const int f2 = code & 7, d2 = f2 >> 1, s2 = f2 & 1;
+            const int tn = face_point_map<DIM, NP>(code, t);
+            const int st2 = dir_stride<NP>(d2);
+            const int nn2 = line_base<NP>(tn, d2) + (s2 ? (NP - 1) * st2 : 0);
+            const bool first = (ee < en) || (ee == en && 2 * dir + side < f2);
+            double so[NEQ], sn[NEQ], n1[DIM];
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                so[v] = sS[v][ib * NN + nown];
+                sn[v] = A.u[v * vs + (size_t)en * NN + nn2];
+            }
+            if (first) {
+#pragma unroll
+                for (int c = 0; c < DIM; c++) n1[c] = side ? sMn[ib * NN + nown][dir][c] : -sMn[ib * NN + nown][dir][c];
+            } else {
+                const double *g = A.ngeom + ((size_t)en * NN + nn2) * (1 + DIM * DIM) + 1 + d2 * DIM;
+#pragma unroll
+                for (int c = 0; c < DIM; c++) n1[c] = s2 ? g[c] : -g[c];
+            }
+            double nmag = n1[0] * n1[0];
+#pragma unroll
+            for (int d = 1; d < DIM; d++) nmag += n1[d] * n1[d];
+            nmag = sqrt(nmag);
+            const SideState<DIM> qo = make_side<DIM>(so, A.P);
+            const SideState<DIM> qn = make_side<DIM>(sn, A.P);
+            double f[NEQ];
+            if (first)
+                face_flux<DIM, SURF>(so, qo, sn, qn, n1, nmag, A.P, A.bug3 != 0, f);
+            else
+                face_flux<DIM, SURF>(sn, qn, so, qo, n1, nmag, A.P, A.bug3 != 0, f);
+            // the own element's outward flux is (first ? f : -f); along +adj_dir: times (side ? 1 : -1)
+            const double sg = (first == (side == 1)) ? 1.0 : -1.0;
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) f[v] *= sg;
+            if constexpr (VISC) {
+                // 1/2 (F_own + F_nb) along the own +adj_dir: the neighbour's trace is stored along its
+                // +adj_d2, which points the other way across the face when s2 == side
+                const double sig = (side != s2) ? 1.0 : -1.0;
+#pragma unroll
+                for (int v = 0; v < NEQ - 1; v++)
+                    f[1 + v] -= 0.5 * (A.Ff[ftrace<DIM, NP>(dir, side, v, ee, t, NE)] +
+                                       sig * A.Ff[ftrace<DIM, NP>(d2, s2, v, en, tn, NE)]);
+            }
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
+            continue;
         }
         double so[NEQ], sn[NEQ];
 #pragma unroll
@@ -541,7 +645,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
     double invJ = 0.0;
     if (active) {
-        invJ = rcp(A.geom[(size_t)e * (1 + DIM * DIM)]);
+        invJ = rcp(GEN ? detJ : A.geom[(size_t)e * (1 + DIM * DIM)]);
         const double p_ref = MIX ? sPref[eb] : 0.0;
         const double h_ref = MIX ? sHref[eb] : 0.0;
 #pragma unroll
@@ -549,7 +653,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             const int st = dir_stride<NP>(dir);
             const int a = ai[dir];
             const int lb = eb * NN + node - a * st;
-            const R *m = sM[eb][dir];
+            const R *m = sM[GEN ? 0 : eb][dir];
 #pragma unroll 1
             for (int k = 1; k < NP; k++) {
                 const int b = (a + k) % NP;
@@ -577,7 +681,14 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     qb.beta = sPd[1][li];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
-                    volume_flux<DIM, true>(qo, qb, m, A.P.gammaM1Inverse, F);
+                    if constexpr (GEN) {
+                        double mp[DIM];   // ComputeMean of the two nodes' metrics
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) mp[c] = (sMn[tid][dir][c] + sMn[li][dir][c]) * 0.5;
+                        volume_flux<DIM, true>(qo, qb, mp, A.P.gammaM1Inverse, F);
+                    } else {
+                        volume_flux<DIM, true>(qo, qb, m, A.P.gammaM1Inverse, F);
+                    }
                 }
                 const R dd = sT[a * NP + b];
 #pragma unroll
@@ -624,7 +735,23 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     }
                     const SideState<DIM> ql = make_side<DIM>(sl, A.P);
                     const SideState<DIM> qr = make_side<DIM>(sr, A.P);
-                    const R *m = sM[ib][dir];
+                    R mg[DIM];
+                    if constexpr (GEN) {
+                        // the subcell metric of interface i+1 from the line's per-node adj rows
+                        const int l0 = ib * NN + line_base<NP>(t, dir);
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) mg[c] = 0.0;
+#pragma unroll
+                        for (int k = 0; k < NP; k++) {
+                            const double wk = A.T.sub[i][k];
+#pragma unroll
+                            for (int c = 0; c < DIM; c++) mg[c] += wk * sMn[l0 + k * st][dir][c];
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) mg[c] = sM[ib][dir][c];
+                    }
+                    const R *m = mg;
                     R nmag = m[0] * m[0];
 #pragma unroll
                     for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
@@ -707,9 +834,13 @@ struct LiftCfg {
     static constexpr int EPB = BLOCK / NN;
 };
 
-template <int DIM, int NP, bool MIX, bool BND = false>
+// GEN (general meshes): per-node metrics and oriented face matching, as in k_rhs_node; the
+// jump term of a face point uses the node's own outward normal (the reference's face normal up
+// to rounding on a conforming mesh).
+template <int DIM, int NP, bool MIX, bool BND = false, bool GEN = false>
 __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES) k_lift(const LiftArgs<NP> A) {
     static_assert(!(MIX && BND), "boundary faces run in FP64");
+    static_assert(!GEN || BND, "general meshes take the boundary-face path");
     using C = LiftCfg<DIM, NP>;
     // derivative contractions, chain rule and viscous flux in R; entropy variables in FP64.
     // Mixed precision contracts w - w_ref (w_ref = the element's node-0 value; D annihilates
@@ -722,6 +853,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     __shared__ double sWref[EPB][NEQ];
     __shared__ LogRef sLR[EPB];           // reference state of the logarithms (the element's node 0)
     __shared__ int sNbr[EPB][2 * DIM];    // face neighbours of the workgroup's elements
+    __shared__ int sOr[GEN ? EPB : 1][2 * DIM];   // GEN: face orientation codes
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -736,6 +868,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     for (int i = tid; i < EPB * 2 * DIM; i += C::BLOCK) {
         const int ib = i / (2 * DIM), f = i - ib * 2 * DIM;
         sNbr[ib][f] = A.nbr[(size_t)min(e0 + ib, NE - 1) * 2 * DIM + f];
+        if constexpr (GEN) sOr[ib][f] = A.forient[(size_t)min(e0 + ib, NE - 1) * 2 * DIM + f];
     }
     double s[NEQ], w[NEQ];
     if (active) {
@@ -774,15 +907,23 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
 #pragma unroll
             for (int v = 0; v < NEQ; v++) so[v] = A.u[v * vs + (size_t)(e0 + fe) * NN + nown];
             conserv_to_entropy_ref<DIM>(so, A.P, sLR[fe], wo);
-            const double *gm = A.geom + (size_t)(e0 + fe) * (1 + DIM * DIM) + 1 + dir * DIM;
+            const double *gm = GEN ? A.ngeom + ((size_t)(e0 + fe) * NN + nown) * (1 + DIM * DIM) + 1 + dir * DIM
+                                   : A.geom + (size_t)(e0 + fe) * (1 + DIM * DIM) + 1 + dir * DIM;
 #pragma unroll
             for (int c = 0; c < DIM; c++) nor[c] = side ? gm[c] : -gm[c];
             bc_lifting_flux<DIM>(A.bc[-en - 1], A.P, wo, so, nor, fl);
 #pragma unroll
             for (int v = 0; v < NEQ; v++) wn[v] = wo[v] + 2.0 * fl[v];
         } else {
+            int nb = nnb;
+            if constexpr (GEN) {
+                const int code = sOr[fe][2 * dir + side];
+                const int d2 = (code & 7) >> 1, s2 = code & 1;
+                const int st2 = dir_stride<NP>(d2);
+                nb = line_base<NP>(face_point_map<DIM, NP>(code, t), d2) + (s2 ? (NP - 1) * st2 : 0);
+            }
 #pragma unroll
-            for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
+            for (int v = 0; v < NEQ; v++) sn[v] = A.u[v * vs + (size_t)en * NN + nb];
             conserv_to_entropy_ref<DIM>(sn, A.P, sLR[fe], wn);
         }
 #pragma unroll
@@ -796,7 +937,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     ai[2] = node / (NP * NP);
     R invJ = R(0), adj[DIM][DIM], Fc[DIM][NEQ - 1];
     if (active) {
-        const double *gm = A.geom + (size_t)e * (1 + DIM * DIM);
+        const double *gm = GEN ? A.ngeom + ((size_t)e * NN + node) * (1 + DIM * DIM) : A.geom + (size_t)e * (1 + DIM * DIM);
         invJ = (R)rcp(gm[0]);
 #pragma unroll
         for (int r = 0; r < DIM; r++)

@@ -55,6 +55,8 @@ struct prandtl_op_s {
     double *d_alpha = nullptr;    // blending: per-element coefficient of the current RHS
     int *d_nbr = nullptr;
     double *d_geom = nullptr;
+    double *d_ngeom = nullptr;     // general mesh: per-node (det J, adj J), NE x n x (1 + dim^2)
+    int *d_forient = nullptr;     // general mesh: face orientation codes, NE x 2 dim
     double *d_dv = nullptr;       // NS: viscous volume term, (neq-1) x NE x n
     double *d_Ff = nullptr;       // NS: viscous face traces, dim x 2 x (neq-1) x NE x Np^(dim-1)
     double *d_k = nullptr;        // RK register
@@ -96,6 +98,13 @@ DTables<NP> make_tables(const prandtl_op_s *op) {
         }
     T.inv_w0 = op->inv_w0;
     for (int i = 0; i < NP; i++) T.inv_w[i] = 1.0 / op->w[i];
+    // subcell metric weights: sub[i][m] = delta_m0 + sum_{l<=i} w_l D_lm
+    for (int i = 0; i < NP; i++)
+        for (int m = 0; m < NP; m++) {
+            double c = (m == 0) ? 1.0 : 0.0;
+            for (int l = 0; l <= i && l < NP - 1; l++) c += op->w[l] * op->D[l * NP + m];
+            T.sub[i][m] = c;
+        }
     return T;
 }
 
@@ -156,9 +165,13 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
         la.Fff = reinterpret_cast<float *>(op->d_Ff);
         la.grad = S.grad;
         la.bc = op->d_bc;
+        la.ngeom = op->d_ngeom;
+        la.forient = op->d_forient;
         const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
         if constexpr (!MIX) {
-            if (op->nbc)
+            if (op->d_ngeom)
+                hipLaunchKernelGGL((k_lift<DIM, NP, false, true, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+            else if (op->nbc)
                 hipLaunchKernelGGL((k_lift<DIM, NP, false, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
             else
                 hipLaunchKernelGGL((k_lift<DIM, NP, false>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
@@ -193,7 +206,21 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.nan_flag = S.nan_check ? op->d_nan : nullptr;
     ra.alpha = op->d_alpha;
     ra.bc = op->d_bc;
+    ra.ngeom = op->d_ngeom;
+    ra.forient = op->d_forient;
     if constexpr (!MIX) {
+        if (op->d_ngeom) {
+            // general mesh: per-node metrics and oriented faces (boundary faces included)
+            if (op->opts.blending) ra.blend = make_blend_params<NP>(op);
+            using LC = NodeCfg<DIM, NP>;
+            const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+            if (op->opts.blending)
+                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            else
+                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, false, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            HIP_TRY(hipGetLastError());
+            return PRANDTL_OK;
+        }
         if (op->opts.blending || op->nbc) {
             // blending: the kernel computes this evaluation's coefficients from its input state
             // (DGSEMOperator.cpp:547-549) and blends; boundary faces: the conditions' fluxes.

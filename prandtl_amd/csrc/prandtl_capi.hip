@@ -223,7 +223,7 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
 
 int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
-    if (op->nbc) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic meshes");
+    if (op->nbc || op->d_ngeom) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic affine meshes");
     if (op->phys.viscous && op->phys.sutherland)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the Navier-Stokes adjoint covers constant viscosity");
     if (op->opts.surface_flux == PRANDTL_SURF_RUSANOV)
@@ -438,6 +438,69 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
     return PRANDTL_OK;
 }
 
+// the neighbour's face point of face point t for orientation code `code` (host copy of
+// face_point_map in dg_kernels.hpp)
+int host_face_point(int dim, int np, int code, int t) {
+    const int t1 = t % np, t2 = t / np;
+    const bool sw = (code & 32) != 0;
+    const int a = sw ? t2 : t1, b = sw ? t1 : t2;
+    const int u1 = (code & 8) ? np - 1 - a : a;
+    const int u2 = (code & 16) ? np - 1 - b : b;
+    return dim == 2 ? u1 : u1 + np * u2;
+}
+
+// node of face point t of local face f = 2 d + s
+int host_face_node(int np, int f, int t) {
+    const int d = f >> 1, s = f & 1;
+    const int st = d == 0 ? 1 : (d == 1 ? np : np * np);
+    return (t % st) + (t / st) * st * np + (s ? (np - 1) * st : 0);
+}
+
+// A general mesh (per-node geometry and/or face orientation codes), with the codes of an
+// aligned lattice where face_orientation is null and the element geometry at every node where
+// node_geometry is null: positive det J at every node, reciprocal face matching whose point maps
+// invert each other, and area vectors that agree across every interior face point.
+int validate_general_mesh(const prandtl_mesh_desc *m, const std::vector<double> &ng, const std::vector<int> &fo) {
+    const int dim = m->dim, NE = m->num_elements, nf = 2 * dim, np = m->order + 1;
+    int nn = 1;
+    for (int d = 0; d < dim; d++) nn *= np;
+    const int G = 1 + dim * dim, L = nn / np;
+    for (size_t i = 0; i < (size_t)NE * nn; i++)
+        if (!(ng[i * G] > 0.0) || !std::isfinite(ng[i * G]))
+            return fail(PRANDTL_ERR_MESH, "element " + std::to_string(i / nn) + ": det J must be positive at every node");
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < nf; f++) {
+            const int en = m->neighbors[(size_t)e * nf + f];
+            const std::string where = "element " + std::to_string(e) + " face " + std::to_string(f);
+            if (en >= NE || (en < 0 && -en - 1 >= m->num_bc))
+                return fail(PRANDTL_ERR_MESH, where + ": a neighbour entry is neither an element nor a boundary condition");
+            if (en < 0) continue;
+            const int code = fo[(size_t)e * nf + f], f2 = code & 7;
+            if ((code & ~63) || f2 >= nf || (dim == 2 && (code & 48)))
+                return fail(PRANDTL_ERR_MESH, where + ": bad face orientation code");
+            if (m->neighbors[(size_t)en * nf + f2] != e || (fo[(size_t)en * nf + f2] & 7) != f)
+                return fail(PRANDTL_ERR_MESH, where + ": neighbour table is not reciprocal");
+            const int back = fo[(size_t)en * nf + f2];
+            for (int t = 0; t < L; t++) {
+                const int tn = host_face_point(dim, np, code, t);
+                if (host_face_point(dim, np, back, tn) != t)
+                    return fail(PRANDTL_ERR_MESH, where + ": face orientation codes of the two sides do not invert");
+                // outward area vectors of the two sides at the shared point: equal and opposite
+                const double *a = &ng[((size_t)e * nn + host_face_node(np, f, t)) * G + 1 + (f >> 1) * dim];
+                const double *b = &ng[((size_t)en * nn + host_face_node(np, f2, tn)) * G + 1 + (f2 >> 1) * dim];
+                const double sa = (f & 1) ? 1.0 : -1.0, sb = (f2 & 1) ? 1.0 : -1.0;
+                double nrm = 0.0, diff = 0.0;
+                for (int c = 0; c < dim; c++) {
+                    nrm += a[c] * a[c];
+                    diff += (sa * a[c] + sb * b[c]) * (sa * a[c] + sb * b[c]);
+                }
+                if (!(diff <= 1e-20 * nrm))
+                    return fail(PRANDTL_ERR_MESH, where + ": face area vectors of neighbours disagree (non-conforming face)");
+            }
+        }
+    return PRANDTL_OK;
+}
+
 int validate_mesh(const prandtl_mesh_desc *m) {
     const int dim = m->dim, NE = m->num_elements, nf = 2 * dim, ng = 1 + dim * dim;
     for (int e = 0; e < NE; e++) {
@@ -485,7 +548,9 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     *out = nullptr;
     if (mesh->dim < 2 || mesh->dim > 3) return fail(PRANDTL_ERR_UNSUPPORTED, "dim must be 2 or 3");
     if (mesh->order < 1 || mesh->order + 1 > MAX_NP) return fail(PRANDTL_ERR_UNSUPPORTED, "order must be 1..4");
-    if (mesh->num_elements <= 0 || !mesh->neighbors || !mesh->geometry) return fail(PRANDTL_ERR_ARG, "empty mesh");
+    if (mesh->num_elements <= 0 || !mesh->neighbors || (!mesh->geometry && !mesh->node_geometry))
+        return fail(PRANDTL_ERR_ARG, "empty mesh");
+    const bool general = mesh->node_geometry || mesh->face_orientation;
     if (!(phys->gamma > 1.0)) return fail(PRANDTL_ERR_ARG, "gamma must exceed 1");
     prandtl_opts o{};
     if (opts) o = *opts;
@@ -511,7 +576,37 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     }
     if (mesh->num_bc > 0 && o.precision != PRANDTL_FP64)
         return fail(PRANDTL_ERR_UNSUPPORTED, "boundary faces run in FP64");
-    int rc = validate_mesh(mesh);
+    if (general && o.precision != PRANDTL_FP64)
+        return fail(PRANDTL_ERR_UNSUPPORTED, "general meshes run in FP64");
+    // general mesh: per-node geometry and orientation codes, filled in from the element geometry
+    // and the aligned-lattice convention where the caller gives none
+    std::vector<double> hng;
+    std::vector<int> hfo;
+    int rc;
+    if (general) {
+        const int dim = mesh->dim, np = mesh->order + 1, G = 1 + dim * dim, nf = 2 * dim;
+        int nn = 1;
+        for (int d = 0; d < dim; d++) nn *= np;
+        const size_t NE = (size_t)mesh->num_elements;
+        if (mesh->node_geometry) {
+            hng.assign(mesh->node_geometry, mesh->node_geometry + NE * nn * G);
+        } else {
+            hng.resize(NE * nn * G);
+            for (size_t e = 0; e < NE; e++)
+                for (int i = 0; i < nn; i++)
+                    std::copy(mesh->geometry + e * G, mesh->geometry + (e + 1) * G, hng.begin() + (e * nn + i) * G);
+        }
+        if (mesh->face_orientation) {
+            hfo.assign(mesh->face_orientation, mesh->face_orientation + NE * nf);
+        } else {
+            hfo.resize(NE * nf);
+            for (size_t e = 0; e < NE; e++)
+                for (int f = 0; f < nf; f++) hfo[e * nf + f] = f ^ 1;
+        }
+        rc = validate_general_mesh(mesh, hng, hfo);
+    } else {
+        rc = validate_mesh(mesh);
+    }
     if (rc) return rc;
 
     prandtl_op_s *op = new prandtl_op_s();
@@ -547,16 +642,33 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         return code;
     };
     const size_t nb = (size_t)op->NE * 2 * op->dim * sizeof(int);
-    const size_t gb = (size_t)op->NE * (1 + op->dim * op->dim) * sizeof(double);
+    const size_t G = 1 + op->dim * op->dim;
+    const size_t gb = (size_t)op->NE * G * sizeof(double);
+    // per-element geometry: the caller's, else node 0's of the per-node geometry (general meshes
+    // read only the per-node arrays)
+    std::vector<double> hgeom;
+    if (!mesh->geometry) {
+        hgeom.resize((size_t)op->NE * G);
+        for (size_t el = 0; el < (size_t)op->NE; el++)
+            std::copy(hng.begin() + el * op->nn * G, hng.begin() + el * op->nn * G + G, hgeom.begin() + el * G);
+    }
+    const double *geometry = mesh->geometry ? mesh->geometry : hgeom.data();
     hipError_t e;
     if ((e = hipMalloc(&op->d_nbr, nb)) != hipSuccess ||
         (e = hipMemcpy(op->d_nbr, mesh->neighbors, nb, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMalloc(&op->d_geom, gb)) != hipSuccess ||
-        (e = hipMemcpy(op->d_geom, mesh->geometry, gb, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(op->d_geom, geometry, gb, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMalloc(&op->d_lam, sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc(&op->d_nan, sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&op->d_alpha, (size_t)op->NE * sizeof(double))) != hipSuccess)
         return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+    if (general) {
+        if ((e = hipMalloc(&op->d_ngeom, hng.size() * sizeof(double))) != hipSuccess ||
+            (e = hipMemcpy(op->d_ngeom, hng.data(), hng.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMalloc(&op->d_forient, hfo.size() * sizeof(int))) != hipSuccess ||
+            (e = hipMemcpy(op->d_forient, hfo.data(), hfo.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+    }
     if (mesh->num_bc > 0) {
         static_assert(sizeof(BcDev) == sizeof(prandtl_bc), "BcDev mirrors prandtl_bc");
         op->nbc = mesh->num_bc;
@@ -579,6 +691,8 @@ void prandtl_op_destroy(prandtl_op op) {
     if (!op) return;
     (void)hipFree(op->d_nbr);
     (void)hipFree(op->d_geom);
+    (void)hipFree(op->d_ngeom);
+    (void)hipFree(op->d_forient);
     (void)hipFree(op->d_dv);
     (void)hipFree(op->d_Ff);
     (void)hipFree(op->d_k);
@@ -785,7 +899,7 @@ int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double 
     if (op->opts.stage_hooks)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the filter or limiter hooks");
     if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
-    if (op->nbc) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic meshes");
+    if (op->nbc || op->d_ngeom) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic affine meshes");
     if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
     hipStream_t st = (hipStream_t)stream;
     const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;

@@ -129,3 +129,59 @@ def test_boundary_condition_validation():
     # mixed precision
     rc, msg, h = _create_bc(m, [wall, wall], precision=capi.MIXED)
     assert rc == -2 and "FP64" in msg
+
+
+def _create_general(m, node_geom, face_orient, bcs=(), **kw):
+    lib = capi.load()
+    nbr = np.ascontiguousarray(m.neighbors, dtype=np.int32)
+    arr = (capi.Bc * max(len(bcs), 1))(*bcs)
+    ng = None if node_geom is None else np.ascontiguousarray(node_geom, dtype=np.float64)
+    fo = None if face_orient is None else np.ascontiguousarray(face_orient, dtype=np.int32)
+    md = capi.MeshDesc(m.dim, m.order, m.num_elements, nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       None, len(bcs), arr if bcs else None,
+                       None if ng is None else ng.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                       None if fo is None else fo.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    ph = capi.Physics(1.4, 0.72, 0.0, 287.0, 0, 0)
+    op = capi.Opts(0, kw.get("precision", 0), 0, 0)
+    h = ctypes.c_void_p()
+    rc = lib.prandtl_op_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
+    if h.value:
+        lib.prandtl_op_destroy(h)
+    return rc, lib.prandtl_last_error().decode()
+
+
+def test_general_mesh_validation():
+    """Mesh checks of prandtl_op_create that run before any device call: a valid general mesh
+    passes them (creation then succeeds, or fails at device allocation without a GPU)."""
+    from . import meshgen as G
+
+    m, _ = G.rotate_frames(G.curved_periodic(2, 3, 4), seed=5, periods=[1.0, 1.0])
+    rc, msg = _create_general(m, m.node_geom, m.face_orient)
+    assert rc == 0 or (rc == -3 and "device" in msg), msg
+    # flipped codes on every face: the two sides' point maps no longer meet
+    rc, msg = _create_general(m, m.node_geom, m.face_orient ^ 8)
+    assert rc == -4 and ("non-conforming" in msg or "invert" in msg), msg
+    # a face that names the wrong face of its neighbour
+    w = m.face_orient.copy()
+    e = int(np.argmax(m.neighbors[:, 0] >= 0))
+    w[e, 0] = (w[e, 0] & ~7) | ((w[e, 0] & 7) ^ 1)
+    rc, msg = _create_general(m, m.node_geom, w)
+    assert rc == -4 and "reciprocal" in msg, msg
+    # swap / second-coordinate bits in 2D
+    w = m.face_orient.copy()
+    w[e, 0] |= 32
+    rc, msg = _create_general(m, m.node_geom, w)
+    assert rc == -4 and "orientation code" in msg, msg
+    # a node with det J <= 0
+    ng = m.node_geom.copy()
+    ng[0, 5, 0] = -1.0
+    rc, msg = _create_general(m, ng, m.face_orient)
+    assert rc == -4 and "det J" in msg, msg
+    # mixed precision
+    rc, msg = _create_general(m, m.node_geom, m.face_orient, precision=capi.MIXED)
+    assert rc == -2 and "FP64" in msg, msg
+    # 3D: swapped and flipped faces pass
+    m3, _ = G.rotate_frames(G.perturbed_multilinear(3, 3, 3), seed=6, periods=[1.0, 1.0, 1.0])
+    assert ((m3.face_orient >> 5) & 1).any()
+    rc, msg = _create_general(m3, m3.node_geom, m3.face_orient)
+    assert rc == 0 or (rc == -3 and "device" in msg), msg

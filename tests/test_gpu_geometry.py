@@ -1,0 +1,222 @@
+"""General meshes on the GPU against the oracle (per-node metrics and oriented faces restated
+in oracle/prandtl_oracle.cpp and pinned in tests/test_oracle_geometry.py), through the C ABI.
+
+* the RHS on curved periodic quadrilaterals and perturbed trilinear hexahedra whose element
+  frames are rotated at random (faces meet flipped, swapped and in other directions): Euler and
+  Navier-Stokes, the three surface fluxes, with and without subcell blending -- every entry
+  within 64 eps of its rounding scale, the wave speed to 1e-12;
+* the reference's Gmsh meshes (the lid-driven cavity with moving no-slip walls; the
+  forward-facing step, unstructured, with supersonic inflow / outflow and slip walls), with
+  blending;
+* aligned lattices given as general meshes (per-node geometry only, codes only) give the
+  lattice kernels' RHS;
+* SSP-RK3 / LSRK4(5) runs on a rotated curved mesh and on the cavity against the oracle's
+  stepping: 1e-11 relative;
+* the C ABI refuses inconsistent face codes, non-conforming faces, and mixed precision and the
+  adjoint on general meshes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from prandtl_amd import capi, cases, mesh as M
+
+from . import meshgen as G
+
+pytestmark = pytest.mark.gpu
+MESHES = os.path.join(os.path.dirname(__file__), "golden", "meshes")
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _state(m, seed, vel=(0.3, -0.2, 0.1), amp=0.04):
+    rng = np.random.default_rng(seed)
+    X = m.node_coords()
+    k = 2 * np.pi
+    rho = 1.0 + 0.1 * np.sin(k * X[0] + 0.3) * np.cos(k * X[1]) + amp * rng.uniform(-1, 1, X[0].shape)
+    v = [vel[d] + 0.1 * np.cos(k * X[(d + 1) % m.dim] + d) + amp * rng.uniform(-1, 1, X[0].shape)
+         for d in range(m.dim)]
+    p = 1.0 + 0.1 * np.cos(k * (X[0] + X[1])) + amp * rng.uniform(-1, 1, X[0].shape)
+    return cases.prim_to_conserv(rho, v, p)
+
+
+def _compare(m, u, kw, device, obcs=None, blending=False, bound=64):
+    import torch
+
+    cbcs = [capi.bc(b.type, *list(b.q)) for b in obcs] if obcs else None
+    op = capi.Operator(m, bcs=cbcs, blending=blending, **kw)
+    d = torch.from_numpy(u).to(device)
+    du = torch.empty_like(d)
+    op.mult(d, du)
+    lam = op.get_max_char_speed()
+    a = op.blend_coefficients(d)[0].cpu().numpy() if blending else None
+    out = du.cpu().numpy()
+    op.close()
+    ph = oracle.phys(**{k: v for k, v in kw.items() if k in ("viscous", "mu", "Pr", "surface_flux", "R_gas")})
+    ref, lref, mag = oracle.rhs_ex(m, ph, u, obcs, alpha=a)
+    s = oracle.scaled_error(out, ref, mag)
+    rel = np.abs(out - ref).max() / np.abs(ref).max()
+    print(f"[geom] rel.err {rel:.3e}, entrywise {s:.2f} eps*mag, lambda {lam:.12e} vs {lref:.12e}", flush=True)
+    assert s <= bound
+    assert abs(lam - lref) <= 1e-12 * lref
+    return out, a
+
+
+def _curved(dim):
+    if dim == 2:
+        m = G.curved_periodic(2, 3, 4)
+        return G.rotate_frames(m, seed=5, periods=[1.0, 1.0])
+    m = G.perturbed_multilinear(3, 3, 3)
+    return G.rotate_frames(m, seed=6, periods=[1.0, 1.0, 1.0])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("viscous", [False, True])
+@pytest.mark.parametrize("surf", [0, 1, 2])
+def test_rotated_curved_meshes(device, dim, viscous, surf):
+    _say(f"rotated curved mesh dim={dim} viscous={viscous} surf={surf}")
+    m, _ = _curved(dim)
+    assert (m.face_orient >> 3).any()
+    u = _state(m, 10 * dim + surf)
+    kw = dict(viscous=viscous, mu=0.02 if viscous else 0.0, Pr=0.72, surface_flux=surf)
+    _compare(m, u, kw, device)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_rotated_curved_meshes_blending(device, dim, viscous):
+    _say(f"rotated curved mesh with blending dim={dim} viscous={viscous}")
+    m, _ = _curved(dim)
+    u = _state(m, 3 + dim)
+    X = m.node_coords()
+    u[0] *= 1.0 + 0.3 * np.tanh((X[0] - 0.5) / 0.03)       # a front, so that blending acts
+    kw = dict(viscous=viscous, mu=0.02 if viscous else 0.0, Pr=0.72)
+    _, a = _compare(m, u, kw, device, blending=True)
+    assert (a > 0).any()
+
+
+def test_gmsh_cavity(device):
+    _say("Gmsh lid-driven cavity, NS, blending")
+    names = {"left": 0, "right": 0, "bottom": 0, "lid": 1}
+    gm = M.gmsh_mesh(os.path.join(MESHES, "LidDrivenCavity.msh"), 3, names)
+    mr, P = G.rotate_frames(gm, seed=9, boundary=_boundary_of(gm))
+    walls = [oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 0.0, 0.0, 0.0), oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 1.0, 0.0, 0.0)]
+    kw = dict(viscous=True, mu=0.02, Pr=0.72, R_gas=287.05)
+    for m in (gm, mr):
+        u = _state(m, 21, vel=(0.0, 0.0))
+        _compare(m, u, kw, device, walls)
+        _compare(m, u, kw, device, walls, blending=True)
+
+
+def _boundary_of(gm):
+    """boundary callable of general_mesh that returns gm's conditions (faces found by corners)"""
+    tol = 1e-9
+    table = {}
+    X = gm.node_coords()
+    for e in range(gm.num_elements):
+        for f in range(2 * gm.dim):
+            if gm.neighbors[e, f] < 0:
+                fn = M.face_nodes(gm.dim, gm.np1, f)
+                rows = [0] if gm.dim == 2 else [0, -1]          # the corners, as general_mesh takes them
+                c = X[:, e, fn[rows][:, [0, -1]].reshape(-1)].T
+                table[tuple(sorted(map(tuple, np.round(c / tol).astype(np.int64))))] = -gm.neighbors[e, f] - 1
+    return lambda c: table[tuple(sorted(map(tuple, np.round(c / tol).astype(np.int64))))]
+
+
+def test_gmsh_step(device):
+    _say("Gmsh forward-facing step, Euler, supersonic in/outflow and slip walls")
+    names = {"left": 0, "right": 1, "top": 2, "bottom": 2}
+    gm = M.gmsh_mesh(os.path.join(MESHES, "ForwardFacingStep.msh"), 3, names)
+    one = np.ones((gm.num_elements, gm.nodes_per_element))
+    a = np.sqrt(1.4)
+    inflow = cases.prim_to_conserv(one[:1, :1], [3.0 * a * one[:1, :1], 0 * one[:1, :1]], one[:1, :1])[:, 0, 0]
+    bcs = [oracle.bc(oracle.BC_SUPERSONIC_INFLOW, *inflow), oracle.bc(oracle.BC_SUPERSONIC_OUTFLOW),
+           oracle.bc(oracle.BC_SLIP_WALL)]
+    u = _state(gm, 31, vel=(3.0 * a, 0.0))
+    _compare(gm, u, {}, device, bcs)
+    _compare(gm, u, {}, device, bcs, blending=True)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lattice_as_general_mesh(device, dim):
+    import torch
+
+    _say(f"aligned lattice as a general mesh dim={dim}")
+    A = np.array([[0.5, 0.1], [0.05, 0.45]]) if dim == 2 else \
+        np.array([[0.5, 0.1, 0.0], [0.05, 0.45, 0.08], [0.02, -0.03, 0.55]])
+    lat = M.periodic_lattice(dim, 3, (3,) * dim, A=A)
+    u = _state(lat, 40 + dim)
+    kw = dict(viscous=True, mu=0.02, Pr=0.72)
+    d = torch.from_numpy(u).to(device)
+    outs = []
+    n = lat.nodes_per_element
+    variants = [lat,
+                M.Mesh(dim, 3, None, None, None, lat.neighbors, lat.geom,
+                       np.repeat(lat.geom[:, None, :], n, axis=1), None, lat.node_coords()),
+                M.Mesh(dim, 3, None, None, None, lat.neighbors, lat.geom, None,
+                       np.tile(np.arange(2 * dim, dtype=np.int32) ^ 1, (lat.num_elements, 1)), lat.node_coords())]
+    for m in variants:
+        op = capi.Operator(m, **kw)
+        du = torch.empty_like(d)
+        op.mult(d, du)
+        outs.append(du.cpu().numpy())
+        op.close()
+    ref, _, mag = oracle.rhs_ex(lat, oracle.phys(viscous=True, mu=0.02, Pr=0.72), u)
+    for o in outs:
+        assert oracle.scaled_error(o, ref, mag) <= 64
+
+
+@pytest.mark.parametrize("scheme,nsteps", [(capi.SSPRK3, 4), (capi.LSRK45, 2)])
+def test_general_mesh_steps(device, scheme, nsteps):
+    import torch
+
+    m2, _ = _curved(2)
+    gm = M.gmsh_mesh(os.path.join(MESHES, "LidDrivenCavity.msh"), 3, {"left": 0, "right": 0, "bottom": 0, "lid": 1})
+    walls = [oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 0.0, 0.0, 0.0), oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 1.0, 0.0, 0.0)]
+    runs = [(m2, _state(m2, 50), dict(viscous=True, mu=0.02, Pr=0.72), None, True),
+            (gm, _state(gm, 51, vel=(0.0, 0.0)), dict(viscous=True, mu=0.02, Pr=0.72, R_gas=287.05), walls, True)]
+    for m, u, kw, bcs, blend in runs:
+        _say(f"steps scheme={scheme} NE={m.num_elements} bcs={bool(bcs)} blending={blend}")
+        lam = cases.max_wave_speed(u)
+        dt = cases.cfl_dt(m, lam, 0.2)
+        ph = oracle.phys(**{k: v for k, v in kw.items() if k in ("viscous", "mu", "Pr", "R_gas")})
+        ref, _ = oracle.rk_run_ex(m, ph, u, scheme, nsteps, dt, bcs, blend=blend)
+        op = capi.Operator(m, bcs=[capi.bc(b.type, *list(b.q)) for b in bcs] if bcs else None, blending=blend, **kw)
+        d = torch.from_numpy(u).to(device)
+        op.rk_run(scheme, d, 0.0, dt, nsteps)
+        torch.cuda.synchronize()
+        out = d.cpu().numpy()
+        op.close()
+        rel = np.abs(out - ref).max() / np.abs(ref).max()
+        print(f"[geom] {nsteps} steps: rel.err {rel:.3e}", flush=True)
+        assert rel <= 1e-11
+
+
+def test_general_mesh_refusals(device):
+    import torch
+
+    _say("general-mesh validation and refusals")
+    m, _ = _curved(2)
+    bad = M.Mesh(2, 3, None, None, None, m.neighbors, m.geom, m.node_geom, m.face_orient ^ 8, m.coords)
+    with pytest.raises(capi.PrandtlError, match="non-conforming|invert"):
+        capi.Operator(bad)
+    wrong = m.face_orient.copy()
+    e = int(np.argmax(m.neighbors[:, 0] >= 0))
+    wrong[e, 0] = (wrong[e, 0] & ~7) | ((wrong[e, 0] & 7) ^ 1)
+    with pytest.raises(capi.PrandtlError, match="reciprocal"):
+        capi.Operator(M.Mesh(2, 3, None, None, None, m.neighbors, m.geom, m.node_geom, wrong, m.coords))
+    ng = m.node_geom.copy()
+    ng[0, 5, 0] = -1.0
+    with pytest.raises(capi.PrandtlError, match="det J"):
+        capi.Operator(M.Mesh(2, 3, None, None, None, m.neighbors, m.geom, ng, m.face_orient, m.coords))
+    with pytest.raises(capi.PrandtlError, match="FP64"):
+        capi.Operator(m, precision=capi.MIXED)
+    op = capi.Operator(m)
+    u = torch.from_numpy(_state(m, 1)).to(device)
+    with pytest.raises(capi.PrandtlError, match="adjoint"):
+        op.vjp(u, u.clone(), torch.empty_like(u))
+    op.close()
