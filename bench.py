@@ -455,6 +455,97 @@ def cavity_case(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def geometry_case(args, dev):
+    """General meshes (per-node metrics, oriented faces):
+    (a) the default TGV 32^3 p=3 NS workload on a curved mesh -- every node of the periodic
+        lattice of [0, 2 pi]^3 moved by x_c += 0.1 prod_d sin(x_d + phase_{c,d}), so every element
+        is a curved cubic map with its own metrics at each node -- against the affine lattice:
+        LSRK4(5) DOF-RHS/s and the per-kernel times of both;
+    (b) the reference's forward-facing-step mesh (Gmsh, unstructured: faces meet in other
+        directions and flipped), uniformly refined twice (54,400 quadrilaterals, p=3: 870,400
+        nodes), Euler at Mach 3 with supersonic inflow / outflow and slip walls, blending on,
+        SSP-RK3 at CFL 0.3 for 200 steps from the uniform stream."""
+    import numpy as np
+    import torch
+
+    from prandtl_amd import capi, cases
+    from prandtl_amd import mesh as M
+
+    out = {"metric": "DOF-RHS evals/s on general meshes (per-node metrics, oriented faces)"}
+    # (a) curved TGV
+    lat = cases.tgv_mesh(n=32, order=3)
+    X = lat.node_coords()
+    Y = X.copy()
+    for c in range(3):
+        s = np.ones_like(X[0])
+        for d in range(3):
+            s = s * np.sin(X[d] + 0.7 * c + 0.3 * d)
+        Y[c] = X[c] + 0.1 * s
+    ng = M.node_geometry(Y, 4)
+    curved = M.Mesh(3, 3, None, None, None, lat.neighbors, np.ascontiguousarray(ng[:, 0, :]), ng, None, Y,
+                    lat.h_min() * 0.7)   # |grad of the displacement| <= 0.3
+    phys = cases.tgv_physics()
+    res = {}
+    for name, m in (("affine", lat), ("curved", curved)):
+        u0 = cases.tgv_state(m)
+        op = capi.Operator(m, **phys)
+        ud = torch.from_numpy(u0).to(dev)
+        lam = op.max_char_speed(ud)
+        dt = cases.cfl_dt(m, lam, 0.5)
+        op.rk_run(capi.LSRK45, ud, 0.0, dt, 5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.rk_run(capi.LSRK45, ud, 0.0, dt, 50)
+        torch.cuda.synchronize()
+        rate = m.num_nodes * 5 * 50 / (time.perf_counter() - t0)
+        lift_ms, rhs_ms = op.profile(ud, epilogue=1, iters=20)
+        res[name] = {"dof_rhs_per_s": rate, "k_lift_ms": lift_ms, "k_rhs_node_lsrk_ms": rhs_ms,
+                     "finite": bool(torch.isfinite(ud).all().item())}
+        op.close()
+        log(f"TGV 32^3 NS {name}: {rate / 1e9:.3f} G DOF-RHS/s, k_lift {lift_ms:.4f} ms, k_rhs {rhs_ms:.4f} ms")
+    out["tgv32_p3_ns"] = res
+    out["curved_cost"] = res["affine"]["dof_rhs_per_s"] / res["curved"]["dof_rhs_per_s"] - 1.0
+    # (b) forward-facing step, refined twice
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "meshes",
+                        "ForwardFacingStep.msh")
+    t0 = time.perf_counter()
+    gm = M.gmsh_mesh(path, 3, {"left": 0, "right": 1, "top": 2, "bottom": 2}, refine=2)
+    build_s = time.perf_counter() - t0
+    a = math.sqrt(1.4)
+    one = np.ones((gm.num_elements, gm.nodes_per_element))
+    u0 = cases.prim_to_conserv(one, [3.0 * a * one, 0.0 * one], one)
+    inflow = u0[:, 0, 0]
+    bcs = [capi.bc(capi.BC_SUPERSONIC_INFLOW, *inflow), capi.bc(capi.BC_SUPERSONIC_OUTFLOW),
+           capi.bc(capi.BC_SLIP_WALL)]
+    op = capi.Operator(gm, bcs=bcs, blending=True)
+    ud = torch.from_numpy(u0).to(dev)
+    lam = op.max_char_speed(ud)
+    dt = cases.cfl_dt(gm, lam, 0.3)
+    op.rk_run(capi.SSPRK3, ud, 0.0, dt, 5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nsteps = 200
+    op.rk_run(capi.SSPRK3, ud, 0.0, dt, nsteps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    al, _ = op.blend_coefficients(ud)
+    u = ud.cpu().numpy()
+    rho, vel, p = cases.conserv_to_prim(u)
+    out["step_mach3"] = {
+        "workload": f"forward-facing step (Gmsh, 3400 quads refined twice: {gm.num_elements} quads p=3, "
+                    f"{gm.num_nodes} nodes), Euler Mach 3, supersonic in/outflow + slip walls, blending, "
+                    f"SSP-RK3 CFL 0.3, {nsteps} steps, dt={dt:.3e}",
+        "mesh_build_s": build_s,
+        "dof_rhs_per_s": gm.num_nodes * 3 * nsteps / wall,
+        "troubled_fraction": float((al > 0).float().mean().item()),
+        "rho_min": float(rho.min()), "p_min": float(p.min()), "finite": bool(np.isfinite(u).all()),
+    }
+    op.close()
+    log(f"step Mach 3: {out['step_mach3']['dof_rhs_per_s'] / 1e9:.3f} G DOF-RHS/s, troubled "
+        f"{out['step_mach3']['troubled_fraction']:.3f}, p_min {out['step_mach3']['p_min']:.3f}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,6 +569,10 @@ def main():
     ap.add_argument("--cavity", action="store_true",
                     help="run the lid-driven cavity (NS, no-slip walls; the reference's 16x16 case and a 256^2 "
                          "throughput run) instead of the default benchmark")
+    ap.add_argument("--geometry", action="store_true",
+                    help="run the general-mesh cases (TGV 32^3 NS on a curved mesh against the affine lattice; "
+                         "the Gmsh forward-facing step refined twice, Mach 3 with blending) instead of the default "
+                         "benchmark")
     ap.add_argument("--adjoint", action="store_true",
                     help="run the discrete-adjoint case (TGV Euler 16^3 p=4 + 64 Fourier modes, 200 RK stages, "
                          "checkpoints every 20 steps) instead of the default benchmark")
@@ -510,6 +605,9 @@ def main():
         return
     if args.cavity:
         cavity_case(args, dev)
+        return
+    if args.geometry:
+        geometry_case(args, dev)
         return
 
     mesh = cases.tgv_mesh(n=args.n, order=args.order)

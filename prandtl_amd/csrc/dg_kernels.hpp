@@ -413,7 +413,6 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     static_assert(!(MIX && BLEND), "subcell blending is FP64 only");
     static_assert(!(MIX && BND), "boundary faces run in FP64");
     static_assert(!(MIX && GEN), "general meshes run in FP64");
-    static_assert(!GEN || BND, "general meshes take the boundary-face path");
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
     __shared__ double sPd[2][EPB * NN];   // rho, beta
     __shared__ R sPv[DIM][EPB * NN];      // velocity
@@ -422,10 +421,14 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     __shared__ double sHref[EPB];         // element reference enthalpy gamma p / ((gamma-1) rho)
     __shared__ int sNbr[EPB][2 * DIM];    // face neighbours of the workgroup's elements
     constexpr int L = ipow(NP, DIM - 1), FITEMS = 2 * DIM * L;
-    __shared__ double sS[NEQ][EPB * NN];  // conserved state
-    __shared__ R sM[GEN ? 1 : EPB][DIM][DIM];            // adj J rows (affine elements)
-    __shared__ double sMn[GEN ? EPB * NN : 1][DIM][DIM];   // GEN: adj J rows at every node
+    // conserved state (GEN without blending reads the element's own traces from global memory
+    // instead: the LDS it saves keeps 4 waves per SIMD in 3D)
+    constexpr bool SSTATE = !(GEN && !BLEND);
+    __shared__ double sS[NEQ][SSTATE ? EPB * NN : 1];
+    __shared__ R sM[GEN ? 1 : EPB][DIM][DIM];              // adj J rows (affine elements)
+    __shared__ double sMr[GEN ? EPB * NN : 1][DIM];        // GEN: adj J row of the current direction, per node
     __shared__ int sOr[GEN ? EPB : 1][2 * DIM];            // GEN: face orientation codes
+    constexpr int G1 = 1 + DIM * DIM;                      // per-node geometry record (GEN)
     __shared__ R sF[EPB * FITEMS][NEQ];   // surface flux (with the viscous face average) per face point
     constexpr int SITEMS = (NP - 1) * L;  // subcell interfaces per element and direction
     static_assert(SITEMS <= FITEMS, "subcell fluxes are staged in the surface-flux buffer");
@@ -465,21 +468,31 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     double qrp = 0.0;   // rho p at the node (BLEND)
     double detJ = 0.0;  // det J at the node (GEN)
     if constexpr (GEN) {
-        if (active) {
-            const double *g = A.ngeom + ((size_t)e * NN + node) * (1 + DIM * DIM);
-            detJ = g[0];
-#pragma unroll
-            for (int r = 0; r < DIM; r++)
-#pragma unroll
-                for (int c = 0; c < DIM; c++) sMn[tid][r][c] = g[1 + r * DIM + c];
-        }
+        if (active) detJ = A.ngeom[((size_t)e * NN + node) * G1];
     }
+    // the conserved state at node nd of the workgroup's element ib
+    auto own_state = [&](int v, int ib, int nd) -> double {
+        if constexpr (SSTATE)
+            return sS[v][ib * NN + nd];
+        else
+            return A.u[v * vs + (size_t)(blk * EPB + ib) * NN + nd];
+    };
+    // GEN: the adj J row `dir` of every node of the workgroup into sMr (callers synchronise)
+    auto load_rows = [&](int dir) {
+        if constexpr (GEN) {
+            if (active) {
+                const double *g = A.ngeom + ((size_t)e * NN + node) * G1 + 1 + dir * DIM;
+#pragma unroll
+                for (int c = 0; c < DIM; c++) sMr[tid][c] = g[c];
+            }
+        }
+    };
     if (active) {
         double s[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
             s[v] = A.u[v * vs + (size_t)e * NN + node];
-            sS[v][tid] = s[v];
+            if constexpr (SSTATE) sS[v][tid] = s[v];
         }
         const SideState<DIM> qs = make_side<DIM>(s, A.P);
         sPd[0][tid] = qs.rho;
@@ -522,9 +535,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 // outward normal, minus the viscous trace the lift kernel stored for it
                 double so[NEQ], nor[DIM], fb[NEQ];
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) so[v] = sS[v][ib * NN + nown];
+                for (int v = 0; v < NEQ; v++) so[v] = own_state(v, ib, nown);
                 const SideState<DIM> qo = make_side<DIM>(so, A.P);
-                const R *m = GEN ? sMn[ib * NN + nown][dir] : sM[ib][dir];
+                const R *m = GEN ? A.ngeom + ((size_t)ee * NN + nown) * G1 + 1 + dir * DIM : sM[ib][dir];
 #pragma unroll
                 for (int c = 0; c < DIM; c++) nor[c] = side ? m[c] : -m[c];
                 lam = fmax(lam, bc_inviscid_flux<DIM, SURF>(A.bc[-en - 1], A.P, A.bug3 != 0, so, qo, nor, fb));
@@ -550,16 +563,16 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             double so[NEQ], sn[NEQ], n1[DIM];
 #pragma unroll
             for (int v = 0; v < NEQ; v++) {
-                so[v] = sS[v][ib * NN + nown];
+                so[v] = own_state(v, ib, nown);
                 sn[v] = A.u[v * vs + (size_t)en * NN + nn2];
             }
-            if (first) {
+            {
+                // CalcOrtho: the lower-numbered element's adj row at its face point, outward
+                const double *g = first ? A.ngeom + ((size_t)ee * NN + nown) * G1 + 1 + dir * DIM
+                                        : A.ngeom + ((size_t)en * NN + nn2) * G1 + 1 + d2 * DIM;
+                const bool out = first ? (side == 1) : (s2 == 1);
 #pragma unroll
-                for (int c = 0; c < DIM; c++) n1[c] = side ? sMn[ib * NN + nown][dir][c] : -sMn[ib * NN + nown][dir][c];
-            } else {
-                const double *g = A.ngeom + ((size_t)en * NN + nn2) * (1 + DIM * DIM) + 1 + d2 * DIM;
-#pragma unroll
-                for (int c = 0; c < DIM; c++) n1[c] = s2 ? g[c] : -g[c];
+                for (int c = 0; c < DIM; c++) n1[c] = out ? g[c] : -g[c];
             }
             double nmag = n1[0] * n1[0];
 #pragma unroll
@@ -592,7 +605,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         double so[NEQ], sn[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
-            so[v] = sS[v][ib * NN + nown];
+            so[v] = own_state(v, ib, nown);
             sn[v] = A.u[v * vs + (size_t)en * NN + nnb];
         }
         R fv[NEQ - 1];
@@ -644,8 +657,55 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
     for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
     double invJ = 0.0;
-    if (active) {
-        invJ = rcp(GEN ? detJ : A.geom[(size_t)e * (1 + DIM * DIM)]);
+    if constexpr (GEN) {
+        // per direction: the nodes' adj rows into sMr, then the pair fluxes with the averaged
+        // metric 1/2 (adj_d(i) + adj_d(j)) (ComputeMean, ChandrashekarFlux.cpp:15)
+        if (active) invJ = rcp(detJ);
+#pragma unroll
+        for (int dir = 0; dir < DIM; dir++) {
+            if (dir > 0) __syncthreads();   // the previous direction's readers of sMr are done
+            load_rows(dir);
+            __syncthreads();
+            if (active) {
+                const int st = dir_stride<NP>(dir);
+                const int a = ai[dir];
+                const int lb = eb * NN + node - a * st;
+                double mo[DIM];
+#pragma unroll
+                for (int c = 0; c < DIM; c++) mo[c] = sMr[tid][c];
+#pragma unroll 1
+                for (int k = 1; k < NP; k++) {
+                    const int b = (a + k) % NP;
+                    const int li = lb + b * st;
+                    NodePrim<DIM> qo, qb;
+                    qo.rho = sPd[0][tid];
+                    qo.beta = sPd[1][tid];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) qo.vel[d] = sPv[d][tid];
+                    qb.rho = sPd[0][li];
+                    qb.beta = sPd[1][li];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
+                    double mp[DIM];
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) mp[c] = (mo[c] + sMr[li][c]) * 0.5;
+                    double F[NEQ];
+                    volume_flux<DIM, true>(qo, qb, mp, A.P.gammaM1Inverse, F);
+                    const double dd = sT[a * NP + b];
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                }
+                if (a == 0 || a == NP - 1) {
+                    const int side = (a == NP - 1) ? 1 : 0;
+                    const int it = eb * FITEMS + (dir * 2 + side) * L + line_index<NP>(node, dir);
+                    const double sg = side ? -1.0 : 1.0;
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
+                }
+            }
+        }
+    } else if (active) {
+        invJ = rcp(A.geom[(size_t)e * (1 + DIM * DIM)]);
         const double p_ref = MIX ? sPref[eb] : 0.0;
         const double h_ref = MIX ? sHref[eb] : 0.0;
 #pragma unroll
@@ -653,7 +713,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             const int st = dir_stride<NP>(dir);
             const int a = ai[dir];
             const int lb = eb * NN + node - a * st;
-            const R *m = sM[GEN ? 0 : eb][dir];
+            const R *m = sM[GEN ? 0 : eb][dir];   // (GEN takes the branch above)
 #pragma unroll 1
             for (int k = 1; k < NP; k++) {
                 const int b = (a + k) % NP;
@@ -681,14 +741,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     qb.beta = sPd[1][li];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
-                    if constexpr (GEN) {
-                        double mp[DIM];   // ComputeMean of the two nodes' metrics
-#pragma unroll
-                        for (int c = 0; c < DIM; c++) mp[c] = (sMn[tid][dir][c] + sMn[li][dir][c]) * 0.5;
-                        volume_flux<DIM, true>(qo, qb, mp, A.P.gammaM1Inverse, F);
-                    } else {
-                        volume_flux<DIM, true>(qo, qb, m, A.P.gammaM1Inverse, F);
-                    }
+                    volume_flux<DIM, true>(qo, qb, m, A.P.gammaM1Inverse, F);
                 }
                 const R dd = sT[a * NP + b];
 #pragma unroll
@@ -720,7 +773,11 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         if (any) {   // the same value in every thread of the workgroup
 #pragma unroll 1
             for (int dir = 0; dir < DIM; dir++) {
-                __syncthreads();   // the previous readers of sF are done
+                __syncthreads();   // the previous readers of sF (and sMr) are done
+                if constexpr (GEN) {
+                    load_rows(dir);
+                    __syncthreads();
+                }
                 const int st = dir_stride<NP>(dir);
                 for (int it = tid; it < EPB * SITEMS; it += C::BLOCK) {
                     const int ib = it / SITEMS, r = it - ib * SITEMS;
@@ -745,7 +802,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         for (int k = 0; k < NP; k++) {
                             const double wk = A.T.sub[i][k];
 #pragma unroll
-                            for (int c = 0; c < DIM; c++) mg[c] += wk * sMn[l0 + k * st][dir][c];
+                            for (int c = 0; c < DIM; c++) mg[c] += wk * sMr[l0 + k * st][c];
                         }
                     } else {
 #pragma unroll
@@ -807,7 +864,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 }
             }
             // the RK accumulation is FP64 in both modes
-            stage_update<NP>(A, v * vs + (size_t)e * NN + node, sS[v][tid], dud, bad);
+            stage_update<NP>(A, v * vs + (size_t)e * NN + node, own_state(v, eb, node), dud, bad);
         }
     }
     finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
@@ -840,7 +897,6 @@ struct LiftCfg {
 template <int DIM, int NP, bool MIX, bool BND = false, bool GEN = false>
 __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES) k_lift(const LiftArgs<NP> A) {
     static_assert(!(MIX && BND), "boundary faces run in FP64");
-    static_assert(!GEN || BND, "general meshes take the boundary-face path");
     using C = LiftCfg<DIM, NP>;
     // derivative contractions, chain rule and viscous flux in R; entropy variables in FP64.
     // Mixed precision contracts w - w_ref (w_ref = the element's node-0 value; D annihilates

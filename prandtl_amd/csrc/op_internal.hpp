@@ -169,8 +169,10 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
         la.forient = op->d_forient;
         const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
         if constexpr (!MIX) {
-            if (op->d_ngeom)
+            if (op->d_ngeom && op->nbc)
                 hipLaunchKernelGGL((k_lift<DIM, NP, false, true, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+            else if (op->d_ngeom)
+                hipLaunchKernelGGL((k_lift<DIM, NP, false, false, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
             else if (op->nbc)
                 hipLaunchKernelGGL((k_lift<DIM, NP, false, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
             else
@@ -214,10 +216,15 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
             if (op->opts.blending) ra.blend = make_blend_params<NP>(op);
             using LC = NodeCfg<DIM, NP>;
             const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
-            if (op->opts.blending)
+            // boundary faces only where the mesh has them (the boundary path costs registers)
+            if (op->opts.blending && op->nbc)
                 hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
-            else
+            else if (op->opts.blending)
+                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            else if (op->nbc)
                 hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, false, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
+            else
+                hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, false, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
             HIP_TRY(hipGetLastError());
             return PRANDTL_OK;
         }

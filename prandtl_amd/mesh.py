@@ -434,7 +434,37 @@ def read_gmsh(path: str) -> dict:
             "cells": {t: (np.array(a, dtype=np.int64), np.array(b, dtype=np.int64)) for t, (a, b) in cells.items()}}
 
 
-def gmsh_mesh(path: str, order: int, bc_index=None, curve=None) -> Mesh:
+def refine_quads(vertices: np.ndarray, quads: np.ndarray, lines=None, levels: int = 1):
+    """Uniform refinement of a quadrilateral mesh (MFEM's Mesh::UniformRefinement, which the
+    reference applies ser_ref_levels times, Simulation.cpp:318-321): each quadrilateral becomes
+    four, with new vertices at the edge midpoints and the centre; boundary lines (tag, v0, v1)
+    split in two.  Returns (vertices, quads, lines)."""
+    V = [tuple(v) for v in np.asarray(vertices, dtype=np.float64)]
+    Q = np.asarray(quads, dtype=np.int64)
+    Ls = None if lines is None else [(int(t), int(a), int(b)) for t, a, b in lines]
+    for _ in range(levels):
+        mids = {}
+
+        def mid(a, b):
+            k = (a, b) if a < b else (b, a)
+            if k not in mids:
+                mids[k] = len(V)
+                V.append(tuple(0.5 * (np.asarray(V[a]) + np.asarray(V[b]))))
+            return mids[k]
+
+        out = []
+        for v0, v1, v2, v3 in Q:
+            m01, m12, m23, m30 = mid(v0, v1), mid(v1, v2), mid(v2, v3), mid(v3, v0)
+            c = len(V)
+            V.append(tuple(0.25 * (np.asarray(V[v0]) + np.asarray(V[v1]) + np.asarray(V[v2]) + np.asarray(V[v3]))))
+            out += [[v0, m01, c, m30], [m01, v1, m12, c], [c, m12, v2, m23], [m30, c, m23, v3]]
+        Q = np.array(out, dtype=np.int64)
+        if Ls is not None:
+            Ls = [piece for t, a, b in Ls for piece in ((t, a, mid(a, b)), (t, mid(a, b), b))]
+    return np.array(V), Q, Ls
+
+
+def gmsh_mesh(path: str, order: int, bc_index=None, curve=None, refine: int = 0) -> Mesh:
     """A general mesh of the linear quadrilaterals (Gmsh type 3; 2D) or hexahedra (type 5; 3D)
     in a Gmsh 2.2 file.  Boundary faces take the physical group of the line (type 1) or
     quadrilateral (type 3, 3D) boundary element on them: ``bc_index`` maps the group's name
@@ -456,6 +486,14 @@ def gmsh_mesh(path: str, order: int, bc_index=None, curve=None) -> Mesh:
         b = V[elems[:, 3]] - V[elems[:, 0]]
         neg = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0] < 0
         elems[neg, 1], elems[neg, 3] = elems[neg, 3].copy(), elems[neg, 1].copy()
+    if refine:
+        if dim != 2:
+            raise ValueError("uniform refinement is implemented for quadrilateral meshes")
+        tags, vs = cells.get(bnd, (np.zeros(0, np.int64), np.zeros((0, 2), np.int64)))
+        V, elems, lines = refine_quads(V, elems, [(t, a, b) for t, (a, b) in zip(tags, vs)], refine)
+        cells = dict(cells)
+        cells[bnd] = (np.array([t for t, _, _ in lines], dtype=np.int64),
+                      np.array([[a, b] for _, a, b in lines], dtype=np.int64).reshape(-1, 2))
     X = multilinear_coords(dim, order, V, elems)
     if curve is not None:
         X = np.asarray(curve(X.reshape(dim, -1)), dtype=np.float64).reshape(X.shape)

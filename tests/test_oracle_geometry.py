@@ -375,3 +375,22 @@ def test_gmsh_step_mesh():
     assert np.max(np.abs(d) / mag.max()) < 64 * EPS
     d, _, mag = oracle.rhs_ex(gm, physics(True), u, bcs=[oracle.bc(oracle.BC_SPECIFIED_STATE, *u[:, 0, 0])])
     assert np.max(np.abs(d) / mag.max()) < 64 * EPS
+
+
+def test_gmsh_refinement():
+    """Uniform refinement (the reference's ser_ref_levels): the cavity refined once is the
+    32 x 32 lattice, walls included; the step refined once keeps its boundary and volume."""
+    gm = M.gmsh_mesh(CAVITY, 2, {"left": 0, "right": 0, "bottom": 0, "lid": 1}, refine=1)
+    lat = M.periodic_lattice(2, 2, (32, 32), lengths=[2.0, 2.0], periodic=(False, False),
+                             boundary={(0, 0): 0, (0, 1): 0, (1, 0): 0, (1, 1): 1})
+    assert gm.num_elements == 1024
+    perm = _match_elements(lat, gm)
+    assert np.abs(gm.node_coords()[:, perm] - lat.node_coords()).max() < 1e-14
+    inv = np.argsort(perm)
+    mapped = np.where(gm.neighbors[perm] >= 0, inv[np.maximum(gm.neighbors[perm], 0)], gm.neighbors[perm])
+    assert np.array_equal(mapped, lat.neighbors)
+    s0 = M.gmsh_mesh(STEP, 1, {"left": 0, "right": 0, "top": 0, "bottom": 0})
+    s1 = M.gmsh_mesh(STEP, 1, {"left": 0, "right": 0, "top": 0, "bottom": 0}, refine=1)
+    assert s1.num_elements == 4 * s0.num_elements
+    assert (s1.neighbors < 0).sum() == 2 * (s0.neighbors < 0).sum()
+    assert abs(s1.quadrature_weights().sum() - s0.quadrature_weights().sum()) < 1e-12 * s0.quadrature_weights().sum()
