@@ -20,6 +20,10 @@
 
 #include <array>
 #include <cstddef>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -296,5 +300,74 @@ private:
 
 // Completes the work queued on the operator's stream.
 inline void Synchronize(const DGSEMOperator &op) { check(prandtl_stream_synchronize(op.Stream())); }
+
+// Checkpoints in the reference's format -- Simulation::Run, src/Simulation/Simulation.cpp:1084-1116
+// (write) and :350-390 (restart): <folder>/Cycle<n>/checkpoint_cycle_<n>.<rank:08d>.chk holds
+// MFEM's GridFunction::Save text of the byNODES DG state (space header L2_T1_<dim>D_P<order>,
+// VDim dim+2, Ordering 0, then one value per line), checkpoint_cycle_<n>.json the time and cycle.
+// Host arrays in the state layout u[v][e][node]; values are written with 17 significant digits.
+inline std::string CheckpointFile(const std::string &folder, int cycle, int rank = 0) {
+    char r[16];
+    std::snprintf(r, sizeof r, "%08d", rank);
+    return folder + "/Cycle" + std::to_string(cycle) + "/checkpoint_cycle_" + std::to_string(cycle) + "." + r + ".chk";
+}
+
+inline void SaveCheckpoint(const std::string &folder, int cycle, double t, const double *u, size_t n, int dim,
+                           int order, int rank = 0) {
+    const std::string dir = folder + "/Cycle" + std::to_string(cycle);
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    if (ec) throw Error(PRANDTL_ERR_ARG, "cannot create " + dir + ": " + ec.message());
+    std::ofstream out(CheckpointFile(folder, cycle, rank));
+    if (!out) throw Error(PRANDTL_ERR_ARG, "cannot write " + CheckpointFile(folder, cycle, rank));
+    out << "FiniteElementSpace\nFiniteElementCollection: L2_T1_" << dim << "D_P" << order << "\nVDim: " << dim + 2
+        << "\nOrdering: 0\n\n";
+    out.precision(17);
+    for (size_t i = 0; i < n; i++) out << u[i] << '\n';
+    if (rank == 0) {
+        std::ofstream meta(dir + "/checkpoint_cycle_" + std::to_string(cycle) + ".json");
+        meta.precision(17);
+        meta << "{\n \"time\": " << t << ",\n \"cycle\": " << cycle << "\n}\n";
+    }
+}
+
+// Reads the checkpoint of `cycle` into u (resized to the file's values); returns its time.  The
+// space must be the DG space of (dim, order) and the meta file's cycle must match.
+inline double LoadCheckpoint(const std::string &folder, int cycle, int dim, int order, std::vector<double> &u,
+                             int rank = 0) {
+    const std::string dir = folder + "/Cycle" + std::to_string(cycle);
+    std::ifstream meta(dir + "/checkpoint_cycle_" + std::to_string(cycle) + ".json");
+    if (!meta) throw Error(PRANDTL_ERR_ARG, "no meta file in " + dir);
+    std::stringstream ms;
+    ms << meta.rdbuf();
+    const std::string m = ms.str();
+    auto field = [&](const char *key) {
+        const size_t k = m.find(std::string("\"") + key + "\"");
+        if (k == std::string::npos) throw Error(PRANDTL_ERR_ARG, std::string("meta file lacks ") + key);
+        return std::stod(m.substr(m.find(':', k) + 1));
+    };
+    const double t = field("time");
+    if ((int)field("cycle") != cycle) throw Error(PRANDTL_ERR_ARG, "mismatch between cycle number and meta file");
+    std::ifstream in(CheckpointFile(folder, cycle, rank));
+    if (!in) throw Error(PRANDTL_ERR_ARG, "cannot read " + CheckpointFile(folder, cycle, rank));
+    std::string line;
+    std::getline(in, line);
+    if (line != "FiniteElementSpace" && line != "MFEM FiniteElementSpace v1.0")
+        throw Error(PRANDTL_ERR_ARG, "not an MFEM grid function");
+    const std::string fec = "L2_T1_" + std::to_string(dim) + "D_P" + std::to_string(order);
+    bool fec_ok = false, ord_ok = false, vdim_ok = false;
+    while (std::getline(in, line) && !line.empty()) {
+        if (line == "FiniteElementCollection: " + fec) fec_ok = true;
+        if (line == "Ordering: 0") ord_ok = true;
+        if (line == "VDim: " + std::to_string(dim + 2)) vdim_ok = true;
+    }
+    if (!fec_ok || !ord_ok || !vdim_ok)
+        throw Error(PRANDTL_ERR_ARG, "checkpoint space is not " + fec + " with VDim " + std::to_string(dim + 2) +
+                                         ", Ordering 0");
+    u.clear();
+    double v;
+    while (in >> v) u.push_back(v);
+    return t;
+}
 
 }  // namespace prandtl

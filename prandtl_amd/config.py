@@ -1,0 +1,200 @@
+"""The reference's JSON case configuration (``TestCases/*/config.json``) mapped onto this library.
+
+ref: Simulation::LoadConfig and the set-up that follows it (src/Simulation/Simulation.cpp:84-880):
+the compile-time switches (PARABOLIC, SUBCELL_FV_BLENDING, SUTHERLAND, USE_FILTER), the physics
+constants, the mesh file with ``ser_ref_levels`` uniform refinements, the boundary conditions by
+boundary name (``conditions.boundary_conditions``, :490-800), the surface flux (:212-220), the
+ODE solver (:177-206), the time-step settings and the checkpoint settings (:111-116, 350-390).
+
+``load_case`` returns a ``Case``: the mesh, the operator's arguments, the boundary-condition table
+(index k <-> boundary name), the stepping settings.  Boundary values named by the configuration
+(the reference's ConditionFactory keys, C++ constants of each case header) come from
+``REFERENCE_BC_CONSTANTS`` for the reference's constant-valued keys, or from ``bc_values``
+(key -> values); function-valued conditions (``"function"``) are not supported.
+"""
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+from . import mesh as M
+
+SURF_CHANDRASHEKAR = 0
+SSPRK3 = 0
+HOOK_FILTER = 1
+
+# The constant boundary values of the reference's cases (conserved states for the state
+# conditions; wall velocity and heat flux for the walls), restated from the case headers.
+REFERENCE_BC_CONSTANTS = {
+    # TestCases/Euler/2D/ForwardFacingStep/ForwardFacingStep.hpp:23
+    "ForwardFacingStepLeftBCVector": [1.4, 1.4 * 3.0, 0.0, 1.0 / (1.4 - 1.0) + 0.5 * 1.4 * 3.0 * 3.0],
+    # TestCases/Euler/2D/Ramp/Ramp.hpp:28
+    "RampInletBCVector": [1.0, 2.0, 0.0, ((1.0 / 1.4) / 0.4) + 0.5 * 1.0 * 2.0 * 2.0],
+    # TestCases/Euler/2D/DoubleMachReflection/DoubleMachReflection.hpp:57
+    "DoubleMachReflectionLeftBottom1BCVector": [
+        8.0, 8.0 * 7.144709581221619, -8.0 * 4.125,
+        116.5 * 1.0 / (1.4 - 1.0) + 0.5 * 8.0 * (7.144709581221619 * 7.144709581221619 + 4.125 * 4.125)],
+    # TestCases/Euler/2D/BackwardFacingStep/BackwardFacingStep.hpp:38
+    "BackwardFacingStepLeftBCVector": [5.9970, 5.9970 * 98.5914, 0.0,
+                                       11666.5 / (1.4 - 1.0) + 0.5 * 5.9970 * 98.5914 * 98.5914],
+    # TestCases/NavierStokes/2D/LidDrivenCavity/LidDrivenCavity.hpp:31, 43, 56
+    "LidDrivenCavityAdiaBCScalar": [0.0],
+    "LidDrivenCavityWallVelBCVector": [0.0, 0.0],
+    "LidDrivenCavityLidVelBCVector": [1.0, 0.0],
+}
+
+# boundary-condition types by the configuration's "type" (Simulation.cpp:520-800)
+BC_TYPES = {"slip": 1, "symmetry": 2, "supersonic-inflow": 3, "supersonic-outflow": 4, "specified-state": 5,
+            "no-slip-adiabatic": 10, "no-slip-isothermal": 11}
+
+# the reference's ODE solvers (Simulation.cpp:177-206) this library steps with a fused kernel
+ODE_SOLVERS = {"RK3SSP": SSPRK3}
+
+
+class ConfigError(ValueError):
+    pass
+
+
+@dataclass
+class Case:
+    path: str
+    dim: int
+    order: int
+    mesh: M.Mesh
+    physics: dict                    # gamma, Pr, mu, R_gas, viscous, sutherland
+    options: dict                    # surface_flux, blending, alpha_max, stage_hooks
+    bc_names: list                   # bc_names[k]: the boundary of condition k
+    bc_specs: list                   # (type, q) per condition
+    ode_solver: str
+    scheme: Optional[int]            # the library's stepper, None when not supported
+    cfl: float
+    dt: Optional[float]
+    variable_dt: bool
+    final_time: float
+    initial_condition: dict
+    checkpoint: dict = field(default_factory=dict)
+    ignored: dict = field(default_factory=dict)
+
+    def bcs(self):
+        """The condition table for prandtl_amd.capi.Operator (capi.Bc structs)."""
+        from . import capi
+
+        return [capi.bc(t, *q) for t, q in self.bc_specs]
+
+    def operator(self, **overrides):
+        """prandtl_amd.capi.Operator of the case (FP64)."""
+        from . import capi
+
+        kw = dict(self.physics)
+        kw.update(self.options)
+        kw.update(overrides)
+        return capi.Operator(self.mesh, bcs=self.bcs() or None, **kw)
+
+
+def _resolve_mesh_file(runtime, config_dir, mesh_dir):
+    name = runtime.get("mesh_file")
+    if not name:
+        raise ConfigError("the configuration names no mesh_file")
+    base = os.path.basename(name)
+    for cand in ([os.path.join(mesh_dir, base)] if mesh_dir else []) + [os.path.join(config_dir, base),
+                                                                          os.path.join(config_dir, name)]:
+        if os.path.exists(cand):
+            return cand
+    raise ConfigError(f"mesh file {name!r} not found (looked in {mesh_dir or config_dir})")
+
+
+def _bc_values(key, bc_values):
+    if bc_values and key in bc_values:
+        return [float(v) for v in bc_values[key]]
+    if key in REFERENCE_BC_CONSTANTS:
+        return list(REFERENCE_BC_CONSTANTS[key])
+    raise ConfigError(f"no values for the boundary key {key!r}: pass bc_values={{{key!r}: [...]}}")
+
+
+def _bc_spec(name, props, dim, bc_values):
+    typ = props.get("type")
+    if typ == "axis":
+        raise ConfigError(f"boundary {name!r}: 'axis' belongs to the axisymmetric build, which is not supported")
+    if typ not in BC_TYPES:
+        raise ConfigError(f"boundary {name!r}: unknown condition type {typ!r}")
+    kind = BC_TYPES[typ]
+    if typ in ("slip", "symmetry", "supersonic-outflow"):
+        return kind, []
+    if typ in ("supersonic-inflow", "specified-state"):
+        if "vector" not in props:
+            raise ConfigError(f"boundary {name!r}: function-valued states are not supported")
+        q = _bc_values(props["vector"], bc_values)
+        if len(q) != dim + 2:
+            raise ConfigError(f"boundary {name!r}: {len(q)} state values for {dim + 2} equations")
+        return kind, q
+    # no-slip walls: velocity vector, then the heat-flux (adiabatic) or temperature (isothermal) scalar
+    vel = props.get("velocity", {})
+    sec = props.get("heat" if typ == "no-slip-adiabatic" else "temperature", {})
+    if "vector" not in vel or "scalar" not in sec:
+        raise ConfigError(f"boundary {name!r}: function-valued wall conditions are not supported")
+    v = _bc_values(vel["vector"], bc_values)
+    s = _bc_values(sec["scalar"], bc_values)
+    if len(v) != dim or len(s) != 1:
+        raise ConfigError(f"boundary {name!r}: wall values of the wrong size")
+    return kind, v + s
+
+
+def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: Optional[int] = None) -> Case:
+    """Read a reference configuration file.  ``mesh_dir``: where to look for the mesh file (by its
+    base name; default the configuration's directory); ``bc_values``: values of boundary keys
+    that REFERENCE_BC_CONSTANTS does not hold; ``order`` overrides the configuration's."""
+    with open(path) as fh:
+        cfg = json.load(fh)
+    ct = cfg.get("compileTime", {})
+    rt = cfg.get("runTime", {})
+    ignored = {}
+    if ct.get("AXISYMMETRIC"):
+        raise ConfigError("the axisymmetric build (AXISYMMETRIC) is not supported")
+    dim = int(rt.get("dim", 2))
+    if dim not in (2, 3):
+        raise ConfigError(f"dim {dim} is not supported (2 or 3)")
+    p = int(order if order is not None else rt.get("order", 3))
+    neq = int(rt.get("num_equations", dim + 2))
+    if neq != dim + 2:
+        raise ConfigError(f"num_equations {neq} != dim + 2")
+    if rt.get("lifting_scheme", "BR1") != "BR1":
+        raise ConfigError(f"lifting scheme {rt.get('lifting_scheme')!r} is not supported (BR1)")
+    if rt.get("numerical_flux", "Chandrashekar") != "Chandrashekar":
+        raise ConfigError(f"numerical flux {rt.get('numerical_flux')!r} (the reference accepts only Chandrashekar)")
+    viscous = bool(ct.get("PARABOLIC", False))
+    physics = dict(gamma=float(rt.get("gamma", 1.4)), Pr=float(rt.get("Pr", 0.72)),
+                   mu=float(rt.get("mu", 0.0)), R_gas=float(rt.get("R_gas", 287.0)), viscous=viscous,
+                   sutherland=bool(ct.get("SUTHERLAND", False)))
+    options = dict(surface_flux=SURF_CHANDRASHEKAR, blending=bool(ct.get("SUBCELL_FV_BLENDING", False)),
+                   alpha_max=float(rt.get("alpha_max", 0.5)),
+                   stage_hooks=HOOK_FILTER if ct.get("USE_FILTER", False) else 0)
+    # boundary conditions in the configuration's order; k indexes the condition table
+    bcs_cfg = rt.get("conditions", {}).get("boundary_conditions", {}) or {}
+    bc_names, bc_specs = [], []
+    for name, props in bcs_cfg.items():
+        bc_names.append(name)
+        bc_specs.append(_bc_spec(name, props, dim, bc_values))
+    bc_index = {n: k for k, n in enumerate(bc_names)}
+    mpath = _resolve_mesh_file(rt, os.path.dirname(os.path.abspath(path)), mesh_dir)
+    refine = int(rt.get("ser_ref_levels", 0) or 0)
+    if mpath.endswith(".msh"):
+        mesh = M.gmsh_mesh(mpath, p, bc_index, refine=refine)
+    else:
+        # the reference's 2D MFEM meshes are periodic (no boundary); boundary names of MFEM meshes are
+        # attribute sets the reference defines for its generated 1D meshes only
+        if bc_names:
+            raise ConfigError("boundary conditions on an MFEM .mesh file are not supported (Gmsh meshes name them)")
+        mesh = M.mfem_mesh(mpath, p, None, refine=refine)
+    if "mesh_ordering" in rt:
+        ignored["mesh_ordering"] = "element reordering permutes elements without changing the RHS"
+    solver = rt.get("ode_solver", "RK3SSP")
+    ic = rt.get("conditions", {}).get("initial_conditions", {}) or {}
+    ckpt = {k: rt[k] for k in ("checkpoint_save", "checkpoint_load", "checkpoint_dt", "checkpoints_folder",
+                               "checkpoint_cycle", "output_file_path") if k in rt}
+    return Case(path=path, dim=dim, order=p, mesh=mesh, physics=physics, options=options, bc_names=bc_names,
+                bc_specs=bc_specs, ode_solver=solver, scheme=ODE_SOLVERS.get(solver),
+                cfl=float(rt.get("cfl", 1.0)), dt=float(rt["dt"]) if "dt" in rt else None,
+                variable_dt=bool(rt.get("variable_dt", False)), final_time=float(rt.get("final_time", 0.0)),
+                initial_condition=dict(ic), checkpoint=ckpt, ignored=ignored)

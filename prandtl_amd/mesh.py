@@ -290,14 +290,40 @@ def _point_keys(P: np.ndarray, periods, tol: float):
     return np.stack(keys, axis=-1)
 
 
-def general_mesh(dim: int, order: int, X: np.ndarray, periods=None, boundary=None, tol=None) -> Mesh:
+def _corner_index(dim: int, bits) -> int:
+    """MFEM's (and Gmsh's) vertex index of the reference corner with coordinates ``bits``."""
+    q = (0, 3, 1, 2)[2 * bits[0] + bits[1]]           # (0,0) 0, (0,1) 3, (1,0) 1, (1,1) 2
+    return q + (4 * bits[2] if dim == 3 else 0)
+
+
+def _face_corner_vertices(dim: int, f: int) -> np.ndarray:
+    """Element vertex index at each corner (c1, c2) of local face f = 2 d + s: array [c2, c1]."""
+    d, s = divmod(f, 2)
+    others = [k for k in range(dim) if k != d]
+    out = np.zeros((2 if dim == 3 else 1, 2), dtype=np.int64)
+    for c2 in range(out.shape[0]):
+        for c1 in range(2):
+            bits = [0] * dim
+            bits[d] = s
+            bits[others[0]] = c1
+            if dim == 3:
+                bits[others[1]] = c2
+            out[c2, c1] = _corner_index(dim, bits)
+    return out
+
+
+def general_mesh(dim: int, order: int, X: np.ndarray, periods=None, boundary=None, tol=None,
+                 vertex_ids=None) -> Mesh:
     """A conforming mesh from the node coordinates X (dim, NE, n) of its elements (each a
     degree-``order`` polynomial map of [0,1]^dim with det J > 0).  Faces are matched by their
     corner points -- modulo ``periods`` (one length or None per direction) for periodic
     meshes -- and the orientation code of each matched face is the one that maps every face
     point onto its neighbour's.  An unmatched face is a boundary face; ``boundary(corners)``
     (corners: (2^(dim-1), dim) points) gives the index k of its condition (neighbour entry
-    -(k+1)).  Per-node metrics come from node_geometry."""
+    -(k+1)).  With ``vertex_ids`` (NE, 2^dim: each element's vertices in MFEM's order) faces are
+    matched by their vertices instead, as MFEM does (periodic meshes then need no periods), and
+    ``boundary`` receives the sorted tuple of the face's vertex ids.  Per-node metrics come from
+    node_geometry."""
     X = np.ascontiguousarray(X, dtype=np.float64)
     np_ = order + 1
     dimX, NE, n = X.shape
@@ -314,15 +340,23 @@ def general_mesh(dim: int, order: int, X: np.ndarray, periods=None, boundary=Non
     corners = [(FN[f][[0] if dim == 2 else cidx][:, cidx]) for f in range(nf)]   # [c2, c1] node ids
     P = np.moveaxis(X, 0, -1)                                                    # (NE, n, dim)
     table = {}
-    ckeys = {}
+    topo = vertex_ids is not None
+    if topo:
+        VI = np.asarray(vertex_ids, dtype=np.int64)
+        assert VI.shape == (NE, 2 ** dim)
+        FCV = [_face_corner_vertices(dim, f) for f in range(nf)]                   # [c2, c1] element vertex index
     for f in range(nf):
-        K = _point_keys(P[:, corners[f].reshape(-1)], periods, tol)             # (NE, ncorner, dim)
-        ckeys[f] = K
-        for e in range(NE):
-            key = tuple(sorted(map(tuple, K[e])))
-            table.setdefault(key, []).append((e, f))
+        if topo:
+            K = VI[:, FCV[f].reshape(-1)]                                           # (NE, ncorner) vertex ids
+            for e in range(NE):
+                table.setdefault(tuple(sorted(K[e].tolist())), []).append((e, f))
+        else:
+            K = _point_keys(P[:, corners[f].reshape(-1)], periods, tol)             # (NE, ncorner, dim)
+            for e in range(NE):
+                table.setdefault(tuple(sorted(map(tuple, K[e]))), []).append((e, f))
     nbr = np.zeros((NE, nf), dtype=np.int32)
     code = np.zeros((NE, nf), dtype=np.int32)
+    cands = (0, 8, 16, 24, 32, 40, 48, 56) if dim == 3 else (0, 8)
     for key, lst in table.items():
         if len(lst) > 2:
             raise ValueError(f"{len(lst)} faces share the corners {key} (a periodic direction needs at least "
@@ -331,19 +365,35 @@ def general_mesh(dim: int, order: int, X: np.ndarray, periods=None, boundary=Non
             e, f = lst[0]
             if boundary is None:
                 raise ValueError(f"face {f} of element {e} has no neighbour and no boundary condition")
-            k = int(boundary(P[e, corners[f].reshape(-1)]))
+            k = int(boundary(key if topo else P[e, corners[f].reshape(-1)]))
             nbr[e, f] = -(k + 1)
             code[e, f] = f ^ 1
             continue
         for (e, f), (e2, f2) in (lst, lst[::-1]):
-            pts = _point_keys(P[e, FN[f]], periods, tol)                          # (Nt2, Nt1, dim)
             found = None
-            for c in (0, 8, 16, 24, 32, 40, 48, 56) if dim == 3 else (0, 8):
-                u1, u2 = _orient_map(dim, np_, c)
-                nb = FN[f2][u2, u1]
-                if np.array_equal(pts, _point_keys(P[e2, nb], periods, tol)):
-                    found = c
-                    break
+            if topo:
+                # corners map to corners: the code that sends each corner to the one with its vertex
+                own = VI[e][FCV[f]]                                                  # [c2, c1]
+                oth = VI[e2][FCV[f2]]
+                for c in cands:
+                    ok = True
+                    for c2 in range(own.shape[0]):
+                        for c1 in range(2):
+                            a, b = (c2, c1) if c & 32 else (c1, c2)
+                            u1 = 1 - a if c & 8 else a
+                            u2 = (1 - b if c & 16 else b) if dim == 3 else 0
+                            ok &= oth[u2, u1] == own[c2, c1]
+                    if ok:
+                        found = c
+                        break
+            else:
+                pts = _point_keys(P[e, FN[f]], periods, tol)                      # (Nt2, Nt1, dim)
+                for c in cands:
+                    u1, u2 = _orient_map(dim, np_, c)
+                    nb = FN[f2][u2, u1]
+                    if np.array_equal(pts, _point_keys(P[e2, nb], periods, tol)):
+                        found = c
+                        break
             if found is None:
                 raise ValueError(f"faces ({e}, {f}) and ({e2}, {f2}) share corners but not their points")
             nbr[e, f] = e2
@@ -372,17 +422,8 @@ def multilinear_coords(dim: int, order: int, vertices: np.ndarray, cells: np.nda
     """Node coordinates (dim, NE, n) of bilinear quadrilaterals / trilinear hexahedra: cells
     (NE, 2^dim) lists each element's vertices in MFEM's (and Gmsh's) order -- counter-clockwise
     from (0,0) in 2D; the bottom face then the top face in 3D."""
-    xi = reference_nodes(dim, order + 1)                    # (dim, n)
     V = np.asarray(vertices, dtype=np.float64)[:, :dim]
-    if dim == 2:
-        x, y = xi
-        N = np.stack([(1 - x) * (1 - y), x * (1 - y), x * y, (1 - x) * y])                     # (4, n)
-    else:
-        x, y, z = xi
-        N = np.stack([(1 - x) * (1 - y) * (1 - z), x * (1 - y) * (1 - z), x * y * (1 - z), (1 - x) * y * (1 - z),
-                      (1 - x) * (1 - y) * z, x * (1 - y) * z, x * y * z, (1 - x) * y * z])
-    C = V[np.asarray(cells)]                                # (NE, 2^dim, dim)
-    return np.einsum("vn,evd->den", N, C)
+    return cell_coords(dim, order, V[np.asarray(cells)])
 
 
 # ---------------------------------------------------------------------------
@@ -434,34 +475,109 @@ def read_gmsh(path: str) -> dict:
             "cells": {t: (np.array(a, dtype=np.int64), np.array(b, dtype=np.int64)) for t, (a, b) in cells.items()}}
 
 
-def refine_quads(vertices: np.ndarray, quads: np.ndarray, lines=None, levels: int = 1):
+def cell_coords(dim: int, order: int, corners: np.ndarray) -> np.ndarray:
+    """Node coordinates (dim, NE, n) of bilinear / trilinear elements given by the coordinates
+    of their corners, (NE, 2^dim, dim) in MFEM's (and Gmsh's) vertex order."""
+    xi = reference_nodes(dim, order + 1)                    # (dim, n)
+    if dim == 2:
+        x, y = xi
+        N = np.stack([(1 - x) * (1 - y), x * (1 - y), x * y, (1 - x) * y])                     # (4, n)
+    else:
+        x, y, z = xi
+        N = np.stack([(1 - x) * (1 - y) * (1 - z), x * (1 - y) * (1 - z), x * y * (1 - z), (1 - x) * y * (1 - z),
+                      (1 - x) * (1 - y) * z, x * (1 - y) * z, x * y * z, (1 - x) * y * z])
+    return np.einsum("vn,evd->den", N, np.asarray(corners, dtype=np.float64)[..., :dim])
+
+
+def refine_quads(cells: np.ndarray, corners: np.ndarray, bfaces=None, levels: int = 1):
     """Uniform refinement of a quadrilateral mesh (MFEM's Mesh::UniformRefinement, which the
-    reference applies ser_ref_levels times, Simulation.cpp:318-321): each quadrilateral becomes
-    four, with new vertices at the edge midpoints and the centre; boundary lines (tag, v0, v1)
-    split in two.  Returns (vertices, quads, lines)."""
-    V = [tuple(v) for v in np.asarray(vertices, dtype=np.float64)]
-    Q = np.asarray(quads, dtype=np.int64)
-    Ls = None if lines is None else [(int(t), int(a), int(b)) for t, a, b in lines]
+    reference applies ser_ref_levels times, Simulation.cpp:293-321): each quadrilateral becomes
+    four, with new vertices at the edge midpoints and the centre.  cells (NE, 4) are vertex ids,
+    corners (NE, 4, 2) the cells' own corner coordinates (so periodic meshes, whose vertex ids
+    wrap, refine correctly; the sub-cells are the bilinear map's images of the four quarter
+    squares); bfaces (optional) is a list of (k, v0, v1) boundary edges, split in two.
+    Returns (cells, corners, bfaces)."""
+    cells = np.asarray(cells, dtype=np.int64)
+    corners = np.asarray(corners, dtype=np.float64)
+    nv = int(cells.max()) + 1 if cells.size else 0
     for _ in range(levels):
         mids = {}
 
         def mid(a, b):
+            nonlocal nv
             k = (a, b) if a < b else (b, a)
             if k not in mids:
-                mids[k] = len(V)
-                V.append(tuple(0.5 * (np.asarray(V[a]) + np.asarray(V[b]))))
+                mids[k] = nv
+                nv += 1
             return mids[k]
 
-        out = []
-        for v0, v1, v2, v3 in Q:
+        NE = cells.shape[0]
+        C = corners
+        # bilinear images of the 3 x 3 points (0, 1/2, 1)^2 of every cell
+        P = {}
+        for i, x in enumerate((0.0, 0.5, 1.0)):
+            for j, y in enumerate((0.0, 0.5, 1.0)):
+                P[i, j] = ((1 - x) * (1 - y)) * C[:, 0] + (x * (1 - y)) * C[:, 1] + (x * y) * C[:, 2] + \
+                          ((1 - x) * y) * C[:, 3]
+        newc = np.zeros((4 * NE, 4), dtype=np.int64)
+        newx = np.zeros((4 * NE, 4, C.shape[2]))
+        for e in range(NE):
+            v0, v1, v2, v3 = (int(v) for v in cells[e])
             m01, m12, m23, m30 = mid(v0, v1), mid(v1, v2), mid(v2, v3), mid(v3, v0)
-            c = len(V)
-            V.append(tuple(0.25 * (np.asarray(V[v0]) + np.asarray(V[v1]) + np.asarray(V[v2]) + np.asarray(V[v3]))))
-            out += [[v0, m01, c, m30], [m01, v1, m12, c], [c, m12, v2, m23], [m30, c, m23, v3]]
-        Q = np.array(out, dtype=np.int64)
-        if Ls is not None:
-            Ls = [piece for t, a, b in Ls for piece in ((t, a, mid(a, b)), (t, mid(a, b), b))]
-    return np.array(V), Q, Ls
+            c = nv
+            nv += 1
+            ids = {(0, 0): v0, (1, 0): m01, (2, 0): v1, (0, 1): m30, (1, 1): c, (2, 1): m12,
+                   (0, 2): v3, (1, 2): m23, (2, 2): v2}
+            for q, (i0, j0) in enumerate(((0, 0), (1, 0), (1, 1), (0, 1))):
+                quad = [(i0, j0), (i0 + 1, j0), (i0 + 1, j0 + 1), (i0, j0 + 1)]
+                newc[4 * e + q] = [ids[t] for t in quad]
+                newx[4 * e + q] = [P[t][e] for t in quad]
+        cells, corners = newc, newx
+        if bfaces is not None:
+            bfaces = [piece for k, a, b in bfaces for piece in ((k, a, mid(a, b)), (k, mid(a, b), b))]
+    return cells, corners, bfaces
+
+
+def _bc_lookup(bfaces, bc_index, where):
+    """boundary callable of general_mesh (vertex ids): boundary face -> condition index"""
+    table = {tuple(sorted(int(v) for v in vs)): name for name, *vs in bfaces}
+    bc_index = bc_index or {}
+
+    def boundary(key):
+        if key not in table:
+            raise ValueError(f"{where}: a boundary face with vertices {key} has no boundary element")
+        name = table[key]
+        if name not in bc_index:
+            raise ValueError(f"{where}: no boundary condition for the boundary group {name!r}")
+        return bc_index[name]
+
+    return boundary
+
+
+def _orient_cells(dim, cells, corners):
+    """Reflect 2D cells whose vertex order gives det J < 0 (MFEM's
+    Mesh::CheckElementOrientation(true) exchanges vertices 1 and 3)."""
+    if dim != 2:
+        return cells, corners
+    a = corners[:, 1] - corners[:, 0]
+    b = corners[:, 3] - corners[:, 0]
+    neg = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0] < 0
+    cells, corners = cells.copy(), corners.copy()
+    cells[neg, 1], cells[neg, 3] = cells[neg, 3].copy(), cells[neg, 1].copy()
+    corners[neg, 1], corners[neg, 3] = corners[neg, 3].copy(), corners[neg, 1].copy()
+    return cells, corners
+
+
+def _assemble(dim, order, cells, corners, bfaces, bc_index, refine, curve, where):
+    cells, corners = _orient_cells(dim, np.asarray(cells, dtype=np.int64), np.asarray(corners, dtype=np.float64))
+    if refine:
+        if dim != 2:
+            raise ValueError("uniform refinement is implemented for quadrilateral meshes")
+        cells, corners, bfaces = refine_quads(cells, corners, bfaces, refine)
+    X = cell_coords(dim, order, corners)
+    if curve is not None:
+        X = np.asarray(curve(X.reshape(dim, -1)), dtype=np.float64).reshape(X.shape)
+    return general_mesh(dim, order, X, boundary=_bc_lookup(bfaces or [], bc_index, where), vertex_ids=cells)
 
 
 def gmsh_mesh(path: str, order: int, bc_index=None, curve=None, refine: int = 0) -> Mesh:
@@ -470,7 +586,8 @@ def gmsh_mesh(path: str, order: int, bc_index=None, curve=None, refine: int = 0)
     quadrilateral (type 3, 3D) boundary element on them: ``bc_index`` maps the group's name
     (or tag) to the condition index k.  Elements whose vertex order gives det J < 0 are
     reflected (vertices 1 and 3 exchanged; MFEM's Mesh::CheckElementOrientation(true) does the
-    same).  ``curve`` (optional) maps node coordinates (dim, m) to curved ones."""
+    same).  ``refine``: levels of uniform refinement (the reference's ser_ref_levels).
+    ``curve`` (optional) maps node coordinates (dim, m) to curved ones."""
     g = read_gmsh(path)
     cells = g["cells"]
     if 5 in cells:
@@ -480,44 +597,101 @@ def gmsh_mesh(path: str, order: int, bc_index=None, curve=None, refine: int = 0)
     else:
         raise ValueError(f"{path}: no quadrilateral or hexahedral elements")
     V = g["vertices"][:, :dim]
-    elems = cells[vol][1].copy()
-    if dim == 2:
-        a = V[elems[:, 1]] - V[elems[:, 0]]
-        b = V[elems[:, 3]] - V[elems[:, 0]]
-        neg = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0] < 0
-        elems[neg, 1], elems[neg, 3] = elems[neg, 3].copy(), elems[neg, 1].copy()
-    if refine:
-        if dim != 2:
-            raise ValueError("uniform refinement is implemented for quadrilateral meshes")
-        tags, vs = cells.get(bnd, (np.zeros(0, np.int64), np.zeros((0, 2), np.int64)))
-        V, elems, lines = refine_quads(V, elems, [(t, a, b) for t, (a, b) in zip(tags, vs)], refine)
-        cells = dict(cells)
-        cells[bnd] = (np.array([t for t, _, _ in lines], dtype=np.int64),
-                      np.array([[a, b] for _, a, b in lines], dtype=np.int64).reshape(-1, 2))
-    X = multilinear_coords(dim, order, V, elems)
-    if curve is not None:
-        X = np.asarray(curve(X.reshape(dim, -1)), dtype=np.float64).reshape(X.shape)
-    span = float(np.max(V.max(axis=0) - V.min(axis=0)))
-    tol = 1e-9 * span
-    bfaces = {}
+    elems = cells[vol][1]
+    bfaces = []
     if bnd in cells:
-        tags, vs = cells[bnd]
-        for tag, vv in zip(tags, vs):
-            P = V[vv]
-            if curve is not None:
-                P = np.asarray(curve(P.T), dtype=np.float64).T
-            key = tuple(sorted(map(tuple, _point_keys(P, None, tol))))
-            name = g["names"].get(int(tag), int(tag))
-            bfaces[key] = name
-    bc_index = bc_index or {}
+        for tag, vv in zip(*cells[bnd]):
+            bfaces.append((g["names"].get(int(tag), int(tag)), *[int(v) for v in vv]))
+    return _assemble(dim, order, elems, V[elems], bfaces, bc_index, refine, curve, path)
 
-    def boundary(corners):
-        key = tuple(sorted(map(tuple, _point_keys(corners, None, tol))))
-        if key not in bfaces:
-            raise ValueError(f"{path}: a boundary face at {corners.tolist()} has no boundary element")
-        name = bfaces[key]
-        if name not in bc_index:
-            raise ValueError(f"{path}: no boundary condition for the physical group {name!r}")
-        return bc_index[name]
 
-    return general_mesh(dim, order, X, boundary=boundary, tol=tol)
+# ---------------------------------------------------------------------------
+# MFEM mesh v1.0 files (the reference's 1D/2D generated cases: IsentropicVortex.mesh,
+# KelvinHelmholtzInstability.mesh, read at Simulation.cpp:272)
+
+def read_mfem_mesh(path: str) -> dict:
+    """Parse an MFEM mesh v1.0 ASCII file: {'dim', 'cells' (NE, 2^dim vertex ids), 'attributes',
+    'boundary' [(attribute, vertex ids...)], 'corners' (NE, 2^dim, dim)}.  Corner coordinates
+    come from the vertices, or, for a periodic mesh, from its 'nodes' section (a first-order L2
+    Gauss-Lobatto grid function: each element's own corner coordinates in lexicographic order)."""
+    with open(path) as fh:
+        toks = [ln.split("#")[0].strip() for ln in fh]
+    lines = [t for t in toks if t]
+    if not lines or not lines[0].startswith("MFEM mesh v1"):
+        raise ValueError(f"{path}: not an MFEM mesh v1.x file")
+    i, dim, cells, attrs, bnd, verts, nodes = 1, None, None, None, [], None, None
+    geom_nv = {1: 2, 3: 4, 5: 8}
+    while i < len(lines):
+        key = lines[i]
+        if key == "dimension":
+            dim = int(lines[i + 1])
+            i += 2
+        elif key == "elements":
+            n = int(lines[i + 1])
+            rows = [[int(v) for v in lines[i + 2 + k].split()] for k in range(n)]
+            for r in rows:
+                if r[1] not in (3, 5) or len(r) != 2 + geom_nv[r[1]]:
+                    raise ValueError(f"{path}: only quadrilaterals and hexahedra are read (geometry {r[1]})")
+            attrs = np.array([r[0] for r in rows], dtype=np.int64)
+            cells = np.array([r[2:] for r in rows], dtype=np.int64)
+            i += 2 + n
+        elif key == "boundary":
+            n = int(lines[i + 1])
+            for k in range(n):
+                r = [int(v) for v in lines[i + 2 + k].split()]
+                bnd.append((r[0], *r[2:]))
+            i += 2 + n
+        elif key == "vertices":
+            nv = int(lines[i + 1])
+            if i + 2 < len(lines) and lines[i + 2] == "nodes":
+                i += 2
+                continue
+            vdim = int(lines[i + 2])
+            verts = np.array([[float(v) for v in lines[i + 3 + k].split()] for k in range(nv)])[:, :vdim]
+            i += 3 + nv
+        elif key == "nodes":
+            hdr = {}
+            j = i + 1
+            while ":" in lines[j] or lines[j] == "FiniteElementSpace":
+                if ":" in lines[j]:
+                    a, b = lines[j].split(":", 1)
+                    hdr[a.strip()] = b.strip()
+                j += 1
+            fec = hdr.get("FiniteElementCollection", "")
+            if not (fec.startswith("L2_T1_") and fec.endswith("_P1")) and not fec.startswith("L2_T"):
+                raise ValueError(f"{path}: nodes in {fec!r} are not read (first-order L2 only)")
+            if not fec.endswith("_P1"):
+                raise ValueError(f"{path}: curved (order > 1) MFEM node grids are not read")
+            vdim = int(hdr.get("VDim", dim))
+            ordering = int(hdr.get("Ordering", 0))
+            vals = np.array([float(v) for ln in lines[j:] for v in ln.split()])
+            nodes = (vals, vdim, ordering)
+            i = len(lines)
+        else:
+            i += 1
+    if dim is None or cells is None:
+        raise ValueError(f"{path}: missing dimension or elements")
+    NE, nc = cells.shape
+    lex_to_corner = [0, 1, 3, 2] if dim == 2 else [0, 1, 3, 2, 4, 5, 7, 6]
+    if nodes is not None:
+        vals, vdim, ordering = nodes
+        total = NE * nc
+        if vals.size < total * vdim:
+            raise ValueError(f"{path}: nodes section too short")
+        vals = vals[:total * vdim]
+        arr = vals.reshape(total, vdim) if ordering == 1 else vals.reshape(vdim, total).T
+        lex = arr.reshape(NE, nc, vdim)
+        corners = np.zeros_like(lex)
+        for L_, c in enumerate(lex_to_corner):
+            corners[:, c] = lex[:, L_]
+    else:
+        corners = verts[cells]
+    return {"dim": dim, "cells": cells, "attributes": attrs, "boundary": bnd, "corners": corners[..., :dim]}
+
+
+def mfem_mesh(path: str, order: int, bc_index=None, refine: int = 0, curve=None) -> Mesh:
+    """A general mesh from an MFEM mesh v1.0 file (quadrilaterals or hexahedra; periodic meshes
+    through their first-order L2 node grid).  ``bc_index`` maps boundary attributes to condition
+    indices; ``refine``: levels of uniform refinement (quadrilaterals)."""
+    g = read_mfem_mesh(path)
+    return _assemble(g["dim"], order, g["cells"], g["corners"], list(g["boundary"]), bc_index, refine, curve, path)

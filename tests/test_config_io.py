@@ -1,0 +1,190 @@
+"""The reference's data formats either side of the operator (CPU): case configurations, MFEM
+meshes and checkpoints.
+
+* the reference's config.json files (tests/golden/configs) map onto the operator's arguments,
+  the boundary-condition table and the stepping settings; what the library cannot run
+  (function-valued boundary states, the axisymmetric build, other fluxes or lifting schemes) is
+  refused by name;
+* the cavity configuration builds the operator of the lid-driven cavity: its RHS (oracle) is
+  the lattice cavity's;
+* MFEM mesh v1.0 files: the periodic vortex mesh refined three times is the 24 x 24 lattice of
+  [-1,1]^2 (coordinates, neighbours through the periodic wrap, the RHS);
+* checkpoints: MFEM GridFunction text with the reference's file names; save/load round trips
+  bit for bit, the C++ mirror and the Python module read each other's files, six-digit files
+  (the reference's default stream precision) read back to six digits, mismatches are refused.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from prandtl_amd import cases, checkpoint, config, mesh as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CONFIGS = os.path.join(HERE, "golden", "configs")
+MESHES = os.path.join(HERE, "golden", "meshes")
+EPS = np.finfo(np.float64).eps
+
+
+def _load(name, **kw):
+    return config.load_case(os.path.join(CONFIGS, name + ".json"), mesh_dir=MESHES, **kw)
+
+
+def test_cavity_configuration():
+    c = _load("LidDrivenCavity")
+    assert (c.dim, c.order) == (2, 3)
+    assert c.physics == dict(gamma=1.4, Pr=0.72, mu=0.02, R_gas=287.05, viscous=True, sutherland=False)
+    assert c.options["blending"] and c.options["alpha_max"] == 0.5 and c.options["stage_hooks"] == 0
+    assert c.bc_names == ["left", "right", "bottom", "lid"]
+    assert c.bc_specs == [(10, [0.0, 0.0, 0.0])] * 3 + [(10, [1.0, 0.0, 0.0])]
+    assert c.scheme == config.SSPRK3 and c.dt == 1e-4 and not c.variable_dt and c.final_time == 3.0
+    assert c.initial_condition["function"] == "LidDrivenCavityIC"
+    assert c.mesh.num_elements == 256 and (c.mesh.neighbors < 0).sum() == 64
+    # the RHS of the configured operator is the lattice cavity's (oracle)
+    lat = M.periodic_lattice(2, 3, (16, 16), lengths=[2.0, 2.0], periodic=(False, False),
+                             boundary={(0, 0): 0, (0, 1): 1, (1, 0): 2, (1, 1): 3})
+    X = lat.node_coords()
+    rho = 1.0 + 0.02 * np.sin(np.pi * X[0]) * np.cos(np.pi * X[1])
+    u = cases.prim_to_conserv(rho, [0.1 * np.sin(np.pi * X[1]), 0.05 * np.cos(np.pi * X[0])], rho * 71.4)
+    ph = oracle.phys(**{k: v for k, v in c.physics.items() if k in ("gamma", "Pr", "mu", "R_gas", "viscous")})
+    bcs = [oracle.bc(t, *q) for t, q in c.bc_specs]
+    # match the Gmsh elements to the lattice's by centres
+    cl = lat.node_coords().mean(axis=2).T
+    cg = c.mesh.node_coords().mean(axis=2).T
+    kg = {tuple(np.round(p, 9)): i for i, p in enumerate(cg)}
+    perm = np.array([kg[tuple(np.round(p, 9))] for p in cl])
+    inv = np.argsort(perm)
+    alpha = np.full(lat.num_elements, 0.2)
+    d1, _, _ = oracle.rhs_ex(c.mesh, ph, u[:, inv], bcs=bcs, alpha=alpha[inv])
+    # bit for bit the operator of the Gmsh mesh built by hand with the cavity's physics and walls
+    gm = M.gmsh_mesh(os.path.join(MESHES, "LidDrivenCavity.msh"), 3, {"left": 0, "right": 1, "bottom": 2, "lid": 3})
+    walls = [oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 0.0, 0.0, 0.0)] * 3 + \
+            [oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 1.0, 0.0, 0.0)]
+    d2, _, _ = oracle.rhs_ex(gm, oracle.phys(viscous=True, mu=0.02, Pr=0.72, R_gas=287.05), u[:, inv], bcs=walls,
+                             alpha=alpha[inv])
+    assert np.array_equal(d1, d2)
+    # and the lattice cavity's, up to the metrics' rounding (the per-node metrics of the file's
+    # mesh come from differentiating its node coordinates)
+    d0, _, mag = oracle.rhs_ex(lat, ph, u, bcs=bcs, alpha=alpha)
+    assert np.max(np.abs(d1[:, perm] - d0)) < 1e-11 * np.abs(d0).max()
+
+
+def test_step_and_periodic_configurations():
+    s = _load("ForwardFacingStep")
+    assert s.mesh.num_elements == 4 * 3400 and not s.physics["viscous"]
+    assert s.bc_names == ["left", "bottom", "right", "top"]
+    assert [t for t, _ in s.bc_specs] == [3, 1, 4, 1]
+    np.testing.assert_allclose(s.bc_specs[0][1], [1.4, 4.2, 0.0, 2.5 + 6.3], rtol=1e-15)
+    v = _load("IsentropicVortex")
+    assert v.mesh.num_elements == 24 * 24 and (v.mesh.neighbors >= 0).all() and v.bc_specs == []
+    k = _load("KelvinHelmholtzInstability")
+    assert k.mesh.num_elements == 96 * 96 and (k.mesh.neighbors >= 0).all()
+    assert v.variable_dt and v.cfl == 1.0 and v.final_time == 2.0
+
+
+def test_refusals():
+    with pytest.raises(config.ConfigError, match="function-valued"):
+        _load("DoubleMachReflection")
+    with pytest.raises(config.ConfigError, match="axisymmetric"):
+        _load("Ramp")
+    with pytest.raises(config.ConfigError, match="not found"):
+        _load("TaylorGreenVortex")          # the reference ships no TaylorGreenVortex.msh
+    # the cavity's values can be overridden; an unknown key must be supplied
+    c = _load("LidDrivenCavity", bc_values={"LidDrivenCavityLidVelBCVector": [0.5, 0.0]})
+    assert c.bc_specs[3] == (10, [0.5, 0.0, 0.0])
+
+
+def test_mfem_periodic_mesh_is_the_lattice():
+    v = M.mfem_mesh(os.path.join(MESHES, "IsentropicVortex.mesh"), 3, refine=3)
+    lat = M.periodic_lattice(2, 3, (24, 24), lengths=[2.0, 2.0], origin=[-1.0, -1.0])
+    cl = lat.node_coords().mean(axis=2).T
+    cv = v.node_coords().mean(axis=2).T
+    kv = {tuple(np.round(p, 6)): i for i, p in enumerate(cv)}
+    perm = np.array([kv[tuple(np.round(p, 6))] for p in cl])
+    # the file's coordinates carry 9 digits (0.333333333)
+    assert np.abs(v.node_coords()[:, perm] - lat.node_coords()).max() < 1e-9
+    inv = np.argsort(perm)
+    assert np.array_equal(inv[v.neighbors[perm]], lat.neighbors)
+    X = lat.node_coords()
+    rho = 1.0 + 0.1 * np.sin(np.pi * X[0]) * np.cos(np.pi * X[1])
+    u = cases.prim_to_conserv(rho, [0.5 + 0 * rho, 0.3 + 0.1 * np.cos(np.pi * X[0])], 1.0 + 0 * rho)
+    ph = oracle.phys()
+    d0, _, mag = oracle.rhs_ex(lat, ph, u)
+    d1, _, _ = oracle.rhs_ex(v, ph, u[:, inv])
+    # the two meshes' metrics differ by the file's 1e-9 rounding of the coordinates
+    assert np.max(np.abs(d1[:, perm] - d0)) < 1e-7 * np.abs(d0).max()
+
+
+def test_checkpoint_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((4, 7, 16)) * np.array([1, 1e-3, 1e3, 1e-300])[:, None, None]
+    folder = str(tmp_path / "Checkpoints")
+    path = checkpoint.save_checkpoint(folder, 120, 0.0125, u, 2, 3)
+    assert path.endswith("Cycle120/checkpoint_cycle_120.00000000.chk")
+    with open(path) as fh:
+        head = [next(fh) for _ in range(5)]
+    assert head == ["FiniteElementSpace\n", "FiniteElementCollection: L2_T1_2D_P3\n", "VDim: 4\n", "Ordering: 0\n", "\n"]
+    back, t, cyc = checkpoint.load_checkpoint(folder, 120, 2, 3, 7)
+    assert np.array_equal(back, u) and t == 0.0125 and cyc == 120
+    with pytest.raises(ValueError, match="dim 2, order 3"):
+        checkpoint.load_checkpoint(folder, 120, 2, 4, 7)
+    # a file written at the reference's default precision (6 digits) reads back to 6 digits
+    with open(path, "w") as fh:
+        fh.write("FiniteElementSpace\nFiniteElementCollection: L2_T1_2D_P3\nVDim: 4\nOrdering: 0\n\n")
+        fh.write("\n".join("%.6g" % x for x in u.reshape(-1)) + "\n")
+    back, _, _ = checkpoint.load_checkpoint(folder, 120, 2, 3, 7)
+    assert np.allclose(back, u, rtol=5e-6, atol=0)
+    with open(checkpoint.meta_path(folder, 120), "w") as fh:
+        fh.write('{"time": 1.0, "cycle": 7}')
+    with pytest.raises(ValueError, match="meta"):
+        checkpoint.load_checkpoint(folder, 120, 2, 3, 7)
+
+
+_CPP = r"""
+#include "prandtl_amd.hpp"
+#include <cstdio>
+int main(int argc, char **argv) {
+    const std::string mode = argv[1], folder = argv[2];
+    if (mode == "write") {
+        std::vector<double> u(5 * 3 * 27);
+        for (size_t i = 0; i < u.size(); i++) u[i] = (i % 7 == 0 ? -1.0 : 1.0) * (0.1 + i) / 3.0;
+        prandtl::SaveCheckpoint(folder, 42, 0.75, u.data(), u.size(), 3, 2);
+        return 0;
+    }
+    std::vector<double> u;
+    const double t = prandtl::LoadCheckpoint(folder, 42, 3, 2, u);
+    std::printf("%.17g %zu", t, u.size());
+    for (double v : u) std::printf(" %.17g", v);
+    std::printf("\n");
+    try {
+        prandtl::LoadCheckpoint(folder, 42, 3, 3, u);
+        return 3;
+    } catch (const prandtl::Error &) {
+    }
+    return 0;
+}
+"""
+
+
+def test_checkpoint_cpp_mirror(tmp_path):
+    src = tmp_path / "ckpt.cpp"
+    src.write_text(_CPP)
+    exe = tmp_path / "ckpt"
+    lib = os.path.join(ROOT, "prandtl_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L", lib, "-lprandtl_amd", f"-Wl,-rpath,{lib}"], check=True)
+    # C++ writes, Python reads
+    f1 = str(tmp_path / "a")
+    subprocess.run([str(exe), "write", f1], check=True)
+    u, t, cyc = checkpoint.load_checkpoint(f1, 42, 3, 2, 3)
+    ref = np.array([(-1.0 if i % 7 == 0 else 1.0) * (0.1 + i) / 3.0 for i in range(5 * 3 * 27)]).reshape(5, 3, 27)
+    assert np.array_equal(u, ref) and t == 0.75 and cyc == 42
+    # Python writes, C++ reads
+    f2 = str(tmp_path / "b")
+    checkpoint.save_checkpoint(f2, 42, 0.5, ref[::-1].copy(), 3, 2)
+    out = subprocess.run([str(exe), "read", f2], check=True, capture_output=True, text=True).stdout.split()
+    assert float(out[0]) == 0.5 and int(out[1]) == ref.size
+    assert np.array_equal(np.array([float(x) for x in out[2:]]), ref[::-1].reshape(-1))
