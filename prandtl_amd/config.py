@@ -198,3 +198,31 @@ def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: 
                 cfl=float(rt.get("cfl", 1.0)), dt=float(rt["dt"]) if "dt" in rt else None,
                 variable_dt=bool(rt.get("variable_dt", False)), final_time=float(rt.get("final_time", 0.0)),
                 initial_condition=dict(ic), checkpoint=ckpt, ignored=ignored)
+
+
+def reference_initial_state(case: Case):
+    """The initial state (dim+2, NE, n) of the reference's constant-state cases, from their
+    initial-condition functions and the configuration's parameters:
+    LidDrivenCavityIC(Ma = x1, gamma = x2): rho = 1 at rest, p = 1/(Ma^2 gamma)
+    (TestCases/NavierStokes/2D/LidDrivenCavity/LidDrivenCavity.hpp:9-19);
+    ForwardFacingStepIC(1/(gamma-1) = x1): rho = 1.4, u = 3, v = 0, p = 1
+    (TestCases/Euler/2D/ForwardFacingStep/ForwardFacingStep.hpp:9-20).  Other cases' initial
+    conditions are functions of x the caller evaluates."""
+    import numpy as np
+
+    ic = case.initial_condition
+    name, par = ic.get("function"), ic.get("params", {})
+    shape = (case.mesh.num_elements, case.mesh.nodes_per_element)
+    u = np.zeros((case.dim + 2,) + shape)
+    if name == "LidDrivenCavityIC":
+        Ma, gamma = float(par.get("x1", 0.0)), float(par.get("x2", 0.0))
+        u[0] = 1.0
+        u[case.dim + 1] = 1.0 / (Ma * Ma * gamma) / (gamma - 1.0)
+        return u
+    if name == "ForwardFacingStepIC":
+        gm1i = float(par.get("x1", 0.0))
+        u[0] = 1.4
+        u[1] = 1.4 * 3.0
+        u[case.dim + 1] = 1.0 * gm1i + 0.5 * 1.4 * 3.0 * 3.0
+        return u
+    raise ConfigError(f"initial condition {name!r} is a function of x: evaluate it at mesh.node_coords()")

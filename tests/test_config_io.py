@@ -188,3 +188,19 @@ def test_checkpoint_cpp_mirror(tmp_path):
     out = subprocess.run([str(exe), "read", f2], check=True, capture_output=True, text=True).stdout.split()
     assert float(out[0]) == 0.5 and int(out[1]) == ref.size
     assert np.array_equal(np.array([float(x) for x in out[2:]]), ref[::-1].reshape(-1))
+
+
+def test_reference_initial_states():
+    c = _load("LidDrivenCavity")
+    u = config.reference_initial_state(c)
+    # rho = 1 at rest, p = 1/(Ma^2 gamma) with Ma = 0.1, gamma = 1.4
+    rho, vel, p = cases.conserv_to_prim(u, gamma=1.4)
+    assert np.all(rho == 1.0) and np.all(vel[0] == 0) and np.allclose(p, 1.0 / (0.01 * 1.4), rtol=1e-15)
+    s = _load("ForwardFacingStep")
+    u = config.reference_initial_state(s)
+    rho, vel, p = cases.conserv_to_prim(u, gamma=1.4)
+    # Mach 3: rho 1.4, p 1, u 3 (c = 1), the supersonic inflow state
+    assert np.allclose(rho, 1.4) and np.allclose(vel[0], 3.0) and np.allclose(p, 1.0, rtol=1e-14)
+    assert np.allclose(u[:, 0, 0], s.bc_specs[0][1], rtol=1e-15)
+    with pytest.raises(config.ConfigError, match="function of x"):
+        config.reference_initial_state(_load("IsentropicVortex"))
