@@ -109,6 +109,9 @@ def _configure(L):
     L.orc_filter_apply.restype = ctypes.c_int
     L.orc_limiter_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]
     L.orc_limiter_apply.restype = ctypes.c_int
+    L.orc_limiter_apply_gen.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
+                                        dp]
+    L.orc_limiter_apply_gen.restype = ctypes.c_int
     L.orc_rk_run_hooked.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, pp, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, ip]
     L.orc_rk_run_hooked.restype = ctypes.c_int
@@ -354,7 +357,12 @@ def filter_apply(mesh, u, bound, gamma=1.4):
 def limiter_apply(mesh, u, gamma=1.4, eps=LIMITER_EPS):
     """Zhang-Shu positivity limiter on a copy of u: (limited u, count)."""
     u = np.array(u, dtype=np.float64, copy=True)
-    c = lib().orc_limiter_apply(mesh.dim, mesh.order, mesh.num_elements, _dp(u), float(gamma), float(eps))
+    ng, _ = _gen(mesh)
+    if ng is not None:
+        c = lib().orc_limiter_apply_gen(mesh.dim, mesh.order, mesh.num_elements, _dp(u), float(gamma), float(eps),
+                                        _dp(ng))
+    else:
+        c = lib().orc_limiter_apply(mesh.dim, mesh.order, mesh.num_elements, _dp(u), float(gamma), float(eps))
     assert c >= 0, c
     return u, c
 

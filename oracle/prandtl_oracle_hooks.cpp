@@ -261,7 +261,9 @@ int FilterApply(int dim, int order, int NE, double *u, double gamma, const doubl
 }
 
 // Zhang-Shu positivity limiter (see the header).  Returns the number of elements changed.
-int LimiterApply(int dim, int order, int NE, double *u, double gamma, double eps) {
+// ngeom (per-node det J, [e][node][1 + dim^2]) non-null: the element mean is the conserved one,
+// sum J w u / sum J w (on affine elements J cancels).
+int LimiterApply(int dim, int order, int NE, double *u, double gamma, double eps, const double *ngeom = nullptr) {
     const int np = order + 1, n = dim == 2 ? np * np : np * np * np, neq = dim + 2;
     const size_t vs = (size_t)NE * n;
     std::vector<double> xg(np), wg(np);
@@ -269,14 +271,18 @@ int LimiterApply(int dim, int order, int NE, double *u, double gamma, double eps
     int count = 0;
 #pragma omp parallel for schedule(static) reduction(+ : count)
     for (int e = 0; e < NE; e++) {
-        double s[125][5], mean[5] = {0, 0, 0, 0, 0};
+        double s[125][5], mean[5] = {0, 0, 0, 0, 0}, wsum = 0.0;
         for (int i = 0; i < n; i++) {
-            const double w = wg[i % np] * wg[(i / np) % np] * (dim == 3 ? wg[i / (np * np)] : 1.0);
+            double w = wg[i % np] * wg[(i / np) % np] * (dim == 3 ? wg[i / (np * np)] : 1.0);
+            if (ngeom) w *= ngeom[((size_t)e * n + i) * (1 + dim * dim)];
+            wsum += w;
             for (int v = 0; v < neq; v++) {
                 s[i][v] = u[v * vs + (size_t)e * n + i];
                 mean[v] += w * s[i][v];
             }
         }
+        if (ngeom)
+            for (int v = 0; v < neq; v++) mean[v] /= wsum;
         bool changed = false;
         // density: theta_1
         double rmin = std::numeric_limits<double>::infinity();
@@ -409,6 +415,12 @@ int orc_limiter_apply(int dim, int order, int NE, double *u, double gamma, doubl
     return LimiterApply(dim, order, NE, u, gamma, eps);
 }
 
+// the limiter on a general mesh (per-node det J in ngeom: the conserved element mean)
+int orc_limiter_apply_gen(int dim, int order, int NE, double *u, double gamma, double eps, const double *ngeom) {
+    if ((dim != 2 && dim != 3) || order < 1 || order > 4) return -1;
+    return LimiterApply(dim, order, NE, u, gamma, eps, ngeom);
+}
+
 // orc_rk_run (prandtl_oracle.cpp) with the stage hooks, in the order of the stepper: the
 // filter's entropy bounds from the state at the start of each step (GetFilterConstraints),
 // then after every stage the filter (FilterSolution) and the limiter (LimitSolution).
@@ -458,7 +470,7 @@ int orc_rk_run_gen(int dim, int order, int NE, const int *nbr, const double *geo
     std::vector<double> k(size), y(size), kk(size), bound(NE), alpha(NE);
     auto hook = [&](double *x) {
         if (hooks & 1) counts[0] += FilterApply(dim, order, NE, x, gamma, bound.data(), nullptr);
-        if (hooks & 2) counts[1] += LimiterApply(dim, order, NE, x, gamma, 1e-8);
+        if (hooks & 2) counts[1] += LimiterApply(dim, order, NE, x, gamma, 1e-8, ngeom);
     };
     auto rhs = [&](const double *x) {
         if (blend) BlendingCoefficient(dim, order, NE, x, gamma, alpha_max, alpha_min, alpha.data(), nullptr);

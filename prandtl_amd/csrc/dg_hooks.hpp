@@ -57,6 +57,8 @@ struct HookArgs {
     double *bound;                 // per-element entropy bound
     int *flag;                     // per-element: the element needs the hook
     unsigned long long *count;     // elements changed: [0] filter, [1] limiter
+    const double *ngeom;           // general mesh: per-node (det J, adj J) -- the limiter's mean is
+                                   // sum J w u / sum J w there; null on affine elements (J cancels)
 };
 
 template <int DIM>
@@ -290,8 +292,11 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
     const size_t vs = (size_t)A.NE * NN;
     const double eps = kLimiterEps;
     double s[NEQ];
+    __shared__ double sW[NN];
     if (node_ok) {
-        const double w = A.T.w[i % NP] * A.T.w[(i / NP) % NP] * (DIM == 3 ? A.T.w[i / (NP * NP)] : 1.0);
+        double w = A.T.w[i % NP] * A.T.w[(i / NP) % NP] * (DIM == 3 ? A.T.w[i / (NP * NP)] : 1.0);
+        if (A.ngeom) w *= A.ngeom[((size_t)e * NN + i) * (1 + DIM * DIM)];
+        sW[i] = w;
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
             s[v] = A.u[v * vs + (size_t)e * NN + i];
@@ -302,6 +307,11 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
     if (i < NEQ) {
         double m = 0.0;
         for (int k = 0; k < NN; k++) m += sR[i][k];
+        if (A.ngeom) {
+            double ws = 0.0;
+            for (int k = 0; k < NN; k++) ws += sW[k];
+            m /= ws;
+        }
         sMean[i] = m;
     }
     __syncthreads();

@@ -317,6 +317,7 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
     a.bound = op->d_fbound;
     a.flag = op->d_hflag;
     a.count = op->d_hcount;
+    a.ngeom = op->d_ngeom;
     using C = HookCfg<DIM, NP>;
     const int sblk = (op->NE + C::EPB - 1) / C::EPB;
     constexpr int ebl = C::NN <= 64 ? 64 : 128;   // one element per workgroup
