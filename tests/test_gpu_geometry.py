@@ -256,3 +256,33 @@ def test_limiter_and_hooked_steps_on_curved_mesh(device):
     rel = np.abs(out - ref).max() / np.abs(ref).max()
     print(f"[geom] hooked steps: rel.err {rel:.3e}, oracle counts {list(counts)}", flush=True)
     assert rel <= 1e-11
+
+
+@pytest.mark.parametrize("order", [1, 2, 4])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_general_meshes_all_orders(device, dim, order):
+    _say(f"rotated general mesh dim={dim} p={order}, NS with blending")
+    if dim == 2:
+        m = G.curved_periodic(2, order, 4)
+        m, _ = G.rotate_frames(m, seed=order, periods=[1.0, 1.0])
+    else:
+        m = G.perturbed_multilinear(3, order, 3)
+        m, _ = G.rotate_frames(m, seed=order, periods=[1.0, 1.0, 1.0])
+    u = _state(m, 70 + order)
+    X = m.node_coords()
+    u[0] *= 1.0 + 0.2 * np.tanh((X[0] - 0.5) / 0.05)
+    _compare(m, u, dict(viscous=True, mu=0.02, Pr=0.72), device)
+    _compare(m, u, dict(viscous=True, mu=0.02, Pr=0.72), device, blending=True)
+
+
+def test_general_3d_walls(device):
+    _say("perturbed trilinear 3D box with no-slip walls and a moving lid")
+    bnd = lambda c: 1 if abs(c[:, 2] - 1.0).max() < 1e-9 else 0
+    m = G.perturbed_multilinear(3, 3, 3, periodic=False, boundary=bnd)
+    m, _ = G.rotate_frames(m, seed=12, boundary=bnd)
+    walls = [oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 0.0, 0.0, 0.0, 0.0),
+             oracle.bc(oracle.BC_NOSLIP_ISOTHERMAL, 0.5, 0.0, 0.0, 1.0 / 287.0)]
+    u = _state(m, 80, vel=(0.05, 0.02, 0.0), amp=0.01)
+    kw = dict(viscous=True, mu=0.02, Pr=0.72, R_gas=287.0)
+    _compare(m, u, kw, device, walls)
+    _compare(m, u, kw, device, walls, blending=True)

@@ -421,3 +421,48 @@ def test_limiter_conserves_on_curved_elements():
     flat = M.Mesh(m.dim, m.order, None, None, None, m.neighbors, m.geom, None, m.face_orient, m.coords)
     out2, _ = oracle.limiter_apply(flat, u)
     assert np.max(np.abs((W * out2[0]).sum(axis=1) - (W * u[0]).sum(axis=1))) > 1e-10
+
+
+def test_gmsh_hexahedra(tmp_path):
+    """A Gmsh 2.2 file of hexahedra (type 5) with quadrilateral boundary elements (type 3) in
+    physical groups: the 3 x 3 x 3 box it describes is the lattice, walls included."""
+    n = 3
+    vid = lambda i, j, k: 1 + i + (n + 1) * (j + (n + 1) * k)
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", "3",
+             '2 1 "sides"', '2 2 "lid"', '3 3 "fluid"', "$EndPhysicalNames", "$Nodes", str((n + 1) ** 3)]
+    for k in range(n + 1):
+        for j in range(n + 1):
+            for i in range(n + 1):
+                lines.append(f"{vid(i, j, k)} {i / n} {j / n} {k / n}")
+    lines.append("$EndNodes")
+    elems = []
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                v = [vid(i, j, k), vid(i + 1, j, k), vid(i + 1, j + 1, k), vid(i, j + 1, k),
+                     vid(i, j, k + 1), vid(i + 1, j, k + 1), vid(i + 1, j + 1, k + 1), vid(i, j + 1, k + 1)]
+                elems.append((5, 3, v))
+    for a in range(n):
+        for b in range(n):
+            elems.append((3, 1, [vid(0, a, b), vid(0, a + 1, b), vid(0, a + 1, b + 1), vid(0, a, b + 1)]))
+            elems.append((3, 1, [vid(n, a, b), vid(n, a + 1, b), vid(n, a + 1, b + 1), vid(n, a, b + 1)]))
+            elems.append((3, 1, [vid(a, 0, b), vid(a + 1, 0, b), vid(a + 1, 0, b + 1), vid(a, 0, b + 1)]))
+            elems.append((3, 1, [vid(a, n, b), vid(a + 1, n, b), vid(a + 1, n, b + 1), vid(a, n, b + 1)]))
+            elems.append((3, 1, [vid(a, b, 0), vid(a + 1, b, 0), vid(a + 1, b + 1, 0), vid(a, b + 1, 0)]))
+            elems.append((3, 2, [vid(a, b, n), vid(a + 1, b, n), vid(a + 1, b + 1, n), vid(a, b + 1, n)]))
+    lines += ["$Elements", str(len(elems))]
+    for k, (t, tag, v) in enumerate(elems):
+        lines.append(f"{k + 1} {t} 2 {tag} {tag} " + " ".join(map(str, v)))
+    lines.append("$EndElements")
+    path = tmp_path / "box.msh"
+    path.write_text("\n".join(lines) + "\n")
+    gm = M.gmsh_mesh(str(path), 3, {"sides": 0, "lid": 1})
+    lat = M.periodic_lattice(3, 3, (n, n, n), lengths=[1.0, 1.0, 1.0], periodic=(False, False, False),
+                             boundary={(0, 0): 0, (0, 1): 0, (1, 0): 0, (1, 1): 0, (2, 0): 0, (2, 1): 1})
+    assert gm.num_elements == 27
+    perm = _match_elements(lat, gm)
+    assert np.abs(gm.node_coords()[:, perm] - lat.node_coords()).max() < 1e-15
+    inv = np.argsort(perm)
+    mapped = np.where(gm.neighbors[perm] >= 0, inv[np.maximum(gm.neighbors[perm], 0)], gm.neighbors[perm])
+    assert np.array_equal(mapped, lat.neighbors)
+    assert np.array_equal(gm.face_orient[perm] & 7, np.tile(np.arange(6) ^ 1, (27, 1)))
