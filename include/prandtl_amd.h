@@ -47,8 +47,11 @@ enum { PRANDTL_FP64 = 0, PRANDTL_MIXED = 1 };
  *   A3: the 3D dissipation term "rho_mean * dw + w_mean" at ChandrashekarFlux.cpp:131. */
 enum { PRANDTL_BUG_A1 = 1, PRANDTL_BUG_A2 = 2, PRANDTL_BUG_A3 = 4 };
 /* ODE solvers -- ref: the MFEM ODESolver chosen at src/Simulation/Simulation.cpp:177-206
- * and stepped at :1010.  LSRK45 is the Carpenter-Kennedy 2N-storage RK4(3)5. */
-enum { PRANDTL_SSPRK3 = 0, PRANDTL_LSRK45 = 1 };
+ * and stepped at :1010.  SSPRK3 is "RK3SSP" (stage update fused into the RHS kernel);
+ * LSRK45 is the Carpenter-Kennedy 2N-storage RK4(3)5 (fused); FORWARD_EULER, RK2 (MFEM's
+ * RK2Solver with its default a = 2/3) and RK4 (MFEM's classic RK4Solver tableau) are
+ * "ForwardEuler", "RK2" and "RK4": RHS evaluations plus one fused stage-combination pass. */
+enum { PRANDTL_SSPRK3 = 0, PRANDTL_LSRK45 = 1, PRANDTL_FORWARD_EULER = 2, PRANDTL_RK2 = 3, PRANDTL_RK4 = 4 };
 
 /* Boundary conditions -- the reference's BdrFaceIntegrator subclasses
  * (src/Integrators/BoundaryConditions/, chosen per boundary attribute at

@@ -260,6 +260,22 @@ protected:
     int scheme() const override { return PRANDTL_LSRK45; }
 };
 
+// MFEM ForwardEulerSolver, RK2Solver (a = 2/3) and RK4Solver (the classic tableau)
+class ForwardEulerSolver : public ODESolver {
+protected:
+    int scheme() const override { return PRANDTL_FORWARD_EULER; }
+};
+
+class RK2Solver : public ODESolver {
+protected:
+    int scheme() const override { return PRANDTL_RK2; }
+};
+
+class RK4Solver : public ODESolver {
+protected:
+    int scheme() const override { return PRANDTL_RK4; }
+};
+
 // Filter -- Filter.hpp:11-30
 class Filter {
 public:

@@ -511,6 +511,45 @@ int orc_rk_run_gen(int dim, int order, int NE, const int *nbr, const double *geo
                 }
                 hook(u);
             }
+        } else if (scheme == 2) {
+            // MFEM ForwardEulerSolver::Step: x.Add(dt, dxdt)
+            if (rhs(u)) return -3;
+            for (size_t i = 0; i < size; i++) u[i] = u[i] + dt * k[i];
+            hook(u);
+        } else if (scheme == 3) {
+            // MFEM RK2Solver::Step with its default a = 2/3, b = 1/(2a): x1 = x + (1-b) dt k;
+            // x += a dt k; x = x1 + b dt f(x)
+            const double a = 2.0 / 3.0, b = 0.5 / a;
+            if (rhs(u)) return -3;
+            for (size_t i = 0; i < size; i++) {
+                kk[i] = u[i] + ((1.0 - b) * dt) * k[i];
+                y[i] = u[i] + (a * dt) * k[i];
+            }
+            hook(y.data());
+            if (rhs(y.data())) return -3;
+            for (size_t i = 0; i < size; i++) u[i] = kk[i] + (b * dt) * k[i];
+            hook(u);
+        } else if (scheme == 4) {
+            // MFEM RK4Solver (ExplicitRKSolver::Step with a = {1/2; 0, 1/2; 0, 0, 1},
+            // b = {1/6, 1/3, 1/3, 1/6}): y_l = x + a dt k (the zero entries add nothing), then
+            // x += b_0 dt k_0, += b_1 dt k_1, ... in order; accumulated here stage by stage
+            static const double A4[3] = {0.5, 0.5, 1.0};
+            static const double B4[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+            const double *in = u;
+            for (int st = 0; st < 4; st++) {
+                if (rhs(in)) return -3;
+                for (size_t i = 0; i < size; i++) {
+                    const double acc = (st == 0 ? u[i] : kk[i]) + (B4[st] * dt) * k[i];
+                    if (st == 3)
+                        u[i] = acc;
+                    else {
+                        kk[i] = acc;
+                        y[i] = u[i] + (A4[st] * dt) * k[i];
+                    }
+                }
+                hook(st == 3 ? u : y.data());
+                in = y.data();
+            }
         } else {
             return -2;
         }

@@ -21,7 +21,7 @@ ERRORS = {-1: "PRANDTL_ERR_ARG", -2: "PRANDTL_ERR_UNSUPPORTED", -3: "PRANDTL_ERR
 SURF_CHANDRASHEKAR, SURF_EC, SURF_RUSANOV = 0, 1, 2
 FP64, MIXED = 0, 1
 BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
-SSPRK3, LSRK45 = 0, 1
+SSPRK3, LSRK45, FORWARD_EULER, RK2, RK4 = 0, 1, 2, 3, 4
 HOOK_FILTER, HOOK_LIMITER = 1, 2
 
 # every symbol the public header declares

@@ -49,8 +49,9 @@ REFERENCE_BC_CONSTANTS = {
 BC_TYPES = {"slip": 1, "symmetry": 2, "supersonic-inflow": 3, "supersonic-outflow": 4, "specified-state": 5,
             "no-slip-adiabatic": 10, "no-slip-isothermal": 11}
 
-# the reference's ODE solvers (Simulation.cpp:177-206) this library steps with a fused kernel
-ODE_SOLVERS = {"RK3SSP": SSPRK3}
+# the reference's ODE solvers (Simulation.cpp:177-206) this library steps (RK6 and RK8 are not
+# available: MFEM's tableaus for them are not restated here)
+ODE_SOLVERS = {"RK3SSP": SSPRK3, "ForwardEuler": 2, "RK2": 3, "RK4": 4}
 
 
 class ConfigError(ValueError):

@@ -16,6 +16,32 @@ __global__ void k_axpby(double *y, double a, const double *x1, double b, const d
         y[i] = a * x1[i] + b * x2[i];
 }
 
+// One stage of an explicit Runge-Kutta method in MFEM's ExplicitRKSolver order, from the stage's
+// RHS k: acc = (first ? x : acc) + (cb dt) k, the running x + sum_j b_j dt k_j; unless last,
+// y = x + (ca dt) k, the next stage's input; when last, the new state acc goes to x.
+__global__ void k_erk_update(double *x, double *acc, double *y, const double *k, double ca, double cb, double dt,
+                             int first, int last, size_t n) {
+    const double bd = cb * dt, ad = ca * dt;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double xi = x[i];
+        const double a = (first ? xi : acc[i]) + bd * k[i];
+        if (last) {
+            x[i] = a;
+        } else {
+            acc[i] = a;
+            y[i] = xi + ad * k[i];
+        }
+    }
+}
+
+// non-finite scan of a state (the explicit RK steppers' NaN check)
+__global__ void k_nan_scan(const double *x, size_t n, int *flag) {
+    int bad = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        bad |= !(x[i] == x[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

@@ -133,7 +133,9 @@ int ensure_rk_buffers(prandtl_op_s *op, int scheme) {
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipDeviceSynchronize());
     }
-    if (scheme == PRANDTL_SSPRK3 && !op->d_y2) {
+    const bool erk = scheme == PRANDTL_FORWARD_EULER || scheme == PRANDTL_RK2 || scheme == PRANDTL_RK4;
+    if (erk && !op->d_k) HIP_TRY(hipMalloc(&op->d_k, bytes));   // the stage RHS
+    if ((scheme == PRANDTL_SSPRK3 || erk) && !op->d_y2) {
         HIP_TRY(hipMalloc(&op->d_y2, bytes));
     }
     return PRANDTL_OK;
@@ -392,7 +394,48 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
             if (rc) return rc;
             op->have_bounds = true;
         }
-        if (scheme == PRANDTL_LSRK45) {
+        if (scheme == PRANDTL_FORWARD_EULER || scheme == PRANDTL_RK2 || scheme == PRANDTL_RK4) {
+            // MFEM's explicit RK steppers: ForwardEulerSolver (x += dt f(x)); RK2Solver with a = 2/3
+            // (x1 = x + (1-b) dt k1, y = x + a dt k1, x = x1 + b dt f(y), b = 1/(2a)); RK4Solver, the
+            // classic tableau a = {1/2; 0, 1/2; 0, 0, 1}, b = {1/6, 1/3, 1/3, 1/6} of ExplicitRKSolver
+            // (y_l = x + a_ll dt k_l, the zero entries contributing nothing; x += sum_l b_l dt k_l).
+            // The stage inputs y ping-pong between d_ub and the caller's buffer; acc in d_y2.
+            static const double rk2a = 2.0 / 3.0, rk2b = 0.5 / rk2a;
+            int ns;
+            double ca[4], cb[4];
+            if (scheme == PRANDTL_FORWARD_EULER) {
+                ns = 1;
+                cb[0] = 1.0;
+            } else if (scheme == PRANDTL_RK2) {
+                ns = 2;
+                ca[0] = rk2a;
+                cb[0] = 1.0 - rk2b;
+                cb[1] = rk2b;
+            } else {
+                ns = 4;
+                ca[0] = 0.5, ca[1] = 0.5, ca[2] = 1.0;
+                cb[0] = 1.0 / 6.0, cb[1] = 1.0 / 3.0, cb[2] = 1.0 / 3.0, cb[3] = 1.0 / 6.0;
+            }
+            double *x = cur, *yin = cur;
+            const unsigned gblk = (unsigned)std::min<size_t>((op->size + 255) / 256, 8192);
+            for (int stg = 0; stg < ns; stg++) {
+                StageSpec S{EPI_MULT, yin, op->d_k, nullptr, nullptr, 0.0, 0.0, dt, nullptr, false, false};
+                rc = run_stage(op, S, st);
+                if (rc) return rc;
+                double *ynext = (x == u) ? op->d_ub : u;
+                const bool last = stg == ns - 1;
+                hipLaunchKernelGGL(k_erk_update, dim3(gblk), dim3(256), 0, st, x, op->d_y2, ynext, op->d_k,
+                                   last ? 0.0 : ca[stg], cb[stg], dt, stg == 0 ? 1 : 0, last ? 1 : 0, op->size);
+                HIP_TRY(hipGetLastError());
+                if (hooks) rc = stage_hooks(op, last ? x : ynext, st);
+                if (rc) return rc;
+                yin = ynext;
+            }
+            if (chk) {
+                hipLaunchKernelGGL(k_nan_scan, dim3(gblk), dim3(256), 0, st, x, op->size, op->d_nan);
+                HIP_TRY(hipGetLastError());
+            }
+        } else if (scheme == PRANDTL_LSRK45) {
             for (int stg = 0; stg < 5; stg++) {
                 double *nxt = (cur == u) ? op->d_ub : u;
                 StageSpec S{EPI_LSRK, cur, nxt, op->d_k, nullptr, CK_A[stg], CK_B[stg], dt, nullptr, false,
@@ -1037,7 +1080,7 @@ int prandtl_rk_step(prandtl_op op, int scheme, double *u, double *t, double dt, 
 int prandtl_rk_run(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
                    prandtl_stream stream) {
     if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
-    if (scheme != PRANDTL_SSPRK3 && scheme != PRANDTL_LSRK45) return fail(PRANDTL_ERR_ARG, "unknown ODE solver");
+    if (scheme < PRANDTL_SSPRK3 || scheme > PRANDTL_RK4) return fail(PRANDTL_ERR_ARG, "unknown ODE solver");
     if (nsteps < 0 || !(dt >= 0.0)) return fail(PRANDTL_ERR_ARG, "bad step count or dt");
     int rc = rk_advance(op, scheme, u, dt, nsteps, nan_check, (hipStream_t)stream);
     if (rc == PRANDTL_OK && t) *t += dt * nsteps;

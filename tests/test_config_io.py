@@ -83,6 +83,20 @@ def test_step_and_periodic_configurations():
     k = _load("KelvinHelmholtzInstability")
     assert k.mesh.num_elements == 96 * 96 and (k.mesh.neighbors >= 0).all()
     assert v.variable_dt and v.cfl == 1.0 and v.final_time == 2.0
+    # MFEM's other solvers by name (the backward-facing step's configuration names RK4)
+    import json
+    import tempfile
+
+    cfg = json.load(open(os.path.join(CONFIGS, "LidDrivenCavity.json")))
+    for name, scheme in (("RK4", 4), ("RK2", 3), ("ForwardEuler", 2), ("RK6", None)):
+        cfg["runTime"]["ode_solver"] = name
+        with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as fh:
+            json.dump(cfg, fh)
+        try:
+            b = config.load_case(fh.name, mesh_dir=MESHES)
+        finally:
+            os.unlink(fh.name)
+        assert b.ode_solver == name and b.scheme == scheme
 
 
 def test_refusals():

@@ -184,3 +184,44 @@ def test_rounding_scale_bounds_build_spread(dim, viscous):
     bad = r[0].copy()
     bad[1, 0, 0] += 1e-9 * max(1.0, abs(bad[1, 0, 0]))
     assert oracle.scaled_error(bad, r[0], r[2]) > 100.0
+
+
+@pytest.mark.parametrize("scheme,order", [(2, 1), (3, 2), (4, 4)])
+def test_explicit_rk_temporal_order(scheme, order):
+    """MFEM's ForwardEuler (1), RK2 (2) and RK4 (4): observed temporal order on the vortex."""
+    m = cases.vortex_mesh(n=4, order=3)
+    u0 = cases.vortex_state(m)
+    ph = oracle.phys()
+    T = 0.4
+    ref, _ = oracle.rk_run_ex(m, ph, u0, 4, 128, T / 128)
+    errs = []
+    base = {1: 32, 2: 16, 4: 8}[order]
+    for nst in (base, 2 * base):
+        uu, _ = oracle.rk_run_ex(m, ph, u0, scheme, nst, T / nst)
+        errs.append(np.abs(uu - ref).max())
+    rate = math.log2(errs[0] / errs[1])
+    assert rate > order - 0.35, (errs, rate)
+
+
+def test_explicit_rk_stage_structure():
+    """The RK2 and RK4 steps equal the Butcher-tableau steps written out with the oracle's RHS
+    (numpy): RK2 with a = 2/3 (b = 3/4) and the classic RK4 tableau."""
+    m = cases.vortex_mesh(n=3, order=2)
+    u0 = cases.vortex_state(m)
+    ph = oracle.phys()
+    dt = 0.02
+    f = lambda x: oracle.rhs_ex(m, ph, x)[0]
+    k1 = f(u0)
+    k2 = f(u0 + (2.0 / 3.0) * dt * k1)
+    rk2 = u0 + dt * (0.25 * k1 + 0.75 * k2)
+    out, _ = oracle.rk_run_ex(m, ph, u0, 3, 1, dt)
+    assert np.abs(out - rk2).max() < 1e-13 * np.abs(u0).max()
+    k1 = f(u0)
+    k2 = f(u0 + 0.5 * dt * k1)
+    k3 = f(u0 + 0.5 * dt * k2)
+    k4 = f(u0 + dt * k3)
+    rk4 = u0 + dt * (k1 + 2 * k2 + 2 * k3 + k4) / 6.0
+    out, _ = oracle.rk_run_ex(m, ph, u0, 4, 1, dt)
+    assert np.abs(out - rk4).max() < 1e-13 * np.abs(u0).max()
+    out, _ = oracle.rk_run_ex(m, ph, u0, 2, 1, dt)
+    assert np.abs(out - (u0 + dt * k1)).max() < 1e-14 * np.abs(u0).max()
