@@ -13,11 +13,18 @@
 //          <dir>/hooks.bin   bad after EntropyFilter (constraints from u0) and PositivityLimiter
 //          <dir>/blend.bin   Mult(u0) of an operator with subcell finite-volume blending
 //          <dir>/alpha.bin   its blending coefficients (one per element)
+//   general mesh (when <dir>/cavity.msh exists: the reference's lid-driven cavity mesh, read and
+//   built by include/prandtl_amd_mesh.hpp; NS, no-slip walls and a moving lid, blending):
+//   reads  <dir>/cav_u0.bin
+//   writes <dir>/cav_dudt.bin Mult(cav_u0)
+//          <dir>/cav_rk4.bin  cav_u0 after 3 RK4Solver steps of dt = 1e-3
+//          <dir>/cav_ng.bin, cav_mesh.bin  the mesh as built (per-node geometry; neighbours, codes)
 #include <cstdio>
 #include <string>
 #include <vector>
 
 #include "prandtl_amd.hpp"
+#include "prandtl_amd_mesh.hpp"
 
 static std::vector<double> load(const std::string &path, size_t n) {
     std::vector<double> v(n);
@@ -88,6 +95,37 @@ int main(int argc, char **argv) {
         bop.ComputeBlendingCoefficient(w, alpha, eta);
         save(dir + "/blend.bin", dudt.HostCopy());
         save(dir + "/alpha.bin", alpha.HostCopy());
+
+        // a general mesh from the reference's Gmsh file
+        if (FILE *probe = std::fopen((dir + "/cavity.msh").c_str(), "r")) {
+            std::fclose(probe);
+            prandtl::CellMesh cm = prandtl::ReadGmsh(dir + "/cavity.msh");
+            prandtl::Mesh cav = prandtl::BuildMesh(cm, 3, {{"left", 0}, {"right", 0}, {"bottom", 0}, {"lid", 1}});
+            cav.bc = {prandtl_bc{PRANDTL_BC_NOSLIP_ADIABATIC, {0.0, 0.0, 0.0}},
+                      prandtl_bc{PRANDTL_BC_NOSLIP_ADIABATIC, {1.0, 0.0, 0.0}}};
+            prandtl::Physics ns{1.4, 0.72, 0.02, 287.05, true, false};
+            prandtl::Options co;
+            co.blending = true;
+            prandtl::DGSEMOperator cop(cav, ns, co);
+            save(dir + "/cav_ng.bin", cav.node_geometry);   // the mesh as built, for the check
+            {
+                FILE *f = std::fopen((dir + "/cav_mesh.bin").c_str(), "wb");
+                std::fwrite(cav.neighbors.data(), sizeof(int), cav.neighbors.size(), f);
+                std::fwrite(cav.face_orientation.data(), sizeof(int), cav.face_orientation.size(), f);
+                std::fclose(f);
+            }
+            const size_t cn = cav.StateSize();
+            prandtl::Vector cu(cn), cd(cn);
+            cu.CopyFromHost(load(dir + "/cav_u0.bin", cn).data());
+            cop.Mult(cu, cd);
+            save(dir + "/cav_dudt.bin", cd.HostCopy());
+            prandtl::RK4Solver rk4;
+            rk4.Init(cop);
+            double ct = 0.0, ch = 1e-3;
+            for (int s = 0; s < 3; s++) rk4.Step(cu, ct, ch);
+            save(dir + "/cav_rk4.bin", cu.HostCopy());
+            std::printf("cavity: %d elements, t %.15e\n", cav.num_elements, ct);
+        }
 
         // errors surface as exceptions carrying the library's message
         try {

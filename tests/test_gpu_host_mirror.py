@@ -3,7 +3,9 @@
 LSRK45Solver Run, EntropyFilter and PositivityLimiter, and an operator with subcell
 finite-volume blending, each result against the oracle (RHS 64 eps x rounding scale
 entrywise; 10 SSP-RK3 and 4 LSRK4(5) steps and the hooks to 1e-12 relative; the blending
-coefficients to 1e-9 and the blended RHS, fed them, 64 eps x rounding scale)."""
+coefficients to 1e-9 and the blended RHS, fed them, 64 eps x rounding scale); and a general
+mesh read from the reference's Gmsh cavity by the mirror's mesh builder (NS, walls, blending,
+RK4Solver)."""
 import os
 import subprocess
 import tempfile
@@ -66,3 +68,50 @@ def test_host_mirror_program(device):
     assert (a > 0).any() and np.abs(a - a_ref).max() <= 1e-9
     ref, _, mag = oracle.rhs_blend(m, ph, u0, a)
     assert oracle.scaled_error(rd("blend.bin"), ref, mag) <= 64
+
+
+def test_host_mirror_general_mesh(device):
+    """The reference's Gmsh cavity read and built by the C++ mirror (include/prandtl_amd_mesh.hpp),
+    NS with walls and blending: Mult and 3 RK4Solver steps against the oracle on the same mesh
+    built by prandtl_amd/mesh.py."""
+    import shutil
+
+    from prandtl_amd import mesh as M
+
+    print("[gpu-case] C++ host mirror: Gmsh cavity general mesh", flush=True)
+    d = tempfile.mkdtemp()
+    exe = build_example(d)
+    m = cases.vortex_mesh(n=16, order=3)
+    cases.vortex_state(m).tofile(os.path.join(d, "u0.bin"))
+    cases.vortex_state(m).tofile(os.path.join(d, "bad.bin"))
+    src = os.path.join(ROOT, "tests", "golden", "meshes", "LidDrivenCavity.msh")
+    shutil.copy(src, os.path.join(d, "cavity.msh"))
+    gm = M.gmsh_mesh(src, 3, {"left": 0, "right": 0, "bottom": 0, "lid": 1})
+    X = gm.node_coords()
+    rho = 1.0 + 0.01 * np.sin(np.pi * X[0]) * np.cos(np.pi * X[1])
+    u0 = cases.prim_to_conserv(rho, [0.1 * np.sin(np.pi * X[1]), 0.05 * np.cos(np.pi * X[0])], rho * 71.43)
+    u0.tofile(os.path.join(d, "cav_u0.bin"))
+    r = subprocess.run([exe, d], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "cavity: 256 elements" in r.stdout
+    rd = lambda name: np.fromfile(os.path.join(d, name)).reshape(u0.shape)
+    # the oracle on the mesh exactly as the mirror built it (its metrics differ from the Python
+    # builder's in the last bits: J is summed in another order)
+    ng = np.fromfile(os.path.join(d, "cav_ng.bin")).reshape(gm.node_geom.shape)
+    mi = np.fromfile(os.path.join(d, "cav_mesh.bin"), dtype=np.int32).reshape(2, gm.num_elements, 4)
+    assert np.array_equal(mi[0], gm.neighbors) and np.array_equal(mi[1], gm.face_orient)
+    assert np.abs(ng - gm.node_geom).max() <= 1e-13 * np.abs(gm.node_geom).max()
+    cm = M.Mesh(2, 3, None, None, None, mi[0].copy(), np.ascontiguousarray(ng[:, 0, :]), ng, mi[1].copy(),
+                gm.coords, gm.hmin)
+    ph = oracle.phys(viscous=True, mu=0.02, Pr=0.72, R_gas=287.05)
+    walls = [oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 0.0, 0.0, 0.0), oracle.bc(oracle.BC_NOSLIP_ADIABATIC, 1.0, 0.0, 0.0)]
+    a_ref, _ = oracle.blend_coef(cm, u0)
+    ref, _, mag = oracle.rhs_ex(cm, ph, u0, walls, alpha=a_ref)
+    s = oracle.scaled_error(rd("cav_dudt.bin"), ref, mag)
+    print(f"[mirror] cavity Mult: {s:.2f} eps*mag", flush=True)
+    assert s <= 64
+    gm = cm
+    rr, _ = oracle.rk_run_ex(gm, ph, u0, 4, 3, 1e-3, walls, blend=True)
+    err = np.abs(rd("cav_rk4.bin") - rr).max() / np.abs(rr).max()
+    print(f"[mirror] cavity RK4: {err:.3e}", flush=True)
+    assert err <= 1e-11
