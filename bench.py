@@ -378,7 +378,8 @@ def cavity_case(args, dev):
     """Lid-driven cavity (the reference's TestCases/NavierStokes/2D/LidDrivenCavity): [0,2]^2,
     16 x 16 quads p=3, rho = 1 at rest, p = 1/(gamma Ma^2) with Ma = 0.1, mu = 0.02 (Re = 100 on
     the lid speed 1 and the side 2), Pr = 0.72, R = 287.05, no-slip adiabatic walls, the lid
-    moving at (1, 0), blending on, SSP-RK3 at dt = 1e-4 to t = 3 (30,000 steps) -- and the same
+    moving at (1, 0), blending on, SSP-RK3 at dt = 1e-4 to t = 3 (30,000 steps), then to the
+    steady state (t = 40, dt = 2e-4) against Ghia, Ghia & Shin (1982) at Re = 100 -- and the same
     physics on 256 x 256 quads (1,048,576 nodes) for throughput, against the periodic mesh of
     that size."""
     import numpy as np
@@ -433,8 +434,23 @@ def cavity_case(args, dev):
         "centreline_u_min": float(us.min()), "centreline_y_at_min": float(ys[np.argmin(us)]),
         "finite": bool(np.isfinite(u).all()),
     }
+    # steady state (t = 40 at dt = 2e-4) against Ghia, Ghia & Shin (1982), Re = 100
+    t_s, dt_s = 0.0, 2e-4
+    uds = torch.from_numpy(u0).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(8):
+        t_s = op.rk_run(capi.SSPRK3, uds, t_s, dt_s, 25000)
+    torch.cuda.synchronize()
+    wall_s = time.perf_counter() - t0
+    rho_s, vel_s, _ = cases.conserv_to_prim(uds.cpu().numpy(), gamma=gamma)
+    uc = cases.lattice_line_values(m, vel_s[0], 1.0, 2.0 * cases.GHIA_RE100_Y)
+    out["ghia_re100"] = {"t": t_s, "steps": 200000, "wall_s": wall_s,
+                         "max_abs_error": float(np.abs(uc - cases.GHIA_RE100_U).max()),
+                         "u_min": float(uc.min()), "u_min_ghia": float(cases.GHIA_RE100_U.min())}
     op.close()
-    log(f"cavity 16x16: {nsteps} steps in {wall:.2f} s, centreline u_min {us.min():.4f} at y={ys[np.argmin(us)]:.3f}")
+    log(f"cavity 16x16: {nsteps} steps in {wall:.2f} s, centreline u_min {us.min():.4f} at y={ys[np.argmin(us)]:.3f}; "
+        f"steady t={t_s:.0f}: max |u - Ghia| {out['ghia_re100']['max_abs_error']:.4f}")
     # (b) throughput at 256 x 256, boundaries against periodic
     for periodic in (False, True):
         m = make(256, periodic)

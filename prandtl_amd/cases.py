@@ -187,3 +187,37 @@ def kinetic_energy(mesh: Mesh, u) -> float:
     rho = u[0]
     ke = 0.5 * sum(u[1 + d] ** 2 for d in range(mesh.dim)) / rho
     return float((W * ke).sum() / W.sum())
+
+
+# Ghia, Ghia & Shin, J. Comput. Phys. 48 (1982) 387-411, Table I: the horizontal velocity u/U on
+# the vertical centreline of the lid-driven cavity at Re = 100 (y/L from the bottom wall).
+GHIA_RE100_Y = np.array([1.0000, 0.9766, 0.9688, 0.9609, 0.9531, 0.8516, 0.7344, 0.6172, 0.5000,
+                         0.4531, 0.2813, 0.1719, 0.1016, 0.0703, 0.0625, 0.0547, 0.0000])
+GHIA_RE100_U = np.array([1.00000, 0.84123, 0.78871, 0.73722, 0.68717, 0.23151, 0.00332, -0.13641, -0.20581,
+                         -0.21090, -0.15662, -0.10150, -0.06434, -0.04775, -0.04192, -0.03717, 0.00000])
+
+
+def lattice_line_values(mesh: Mesh, field: np.ndarray, x0: float, ys: np.ndarray) -> np.ndarray:
+    """A nodal field (NE, n) of a 2D Cartesian lattice evaluated on the vertical line x = x0 at
+    heights ys: x0 must lie on element faces (or nodes) of the lattice; along y the element's GLL
+    interpolant is evaluated (Lagrange basis of its Np nodes)."""
+    from .mesh import gll_nodes
+
+    assert mesh.dim == 2 and mesh.n is not None
+    np_ = mesh.np1
+    xg, _ = gll_nodes(np_)
+    hx = mesh.A[0, 0]
+    hy = mesh.A[1, 1]
+    ex = int(np.clip(np.floor((x0 - mesh.origin[0]) / hx + 1e-9), 0, mesh.n[0] - 1))
+    xi = (x0 - mesh.origin[0]) / hx - ex
+    i = int(np.argmin(np.abs(xg - xi)))
+    assert abs(xg[i] - xi) < 1e-9, "x0 must be a node line of the lattice"
+    out = np.zeros(len(ys))
+    for k, y in enumerate(ys):
+        ey = int(np.clip(np.floor((y - mesh.origin[1]) / hy), 0, mesh.n[1] - 1))
+        eta = (y - mesh.origin[1]) / hy - ey
+        e = ex + mesh.n[0] * ey
+        L = np.array([np.prod([(eta - xg[m]) / (xg[j] - xg[m]) for m in range(np_) if m != j]) for j in range(np_)])
+        vals = field[e].reshape(np_, np_)[:, i]          # node (i, j): index i + Np j
+        out[k] = float(L @ vals)
+    return out

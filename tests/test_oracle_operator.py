@@ -225,3 +225,15 @@ def test_explicit_rk_stage_structure():
     assert np.abs(out - rk4).max() < 1e-13 * np.abs(u0).max()
     out, _ = oracle.rk_run_ex(m, ph, u0, 2, 1, dt)
     assert np.abs(out - (u0 + dt * k1)).max() < 1e-14 * np.abs(u0).max()
+
+
+def test_lattice_line_values_interpolates():
+    """cases.lattice_line_values (the Ghia comparison's sampler) reproduces a cubic field exactly
+    on a node line of a p=3 lattice."""
+    m = M.periodic_lattice(2, 3, (4, 5), lengths=[2.0, 2.5], periodic=(False, False),
+                           boundary={(0, 0): 0, (0, 1): 0, (1, 0): 0, (1, 1): 0})
+    X = m.node_coords()
+    f = lambda x, y: 1.0 + 0.3 * y - 0.2 * y ** 2 + 0.05 * y ** 3 + 0.1 * x
+    ys = np.linspace(0.0, 2.5, 23)
+    got = cases.lattice_line_values(m, f(X[0], X[1]), 1.0, ys)
+    assert np.abs(got - f(1.0, ys)).max() < 1e-13
