@@ -191,22 +191,29 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long *p, double v) 
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-// Stage epilogue: writes the RHS (Mult) or applies the fused RK stage update to one value.
+// Stage epilogue: writes the RHS (Mult) or applies the fused RK stage update to one value;
+// x is the RK register (LSRK) or the base state of the Shu-Osher combination (SSP).
 template <int NP>
-__device__ __forceinline__ void stage_update(const RhsArgs<NP> &A, size_t gi, double u_in, double du, int &bad) {
+__device__ __forceinline__ void stage_update_x(const RhsArgs<NP> &A, size_t gi, double u_in, double du, double x, int &bad) {
     if (A.mode == EPI_MULT) {
         A.out[gi] = du;
     } else if (A.mode == EPI_LSRK) {
-        const double kk = A.ca * A.k[gi] + A.dt * du;
+        const double kk = A.ca * x + A.dt * du;
         A.k[gi] = kk;
         const double un = u_in + A.cb * kk;
         A.out[gi] = un;
         bad |= !(un == un);
     } else {
-        const double un = A.ca * A.x[gi] + A.cb * (u_in + A.dt * du);
+        const double un = A.ca * x + A.cb * (u_in + A.dt * du);
         A.out[gi] = un;
         bad |= !(un == un);
     }
+}
+
+template <int NP>
+__device__ __forceinline__ void stage_update(const RhsArgs<NP> &A, size_t gi, double u_in, double du, int &bad) {
+    const double x = A.mode == EPI_LSRK ? A.k[gi] : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
+    stage_update_x<NP>(A, gi, u_in, du, x, bad);
 }
 
 template <int NP>
@@ -647,6 +654,32 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
         for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
     }
+    // Affine elements without blending: the epilogue's operands (the RK register or base state,
+    // the viscous volume term) are loaded here, so their HBM latency overlaps the volume loop
+    // instead of stalling the epilogue; the face terms are folded into acc (FOLD) to keep the
+    // register budget.  Blending and general meshes keep their own registers for their work.
+    constexpr bool PRE = !BLEND && !GEN;
+    double pre_x[NEQ];
+    R pre_dv[NEQ - 1];
+    if constexpr (PRE) {
+        if (active) {
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                const size_t gi = v * vs + (size_t)e * NN + node;
+                pre_x[v] = A.mode == EPI_LSRK ? A.k[gi] : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
+            }
+            if constexpr (VISC) {
+#pragma unroll
+                for (int v = 0; v < NEQ - 1; v++) {
+                    if constexpr (MIX)
+                        pre_dv[v] = A.dvf[v * vs + (size_t)e * NN + node];
+                    else
+                        pre_dv[v] = A.dv[v * vs + (size_t)e * NN + node];
+                }
+            }
+        }
+    }
+    constexpr bool FOLD = PRE;
     __syncthreads();
     int bad = 0;
     int ai[3];
@@ -750,9 +783,16 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             if (a == 0 || a == NP - 1) {
                 const int side = (a == NP - 1) ? 1 : 0;
                 const int it = eb * FITEMS + (dir * 2 + side) * L + line_index<NP>(node, dir);
-                const R sg = side ? R(-1) : R(1);
+                if constexpr (FOLD) {
+                    // du = -(1/J)(acc - accs/w0): the face term enters acc as -/+ f/w0
+                    const R sg = (R)(side ? A.T.inv_w0 : -A.T.inv_w0);
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
+                    for (int v = 0; v < NEQ; v++) acc[v] += sg * sF[it][v];
+                } else {
+                    const R sg = side ? R(-1) : R(1);
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
+                }
             }
         }
     }
@@ -844,6 +884,8 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             R du;
             if constexpr (BLEND) {
                 du = acc[v];
+            } else if constexpr (FOLD) {
+                du = -acc[v] * rinvJ;
             } else {
                 du = -acc[v] * rinvJ + accs[v] * sw;
             }
@@ -857,14 +899,20 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             }
             if constexpr (VISC) {
                 if (v > 0) {
-                    if constexpr (MIX)
+                    if constexpr (PRE)
+                        dud += pre_dv[v - 1];
+                    else if constexpr (MIX)
                         dud += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
                     else
                         dud += A.dv[(v - 1) * vs + (size_t)e * NN + node];
                 }
             }
             // the RK accumulation is FP64 in both modes
-            stage_update<NP>(A, v * vs + (size_t)e * NN + node, own_state(v, eb, node), dud, bad);
+            const size_t gi = v * vs + (size_t)e * NN + node;
+            if constexpr (PRE)
+                stage_update_x<NP>(A, gi, own_state(v, eb, node), dud, pre_x[v], bad);
+            else
+                stage_update<NP>(A, gi, own_state(v, eb, node), dud, bad);
         }
     }
     finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
