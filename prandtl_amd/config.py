@@ -4,7 +4,8 @@ ref: Simulation::LoadConfig and the set-up that follows it (src/Simulation/Simul
 the compile-time switches (PARABOLIC, SUBCELL_FV_BLENDING, SUTHERLAND, USE_FILTER), the physics
 constants, the mesh file with ``ser_ref_levels`` uniform refinements, the boundary conditions by
 boundary name (``conditions.boundary_conditions``, :490-800), the surface flux (:212-220), the
-ODE solver (:177-206), the time-step settings and the checkpoint settings (:111-116, 350-390).
+ODE solver (:177-206), the time-step settings, the checkpoint settings (:111-116, 350-390) and the
+visualization settings (:110-135; ``Case.paraview_collection`` and ``Case.save_schedule``).
 
 ``load_case`` returns a ``Case``: the mesh, the operator's arguments, the boundary-condition table
 (index k <-> boundary name), the stepping settings.  Boundary values named by the configuration
@@ -76,6 +77,7 @@ class Case:
     final_time: float
     initial_condition: dict
     checkpoint: dict = field(default_factory=dict)
+    visualization: dict = field(default_factory=dict)
     ignored: dict = field(default_factory=dict)
 
     def bcs(self):
@@ -92,6 +94,25 @@ class Case:
         kw.update(self.options)
         kw.update(overrides)
         return capi.Operator(self.mesh, bcs=self.bcs() or None, **kw)
+
+    def paraview_collection(self, prefix_path: Optional[str] = None, **kw):
+        """The ParaView collection the reference's driver writes for this case
+        (Simulation.cpp:829-853): "ParaView_folder" (default "ParaView") under "output_file_path"
+        (or ``prefix_path``), high-order cells with levels of detail = the order."""
+        from .paraview import ParaViewDataCollection
+
+        vis = self.visualization
+        prefix = prefix_path if prefix_path is not None else vis.get("output_file_path", "")
+        return ParaViewDataCollection(vis.get("ParaView_folder", "ParaView"), self.mesh, prefix,
+                                      levels_of_detail=self.order, **kw)
+
+    def save_schedule(self, t0: float = 0.0):
+        """The reference's output times (initial_save_dt, refined_save_dt, refined_trigger)."""
+        from .paraview import SaveSchedule
+
+        vis = self.visualization
+        dt1 = float(vis.get("initial_save_dt", 0.01))
+        return SaveSchedule(t0, dt1, float(vis.get("refined_save_dt", dt1)), float(vis.get("refined_trigger", 2.0)))
 
 
 def _resolve_mesh_file(runtime, config_dir, mesh_dir):
@@ -194,11 +215,16 @@ def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: 
     ic = rt.get("conditions", {}).get("initial_conditions", {}) or {}
     ckpt = {k: rt[k] for k in ("checkpoint_save", "checkpoint_load", "checkpoint_dt", "checkpoints_folder",
                                "checkpoint_cycle", "output_file_path") if k in rt}
+    # Simulation.cpp:110-135: the visualization switches and the output schedule
+    vis = {k: rt[k] for k in ("visualize", "paraview", "visit", "vis_steps", "initial_save_dt", "refined_save_dt",
+                              "refined_trigger", "ParaView_folder", "output_file_path") if k in rt}
+    if vis.get("visualize") and vis.get("visit") and not vis.get("paraview"):
+        ignored["visit"] = "VisIt output is not written (ParaView output is)"
     return Case(path=path, dim=dim, order=p, mesh=mesh, physics=physics, options=options, bc_names=bc_names,
                 bc_specs=bc_specs, ode_solver=solver, scheme=ODE_SOLVERS.get(solver),
                 cfl=float(rt.get("cfl", 1.0)), dt=float(rt["dt"]) if "dt" in rt else None,
                 variable_dt=bool(rt.get("variable_dt", False)), final_time=float(rt.get("final_time", 0.0)),
-                initial_condition=dict(ic), checkpoint=ckpt, ignored=ignored)
+                initial_condition=dict(ic), checkpoint=ckpt, visualization=vis, ignored=ignored)
 
 
 def reference_initial_state(case: Case):
