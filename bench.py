@@ -595,8 +595,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1). "
-              "The sharding design is described in DESIGN.md.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (run with --gpus 1); "
+              "the element partitioning it would use is described with its design notes.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -30,7 +30,7 @@ typedef void *prandtl_stream; /* a hipStream_t */
 enum {
     PRANDTL_OK = 0,
     PRANDTL_ERR_ARG = -1,         /* invalid argument */
-    PRANDTL_ERR_UNSUPPORTED = -2, /* valid request outside what this build implements */
+    PRANDTL_ERR_UNSUPPORTED = -2, /* valid request outside what this library implements */
     PRANDTL_ERR_HIP = -3,         /* HIP runtime error */
     PRANDTL_ERR_MESH = -4,        /* inconsistent mesh description */
     PRANDTL_ERR_NAN = -5          /* non-finite state detected */
