@@ -18,8 +18,17 @@
 // -ln 1e-10] and raises zeta to the bracket's admissible end; the scan resumes after it.  A
 // workgroup (one element, one thread per node) evaluates all remaining nodes at once and
 // picks the first inadmissible one with an LDS atomicMin, which visits exactly the nodes the
-// sequential scan visits, in its order, at its zeta.  Root-finding and the constraint tests
-// are FP64.
+// sequential scan visits, in its order, at its zeta.  The bisection itself runs in rounds of up
+// to log2(block size) steps: in a round every lane evaluates one node of the round's bisection
+// tree (heap order: node 1 the bracket's midpoint, 2l and 2l+1 the midpoints of node l's left and
+// right halves), each computed by the halvings the sequential bisection would perform to reach
+// it, and the path through the tree (right where the midpoint is inadmissible) ends on the
+// sequential bisection's bracket, bit for bit.  Root-finding and the constraint tests are FP64.
+//
+// The detection scans append the elements that need a hook to a list (one counter per hook;
+// the scan that fills one counter clears the other, which the previous application of that
+// hook has finished reading), and the filter and limiter walk the list with a fixed grid, so
+// elements that need nothing cost no workgroup.
 //
 // Positivity limiter -- the reference declares only the Limiter interface
 // (src/Limiters/Limiter.hpp:10-16).  This is the scaling limiter of Zhang & Shu (J. Comput.
@@ -57,6 +66,9 @@ struct HookArgs {
     double *bound;                 // per-element entropy bound
     int *flag;                     // per-element: the element needs the hook
     unsigned long long *count;     // elements changed: [0] filter, [1] limiter
+    int *list;                     // the elements that need the hook (k_hook_scan MODE 1, 2)
+    unsigned int *nlist;           // their number (appended by the scan, read by the hook)
+    unsigned int *nlist_next;      // the other counter of this hook: cleared by the scan
     const double *ngeom;           // general mesh: per-node (det J, adj J) -- the limiter's mean is
                                    // sum J w u / sum J w there; null on affine elements (J cancels)
 };
@@ -95,6 +107,7 @@ __global__ void __launch_bounds__(256) k_hook_scan(const HookArgs<NP> A) {
     using C = HookCfg<DIM, NP>;
     constexpr int NN = C::NN, EPB = C::EPB;
     __shared__ double sm[EPB * NN];
+    __shared__ unsigned s_cnt, s_base;   // the block's appends to the list (MODE 1, 2)
     const int tid = threadIdx.x, eb = tid / NN, node = tid - eb * NN;
     const int e = blockIdx.x * EPB + eb;
     const size_t vs = (size_t)A.NE * NN;
@@ -111,18 +124,31 @@ __global__ void __launch_bounds__(256) k_hook_scan(const HookArgs<NP> A) {
         else
             sm[tid] = (s[0] < kLimiterEps || p < kLimiterEps) ? 1.0 : 0.0;
     }
+    if (MODE != 0 && tid == 0) {
+        s_cnt = 0u;
+        if (blockIdx.x == 0) *A.nlist_next = 0u;
+    }
     __syncthreads();
-    if (active && node == 0) {
-        if (MODE == 0) {
+    if (MODE == 0) {
+        if (active && node == 0) {
             double m = 1.7976931348623157e308;   // el_min_e = DBL_MAX (EntropyFilter.cpp:120)
             for (int i = 0; i < NN; i++) m = fmin(m, sm[eb * NN + i]);
             A.emin[e] = m;
-        } else {
-            double f = 0.0;
-            for (int i = 0; i < NN; i++) f = fmax(f, sm[eb * NN + i]);
-            A.flag[e] = f > 0.0 ? 1 : 0;
         }
+        return;
     }
+    // the block's flagged elements take consecutive list slots: one global atomic per block
+    int slot = -1;
+    if (active && node == 0) {
+        double f = 0.0;
+        for (int i = 0; i < NN; i++) f = fmax(f, sm[eb * NN + i]);
+        A.flag[e] = f > 0.0 ? 1 : 0;
+        if (f > 0.0) slot = (int)atomicAdd(&s_cnt, 1u);
+    }
+    __syncthreads();
+    if (tid == 0 && s_cnt > 0u) s_base = atomicAdd(A.nlist, s_cnt);
+    __syncthreads();
+    if (slot >= 0) A.list[s_base + (unsigned)slot] = e;
 }
 
 // entropy bound: minimum over the element and its 2 dim face neighbours
@@ -144,19 +170,23 @@ __device__ __forceinline__ double filter_sigma(double zeta, int j) {
     return j == 0 ? 1.0 : exp(-zeta * (double)(j * j));
 }
 
-// FilterSolution: one element per workgroup, one thread per node
+// FilterSolution: one element per workgroup at a time, one thread per node; the workgroups
+// walk the list of detected elements
 template <int DIM, int NP>
 __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
     constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
     static_assert(NN <= 128, "one element per workgroup of at most 128 threads");
     __shared__ double sU[NEQ][NN], sT[NEQ][NN];
+    __shared__ double sH[NP][NEQ];      // the expansion of the node being root-found
+    __shared__ int sViol[128];          // a round's verdicts, heap-indexed
     __shared__ int s_first;
-    __shared__ double s_zeta;
-    const int e = blockIdx.x;
-    if (!A.flag[e]) return;   // detection (EntropyFilter.cpp:183-191) by k_hook_scan<.., 1>
     const int i = threadIdx.x;
+    const int tdepth = 31 - __clz((int)blockDim.x);   // steps per round: 2^tdepth - 1 lanes evaluate
     const bool node_ok = i < NN;
     const size_t vs = (size_t)A.NE * NN;
+    const unsigned nlist = *A.nlist;
+    for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int e = A.list[li];   // detected (EntropyFilter.cpp:183-191) by k_hook_scan<.., 1>
     const double eb = A.bound[e];
     if (node_ok) {
 #pragma unroll
@@ -241,28 +271,55 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
         const int first = s_first;
         if (first >= NN) break;
         if (i == first) {
-            double z1 = zeta, z2 = zmax;
-            for (int it = 0; it < kFilterNIter; it++) {
-                const double z3 = 0.5 * (z1 + z2);
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) sH[j][v] = H[j][v];
+        }
+        __syncthreads();
+        // the 20-step bisection on [zeta, zmax] for node `first`, in rounds of tree evaluations
+        double z1 = zeta, z2 = zmax;
+        for (int done = 0; done < kFilterNIter;) {
+            const int d = min(tdepth, kFilterNIter - done);
+            if (i >= 1 && i < (1 << d)) {
+                // heap node i: follow its path from the root, halving as the bisection does
+                double a = z1, b = z2;
+                const int depth = 31 - __clz(i);
+                for (int k = depth - 1; k >= 0; k--) {
+                    const double m = 0.5 * (a + b);
+                    if ((i >> k) & 1)
+                        a = m;
+                    else
+                        b = m;
+                }
+                const double z3 = 0.5 * (a + b);
                 double st[NEQ];
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) {
-                    double a = H[0][v];
+                    double acc = sH[0][v];
 #pragma unroll
-                    for (int j = 1; j < NP; j++) a += H[j][v] * filter_sigma<NP>(z3, j);
-                    st[v] = a;
+                    for (int j = 1; j < NP; j++) acc += sH[j][v] * filter_sigma<NP>(z3, j);
+                    st[v] = acc;
                 }
-                if (filter_violates<DIM>(st, A.gamma, A.gammaM1, eb))
-                    z1 = z3;
-                else
-                    z2 = z3;
+                sViol[i] = filter_violates<DIM>(st, A.gamma, A.gammaM1, eb) ? 1 : 0;
             }
-            s_zeta = z2;
+            __syncthreads();
+            int nd = 1;
+            for (int k = 0; k < d; k++) {
+                const double z3 = 0.5 * (z1 + z2);
+                if (sViol[nd]) {
+                    z1 = z3;
+                    nd = 2 * nd + 1;
+                } else {
+                    z2 = z3;
+                    nd = 2 * nd;
+                }
+            }
+            done += d;
+            __syncthreads();   // sViol is rewritten by the next round
         }
-        __syncthreads();
-        zeta = s_zeta;
+        zeta = z2;
         start = first + 1;
-        __syncthreads();   // s_first is reset at the top of the next pass
     }
     // reconstruction (EntropyFilter.cpp:214-238)
     if (node_ok) {
@@ -275,9 +332,12 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
         }
     }
     if (i == 0) atomicAdd(A.count, 1ull);
+    __syncthreads();   // sU is rewritten for the next element
+    }
 }
 
-// Zhang-Shu limiter: one element per workgroup, one thread per node
+// Zhang-Shu limiter: one element per workgroup at a time, one thread per node; the workgroups
+// walk the list of elements with a node below the bounds (admissible elements are untouched)
 template <int DIM, int NP>
 __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
     constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
@@ -285,14 +345,15 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
     __shared__ double sR[NEQ][NN];
     __shared__ double sMean[NEQ];
     __shared__ double s_rmin, s_th2;
-    const int e = blockIdx.x;
-    if (!A.flag[e]) return;   // admissible element (k_hook_scan<.., 2>): untouched
+    __shared__ double sW[NN];
     const int i = threadIdx.x;
     const bool node_ok = i < NN;
     const size_t vs = (size_t)A.NE * NN;
     const double eps = kLimiterEps;
+    const unsigned nlist = *A.nlist;
+    for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int e = A.list[li];
     double s[NEQ];
-    __shared__ double sW[NN];
     if (node_ok) {
         double w = A.T.w[i % NP] * A.T.w[(i / NP) % NP] * (DIM == 3 ? A.T.w[i / (NP * NP)] : 1.0);
         if (A.ngeom) w *= A.ngeom[((size_t)e * NN + i) * (1 + DIM * DIM)];
@@ -357,6 +418,8 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
         for (int v = 0; v < NEQ; v++) A.u[v * vs + (size_t)e * NN + i] = s[v];
     }
     if (i == 0) atomicAdd(A.count + 1, 1ull);
+    __syncthreads();   // sR, sMean, s_rmin, s_th2 are rewritten for the next element
+    }
 }
 
 }  // namespace prandtl

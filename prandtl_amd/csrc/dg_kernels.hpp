@@ -112,7 +112,7 @@ struct RhsArgs {
     double *k;             // EPI_LSRK: 2N register
     const double *x;       // EPI_SSP: base state of the Shu-Osher combination
     double ca, cb, dt;
-    unsigned long long *lam_bits;
+    double *lam_part;      // want_lambda: each wave's max |V| + c, at its global wave index
     int *nan_flag;
     const BcDev *bc;       // boundary conditions (neighbour -(k+1) -> bc[k]; k_rhs_node<..., BND = true>)
     double *alpha;         // k_rhs_node<..., BLEND = true>: the coefficients it computed (may be null)
@@ -219,8 +219,10 @@ __device__ __forceinline__ void stage_update(const RhsArgs<NP> &A, size_t gi, do
 template <int NP>
 __device__ __forceinline__ void finish_block(const RhsArgs<NP> &A, double lam, int bad) {
     if (A.want_lambda) {
+        // one plain store per wave (a single reduction kernel follows): atomics on one address
+        // from every wave of the grid serialise
         for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
-        if ((threadIdx.x & 63) == 0 && lam > 0.0) atomic_max_pos(A.lam_bits, lam);
+        if ((threadIdx.x & 63) == 0) A.lam_part[((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6] = lam;
     }
     if (A.nan_flag && __any(bad)) {
         if ((threadIdx.x & 63) == 0) atomicOr(A.nan_flag, 1);

@@ -64,9 +64,10 @@ __global__ void k_copy_f64(double *__restrict__ dst, const double *__restrict__ 
     for (; i < n; i += st) dst[i] = src[i];
 }
 
-// max over nodes of |V| + c (the reference's max_char_speed, ChandrashekarFlux.cpp:61-65)
+// max over nodes of |V| + c (the reference's max_char_speed, ChandrashekarFlux.cpp:61-65): each
+// wave stores its maximum at its global wave index; k_reduce_max combines them
 template <int DIM>
-__global__ void k_max_speed(const double *u, size_t nnodes, double gamma, unsigned long long *lam_bits) {
+__global__ void k_max_speed(const double *u, size_t nnodes, double gamma, double *lam_part) {
     constexpr int NEQ = DIM + 2;
     double lam = 0.0;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -81,7 +82,23 @@ __global__ void k_max_speed(const double *u, size_t nnodes, double gamma, unsign
         lam = fmax(lam, make_side<DIM>(s, P).lam);
     }
     for (int o = 32; o > 0; o >>= 1) lam = fmax(lam, __shfl_xor(lam, o));
-    if ((threadIdx.x & 63) == 0 && lam > 0.0) atomic_max_pos(lam_bits, lam);
+    if ((threadIdx.x & 63) == 0) lam_part[((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6] = lam;
+}
+
+// *out = the bits of max(0, part[0..n-1]) (positive doubles order like their bit patterns); one
+// workgroup
+__global__ void __launch_bounds__(1024) k_reduce_max(const double *part, int n, unsigned long long *out) {
+    __shared__ double s[16];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, part[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = fmax(r, s[w]);
+        *out = (unsigned long long)__double_as_longlong(r);
+    }
 }
 
 }  // namespace prandtl

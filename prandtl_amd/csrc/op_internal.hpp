@@ -66,6 +66,9 @@ struct prandtl_op_s {
     double *d_fbound = nullptr;   // stage hooks: per-element entropy bound of the filter
     int *d_hflag = nullptr;       // stage hooks: per-element trigger flags
     unsigned long long *d_hcount = nullptr;   // stage hooks: elements filtered, limited
+    int *d_hlist = nullptr;       // stage hooks: the elements a hook applies to
+    unsigned int *d_hlistn = nullptr;         // their counters: two per hook (filter, limiter)
+    int hook_par[2] = {0, 0};     // which of its two counters a hook's next scan fills
     bool have_bounds = false;     // filter constraints computed
     double *d_vq = nullptr;       // adjoint: per-node quantities, (dim+6) x NE x n
     double *d_vgf = nullptr;      // NS adjoint: lifting adjoint at face points, dim x 2 x neq x NE x Np^(dim-1)
@@ -73,6 +76,8 @@ struct prandtl_op_s {
     double *d_vsb = nullptr;      // NS adjoint: partial adjoint of the state
     double *d_vlam = nullptr;     // adjoint: stage cotangent lambda
     unsigned long long *d_lam = nullptr;
+    double *d_lam_part = nullptr; // per-wave maxima of |V| + c (2 NE + 64 entries, at least 8192)
+    int lam_waves = 0;            // waves of the last flux kernel launched (their maxima in d_lam_part)
     int *d_nan = nullptr;
     bool lam_valid = false;
     // taped forward run for the discrete adjoint: the state at the start of every segment
@@ -183,6 +188,7 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
         HIP_TRY(hipGetLastError());
       }
     }
+    op->lam_waves = 0;
     if (!(S.phases & 2)) return PRANDTL_OK;
     RhsArgs<NP> ra;
     ra.T = make_tables<NP>(op);
@@ -204,7 +210,7 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.ca = S.ca;
     ra.cb = S.cb;
     ra.dt = S.dt;
-    ra.lam_bits = op->d_lam;
+    ra.lam_part = op->d_lam_part;
     ra.nan_flag = S.nan_check ? op->d_nan : nullptr;
     ra.alpha = op->d_alpha;
     ra.bc = op->d_bc;
@@ -216,6 +222,7 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
             if (op->opts.blending) ra.blend = make_blend_params<NP>(op);
             using LC = NodeCfg<DIM, NP>;
             const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+            op->lam_waves = nblk * LC::BLOCK / 64;
             // boundary faces only where the mesh has them (the boundary path costs registers)
             if (op->opts.blending && op->nbc)
                 hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
@@ -235,6 +242,7 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
             if (op->opts.blending) ra.blend = make_blend_params<NP>(op);
             using LC = NodeCfg<DIM, NP>;
             const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+            op->lam_waves = nblk * LC::BLOCK / 64;
             if (op->opts.blending && op->nbc)
                 hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
             else if (op->opts.blending)
@@ -251,10 +259,12 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     if constexpr (VISC || MIX) {
         using LC = NodeCfg<DIM, NP>;
         const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+        op->lam_waves = nblk * LC::BLOCK / 64;
         hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, MIX>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
     } else {
         using LC = LineCfg<DIM, NP>;
         const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+        op->lam_waves = nblk * LC::BLOCK / 64;
         hipLaunchKernelGGL((k_rhs<DIM, NP, VISC, SURF>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
     }
     HIP_TRY(hipGetLastError());

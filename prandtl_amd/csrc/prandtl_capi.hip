@@ -115,11 +115,13 @@ int dispatch_np(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
 }
 
 int run_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
-    if (S.want_lambda) {
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, op->d_lam, 0ull, 1);
+    const int rc = op->dim == 2 ? dispatch_np<2>(op, S, st) : dispatch_np<3>(op, S, st);
+    if (rc == PRANDTL_OK && S.want_lambda) {
+        // the flux kernel left one maximum per wave
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, op->d_lam_part, op->lam_waves, op->d_lam);
         HIP_TRY(hipGetLastError());
     }
-    return op->dim == 2 ? dispatch_np<2>(op, S, st) : dispatch_np<3>(op, S, st);
+    return rc;
 }
 
 int ensure_rk_buffers(prandtl_op_s *op, int scheme) {
@@ -297,7 +299,10 @@ int ensure_hook_buffers(prandtl_op_s *op) {
         HIP_TRY(hipMalloc(&op->d_fbound, (size_t)op->NE * sizeof(double)));
         HIP_TRY(hipMalloc(&op->d_hflag, (size_t)op->NE * sizeof(int)));
         HIP_TRY(hipMalloc(&op->d_hcount, 2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&op->d_hlist, (size_t)op->NE * sizeof(int)));
+        HIP_TRY(hipMalloc(&op->d_hlistn, 4 * sizeof(unsigned int)));
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 2);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, 0, reinterpret_cast<int *>(op->d_hlistn), 0, 4);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipDeviceSynchronize());
     }
@@ -320,18 +325,31 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
     a.flag = op->d_hflag;
     a.count = op->d_hcount;
     a.ngeom = op->d_ngeom;
+    a.list = nullptr;
+    a.nlist = a.nlist_next = nullptr;
     using C = HookCfg<DIM, NP>;
     const int sblk = (op->NE + C::EPB - 1) / C::EPB;
-    constexpr int ebl = C::NN <= 64 ? 64 : 128;   // one element per workgroup
+    constexpr int ebl = C::NN <= 64 ? 64 : 128;   // one element per workgroup at a time
+    // the filter and limiter walk the scan's list of elements with a fixed grid
+    const int lblk = std::min(op->NE, 2048);
+    if (what != HOOK_CONSTRAINTS) {
+        // the scan appends to one of the hook's two counters and clears the other, which the
+        // previous application of the hook (earlier on this stream) has finished reading
+        const int h = what == HOOK_FILTER ? 0 : 1, p = op->hook_par[h];
+        a.list = op->d_hlist;
+        a.nlist = op->d_hlistn + 2 * h + p;
+        a.nlist_next = op->d_hlistn + 2 * h + (1 - p);
+        op->hook_par[h] = 1 - p;
+    }
     if (what == HOOK_CONSTRAINTS) {
         hipLaunchKernelGGL((k_hook_scan<DIM, NP, 0>), dim3(sblk), dim3(256), 0, st, a);
         hipLaunchKernelGGL((k_filter_bound<DIM, NP>), dim3((op->NE + 255) / 256), dim3(256), 0, st, a);
     } else if (what == HOOK_FILTER) {
         hipLaunchKernelGGL((k_hook_scan<DIM, NP, 1>), dim3(sblk), dim3(256), 0, st, a);
-        hipLaunchKernelGGL((k_filter<DIM, NP>), dim3(op->NE), dim3(ebl), 0, st, a);
+        hipLaunchKernelGGL((k_filter<DIM, NP>), dim3(lblk), dim3(ebl), 0, st, a);
     } else {
         hipLaunchKernelGGL((k_hook_scan<DIM, NP, 2>), dim3(sblk), dim3(256), 0, st, a);
-        hipLaunchKernelGGL((k_limiter<DIM, NP>), dim3(op->NE), dim3(ebl), 0, st, a);
+        hipLaunchKernelGGL((k_limiter<DIM, NP>), dim3(lblk), dim3(ebl), 0, st, a);
     }
     HIP_TRY(hipGetLastError());
     return PRANDTL_OK;
@@ -703,6 +721,7 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         (e = hipMalloc(&op->d_geom, gb)) != hipSuccess ||
         (e = hipMemcpy(op->d_geom, geometry, gb, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMalloc(&op->d_lam, sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&op->d_lam_part, std::max<size_t>(2 * (size_t)op->NE + 64, 8192) * sizeof(double))) != hipSuccess ||
         (e = hipMalloc(&op->d_nan, sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&op->d_alpha, (size_t)op->NE * sizeof(double))) != hipSuccess)
         return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
@@ -751,7 +770,10 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_fbound);
     (void)hipFree(op->d_hflag);
     (void)hipFree(op->d_hcount);
+    (void)hipFree(op->d_hlist);
+    (void)hipFree(op->d_hlistn);
     (void)hipFree(op->d_lam);
+    (void)hipFree(op->d_lam_part);
     (void)hipFree(op->d_nan);
     (void)hipFree(op->d_alpha);
     (void)hipFree(op->d_bc);
@@ -794,14 +816,13 @@ int prandtl_op_get_max_char_speed(prandtl_op op, double *lambda_host) {
 int prandtl_op_max_char_speed(prandtl_op op, const double *u, double *lambda_host, prandtl_stream stream) {
     if (!op || !u || !lambda_host) return fail(PRANDTL_ERR_ARG, "null argument");
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, op->d_lam, 0ull, 1);
-    HIP_TRY(hipGetLastError());
     const size_t nnodes = (size_t)op->NE * op->nn;
-    const int nblk = (int)std::min<size_t>(2048, (nnodes + 255) / 256);
+    const int nblk = (int)std::min<size_t>(2048, (nnodes + 255) / 256);   // <= 8192 waves
     if (op->dim == 2)
-        hipLaunchKernelGGL(k_max_speed<2>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam);
+        hipLaunchKernelGGL(k_max_speed<2>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam_part);
     else
-        hipLaunchKernelGGL(k_max_speed<3>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam);
+        hipLaunchKernelGGL(k_max_speed<3>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam_part);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, op->d_lam_part, nblk * 4, op->d_lam);
     HIP_TRY(hipGetLastError());
     unsigned long long bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, op->d_lam, sizeof(bits), hipMemcpyDeviceToHost, st));
