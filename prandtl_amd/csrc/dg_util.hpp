@@ -89,8 +89,18 @@ __global__ void k_max_speed(const double *u, size_t nnodes, double gamma, double
 // workgroup
 __global__ void __launch_bounds__(1024) k_reduce_max(const double *part, int n, unsigned long long *out) {
     __shared__ double s[16];
-    double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, part[i]);
+    // eight independent maxima per thread keep eight loads in flight
+    const int bd = blockDim.x;
+    double mk[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + 7 * bd < n; i += 8 * bd) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) mk[k] = fmax(mk[k], part[i + k * bd]);
+    }
+    for (; i < n; i += bd) mk[0] = fmax(mk[0], part[i]);
+    double m = mk[0];
+#pragma unroll
+    for (int k = 1; k < 8; k++) m = fmax(m, mk[k]);
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
     __syncthreads();
