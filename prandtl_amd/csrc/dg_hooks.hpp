@@ -148,7 +148,9 @@ __global__ void __launch_bounds__(256) k_hook_scan(const HookArgs<NP> A) {
     __syncthreads();
     if (tid == 0 && s_cnt > 0u) s_base = atomicAdd(A.nlist, s_cnt);
     __syncthreads();
-    if (slot >= 0) A.list[s_base + (unsigned)slot] = e;
+    // a counter that did not start at zero (a hook application that never ran) cannot write
+    // past the list
+    if (slot >= 0 && s_base + (unsigned)slot < (unsigned)A.NE) A.list[s_base + (unsigned)slot] = e;
 }
 
 // entropy bound: minimum over the element and its 2 dim face neighbours
@@ -184,7 +186,7 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
     const int tdepth = 31 - __clz((int)blockDim.x);   // steps per round: 2^tdepth - 1 lanes evaluate
     const bool node_ok = i < NN;
     const size_t vs = (size_t)A.NE * NN;
-    const unsigned nlist = *A.nlist;
+    const unsigned nlist = min(*A.nlist, (unsigned)A.NE);
     for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
     const int e = A.list[li];   // detected (EntropyFilter.cpp:183-191) by k_hook_scan<.., 1>
     const double eb = A.bound[e];
@@ -350,7 +352,7 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
     const bool node_ok = i < NN;
     const size_t vs = (size_t)A.NE * NN;
     const double eps = kLimiterEps;
-    const unsigned nlist = *A.nlist;
+    const unsigned nlist = min(*A.nlist, (unsigned)A.NE);
     for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
     const int e = A.list[li];
     double s[NEQ];

@@ -330,8 +330,9 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
     using C = HookCfg<DIM, NP>;
     const int sblk = (op->NE + C::EPB - 1) / C::EPB;
     constexpr int ebl = C::NN <= 64 ? 64 : 128;   // one element per workgroup at a time
-    // the filter and limiter walk the scan's list of elements with a fixed grid
-    const int lblk = std::min(op->NE, 2048);
+    // the filter and limiter walk the scan's list of elements with a fixed grid (the limiter's
+    // list is usually empty: a smaller grid costs less to launch)
+    const int lblk = std::min(op->NE, 2048), lblk_lim = std::min(op->NE, 512);
     if (what != HOOK_CONSTRAINTS) {
         // the scan appends to one of the hook's two counters and clears the other, which the
         // previous application of the hook (earlier on this stream) has finished reading
@@ -349,7 +350,7 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
         hipLaunchKernelGGL((k_filter<DIM, NP>), dim3(lblk), dim3(ebl), 0, st, a);
     } else {
         hipLaunchKernelGGL((k_hook_scan<DIM, NP, 2>), dim3(sblk), dim3(256), 0, st, a);
-        hipLaunchKernelGGL((k_limiter<DIM, NP>), dim3(lblk), dim3(ebl), 0, st, a);
+        hipLaunchKernelGGL((k_limiter<DIM, NP>), dim3(lblk_lim), dim3(ebl), 0, st, a);
     }
     HIP_TRY(hipGetLastError());
     return PRANDTL_OK;
