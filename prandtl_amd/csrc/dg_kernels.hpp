@@ -186,11 +186,6 @@ __device__ __forceinline__ int face_point_map(int code, int t) {
     return DIM == 2 ? u1 : u1 + NP * u2;
 }
 
-__device__ __forceinline__ void atomic_max_pos(unsigned long long *p, double v) {
-    // positive doubles order like their bit patterns
-    atomicMax(p, (unsigned long long)__double_as_longlong(v));
-}
-
 // Stage epilogue: writes the RHS (Mult) or applies the fused RK stage update to one value;
 // x is the RK register (LSRK) or the base state of the Shu-Osher combination (SSP).
 template <int NP>
