@@ -522,8 +522,7 @@ def geometry_case(args, dev):
     out["tgv32_p3_ns"] = res
     out["curved_cost"] = res["affine"]["dof_rhs_per_s"] / res["curved"]["dof_rhs_per_s"] - 1.0
     # (b) forward-facing step, refined twice
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "meshes",
-                        "ForwardFacingStep.msh")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "meshes", "ForwardFacingStep.msh")
     t0 = time.perf_counter()
     gm = M.gmsh_mesh(path, 3, {"left": 0, "right": 1, "top": 2, "bottom": 2}, refine=2)
     build_s = time.perf_counter() - t0

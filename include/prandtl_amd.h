@@ -170,7 +170,8 @@ int prandtl_op_mult(prandtl_op op, const double *u, double *dudt, prandtl_stream
 int prandtl_op_mult_grad(prandtl_op op, const double *u, double *dudt, double *grad, prandtl_stream stream);
 
 /* Max characteristic speed max(|V| + c) of the last Mult (Euler and NS) --
- * ref: DGSEMOperator::GetMaxCharSpeed, DGSEMOperator.hpp:106-109.  Synchronises. */
+ * ref: DGSEMOperator::GetMaxCharSpeed, DGSEMOperator.hpp:106-109.  Waits for the stream the
+ * last Mult ran on; prandtl_op_max_char_speed does not change it. */
 int prandtl_op_get_max_char_speed(prandtl_op op, double *lambda_host);
 
 /* max(|V| + c) over the nodes of u (the same quantity, computed from u alone). */
@@ -214,7 +215,9 @@ int prandtl_filter_apply(prandtl_op op, double *u, prandtl_stream stream);
 int prandtl_limiter_apply(prandtl_op op, double *u, prandtl_stream stream);
 
 /* Elements changed by the filter and by the limiter since the previous call (resets the
- * counters; synchronises the device). */
+ * counters; synchronises the device).  The filter and limiter of one operator are stream-ordered:
+ * calls on different streams must be synchronised in between; an application that finds its
+ * element list out of date changes nothing and this call then returns PRANDTL_ERR_ARG. */
 int prandtl_hook_counts(prandtl_op op, long long *filtered, long long *limited);
 
 /* Discrete adjoint (no counterpart in the reference, which has no derivatives).
@@ -228,6 +231,12 @@ int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *ou
  * at the start of every ckpt_every-step segment in the handle, for prandtl_rk_adjoint. */
 int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
                         prandtl_stream stream);
+
+/* Identifier of the last taped run of the handle (no counterpart in the reference): unique over
+ * the process's handles; 0 when there is none, when a taped run failed, or while one is being
+ * written.  A caller that tapes more than one run on a handle compares it with the identifier read
+ * after its own run before calling prandtl_rk_adjoint, which sweeps the handle's last tape. */
+int prandtl_rk_tape_id(prandtl_op op, unsigned long long *id);
 
 /* Reverse sweep through the last taped run: ubar holds dJ/du at the final state on entry
  * and dJ/du at the initial state on return.  Each segment is recomputed forward from its

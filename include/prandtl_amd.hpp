@@ -189,7 +189,15 @@ public:
 
     // dudt = L(u), fully overwritten (DGSEMOperator.cpp:537-591)
     void Mult(const Vector &u, Vector &dudt) const override {
+        CheckState(u, "u");
+        CheckState(dudt, "dudt");
         check(prandtl_op_mult(op_, u.Read(), dudt.Write(), stream_));
+    }
+    // a state vector of this operator: (dim + 2) x elements x nodes doubles (GetTrueVSize)
+    void CheckState(const Vector &x, const char *what) const {
+        if (x.Size() != (size_t)Height())
+            throw Error(PRANDTL_ERR_ARG, std::string(what) + " holds " + std::to_string(x.Size()) +
+                                             " values; the operator's state holds " + std::to_string(Height()));
     }
     // max(|V| + c) of the last Mult (DGSEMOperator.hpp:106-109)
     double GetMaxCharSpeed() const {
@@ -199,6 +207,7 @@ public:
     }
     // the same quantity from a state alone
     double MaxCharSpeed(const Vector &u) const {
+        CheckState(u, "u");
         double l = 0.0;
         check(prandtl_op_max_char_speed(op_, u.Read(), &l, stream_));
         return l;
@@ -207,6 +216,7 @@ public:
     // alpha and eta grid functions, DGSEMOperator::ComputeBlendingCoefficient); each Vector has
     // num_elements entries
     void ComputeBlendingCoefficient(const Vector &u, Vector &alpha, Vector &eta) const {
+        CheckState(u, "u");
         if (alpha.Size() != (size_t)num_elements_ || eta.Size() != (size_t)num_elements_)
             throw Error(PRANDTL_ERR_ARG, "alpha and eta hold one value per element");
         check(prandtl_blend_coefficients(op_, u.Read(), alpha.Write(), eta.Write(), stream_));
@@ -233,12 +243,14 @@ public:
     // x advanced by one step in place; t += dt (dt unchanged: fixed-step solvers)
     virtual void Step(Vector &x, double &t, double &dt) {
         if (!op_) throw Error(PRANDTL_ERR_ARG, "ODESolver::Step before Init");
+        op_->CheckState(x, "x");
         check(prandtl_rk_step(op_->Handle(), scheme(), x.ReadWrite(), &t, dt, op_->Stream()));
         op_->SetTime(t);
     }
     // nsteps steps at once (the time loop without output), NaN scan every nan_check steps
     void Run(Vector &x, double &t, double dt, int nsteps, int nan_check = 0) {
         if (!op_) throw Error(PRANDTL_ERR_ARG, "ODESolver::Run before Init");
+        op_->CheckState(x, "x");
         check(prandtl_rk_run(op_->Handle(), scheme(), x.ReadWrite(), &t, dt, nsteps, nan_check, op_->Stream()));
         op_->SetTime(t);
     }
@@ -289,9 +301,13 @@ class EntropyFilter : public Filter {
 public:
     explicit EntropyFilter(const DGSEMOperator &op) : op_(op) {}
     void GetFilterConstraints(const Vector &x) override {
+        op_.CheckState(x, "x");
         check(prandtl_filter_constraints(op_.Handle(), x.Read(), op_.Stream()));
     }
-    void FilterSolution(Vector &sol) override { check(prandtl_filter_apply(op_.Handle(), sol.ReadWrite(), op_.Stream())); }
+    void FilterSolution(Vector &sol) override {
+        op_.CheckState(sol, "sol");
+        check(prandtl_filter_apply(op_.Handle(), sol.ReadWrite(), op_.Stream()));
+    }
 
 private:
     const DGSEMOperator &op_;
@@ -308,7 +324,10 @@ public:
 class PositivityLimiter : public Limiter {
 public:
     explicit PositivityLimiter(const DGSEMOperator &op) : op_(op) {}
-    void LimitSolution(Vector &sol) override { check(prandtl_limiter_apply(op_.Handle(), sol.ReadWrite(), op_.Stream())); }
+    void LimitSolution(Vector &sol) override {
+        op_.CheckState(sol, "sol");
+        check(prandtl_limiter_apply(op_.Handle(), sol.ReadWrite(), op_.Stream()));
+    }
 
 private:
     const DGSEMOperator &op_;

@@ -31,7 +31,7 @@ EXPORTED = [
     "prandtl_op_profile", "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_last_error",
     "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
     "prandtl_device_alloc", "prandtl_device_free", "prandtl_memcpy", "prandtl_stream_synchronize",
-    "prandtl_blend_coefficients",
+    "prandtl_blend_coefficients", "prandtl_rk_tape_id",
 ]
 ALPHA_MAX, ALPHA_MIN = 0.5, 0.001   # the reference's blending defaults
 
@@ -105,6 +105,7 @@ def load():
         L.prandtl_op_vjp.argtypes = [vp, vp, vp, vp, vp]
         L.prandtl_rk_run_tape.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.prandtl_rk_adjoint.argtypes = [vp, vp, vp]
+        L.prandtl_rk_tape_id.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
         L.prandtl_filter_constraints.argtypes = [vp, vp, vp]
         L.prandtl_filter_apply.argtypes = [vp, vp, vp]
         L.prandtl_limiter_apply.argtypes = [vp, vp, vp]
@@ -116,7 +117,7 @@ def load():
                   "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile",
                   "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_filter_constraints",
                   "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
-                  "prandtl_blend_coefficients"):
+                  "prandtl_blend_coefficients", "prandtl_rk_tape_id"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -128,11 +129,29 @@ def _check(rc: int):
         raise PrandtlError(f"{ERRORS.get(rc, rc)}: {msg}")
 
 
-def _ptr(t) -> ctypes.c_void_p:
+def _ptr(t, numel=None, name="tensor") -> ctypes.c_void_p:
+    """Device pointer of a contiguous float64 CUDA tensor of `numel` values (checked: the kernels
+    index the whole state, so a short buffer would be written past its end)."""
     import torch
 
-    assert isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    if not isinstance(t, torch.Tensor):
+        raise PrandtlError(f"{name}: expected a torch.Tensor, got {type(t).__name__}")
+    if numel is not None and t.numel() != numel:
+        raise PrandtlError(f"{name}: {t.numel()} values; the operator's state holds {numel}")
+    if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+        raise PrandtlError(f"{name}: expected a contiguous float64 device tensor "
+                           f"(got {t.dtype} on {t.device}, contiguous={t.is_contiguous()})")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _ptrs(*specs):
+    """_ptr over (tensor, numel, name) triples, every size checked before any placement."""
+    import torch
+
+    for t, n, name in specs:
+        if isinstance(t, torch.Tensor) and n is not None and t.numel() != n:
+            raise PrandtlError(f"{name}: {t.numel()} values; the operator's state holds {n}")
+    return [_ptr(t, n, name) for t, n, name in specs]
 
 
 def _stream(stream=None) -> ctypes.c_void_p:
@@ -187,10 +206,12 @@ class Operator:
         return int(load().prandtl_op_size(self._h))
 
     def mult(self, u, dudt, stream=None):
-        _check(load().prandtl_op_mult(self._h, _ptr(u), _ptr(dudt), _stream(stream)))
+        pu, pd = _ptrs((u, self.size, "u"), (dudt, self.size, "dudt"))
+        _check(load().prandtl_op_mult(self._h, pu, pd, _stream(stream)))
 
     def mult_grad(self, u, dudt, grad, stream=None):
-        _check(load().prandtl_op_mult_grad(self._h, _ptr(u), _ptr(dudt), _ptr(grad), _stream(stream)))
+        pu, pd, pg = _ptrs((u, self.size, "u"), (dudt, self.size, "dudt"), (grad, self.mesh.dim * self.size, "grad"))
+        _check(load().prandtl_op_mult_grad(self._h, pu, pd, pg, _stream(stream)))
 
     def get_max_char_speed(self) -> float:
         v = ctypes.c_double(0.0)
@@ -199,39 +220,46 @@ class Operator:
 
     def max_char_speed(self, u, stream=None) -> float:
         v = ctypes.c_double(0.0)
-        _check(load().prandtl_op_max_char_speed(self._h, _ptr(u), ctypes.byref(v), _stream(stream)))
+        _check(load().prandtl_op_max_char_speed(self._h, _ptr(u, self.size, "u"), ctypes.byref(v), _stream(stream)))
         return v.value
 
     def profile(self, u, epilogue=0, iters=20, stream=None):
         """Average ms per launch of (lift pass, flux-differencing pass)."""
         ms = (ctypes.c_double * 2)()
-        _check(load().prandtl_op_profile(self._h, _ptr(u), int(epilogue), int(iters), ms, _stream(stream)))
+        _check(load().prandtl_op_profile(self._h, _ptr(u, self.size, "u"), int(epilogue), int(iters), ms, _stream(stream)))
         return float(ms[0]), float(ms[1])
 
     def rk_step(self, scheme, u, t, dt, stream=None) -> float:
         tt = ctypes.c_double(t)
-        _check(load().prandtl_rk_step(self._h, int(scheme), _ptr(u), ctypes.byref(tt), float(dt), _stream(stream)))
+        _check(load().prandtl_rk_step(self._h, int(scheme), _ptr(u, self.size, "u"), ctypes.byref(tt), float(dt), _stream(stream)))
         return tt.value
 
     def rk_run(self, scheme, u, t, dt, nsteps, nan_check=0, stream=None) -> float:
         tt = ctypes.c_double(t)
-        _check(load().prandtl_rk_run(self._h, int(scheme), _ptr(u), ctypes.byref(tt), float(dt), int(nsteps),
+        _check(load().prandtl_rk_run(self._h, int(scheme), _ptr(u, self.size, "u"), ctypes.byref(tt), float(dt), int(nsteps),
                                      int(nan_check), _stream(stream)))
         return tt.value
 
     def vjp(self, u, lam, out, stream=None):
         """out = (dL/du)^T lam at u."""
-        _check(load().prandtl_op_vjp(self._h, _ptr(u), _ptr(lam), _ptr(out), _stream(stream)))
+        pu, pl, po = _ptrs((u, self.size, "u"), (lam, self.size, "lam"), (out, self.size, "out"))
+        _check(load().prandtl_op_vjp(self._h, pu, pl, po, _stream(stream)))
 
     def rk_run_tape(self, u, t, dt, nsteps, ckpt_every, stream=None, scheme=LSRK45) -> float:
         tt = ctypes.c_double(t)
-        _check(load().prandtl_rk_run_tape(self._h, int(scheme), _ptr(u), ctypes.byref(tt), float(dt), int(nsteps),
+        _check(load().prandtl_rk_run_tape(self._h, int(scheme), _ptr(u, self.size, "u"), ctypes.byref(tt), float(dt), int(nsteps),
                                           int(ckpt_every), _stream(stream)))
         return tt.value
 
+    def tape_id(self) -> int:
+        """Identifier of the handle's last taped run (0: none, or it failed)."""
+        v = ctypes.c_ulonglong(0)
+        _check(load().prandtl_rk_tape_id(self._h, ctypes.byref(v)))
+        return int(v.value)
+
     def rk_adjoint(self, ubar, stream=None):
         """ubar: dJ/du_final on entry, dJ/du_initial on return (for the last taped run)."""
-        _check(load().prandtl_rk_adjoint(self._h, _ptr(ubar), _stream(stream)))
+        _check(load().prandtl_rk_adjoint(self._h, _ptr(ubar, self.size, "ubar"), _stream(stream)))
 
     # subcell blending -- DGSEMOperator::ComputeBlendingCoefficient (the alpha and eta grid functions)
     def blend_coefficients(self, u, stream=None):
@@ -240,18 +268,18 @@ class Operator:
 
         alpha = torch.empty(self.mesh.num_elements, dtype=torch.float64, device=u.device)
         eta = torch.empty_like(alpha)
-        _check(load().prandtl_blend_coefficients(self._h, _ptr(u), _ptr(alpha), _ptr(eta), _stream(stream)))
+        _check(load().prandtl_blend_coefficients(self._h, _ptr(u, self.size, "u"), _ptr(alpha), _ptr(eta), _stream(stream)))
         return alpha, eta
 
     # stage hooks -- Filter::GetFilterConstraints / FilterSolution, Limiter::LimitSolution
     def filter_constraints(self, u, stream=None):
-        _check(load().prandtl_filter_constraints(self._h, _ptr(u), _stream(stream)))
+        _check(load().prandtl_filter_constraints(self._h, _ptr(u, self.size, "u"), _stream(stream)))
 
     def filter_apply(self, u, stream=None):
-        _check(load().prandtl_filter_apply(self._h, _ptr(u), _stream(stream)))
+        _check(load().prandtl_filter_apply(self._h, _ptr(u, self.size, "u"), _stream(stream)))
 
     def limiter_apply(self, u, stream=None):
-        _check(load().prandtl_limiter_apply(self._h, _ptr(u), _stream(stream)))
+        _check(load().prandtl_limiter_apply(self._h, _ptr(u, self.size, "u"), _stream(stream)))
 
     def hook_counts(self):
         """(elements filtered, elements limited) since the previous call."""

@@ -178,7 +178,7 @@ def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: 
     if dim not in (2, 3):
         raise ConfigError(f"dim {dim} is not supported (2 or 3)")
     p = int(order if order is not None else rt.get("order", 3))
-    neq = int(rt.get("num_equations", dim + 2))
+    neq = int(rt.get("num_equations", 4))   # the reference's default (Simulation.cpp:102)
     if neq != dim + 2:
         raise ConfigError(f"num_equations {neq} != dim + 2")
     if rt.get("lifting_scheme", "BR1") != "BR1":
@@ -187,7 +187,7 @@ def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: 
         raise ConfigError(f"numerical flux {rt.get('numerical_flux')!r} (the reference accepts only Chandrashekar)")
     viscous = bool(ct.get("PARABOLIC", False))
     physics = dict(gamma=float(rt.get("gamma", 1.4)), Pr=float(rt.get("Pr", 0.72)),
-                   mu=float(rt.get("mu", 0.0)), R_gas=float(rt.get("R_gas", 287.0)), viscous=viscous,
+                   mu=float(rt.get("mu", 0.02)), R_gas=float(rt.get("R_gas", 287.05)), viscous=viscous,
                    sutherland=bool(ct.get("SUTHERLAND", False)))
     options = dict(surface_flux=SURF_CHANDRASHEKAR, blending=bool(ct.get("SUBCELL_FV_BLENDING", False)),
                    alpha_max=float(rt.get("alpha_max", 0.5)),
@@ -200,7 +200,9 @@ def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: 
         bc_specs.append(_bc_spec(name, props, dim, bc_values))
     bc_index = {n: k for k, n in enumerate(bc_names)}
     mpath = _resolve_mesh_file(rt, os.path.dirname(os.path.abspath(path)), mesh_dir)
-    refine = int(rt.get("ser_ref_levels", 0) or 0)
+    # serial refinement, then the parallel mesh's uniform refinement (Simulation.cpp:329-336): both
+    # split every element into 2^dim, so they fold into one count
+    refine = int(rt.get("ser_ref_levels", 0) or 0) + int(rt.get("par_ref_levels", 0) or 0)
     if mpath.endswith(".msh"):
         mesh = M.gmsh_mesh(mpath, p, bc_index, refine=refine)
     else:

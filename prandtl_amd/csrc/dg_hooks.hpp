@@ -65,7 +65,8 @@ struct HookArgs {
     double *emin;                  // per-element minimum nodal entropy
     double *bound;                 // per-element entropy bound
     int *flag;                     // per-element: the element needs the hook
-    unsigned long long *count;     // elements changed: [0] filter, [1] limiter
+    unsigned long long *count;     // elements changed: [0] filter, [1] limiter; [2] nonzero: a
+                                   // hook found its list counter above NE (hooks run out of order)
     int *list;                     // the elements that need the hook (k_hook_scan MODE 1, 2)
     unsigned int *nlist;           // their number (appended by the scan, read by the hook)
     unsigned int *nlist_next;      // the other counter of this hook: cleared by the scan
@@ -186,7 +187,13 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
     const int tdepth = 31 - __clz((int)blockDim.x);   // steps per round: 2^tdepth - 1 lanes evaluate
     const bool node_ok = i < NN;
     const size_t vs = (size_t)A.NE * NN;
-    const unsigned nlist = min(*A.nlist, (unsigned)A.NE);
+    // the scan before this hook (on the same stream) started the counter at zero, so it holds at
+    // most NE; more means the hooks of this operator ran out of order: report, change nothing
+    unsigned nlist = *A.nlist;
+    if (nlist > (unsigned)A.NE) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) A.count[2] = 1ull;
+        nlist = 0u;
+    }
     for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
     const int e = A.list[li];   // detected (EntropyFilter.cpp:183-191) by k_hook_scan<.., 1>
     const double eb = A.bound[e];
@@ -352,7 +359,13 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
     const bool node_ok = i < NN;
     const size_t vs = (size_t)A.NE * NN;
     const double eps = kLimiterEps;
-    const unsigned nlist = min(*A.nlist, (unsigned)A.NE);
+    // the scan before this hook (on the same stream) started the counter at zero, so it holds at
+    // most NE; more means the hooks of this operator ran out of order: report, change nothing
+    unsigned nlist = *A.nlist;
+    if (nlist > (unsigned)A.NE) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) A.count[2] = 1ull;
+        nlist = 0u;
+    }
     for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
     const int e = A.list[li];
     double s[NEQ];

@@ -410,8 +410,16 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
 // at the face point, outward), by both elements' work items; the FV subcell metrics come from
 // the line's per-node adj rows (the DTables sub weights).
 // ---------------------------------------------------------------------------
-template <int DIM, int NP, bool VISC, int SURF, bool MIX, bool BLEND = false, bool BND = false, bool GEN = false>
+//
+// BUG1 (bug_compat A1, 3D Navier-Stokes on affine FP64 meshes): the reference's NS element stores,
+// for the zeta pair of node (i, j, k) with its partner (i, j, m > k), the last eta flux of node
+// (i, j, k) in the partner's slot (H_inviscid(id2).SetCol(k, g), DGSEMIntegrator.cpp:702): the
+// flux between (i, j, k) and (i, Np-1, k) with the eta metric, or 0 when j = Np-1 (g is zeroed
+// before the eta loop).  A node's lower-zeta partners then contribute that flux instead.
+template <int DIM, int NP, bool VISC, int SURF, bool MIX, bool BLEND = false, bool BND = false, bool GEN = false,
+          bool BUG1 = false>
 __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES) k_rhs_node(const RhsArgs<NP> A) {
+    static_assert(!(BUG1 && (DIM != 3 || !VISC || MIX || BND || GEN)), "bug_compat A1: 3D NS, affine, FP64");
     using C = NodeCfg<DIM, NP>;
     using R = typename std::conditional<MIX, float, double>::type;  // flux arithmetic
     static_assert(!(MIX && BLEND), "subcell blending is FP64 only");
@@ -761,6 +769,26 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         qb.vel[d] = sPv[d][li];
                     }
                     volume_flux_mx<DIM>(qo, qb, m, A.P.gammaM1Inverse, p_ref, h_ref, F);
+                } else if (BUG1 && dir == 2 && b < a) {
+                    // the lower-zeta partner's slot holds its own last eta flux (:702 as written)
+                    const int j = ai[1];
+                    if (j == NP - 1) {
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) F[v] = R(0);
+                    } else {
+                        const int lj = li + (NP - 1 - j) * NP;   // node (i, Np-1, b)
+                        NodePrim<DIM> qo, qb;
+                        qo.rho = sPd[0][li];
+                        qo.beta = sPd[1][li];
+                        qb.rho = sPd[0][lj];
+                        qb.beta = sPd[1][lj];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) {
+                            qo.vel[d] = sPv[d][li];
+                            qb.vel[d] = sPv[d][lj];
+                        }
+                        volume_flux<DIM, true>(qo, qb, sM[eb][1], A.P.gammaM1Inverse, F);
+                    }
                 } else {
                     NodePrim<DIM> qo, qb;
                     qo.rho = sPd[0][tid];
@@ -939,9 +967,15 @@ struct LiftCfg {
 // GEN (general meshes): per-node metrics and oriented face matching, as in k_rhs_node; the
 // jump term of a face point uses the node's own outward normal (the reference's face normal up
 // to rounding on a conforming mesh).
-template <int DIM, int NP, bool MIX, bool BND = false, bool GEN = false>
+//
+// BUG2 (bug_compat A2, affine FP64 meshes): the viscous flux of the volume term is evaluated with
+// the conserved state of the element's last node (the reference reads el_u at a stale id1,
+// DGSEMIntegrator.cpp:620 in 3D and :770 in 2D, which the blended build leaves at n-1) and each
+// node's own gradient; the face traces keep the node's own state (AssembleFaceVector).
+template <int DIM, int NP, bool MIX, bool BND = false, bool GEN = false, bool BUG2 = false>
 __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES) k_lift(const LiftArgs<NP> A) {
     static_assert(!(MIX && BND), "boundary faces run in FP64");
+    static_assert(!(BUG2 && (MIX || BND || GEN)), "bug_compat A2 runs on affine FP64 meshes");
     using C = LiftCfg<DIM, NP>;
     // derivative contractions, chain rule and viscous flux in R; entropy variables in FP64.
     // Mixed precision contracts w - w_ref (w_ref = the element's node-0 value; D annihilates
@@ -1095,6 +1129,13 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
         }
         R flux[DIM + 1][DIM];
         viscous_flux<DIM, R>(sr, g, A.P, flux);
+        R fluxb[BUG2 ? DIM + 1 : 1][DIM];
+        if constexpr (BUG2) {
+            R sl[NEQ];
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) sl[v] = (R)A.u[v * vs + (size_t)e * NN + NN - 1];
+            viscous_flux<DIM, R>(sl, g, A.P, fluxb);
+        }
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             const int a = ai[d];
@@ -1104,7 +1145,14 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
                 R f = R(0);
 #pragma unroll
                 for (int c = 0; c < DIM; c++) f += adj[d][c] * flux[v][c];
-                Fc[d][v] = f;
+                if constexpr (BUG2) {
+                    R fb = R(0);
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) fb += adj[d][c] * fluxb[v][c];
+                    Fc[d][v] = fb;
+                } else {
+                    Fc[d][v] = f;
+                }
                 if constexpr (MIX) {
                     if (a == 0) A.Fff[ftrace<DIM, NP>(d, 0, v, e, t, NE)] = f;
                     if (a == NP - 1) A.Fff[ftrace<DIM, NP>(d, 1, v, e, t, NE)] = f;

@@ -75,14 +75,16 @@ struct prandtl_op_s {
     double *d_vwb = nullptr;      // NS adjoint: partial adjoint of the entropy variables
     double *d_vsb = nullptr;      // NS adjoint: partial adjoint of the state
     double *d_vlam = nullptr;     // adjoint: stage cotangent lambda
-    unsigned long long *d_lam = nullptr;
+    unsigned long long *d_lam = nullptr;   // [0]: max |V| + c of the last Mult; [1]: of the last max_char_speed
+    hipStream_t lam_stream = nullptr;      // the stream of the last Mult (its d_lam[0] is written there)
     double *d_lam_part = nullptr; // per-wave maxima of |V| + c (2 NE + 64 entries, at least 8192)
     int lam_waves = 0;            // waves of the last flux kernel launched (their maxima in d_lam_part)
     int *d_nan = nullptr;
     bool lam_valid = false;
     // taped forward run for the discrete adjoint: the state at the start of every segment
     int tape_steps = 0, tape_every = 0, tape_scheme = PRANDTL_LSRK45;
-    double tape_dt = 0.0;
+    double tape_dt = 0.0, tape_t0 = 0.0;
+    unsigned long long tape_id = 0;   // identifies the taped run (0: none, or the last one failed)
     std::vector<double *> tape_ckpt;
     // adjoint sweep storage, kept between calls: the stage inputs of one segment (5 per step;
     // slot 0 points at the segment's checkpoint) and the double-buffered adjoint registers
@@ -174,7 +176,9 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
         la.forient = op->d_forient;
         const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
         if constexpr (!MIX) {
-            if (op->d_ngeom && op->nbc)
+            if (op->opts.bug_compat & PRANDTL_BUG_A2)   // affine, no boundary faces (checked at create)
+                hipLaunchKernelGGL((k_lift<DIM, NP, false, false, false, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
+            else if (op->d_ngeom && op->nbc)
                 hipLaunchKernelGGL((k_lift<DIM, NP, false, true, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
             else if (op->d_ngeom)
                 hipLaunchKernelGGL((k_lift<DIM, NP, false, false, true>), dim3(nblk), dim3(NC::BLOCK), 0, st, la);
@@ -234,6 +238,23 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
                 hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, false, false, true>), dim3(nblk), dim3(LC::BLOCK), 0, st, ra);
             HIP_TRY(hipGetLastError());
             return PRANDTL_OK;
+        }
+        if constexpr (VISC && DIM == 3) {
+            if (op->opts.bug_compat & PRANDTL_BUG_A1) {
+                // the reference's zeta flux store (affine, no boundary faces: checked at create)
+                if (op->opts.blending) ra.blend = make_blend_params<NP>(op);
+                using LC = NodeCfg<DIM, NP>;
+                const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
+                op->lam_waves = nblk * LC::BLOCK / 64;
+                if (op->opts.blending)
+                    hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, true, false, false, true>), dim3(nblk),
+                                       dim3(LC::BLOCK), 0, st, ra);
+                else
+                    hipLaunchKernelGGL((k_rhs_node<DIM, NP, VISC, SURF, false, false, false, false, true>), dim3(nblk),
+                                       dim3(LC::BLOCK), 0, st, ra);
+                HIP_TRY(hipGetLastError());
+                return PRANDTL_OK;
+            }
         }
         if (op->opts.blending || op->nbc) {
             // blending: the kernel computes this evaluation's coefficients from its input state

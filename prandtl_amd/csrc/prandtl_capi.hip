@@ -6,6 +6,8 @@
 #include "op_internal.hpp"
 #include "dg_util.hpp"
 
+#include <atomic>
+
 using namespace prandtl;
 
 namespace prandtl {
@@ -58,6 +60,12 @@ void gll_rule(int np, double *x, double *w) {
 }  // namespace prandtl
 
 namespace {
+
+// identifiers of taped runs, unique over the handles of the process
+unsigned long long next_tape_id() {
+    static std::atomic<unsigned long long> counter{0};
+    return ++counter;
+}
 
 // Barycentric differentiation matrix and its surface-corrected forms
 // (DGSEMIntegrator.cpp:45-80), stored as mathematical matrices M[i][j].
@@ -233,6 +241,8 @@ int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     if (op->opts.surface_flux == PRANDTL_SURF_RUSANOV)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers the Chandrashekar and EC surface fluxes");
     if (op->opts.precision != PRANDTL_FP64) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint is FP64");
+    if (op->phys.viscous && (op->opts.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
+        return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint differentiates the intended operator, not bug_compat A1/A2");
     if (op->dim == 2) {
         switch (op->np) {
             case 2: return launch_vjp<2, 2>(op, S, st);
@@ -298,10 +308,10 @@ int ensure_hook_buffers(prandtl_op_s *op) {
         HIP_TRY(hipMalloc(&op->d_emin, (size_t)op->NE * sizeof(double)));
         HIP_TRY(hipMalloc(&op->d_fbound, (size_t)op->NE * sizeof(double)));
         HIP_TRY(hipMalloc(&op->d_hflag, (size_t)op->NE * sizeof(int)));
-        HIP_TRY(hipMalloc(&op->d_hcount, 2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&op->d_hcount, 3 * sizeof(unsigned long long)));
         HIP_TRY(hipMalloc(&op->d_hlist, (size_t)op->NE * sizeof(int)));
         HIP_TRY(hipMalloc(&op->d_hlistn, 4 * sizeof(unsigned int)));
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 2);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 3);
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, 0, reinterpret_cast<int *>(op->d_hlistn), 0, 4);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipDeviceSynchronize());
@@ -333,14 +343,14 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
     // the filter and limiter walk the scan's list of elements with a fixed grid (the limiter's
     // list is usually empty: a smaller grid costs less to launch)
     const int lblk = std::min(op->NE, 2048), lblk_lim = std::min(op->NE, 512);
+    const int h = what == HOOK_FILTER ? 0 : 1, par = op->hook_par[h];
     if (what != HOOK_CONSTRAINTS) {
         // the scan appends to one of the hook's two counters and clears the other, which the
-        // previous application of the hook (earlier on this stream) has finished reading
-        const int h = what == HOOK_FILTER ? 0 : 1, p = op->hook_par[h];
+        // previous application of the hook (earlier on this stream: the hooks of one operator are
+        // stream-ordered) has finished reading
         a.list = op->d_hlist;
-        a.nlist = op->d_hlistn + 2 * h + p;
-        a.nlist_next = op->d_hlistn + 2 * h + (1 - p);
-        op->hook_par[h] = 1 - p;
+        a.nlist = op->d_hlistn + 2 * h + par;
+        a.nlist_next = op->d_hlistn + 2 * h + (1 - par);
     }
     if (what == HOOK_CONSTRAINTS) {
         hipLaunchKernelGGL((k_hook_scan<DIM, NP, 0>), dim3(sblk), dim3(256), 0, st, a);
@@ -353,6 +363,8 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
         hipLaunchKernelGGL((k_limiter<DIM, NP>), dim3(lblk_lim), dim3(ebl), 0, st, a);
     }
     HIP_TRY(hipGetLastError());
+    // the counters switch roles only once this application is queued
+    if (what != HOOK_CONSTRAINTS) op->hook_par[h] = 1 - par;
     return PRANDTL_OK;
 }
 
@@ -625,8 +637,11 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         return fail(PRANDTL_ERR_ARG, "alpha_min must lie in [0, 0.5) and alpha_max in [0, 1]");
     if (o.blending && o.precision != PRANDTL_FP64)
         return fail(PRANDTL_ERR_UNSUPPORTED, "subcell blending runs in FP64");
-    if (phys->viscous && (o.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
-        return fail(PRANDTL_ERR_UNSUPPORTED, "bug_compat A1/A2 are reproduced by the CPU oracle only");
+    if (o.bug_compat & ~(PRANDTL_BUG_A1 | PRANDTL_BUG_A2 | PRANDTL_BUG_A3))
+        return fail(PRANDTL_ERR_ARG, "unknown bug_compat bit");
+    if (phys->viscous && (o.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)) &&
+        (general || mesh->num_bc > 0 || o.precision != PRANDTL_FP64))
+        return fail(PRANDTL_ERR_UNSUPPORTED, "bug_compat A1/A2 run on periodic affine meshes in FP64");
     if (phys->viscous && (!(phys->Pr > 0.0) || phys->mu < 0.0)) return fail(PRANDTL_ERR_ARG, "bad Pr or mu");
     if (mesh->num_bc < 0 || (mesh->num_bc > 0 && !mesh->bc)) return fail(PRANDTL_ERR_ARG, "bad boundary-condition table");
     for (int k = 0; k < mesh->num_bc; k++) {
@@ -721,7 +736,7 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         (e = hipMemcpy(op->d_nbr, mesh->neighbors, nb, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMalloc(&op->d_geom, gb)) != hipSuccess ||
         (e = hipMemcpy(op->d_geom, geometry, gb, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMalloc(&op->d_lam, sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&op->d_lam, 2 * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc(&op->d_lam_part, std::max<size_t>(2 * (size_t)op->NE + 64, 8192) * sizeof(double))) != hipSuccess ||
         (e = hipMalloc(&op->d_nan, sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&op->d_alpha, (size_t)op->NE * sizeof(double))) != hipSuccess)
@@ -791,6 +806,7 @@ int prandtl_op_mult(prandtl_op op, const double *u, double *dudt, prandtl_stream
     StageSpec S{EPI_MULT, u, dudt, nullptr, nullptr, 0.0, 0.0, 0.0, nullptr, true, false};
     int rc = run_stage(op, S, (hipStream_t)stream);
     op->lam_valid = (rc == PRANDTL_OK);
+    op->lam_stream = (hipStream_t)stream;
     return rc;
 }
 
@@ -800,14 +816,18 @@ int prandtl_op_mult_grad(prandtl_op op, const double *u, double *dudt, double *g
     StageSpec S{EPI_MULT, u, dudt, nullptr, nullptr, 0.0, 0.0, 0.0, grad, true, false};
     int rc = run_stage(op, S, (hipStream_t)stream);
     op->lam_valid = (rc == PRANDTL_OK);
+    op->lam_stream = (hipStream_t)stream;
     return rc;
 }
 
 int prandtl_op_get_max_char_speed(prandtl_op op, double *lambda_host) {
     if (!op || !lambda_host) return fail(PRANDTL_ERR_ARG, "null argument");
     if (!op->lam_valid) return fail(PRANDTL_ERR_ARG, "no Mult has run on this operator");
+    // ordered after the Mult on its own stream (a non-blocking caller stream is not ordered with
+    // a plain copy)
     unsigned long long bits = 0;
-    HIP_TRY(hipMemcpy(&bits, op->d_lam, sizeof(bits), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(&bits, op->d_lam, sizeof(bits), hipMemcpyDeviceToHost, op->lam_stream));
+    HIP_TRY(hipStreamSynchronize(op->lam_stream));
     double v;
     std::memcpy(&v, &bits, sizeof(v));
     *lambda_host = v;
@@ -823,10 +843,11 @@ int prandtl_op_max_char_speed(prandtl_op op, const double *u, double *lambda_hos
         hipLaunchKernelGGL(k_max_speed<2>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam_part);
     else
         hipLaunchKernelGGL(k_max_speed<3>, dim3(nblk), dim3(256), 0, st, u, nnodes, op->P.gamma, op->d_lam_part);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, op->d_lam_part, nblk * 4, op->d_lam);
+    // its own result word: the last Mult's speed (d_lam[0]) stays what get_max_char_speed returns
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, op->d_lam_part, nblk * 4, op->d_lam + 1);
     HIP_TRY(hipGetLastError());
     unsigned long long bits = 0;
-    HIP_TRY(hipMemcpyAsync(&bits, op->d_lam, sizeof(bits), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&bits, op->d_lam + 1, sizeof(bits), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     double v;
     std::memcpy(&v, &bits, sizeof(v));
@@ -910,12 +931,16 @@ int prandtl_hook_counts(prandtl_op op, long long *filtered, long long *limited) 
     if (!op) return fail(PRANDTL_ERR_ARG, "null argument");
     int rc = ensure_hook_buffers(op);
     if (rc) return rc;
-    unsigned long long c[2] = {0, 0};
+    unsigned long long c[3] = {0, 0, 0};
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(c, op->d_hcount, sizeof(c), hipMemcpyDeviceToHost));
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 2);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 3);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
+    if (c[2])
+        return fail(PRANDTL_ERR_ARG, "a filter or limiter application found its element list out of date: the "
+                                     "hooks of one operator must run in order on one stream (or synchronise "
+                                     "between streams); the affected applications changed nothing");
     if (filtered) *filtered = (long long)c[0];
     if (limited) *limited = (long long)c[1];
     return PRANDTL_OK;
@@ -968,6 +993,9 @@ int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double 
     if (op->nbc || op->d_ngeom) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic affine meshes");
     if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
     hipStream_t st = (hipStream_t)stream;
+    // the previous tape is gone from here on: a run that fails leaves no tape
+    op->tape_id = 0;
+    op->tape_steps = 0;
     const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
     if ((int)op->tape_ckpt.size() != nseg) {   // checkpoint storage is kept for runs of the same length
         release_adjoint_storage(op);            // its slot 0 points at a checkpoint
@@ -980,23 +1008,25 @@ int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double 
                 return fail(PRANDTL_ERR_HIP, "checkpoint storage allocation failed");
             }
     }
-    op->tape_steps = nsteps;
-    op->tape_every = ckpt_every;
-    op->tape_dt = dt;
-    op->tape_scheme = scheme;
     for (int k = 0; k < nseg; k++) {
         HIP_TRY(hipMemcpyAsync(op->tape_ckpt[k], u, op->size * sizeof(double), hipMemcpyDeviceToDevice, st));
         const int n = std::min(ckpt_every, nsteps - k * ckpt_every);
         int rc = rk_advance(op, scheme, u, dt, n, 0, st);
         if (rc) return rc;
     }
+    op->tape_steps = nsteps;
+    op->tape_every = ckpt_every;
+    op->tape_dt = dt;
+    op->tape_scheme = scheme;
+    op->tape_t0 = t ? *t : 0.0;
+    op->tape_id = next_tape_id();
     if (t) *t += dt * nsteps;
     return PRANDTL_OK;
 }
 
 int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
     if (!op || !ubar) return fail(PRANDTL_ERR_ARG, "null argument");
-    if (op->tape_ckpt.empty()) return fail(PRANDTL_ERR_ARG, "no taped run on this operator");
+    if (op->tape_ckpt.empty() || op->tape_id == 0) return fail(PRANDTL_ERR_ARG, "no taped run on this operator");
     hipStream_t st = (hipStream_t)stream;
     const size_t bytes = op->size * sizeof(double);
     const int C = op->tape_every, nsteps = op->tape_steps, nseg = (int)op->tape_ckpt.size();
@@ -1092,6 +1122,12 @@ int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
     hipError_t e = hipStreamSynchronize(st);
     if (rc) return rc;
     if (e != hipSuccess) return fail(PRANDTL_ERR_HIP, hipGetErrorString(e));
+    return PRANDTL_OK;
+}
+
+int prandtl_rk_tape_id(prandtl_op op, unsigned long long *id) {
+    if (!op || !id) return fail(PRANDTL_ERR_ARG, "null argument");
+    *id = op->tape_id;
     return PRANDTL_OK;
 }
 

@@ -1,28 +1,45 @@
 """Thin torch binding of the discrete adjoint (tests and the adjoint benchmark).
 
-RKSolve.apply(u0, op, dt, nsteps, ckpt_every[, scheme]) advances u0 by nsteps LSRK4(5) (or
-SSP-RK3) steps through the C ABI (prandtl_rk_run_tape) and, on backward, returns
+RKSolve.apply(u0, op, dt, nsteps, ckpt_every[, scheme[, t0]]) advances u0 by nsteps LSRK4(5) (or
+SSP-RK3) steps from time t0 through the C ABI (prandtl_rk_run_tape) and, on backward, returns
 dJ/du0 = (du_T/du0)^T dJ/du_T from prandtl_rk_adjoint: the hand-written operator adjoint and the
 reverse RK stages, with the stage inputs recomputed segment by segment from the checkpoints.
+The handle keeps one tape: backward checks that it is still the forward's (check_tape).
 """
 from __future__ import annotations
 
 import torch
 
 
+class TapeMismatch(RuntimeError):
+    """The operator's last taped run is not the one a backward pass differentiates."""
+
+
+def check_tape(op, tape_id):
+    """Raise TapeMismatch unless the handle's last taped run is `tape_id` (prandtl_rk_adjoint sweeps
+    the handle's last tape: a later taped run on the same operator replaces it)."""
+    cur = op.tape_id()
+    if tape_id == 0 or cur != tape_id:
+        raise TapeMismatch(f"the operator's last taped run is {cur}, this backward needs run {tape_id}: "
+                           f"another taped run on the same operator replaced it (run the backward first, or "
+                           f"use one operator per trajectory)")
+
+
 class RKSolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, op, dt, nsteps, ckpt_every, scheme=1):
+    def forward(ctx, u0, op, dt, nsteps, ckpt_every, scheme=1, t0=0.0):
         u = u0.detach().clone().contiguous()
-        op.rk_run_tape(u, 0.0, dt, nsteps, ckpt_every, scheme=scheme)
+        op.rk_run_tape(u, float(t0), dt, nsteps, ckpt_every, scheme=scheme)
         ctx.op = op
+        ctx.tape_id = op.tape_id()
         return u
 
     @staticmethod
     def backward(ctx, ubar):
+        check_tape(ctx.op, ctx.tape_id)
         g = ubar.detach().clone().contiguous()
         ctx.op.rk_adjoint(g)
-        return g, None, None, None, None, None
+        return g, None, None, None, None, None, None
 
 
 def central_fd(f, x, h):

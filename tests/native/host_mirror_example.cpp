@@ -136,6 +136,15 @@ int main(int argc, char **argv) {
         } catch (const prandtl::Error &e) {
             std::printf("rejected as expected (%d): %s\n", e.code(), e.what());
         }
+        // a vector of the wrong size never reaches the kernels
+        try {
+            prandtl::Vector short_dudt((size_t)op.Height() - 1);
+            op.Mult(u, short_dudt);
+            std::printf("short vector not rejected\n");
+            return 1;
+        } catch (const prandtl::Error &e) {
+            std::printf("rejected as expected (%d): %s\n", e.code(), e.what());
+        }
     } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

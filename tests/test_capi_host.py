@@ -68,8 +68,9 @@ def test_rejects_nonconforming_faces_and_bad_jacobian():
 
 
 @pytest.mark.parametrize("kw,code", [({"gamma": 1.0}, -1), ({"surface_flux": 9}, -1), ({"precision": 7}, -1),
-                                     ({"viscous": 1, "bug_compat": 1}, -2), ({"viscous": 1, "Pr": 0.0}, -1),
-                                     ({"stage_hooks": 4}, -1)])
+                                     ({"viscous": 1, "bug_compat": 1, "precision": 1}, -2),
+                                     ({"viscous": 1, "bug_compat": 2, "precision": 1}, -2), ({"bug_compat": 8}, -1),
+                                     ({"viscous": 1, "Pr": 0.0}, -1), ({"stage_hooks": 4}, -1)])
 def test_rejects_bad_options(kw, code):
     m = M.periodic_lattice(2, 3, (2, 2), lengths=(1, 1))
     rc, msg, _ = _create(m, **kw)
@@ -185,3 +186,39 @@ def test_general_mesh_validation():
     assert ((m3.face_orient >> 5) & 1).any()
     rc, msg = _create_general(m3, m3.node_geom, m3.face_orient)
     assert rc == 0 or (rc == -3 and "device" in msg), msg
+
+
+class _SizedOperator(capi.Operator):
+    """An Operator without a device handle whose state holds `n` values: the binding's argument
+    checks run before any call into the library."""
+
+    def __init__(self, mesh, n):
+        self.mesh = mesh
+        self._h = None
+        self._n = n
+
+    @property
+    def size(self):
+        return self._n
+
+
+def test_binding_rejects_mis_sized_buffers():
+    import torch
+
+    m = M.periodic_lattice(3, 2, (2, 2, 2), lengths=(1, 1, 1))
+    n = 5 * m.num_elements * 27
+    op = _SizedOperator(m, n)
+    good, short = torch.zeros(n, dtype=torch.float64), torch.zeros(n - 1, dtype=torch.float64)
+    calls = [lambda: op.mult(short, good), lambda: op.mult(good, short), lambda: op.rk_step(1, short, 0.0, 1e-3),
+             lambda: op.rk_run(1, short, 0.0, 1e-3, 2), lambda: op.vjp(good, short, good),
+             lambda: op.filter_apply(short), lambda: op.limiter_apply(short), lambda: op.filter_constraints(short),
+             lambda: op.max_char_speed(short), lambda: op.rk_adjoint(short),
+             lambda: op.mult_grad(good, good, torch.zeros(n, dtype=torch.float64))]   # grad holds dim x size
+    for c in calls:
+        with pytest.raises(capi.PrandtlError, match="the operator's state holds"):
+            c()
+    # right size, wrong place or type: refused too (no assert that python -O strips)
+    with pytest.raises(capi.PrandtlError, match="float64 device tensor"):
+        op.mult(good, good.clone())
+    with pytest.raises(capi.PrandtlError, match="expected a torch.Tensor"):
+        op.mult(np.zeros(n), good)

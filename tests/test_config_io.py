@@ -25,7 +25,7 @@ from prandtl_amd import cases, checkpoint, config, mesh as M
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CONFIGS = os.path.join(HERE, "golden", "configs")
-MESHES = os.path.join(HERE, "golden", "meshes")
+MESHES = os.path.join(os.path.dirname(HERE), "data", "meshes")
 EPS = np.finfo(np.float64).eps
 
 
@@ -109,6 +109,52 @@ def test_refusals():
     # the cavity's values can be overridden; an unknown key must be supplied
     c = _load("LidDrivenCavity", bc_values={"LidDrivenCavityLidVelBCVector": [0.5, 0.0]})
     assert c.bc_specs[3] == (10, [0.5, 0.0, 0.0])
+
+
+def _load_modified(name, edit):
+    import json
+    import tempfile
+
+    cfg = json.load(open(os.path.join(CONFIGS, name + ".json")))
+    edit(cfg["runTime"])
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as fh:
+        json.dump(cfg, fh)
+    try:
+        return config.load_case(fh.name, mesh_dir=MESHES)
+    finally:
+        os.unlink(fh.name)
+
+
+def test_reference_defaults_of_omitted_keys():
+    # LoadConfig's defaults (Simulation.cpp:102,154-157): mu 0.02, R_gas 287.05, Pr 0.72, gamma 1.4,
+    # num_equations 4
+    def drop(rt):
+        for k in ("mu", "R_gas", "Pr", "gamma", "num_equations"):
+            rt.pop(k, None)
+
+    c = _load_modified("LidDrivenCavity", drop)
+    assert c.physics == dict(gamma=1.4, Pr=0.72, mu=0.02, R_gas=287.05, viscous=True, sutherland=False)
+
+    def three_d(rt):
+        rt.pop("num_equations", None)
+        rt["dim"] = 3
+
+    with pytest.raises(config.ConfigError, match="num_equations 4"):
+        _load_modified("LidDrivenCavity", three_d)
+
+
+def test_parallel_refinement_levels():
+    # par_ref_levels refines the parallel mesh uniformly after the serial refinement
+    # (Simulation.cpp:329-336): each level splits every quadrilateral in four
+    base = _load("LidDrivenCavity")
+
+    def par1(rt):
+        rt["par_ref_levels"] = 1
+
+    c = _load_modified("LidDrivenCavity", par1)
+    assert c.mesh.num_elements == 4 * base.mesh.num_elements
+    assert (c.mesh.neighbors < 0).sum() == 2 * (base.mesh.neighbors < 0).sum()
+    assert "par_ref_levels" not in c.ignored
 
 
 def test_mfem_periodic_mesh_is_the_lattice():

@@ -19,7 +19,7 @@ from prandtl_amd import capi, cases, checkpoint, config
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 CONFIGS = os.path.join(HERE, "golden", "configs")
-MESHES = os.path.join(HERE, "golden", "meshes")
+MESHES = os.path.join(os.path.dirname(HERE), "data", "meshes")
 
 
 def _say(msg):

@@ -26,7 +26,7 @@ from prandtl_amd import capi, cases, mesh as M
 from . import meshgen as G
 
 pytestmark = pytest.mark.gpu
-MESHES = os.path.join(os.path.dirname(__file__), "golden", "meshes")
+MESHES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data", "meshes")
 
 
 def _say(msg):

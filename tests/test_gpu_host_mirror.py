@@ -84,7 +84,7 @@ def test_host_mirror_general_mesh(device):
     m = cases.vortex_mesh(n=16, order=3)
     cases.vortex_state(m).tofile(os.path.join(d, "u0.bin"))
     cases.vortex_state(m).tofile(os.path.join(d, "bad.bin"))
-    src = os.path.join(ROOT, "tests", "golden", "meshes", "LidDrivenCavity.msh")
+    src = os.path.join(ROOT, "data", "meshes", "LidDrivenCavity.msh")
     shutil.copy(src, os.path.join(d, "cavity.msh"))
     gm = M.gmsh_mesh(src, 3, {"left": 0, "right": 0, "bottom": 0, "lid": 1})
     X = gm.node_coords()

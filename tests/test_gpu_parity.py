@@ -155,6 +155,115 @@ def test_bug_a3_3d(device):
         assert_parity(d, ref, reff, "bug A3", mag)
 
 
+BUGS_A12 = [capi.BUG_A1, capi.BUG_A2, capi.BUG_A1 | capi.BUG_A2 | capi.BUG_A3]
+
+
+def _tgv_ns(n, order=3, seed=2):
+    m = cases.tgv_mesh(n=n, order=order)
+    u = cases.tgv_state(m)
+    u = u * (1.0 + 1e-4 * np.random.default_rng(seed).uniform(-1, 1, u.shape))
+    return m, u, cases.tgv_physics()
+
+
+@pytest.mark.parametrize("bugs", BUGS_A12)
+def test_bug_a1_a2_tgv(device, bugs):
+    # the reference's NS volume term as written (A1: the zeta flux store at DGSEMIntegrator.cpp:702;
+    # A2: the viscous flux at the stale node, :620) on the headline Taylor-Green state
+    _say(f"bug_compat {bugs}, TGV 6^3 p=3 NS")
+    m, u, ph = _tgv_ns(6)
+    kw = dict(gamma=ph["gamma"], Pr=ph["Pr"], mu=ph["mu"], viscous=True)
+    d, lam, g = gpu_rhs(m, u, device, grad=True, bug_compat=bugs, **ph)
+    oph = oracle.phys(bug_compat=bugs, **kw)
+    ref, lref, mag, gref, gmag = oracle.rhs_mag(m, oph, u, want_grad=True)
+    reff, _, greff = oracle.rhs(m, oph, u, want_grad=True, fma=True)
+    assert_parity(d, ref, reff, f"tgv ns bug {bugs}", mag)
+    # the gradients are the intended ones (the slips are in the volume assembly)
+    _, _, gint = oracle.rhs(m, oracle.phys(**kw), u, want_grad=True)
+    assert np.array_equal(gref, gint)
+    assert_parity(g, gref, greff, "tgv ns gradients", gmag)
+    # the switch changes the operator (its own path ran)
+    intended, _ = oracle.rhs(m, oracle.phys(**kw), u)
+    print(f"[bug] |as written - intended| / |intended| = {relerr(ref, intended):.3e}", flush=True)
+    assert relerr(ref, intended) > 1e-6
+    assert abs(lam - lref) <= 1e-14 * lref
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_bug_a1_a2_sheared(device, order):
+    _say(f"bug_compat A1|A2|A3 sheared 3D NS p={order}, and A2 in 2D")
+    bugs = capi.BUG_A1 | capi.BUG_A2 | capi.BUG_A3
+    m = sheared_mesh(3, order, (2, 3, 2))
+    u = perturbed_state(m, 11 + order)
+    kw = dict(viscous=True, mu=0.01, Pr=0.7)
+    for b in (bugs, capi.BUG_A1, capi.BUG_A2):
+        d, _, _ = gpu_rhs(m, u, device, bug_compat=b, **kw)
+        ref, _, mag = oracle.rhs_mag(m, oracle.phys(bug_compat=b, **kw), u)
+        reff, _ = oracle.rhs(m, oracle.phys(bug_compat=b, **kw), u, fma=True)
+        assert_parity(d, ref, reff, f"sheared 3D bug {b}", mag)
+    m2 = sheared_mesh(2, order, (3, 4))
+    u2 = perturbed_state(m2, 5 + order)
+    d, _, _ = gpu_rhs(m2, u2, device, bug_compat=capi.BUG_A2, **kw)
+    ref, _, mag = oracle.rhs_mag(m2, oracle.phys(bug_compat=capi.BUG_A2, **kw), u2)
+    reff, _ = oracle.rhs(m2, oracle.phys(bug_compat=capi.BUG_A2, **kw), u2, fma=True)
+    assert_parity(d, ref, reff, "sheared 2D bug A2", mag)
+
+
+def test_bug_a1_a2_blended(device):
+    # every reference configuration builds with SUBCELL_FV_BLENDING, the setting under which A2's
+    # stale index is node n-1 of the same element
+    import torch
+
+    _say("bug_compat A1|A2 with blending, 3D NS")
+    m = M.periodic_lattice(3, 3, (6, 3, 2), lengths=[1.0, 0.5, 0.4])
+    X = m.node_coords()
+    step = np.tanh((X[0] - 0.5) / 0.03)
+    rng = np.random.default_rng(4)
+    u = cases.prim_to_conserv(1.0 + 0.3 * step + 0.05 * np.sin(2 * np.pi * X[2]),
+                              [0.2 * np.cos(2 * np.pi * X[2] + d) for d in range(3)],
+                              1.0 + 0.24 * step + 0.02 * rng.uniform(-1, 1, X[0].shape))
+    kw = dict(viscous=True, mu=2e-3, Pr=0.71)
+    bugs = capi.BUG_A1 | capi.BUG_A2
+    op = capi.Operator(m, blending=True, bug_compat=bugs, **kw)
+    d = torch.from_numpy(u).to(device)
+    du = torch.empty_like(d)
+    op.mult(d, du)
+    a, _ = op.blend_coefficients(d)
+    torch.cuda.synchronize()
+    a = a.cpu().numpy()
+    out = du.cpu().numpy()
+    op.close()
+    ref, _, mag = oracle.rhs_blend(m, oracle.phys(bug_compat=bugs, **kw), u, a)
+    s = oracle.scaled_error(out, ref, mag)
+    print(f"[bug] blended: alpha>0 in {int((a > 0).sum())}/{a.size}; rel.err {relerr(out, ref):.3e}, "
+          f"entrywise {s:.2f} eps*mag", flush=True)
+    assert (a > 0).any() and s <= C_ROUND
+
+
+def test_bug_a1_a2_steps(device):
+    import torch
+
+    _say("bug_compat A1|A2 LSRK4(5) and SSP-RK3 steps, TGV 4^3 NS")
+    m, u, ph = _tgv_ns(4)
+    bugs = capi.BUG_A1 | capi.BUG_A2
+    oph = oracle.phys(bug_compat=bugs, gamma=ph["gamma"], Pr=ph["Pr"], mu=ph["mu"], viscous=True)
+    for scheme, nsteps in ((capi.LSRK45, 3), (capi.SSPRK3, 3)):
+        op = capi.Operator(m, bug_compat=bugs, **ph)
+        d = torch.from_numpy(u.copy()).to(device)
+        op.rk_run(scheme, d, 0.0, 1e-3, nsteps)
+        torch.cuda.synchronize()
+        out = d.cpu().numpy()
+        op.close()
+        ref = oracle.rk_run(m, oph, u, scheme, nsteps, 1e-3)
+        print(f"[bug] scheme {scheme}, {nsteps} steps: rel.err {relerr(out, ref):.3e}", flush=True)
+        assert relerr(out, ref) <= TOL
+    # the adjoint differentiates the intended operator only
+    op = capi.Operator(m, bug_compat=bugs, **ph)
+    d = torch.from_numpy(u).to(device)
+    with pytest.raises(capi.PrandtlError, match="UNSUPPORTED"):
+        op.vjp(d, torch.ones_like(d), torch.empty_like(d))
+    op.close()
+
+
 @pytest.mark.parametrize("scheme", [capi.SSPRK3, capi.LSRK45])
 @pytest.mark.parametrize("viscous", [False, True])
 def test_rk_steps(device, scheme, viscous):

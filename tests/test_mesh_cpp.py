@@ -13,7 +13,7 @@ from prandtl_amd import mesh as M
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-MESHES = os.path.join(HERE, "golden", "meshes")
+MESHES = os.path.join(os.path.dirname(HERE), "data", "meshes")
 
 
 @pytest.fixture(scope="module")

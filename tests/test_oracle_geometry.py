@@ -29,7 +29,7 @@ from . import meshgen as G
 from .numpy_ref import euler_flux_n
 
 EPS = np.finfo(np.float64).eps
-MESHES = os.path.join(os.path.dirname(__file__), "golden", "meshes")
+MESHES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data", "meshes")
 CAVITY = os.path.join(MESHES, "LidDrivenCavity.msh")
 STEP = os.path.join(MESHES, "ForwardFacingStep.msh")
 

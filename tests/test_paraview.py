@@ -13,7 +13,7 @@ from prandtl_amd import paraview as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CFG = os.path.join(HERE, "golden", "configs")
-MESHES = os.path.join(HERE, "golden", "meshes")
+MESHES = os.path.join(os.path.dirname(HERE), "data", "meshes")
 
 # VTK's linear cells (vtkQuad, vtkHexahedron): corners as lattice coordinates, the edges as corner
 # pairs (each edge runs from its first to its second corner) and the faces as corner quadruples
