@@ -31,7 +31,7 @@ EXPORTED = [
     "prandtl_op_profile", "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_last_error",
     "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
     "prandtl_device_alloc", "prandtl_device_free", "prandtl_memcpy", "prandtl_stream_synchronize",
-    "prandtl_blend_coefficients", "prandtl_rk_tape_id",
+    "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc",
 ]
 ALPHA_MAX, ALPHA_MIN = 0.5, 0.001   # the reference's blending defaults
 
@@ -103,6 +103,7 @@ def load():
         L.prandtl_rk_run.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.prandtl_op_profile.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, dp, vp]
         L.prandtl_op_vjp.argtypes = [vp, vp, vp, vp, vp]
+        L.prandtl_op_vjp_noalloc.argtypes = [vp, vp, vp, vp, vp]
         L.prandtl_rk_run_tape.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.prandtl_rk_adjoint.argtypes = [vp, vp, vp]
         L.prandtl_rk_tape_id.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
@@ -117,7 +118,7 @@ def load():
                   "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile",
                   "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_filter_constraints",
                   "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
-                  "prandtl_blend_coefficients", "prandtl_rk_tape_id"):
+                  "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib

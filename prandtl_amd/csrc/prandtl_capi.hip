@@ -7,6 +7,7 @@
 #include "dg_util.hpp"
 
 #include <atomic>
+#include <mutex>
 
 using namespace prandtl;
 
@@ -168,11 +169,51 @@ struct VjpSpec {
     const double *x = nullptr;   // VJP_SSP additive term
 };
 
-template <int DIM, int NP>
-int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
+// device work arrays of one operator-adjoint evaluation
+struct VjpBuffers {
+    double *q = nullptr;      // per-node quantities, (dim + 6) x nodes
+    double *lamw = nullptr;   // VJP_LSRK: the stage cotangent (state size; unused by VJP_PLAIN, VJP_SSP)
+    double *gf = nullptr;     // NS: lifting adjoint at face points, dim x 2 x neq x NE x Np^(dim-1)
+    double *wb = nullptr;     // NS: partial adjoint of the entropy variables (state size)
+    double *sb = nullptr;     // NS: partial adjoint of the state (state size)
+};
+
+// doubles of each array (gf, wb, sb only for Navier-Stokes)
+struct VjpSizes {
+    size_t q, lamw, gf, wb, sb;
+};
+
+VjpSizes vjp_sizes(const prandtl_op_s *op) {
     const size_t nodes = (size_t)op->NE * op->nn;
-    if (!op->d_vq) HIP_TRY(hipMalloc(&op->d_vq, (size_t)(DIM + 6) * nodes * sizeof(double)));
-    if (!op->d_vlam) HIP_TRY(hipMalloc(&op->d_vlam, op->size * sizeof(double)));
+    VjpSizes z{(size_t)(op->dim + 6) * nodes, op->size, 0, 0, 0};
+    if (op->phys.viscous) {
+        z.gf = (size_t)op->dim * 2 * op->neq * op->NE * (op->nn / op->np);
+        z.wb = z.sb = op->size;
+    }
+    return z;
+}
+
+// the handle's own work arrays, allocated on first use (the allocating entry points)
+int vjp_handle_buffers(prandtl_op_s *op, VjpBuffers &B) {
+    const VjpSizes z = vjp_sizes(op);
+    if (!op->d_vq) HIP_TRY(hipMalloc(&op->d_vq, z.q * sizeof(double)));
+    if (!op->d_vlam) HIP_TRY(hipMalloc(&op->d_vlam, z.lamw * sizeof(double)));
+    if (op->phys.viscous && !op->d_vgf) {
+        HIP_TRY(hipMalloc(&op->d_vgf, z.gf * sizeof(double)));
+        HIP_TRY(hipMalloc(&op->d_vwb, z.wb * sizeof(double)));
+        HIP_TRY(hipMalloc(&op->d_vsb, z.sb * sizeof(double)));
+    }
+    B.q = op->d_vq;
+    B.lamw = op->d_vlam;
+    B.gf = op->d_vgf;
+    B.wb = op->d_vwb;
+    B.sb = op->d_vsb;
+    return PRANDTL_OK;
+}
+
+template <int DIM, int NP>
+int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st, const VjpBuffers &B) {
+    const size_t nodes = (size_t)op->NE * op->nn;
     VjpArgs<NP> a;
     a.T = make_tables<NP>(op);
     a.P = op->P;
@@ -186,8 +227,8 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     a.x = S.x;
     a.ub = S.ub;
     a.kb = S.kb;
-    a.q = op->d_vq;
-    a.lamw = op->d_vlam;
+    a.q = B.q;
+    a.lamw = B.lamw;
     a.out = S.out;
     a.kb_out = S.kb_out;
     a.ca = S.ca;
@@ -199,11 +240,6 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     if (op->phys.viscous) {
         // the viscous part: recompute the lifting and reverse it (k_vjp_visc1) before the
         // inviscid adjoint writes the output, and complete it afterwards (k_vjp_visc2)
-        if (!op->d_vgf) {
-            HIP_TRY(hipMalloc(&op->d_vgf, (size_t)DIM * 2 * op->neq * op->NE * (op->nn / op->np) * sizeof(double)));
-            HIP_TRY(hipMalloc(&op->d_vwb, op->size * sizeof(double)));
-            HIP_TRY(hipMalloc(&op->d_vsb, op->size * sizeof(double)));
-        }
         va.T = a.T;
         va.P = op->P;
         va.NE = op->NE;
@@ -213,10 +249,10 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
         va.nbr = op->d_nbr;
         va.geom = op->d_geom;
         va.u = S.u;
-        va.lam = S.mode == VJP_LSRK ? op->d_vlam : S.lam;
-        va.Gf = op->d_vgf;
-        va.WB = op->d_vwb;
-        va.SB = op->d_vsb;
+        va.lam = S.mode == VJP_LSRK ? B.lamw : S.lam;
+        va.Gf = B.gf;
+        va.WB = B.wb;
+        va.SB = B.sb;
         va.out = S.out;
         using VC = LiftCfg<DIM, NP>;
         hipLaunchKernelGGL((k_vjp_visc1<DIM, NP>), dim3((op->NE + VC::EPB - 1) / VC::EPB), dim3(VC::BLOCK), 0, st, va);
@@ -233,7 +269,8 @@ int launch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     return PRANDTL_OK;
 }
 
-int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
+// the adjoint's requirements on the operator
+int vjp_supported(const prandtl_op_s *op) {
     if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
     if (op->nbc || op->d_ngeom) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic affine meshes");
     if (op->phys.viscous && op->phys.sutherland)
@@ -243,22 +280,91 @@ int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
     if (op->opts.precision != PRANDTL_FP64) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint is FP64");
     if (op->phys.viscous && (op->opts.bug_compat & (PRANDTL_BUG_A1 | PRANDTL_BUG_A2)))
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint differentiates the intended operator, not bug_compat A1/A2");
+    return PRANDTL_OK;
+}
+
+int dispatch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st, const VjpBuffers &B) {
     if (op->dim == 2) {
         switch (op->np) {
-            case 2: return launch_vjp<2, 2>(op, S, st);
-            case 3: return launch_vjp<2, 3>(op, S, st);
-            case 4: return launch_vjp<2, 4>(op, S, st);
-            case 5: return launch_vjp<2, 5>(op, S, st);
+            case 2: return launch_vjp<2, 2>(op, S, st, B);
+            case 3: return launch_vjp<2, 3>(op, S, st, B);
+            case 4: return launch_vjp<2, 4>(op, S, st, B);
+            case 5: return launch_vjp<2, 5>(op, S, st, B);
         }
     } else {
         switch (op->np) {
-            case 2: return launch_vjp<3, 2>(op, S, st);
-            case 3: return launch_vjp<3, 3>(op, S, st);
-            case 4: return launch_vjp<3, 4>(op, S, st);
-            case 5: return launch_vjp<3, 5>(op, S, st);
+            case 2: return launch_vjp<3, 2>(op, S, st, B);
+            case 3: return launch_vjp<3, 3>(op, S, st, B);
+            case 4: return launch_vjp<3, 4>(op, S, st, B);
+            case 5: return launch_vjp<3, 5>(op, S, st, B);
         }
     }
     return fail(PRANDTL_ERR_UNSUPPORTED, "order out of range");
+}
+
+int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
+    int rc = vjp_supported(op);
+    if (rc) return rc;
+    VjpBuffers B;
+    rc = vjp_handle_buffers(op, B);
+    if (rc) return rc;
+    return dispatch_vjp(op, S, st, B);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free operator adjoint (prandtl_op_vjp_noalloc): the work arrays live in one
+// file-scope device array of fixed size instead of handle memory allocated on first use.  Each
+// device holds its own copy at its own address, looked up once per device with
+// hipGetSymbolAddress; host code uses only that address (the array's name is a host-side
+// stand-in).  Every array is fully written before it is read, so nothing is cleared.
+// ---------------------------------------------------------------------------
+constexpr size_t kVjpScratchBytes = (size_t)64 << 20;   // 64 MiB per device
+static __device__ __align__(256) double scratch[kVjpScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void **out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PRANDTL_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PRANDTL_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PRANDTL_OK;
+}
+
+// the work arrays carved from the device's scratch array, each 256-byte aligned
+int vjp_scratch_buffers(const prandtl_op_s *op, VjpBuffers &B) {
+    const VjpSizes z = vjp_sizes(op);
+    auto up = [](size_t n) { return (n * sizeof(double) + 255) & ~(size_t)255; };
+    const size_t need = up(z.q) + up(z.gf) + up(z.wb) + up(z.sb);   // VJP_PLAIN reads no lamw
+    if (need > kVjpScratchBytes)
+        return fail(PRANDTL_ERR_ARG, "prandtl_op_vjp_noalloc: this operator needs " + std::to_string(need) +
+                                         " bytes of work space, above its fixed " +
+                                         std::to_string(kVjpScratchBytes) + " (use prandtl_op_vjp)");
+    void *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    char *p = static_cast<char *>(base);
+    B.q = reinterpret_cast<double *>(p);
+    p += up(z.q);
+    if (op->phys.viscous) {
+        B.gf = reinterpret_cast<double *>(p);
+        p += up(z.gf);
+        B.wb = reinterpret_cast<double *>(p);
+        p += up(z.wb);
+        B.sb = reinterpret_cast<double *>(p);
+    }
+    B.lamw = nullptr;
+    return PRANDTL_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -981,6 +1087,18 @@ int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *ou
     if (out == u || out == lam) return fail(PRANDTL_ERR_ARG, "out must not alias u or lam");
     VjpSpec S{VJP_PLAIN, u, lam, nullptr, nullptr, out, nullptr, 0.0, 0.0, 0.0};
     return run_vjp(op, S, (hipStream_t)stream);
+}
+
+int prandtl_op_vjp_noalloc(prandtl_op op, const double *u, const double *lam, double *out, prandtl_stream stream) {
+    if (!op || !u || !lam || !out) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (out == u || out == lam) return fail(PRANDTL_ERR_ARG, "out must not alias u or lam");
+    int rc = vjp_supported(op);
+    if (rc) return rc;
+    VjpBuffers B;
+    rc = vjp_scratch_buffers(op, B);
+    if (rc) return rc;
+    VjpSpec S{VJP_PLAIN, u, lam, nullptr, nullptr, out, nullptr, 0.0, 0.0, 0.0};
+    return dispatch_vjp(op, S, (hipStream_t)stream, B);
 }
 
 int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
