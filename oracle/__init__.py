@@ -16,8 +16,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _FMA_PATH = os.path.join(_HERE, "liboracle_fma.so")
+_LD_PATH = os.path.join(_HERE, "liboracle_ld.so")
 _lib = None
 _lib_fma = None
+_lib_ld = None
 
 SURF_CHANDRASHEKAR, SURF_EC, SURF_RUSANOV = 0, 1, 2
 BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
@@ -62,8 +64,8 @@ class OrcPhys(ctypes.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so and liboracle_fma.so with make (g++)."""
-    if force or not (os.path.exists(_LIB_PATH) and os.path.exists(_FMA_PATH)):
+    """Compile liboracle.so, liboracle_fma.so and liboracle_ld.so with make (g++)."""
+    if force or not all(os.path.exists(p) for p in (_LIB_PATH, _FMA_PATH, _LD_PATH)):
         args = ["make", "-s"] + (["-B"] if force else []) + ["all"]
         subprocess.run(args, cwd=_HERE, check=True)
     return _LIB_PATH
@@ -289,6 +291,32 @@ def rhs_mag(mesh, ph, u, want_grad=False):
     if want_grad:
         return dudt, lam.value, mag, grad, gmag
     return dudt, lam.value, mag
+
+
+def rhs_truth(mesh, ph, u):
+    """The operator in extended precision (liboracle_ld.so, long double throughout): (hi, lo, lambda)
+    with hi the value rounded to double and lo the rounding residual, so that the error of a double
+    result x is (x - hi) - lo.  Affine periodic meshes."""
+    global _lib_ld
+    if _lib_ld is None:
+        if not os.path.exists(_LD_PATH):
+            build()
+        L = ctypes.CDLL(_LD_PATH)
+        L.orc_ld_rhs.restype = ctypes.c_int
+        _lib_ld = L
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    hi = np.zeros_like(u)
+    lo = np.zeros_like(u)
+    lam = ctypes.c_double(0.0)
+    rc = _lib_ld.orc_ld_rhs(mesh.dim, mesh.order, mesh.num_elements, _ip(_lattice(mesh).neighbors), _dp(mesh.geom),
+                            ctypes.byref(ph), _dp(u), _dp(hi), _dp(lo), ctypes.byref(lam))
+    assert rc == 0, rc
+    return hi, lo, lam.value
+
+
+def truth_error(x, hi, lo) -> float:
+    """||x - truth||_inf / ||truth||_inf for a double result x and rhs_truth's (hi, lo)."""
+    return float(np.abs((x - hi) - lo).max() / np.abs(hi).max())
 
 
 def scaled_error(a, ref, mag) -> float:

@@ -33,11 +33,19 @@
 #include <vector>
 
 #include "prandtl_oracle.h"
+
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 
 namespace {
+
+// The arithmetic type: double (the oracle), or long double for the extended-precision build
+// that serves as the exact discrete operator (prandtl_oracle_ld.cpp).
+using real = ORACLE_REAL;
 
 constexpr int MAXNP = 10;
 constexpr int MAXEQ = 5;
@@ -47,58 +55,58 @@ constexpr int MAXEQ = 5;
 // ---------------------------------------------------------------------------
 struct Phys {
     int dim, neq;
-    double gamma, gammaM1, gammaM1Inverse, Pr, PrInverse, mu, R_gas;
-    double mu_bulk = 2.0 / 3.0;   // Physics.hpp:34
-    double mu0 = 1.716e-5;        // Physics.hpp:35
-    double T0 = 273.15;           // Physics.hpp:36
-    double Ts = 110.4;            // Physics.hpp:37
+    real gamma, gammaM1, gammaM1Inverse, Pr, PrInverse, mu, R_gas;
+    real mu_bulk = 2.0 / 3.0;   // Physics.hpp:34
+    real mu0 = 1.716e-5;        // Physics.hpp:35
+    real T0 = 273.15;           // Physics.hpp:36
+    real Ts = 110.4;            // Physics.hpp:37
     int viscous, sutherland, surface_flux, bug;
 };
 
 // ---------------------------------------------------------------------------
 // ComputeLogMean -- ref: src/BasicOperations/BasicOperations.cpp:6-12
 // ---------------------------------------------------------------------------
-inline double LogMean(double x, double y, double eps = 1e-4) {
-    double xi = y / x;
-    double u = (xi * (xi - 2.0) + 1.0) / (xi * (xi + 2.0) + 1);
-    double ln_mean = (u < eps) ? (x + y) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + 15.0 * u)))
+inline real LogMean(real x, real y, real eps = 1e-4) {
+    real xi = y / x;
+    real u = (xi * (xi - 2.0) + 1.0) / (xi * (xi + 2.0) + 1);
+    real ln_mean = (u < eps) ? (x + y) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + 15.0 * u)))
                                : (y - x) / std::log(xi);
     return ln_mean;
 }
 
 // ComputeSoundSpeed -- ref: BasicOperations.cpp:53-56 (b = 0)
-inline double SoundSpeed(double p, double rho, double gamma) {
+inline real SoundSpeed(real p, real rho, real gamma) {
     return std::sqrt(gamma * p / (1.0 - 0.0 * rho) / rho);
 }
 
 // ---------------------------------------------------------------------------
 // ChandrashekarFlux::ComputeVolumeFlux -- ref: src/RiemannSolvers/ChandrashekarFlux.cpp:11-83
 // ---------------------------------------------------------------------------
-double VolumeFlux(const Phys &P, const double *state1, const double *state2,
-                  const double *metric1, const double *metric2, double *F_tilde) {
+real VolumeFlux(const Phys &P, const real *state1, const real *state2,
+                  const real *metric1, const real *metric2, real *F_tilde) {
     const int dim = P.dim, neq = P.neq;
-    double metric[3], mom[3];
+    real metric[3], mom[3];
     for (int d = 0; d < dim; d++) metric[d] = (metric1[d] + metric2[d]) * 0.5;  // ComputeMean :15, BasicOperations.cpp:24-29
 
-    double rho1 = state1[0], rho2 = state2[0];
-    double rho_ln = LogMean(rho1, rho2);
-    double u1 = state1[1] / rho1, u2 = state2[1] / rho2;
-    double u_mean = 0.5 * (u1 + u2);
-    double V_sq1 = u1 * u1, V_sq2 = u2 * u2;
-    double Vn = u_mean * metric[0];
+    real rho1 = state1[0], rho2 = state2[0];
+    real rho_ln = LogMean(rho1, rho2);
+    real u1 = state1[1] / rho1, u2 = state2[1] / rho2;
+    real u_mean = 0.5 * (u1 + u2);
+    real V_sq1 = u1 * u1, V_sq2 = u2 * u2;
+    real Vn = u_mean * metric[0];
     mom[0] = rho_ln * u_mean;
-    double h_hat = -0.25 * (u1 * u1 + u2 * u2) + u_mean * u_mean;
+    real h_hat = -0.25 * (u1 * u1 + u2 * u2) + u_mean * u_mean;
     if (dim > 1) {
-        double v1 = state1[2] / rho1, v2 = state2[2] / rho2;
-        double v_mean = 0.5 * (v1 + v2);
+        real v1 = state1[2] / rho1, v2 = state2[2] / rho2;
+        real v_mean = 0.5 * (v1 + v2);
         V_sq1 += v1 * v1;
         V_sq2 += v2 * v2;
         Vn += v_mean * metric[1];
         mom[1] = rho_ln * v_mean;
         h_hat += -0.25 * (v1 * v1 + v2 * v2) + v_mean * v_mean;
         if (dim > 2) {
-            double w1 = state1[3] / rho1, w2 = state2[3] / rho2;
-            double w_mean = 0.5 * (w1 + w2);
+            real w1 = state1[3] / rho1, w2 = state2[3] / rho2;
+            real w_mean = 0.5 * (w1 + w2);
             V_sq1 += w1 * w1;
             V_sq2 += w2 * w2;
             Vn += w_mean * metric[2];
@@ -106,14 +114,14 @@ double VolumeFlux(const Phys &P, const double *state1, const double *state2,
             h_hat += -0.25 * (w1 * w1 + w2 * w2) + w_mean * w_mean;
         }
     }
-    double p1 = P.gammaM1 * (state1[neq - 1] - 0.5 * rho1 * V_sq1);   // :58
-    double p2 = P.gammaM1 * (state2[neq - 1] - 0.5 * rho2 * V_sq2);   // :59
-    double V_mag1 = std::sqrt(V_sq1), V_mag2 = std::sqrt(V_sq2);
-    double lambda_max = V_mag1 + SoundSpeed(p1, rho1, P.gamma);
+    real p1 = P.gammaM1 * (state1[neq - 1] - 0.5 * rho1 * V_sq1);   // :58
+    real p2 = P.gammaM1 * (state2[neq - 1] - 0.5 * rho2 * V_sq2);   // :59
+    real V_mag1 = std::sqrt(V_sq1), V_mag2 = std::sqrt(V_sq2);
+    real lambda_max = V_mag1 + SoundSpeed(p1, rho1, P.gamma);
     lambda_max = std::max(lambda_max, V_mag2 + SoundSpeed(p2, rho2, P.gamma));
-    double beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;           // :67-68
-    double beta_ln = LogMean(beta1, beta2);
-    double p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);              // :71
+    real beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;           // :67-68
+    real beta_ln = LogMean(beta1, beta2);
+    real p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);              // :71
     h_hat += 0.5 / beta_ln * P.gammaM1Inverse + p_hat / rho_ln;        // :73
     F_tilde[0] = rho_ln * Vn;
     for (int d = 0; d < dim; d++) F_tilde[1 + d] = Vn * mom[d] + p_hat * metric[d];
@@ -125,30 +133,30 @@ double VolumeFlux(const Phys &P, const double *state1, const double *state2,
 // ChandrashekarFlux::ComputeFaceFlux -- ref: ChandrashekarFlux.cpp:85-161
 // bug & 4 reproduces the 3D dissipation typo at :131 ("rho_mean*dw + w_mean").
 // ---------------------------------------------------------------------------
-double FaceFluxChandrashekar(const Phys &P, const double *state1, const double *state2,
-                             const double *nor, double *flux) {
+real FaceFluxChandrashekar(const Phys &P, const real *state1, const real *state2,
+                             const real *nor, real *flux) {
     const int dim = P.dim, neq = P.neq;
-    double mom[3];
-    double nor_mag = 0.0;
+    real mom[3];
+    real nor_mag = 0.0;
     for (int d = 0; d < dim; d++) nor_mag += nor[d] * nor[d];
     nor_mag = std::sqrt(nor_mag);                                      // :88 Norml2
 
-    double rho1 = state1[0], rho2 = state2[0];
-    double rho_mean = 0.5 * (rho1 + rho2);
-    double rho_ln = LogMean(rho1, rho2);
-    double drho = rho2 - rho1;
-    double u1 = state1[1] / rho1, u2 = state2[1] / rho2;
-    double u_mean = 0.5 * (u1 + u2);
-    double du = u2 - u1;
-    double V_sq1 = u1 * u1, V_sq2 = u2 * u2;
-    double Vn = u_mean * nor[0];
+    real rho1 = state1[0], rho2 = state2[0];
+    real rho_mean = 0.5 * (rho1 + rho2);
+    real rho_ln = LogMean(rho1, rho2);
+    real drho = rho2 - rho1;
+    real u1 = state1[1] / rho1, u2 = state2[1] / rho2;
+    real u_mean = 0.5 * (u1 + u2);
+    real du = u2 - u1;
+    real V_sq1 = u1 * u1, V_sq2 = u2 * u2;
+    real Vn = u_mean * nor[0];
     mom[0] = rho_ln * u_mean;
-    double h_hat = -0.25 * (u1 * u1 + u2 * u2) + u_mean * u_mean;
-    double diss = 0.5 * drho * u1 * u2 + rho_mean * du * u_mean;       // :105
+    real h_hat = -0.25 * (u1 * u1 + u2 * u2) + u_mean * u_mean;
+    real diss = 0.5 * drho * u1 * u2 + rho_mean * du * u_mean;       // :105
     if (dim > 1) {
-        double v1 = state1[2] / rho1, v2 = state2[2] / rho2;
-        double v_mean = 0.5 * (v1 + v2);
-        double dv = v2 - v1;
+        real v1 = state1[2] / rho1, v2 = state2[2] / rho2;
+        real v_mean = 0.5 * (v1 + v2);
+        real dv = v2 - v1;
         V_sq1 += v1 * v1;
         V_sq2 += v2 * v2;
         Vn += v_mean * nor[1];
@@ -156,9 +164,9 @@ double FaceFluxChandrashekar(const Phys &P, const double *state1, const double *
         h_hat += -0.25 * (v1 * v1 + v2 * v2) + v_mean * v_mean;
         diss += 0.5 * drho * v1 * v2 + rho_mean * dv * v_mean;        // :118
         if (dim > 2) {
-            double w1 = state1[3] / rho1, w2 = state2[3] / rho2;
-            double w_mean = 0.5 * (w1 + w2);
-            double dw = w2 - w1;
+            real w1 = state1[3] / rho1, w2 = state2[3] / rho2;
+            real w_mean = 0.5 * (w1 + w2);
+            real dw = w2 - w1;
             V_sq1 += w1 * w1;
             V_sq2 += w2 * w2;
             Vn += w_mean * nor[2];
@@ -170,14 +178,14 @@ double FaceFluxChandrashekar(const Phys &P, const double *state1, const double *
                 diss += 0.5 * drho * w1 * w2 + rho_mean * dw * w_mean;     // :131 intended
         }
     }
-    double p1 = P.gammaM1 * (state1[neq - 1] - 0.5 * rho1 * V_sq1);
-    double p2 = P.gammaM1 * (state2[neq - 1] - 0.5 * rho2 * V_sq2);
-    double V_mag1 = std::sqrt(V_sq1), V_mag2 = std::sqrt(V_sq2);
-    double lambda_max = V_mag1 + SoundSpeed(p1, rho1, P.gamma);
+    real p1 = P.gammaM1 * (state1[neq - 1] - 0.5 * rho1 * V_sq1);
+    real p2 = P.gammaM1 * (state2[neq - 1] - 0.5 * rho2 * V_sq2);
+    real V_mag1 = std::sqrt(V_sq1), V_mag2 = std::sqrt(V_sq2);
+    real lambda_max = V_mag1 + SoundSpeed(p1, rho1, P.gamma);
     lambda_max = std::max(lambda_max, V_mag2 + SoundSpeed(p2, rho2, P.gamma));
-    double beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;
-    double beta_ln = LogMean(beta1, beta2);
-    double p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);
+    real beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;
+    real beta_ln = LogMean(beta1, beta2);
+    real p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);
     h_hat += 0.5 / beta_ln * P.gammaM1Inverse + p_hat / rho_ln;
     diss += 0.5 * drho * P.gammaM1Inverse / beta_ln +
             0.5 * rho_mean * P.gammaM1Inverse * (1.0 / beta2 - 1.0 / beta1);   // :151
@@ -192,15 +200,15 @@ double FaceFluxChandrashekar(const Phys &P, const double *state1, const double *
 // consistency tests).  The reference gets it from mfem::EulerFlux
 // (NavierStokesFlux : EulerFlux, src/Fluxes/NavierStokesFlux.hpp:10); the formula is
 // the textbook one.
-double EulerFluxDotN(const Phys &P, const double *s, const double *nor, double *fn) {
+real EulerFluxDotN(const Phys &P, const real *s, const real *nor, real *fn) {
     const int dim = P.dim, neq = P.neq;
-    double rho = s[0], vel[3], vsq = 0.0, vn = 0.0;
+    real rho = s[0], vel[3], vsq = 0.0, vn = 0.0;
     for (int d = 0; d < dim; d++) {
         vel[d] = s[1 + d] / rho;
         vsq += vel[d] * vel[d];
         vn += vel[d] * nor[d];
     }
-    double p = P.gammaM1 * (s[neq - 1] - 0.5 * rho * vsq);
+    real p = P.gammaM1 * (s[neq - 1] - 0.5 * rho * vsq);
     fn[0] = rho * vn;
     for (int d = 0; d < dim; d++) fn[1 + d] = s[1 + d] * vn + p * nor[d];
     fn[neq - 1] = (s[neq - 1] + p) * vn;
@@ -211,14 +219,14 @@ double EulerFluxDotN(const Phys &P, const double *s, const double *nor, double *
 // kind 1 = the entropy-conservative flux alone (ComputeVolumeFlux with metric = nor);
 // kind 2 = Rusanov / local Lax-Friedrichs on the central flux:
 //   f* = 1/2 (F(u1) + F(u2)).n - 1/2 lambda_max |n| (u2 - u1).
-double FaceFlux(const Phys &P, const double *s1, const double *s2, const double *nor, double *flux) {
+real FaceFlux(const Phys &P, const real *s1, const real *s2, const real *nor, real *flux) {
     if (P.surface_flux == 0) return FaceFluxChandrashekar(P, s1, s2, nor, flux);
     if (P.surface_flux == 1) return VolumeFlux(P, s1, s2, nor, nor, flux);
-    double f1[MAXEQ], f2[MAXEQ];
-    double l1 = EulerFluxDotN(P, s1, nor, f1);
-    double l2 = EulerFluxDotN(P, s2, nor, f2);
-    double lam = std::max(l1, l2);
-    double nm = 0.0;
+    real f1[MAXEQ], f2[MAXEQ];
+    real l1 = EulerFluxDotN(P, s1, nor, f1);
+    real l2 = EulerFluxDotN(P, s2, nor, f2);
+    real lam = std::max(l1, l2);
+    real nm = 0.0;
     for (int d = 0; d < P.dim; d++) nm += nor[d] * nor[d];
     nm = std::sqrt(nm);
     for (int v = 0; v < P.neq; v++) flux[v] = 0.5 * (f1[v] + f2[v]) - 0.5 * lam * nm * (s2[v] - s1[v]);
@@ -228,46 +236,46 @@ double FaceFlux(const Phys &P, const double *s1, const double *s2, const double 
 // ---------------------------------------------------------------------------
 // Conserv2Entropy -- ref: BasicOperations.cpp:81-104
 // ---------------------------------------------------------------------------
-void Conserv2Entropy(const Phys &P, const double *state, double *ent) {
+void Conserv2Entropy(const Phys &P, const real *state, real *ent) {
     const int neq = P.neq, dim = P.dim;
-    double vel[3], v_sq = 0.0;
+    real vel[3], v_sq = 0.0;
     for (int d = 0; d < dim; d++) vel[d] = state[1 + d] / state[0];
     for (int d = 0; d < dim; d++) v_sq += vel[d] * vel[d];
     v_sq = 0.5 * v_sq;
-    double p = P.gammaM1 * (state[neq - 1] - state[0] * v_sq);
-    double beta = state[0] / p;
-    double s = std::log(p * std::pow(state[0], -P.gamma));
+    real p = P.gammaM1 * (state[neq - 1] - state[0] * v_sq);
+    real beta = state[0] / p;
+    real s = std::log(p * std::pow(state[0], -P.gamma));
     ent[0] = (P.gamma - s) * P.gammaM1Inverse - beta * v_sq;
     for (int d = 0; d < dim; d++) ent[1 + d] = beta * vel[d];
     ent[neq - 1] = -beta;
 }
 
 // Entropy2Conserv -- ref: BasicOperations.cpp:106-124
-void Entropy2Conserv(const Phys &P, const double *ent, double *state) {
+void Entropy2Conserv(const Phys &P, const real *ent, real *state) {
     const int dim = P.dim;
-    const double beta = -ent[dim + 1];
-    double vel[3], vv = 0.0;
+    const real beta = -ent[dim + 1];
+    real vel[3], vv = 0.0;
     for (int d = 0; d < dim; d++) vel[d] = ent[1 + d] / beta;
     for (int d = 0; d < dim; d++) vv += vel[d] * vel[d];
-    const double s = P.gamma - (ent[0] + 0.5 * beta * vv) * P.gammaM1;
+    const real s = P.gamma - (ent[0] + 0.5 * beta * vv) * P.gammaM1;
     state[0] = std::pow(std::exp(-s) / beta, P.gammaM1Inverse);
     for (int d = 0; d < dim; d++) state[1 + d] = state[0] * vel[d];
     state[dim + 1] = state[0] * (1.0 / beta * P.gammaM1Inverse + 0.5 * vv);
 }
 
 // EntropyGrad2PrimGrad (one node, one direction, in place) -- ref: BasicOperations.cpp:126-170
-void EntropyGrad2PrimGrad(const Phys &P, const double *state, double *g) {
+void EntropyGrad2PrimGrad(const Phys &P, const real *state, real *g) {
     const int dim = P.dim, neq = P.neq;
-    double vel[3], v_sq = 0.0;
+    real vel[3], v_sq = 0.0;
     for (int d = 0; d < dim; d++) vel[d] = state[1 + d] / state[0];
     for (int d = 0; d < dim; d++) v_sq += vel[d] * vel[d];
     v_sq = 0.5 * v_sq;
-    double p = P.gammaM1 * (state[neq - 1] - state[0] * v_sq);
-    double KE = v_sq * state[0];
-    double gs[MAXEQ];
+    real p = P.gammaM1 * (state[neq - 1] - state[0] * v_sq);
+    real KE = v_sq * state[0];
+    real gs[MAXEQ];
     for (int v = 0; v < neq; v++) gs[v] = g[v];
-    const double ge = gs[neq - 1];
-    double mdot = 0.0;
+    const real ge = gs[neq - 1];
+    real mdot = 0.0;
     for (int d = 0; d < dim; d++) {
         g[1 + d] = p / state[0] * (gs[1 + d] + vel[d] * ge);
     }
@@ -281,33 +289,33 @@ void EntropyGrad2PrimGrad(const Phys &P, const double *state, double *g) {
 // :60-90 (2D).  Sutherland: NavierStokesFlux.hpp:32-36.  flux is neq x dim,
 // row-major (flux[v*dim + d]); row 0 stays zero (DGSEMIntegrator.cpp:106).
 // ---------------------------------------------------------------------------
-void ViscousFlux(const Phys &P, const double *state, const double *dqdx, const double *dqdy,
-                 const double *dqdz, double *flux) {
+void ViscousFlux(const Phys &P, const real *state, const real *dqdx, const real *dqdy,
+                 const real *dqdz, real *flux) {
     const int dim = P.dim, neq = P.neq;
-    double prim[MAXEQ];                                         // Conserv2Prim BasicOperations.cpp:172-186
+    real prim[MAXEQ];                                         // Conserv2Prim BasicOperations.cpp:172-186
     prim[0] = state[0];
     for (int d = 0; d < dim; d++) prim[1 + d] = state[1 + d] / state[0];
     {
-        double vv = 0.0;
+        real vv = 0.0;
         for (int d = 0; d < dim; d++) vv += prim[1 + d] * prim[1 + d];
         prim[neq - 1] = P.gammaM1 * (state[neq - 1] - 0.5 * state[0] * vv);  // ComputePressure :31-37
     }
-    double mu = P.mu;
+    real mu = P.mu;
     if (P.sutherland) {
-        double T = prim[neq - 1] / (prim[0] * P.R_gas);
+        real T = prim[neq - 1] / (prim[0] * P.R_gas);
         mu = P.mu0 * (P.T0 + P.Ts) / (T + P.Ts) * (T / P.T0) * std::sqrt(T / P.T0);
     }
-    const double lambda = mu * P.gamma * P.PrInverse;
+    const real lambda = mu * P.gamma * P.PrInverse;
     for (int i = 0; i < neq * dim; i++) flux[i] = 0.0;
-    auto F = [&](int r, int c) -> double & { return flux[r * dim + c]; };
+    auto F = [&](int r, int c) -> real & { return flux[r * dim + c]; };
     if (dim == 3) {
-        const double drdx = dqdx[0], dudx = dqdx[1], dvdx = dqdx[2], dwdx = dqdx[3], dpdx = dqdx[4];
-        const double drdy = dqdy[0], dudy = dqdy[1], dvdy = dqdy[2], dwdy = dqdy[3], dpdy = dqdy[4];
-        const double drdz = dqdz[0], dudz = dqdz[1], dvdz = dqdz[2], dwdz = dqdz[3], dpdz = dqdz[4];
-        double div = dudx + dvdy + dwdz;
-        double cv_dTdx = P.gammaM1Inverse / prim[0] * (dpdx - prim[4] / prim[0] * drdx);
-        double cv_dTdy = P.gammaM1Inverse / prim[0] * (dpdy - prim[4] / prim[0] * drdy);
-        double cv_dTdz = P.gammaM1Inverse / prim[0] * (dpdz - prim[4] / prim[0] * drdz);
+        const real drdx = dqdx[0], dudx = dqdx[1], dvdx = dqdx[2], dwdx = dqdx[3], dpdx = dqdx[4];
+        const real drdy = dqdy[0], dudy = dqdy[1], dvdy = dqdy[2], dwdy = dqdy[3], dpdy = dqdy[4];
+        const real drdz = dqdz[0], dudz = dqdz[1], dvdz = dqdz[2], dwdz = dqdz[3], dpdz = dqdz[4];
+        real div = dudx + dvdy + dwdz;
+        real cv_dTdx = P.gammaM1Inverse / prim[0] * (dpdx - prim[4] / prim[0] * drdx);
+        real cv_dTdy = P.gammaM1Inverse / prim[0] * (dpdy - prim[4] / prim[0] * drdy);
+        real cv_dTdz = P.gammaM1Inverse / prim[0] * (dpdz - prim[4] / prim[0] * drdz);
         F(1, 0) = mu * (2.0 * dudx - P.mu_bulk * div);
         F(2, 0) = mu * (dudy + dvdx);
         F(3, 0) = mu * (dudz + dwdx);
@@ -321,11 +329,11 @@ void ViscousFlux(const Phys &P, const double *state, const double *dqdx, const d
         F(3, 2) = mu * (2.0 * dwdz - P.mu_bulk * div);
         F(4, 2) = prim[1] * F(1, 2) + prim[2] * F(2, 2) + prim[3] * F(3, 2) + lambda * cv_dTdz;
     } else {
-        const double drdx = dqdx[0], dudx = dqdx[1], dvdx = dqdx[2], dpdx = dqdx[3];
-        const double drdy = dqdy[0], dudy = dqdy[1], dvdy = dqdy[2], dpdy = dqdy[3];
-        double div = dudx + dvdy;
-        double cv_dTdx = P.gammaM1Inverse / prim[0] * (dpdx - prim[3] / prim[0] * drdx);
-        double cv_dTdy = P.gammaM1Inverse / prim[0] * (dpdy - prim[3] / prim[0] * drdy);
+        const real drdx = dqdx[0], dudx = dqdx[1], dvdx = dqdx[2], dpdx = dqdx[3];
+        const real drdy = dqdy[0], dudy = dqdy[1], dvdy = dqdy[2], dpdy = dqdy[3];
+        real div = dudx + dvdy;
+        real cv_dTdx = P.gammaM1Inverse / prim[0] * (dpdx - prim[3] / prim[0] * drdx);
+        real cv_dTdy = P.gammaM1Inverse / prim[0] * (dpdy - prim[3] / prim[0] * drdy);
         F(1, 0) = mu * (2.0 * dudx - P.mu_bulk * div);
         F(2, 0) = mu * (dudy + dvdx);
         F(3, 0) = prim[1] * F(1, 0) + prim[2] * F(2, 0) + lambda * cv_dTdx;
@@ -337,35 +345,35 @@ void ViscousFlux(const Phys &P, const double *state, const double *dqdx, const d
 
 // Rounding scale of ViscousFlux: the same formula with every term in absolute value and
 // the gradients replaced by their rounding scales (gmag >= |grad| entrywise).
-void ViscousFluxMag(const Phys &P, const double *state, const double *ax, const double *ay, const double *az,
-                    double *flux) {
+void ViscousFluxMag(const Phys &P, const real *state, const real *ax, const real *ay, const real *az,
+                    real *flux) {
     const int dim = P.dim, neq = P.neq;
-    const double *g[3] = {ax, ay, az};
-    const double rho = state[0];
-    double vel[3], vsq = 0.0;
+    const real *g[3] = {ax, ay, az};
+    const real rho = state[0];
+    real vel[3], vsq = 0.0;
     for (int d = 0; d < dim; d++) {
         vel[d] = state[1 + d] / rho;
         vsq += vel[d] * vel[d];
     }
-    const double p = P.gammaM1 * (state[neq - 1] - 0.5 * rho * vsq);
-    double mu = P.mu;
+    const real p = P.gammaM1 * (state[neq - 1] - 0.5 * rho * vsq);
+    real mu = P.mu;
     if (P.sutherland) {
-        double T = p / (rho * P.R_gas);
+        real T = p / (rho * P.R_gas);
         mu = P.mu0 * (P.T0 + P.Ts) / (T + P.Ts) * (T / P.T0) * std::sqrt(T / P.T0);
     }
     mu = std::fabs(mu);
-    const double lambda = mu * P.gamma * P.PrInverse;
-    double div = 0.0;
+    const real lambda = mu * P.gamma * P.PrInverse;
+    real div = 0.0;
     for (int d = 0; d < dim; d++) div += g[d][1 + d];
     for (int i = 0; i < neq * dim; i++) flux[i] = 0.0;
     for (int c = 0; c < dim; c++) {
-        double e = 0.0;
+        real e = 0.0;
         for (int r = 0; r < dim; r++) {
-            const double t = (r == c) ? mu * (2.0 * g[c][1 + r] + P.mu_bulk * div) : mu * (g[c][1 + r] + g[r][1 + c]);
+            const real t = (r == c) ? mu * (2.0 * g[c][1 + r] + P.mu_bulk * div) : mu * (g[c][1 + r] + g[r][1 + c]);
             flux[(1 + r) * dim + c] = t;
             e += std::fabs(vel[r]) * t;
         }
-        const double cv = P.gammaM1Inverse / rho * (g[c][neq - 1] + std::fabs(p / rho) * g[c][0]);
+        const real cv = P.gammaM1Inverse / rho * (g[c][neq - 1] + std::fabs(p / rho) * g[c][0]);
         flux[(neq - 1) * dim + c] = e + lambda * cv;
     }
 }
@@ -387,7 +395,7 @@ void LegendreAndDeriv(int N, long double z, long double &P, long double &dP) {
     dP = N * (z * p1 - p0) / (z * z - 1.0L);  // valid for |z| < 1
 }
 
-void GLL(int np, double *x, double *w) {
+void GLL(int np, real *x, real *w) {
     const int N = np - 1;
     std::vector<long double> z(np);
     z[0] = -1.0L;
@@ -403,7 +411,7 @@ void GLL(int np, double *x, double *w) {
             long double d2P = (2.0L * zz * dP - N * (N + 1.0L) * P) / (1.0L - zz * zz);
             long double dz = dP / d2P;
             zz -= dz;
-            if (std::fabs((double)dz) < 1e-30) break;
+            if (std::fabs((real)dz) < 1e-30) break;
         }
         z[j] = zz;
     }
@@ -415,8 +423,8 @@ void GLL(int np, double *x, double *w) {
             LegendreAndDeriv(N, z[j], P, dP);
         }
         long double wj = 1.0L / (N * (N + 1.0L) * P * P);  // on [0,1]: half of 2/(N(N+1)P^2)
-        x[j] = (double)(0.5L * (z[j] + 1.0L));
-        w[j] = (double)wj;
+        x[j] = (real)(0.5L * (z[j] + 1.0L));
+        w[j] = (real)wj;
     }
     x[0] = 0.0;
     x[N] = 1.0;
@@ -424,8 +432,8 @@ void GLL(int np, double *x, double *w) {
 
 // D, Dhat, Dhat2 -- ref: DGSEMIntegrator.cpp:41-82.  Stored as the mathematical
 // matrices (D[i*np+j] = D_ij), i.e. before the reference's transposes.
-void DMatrices(int np, const double *x, const double *w, double *D, double *Dhat, double *Dhat2) {
-    double wBary[MAXNP];
+void DMatrices(int np, const real *x, const real *w, real *D, real *Dhat, real *Dhat2) {
+    real wBary[MAXNP];
     for (int i = 0; i < np; i++) wBary[i] = 1.0;
     for (int i = 1; i < np; i++)
         for (int j = 0; j < i; j++) {
@@ -470,15 +478,15 @@ enum BcType {
 };
 
 // ComputePressure -- BasicOperations.cpp:31-37 (V = m/rho as Vector /= rho, i.e. times 1/rho)
-double PressureOf(const Phys &P, const double *s) {
-    const double inv = 1.0 / s[0];
-    double vv = 0.0;
+real PressureOf(const Phys &P, const real *s) {
+    const real inv = 1.0 / s[0];
+    real vv = 0.0;
     for (int d = 0; d < P.dim; d++) vv += (s[1 + d] * inv) * (s[1 + d] * inv);
     return P.gammaM1 * (s[P.neq - 1] - 0.5 * s[0] * vv);
 }
 
-void NormalizeVec(int dim, double *v) {   // Normalize, BasicOperations.hpp:39-42
-    double n = 0.0;
+void NormalizeVec(int dim, real *v) {   // Normalize, BasicOperations.hpp:39-42
+    real n = 0.0;
     for (int d = 0; d < dim; d++) n += v[d] * v[d];
     n = std::sqrt(n);
     for (int d = 0; d < dim; d++) v[d] /= n;
@@ -486,25 +494,25 @@ void NormalizeVec(int dim, double *v) {   // Normalize, BasicOperations.hpp:39-4
 
 // Normal (BasicOperations.cpp:221-235), Cross (:211-219), RotateState (:238-259): the momentum
 // in the frame (n, t1[, n x t1]).
-void RotateStateRef(int dim, double *state, const double *n) {
-    double t1[3];
+void RotateStateRef(int dim, real *state, const real *n) {
+    real t1[3];
     if (dim == 2) {
         t1[0] = -n[1];
         t1[1] = n[0];
     } else {
-        const double r = n[1] / n[0];
+        const real r = n[1] / n[0];
         t1[0] = std::abs(n[0]) > 1e-10 ? -r / std::sqrt(1.0 + std::pow(r, 2)) : -1.0;
         t1[1] = std::abs(n[0]) > 1e-10 ? 1.0 / std::sqrt(1.0 + std::pow(r, 2)) : 0.0;
         t1[2] = 0.0;
     }
-    double *m = state + 1;
-    double mu = 0.0, mv = 0.0;
+    real *m = state + 1;
+    real mu = 0.0, mv = 0.0;
     for (int d = 0; d < dim; d++) {
         mu += m[d] * n[d];
         mv += m[d] * t1[d];
     }
     if (dim == 3) {
-        const double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+        const real t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
         m[2] = m[0] * t2[0] + m[1] * t2[1] + m[2] * t2[2];
     }
     m[0] = mu;
@@ -512,7 +520,7 @@ void RotateStateRef(int dim, double *state, const double *n) {
 }
 
 // ComputeOuterInviscidState of the BCs that define an outer state.
-void OuterState(const Phys &P, const orc_bc &bc, const double *s1, const double *nor, double *s2) {
+void OuterState(const Phys &P, const orc_bc &bc, const real *s1, const real *nor, real *s2) {
     const int dim = P.dim, neq = P.neq;
     switch (bc.type) {
         case BC_SUPERSONIC_INFLOW:    // SupersonicInflowBdrFaceIntegrator.cpp:34-48
@@ -523,41 +531,41 @@ void OuterState(const Phys &P, const orc_bc &bc, const double *s1, const double 
             for (int v = 0; v < neq; v++) s2[v] = s1[v];
             break;
         case BC_RIEMANN_INVARIANT: {  // RiemannInvariantBdrFaceIntegrator.cpp:44-117
-            double un[3], Vo[3], Vi[3];
+            real un[3], Vo[3], Vi[3];
             for (int d = 0; d < dim; d++) un[d] = nor[d];
             NormalizeVec(dim, un);
-            const double rho_o = bc.q[0], p_o = bc.q[neq - 1];
+            const real rho_o = bc.q[0], p_o = bc.q[neq - 1];
             for (int d = 0; d < dim; d++) Vo[d] = bc.q[1 + d];
-            const double inv = 1.0 / s1[0];
-            double Vn_i = 0.0, Vn_o = 0.0;
+            const real inv = 1.0 / s1[0];
+            real Vn_i = 0.0, Vn_o = 0.0;
             for (int d = 0; d < dim; d++) {
                 Vi[d] = s1[1 + d] * inv;
                 Vn_i += Vi[d] * un[d];
                 Vn_o += Vo[d] * un[d];
             }
-            const double p_i = PressureOf(P, s1);
-            const double a_o = SoundSpeed(p_o, rho_o, P.gamma), a_i = SoundSpeed(p_i, s1[0], P.gamma);
-            const double gI = P.gammaM1Inverse;
-            const double R_minus = (std::abs(Vn_o) >= a_o && Vn_i >= 0.0) ? Vn_i - 2.0 * a_i * gI : Vn_o - 2.0 * a_o * gI;
-            const double R_plus = (std::abs(Vn_o) >= a_o && Vn_i < 0.0) ? Vn_o + 2.0 * a_o * gI : Vn_i + 2.0 * a_i * gI;
-            const double Vn_b = 0.5 * (R_plus + R_minus);
-            const double a_b = 0.25 * P.gammaM1 * (R_plus - R_minus);
-            const double dVn_i = Vn_b - Vn_i, dVn_o = Vn_b - Vn_o;
-            const double gInv = 1.0 / P.gamma;
+            const real p_i = PressureOf(P, s1);
+            const real a_o = SoundSpeed(p_o, rho_o, P.gamma), a_i = SoundSpeed(p_i, s1[0], P.gamma);
+            const real gI = P.gammaM1Inverse;
+            const real R_minus = (std::abs(Vn_o) >= a_o && Vn_i >= 0.0) ? Vn_i - 2.0 * a_i * gI : Vn_o - 2.0 * a_o * gI;
+            const real R_plus = (std::abs(Vn_o) >= a_o && Vn_i < 0.0) ? Vn_o + 2.0 * a_o * gI : Vn_i + 2.0 * a_i * gI;
+            const real Vn_b = 0.5 * (R_plus + R_minus);
+            const real a_b = 0.25 * P.gammaM1 * (R_plus - R_minus);
+            const real dVn_i = Vn_b - Vn_i, dVn_o = Vn_b - Vn_o;
+            const real gInv = 1.0 / P.gamma;
             s2[0] = (Vn_i < 0.0) ? std::pow(a_b * a_b * gInv * rho_o / p_o, gI) * rho_o
                                  : std::pow(a_b * a_b * gInv * s1[0] / p_i, gI) * s1[0];
-            double e = 0.0;
+            real e = 0.0;
             for (int d = 0; d < dim; d++) {
                 s2[1 + d] = (Vn_i >= 0.0) ? s2[0] * (Vi[d] + dVn_i * un[d]) : s2[0] * (Vo[d] + dVn_o * un[d]);
                 e += s2[1 + d] * s2[1 + d];
             }
-            const double p_b = a_b * a_b * s2[0] * gInv;
+            const real p_b = a_b * a_b * s2[0] * gInv;
             s2[neq - 1] = e * (0.5 / s2[0]) + p_b * gI;
             break;
         }
         case BC_SUBSONIC_INFLOW_RV: { // SubsonicInflowRVBdrFaceIntegrator.cpp:22-52
-            const double r = bc.q[0];
-            double dke = 0.0, uu = 0.0;
+            const real r = bc.q[0];
+            real dke = 0.0, uu = 0.0;
             s2[0] = r;
             s2[neq - 1] = s1[neq - 1];
             for (int d = 0; d < dim; d++) {
@@ -571,8 +579,8 @@ void OuterState(const Phys &P, const orc_bc &bc, const double *s1, const double 
             break;
         }
         case BC_SUBSONIC_INFLOW_PTTT: {  // SubsonicInflowPtTtAngBdrFaceIntegrator.cpp:48-73
-            const double pt = bc.q[0], Tt = bc.q[1], theta = bc.q[2], phi = bc.q[3], cp = bc.q[4];
-            double Vc[3] = {std::cos(theta * M_PI / 180.0), std::sin(theta * M_PI / 180.0), 0.0};
+            const real pt = bc.q[0], Tt = bc.q[1], theta = bc.q[2], phi = bc.q[3], cp = bc.q[4];
+            real Vc[3] = {std::cos(theta * M_PI / 180.0), std::sin(theta * M_PI / 180.0), 0.0};
             if (dim > 2) {
                 // the reference writes cos(phi) to V_comps(3), outside the 3-vector (:19, :39); the
                 // intended z component is V_comps(2)
@@ -580,17 +588,17 @@ void OuterState(const Phys &P, const orc_bc &bc, const double *s1, const double 
                 Vc[1] *= std::sin(phi * M_PI / 180.0);
                 Vc[2] = std::cos(phi * M_PI / 180.0);
             }
-            const double p = PressureOf(P, s1);
-            double V_sq = 2.0 * cp * Tt * (1.0 - std::pow(p / pt, P.gammaM1 / P.gamma));
-            V_sq = std::max(0.0, V_sq);
+            const real p = PressureOf(P, s1);
+            real V_sq = 2.0 * cp * Tt * (1.0 - std::pow(p / pt, P.gammaM1 / P.gamma));
+            V_sq = std::max<real>(0.0, V_sq);
             s2[0] = (P.gamma * P.gammaM1Inverse) * p / (cp * Tt - 0.5 * V_sq);
             for (int d = 0; d < dim; d++) s2[1 + d] = s2[0] * std::sqrt(V_sq) * Vc[d];
             s2[neq - 1] = p * P.gammaM1Inverse + 0.5 * s2[0] * V_sq;
             break;
         }
         case BC_SUBSONIC_OUTFLOW_P: {  // SubsonicOutflowPBdrFaceIntegrator.cpp:17-28
-            const double inv = 1.0 / s1[0];
-            double vv = 0.0;
+            const real inv = 1.0 / s1[0];
+            real vv = 0.0;
             for (int d = 0; d < dim; d++) vv += (s1[1 + d] * inv) * (s1[1 + d] * inv);
             for (int v = 0; v < neq; v++) s2[v] = s1[v];
             s2[neq - 1] = bc.q[0] * P.gammaM1Inverse + 0.5 * s1[0] * vv;
@@ -604,38 +612,38 @@ void OuterState(const Phys &P, const orc_bc &bc, const double *s1, const double 
 // ComputeBdrFaceInviscidFlux: the slip wall's (SlipWallBdrFaceIntegrator.cpp:15-57, inherited by
 // the no-slip walls), the symmetry BC's (SymmetryBdrFaceIntegrator.cpp:10-32), otherwise the
 // surface flux with the outer state (BdrFaceIntegrator.cpp:245-253).
-double BdrInviscidFlux(const Phys &P, const orc_bc &bc, const double *s1, const double *nor, double *flux) {
+real BdrInviscidFlux(const Phys &P, const orc_bc &bc, const real *s1, const real *nor, real *flux) {
     const int dim = P.dim, neq = P.neq;
-    double s2[MAXEQ];
+    real s2[MAXEQ];
     if (bc.type == BC_SLIP_WALL || bc.type == BC_NOSLIP_ADIABATIC || bc.type == BC_NOSLIP_ISOTHERMAL) {
-        double un[3];
+        real un[3];
         for (int d = 0; d < dim; d++) un[d] = nor[d];
         NormalizeVec(dim, un);
         for (int v = 0; v < neq; v++) s2[v] = s1[v];
         RotateStateRef(dim, s2, un);
         // Conserv2Prim (BasicOperations.cpp:172-186)
-        double prim[MAXEQ];
+        real prim[MAXEQ];
         prim[0] = s2[0];
         for (int d = 0; d < dim; d++) prim[1 + d] = s2[1 + d] / s2[0];
         prim[neq - 1] = PressureOf(P, s2);
-        double vv = 0.0;
+        real vv = 0.0;
         for (int d = 0; d < dim; d++) vv += prim[1 + d] * prim[1 + d];
-        const double an = SoundSpeed(prim[neq - 1], prim[0], P.gamma);
-        const double gP1 = 1.0 + P.gamma, gP1I = 1.0 / gP1;
-        double p_star;
+        const real an = SoundSpeed(prim[neq - 1], prim[0], P.gamma);
+        const real gP1 = 1.0 + P.gamma, gP1I = 1.0 / gP1;
+        real p_star;
         if (prim[1] > 0.0)
             p_star = prim[neq - 1] + 0.25 * prim[1] * gP1 * prim[0] *
                                          (prim[1] + std::sqrt(std::pow(prim[1], 2) +
                                                               8.0 * gP1I / prim[0] * prim[neq - 1] * (P.gammaM1 * gP1I + 1.0)));
         else
             p_star = prim[neq - 1] *
-                     std::pow(std::max(1.0 + 0.5 * P.gammaM1 * prim[1] / an, 0.0001), 2.0 * P.gamma * P.gammaM1Inverse);
+                     std::pow(std::max<real>(1.0 + 0.5 * P.gammaM1 * prim[1] / an, 0.0001), 2.0 * P.gamma * P.gammaM1Inverse);
         for (int v = 0; v < neq; v++) flux[v] = 0.0;
         for (int d = 0; d < dim; d++) flux[1 + d] = p_star * nor[d];
         return std::sqrt(vv) + an;
     }
     if (bc.type == BC_SYMMETRY) {
-        double un[3], mn = 0.0;
+        real un[3], mn = 0.0;
         for (int d = 0; d < dim; d++) un[d] = nor[d];
         NormalizeVec(dim, un);
         for (int v = 0; v < neq; v++) s2[v] = s1[v];
@@ -648,12 +656,12 @@ double BdrInviscidFlux(const Phys &P, const orc_bc &bc, const double *s1, const 
 }
 
 // ComputeBdrFaceViscousFlux: F_v . nor (row 0 zero); dq[c] the element's primitive gradients.
-void BdrViscousFlux(const Phys &P, const orc_bc &bc, const double *s1, const double (*dq)[MAXEQ], const double *nor,
-                    double *fn) {
+void BdrViscousFlux(const Phys &P, const orc_bc &bc, const real *s1, const real (*dq)[MAXEQ], const real *nor,
+                    real *fn) {
     const int dim = P.dim, neq = P.neq;
-    double s2[MAXEQ], zero[3][MAXEQ] = {}, flux[MAXEQ * 3];
-    const double(*g)[MAXEQ] = dq;
-    const double *sv = s1;
+    real s2[MAXEQ], zero[3][MAXEQ] = {}, flux[MAXEQ * 3];
+    const real(*g)[MAXEQ] = dq;
+    const real *sv = s1;
     switch (bc.type) {
         case BC_SUPERSONIC_OUTFLOW:    // state2 = state1, the element's gradients
         case BC_NOSLIP_ADIABATIC:
@@ -670,26 +678,26 @@ void BdrViscousFlux(const Phys &P, const orc_bc &bc, const double *s1, const dou
     }
     ViscousFlux(P, sv, g[0], g[1], dim > 2 ? g[2] : nullptr, flux);
     for (int v = 0; v < neq; v++) {
-        double a = 0.0;
+        real a = 0.0;
         for (int c = 0; c < dim; c++) a += flux[v * dim + c] * nor[c];   // flux_mat.Mult(nor)
         fn[v] = a;
     }
     if (bc.type == BC_NOSLIP_ADIABATIC) {   // NoSlipAdiabWallBdrFaceIntegrator.cpp:30-44
-        double nn = 0.0;
+        real nn = 0.0;
         for (int c = 0; c < dim; c++) nn += nor[c] * nor[c];
-        const double qn = bc.q[dim] * std::sqrt(nn);
-        double e = 0.0;
+        const real qn = bc.q[dim] * std::sqrt(nn);
+        real e = 0.0;
         for (int d = 0; d < dim; d++) e += bc.q[d] * fn[1 + d];
         fn[neq - 1] = e + qn;
     }
 }
 
 // ComputeBdrFaceLiftingFlux: w1 the element's entropy variables at the face point.
-void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const double *w1, const double *nor, double *fn) {
+void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const real *w1, const real *nor, real *fn) {
     const int dim = P.dim, neq = P.neq;
     if (bc.type == BC_NOSLIP_ADIABATIC || bc.type == BC_NOSLIP_ISOTHERMAL) {
         // NoSlipAdiabWallBdrFaceIntegrator.cpp:74-95, NoSlipIsothWallBdrFaceIntegrator.cpp:41-66
-        const double v = bc.type == BC_NOSLIP_ADIABATIC ? -w1[neq - 1] : 1.0 / (P.R_gas * bc.q[dim]);
+        const real v = bc.type == BC_NOSLIP_ADIABATIC ? -w1[neq - 1] : 1.0 / (P.R_gas * bc.q[dim]);
         fn[0] = w1[0];
         for (int d = 0; d < dim; d++) fn[1 + d] = bc.q[d] * v;
         fn[neq - 1] = -v;
@@ -697,7 +705,7 @@ void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const double *w1, const dou
         return;
     }
     // BdrFaceIntegrator.cpp:275-282
-    double s1[MAXEQ], s2[MAXEQ], w2[MAXEQ];
+    real s1[MAXEQ], s2[MAXEQ], w2[MAXEQ];
     Entropy2Conserv(P, w1, s1);
     OuterState(P, bc, s1, nor, s2);
     Conserv2Entropy(P, s2, w2);
@@ -712,23 +720,23 @@ void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const double *w1, const dou
 struct Mesh {
     int dim, np, nn, NE;
     const int *nbr;                 // neighbour, or -(k+1) for a boundary face with condition bc[k]
-    const double *geom;
+    const real *geom;
     const orc_bc *bc = nullptr;
     int nbc = 0;
-    double x[MAXNP], w[MAXNP], D[MAXNP * MAXNP], Dhat[MAXNP * MAXNP], Dhat2[MAXNP * MAXNP];
-    double J(int e) const { return geom[(size_t)e * (1 + dim * dim)]; }
-    double adj(int e, int r, int c) const { return geom[(size_t)e * (1 + dim * dim) + 1 + r * dim + c]; }
+    real x[MAXNP], w[MAXNP], D[MAXNP * MAXNP], Dhat[MAXNP * MAXNP], Dhat2[MAXNP * MAXNP];
+    real J(int e) const { return geom[(size_t)e * (1 + dim * dim)]; }
+    real adj(int e, int r, int c) const { return geom[(size_t)e * (1 + dim * dim) + 1 + r * dim + c]; }
     int lex(int i, int j, int k) const { return i + np * (j + np * k); }
     // general geometry: per-node (det J, adj J) -- the reference evaluates Tr.Weight() and
     // Tr.AdjugateJacobian() at every node (DGSEMIntegrator.cpp:254-276) -- and per-face
     // orientation codes (neighbour's local face 2 d' + s', bit 3 flips the first face
     // coordinate, bit 4 the second, bit 5 swaps them); null: affine elements, aligned faces.
-    const double *ngeom = nullptr;
+    const real *ngeom = nullptr;
     const int *forient = nullptr;
-    double Jn(int e, int node) const {
+    real Jn(int e, int node) const {
         return ngeom ? ngeom[((size_t)e * nn + node) * (1 + dim * dim)] : J(e);
     }
-    double adjn(int e, int node, int r, int c) const {
+    real adjn(int e, int node, int r, int c) const {
         return ngeom ? ngeom[((size_t)e * nn + node) * (1 + dim * dim) + 1 + r * dim + c] : adj(e, r, c);
     }
     int nbface(int e, int f) const { return forient ? (forient[(size_t)e * 2 * dim + f] & 7) : (f ^ 1); }
@@ -763,22 +771,22 @@ inline int line_node(const Mesh &M, int d, int a, int t1, int t2) {
 // NS 3D element -- ref: :601-750 (with `visc` = the per-node contravariant viscous
 // fluxes F_viscous/G_viscous/H_viscous, and the A1 bug under P.bug & 1).
 // ---------------------------------------------------------------------------
-double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, const double *Fvis,
-                       double *el_dudt, double *el_mag, const double *FvisMag, const double *alpha) {
+real ElementInviscid(const Mesh &M, const Phys &P, int e, const real *el_u, const real *Fvis,
+                       real *el_dudt, real *el_mag, const real *FvisMag, const real *alpha) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq;
     const int Npy = dim > 1 ? np : 1, Npz = dim > 2 ? np : 1;
-    double lam = 0.0;
+    real lam = 0.0;
     // F_inviscid(id)(v, m): DenseTensor(num_equations, Np, n) -- :120-122
-    std::vector<double> Fi((size_t)n * np * neq), Gi((size_t)n * np * neq), Hi((size_t)n * np * neq);
-    auto FI = [&](std::vector<double> &T, int id, int v, int m) -> double & {
+    std::vector<real> Fi((size_t)n * np * neq), Gi((size_t)n * np * neq), Hi((size_t)n * np * neq);
+    auto FI = [&](std::vector<real> &T, int id, int v, int m) -> real & {
         return T[((size_t)id * np + m) * neq + v];
     };
-    double s1[MAXEQ], s2[MAXEQ], f[MAXEQ], g[MAXEQ], h[MAXEQ], m1[3], m2[3];
+    real s1[MAXEQ], s2[MAXEQ], f[MAXEQ], g[MAXEQ], h[MAXEQ], m1[3], m2[3];
     for (int k = 0; k < Npz; k++)
         for (int j = 0; j < Npy; j++)
             for (int i = 0; i < np; i++) {
                 const int id1 = M.lex(i, j, k);
-                const double J = M.Jn(e, id1);                             // Tr.Weight() at id1 (:254)
+                const real J = M.Jn(e, id1);                             // Tr.Weight() at id1 (:254)
                 for (int v = 0; v < neq; v++) s1[v] = el_u[v * n + id1];
                 for (int c = 0; c < dim; c++) m1[c] = M.adjn(e, id1, 0, c);
                 for (int v = 0; v < neq; v++) FI(Fi, id1, v, i) = 0.0;      // f = 0; SetCol(i, f)
@@ -823,9 +831,9 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
                     }
                 }
                 // D-hat-2 accumulation -- :368-390 / :707-738
-                double dUi[MAXEQ], dUv[MAXEQ];
+                real dUi[MAXEQ], dUv[MAXEQ];
                 for (int v = 0; v < neq; v++) {
-                    double acc = 0.0;
+                    real acc = 0.0;
                     for (int m = 0; m < np; m++) acc += FI(Fi, id1, v, m) * M.Dhat2[i * np + m];
                     dUi[v] = acc;
                 }
@@ -855,16 +863,16 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
                     }
                 }
                 for (int v = 0; v < neq; v++) {
-                    double dU = -dUi[v];                                  // Neg :392 / :740
+                    real dU = -dUi[v];                                  // Neg :392 / :740
                     if (alpha) dU *= (1.0 - *alpha);                      // :395 / :742
                     if (Fvis) dU = dU + dUv[v];                           // add :744
                     el_dudt[v * n + id1] += dU * (1.0 / J);               // :403 / :745 (Vector /= c is *= 1/c), AddRow
                 }
                 if (el_mag) {
                     // rounding scale: the same sums with every term in absolute value
-                    const double sa = alpha ? std::fabs(1.0 - *alpha) : 1.0;
+                    const real sa = alpha ? std::fabs(1.0 - *alpha) : 1.0;
                     for (int v = 0; v < neq; v++) {
-                        double a = 0.0;
+                        real a = 0.0;
                         for (int m = 0; m < np; m++) {
                             a += sa * std::fabs(FI(Fi, id1, v, m) * M.Dhat2[i * np + m]);
                             if (dim > 1) a += sa * std::fabs(FI(Gi, id1, v, m) * M.Dhat2[j * np + m]);
@@ -893,28 +901,28 @@ double ElementInviscid(const Mesh &M, const Phys &P, int e, const double *el_u, 
 // sum_m D_lm adj_d(x_m) (:1194-1211; D_T holds D^T, so its column l is row l of D), evaluated
 // as written: on the affine elements here adj is constant and the sums vanish to rounding.
 // ---------------------------------------------------------------------------
-double ElementSubcellFV(const Mesh &M, const Phys &P, int e, const double *el_u, double alpha, double *el_dudt,
-                        double *el_mag) {
+real ElementSubcellFV(const Mesh &M, const Phys &P, int e, const real *el_u, real alpha, real *el_dudt,
+                        real *el_mag) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq;
     const int Nt1 = dim > 1 ? np : 1, Nt2 = dim > 2 ? np : 1;
-    double lam = 0.0;
-    std::vector<double> fv((size_t)n * neq, 0.0), fm((size_t)n * neq, 0.0);
+    real lam = 0.0;
+    std::vector<real> fv((size_t)n * neq, 0.0), fm((size_t)n * neq, 0.0);
     for (int d = 0; d < dim; d++) {
         for (int t2 = 0; t2 < Nt2; t2++)
             for (int t1 = 0; t1 < Nt1; t1++) {
                 // subcell normals of this line (ComputeSubcellMetrics :1194-1211): metric1 at the
                 // line's first node plus the weighted D-sums of the line's adj rows
-                double nor[MAXNP][3];
+                real nor[MAXNP][3];
                 for (int i = 0; i < np; i++) {
                     for (int c = 0; c < dim; c++) nor[i][c] = M.adjn(e, line_node(M, d, 0, t1, t2), d, c);
                     for (int l = 0; l < i; l++) {
-                        double tmp[3] = {0.0, 0.0, 0.0};
+                        real tmp[3] = {0.0, 0.0, 0.0};
                         for (int m = 0; m < np; m++)
                             for (int c = 0; c < dim; c++) tmp[c] += M.adjn(e, line_node(M, d, m, t1, t2), d, c) * M.D[l * np + m];
                         for (int c = 0; c < dim; c++) nor[i][c] += tmp[c] * M.w[l];
                     }
                 }
-                double dU[MAXEQ], aU[MAXEQ], s1[MAXEQ], s2[MAXEQ], f[MAXEQ];
+                real dU[MAXEQ], aU[MAXEQ], s1[MAXEQ], s2[MAXEQ], f[MAXEQ];
                 for (int v = 0; v < neq; v++) dU[v] = aU[v] = 0.0;
                 int id1 = line_node(M, d, 0, t1, t2);
                 for (int v = 0; v < neq; v++) s1[v] = el_u[v * n + id1];
@@ -922,8 +930,8 @@ double ElementSubcellFV(const Mesh &M, const Phys &P, int e, const double *el_u,
                     const int id2 = line_node(M, d, i + 1, t1, t2);
                     for (int v = 0; v < neq; v++) s2[v] = el_u[v * n + id2];
                     lam = std::max(lam, FaceFlux(P, s1, s2, nor[i + 1], f));
-                    const double J = M.Jn(e, id1);                           // Tr.Weight() at id1
-                    const double sc = 1.0 / (J * M.w[i]);                  // Vector /= c multiplies by 1/c
+                    const real J = M.Jn(e, id1);                           // Tr.Weight() at id1
+                    const real sc = 1.0 / (J * M.w[i]);                  // Vector /= c multiplies by 1/c
                     for (int v = 0; v < neq; v++) {
                         dU[v] -= f[v];
                         fv[(size_t)v * n + id1] += dU[v] * sc;             // SetRow (xi) / AddRow (eta, zeta)
@@ -934,8 +942,8 @@ double ElementSubcellFV(const Mesh &M, const Phys &P, int e, const double *el_u,
                     for (int v = 0; v < neq; v++) s1[v] = s2[v];
                     id1 = id2;
                 }
-                const double J = M.Jn(e, id1);
-                const double sc = 1.0 / (J * M.w[np - 1]);
+                const real J = M.Jn(e, id1);
+                const real sc = 1.0 / (J * M.w[np - 1]);
                 for (int v = 0; v < neq; v++) {
                     fv[(size_t)v * n + id1] += dU[v] * sc;
                     fm[(size_t)v * n + id1] += aU[v] / std::fabs(J * M.w[np - 1]);
@@ -956,9 +964,9 @@ struct Rhs {
 };
 
 // el_u gather (GetElementVDofs + GetSubVector, byNODES): el_u[v*n + node]
-inline void gather(const Mesh &M, int neq, const double *u, int e, double *el) {
+inline void gather(const Mesh &M, int neq, const real *u, int e, real *el) {
     for (int v = 0; v < neq; v++)
-        std::memcpy(el + (size_t)v * M.nn, u + gidx(M, v, e, 0), sizeof(double) * M.nn);
+        std::memcpy(el + (size_t)v * M.nn, u + gidx(M, v, e, 0), sizeof(real) * M.nn);
 }
 
 // ---------------------------------------------------------------------------
@@ -969,14 +977,14 @@ inline void gather(const Mesh &M, int neq, const double *u, int e, double *el) {
 // ComputeGlobalPrimitiveGradVector :96-159.  grad holds dim arrays in the
 // layout of u.
 // ---------------------------------------------------------------------------
-void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *grad, double *gmag) {
+void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const real *u, real *grad, real *gmag) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq, NE = M.NE;
     const size_t size = (size_t)neq * NE * n;
-    std::vector<double> went(size);
+    std::vector<real> went(size);
     // ComputeGlobalEntropyVector
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < NE; e++) {
-        double s[MAXEQ], w[MAXEQ];
+        real s[MAXEQ], w[MAXEQ];
         for (int node = 0; node < n; node++) {
             for (int v = 0; v < neq; v++) s[v] = u[gidx(M, v, e, node)];
             Conserv2Entropy(P, s, w);
@@ -991,8 +999,8 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
             for (int j = 0; j < Npy; j++)
                 for (int i = 0; i < np; i++) {
                     const int id = M.lex(i, j, k);
-                    const double J = M.Jn(e, id);
-                    double gm1[MAXEQ][3];                          // grad_mat1 (neq x dim)
+                    const real J = M.Jn(e, id);
+                    real gm1[MAXEQ][3];                          // grad_mat1 (neq x dim)
                     for (int v = 0; v < neq; v++)
                         for (int c = 0; c < 3; c++) gm1[v][c] = 0.0;
                     for (int l = 0; l < np; l++) {
@@ -1005,12 +1013,12 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                     // grad_mat2 = grad_mat1 * adj; *= 1/J  -- :205-206
                     for (int v = 0; v < neq; v++)
                         for (int c = 0; c < dim; c++) {
-                            double acc = 0.0;
+                            real acc = 0.0;
                             for (int r = 0; r < dim; r++) acc += gm1[v][r] * M.adjn(e, id, r, c);
                             grad[(size_t)c * size + gidx(M, v, e, id)] = acc * (1.0 / J);
                         }
                     if (gmag) {
-                        double am[MAXEQ][3];
+                        real am[MAXEQ][3];
                         for (int v = 0; v < neq; v++)
                             for (int r = 0; r < 3; r++) am[v][r] = 0.0;
                         for (int l = 0; l < np; l++)
@@ -1021,7 +1029,7 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                             }
                         for (int v = 0; v < neq; v++)
                             for (int c = 0; c < dim; c++) {
-                                double acc = 0.0;
+                                real acc = 0.0;
                                 for (int r = 0; r < dim; r++) acc += am[v][r] * std::fabs(M.adjn(e, id, r, c));
                                 gmag[(size_t)c * size + gidx(M, v, e, id)] = acc / std::fabs(J);
                             }
@@ -1044,17 +1052,17 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                         int u1, u2;
                         M.nbpoint(e1, f1, t1, t2, u1, u2);
                         const int n1 = line_node(M, d1, s1 ? np - 1 : 0, t1, t2), n2 = line_node(M, d2, s2 ? np - 1 : 0, u1, u2);
-                        const double J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
-                        double nor[3];
+                        const real J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
+                        real nor[3];
                         for (int c = 0; c < dim; c++) nor[c] = (s1 ? 1.0 : -1.0) * M.adjn(e1, n1, d1, c);   // outward from e1
                         for (int v = 0; v < neq; v++) {
-                            const double f = 0.5 * (went[gidx(M, v, e2, n2)] - went[gidx(M, v, e1, n1)]);
+                            const real f = 0.5 * (went[gidx(M, v, e2, n2)] - went[gidx(M, v, e1, n1)]);
                             for (int c = 0; c < dim; c++) {
-                                const double fc = f * nor[c];
+                                const real fc = f * nor[c];
                                 grad[(size_t)c * size + gidx(M, v, e1, n1)] += 1.0 / (M.w[0] * J1) * fc;
                                 grad[(size_t)c * size + gidx(M, v, e2, n2)] += 1.0 / (M.w[0] * J2) * fc;
                                 if (gmag) {
-                                    const double a = 0.5 * (std::fabs(went[gidx(M, v, e2, n2)]) +
+                                    const real a = 0.5 * (std::fabs(went[gidx(M, v, e2, n2)]) +
                                                             std::fabs(went[gidx(M, v, e1, n1)])) * std::fabs(nor[c]);
                                     gmag[(size_t)c * size + gidx(M, v, e1, n1)] += a / (M.w[0] * std::fabs(J1));
                                     gmag[(size_t)c * size + gidx(M, v, e2, n2)] += a / (M.w[0] * std::fabs(J2));
@@ -1069,20 +1077,20 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
         for (int e1 = 0; e1 < NE; e1++) {
             const int e2 = M.nbr[(size_t)e1 * 2 * dim + 2 * d + 1];
             if (e2 < 0) continue;   // a boundary face (below)
-            double nor[3];
+            real nor[3];
             for (int t2 = 0; t2 < Nt2; t2++)
                 for (int t1 = 0; t1 < Nt1; t1++) {
                     const int n1 = line_node(M, d, np - 1, t1, t2), n2 = line_node(M, d, 0, t1, t2);
-                    const double J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
+                    const real J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
                     for (int c = 0; c < dim; c++) nor[c] = M.adjn(e1, n1, d, c);   // CalcOrtho
                     for (int v = 0; v < neq; v++) {
-                        const double f = 0.5 * (went[gidx(M, v, e2, n2)] - went[gidx(M, v, e1, n1)]);  // subtract(0.5,...) :51
+                        const real f = 0.5 * (went[gidx(M, v, e2, n2)] - went[gidx(M, v, e1, n1)]);  // subtract(0.5,...) :51
                         for (int c = 0; c < dim; c++) {
-                            const double fc = f * nor[c];
+                            const real fc = f * nor[c];
                             grad[(size_t)c * size + gidx(M, v, e1, n1)] += 1.0 / (M.w[0] * J1) * fc;
                             grad[(size_t)c * size + gidx(M, v, e2, n2)] += 1.0 / (M.w[0] * J2) * fc;
                             if (gmag) {
-                                const double a = 0.5 * (std::fabs(went[gidx(M, v, e2, n2)]) +
+                                const real a = 0.5 * (std::fabs(went[gidx(M, v, e2, n2)]) +
                                                         std::fabs(went[gidx(M, v, e1, n1)])) * std::fabs(nor[c]);
                                 gmag[(size_t)c * size + gidx(M, v, e1, n1)] += a / (M.w[0] * std::fabs(J1));
                                 gmag[(size_t)c * size + gidx(M, v, e2, n2)] += a / (M.w[0] * std::fabs(J2));
@@ -1106,10 +1114,10 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                     for (int t2 = 0; t2 < Nt2; t2++)
                         for (int t1 = 0; t1 < Nt1; t1++) {
                             const int n1 = line_node(M, d, side ? np - 1 : 0, t1, t2);
-                            const double J = M.Jn(e, n1);
-                            double nor[3];
+                            const real J = M.Jn(e, n1);
+                            real nor[3];
                             for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adjn(e, n1, d, c);   // outward
-                            double w1[MAXEQ], f[MAXEQ];
+                            real w1[MAXEQ], f[MAXEQ];
                             for (int v = 0; v < neq; v++) w1[v] = went[gidx(M, v, e, n1)];
                             BdrLiftingFlux(P, bc, w1, nor, f);
                             for (int v = 0; v < neq; v++)
@@ -1126,7 +1134,7 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
     // ComputeGlobalPrimitiveGradVector: EntropyGrad2PrimGrad per direction
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < NE; e++) {
-        double s[MAXEQ], g[MAXEQ];
+        real s[MAXEQ], g[MAXEQ];
         for (int node = 0; node < n; node++) {
             for (int v = 0; v < neq; v++) s[v] = u[gidx(M, v, e, node)];
             for (int c = 0; c < dim; c++) {
@@ -1135,17 +1143,17 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
                 for (int v = 0; v < neq; v++) grad[(size_t)c * size + gidx(M, v, e, node)] = g[v];
                 if (gmag) {
                     // the chain rule applied to the magnitudes, term by term in absolute value
-                    double a[MAXEQ], o[MAXEQ];
+                    real a[MAXEQ], o[MAXEQ];
                     for (int v = 0; v < neq; v++) a[v] = gmag[(size_t)c * size + gidx(M, v, e, node)];
-                    const double rho = s[0];
-                    double vel[3], vsq = 0.0;
+                    const real rho = s[0];
+                    real vel[3], vsq = 0.0;
                     for (int d = 0; d < dim; d++) {
                         vel[d] = s[1 + d] / rho;
                         vsq += vel[d] * vel[d];
                     }
-                    const double p = P.gammaM1 * (s[neq - 1] - 0.5 * rho * vsq);
-                    const double por = std::fabs(p / rho), ae = a[neq - 1];
-                    double mdot = 0.0;
+                    const real p = P.gammaM1 * (s[neq - 1] - 0.5 * rho * vsq);
+                    const real por = std::fabs(p / rho), ae = a[neq - 1];
+                    real mdot = 0.0;
                     for (int d = 0; d < dim; d++) {
                         o[1 + d] = por * (a[1 + d] + std::fabs(vel[d]) * ae);
                         mdot += std::fabs(s[1 + d]) * o[1 + d];
@@ -1167,12 +1175,12 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const double *u, double *g
 // DGSEMIntegrator::AssembleFaceVector (Euler :151-222, NS 3D :435-494, NS 2D :496-549).
 // Returns lambda_max over all pair and face flux evaluations (:270,:188).
 // ---------------------------------------------------------------------------
-double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, double *grad_out, double *mag = nullptr,
-               double *gmag = nullptr, const double *alpha = nullptr) {
+real RhsEval(const Mesh &M, const Phys &P, const real *u, real *dudt, real *grad_out, real *mag = nullptr,
+               real *gmag = nullptr, const real *alpha = nullptr) {
     const int np = M.np, n = M.nn, dim = P.dim, neq = P.neq, NE = M.NE;
     const size_t size = (size_t)neq * NE * n;
-    std::vector<double> grad_local, gmag_local;
-    double *grad = grad_out;
+    std::vector<real> grad_local, gmag_local;
+    real *grad = grad_out;
     if (P.viscous) {
         if (!grad) {
             grad_local.resize(size * dim);
@@ -1186,19 +1194,19 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
     }
     std::fill(dudt, dudt + size, 0.0);
     if (mag) std::fill(mag, mag + size, 0.0);
-    double lam = 0.0;
+    real lam = 0.0;
     // element loop
 #pragma omp parallel for schedule(static) reduction(max : lam)
     for (int e = 0; e < NE; e++) {
-        std::vector<double> el_u((size_t)n * neq), el_dudt((size_t)n * neq, 0.0), el_mag, Fvis;
+        std::vector<real> el_u((size_t)n * neq), el_dudt((size_t)n * neq, 0.0), el_mag, Fvis;
         if (mag) el_mag.assign((size_t)n * neq, 0.0);
         gather(M, neq, u, e, el_u.data());
-        std::vector<double> FvisMag;
+        std::vector<real> FvisMag;
         if (P.viscous) {
             // per-node viscous flux, mapped with adj -- :617-641
             Fvis.assign((size_t)3 * n * neq, 0.0);
             if (mag) FvisMag.assign((size_t)3 * n * neq, 0.0);
-            double s[MAXEQ], dq[3][MAXEQ], flux[MAXEQ * 3];
+            real s[MAXEQ], dq[3][MAXEQ], flux[MAXEQ * 3];
             for (int i = 0; i < n; i++) {
                 // A2: the reference reads state1 at the stale id1 (:620); under bug_compat
                 // this is taken as node n-1 of the same element (blending on leaves id1 = n-1).
@@ -1210,18 +1218,18 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                 // MultABt(adj1, flux_mat1, flux_mat): flux_mat(r, v) = sum_c adj(r,c) flux(v,c)
                 for (int r = 0; r < dim; r++)
                     for (int v = 0; v < neq; v++) {
-                        double acc = 0.0;
+                        real acc = 0.0;
                         for (int c = 0; c < dim; c++) acc += M.adjn(e, i, r, c) * flux[v * dim + c];
                         Fvis[((size_t)r * n + i) * neq + v] = acc;
                     }
                 if (mag) {
-                    double gq[3][MAXEQ], fm[MAXEQ * 3];
+                    real gq[3][MAXEQ], fm[MAXEQ * 3];
                     for (int c = 0; c < dim; c++)
                         for (int v = 0; v < neq; v++) gq[c][v] = gmag[(size_t)c * size + gidx(M, v, e, i)];
                     ViscousFluxMag(P, s, gq[0], gq[1], dim > 2 ? gq[2] : nullptr, fm);
                     for (int r = 0; r < dim; r++)
                         for (int v = 0; v < neq; v++) {
-                            double acc = 0.0;
+                            real acc = 0.0;
                             for (int c = 0; c < dim; c++) acc += std::fabs(M.adjn(e, i, r, c)) * fm[v * dim + c];
                             FvisMag[((size_t)r * n + i) * neq + v] = acc;
                         }
@@ -1230,11 +1238,11 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
         }
         if (alpha) {
             // SUBCELL_FV_BLENDING: the FV term first, into the zeroed element vector (:240-241)
-            const double l = ElementSubcellFV(M, P, e, el_u.data(), alpha[e], el_dudt.data(),
+            const real l = ElementSubcellFV(M, P, e, el_u.data(), alpha[e], el_dudt.data(),
                                               mag ? el_mag.data() : nullptr);
             lam = std::max(lam, l);
         }
-        double l = ElementInviscid(M, P, e, el_u.data(), P.viscous ? Fvis.data() : nullptr, el_dudt.data(),
+        real l = ElementInviscid(M, P, e, el_u.data(), P.viscous ? Fvis.data() : nullptr, el_dudt.data(),
                                    mag ? el_mag.data() : nullptr, FvisMag.empty() ? nullptr : FvisMag.data(),
                                    alpha ? alpha + e : nullptr);
         lam = std::max(lam, l);
@@ -1261,15 +1269,15 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
             const int f2 = M.nbface(e1, f1);
             if (M.forient && (e2 < e1 || (e2 == e1 && f2 < f1))) continue;
             const int d1 = f1 >> 1, sd1 = f1 & 1, d2 = f2 >> 1, sd2 = f2 & 1;
-            double nor[3], s1[MAXEQ], s2[MAXEQ], fnum[MAXEQ], fl1[MAXEQ * 3], fl2[MAXEQ * 3];
-            double dq[3][MAXEQ];
+            real nor[3], s1[MAXEQ], s2[MAXEQ], fnum[MAXEQ], fl1[MAXEQ * 3], fl2[MAXEQ * 3];
+            real dq[3][MAXEQ];
             for (int t2 = 0; t2 < Nt2; t2++)
                 for (int t1 = 0; t1 < Nt1; t1++) {
                     int u1, u2;
                     M.nbpoint(e1, f1, t1, t2, u1, u2);
                     const int n1 = line_node(M, d1, sd1 ? np - 1 : 0, t1, t2);
                     const int n2 = line_node(M, d2, sd2 ? np - 1 : 0, u1, u2);
-                    const double J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
+                    const real J1 = M.Jn(e1, n1), J2 = M.Jn(e2, n2);
                     // CalcOrtho of the face Jacobian: e1's adj row d1 at the face point, outward
                     for (int c = 0; c < dim; c++) nor[c] = (sd1 ? 1.0 : -1.0) * M.adjn(e1, n1, d1, c);
                     for (int v = 0; v < neq; v++) {
@@ -1277,7 +1285,7 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                         s2[v] = u[gidx(M, v, e2, n2)];
                     }
                     lam = std::max(lam, FaceFlux(P, s1, s2, nor, fnum));
-                    double dU1[MAXEQ], dU2[MAXEQ], aU[MAXEQ];
+                    real dU1[MAXEQ], dU2[MAXEQ], aU[MAXEQ];
                     for (int v = 0; v < neq; v++) {
                         dU1[v] = -fnum[v];
                         dU2[v] = fnum[v];
@@ -1290,7 +1298,7 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                         for (int c = 0; c < dim; c++)
                             for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e2, n2)];
                         ViscousFlux(P, s2, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, fl2);
-                        double fm1[MAXEQ * 3], fm2[MAXEQ * 3];
+                        real fm1[MAXEQ * 3], fm2[MAXEQ * 3];
                         if (mag) {
                             for (int c = 0; c < dim; c++)
                                 for (int v = 0; v < neq; v++) dq[c][v] = gmag[(size_t)c * size + gidx(M, v, e1, n1)];
@@ -1300,8 +1308,8 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                             ViscousFluxMag(P, s2, dq[0], dq[1], dim > 2 ? dq[2] : nullptr, fm2);
                         }
                         for (int v = 0; v < neq; v++) {
-                            double fv = 0.0;                   // (flux_mat1 + flux_mat2) * 0.5, Mult(nor)
-                            double af = 0.0;
+                            real fv = 0.0;                   // (flux_mat1 + flux_mat2) * 0.5, Mult(nor)
+                            real af = 0.0;
                             for (int c = 0; c < dim; c++) {
                                 fv += ((fl1[v * dim + c] + fl2[v * dim + c]) * 0.5) * nor[c];
                                 if (mag) af += 0.5 * (fm1[v * dim + c] + fm2[v * dim + c]) * std::fabs(nor[c]);
@@ -1335,10 +1343,10 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                     for (int t2 = 0; t2 < Nt2; t2++)
                         for (int t1 = 0; t1 < Nt1; t1++) {
                             const int n1 = line_node(M, d, side ? np - 1 : 0, t1, t2);
-                            const double J = M.Jn(e, n1);
-                            double nor[3];
+                            const real J = M.Jn(e, n1);
+                            real nor[3];
                             for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adjn(e, n1, d, c);
-                            double s1[MAXEQ], f[MAXEQ], dU[MAXEQ], aU[MAXEQ];
+                            real s1[MAXEQ], f[MAXEQ], dU[MAXEQ], aU[MAXEQ];
                             for (int v = 0; v < neq; v++) s1[v] = u[gidx(M, v, e, n1)];
                             lam = std::max(lam, BdrInviscidFlux(P, bc, s1, nor, f));
                             for (int v = 0; v < neq; v++) {
@@ -1346,7 +1354,7 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                                 aU[v] = std::fabs(f[v]);
                             }
                             if (P.viscous) {
-                                double dq[3][MAXEQ], fv[MAXEQ];
+                                real dq[3][MAXEQ], fv[MAXEQ];
                                 for (int c = 0; c < dim; c++)
                                     for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e, n1)];
                                 BdrViscousFlux(P, bc, s1, dq, nor, fv);
@@ -1355,11 +1363,11 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
                                     if (mag) {
                                         // rounding scale: the element-side viscous flux's scale (a bound for
                                         // each BC's flux of the trace and its gradients)
-                                        double gq[3][MAXEQ], fm[MAXEQ * 3];
+                                        real gq[3][MAXEQ], fm[MAXEQ * 3];
                                         for (int c = 0; c < dim; c++)
                                             for (int k = 0; k < neq; k++) gq[c][k] = gmag[(size_t)c * size + gidx(M, k, e, n1)];
                                         ViscousFluxMag(P, s1, gq[0], gq[1], dim > 2 ? gq[2] : nullptr, fm);
-                                        double a = std::fabs(fv[v]);
+                                        real a = std::fabs(fv[v]);
                                         for (int c = 0; c < dim; c++) a += fm[v * dim + c] * std::fabs(nor[c]);
                                         aU[v] += a;
                                     }
@@ -1376,7 +1384,7 @@ double RhsEval(const Mesh &M, const Phys &P, const double *u, double *dudt, doub
     return lam;
 }
 
-Phys MakePhys(int dim, double gamma, double Pr, double mu, double R_gas, int viscous, int sutherland,
+Phys MakePhys(int dim, real gamma, real Pr, real mu, real R_gas, int viscous, int sutherland,
               int surface_flux, int bug) {
     Phys P;
     P.dim = dim;
@@ -1395,7 +1403,7 @@ Phys MakePhys(int dim, double gamma, double Pr, double mu, double R_gas, int vis
     return P;
 }
 
-void MakeMesh(Mesh &M, int dim, int order, int NE, const int *nbr, const double *geom) {
+void MakeMesh(Mesh &M, int dim, int order, int NE, const int *nbr, const real *geom) {
     M.dim = dim;
     M.np = order + 1;
     M.nn = 1;
@@ -1409,6 +1417,7 @@ void MakeMesh(Mesh &M, int dim, int order, int NE, const int *nbr, const double 
 
 }  // namespace
 
+#ifndef ORACLE_NO_C_API
 // ===========================================================================
 // C interface (ctypes)
 // ===========================================================================
@@ -1648,3 +1657,4 @@ int orc_rk_run(int dim, int order, int NE, const int *nbr, const double *geom, c
 }
 
 }  // extern "C"
+#endif  // ORACLE_NO_C_API
