@@ -154,6 +154,19 @@ __device__ __forceinline__ constexpr int line_base(int l, int dir) {
     return (l % st) + (l / st) * st * NP;
 }
 
+// line_base for a direction known only at run time: the compile-time-stride forms, selected
+// (a division by a run-time stride would be a generic integer division)
+template <int DIM, int NP>
+__device__ __forceinline__ int line_base_rt(int l, int dir) {
+    const int b0 = line_base<NP>(l, 0), b1 = line_base<NP>(l, 1);
+    if constexpr (DIM == 2) {
+        return dir == 0 ? b0 : b1;
+    } else {
+        const int b2 = line_base<NP>(l, 2);
+        return dir == 0 ? b0 : (dir == 1 ? b1 : b2);
+    }
+}
+
 template <int NP>
 __device__ __forceinline__ constexpr int dir_stride(int dir) {
     return dir == 0 ? 1 : (dir == 1 ? NP : NP * NP);
@@ -447,6 +460,14 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     static_assert(!BLEND || EPB * FITEMS * NEQ >= 3 * EPB * NN, "the coefficient scratch fits the surface-flux buffer");
     __shared__ double sBSum[BLEND ? EPB : 1][3];
     __shared__ double sAlpha[BLEND ? EPB : 1][2];   // alpha, eta
+    // affine FP64 interior faces take the element's own side from LDS: the primitives above and
+    // each node's max wave speed, and each face direction's |adj row|
+    constexpr bool LSIDE = !GEN && !MIX;
+    __shared__ double sLam[LSIDE ? EPB * NN : 1];
+    __shared__ R sNm[GEN ? 1 : EPB][DIM];
+    // volume pairs: each symmetric pair evaluated once when the lines of an element lie in one
+    // wave (the partner's value arrives by a cross-lane shuffle)
+    constexpr bool ROT = !GEN && !BUG1 && (C::BLOCK == 64 || NN == 64);
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -474,6 +495,15 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             const int ib = i / (DIM * DIM), r = i - ib * DIM * DIM;
             const int ee = min(blk * EPB + ib, NE - 1);
             sM[ib][r / DIM][r % DIM] = (R)A.geom[(size_t)ee * (1 + DIM * DIM) + 1 + r];
+        }
+        for (int i = tid; i < EPB * DIM; i += C::BLOCK) {
+            const int ib = i / DIM, d = i - ib * DIM;
+            const int ee = min(blk * EPB + ib, NE - 1);
+            const double *g = A.geom + (size_t)ee * (1 + DIM * DIM) + 1 + d * DIM;
+            R nm = (R)g[0] * (R)g[0];
+#pragma unroll
+            for (int c = 1; c < DIM; c++) nm += (R)g[c] * (R)g[c];
+            sNm[ib][d] = sqrt(nm);
         }
     }
     double lam = 0.0;
@@ -516,6 +546,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             sHref[eb] = A.P.gamma * A.P.gammaM1Inverse * qs.p * qs.irho;
         }
         lam = qs.lam;
+        if constexpr (LSIDE) sLam[tid] = qs.lam;
         if constexpr (BLEND) qrp = rho_p<DIM>(s, A.P);
     }
     __syncthreads();
@@ -536,7 +567,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         if (ee >= NE) continue;
         const int dir = r / (2 * L), side = (r / L) & 1, t = r - (r / L) * L;
         const int st = dir_stride<NP>(dir);
-        const int base = line_base<NP>(t, dir);
+        const int base = line_base_rt<DIM, NP>(t, dir);
         const int nown = base + (side ? (NP - 1) * st : 0);
         const int nnb = base + (side ? 0 : (NP - 1) * st);
         const int en = sNbr[ib][2 * dir + side];
@@ -632,13 +663,19 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                                    A.Ff[ftrace<DIM, NP>(dir, 1 - side, v, en, t, NE)]);
             }
         }
-        const SideState<DIM> qo = make_side<DIM>(so, A.P);
+        SideState<DIM> qo;
+        if constexpr (LSIDE) {
+            const int lo = ib * NN + nown;
+            double vo[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) vo[d] = sPv[d][lo];
+            qo = side_from_prim<DIM>(sPd[0][lo], sPd[1][lo], vo, sLam[lo]);
+        } else {
+            qo = make_side<DIM>(so, A.P);
+        }
         const SideState<DIM> qn = make_side<DIM>(sn, A.P);
         const R *m = sM[ib][dir];
-        R nmag = m[0] * m[0];
-#pragma unroll
-        for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
-        nmag = sqrt(nmag);
+        const R nmag = sNm[ib][dir];
         R f[NEQ];
         if constexpr (MIX) {
             const double p_ref = sPref[ib], h_ref = sHref[ib];
@@ -752,11 +789,8 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             const int a = ai[dir];
             const int lb = eb * NN + node - a * st;
             const R *m = sM[GEN ? 0 : eb][dir];   // (GEN takes the branch above)
-#pragma unroll 1
-            for (int k = 1; k < NP; k++) {
-                const int b = (a + k) % NP;
-                const int li = lb + b * st;
-                R F[NEQ];
+            // the two-point flux with the node at LDS index li (the line partner b)
+            auto pair_flux = [&](int b, int li, R *F) {
                 if constexpr (MIX) {
                     NodePrimMx<DIM> qo, qb;
                     qo.rho = sPd[0][tid];
@@ -801,9 +835,38 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
                     volume_flux<DIM, true>(qo, qb, m, A.P.gammaM1Inverse, F);
                 }
-                const R dd = sT[a * NP + b];
+            };
+            if constexpr (ROT) {
+                // each symmetric pair once: node a evaluates (a, a+k) for k <= Np/2 and receives
+                // (a-k, a) from the lane of node a-k (the same flux, F(u_i,u_j) = F(u_j,u_i));
+                // 2k = Np pairs are evaluated by both of their nodes
+                const int lane = tid & 63;
+#pragma unroll 1
+                for (int k = 1; 2 * k <= NP; k++) {
+                    const int b = (a + k) % NP;
+                    R F[NEQ];
+                    pair_flux(b, lb + b * st, F);
+                    const R dd = sT[a * NP + b];
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                    for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                    if (2 * k != NP) {
+                        const int c = (a + NP - k) % NP;
+                        const int src = lane + (c - a) * st;
+                        const R dc = sT[a * NP + c];
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) acc[v] += dc * __shfl(F[v], src, 64);
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (int k = 1; k < NP; k++) {
+                    const int b = (a + k) % NP;
+                    R F[NEQ];
+                    pair_flux(b, lb + b * st, F);
+                    const R dd = sT[a * NP + b];
+#pragma unroll
+                    for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                }
             }
             if (a == 0 || a == NP - 1) {
                 const int side = (a == NP - 1) ? 1 : 0;

@@ -44,6 +44,7 @@ struct NodePrim {
 template <int DIM>
 struct SideState {
     double rho, irho, vel[DIM], vsq, p, beta, lam;
+    double por;   // p / rho
 };
 
 template <int DIM>
@@ -59,8 +60,27 @@ __device__ __forceinline__ SideState<DIM> make_side(const double *s, const PhysC
     }
     q.p = P.gammaM1 * (s[DIM + 1] - 0.5 * q.rho * q.vsq);       // ChandrashekarFlux.cpp:58
     const double por = q.p * q.irho;
+    q.por = por;
     q.beta = 0.5 * rcp(por);                                   // :67  rho / (2 p)
     q.lam = sqrt(q.vsq) + sqrt(P.gamma * por);                 // :61-65
+    return q;
+}
+
+// The surface fluxes' view of a node whose primitives a kernel already holds (density, velocity,
+// beta = rho / (2 p), max wave speed): p / rho = 1 / (2 beta), p = rho p / rho.  irho and vsq are
+// not set (the surface fluxes do not read them).
+template <int DIM>
+__device__ __forceinline__ SideState<DIM> side_from_prim(double rho, double beta, const double *vel, double lam) {
+    SideState<DIM> q;
+    q.rho = rho;
+    q.beta = beta;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) q.vel[d] = vel[d];
+    q.lam = lam;
+    q.por = 0.5 * rcp(beta);
+    q.p = rho * q.por;
+    q.irho = 0.0;
+    q.vsq = 0.0;
     return q;
 }
 
@@ -193,7 +213,7 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
         h_hat += 0.5 * il_b * P.gammaM1Inverse + p_hat * il_rho;
         // 1/beta = 2 p / rho
         diss += 0.5 * drho * P.gammaM1Inverse * il_b +
-                rho_mean * P.gammaM1Inverse * (q2.p * q2.irho - q1.p * q1.irho);        // :151
+                rho_mean * P.gammaM1Inverse * (q2.por - q1.por);                        // :151
         const double hl = 0.5 * lam * nmag;
         const double mflux = rho_ln * Vn;
         flux[0] = mflux - hl * drho;
@@ -293,7 +313,7 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
         // diss - h_ref drho: the energy jump measured from h_ref times the density jump
         const double drho64 = q2.rho - q1.rho;
         diss += (float)(drho64 * (0.5 * P.gammaM1Inverse * il_b - h_ref)) +
-                rho_mean * g1 * (float)(q2.p * q2.irho - q1.p * q1.irho);
+                rho_mean * g1 * (float)(q2.por - q1.por);
         const float hl = 0.5f * lam * nmag;
         const float mflux = rho_ln * Vn;
         const float pd = (float)(p_hat - p_ref);
