@@ -39,6 +39,10 @@ FP64_PEAK_TFLOPS = 78.6
 #   k_rhs  : Mult epilogue: read u (40) + dv (32) + Ff (48), write dudt (40)        = 160
 #            LSRK epilogue: read u, dv, Ff, k (160), write k and u' (80)            = 240
 BYTES_PER_NODE = {"lift": 120.0, "rhs_mult": 160.0, "rhs_lsrk": 240.0}
+# One RHS evaluation's two-pass minimum (BR1 materialises the gradient information between the
+# passes): pass A reads u (40) and writes the gradient information (120); pass B reads u (40) and
+# it (120) and writes du (40)
+STAGE_BYTES_PER_NODE = 360.0
 
 # HBM bytes per launch of the dominant kernel (k_rhs_node FP64, LSRK stage, TGV 32^3 p=3),
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (two runs; counter KiB x 1024, the
@@ -51,30 +55,53 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh, u, phys, threads, budget_s=15.0):
-    """The CPU oracle (a restatement of the reference's algorithm) on the host cores."""
+def physical_cores():
+    """Physical cores of the host (distinct (core, socket) pairs of `lscpu -p`), or None."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu", "-p=core,socket"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    pairs = {tuple(line.split(",")) for line in out.splitlines() if line and not line.startswith("#")}
+    return len(pairs) or None
+
+
+def cpu_baseline(mesh, u, phys, threads, budget_s=15.0, single_evals=3):
+    """The CPU oracle (a restatement of the reference's algorithm) on the host cores, and on one."""
     import oracle
 
     oracle.build()
-    used = oracle.set_threads(threads)
     ph = oracle.phys(gamma=phys["gamma"], Pr=phys["Pr"], mu=phys["mu"], viscous=True)
-    oracle.rhs(mesh, ph, u)  # warm-up
-    times = []
-    t_start = time.perf_counter()
-    while True:
-        t0 = time.perf_counter()
-        oracle.rhs(mesh, ph, u)
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > budget_s or len(times) >= 20:
-            break
+
+    def timed(n_threads, max_evals, budget):
+        used = oracle.set_threads(n_threads)
+        oracle.rhs(mesh, ph, u)  # warm-up
+        times = []
+        t_start = time.perf_counter()
+        while True:
+            t0 = time.perf_counter()
+            oracle.rhs(mesh, ph, u)
+            times.append(time.perf_counter() - t0)
+            if time.perf_counter() - t_start > budget or len(times) >= max_evals:
+                break
+        return used, times
+
+    used, times = timed(threads, 20, budget_s)
     med = statistics.median(times)
+    _, times1 = timed(1, single_evals, budget_s)
+    med1 = statistics.median(times1)
+    oracle.set_threads(threads)
     return {
         "value": mesh.num_nodes / med,
         "unit": "DOF-RHS evals/s",
         "cores": used,
         "kind": "port",
         "sample": f"{len(times)} oracle RHS evaluations of the same TGV {mesh.n[0]}^3 p={mesh.order} NS state "
-                  f"(median {med:.3f} s each, {used} OpenMP threads, {budget_s:.0f} s budget)",
+                  f"(median {med:.3f} s each, {used} OpenMP threads, {budget_s:.0f} s budget); "
+                  f"{len(times1)} more on one thread (median {med1:.3f} s)",
+        "value_1core": mesh.num_nodes / med1,
+        "physical_cores": physical_cores(),
     }
 
 
@@ -561,6 +588,61 @@ def geometry_case(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def vortex_case(dev, steps=100, reps=5):
+    """The 2D isentropic vortex (16x16 quads, p=3, 4,096 nodes; Chandrashekar volume flux, Rusanov
+    surface flux, SSP-RK3 at CFL 0.5, 100 steps): launch-bound, so the 100 steps are also captured
+    once in a HIP graph (torch.cuda.CUDAGraph around prandtl_rk_run) and replayed.  Returns the
+    DOF-RHS rates of the eager and the replayed run (median of `reps` timed runs each)."""
+    import torch
+
+    from prandtl_amd import capi, cases
+
+    mesh = cases.vortex_mesh(n=16, order=3)
+    u0 = torch.from_numpy(cases.vortex_state(mesh)).to(dev)
+    op = capi.Operator(mesh, surface_flux=capi.SURF_RUSANOV)
+    lam = op.max_char_speed(u0)
+    dt = cases.cfl_dt(mesh, lam, 0.5)
+    u = u0.clone()
+    op.rk_run(capi.SSPRK3, u, 0.0, dt, 10)   # warm-up; allocates the stepper's registers
+    eager = []
+    for _ in range(reps):
+        u.copy_(u0)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        op.rk_run(capi.SSPRK3, u, 0.0, dt, steps)
+        b.record()
+        torch.cuda.synchronize()
+        eager.append(a.elapsed_time(b))
+    ref = u.clone()
+    ug = u0.clone()
+    g = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(g, stream=cap):
+            op.rk_run(capi.SSPRK3, ug, 0.0, dt, steps)
+    torch.cuda.current_stream().wait_stream(cap)
+    graph = []
+    for _ in range(reps):
+        ug.copy_(u0)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        graph.append(a.elapsed_time(b))
+        assert torch.equal(ug, ref), "graph replay differs from the eager run"
+    op.close()
+    rhs = mesh.num_nodes * 3 * steps
+    e_ms, g_ms = statistics.median(eager), statistics.median(graph)
+    return {"workload": f"2D Euler isentropic vortex 16x16 quads p=3 ({mesh.num_nodes} nodes), Chandrashekar "
+                        f"volume + Rusanov surface flux, SSP-RK3 CFL 0.5 (dt={dt:.4e}), {steps} steps",
+            "eager_dof_rhs_per_s": rhs / (e_ms * 1e-3), "graph_dof_rhs_per_s": rhs / (g_ms * 1e-3),
+            "eager_ms": e_ms, "graph_ms": g_ms, "launches_per_step": 3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -588,6 +670,9 @@ def main():
                     help="run the general-mesh cases (TGV 32^3 NS on a curved mesh against the affine lattice; "
                          "the Gmsh forward-facing step refined twice, Mach 3 with blending) instead of the default "
                          "benchmark")
+    ap.add_argument("--vortex", action="store_true",
+                    help="run only the 2D vortex case (16x16 p=3, 100 SSP-RK3 steps, eager and replayed from a "
+                         "HIP graph)")
     ap.add_argument("--adjoint", action="store_true",
                     help="run the discrete-adjoint case (TGV Euler 16^3 p=4 + 64 Fourier modes, 200 RK stages, "
                          "checkpoints every 20 steps) instead of the default benchmark")
@@ -611,6 +696,13 @@ def main():
         return
     if args.adjoint:
         adjoint_case(dev)
+        return
+    if args.vortex:
+        v = vortex_case(dev)
+        print(json.dumps({"metric": "DOF-RHS evals/s, 2D Euler isentropic vortex (SSP-RK3, 3 RHS per step)",
+                          "value": v["graph_dof_rhs_per_s"], "unit": "DOF-RHS evals/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "f64", "data": "synthetic (analytic Shu vortex)",
+                          "config": {"workload": v["workload"]}, **v}), flush=True)
         return
     if args.khi:
         khi_case(args, dev)
@@ -650,6 +742,33 @@ def main():
     value = nodes * rhs_evals / (ms / 1e3)
     log(f"{args.steps} LSRK4(5) steps: {ms:.2f} ms, {value / 1e9:.3f} G DOF-RHS/s")
 
+    # ---- the dominant kernels' durations inside the same loop -------------------
+    # every lifting and flux-differencing launch of a further run of the timed steps bracketed by
+    # HIP events on the launch stream (prandtl_op_timing)
+    op.timing(True)
+    op.rk_run(capi.LSRK45, ud, t, dt, min(args.steps, 50))
+    (loop_lift_ms, loop_rhs_ms), loop_counts = op.timing_read()
+    op.timing(False)
+    log(f"in-loop kernel durations: k_lift {loop_lift_ms:.4f} ms, k_rhs_node {loop_rhs_ms:.4f} ms "
+        f"({loop_counts[1]} launches)")
+
+    # ---- the same steps with the Rusanov surface flux -----------------------------
+    opr = capi.Operator(mesh, surface_flux=capi.SURF_RUSANOV, **phys)
+    ur = torch.from_numpy(u0).to(dev)
+    opr.rk_run(capi.LSRK45, ur, 0.0, dt, args.warmup)
+    torch.cuda.synchronize()
+    e4 = torch.cuda.Event(enable_timing=True)
+    e5 = torch.cuda.Event(enable_timing=True)
+    e4.record(stream)
+    opr.rk_run(capi.LSRK45, ur, 0.0, dt, args.steps)
+    e5.record(stream)
+    torch.cuda.synchronize()
+    rus_ms = e4.elapsed_time(e5)
+    rus_value = nodes * rhs_evals / (rus_ms / 1e3)
+    log(f"Rusanov surface flux: {rus_ms:.2f} ms, {rus_value / 1e9:.3f} G DOF-RHS/s")
+    opr.close()
+    del ur
+
     # ---- the same steps in FP32-mixed precision ---------------------------------
     opm = capi.Operator(mesh, precision=capi.MIXED, **phys)
     um = torch.from_numpy(u0).to(dev)
@@ -683,13 +802,17 @@ def main():
         mult_ms.append(a.elapsed_time(b))
     mult_med = statistics.median(mult_ms)
 
-    # ---- per-kernel timing (HIP events on the launch stream) -------------------
+    # ---- per-kernel timing back to back (HIP events on the launch stream) ---------
     lift_ms, rhs_ms = op.profile(ud, epilogue=1, iters=20)
     lift_mult_ms, rhs_mult_ms = op.profile(ud, epilogue=0, iters=20)
-    dom = "k_rhs_node" if rhs_ms >= lift_ms else "k_lift"
-    dom_ms = max(rhs_ms, lift_ms)
+    # the roofline of the dominant kernel from its average duration inside the stepping loop
+    dom = "k_rhs_node" if loop_rhs_ms >= loop_lift_ms else "k_lift"
+    dom_ms = max(loop_rhs_ms, loop_lift_ms)
     dom_bytes = nodes * (BYTES_PER_NODE["rhs_lsrk"] if dom == "k_rhs_node" else BYTES_PER_NODE["lift"])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    # the whole stage against the two-pass minimum traffic of the RHS + stage update (360 B/node)
+    stage_ms = ms / rhs_evals
+    stage_achieved = STAGE_BYTES_PER_NODE * nodes / (stage_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
         "kernel": f"{dom} (LSRK4(5) stage epilogue)" if dom == "k_rhs_node" else dom,
@@ -699,11 +822,23 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": (RHS_LSRK_TRAFFIC_TGV32_P3 if (dom == "k_rhs_node" and args.n == 32 and args.order == 3) else None),
         "traffic_source": "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, profiles/pmc_tgv32_p3_fp64_mixed.md",
-        "kernel_ms": {"k_lift": round(lift_ms, 4), "k_rhs_lsrk": round(rhs_ms, 4),
-                      "k_rhs_mult": round(rhs_mult_ms, 4)},
+        "duration_source": f"mean of {loop_counts[1]} launches inside a run of the timed LSRK4(5) steps, "
+                           f"HIP events around each launch on the launch stream",
+        "kernel_ms": {"k_lift": round(loop_lift_ms, 4), "k_rhs_lsrk": round(loop_rhs_ms, 4)},
+        "kernel_ms_back_to_back": {"k_lift": round(lift_ms, 4), "k_rhs_lsrk": round(rhs_ms, 4),
+                                   "k_rhs_mult": round(rhs_mult_ms, 4)},
         "bytes_per_node": BYTES_PER_NODE,
+        "stage": {"achieved": round(stage_achieved, 1), "frac": round(stage_achieved / HBM_PEAK_GBS, 4),
+                  "bytes_per_node": STAGE_BYTES_PER_NODE, "ms": round(stage_ms, 5),
+                  "note": "one LSRK4(5) stage (lifting pass + flux pass with fused update) over its two-pass "
+                          "minimum traffic"},
     }
     del du
+
+    # ---- the small 2D case, eager and from a HIP graph ---------------------------
+    vortex = vortex_case(dev)
+    log(f"vortex 16x16 p=3: eager {vortex['eager_dof_rhs_per_s'] / 1e9:.3f}, graph "
+        f"{vortex['graph_dof_rhs_per_s'] / 1e9:.3f} G DOF-RHS/s")
 
     # ---- CPU baseline ------------------------------------------------------------
     cpu = None
@@ -736,6 +871,8 @@ def main():
         },
         "mixed_value": mixed_value,
         "mixed_speedup": mixed_value / value,
+        "rusanov_value": rus_value,
+        "vortex_2d": vortex,
         "mult_dof_rhs_per_s": nodes / (mult_med * 1e-3),
         "mult_ms": mult_med,
         "t_final": t,

@@ -261,6 +261,15 @@ int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream);
  * in place of the reference's run timer (src/Simulation/Simulation.cpp:910-914,1162-1171). */
 int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, double *ms, prandtl_stream stream);
 
+/* Launch timing inside a caller's own loop (instrumentation, like prandtl_op_profile; the reference
+ * times whole runs, Simulation.cpp:910-914).  While enabled, every lifting pass (kind 0) and
+ * flux-differencing pass (kind 1) this handle launches is bracketed by HIP events on its stream.
+ * prandtl_op_timing_read waits for the recorded launches and returns each kind's average duration
+ * in ms (ms[2]) and, when count is not NULL, the number of launches (count[2]); it then forgets
+ * them.  Not for use under HIP graph capture. */
+int prandtl_op_timing(prandtl_op op, int enable);
+int prandtl_op_timing_read(prandtl_op op, double *ms, long long *count);
+
 /* Device memory and copies for host programs that do not use HIP themselves (the C++ mirror
  * in prandtl_amd.hpp).  No counterpart in the reference, whose vectors live in host memory.
  * prandtl_memcpy is synchronous; kind is one of PRANDTL_COPY_*. */

@@ -31,7 +31,8 @@ EXPORTED = [
     "prandtl_op_profile", "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_last_error",
     "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
     "prandtl_device_alloc", "prandtl_device_free", "prandtl_memcpy", "prandtl_stream_synchronize",
-    "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc",
+    "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc", "prandtl_op_timing",
+    "prandtl_op_timing_read",
 ]
 ALPHA_MAX, ALPHA_MIN = 0.5, 0.001   # the reference's blending defaults
 
@@ -104,6 +105,8 @@ def load():
         L.prandtl_op_profile.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, dp, vp]
         L.prandtl_op_vjp.argtypes = [vp, vp, vp, vp, vp]
         L.prandtl_op_vjp_noalloc.argtypes = [vp, vp, vp, vp, vp]
+        L.prandtl_op_timing.argtypes = [vp, ctypes.c_int]
+        L.prandtl_op_timing_read.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.prandtl_rk_run_tape.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.prandtl_rk_adjoint.argtypes = [vp, vp, vp]
         L.prandtl_rk_tape_id.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
@@ -118,7 +121,8 @@ def load():
                   "prandtl_op_max_char_speed", "prandtl_rk_step", "prandtl_rk_run", "prandtl_op_profile",
                   "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_filter_constraints",
                   "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
-                  "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc"):
+                  "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc",
+                  "prandtl_op_timing", "prandtl_op_timing_read"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -229,6 +233,17 @@ class Operator:
         ms = (ctypes.c_double * 2)()
         _check(load().prandtl_op_profile(self._h, _ptr(u, self.size, "u"), int(epilogue), int(iters), ms, _stream(stream)))
         return float(ms[0]), float(ms[1])
+
+    def timing(self, enable: bool):
+        """Bracket every lifting / flux-differencing launch of this handle with HIP events."""
+        _check(load().prandtl_op_timing(self._h, int(bool(enable))))
+
+    def timing_read(self):
+        """((lift ms, flux ms) average durations, (lift launches, flux launches)) since the last read."""
+        ms = (ctypes.c_double * 2)()
+        n = (ctypes.c_longlong * 2)()
+        _check(load().prandtl_op_timing_read(self._h, ms, n))
+        return (float(ms[0]), float(ms[1])), (int(n[0]), int(n[1]))
 
     def rk_step(self, scheme, u, t, dt, stream=None) -> float:
         tt = ctypes.c_double(t)

@@ -90,6 +90,12 @@ struct prandtl_op_s {
     // slot 0 points at the segment's checkpoint) and the double-buffered adjoint registers
     std::vector<double *> adj_stage;
     double *adj_ub2 = nullptr, *adj_kb = nullptr, *adj_kb2 = nullptr;
+    // launch timing (prandtl_op_timing): HIP events around every lifting (kind 0) and
+    // flux-differencing (kind 1) launch while enabled; pool reused between reads
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    size_t tev_used = 0;
+    std::vector<int> tkind;   // kind of event pair i (events 2i, 2i+1)
 };
 
 namespace prandtl {
@@ -277,7 +283,12 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     // Navier-Stokes: one thread per node (the node's state stays in registers through the
     // fused epilogue, no LDS accumulator: 4 waves/SIMD); Euler: one thread per GLL line
     // (each symmetric pair flux evaluated once).  Chosen by measurement at TGV 32^3, p=3.
-    if constexpr (VISC || MIX) {
+    // Euler on small meshes (fewer line-kernel workgroups than two per CU): the node kernel, whose
+    // grid is Np times larger and whose threads each carry less of the work (the launch is latency-
+    // bound there, not throughput-bound)
+    constexpr int kSmallGrid = 512;
+    const bool small = (op->NE + LineCfg<DIM, NP>::EPB - 1) / LineCfg<DIM, NP>::EPB < kSmallGrid;
+    if (VISC || MIX || small) {
         using LC = NodeCfg<DIM, NP>;
         const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
         op->lam_waves = nblk * LC::BLOCK / 64;

@@ -123,8 +123,42 @@ int dispatch_np(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     return fail(PRANDTL_ERR_UNSUPPORTED, "order out of range");
 }
 
+// the next pair of timing events (created on first use)
+int timing_events(prandtl_op_s *op, int kind, hipEvent_t &a, hipEvent_t &b) {
+    while (op->tev.size() < op->tev_used + 2) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        op->tev.push_back(e);
+    }
+    a = op->tev[op->tev_used];
+    b = op->tev[op->tev_used + 1];
+    op->tev_used += 2;
+    op->tkind.push_back(kind);
+    return PRANDTL_OK;
+}
+
+int dispatch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
+    return op->dim == 2 ? dispatch_np<2>(op, S, st) : dispatch_np<3>(op, S, st);
+}
+
 int run_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
-    const int rc = op->dim == 2 ? dispatch_np<2>(op, S, st) : dispatch_np<3>(op, S, st);
+    int rc;
+    if (op->timing && S.phases == 3) {
+        // each pass between its own pair of events
+        rc = PRANDTL_OK;
+        for (int ph = op->phys.viscous ? 0 : 1; ph < 2 && rc == PRANDTL_OK; ph++) {
+            hipEvent_t a, b;
+            rc = timing_events(op, ph, a, b);
+            if (rc) break;
+            StageSpec P = S;
+            P.phases = 1 << ph;
+            HIP_TRY(hipEventRecord(a, st));
+            rc = dispatch_stage(op, P, st);
+            HIP_TRY(hipEventRecord(b, st));
+        }
+    } else {
+        rc = dispatch_stage(op, S, st);
+    }
     if (rc == PRANDTL_OK && S.want_lambda) {
         // the flux kernel left one maximum per wave
         hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, op->d_lam_part, op->lam_waves, op->d_lam);
@@ -899,6 +933,7 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_nan);
     (void)hipFree(op->d_alpha);
     (void)hipFree(op->d_bc);
+    for (hipEvent_t e : op->tev) (void)hipEventDestroy(e);
     for (double *p : op->tape_ckpt) (void)hipFree(p);
     release_adjoint_storage(op);
     delete op;
@@ -991,6 +1026,32 @@ int prandtl_op_profile(prandtl_op op, const double *u, int epilogue, int iters, 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return PRANDTL_OK;
+}
+
+int prandtl_op_timing(prandtl_op op, int enable) {
+    if (!op) return fail(PRANDTL_ERR_ARG, "null argument");
+    op->timing = enable != 0;
+    return PRANDTL_OK;
+}
+
+int prandtl_op_timing_read(prandtl_op op, double *ms, long long *count) {
+    if (!op || !ms) return fail(PRANDTL_ERR_ARG, "null argument");
+    double sum[2] = {0.0, 0.0};
+    long long n[2] = {0, 0};
+    for (size_t i = 0; i < op->tkind.size(); i++) {
+        HIP_TRY(hipEventSynchronize(op->tev[2 * i + 1]));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, op->tev[2 * i], op->tev[2 * i + 1]));
+        sum[op->tkind[i]] += t;
+        n[op->tkind[i]]++;
+    }
+    for (int k = 0; k < 2; k++) {
+        ms[k] = n[k] ? sum[k] / n[k] : 0.0;
+        if (count) count[k] = n[k];
+    }
+    op->tkind.clear();
+    op->tev_used = 0;
     return PRANDTL_OK;
 }
 
