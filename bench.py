@@ -236,6 +236,17 @@ def adjoint_case(dev):
     for c in checks:
         log(f"adjoint {c['param']}: {c['adjoint']:.12e} FD6 {c['fd6']:.12e} rel {c['rel_err']:.2e} "
             f"(FD2 eps=1e-6 {c['fd2_eps1e-6']:.12e}, rel {c['rel_err_fd2']:.2e})")
+    # checkpoint schedules: the segments of the case (every 20 steps) against binomial (revolve)
+    # checkpointing with a few snapshot buffers, on this case and on the NS TGV 64^3 p=3
+    sched = [schedule_cost(op, u0, dt, nsteps, capi.TAPE_SEGMENTS, ckpt, t_fwd, mesh.num_nodes)]
+    for snaps in (2, 4):
+        sched.append(schedule_cost(op, u0, dt, nsteps, capi.TAPE_REVOLVE, snaps, t_fwd, mesh.num_nodes))
+    for c in sched:
+        log(f"adjoint schedule {c['schedule']}: {c['states']} states ({c['state_gb']:.2f} GB), "
+            f"{c['recomputed_steps']} recomputed steps, adjoint/forward {c['adjoint_over_forward']:.2f}")
+    op.close()
+    del op
+    ns = ns_adjoint_schedules(dev)
     out = {
         "metric": "adjoint/forward cost, 3D Euler TGV 16^3 p=4 (200 LSRK4(5) stages, checkpoint every 20 steps)",
         "config": {"workload": f"TGV Euler 16^3 hexes p=4 ({mesh.num_nodes} nodes), Ma=0.1, 64 Fourier modes "
@@ -246,9 +257,61 @@ def adjoint_case(dev):
         "forward_dof_rhs_per_s": mesh.num_nodes * 5 * nsteps / t_fwd,
         "gradient_checks": checks,
         "max_rel_err": max(c["rel_err"] for c in checks),
+        "schedules": sched,
+        "ns_tgv64_schedules": ns,
     }
-    op.close()
     print(json.dumps(out), flush=True)
+
+
+def schedule_cost(op, u0, dt, nsteps, schedule, param, t_fwd, nodes, scheme=1):
+    """One taped run with the given checkpoint schedule and its reverse sweep: the sweep's time over
+    the plain forward run's, the device states the sweep holds and the steps it recomputes."""
+    import torch
+
+    from prandtl_amd import capi
+
+    u = u0.clone()
+    op.rk_run_tape_schedule(u, 0.0, dt, nsteps, schedule, param, scheme=scheme)
+    g = torch.ones_like(u)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.rk_adjoint(g)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    states, rec = op.tape_info()
+    name = ("segments of %d steps" if schedule == capi.TAPE_SEGMENTS else "revolve, %d snapshots") % param
+    return {"schedule": name, "states": states, "state_gb": states * u.numel() * 8 / 1e9,
+            "recomputed_steps": rec, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd}
+
+
+def ns_adjoint_schedules(dev, n=64, nsteps=10):
+    """The NS TGV n^3 p=3 adjoint over nsteps LSRK4(5) steps: segment and revolve schedules."""
+    import torch
+
+    from prandtl_amd import capi, cases
+
+    mesh = cases.tgv_mesh(n=n, order=3)
+    op = capi.Operator(mesh, **cases.tgv_physics())
+    u0 = torch.from_numpy(cases.tgv_state(mesh)).to(dev)
+    dt = cases.cfl_dt(mesh, 11.0, 0.5)
+    u = u0.clone()
+    op.rk_run(capi.LSRK45, u, 0.0, dt, 1)
+    u = u0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.rk_run(capi.LSRK45, u, 0.0, dt, nsteps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    res = [schedule_cost(op, u0, dt, nsteps, capi.TAPE_SEGMENTS, nsteps, t_fwd, mesh.num_nodes)]
+    for snaps in (1, 3):
+        res.append(schedule_cost(op, u0, dt, nsteps, capi.TAPE_REVOLVE, snaps, t_fwd, mesh.num_nodes))
+    for c in res:
+        log(f"NS TGV {n}^3 adjoint, {nsteps} steps, {c['schedule']}: {c['states']} states "
+            f"({c['state_gb']:.1f} GB), {c['recomputed_steps']} recomputed steps, adjoint/forward "
+            f"{c['adjoint_over_forward']:.2f}")
+    op.close()
+    return {"workload": f"TGV Navier-Stokes {n}^3 hexes p=3 ({mesh.num_nodes} nodes), {nsteps} LSRK4(5) steps",
+            "forward_s": t_fwd, "schedules": res}
 
 
 def khi_case(args, dev):

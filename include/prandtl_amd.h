@@ -241,6 +241,25 @@ int prandtl_op_vjp_noalloc(prandtl_op op, const double *u, const double *lam, do
 int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
                         prandtl_stream stream);
 
+/* Checkpoint schedules of a taped run (no counterpart in the reference):
+ *   SEGMENTS: the state at the start of every param-step segment; the reverse sweep recomputes
+ *             each segment once keeping its stage inputs (memory: nsteps/param + 5 param states
+ *             for LSRK4(5), 3 param for SSP-RK3; one forward run recomputed);
+ *   REVOLVE:  Griewank's binomial checkpointing with param snapshot buffers (memory: param + 1
+ *             checkpoints + one step's stage inputs + a work state; the fewest recomputed steps for
+ *             that memory), at most 4096 steps. */
+enum { PRANDTL_TAPE_SEGMENTS = 0, PRANDTL_TAPE_REVOLVE = 1 };
+
+/* prandtl_rk_run_tape with a chosen checkpoint schedule (prandtl_rk_run_tape is SEGMENTS with
+ * param = ckpt_every). */
+int prandtl_rk_run_tape_schedule(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int schedule,
+                                 int param, prandtl_stream stream);
+
+/* The last taped run's memory and recomputation: the states the reverse sweep holds on the device
+ * (checkpoints, stage inputs, work; the three adjoint registers not counted) and the forward steps
+ * the last prandtl_rk_adjoint ran (-1 before it ran). */
+int prandtl_rk_tape_info(prandtl_op op, long long *states, long long *recomputed_steps);
+
 /* Identifier of the last taped run of the handle (no counterpart in the reference): unique over
  * the process's handles; 0 when there is none, when a taped run failed, or while one is being
  * written.  A caller that tapes more than one run on a handle compares it with the identifier read

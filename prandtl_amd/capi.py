@@ -23,6 +23,7 @@ FP64, MIXED = 0, 1
 BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
 SSPRK3, LSRK45, FORWARD_EULER, RK2, RK4 = 0, 1, 2, 3, 4
 HOOK_FILTER, HOOK_LIMITER = 1, 2
+TAPE_SEGMENTS, TAPE_REVOLVE = 0, 1
 
 # every symbol the public header declares
 EXPORTED = [
@@ -32,7 +33,7 @@ EXPORTED = [
     "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
     "prandtl_device_alloc", "prandtl_device_free", "prandtl_memcpy", "prandtl_stream_synchronize",
     "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc", "prandtl_op_timing",
-    "prandtl_op_timing_read",
+    "prandtl_op_timing_read", "prandtl_rk_run_tape_schedule", "prandtl_rk_tape_info",
 ]
 ALPHA_MAX, ALPHA_MIN = 0.5, 0.001   # the reference's blending defaults
 
@@ -105,6 +106,9 @@ def load():
         L.prandtl_op_profile.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, dp, vp]
         L.prandtl_op_vjp.argtypes = [vp, vp, vp, vp, vp]
         L.prandtl_op_vjp_noalloc.argtypes = [vp, vp, vp, vp, vp]
+        L.prandtl_rk_run_tape_schedule.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_int, vp]
+        L.prandtl_rk_tape_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
         L.prandtl_op_timing.argtypes = [vp, ctypes.c_int]
         L.prandtl_op_timing_read.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.prandtl_rk_run_tape.argtypes = [vp, ctypes.c_int, vp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
@@ -122,7 +126,8 @@ def load():
                   "prandtl_op_vjp", "prandtl_rk_run_tape", "prandtl_rk_adjoint", "prandtl_filter_constraints",
                   "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
                   "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc",
-                  "prandtl_op_timing", "prandtl_op_timing_read"):
+                  "prandtl_op_timing", "prandtl_op_timing_read", "prandtl_rk_run_tape_schedule",
+                  "prandtl_rk_tape_info"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -266,6 +271,21 @@ class Operator:
         _check(load().prandtl_rk_run_tape(self._h, int(scheme), _ptr(u, self.size, "u"), ctypes.byref(tt), float(dt), int(nsteps),
                                           int(ckpt_every), _stream(stream)))
         return tt.value
+
+    def rk_run_tape_schedule(self, u, t, dt, nsteps, schedule, param, stream=None, scheme=LSRK45) -> float:
+        """A taped run with checkpoint schedule TAPE_SEGMENTS (param = steps per segment) or TAPE_REVOLVE
+        (param = snapshot buffers)."""
+        tt = ctypes.c_double(t)
+        _check(load().prandtl_rk_run_tape_schedule(self._h, int(scheme), _ptr(u, self.size, "u"), ctypes.byref(tt),
+                                                   float(dt), int(nsteps), int(schedule), int(param),
+                                                   _stream(stream)))
+        return tt.value
+
+    def tape_info(self):
+        """(device states held by the reverse sweep, forward steps the last reverse sweep ran)."""
+        a, b = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        _check(load().prandtl_rk_tape_info(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
 
     def tape_id(self) -> int:
         """Identifier of the handle's last taped run (0: none, or it failed)."""

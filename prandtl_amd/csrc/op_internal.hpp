@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,9 @@ constexpr int MAX_NP = 5;
 
 // GLL rule on [0,1] (prandtl_capi.hip)
 void gll_rule(int np, double *x, double *w);
+
+// binomial checkpointing plan of a taped run (prandtl_capi.hip)
+struct RevolvePlan;
 
 }  // namespace prandtl
 
@@ -89,7 +93,14 @@ struct prandtl_op_s {
     // adjoint sweep storage, kept between calls: the stage inputs of one segment (5 per step;
     // slot 0 points at the segment's checkpoint) and the double-buffered adjoint registers
     std::vector<double *> adj_stage;
-    double *adj_ub2 = nullptr, *adj_kb = nullptr, *adj_kb2 = nullptr;
+    double *adj_ub2 = nullptr, *adj_kb = nullptr, *adj_kb2 = nullptr, *adj_work = nullptr;
+    // checkpoint schedule of the taped run: the steps whose states tape_ckpt holds after the run
+    // (segments: every segment start; revolve: the initial state and the first snapshots), the
+    // schedule and its parameter, the revolve plan, and the forward steps the last reverse sweep ran
+    std::vector<int> tape_marks;
+    int tape_schedule = 0, tape_param = 0;
+    std::shared_ptr<prandtl::RevolvePlan> revolve;
+    long long tape_recomputed = -1;
     // launch timing (prandtl_op_timing): HIP events around every lifting (kind 0) and
     // flux-differencing (kind 1) launch while enabled; pool reused between reads
     bool timing = false;

@@ -7,6 +7,8 @@
 #include "dg_util.hpp"
 
 #include <atomic>
+#include <functional>
+#include <memory>
 #include <mutex>
 
 using namespace prandtl;
@@ -58,9 +60,58 @@ void gll_rule(int np, double *x, double *w) {
     x[N] = 1.0;
 }
 
+
+constexpr int kRevolveMaxSteps = 4096;
+
+// Binomial checkpointing (Griewank, "Achieving logarithmic growth of temporal and spatial
+// complexity in reverse automatic differentiation", 1992; the schedule of Griewank & Walther's
+// revolve): T(n, s) = fewest forward steps to reverse n steps from their start state with s
+// snapshot buffers, T(1, s) = 0, T(n, 0) = n (n - 1) / 2,
+// T(n, s) = min_k [k + T(n - k, s - 1) + T(k, s)] (advance k steps, snapshot, reverse the n - k
+// after it with s - 1 buffers, then the k before it with s); split(n, s) is the minimising k.
+// Each reversed step also recomputes its own stages once (not counted in T).
+struct RevolvePlan {
+    int N = 0, S = 0;
+    std::vector<long long> T;   // T[s][n], s = 0..S, n = 0..N
+    std::vector<int> K;         // argmin k
+    long long cost(int n, int s) const { return T[(size_t)s * (N + 1) + n]; }
+    int split(int n, int s) const { return K[(size_t)s * (N + 1) + n]; }
+    void build(int n_max, int s_max) {
+        N = n_max;
+        S = s_max;
+        T.assign((size_t)(S + 1) * (N + 1), 0);
+        K.assign((size_t)(S + 1) * (N + 1), 1);
+        for (int n = 1; n <= N; n++) T[n] = (long long)n * (n - 1) / 2;
+        for (int s = 1; s <= S; s++) {
+            long long *Ts = &T[(size_t)s * (N + 1)];
+            const long long *Tp = &T[(size_t)(s - 1) * (N + 1)];
+            int *Ks = &K[(size_t)s * (N + 1)];
+            for (int n = 2; n <= N; n++) {
+                long long best = -1;
+                int bk = 1;
+                for (int k = 1; k < n; k++) {
+                    const long long c = k + Tp[n - k] + Ts[k];
+                    if (best < 0 || c < best) best = c, bk = k;
+                }
+                Ts[n] = best;
+                Ks[n] = bk;
+            }
+        }
+    }
+};
+
 }  // namespace prandtl
 
 namespace {
+
+// the plan of the handle's taped run (kept with the handle)
+const RevolvePlan &revolve_plan(prandtl_op_s *op, int nsteps, int snaps) {
+    if (!op->revolve || op->revolve->N < nsteps || op->revolve->S < snaps) {
+        op->revolve = std::make_shared<RevolvePlan>();
+        op->revolve->build(nsteps, std::min(snaps, nsteps));
+    }
+    return *op->revolve;
+}
 
 // identifiers of taped runs, unique over the handles of the process
 unsigned long long next_tape_id() {
@@ -192,7 +243,8 @@ void release_adjoint_storage(prandtl_op_s *op) {
     (void)hipFree(op->adj_ub2);
     (void)hipFree(op->adj_kb);
     (void)hipFree(op->adj_kb2);
-    op->adj_ub2 = op->adj_kb = op->adj_kb2 = nullptr;
+    (void)hipFree(op->adj_work);
+    op->adj_ub2 = op->adj_kb = op->adj_kb2 = op->adj_work = nullptr;
 }
 
 struct VjpSpec {
@@ -1164,41 +1216,73 @@ int prandtl_op_vjp_noalloc(prandtl_op op, const double *u, const double *lam, do
 
 int prandtl_rk_run_tape(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int ckpt_every,
                         prandtl_stream stream) {
+    return prandtl_rk_run_tape_schedule(op, scheme, u, t, dt, nsteps, PRANDTL_TAPE_SEGMENTS, ckpt_every, stream);
+}
+
+int prandtl_rk_run_tape_schedule(prandtl_op op, int scheme, double *u, double *t, double dt, int nsteps, int schedule,
+                                 int param, prandtl_stream stream) {
     if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
     if (scheme != PRANDTL_LSRK45 && scheme != PRANDTL_SSPRK3) return fail(PRANDTL_ERR_ARG, "unknown scheme");
+    if (schedule != PRANDTL_TAPE_SEGMENTS && schedule != PRANDTL_TAPE_REVOLVE)
+        return fail(PRANDTL_ERR_ARG, "unknown checkpoint schedule");
     if (op->opts.stage_hooks)
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the filter or limiter hooks");
     if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
     if (op->nbc || op->d_ngeom) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic affine meshes");
-    if (nsteps <= 0 || ckpt_every <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, checkpoint interval or dt");
+    if (nsteps <= 0 || param <= 0 || !(dt > 0.0)) return fail(PRANDTL_ERR_ARG, "bad steps, schedule parameter or dt");
+    if (schedule == PRANDTL_TAPE_REVOLVE && nsteps > kRevolveMaxSteps)
+        return fail(PRANDTL_ERR_ARG, "the revolve schedule takes at most " + std::to_string(kRevolveMaxSteps) + " steps");
     hipStream_t st = (hipStream_t)stream;
     // the previous tape is gone from here on: a run that fails leaves no tape
     op->tape_id = 0;
     op->tape_steps = 0;
-    const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
-    if ((int)op->tape_ckpt.size() != nseg) {   // checkpoint storage is kept for runs of the same length
+    // checkpoint buffers and the steps whose states the forward run stores in them
+    std::vector<int> marks;
+    if (schedule == PRANDTL_TAPE_SEGMENTS) {
+        for (int k = 0; k * param < nsteps; k++) marks.push_back(k * param);
+    } else {
+        // revolve: the initial state and the first snapshot of every level of the schedule's
+        // rightmost branch (the reverse sweep's first descent), taken on the way
+        marks.push_back(0);
+        const RevolvePlan &plan = revolve_plan(op, nsteps, param);
+        int a = 0, sl = param;
+        while (nsteps - a > 1 && sl > 0) {
+            a += plan.split(nsteps - a, sl);
+            marks.push_back(a);
+            sl--;
+        }
+    }
+    const size_t nbuf = schedule == PRANDTL_TAPE_SEGMENTS ? marks.size() : (size_t)param + 1;
+    if (op->tape_ckpt.size() != nbuf) {
         release_adjoint_storage(op);            // its slot 0 points at a checkpoint
         for (double *p : op->tape_ckpt) (void)hipFree(p);
-        op->tape_ckpt.assign(nseg, nullptr);
-        for (int k = 0; k < nseg; k++)
+        op->tape_ckpt.assign(nbuf, nullptr);
+        for (size_t k = 0; k < nbuf; k++)
             if (hipMalloc(&op->tape_ckpt[k], op->size * sizeof(double)) != hipSuccess) {
                 for (double *p : op->tape_ckpt) (void)hipFree(p);
                 op->tape_ckpt.clear();
                 return fail(PRANDTL_ERR_HIP, "checkpoint storage allocation failed");
             }
     }
-    for (int k = 0; k < nseg; k++) {
-        HIP_TRY(hipMemcpyAsync(op->tape_ckpt[k], u, op->size * sizeof(double), hipMemcpyDeviceToDevice, st));
-        const int n = std::min(ckpt_every, nsteps - k * ckpt_every);
-        int rc = rk_advance(op, scheme, u, dt, n, 0, st);
+    int cur = 0;
+    for (size_t k = 0; k < marks.size(); k++) {
+        int rc = rk_advance(op, scheme, u, dt, marks[k] - cur, 0, st);
         if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(op->tape_ckpt[k], u, op->size * sizeof(double), hipMemcpyDeviceToDevice, st));
+        cur = marks[k];
     }
+    int rc = rk_advance(op, scheme, u, dt, nsteps - cur, 0, st);
+    if (rc) return rc;
+    op->tape_marks = marks;
+    op->tape_schedule = schedule;
+    op->tape_param = param;
     op->tape_steps = nsteps;
-    op->tape_every = ckpt_every;
+    op->tape_every = schedule == PRANDTL_TAPE_SEGMENTS ? param : 1;
     op->tape_dt = dt;
     op->tape_scheme = scheme;
     op->tape_t0 = t ? *t : 0.0;
     op->tape_id = next_tape_id();
+    op->tape_recomputed = -1;
     if (t) *t += dt * nsteps;
     return PRANDTL_OK;
 }
@@ -1208,99 +1292,163 @@ int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
     if (op->tape_ckpt.empty() || op->tape_id == 0) return fail(PRANDTL_ERR_ARG, "no taped run on this operator");
     hipStream_t st = (hipStream_t)stream;
     const size_t bytes = op->size * sizeof(double);
-    const int C = op->tape_every, nsteps = op->tape_steps, nseg = (int)op->tape_ckpt.size();
-    int rc = ensure_rk_buffers(op, PRANDTL_LSRK45);
+    const int C = op->tape_every, nsteps = op->tape_steps;
+    const bool ssp = op->tape_scheme == PRANDTL_SSPRK3;
+    const int per = ssp ? 3 : 5;   // stage inputs kept per step
+    int rc = ensure_rk_buffers(op, op->tape_scheme);
     if (rc) return rc;
-    // stage inputs of one segment (5 per step; slot 0 is the segment's checkpoint itself), and
-    // the adjoint registers (ub, kb double-buffered: a stage reads its neighbours' adjoints)
-    if (op->adj_stage.size() != (size_t)C * 5) {
+    if (!ssp) rc = ensure_rk_buffers(op, PRANDTL_LSRK45);
+    if (rc) return rc;
+    // stage inputs of one block of steps (slot 0 is the block's start state itself), the adjoint
+    // registers (double-buffered: a stage reads its neighbours' adjoints) and, for revolve, a work
+    // state for advancing
+    const size_t nstage = (size_t)C * per;
+    if (op->adj_stage.size() != nstage) {
         release_adjoint_storage(op);
-        op->adj_stage.assign((size_t)C * 5, nullptr);
+        op->adj_stage.assign(nstage, nullptr);
         for (size_t i = 1; i < op->adj_stage.size(); i++)
             if (hipMalloc(&op->adj_stage[i], bytes) != hipSuccess) {
                 release_adjoint_storage(op);
                 return fail(PRANDTL_ERR_HIP, "adjoint stage storage allocation failed");
             }
         if (hipMalloc(&op->adj_ub2, bytes) != hipSuccess || hipMalloc(&op->adj_kb, bytes) != hipSuccess ||
-            hipMalloc(&op->adj_kb2, bytes) != hipSuccess) {
+            hipMalloc(&op->adj_kb2, bytes) != hipSuccess || hipMalloc(&op->adj_work, bytes) != hipSuccess) {
             release_adjoint_storage(op);
             return fail(PRANDTL_ERR_HIP, "adjoint register allocation failed");
         }
     }
     std::vector<double *> &stage = op->adj_stage;
-    double *ub2 = op->adj_ub2, *kb = op->adj_kb, *kb2 = op->adj_kb2;
-    if (op->tape_scheme == PRANDTL_SSPRK3) {
-        // SSP-RK3 (Shu-Osher): y1 = u + dt L(u), y2 = 3/4 u + 1/4 (y1 + dt L(y1)),
-        // u' = 1/3 u + 2/3 (y2 + dt L(y2)).  Reverse, given ub = adjoint of u':
-        //   yb2 = 2/3 (ub + dt J(y2)^T ub),  yb1 = 1/4 (yb2 + dt J(y1)^T yb2),
-        //   ub <- 1/3 ub + 3/4 yb2 + yb1 + dt J(u)^T yb1.
-        rc = ensure_rk_buffers(op, PRANDTL_SSPRK3);
-        if (rc) return rc;
-        const double dt = op->tape_dt;
-        double *U = ubar, *Y2 = kb, *Y1 = kb2, *X = ub2;
-        for (int seg = nseg - 1; seg >= 0 && rc == PRANDTL_OK; seg--) {
-            const int n = std::min(C, nsteps - seg * C);
-            // recompute the segment: stage[3j] = u^j, stage[3j+1] = y1, stage[3j+2] = y2
-            stage[0] = op->tape_ckpt[seg];
-            for (int j = 0; j < n && rc == PRANDTL_OK; j++) {
+    const double dt = op->tape_dt;
+    long long recomputed = 0;
+    // LSRK4(5): adjoint registers (ub, kb); SSP-RK3: (U, Y2, Y1, X)
+    double *ub_cur = ubar, *ub_nxt = op->adj_ub2, *kb_cur = op->adj_kb, *kb_nxt = op->adj_kb2;
+    double *U = ubar, *Y2 = op->adj_kb, *Y1 = op->adj_kb2, *X = op->adj_ub2;
+    if (!ssp) hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, kb_cur, 0.0, op->size);
+    // reverse the n steps that start from state x0 (not modified): recompute them keeping every
+    // stage input, then run the stage adjoints backwards
+    auto reverse_block = [&](double *x0, int n) -> int {
+        int r = PRANDTL_OK;
+        recomputed += n;
+        stage[0] = x0;
+        if (ssp) {
+            // SSP-RK3 (Shu-Osher): y1 = u + dt L(u), y2 = 3/4 u + 1/4 (y1 + dt L(y1)),
+            // u' = 1/3 u + 2/3 (y2 + dt L(y2)).  Reverse, given U = adjoint of u':
+            //   yb2 = 2/3 (U + dt J(y2)^T U),  yb1 = 1/4 (yb2 + dt J(y1)^T yb2),
+            //   U <- 1/3 U + 3/4 yb2 + yb1 + dt J(u)^T yb1.
+            for (int j = 0; j < n && r == PRANDTL_OK; j++) {
                 double *u0 = stage[3 * j];
                 StageSpec S1{EPI_SSP, u0, stage[3 * j + 1], nullptr, u0, 0.0, 1.0, dt, nullptr, false, false};
-                rc = run_stage(op, S1, st);
+                r = run_stage(op, S1, st);
                 StageSpec S2{EPI_SSP, stage[3 * j + 1], stage[3 * j + 2], nullptr, u0, 0.75, 0.25, dt, nullptr,
                              false, false};
-                if (!rc) rc = run_stage(op, S2, st);
-                if (!rc && j + 1 < n) {
+                if (!r) r = run_stage(op, S2, st);
+                if (!r && j + 1 < n) {
                     StageSpec S3{EPI_SSP, stage[3 * j + 2], stage[3 * j + 3], nullptr, u0, 1.0 / 3.0, 2.0 / 3.0, dt,
                                  nullptr, false, false};
-                    rc = run_stage(op, S3, st);
+                    r = run_stage(op, S3, st);
                 }
             }
-            for (int j = n - 1; j >= 0 && rc == PRANDTL_OK; j--) {
+            for (int j = n - 1; j >= 0 && r == PRANDTL_OK; j--) {
                 VjpSpec Va{VJP_SSP, stage[3 * j + 2], U, nullptr, nullptr, Y2, nullptr, 0.0, 2.0 / 3.0, dt, nullptr};
-                rc = run_vjp(op, Va, st);
+                r = run_vjp(op, Va, st);
                 VjpSpec Vb{VJP_SSP, stage[3 * j + 1], Y2, nullptr, nullptr, Y1, nullptr, 0.0, 0.25, dt, nullptr};
-                if (!rc) rc = run_vjp(op, Vb, st);
-                if (!rc) {
+                if (!r) r = run_vjp(op, Vb, st);
+                if (!r) {
                     hipLaunchKernelGGL(k_axpby, dim3(2048), dim3(256), 0, st, X, 1.0 / 3.0, (const double *)U, 0.75,
                                        (const double *)Y2, op->size);
                     VjpSpec Vc{VJP_SSP, stage[3 * j], Y1, nullptr, nullptr, X, nullptr, 0.0, 1.0, dt, X};
-                    rc = run_vjp(op, Vc, st);
+                    r = run_vjp(op, Vc, st);
                 }
                 std::swap(U, X);
             }
+            return r;
         }
-        if (rc == PRANDTL_OK && U != ubar) (void)hipMemcpyAsync(ubar, U, bytes, hipMemcpyDeviceToDevice, st);
-        hipError_t e = hipStreamSynchronize(st);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(PRANDTL_ERR_HIP, hipGetErrorString(e));
-        return PRANDTL_OK;
-    }
-    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, kb, 0.0, op->size);
-    double *ub_cur = ubar, *ub_nxt = ub2, *kb_cur = kb, *kb_nxt = kb2;
-    for (int seg = nseg - 1; seg >= 0 && rc == PRANDTL_OK; seg--) {
-        const int n = std::min(C, nsteps - seg * C);
-        // recompute the segment forward from its checkpoint, keeping every stage input (the
-        // last stage's output, the segment's end state, is not needed)
-        stage[0] = op->tape_ckpt[seg];
-        for (int j = 0; j < 5 * n - 1 && rc == PRANDTL_OK; j++) {
+        // LSRK4(5): recompute the stage inputs (the last stage's output is not needed)
+        for (int j = 0; j < 5 * n - 1 && r == PRANDTL_OK; j++) {
             const int stg = j % 5;
-            StageSpec S{EPI_LSRK, stage[j], stage[j + 1], op->d_k, nullptr, CK_A[stg], CK_B[stg], op->tape_dt,
-                        nullptr, false, false};
-            rc = run_stage(op, S, st);
+            StageSpec S{EPI_LSRK, stage[j], stage[j + 1], op->d_k, nullptr, CK_A[stg], CK_B[stg], dt, nullptr, false,
+                        false};
+            r = run_stage(op, S, st);
         }
-        // reverse through the segment's stages
-        for (int j = 5 * n - 1; j >= 0 && rc == PRANDTL_OK; j--) {
+        for (int j = 5 * n - 1; j >= 0 && r == PRANDTL_OK; j--) {
             const int stg = j % 5;
-            VjpSpec V{VJP_LSRK, stage[j], nullptr, ub_cur, kb_cur, ub_nxt, kb_nxt, CK_A[stg], CK_B[stg], op->tape_dt};
-            rc = run_vjp(op, V, st);
+            VjpSpec V{VJP_LSRK, stage[j], nullptr, ub_cur, kb_cur, ub_nxt, kb_nxt, CK_A[stg], CK_B[stg], dt};
+            r = run_vjp(op, V, st);
             std::swap(ub_cur, ub_nxt);
             std::swap(kb_cur, kb_nxt);
         }
+        return r;
+    };
+    if (op->tape_schedule == PRANDTL_TAPE_SEGMENTS) {
+        const int nseg = (int)op->tape_ckpt.size();
+        for (int seg = nseg - 1; seg >= 0 && rc == PRANDTL_OK; seg--)
+            rc = reverse_block(op->tape_ckpt[seg], std::min(C, nsteps - seg * C));
+    } else {
+        // Griewank's binomial schedule: reverse(a, b, s) reverses steps [a, b) from the state at a
+        // with s free snapshot buffers -- a snapshot at the optimal split m, the steps after it
+        // with s-1 buffers, then the steps before it with s
+        const RevolvePlan &plan = revolve_plan(op, nsteps, op->tape_param);
+        std::vector<double *> pool(op->tape_ckpt.begin() + 1, op->tape_ckpt.end());
+        std::vector<int> taken(op->tape_marks.begin() + 1, op->tape_marks.end());   // pool[i] holds step taken[i]
+        taken.resize(pool.size(), -1);
+        std::function<int(int, int, int, double *)> rev = [&](int a, int b, int sl, double *xa) -> int {
+            const int n = b - a;
+            if (n == 1) return reverse_block(xa, 1);
+            if (sl == 0) {
+                // no buffer left: advance from a to every step, last first
+                for (int j = b - 1; j >= a; j--) {
+                    int r;
+                    if (j == a) {
+                        r = reverse_block(xa, 1);
+                    } else {
+                        HIP_TRY(hipMemcpyAsync(op->adj_work, xa, bytes, hipMemcpyDeviceToDevice, st));
+                        r = rk_advance(op, op->tape_scheme, op->adj_work, dt, j - a, 0, st);
+                        recomputed += j - a;
+                        if (!r) r = reverse_block(op->adj_work, 1);
+                    }
+                    if (r) return r;
+                }
+                return PRANDTL_OK;
+            }
+            const int m = a + plan.split(n, sl);
+            int slot = -1;
+            for (size_t i = 0; i < pool.size(); i++)
+                if (taken[i] == m) slot = (int)i;
+            if (slot < 0) {
+                for (size_t i = 0; i < pool.size() && slot < 0; i++)
+                    if (taken[i] < 0) slot = (int)i;
+                if (slot < 0) return fail(PRANDTL_ERR_HIP, "revolve: no free snapshot buffer");
+                HIP_TRY(hipMemcpyAsync(pool[slot], xa, bytes, hipMemcpyDeviceToDevice, st));
+                const int r = rk_advance(op, op->tape_scheme, pool[slot], dt, m - a, 0, st);
+                if (r) return r;
+                recomputed += m - a;
+                taken[slot] = m;
+            }
+            int r = rev(m, b, sl - 1, pool[slot]);
+            taken[slot] = -1;
+            if (r) return r;
+            return rev(a, m, sl, xa);
+        };
+        rc = rev(0, nsteps, op->tape_param, op->tape_ckpt[0]);
     }
-    if (rc == PRANDTL_OK && ub_cur != ubar) (void)hipMemcpyAsync(ubar, ub_cur, bytes, hipMemcpyDeviceToDevice, st);
+    double *res = ssp ? U : ub_cur;
+    if (rc == PRANDTL_OK && res != ubar) (void)hipMemcpyAsync(ubar, res, bytes, hipMemcpyDeviceToDevice, st);
     hipError_t e = hipStreamSynchronize(st);
     if (rc) return rc;
     if (e != hipSuccess) return fail(PRANDTL_ERR_HIP, hipGetErrorString(e));
+    op->tape_recomputed = recomputed;
+    return PRANDTL_OK;
+}
+
+int prandtl_rk_tape_info(prandtl_op op, long long *states, long long *recomputed_steps) {
+    if (!op) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (op->tape_id == 0) return fail(PRANDTL_ERR_ARG, "no taped run on this operator");
+    const int per = op->tape_scheme == PRANDTL_SSPRK3 ? 3 : 5;
+    // checkpoints + the stage inputs of one block (its first is a checkpoint) + the work state
+    const long long held = (long long)op->tape_ckpt.size() + (long long)op->tape_every * per - 1 +
+                           (op->tape_schedule == PRANDTL_TAPE_REVOLVE ? 1 : 0);
+    if (states) *states = held;
+    if (recomputed_steps) *recomputed_steps = op->tape_recomputed;
     return PRANDTL_OK;
 }
 
