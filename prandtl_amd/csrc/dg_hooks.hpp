@@ -42,12 +42,6 @@
 
 namespace prandtl {
 
-constexpr double kFilterRhoMin = 1e-8;   // EntropyFilter.hpp:59
-constexpr double kFilterPMin = 1e-8;     // :60
-constexpr double kFilterEntTol = 1e-4;   // :61
-constexpr double kFilterEpsilon = 1e-10; // :62 (zeta_max = -ln epsilon)
-constexpr int kFilterNIter = 20;         // :63
-constexpr double kLimiterEps = 1e-8;
 
 template <int NP>
 struct HookTables {
@@ -72,20 +66,11 @@ struct HookArgs {
     unsigned int *nlist_next;      // the other counter of this hook: cleared by the scan
     const double *ngeom;           // general mesh: per-node (det J, adj J) -- the limiter's mean is
                                    // sum J w u / sum J w there; null on affine elements (J cancels)
+    int inline_limit;              // k_filter: apply the limiter to a filtered element that is
+                                   // still inadmissible (the stage-fused hooks: no limiter pass)
+    int track_emin;                // k_filter, k_limiter: rewrite emin of the elements they change
 };
 
-template <int DIM>
-__device__ __forceinline__ double hook_pressure(const double *s, double gammaM1) {
-    double v2 = 0.0;
-#pragma unroll
-    for (int d = 1; d <= DIM; d++) v2 += (s[d] / s[0]) * (s[d] / s[0]);
-    return gammaM1 * (s[DIM + 1] - 0.5 * s[0] * v2);
-}
-
-// rho ln(p rho^-gamma) for rho, p > 0, else +inf (BasicOperations.cpp:39-46)
-__device__ __forceinline__ double hook_entropy(double rho, double p, double gamma) {
-    return (rho > 0.0 && p > 0.0) ? rho * (log(p) - gamma * log(rho)) : __builtin_inf();
-}
 
 template <int DIM>
 __device__ __forceinline__ bool filter_violates(const double *s, double gamma, double gammaM1, double eb) {
@@ -331,6 +316,7 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
         start = first + 1;
     }
     // reconstruction (EntropyFilter.cpp:214-238)
+    double sf[NEQ];
     if (node_ok) {
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
@@ -338,46 +324,36 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
 #pragma unroll
             for (int j = 1; j < NP; j++) a += H[j][v] * filter_sigma<NP>(zeta, j);
             A.u[v * vs + (size_t)e * NN + i] = a;
+            sf[v] = a;
         }
     }
     if (i == 0) atomicAdd(A.count, 1ull);
     __syncthreads();   // sU is rewritten for the next element
+    if (A.inline_limit) {
+        // the limiter's trigger on the filtered state (elements the filter did not visit pass its
+        // detection, hence rho, p >= 1e-8: only filtered elements can need the limiter)
+        const int bad = node_ok && (sf[0] < kLimiterEps || hook_pressure<DIM>(sf, A.gammaM1) < kLimiterEps);
+        if (__syncthreads_or(bad)) limit_element<DIM, NP, NN>(A, e, i, sf, sU, sT[0], sT[1], sT[2]);
+    }
+    if (A.track_emin) element_emin<DIM, NP, NN>(A, e, i, sf, sU[0]);
     }
 }
 
-// Zhang-Shu limiter: one element per workgroup at a time, one thread per node; the workgroups
-// walk the list of elements with a node below the bounds (admissible elements are untouched)
-template <int DIM, int NP>
-__global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
-    constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
-    static_assert(NN <= 128, "one element per workgroup of at most 128 threads");
-    __shared__ double sR[NEQ][NN];
-    __shared__ double sMean[NEQ];
-    __shared__ double s_rmin, s_th2;
-    __shared__ double sW[NN];
-    const int i = threadIdx.x;
+// The Zhang-Shu limiter of element e in place: node i (< NN) holds its state in s; the workgroup
+// calls this together (it synchronises).  sR (NEQ x NN), sMean (NEQ), sW (NN), sScal (2) are LDS.
+template <int DIM, int NP, int NN>
+__device__ void limit_element(const HookArgs<NP> &A, int e, int i, double *s, double (*sR)[NN], double *sMean,
+                              double *sW, double *sScal) {
+    constexpr int NEQ = DIM + 2;
     const bool node_ok = i < NN;
     const size_t vs = (size_t)A.NE * NN;
     const double eps = kLimiterEps;
-    // the scan before this hook (on the same stream) started the counter at zero, so it holds at
-    // most NE; more means the hooks of this operator ran out of order: report, change nothing
-    unsigned nlist = *A.nlist;
-    if (nlist > (unsigned)A.NE) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) A.count[2] = 1ull;
-        nlist = 0u;
-    }
-    for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
-    const int e = A.list[li];
-    double s[NEQ];
     if (node_ok) {
         double w = A.T.w[i % NP] * A.T.w[(i / NP) % NP] * (DIM == 3 ? A.T.w[i / (NP * NP)] : 1.0);
         if (A.ngeom) w *= A.ngeom[((size_t)e * NN + i) * (1 + DIM * DIM)];
         sW[i] = w;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-            s[v] = A.u[v * vs + (size_t)e * NN + i];
-            sR[v][i] = w * s[v];
-        }
+        for (int v = 0; v < NEQ; v++) sR[v][i] = w * s[v];
     }
     __syncthreads();
     if (i < NEQ) {
@@ -396,16 +372,17 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
     if (i == 0) {
         double r = sR[0][0];
         for (int k = 1; k < NN; k++) r = fmin(r, sR[0][k]);
-        s_rmin = r;
-        s_th2 = 1.0;
+        sScal[0] = r;     // rho_min
+        sScal[1] = 1.0;   // theta_2
     }
     __syncthreads();
     double mean[NEQ];
 #pragma unroll
     for (int v = 0; v < NEQ; v++) mean[v] = sMean[v];
     // density (theta_1)
-    if (s_rmin < eps && node_ok) {
-        const double th = fmax(0.0, fmin(1.0, (mean[0] - eps) / (mean[0] - s_rmin)));
+    const double rmin = sScal[0];
+    if (rmin < eps && node_ok) {
+        const double th = fmax(0.0, fmin(1.0, (mean[0] - eps) / (mean[0] - rmin)));
         s[0] = mean[0] + th * (s[0] - mean[0]);
     }
     // pressure (theta_2 = min_i t_i)
@@ -421,10 +398,10 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
                 hi = mid;
         }
         // fmin over the workgroup through the ordered bit pattern of a non-negative double
-        atomicMin(reinterpret_cast<unsigned long long *>(&s_th2), (unsigned long long)__double_as_longlong(lo));
+        atomicMin(reinterpret_cast<unsigned long long *>(&sScal[1]), (unsigned long long)__double_as_longlong(lo));
     }
     __syncthreads();
-    const double th2 = s_th2;
+    const double th2 = sScal[1];
     if (node_ok) {
         if (th2 < 1.0)
 #pragma unroll
@@ -433,7 +410,52 @@ __global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
         for (int v = 0; v < NEQ; v++) A.u[v * vs + (size_t)e * NN + i] = s[v];
     }
     if (i == 0) atomicAdd(A.count + 1, 1ull);
-    __syncthreads();   // sR, sMean, s_rmin, s_th2 are rewritten for the next element
+    __syncthreads();   // the LDS scratch is rewritten by the caller's next use
+}
+
+// The minimum nodal entropy of element e (node i holds its state in s) into A.emin[e], with the
+// scan's starting value DBL_MAX; sR (NN) is LDS scratch; the workgroup calls this together.
+template <int DIM, int NP, int NN>
+__device__ void element_emin(const HookArgs<NP> &A, int e, int i, const double *s, double *sR) {
+    if (i < NN) sR[i] = hook_entropy(s[0], hook_pressure<DIM>(s, A.gammaM1), A.gamma);
+    __syncthreads();
+    if (i == 0) {
+        double m = 1.7976931348623157e308;
+        for (int k = 0; k < NN; k++) m = fmin(m, sR[k]);
+        A.emin[e] = m;
+    }
+    __syncthreads();
+}
+
+// Zhang-Shu limiter: one element per workgroup at a time, one thread per node; the workgroups
+// walk the list of elements with a node below the bounds (admissible elements are untouched)
+template <int DIM, int NP>
+__global__ void __launch_bounds__(128) k_limiter(const HookArgs<NP> A) {
+    constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
+    static_assert(NN <= 128, "one element per workgroup of at most 128 threads");
+    __shared__ double sR[NEQ][NN];
+    __shared__ double sMean[NEQ];
+    __shared__ double sScal[2];
+    __shared__ double sW[NN];
+    const int i = threadIdx.x;
+    const bool node_ok = i < NN;
+    const size_t vs = (size_t)A.NE * NN;
+    // the detection before this hook (on the same stream) started the counter at zero, so it holds
+    // at most NE; more means the hooks of this operator ran out of order: report, change nothing
+    unsigned nlist = *A.nlist;
+    if (nlist > (unsigned)A.NE) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) A.count[2] = 1ull;
+        nlist = 0u;
+    }
+    for (unsigned li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int e = A.list[li];
+        double s[NEQ];
+        if (node_ok) {
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + i];
+        }
+        limit_element<DIM, NP, NN>(A, e, i, s, sR, sMean, sW, sScal);
+        if (A.track_emin) element_emin<DIM, NP, NN>(A, e, i, s, sW);
     }
 }
 

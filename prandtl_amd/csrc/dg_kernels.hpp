@@ -119,7 +119,19 @@ struct RhsArgs {
     BlendParams<NP> blend; // the same: Persson-Peraire indicator and blending constants
     const double *ngeom;   // general mesh (k_rhs_node<..., GEN = true>): per-node (det J, adj J) [e][node][1+dim^2]
     const int *forient;    // general mesh: face orientation codes [e][2 dim]
+    // stage hooks detected in the epilogue of a stage update (k_rhs_node, elements within a wave):
+    int hook_mode;         // HOOK_DETECT_* bits
+    const double *hbound;  // the entropy filter's bound per element
+    double *hemin;         // minimum nodal entropy of each element of the stage output
+    int *hlist_f;          // elements the filter must visit, and their counter (the other counter,
+    unsigned *hcnt_f;      // hcnt_f_next, is cleared for the next application)
+    unsigned *hcnt_f_next;
+    int *hlist_l;          // elements the limiter must visit (the filter off), and counters
+    unsigned *hcnt_l;
+    unsigned *hcnt_l_next;
 };
+
+enum HookDetect { HOOK_DETECT_FILTER = 1, HOOK_DETECT_LIMITER = 2, HOOK_DETECT_EMIN = 4 };
 
 template <int NP>
 struct LiftArgs {
@@ -202,26 +214,97 @@ __device__ __forceinline__ int face_point_map(int code, int t) {
 // Stage epilogue: writes the RHS (Mult) or applies the fused RK stage update to one value;
 // x is the RK register (LSRK) or the base state of the Shu-Osher combination (SSP).
 template <int NP>
-__device__ __forceinline__ void stage_update_x(const RhsArgs<NP> &A, size_t gi, double u_in, double du, double x, int &bad) {
+__device__ __forceinline__ double stage_update_x(const RhsArgs<NP> &A, size_t gi, double u_in, double du, double x,
+                                                 int &bad) {
     if (A.mode == EPI_MULT) {
         A.out[gi] = du;
+        return du;
     } else if (A.mode == EPI_LSRK) {
         const double kk = A.ca * x + A.dt * du;
         A.k[gi] = kk;
         const double un = u_in + A.cb * kk;
         A.out[gi] = un;
         bad |= !(un == un);
+        return un;
     } else {
         const double un = A.ca * x + A.cb * (u_in + A.dt * du);
         A.out[gi] = un;
         bad |= !(un == un);
+        return un;
     }
 }
 
 template <int NP>
-__device__ __forceinline__ void stage_update(const RhsArgs<NP> &A, size_t gi, double u_in, double du, int &bad) {
+__device__ __forceinline__ double stage_update(const RhsArgs<NP> &A, size_t gi, double u_in, double du, int &bad) {
     const double x = A.mode == EPI_LSRK ? A.k[gi] : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
-    stage_update_x<NP>(A, gi, u_in, du, x, bad);
+    return stage_update_x<NP>(A, gi, u_in, du, x, bad);
+}
+
+// Stage hooks detected on the stage output s (one node per lane, an element's NN nodes in
+// consecutive lanes of one wave, its first node at `node` 0): the entropy filter's detection
+// against the element's bound (EntropyFilter.cpp:183-191), the limiter's trigger, the minimum nodal
+// entropy (the next step's filter constraints when this is a step's last stage).  Per element:
+// a segmented tree reduction over its lanes; the elements found are appended to the hooks' lists
+// with one atomic per wave.  Every lane of the wave calls this.
+template <int DIM, int NP, int NN>
+__device__ __forceinline__ void stage_hook_detect(const RhsArgs<NP> &A, bool active, int e, int node,
+                                                  const double *s) {
+    const int lane = threadIdx.x & 63;
+    double ent = __builtin_inf();
+    int fv = 0, lv = 0;
+    if (active) {
+        const double p = hook_pressure<DIM>(s, A.P.gammaM1);
+        if (A.hook_mode & HOOK_DETECT_EMIN) {
+            ent = hook_entropy(s[0], p, A.P.gamma);
+            if (A.hook_mode & HOOK_DETECT_FILTER)
+                fv = (s[0] < kFilterRhoMin || p < kFilterPMin || ent < A.hbound[e] - kFilterEntTol) ? 1 : 0;
+        } else if (A.hook_mode & HOOK_DETECT_FILTER) {
+            fv = (s[0] < kFilterRhoMin || p < kFilterPMin) ? 1 : 0;
+            if (!fv) {
+                // the entropy test: an FP32 estimate decides wherever it lies clearly above the
+                // bound (rho, p > 0 here); near it, the FP64 entropy decides, as the scan does
+                const double thr = A.hbound[e] - kFilterEntTol;
+                const float rf = (float)s[0], pf = (float)p;
+                const float lr = __builtin_amdgcn_logf(rf) * 0.69314718f, lp = __builtin_amdgcn_logf(pf) * 0.69314718f;
+                const double est = (double)rf * ((double)lp - A.P.gamma * (double)lr);
+                const double err = 4e-6 * s[0] * (1.0 + fabs((double)lp) + A.P.gamma * fabs((double)lr));
+                if (!(est - err >= thr)) fv = hook_entropy(s[0], p, A.P.gamma) < thr ? 1 : 0;
+            }
+        }
+        if (A.hook_mode & HOOK_DETECT_LIMITER) lv = (s[0] < kLimiterEps || p < kLimiterEps) ? 1 : 0;
+    }
+#pragma unroll
+    for (int off = 1; off < NN; off *= 2) {
+        const double eo = __shfl(ent, lane + off, 64);
+        const int fo = __shfl(fv, lane + off, 64);
+        const int lo = __shfl(lv, lane + off, 64);
+        if (node + off < NN) {
+            ent = fmin(ent, eo);
+            fv |= fo;
+            lv |= lo;
+        }
+    }
+    const bool head = active && node == 0;
+    if (head && (A.hook_mode & HOOK_DETECT_EMIN)) A.hemin[e] = fmin(1.7976931348623157e308, ent);   // DBL_MAX start
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the counters the previous application of each hook read are free for the next one
+        if (A.hook_mode & HOOK_DETECT_FILTER) *A.hcnt_f_next = 0u;
+        if (A.hook_mode & HOOK_DETECT_LIMITER) *A.hcnt_l_next = 0u;
+    }
+    auto append = [&](bool mine, int *list, unsigned *cnt) {
+        const unsigned long long m = __ballot(mine);
+        if (m == 0ull) return;
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned base = 0u;
+        if (lane == leader) base = atomicAdd(cnt, (unsigned)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (mine) {
+            const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+            if (slot < (unsigned)A.NE) list[slot] = e;
+        }
+    };
+    if (A.hook_mode & HOOK_DETECT_FILTER) append(head && fv, A.hlist_f, A.hcnt_f);
+    if (A.hook_mode & HOOK_DETECT_LIMITER) append(head && lv, A.hlist_l, A.hcnt_l);
 }
 
 template <int NP>
@@ -962,6 +1045,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             }
         }
     }
+    double snew[NEQ];   // the node's stage output (stage hooks)
     if (active) {
         const R sw = (R)(A.T.inv_w0 * invJ);
         const R rinvJ = (R)invJ;
@@ -998,10 +1082,14 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             // the RK accumulation is FP64 in both modes
             const size_t gi = v * vs + (size_t)e * NN + node;
             if constexpr (PRE)
-                stage_update_x<NP>(A, gi, own_state(v, eb, node), dud, pre_x[v], bad);
+                snew[v] = stage_update_x<NP>(A, gi, own_state(v, eb, node), dud, pre_x[v], bad);
             else
-                stage_update<NP>(A, gi, own_state(v, eb, node), dud, bad);
+                snew[v] = stage_update<NP>(A, gi, own_state(v, eb, node), dud, bad);
         }
+    }
+    if constexpr (C::BLOCK == 64 || NN == 64) {
+        // an element's nodes lie in one wave: the hooks' detection from the registers
+        if (A.hook_mode) stage_hook_detect<DIM, NP, NN>(A, active, e, node, snew);
     }
     finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
 }

@@ -233,6 +233,30 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
 }
 
 // ---------------------------------------------------------------------------
+// Stage-hook pointwise tests (the entropy filter's detection and the positivity limiter's
+// trigger), shared by the hook kernels and the stage epilogue that detects for them.
+// ---------------------------------------------------------------------------
+constexpr double kFilterRhoMin = 1e-8;   // EntropyFilter.hpp:59
+constexpr double kFilterPMin = 1e-8;     // :60
+constexpr double kFilterEntTol = 1e-4;   // :61
+constexpr double kFilterEpsilon = 1e-10; // :62 (zeta_max = -ln epsilon)
+constexpr int kFilterNIter = 20;         // :63
+constexpr double kLimiterEps = 1e-8;
+
+template <int DIM>
+__device__ __forceinline__ double hook_pressure(const double *s, double gammaM1) {
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 1; d <= DIM; d++) v2 += (s[d] / s[0]) * (s[d] / s[0]);
+    return gammaM1 * (s[DIM + 1] - 0.5 * s[0] * v2);
+}
+
+// rho ln(p rho^-gamma) for rho, p > 0, else +inf (BasicOperations.cpp:39-46)
+__device__ __forceinline__ double hook_entropy(double rho, double p, double gamma) {
+    return (rho > 0.0 && p > 0.0) ? rho * (log(p) - gamma * log(rho)) : __builtin_inf();
+}
+
+// ---------------------------------------------------------------------------
 // Mixed precision (PRANDTL_MIXED).  The logarithmic means, the reciprocals of the sums
 // they share and p-hat stay FP64; the rest of the two-point and surface fluxes is FP32.
 // The momentum fluxes carry p-hat - p_ref with p_ref a constant of the element whose

@@ -71,6 +71,7 @@ struct prandtl_op_s {
     int *d_hflag = nullptr;       // stage hooks: per-element trigger flags
     unsigned long long *d_hcount = nullptr;   // stage hooks: elements filtered, limited
     int *d_hlist = nullptr;       // stage hooks: the elements a hook applies to
+    int *d_hlist2 = nullptr;      // stage-fused hooks: the limiter's list (the filter's is d_hlist)
     unsigned int *d_hlistn = nullptr;         // their counters: two per hook (filter, limiter)
     int hook_par[2] = {0, 0};     // which of its two counters a hook's next scan fills
     bool have_bounds = false;     // filter constraints computed
@@ -169,6 +170,7 @@ struct StageSpec {
     bool want_lambda;
     bool nan_check;
     int phases = 3;      // bit 0: k_lift (NS), bit 1: k_rhs
+    int hook_mode = 0;   // HOOK_DETECT_* bits: stage hooks detected in the epilogue
 };
 
 template <int DIM, int NP, bool VISC, int SURF, bool MIX>
@@ -237,6 +239,18 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.bc = op->d_bc;
     ra.ngeom = op->d_ngeom;
     ra.forient = op->d_forient;
+    ra.hook_mode = S.hook_mode;
+    ra.hbound = op->d_fbound;
+    ra.hemin = op->d_emin;
+    ra.hlist_f = op->d_hlist;
+    ra.hlist_l = op->d_hlist2;
+    ra.hcnt_f = ra.hcnt_f_next = ra.hcnt_l = ra.hcnt_l_next = nullptr;
+    if (S.hook_mode) {
+        ra.hcnt_f = op->d_hlistn + op->hook_par[0];
+        ra.hcnt_f_next = op->d_hlistn + (1 - op->hook_par[0]);
+        ra.hcnt_l = op->d_hlistn + 2 + op->hook_par[1];
+        ra.hcnt_l_next = op->d_hlistn + 2 + (1 - op->hook_par[1]);
+    }
     if constexpr (!MIX) {
         if (op->d_ngeom) {
             // general mesh: per-node metrics and oriented faces (boundary faces included)
@@ -294,12 +308,14 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     // Navier-Stokes: one thread per node (the node's state stays in registers through the
     // fused epilogue, no LDS accumulator: 4 waves/SIMD); Euler: one thread per GLL line
     // (each symmetric pair flux evaluated once).  Chosen by measurement at TGV 32^3, p=3.
-    // Euler on small meshes (fewer line-kernel workgroups than two per CU): the node kernel, whose
-    // grid is Np times larger and whose threads each carry less of the work (the launch is latency-
-    // bound there, not throughput-bound)
+    // Euler: the node kernel where its lines lie in one wave (each symmetric pair evaluated once
+    // through cross-lane shuffles; KHI 256^2 p=3: 38.2 against 47.7 us per stage for the line
+    // kernel), and on small meshes (fewer line-kernel workgroups than two per CU: the launch is
+    // latency-bound there); otherwise the line kernel (each pair once, without shuffles)
     constexpr int kSmallGrid = 512;
+    constexpr bool wave_lines = NodeCfg<DIM, NP>::BLOCK == 64 || NodeCfg<DIM, NP>::NN == 64;
     const bool small = (op->NE + LineCfg<DIM, NP>::EPB - 1) / LineCfg<DIM, NP>::EPB < kSmallGrid;
-    if (VISC || MIX || small) {
+    if (VISC || MIX || wave_lines || small) {
         using LC = NodeCfg<DIM, NP>;
         const int nblk = (op->NE + LC::EPB - 1) / LC::EPB;
         op->lam_waves = nblk * LC::BLOCK / 64;

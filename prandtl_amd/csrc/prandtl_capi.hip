@@ -502,6 +502,7 @@ int ensure_hook_buffers(prandtl_op_s *op) {
         HIP_TRY(hipMalloc(&op->d_hflag, (size_t)op->NE * sizeof(int)));
         HIP_TRY(hipMalloc(&op->d_hcount, 3 * sizeof(unsigned long long)));
         HIP_TRY(hipMalloc(&op->d_hlist, (size_t)op->NE * sizeof(int)));
+        HIP_TRY(hipMalloc(&op->d_hlist2, (size_t)op->NE * sizeof(int)));
         HIP_TRY(hipMalloc(&op->d_hlistn, 4 * sizeof(unsigned int)));
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, 0, op->d_hcount, 0ull, 3);
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, 0, reinterpret_cast<int *>(op->d_hlistn), 0, 4);
@@ -511,7 +512,13 @@ int ensure_hook_buffers(prandtl_op_s *op) {
     return PRANDTL_OK;
 }
 
-enum HookOp { HOOK_CONSTRAINTS = 0, HOOK_FILTER = 1, HOOK_LIMITER = 2 };
+enum HookOp {
+    HOOK_CONSTRAINTS = 0, HOOK_FILTER = 1, HOOK_LIMITER = 2,
+    // the stage-fused hooks (detection in the stage epilogue): the filter's bounds from the tracked
+    // minima; the filter (with the limiter inline when both hooks are on), at a step's last stage
+    // also rewriting the minima of the elements it changes; the limiter alone
+    HOOK_BOUND = 3, HOOK_FUSED_FILTER = 4, HOOK_FUSED_FILTER_LAST = 5, HOOK_FUSED_LIMITER = 6
+};
 
 template <int DIM, int NP>
 int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
@@ -529,12 +536,14 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
     a.ngeom = op->d_ngeom;
     a.list = nullptr;
     a.nlist = a.nlist_next = nullptr;
+    a.inline_limit = 0;
+    a.track_emin = 0;
     using C = HookCfg<DIM, NP>;
     const int sblk = (op->NE + C::EPB - 1) / C::EPB;
     constexpr int ebl = C::NN <= 64 ? 64 : 128;   // one element per workgroup at a time
     // the filter and limiter walk the scan's list of elements with a fixed grid (the limiter's
     // list is usually empty: a smaller grid costs less to launch)
-    const int lblk = std::min(op->NE, 2048), lblk_lim = std::min(op->NE, 512);
+    const int lblk = std::min(op->NE, 8192), lblk_lim = std::min(op->NE, 512);
     const int h = what == HOOK_FILTER ? 0 : 1, par = op->hook_par[h];
     if (what != HOOK_CONSTRAINTS) {
         // the scan appends to one of the hook's two counters and clears the other, which the
@@ -544,9 +553,26 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
         a.nlist = op->d_hlistn + 2 * h + par;
         a.nlist_next = op->d_hlistn + 2 * h + (1 - par);
     }
-    if (what == HOOK_CONSTRAINTS) {
-        hipLaunchKernelGGL((k_hook_scan<DIM, NP, 0>), dim3(sblk), dim3(256), 0, st, a);
+    if (what == HOOK_CONSTRAINTS || what == HOOK_BOUND) {
+        // the bounds from the elements' minimum entropies: scanned from u, or (HOOK_BOUND) as the
+        // fused hooks of the previous step's last stage left them
+        if (what == HOOK_CONSTRAINTS) hipLaunchKernelGGL((k_hook_scan<DIM, NP, 0>), dim3(sblk), dim3(256), 0, st, a);
         hipLaunchKernelGGL((k_filter_bound<DIM, NP>), dim3((op->NE + 255) / 256), dim3(256), 0, st, a);
+    } else if (what == HOOK_FUSED_FILTER || what == HOOK_FUSED_LIMITER || what == HOOK_FUSED_FILTER_LAST) {
+        // the lists the stage epilogue filled (stage_hook_detect): no scan
+        const bool filt = what != HOOK_FUSED_LIMITER;
+        a.list = filt ? op->d_hlist : op->d_hlist2;
+        a.nlist = op->d_hlistn + (filt ? 0 : 2) + op->hook_par[filt ? 0 : 1];
+        a.inline_limit = filt && (op->opts.stage_hooks & PRANDTL_HOOK_LIMITER) ? 1 : 0;
+        a.track_emin = what == HOOK_FUSED_FILTER_LAST ? 1 : 0;
+        if (filt)
+            hipLaunchKernelGGL((k_filter<DIM, NP>), dim3(lblk), dim3(ebl), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_limiter<DIM, NP>), dim3(lblk_lim), dim3(ebl), 0, st, a);
+        HIP_TRY(hipGetLastError());
+        // the hook whose counters the epilogue used switches them (the other's are untouched)
+        op->hook_par[filt ? 0 : 1] = 1 - op->hook_par[filt ? 0 : 1];
+        return PRANDTL_OK;
     } else if (what == HOOK_FILTER) {
         hipLaunchKernelGGL((k_hook_scan<DIM, NP, 1>), dim3(sblk), dim3(256), 0, st, a);
         hipLaunchKernelGGL((k_filter<DIM, NP>), dim3(lblk), dim3(ebl), 0, st, a);
@@ -556,7 +582,7 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
     }
     HIP_TRY(hipGetLastError());
     // the counters switch roles only once this application is queued
-    if (what != HOOK_CONSTRAINTS) op->hook_par[h] = 1 - par;
+    if (what == HOOK_FILTER || what == HOOK_LIMITER) op->hook_par[h] = 1 - par;
     return PRANDTL_OK;
 }
 
@@ -609,11 +635,26 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
     }
     double *cur = u;
     const bool hooks = op->opts.stage_hooks != 0;
+    // the fused steppers detect the hooks in the stage epilogue when the flux kernel holds each
+    // element in one wave (all but 3D p = 4)
+    const bool fused = hooks && (scheme == PRANDTL_LSRK45 || scheme == PRANDTL_SSPRK3) &&
+                       !(op->dim == 3 && op->np == 5);
+    const bool hf = (op->opts.stage_hooks & PRANDTL_HOOK_FILTER) != 0;
+    const int hmode = !fused ? 0 : (hf ? HOOK_DETECT_FILTER : HOOK_DETECT_LIMITER);
+    if (fused) {
+        rc = ensure_hook_buffers(op);
+        if (rc) return rc;
+    }
+    // after a fused stage: the filter (and the limiter inline) or the limiter on the detected lists
+    auto fused_hooks = [&](double *x, bool last) -> int {
+        return run_hook(op, hf ? (last ? HOOK_FUSED_FILTER_LAST : HOOK_FUSED_FILTER) : HOOK_FUSED_LIMITER, x, st);
+    };
     for (int s = 0; s < nsteps; s++) {
         const bool chk = nan_check > 0 && ((s + 1) % nan_check == 0 || s + 1 == nsteps);
-        if (op->opts.stage_hooks & PRANDTL_HOOK_FILTER) {
-            // GetFilterConstraints on the state at the start of the step
-            rc = run_hook(op, HOOK_CONSTRAINTS, cur, st);
+        if (hf) {
+            // GetFilterConstraints on the state at the start of the step: scanned, or (fused, after
+            // the first step) the minima the previous step's last stage left
+            rc = run_hook(op, (fused && s > 0) ? HOOK_BOUND : HOOK_CONSTRAINTS, cur, st);
             if (rc) return rc;
             op->have_bounds = true;
         }
@@ -663,8 +704,9 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
                 double *nxt = (cur == u) ? op->d_ub : u;
                 StageSpec S{EPI_LSRK, cur, nxt, op->d_k, nullptr, CK_A[stg], CK_B[stg], dt, nullptr, false,
                             chk && stg == 4};
+                S.hook_mode = hmode | (hf && stg == 4 ? HOOK_DETECT_EMIN : 0);
                 rc = run_stage(op, S, st);
-                if (!rc && hooks) rc = stage_hooks(op, nxt, st);
+                if (!rc && hooks) rc = fused ? fused_hooks(nxt, stg == 4) : stage_hooks(op, nxt, st);
                 if (rc) return rc;
                 cur = nxt;
             }
@@ -673,16 +715,19 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
             // x = 1/3 x + 2/3 (y2 + dt L(y2)).  x stays in `cur`.
             double *y = (cur == u) ? op->d_ub : u;
             StageSpec S1{EPI_SSP, cur, y, nullptr, cur, 0.0, 1.0, dt, nullptr, false, false};
+            S1.hook_mode = hmode;
             rc = run_stage(op, S1, st);
-            if (!rc && hooks) rc = stage_hooks(op, y, st);
+            if (!rc && hooks) rc = fused ? fused_hooks(y, false) : stage_hooks(op, y, st);
             if (rc) return rc;
             StageSpec S2{EPI_SSP, y, op->d_y2, nullptr, cur, 3.0 / 4.0, 1.0 / 4.0, dt, nullptr, false, false};
+            S2.hook_mode = hmode;
             rc = run_stage(op, S2, st);
-            if (!rc && hooks) rc = stage_hooks(op, op->d_y2, st);
+            if (!rc && hooks) rc = fused ? fused_hooks(op->d_y2, false) : stage_hooks(op, op->d_y2, st);
             if (rc) return rc;
             StageSpec S3{EPI_SSP, op->d_y2, cur, nullptr, cur, 1.0 / 3.0, 2.0 / 3.0, dt, nullptr, false, chk};
+            S3.hook_mode = hmode | (hf ? HOOK_DETECT_EMIN : 0);
             rc = run_stage(op, S3, st);
-            if (!rc && hooks) rc = stage_hooks(op, cur, st);
+            if (!rc && hooks) rc = fused ? fused_hooks(cur, true) : stage_hooks(op, cur, st);
             if (rc) return rc;
         }
         if (chk) {
@@ -979,6 +1024,7 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_hflag);
     (void)hipFree(op->d_hcount);
     (void)hipFree(op->d_hlist);
+    (void)hipFree(op->d_hlist2);
     (void)hipFree(op->d_hlistn);
     (void)hipFree(op->d_lam);
     (void)hipFree(op->d_lam_part);
