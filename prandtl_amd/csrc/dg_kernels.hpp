@@ -35,6 +35,9 @@
 #ifndef PRANDTL_NODE_WAVES
 #define PRANDTL_NODE_WAVES 4
 #endif
+#ifndef PRANDTL_LIFT_MFMA
+#define PRANDTL_LIFT_MFMA 1
+#endif
 #ifndef PRANDTL_LIFT_WAVES
 #define PRANDTL_LIFT_WAVES 4   // 128 VGPRs: at 3D p=3 FP64, 4 waves/SIMD beat 3 without the cap
 #endif
@@ -1134,6 +1137,8 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     using R = typename std::conditional<MIX, float, double>::type;
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB, L = C::L;
     constexpr int NW = NEQ * EPB * C::FITEMS, NFV = DIM * (NEQ - 1) * EPB * NN;
+    // FP64 3D p = 3: the tensor contractions on the matrix cores (an element is one wave)
+    constexpr bool LIFT_MFMA = PRANDTL_LIFT_MFMA && DIM == 3 && NP == 4 && !MIX;
     __shared__ R sW[NEQ][EPB * NN];
     __shared__ R sU[NW > NFV ? NW : NFV];  // neighbour w at face points, then F~^d
     __shared__ double sWref[EPB][NEQ];
@@ -1233,17 +1238,38 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
 #pragma unroll
         for (int v = 0; v < NEQ; v++) wrel[v] = (R)(w[v] - wref[v]);
         R gref[DIM][NEQ];
+        if constexpr (LIFT_MFMA) {
+            // the D-line contractions on the matrix cores: one v_mfma_f64_4x4x4f64 (four 4x4x4
+            // blocks; A at lane m + 4 blk + 16 k, B at lane n + 4 blk + 16 k, D at lane
+            // n + 4 blk + 16 m) per direction and variable for the element's 64 nodes (lane = node
+            // i + 4 j + 16 k).  zeta: A = D, B = the node's own w; xi: A = w at the node with the
+            // lane's bits 0-1 and 4-5 swapped, B = D; eta: A = D, B = w at the node with bits 2-3 and
+            // 4-5 swapped, the result on the lane with the same swap
+            const int lw = tid & 63, base = tid - lw;
+            const int sx = ((lw >> 4) & 3) | (lw & 12) | ((lw & 3) << 4);
+            const int sy = (lw & 3) | (((lw >> 4) & 3) << 2) | (((lw >> 2) & 3) << 4);
+            const double Dc = A.T.D[lw & 3][lw >> 4];
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) {
+                gref[2][v] = __builtin_amdgcn_mfma_f64_4x4x4f64(Dc, (double)sW[v][tid], 0.0, 0, 0, 0);
+                gref[0][v] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)sW[v][base + sx], Dc, 0.0, 0, 0, 0);
+                const double gy = __builtin_amdgcn_mfma_f64_4x4x4f64(Dc, (double)sW[v][base + sy], 0.0, 0, 0, 0);
+                gref[1][v] = __shfl(gy, sy, 64);
+            }
+        }
 #pragma unroll
         for (int dir = 0; dir < DIM; dir++) {
             const int st = dir_stride<NP>(dir);
             const int a = ai[dir];
             const int lb = tid - a * st;
+            if constexpr (!LIFT_MFMA) {
 #pragma unroll
-            for (int v = 0; v < NEQ; v++) {
-                R t = R(0);
+                for (int v = 0; v < NEQ; v++) {
+                    R t = R(0);
 #pragma unroll
-                for (int b = 0; b < NP; b++) t += (R)A.T.D[a][b] * sW[v][lb + b * st];
-                gref[dir][v] = t;
+                    for (int b = 0; b < NP; b++) t += (R)A.T.D[a][b] * sW[v][lb + b * st];
+                    gref[dir][v] = t;
+                }
             }
             if (a == NP - 1 || a == 0) {
                 // face jump 1/2 (w+ - w-) / w0, the same sign on both sides (LiftingBR1.cpp:51-65)
@@ -1344,18 +1370,36 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     __syncthreads();
     if (!active) return;
     R dvv[NEQ - 1];
+    if constexpr (LIFT_MFMA) {
+        // the D-hat divergence on the matrix cores, as the gradient above: zeta and xi accumulate
+        // in one result (both land on the node's lane), eta is moved to its lane
+        const int lw = tid & 63, base = tid - lw;
+        const int sx = ((lw >> 4) & 3) | (lw & 12) | ((lw & 3) << 4);
+        const int sy = (lw & 3) | (((lw >> 4) & 3) << 2) | (((lw >> 2) & 3) << 4);
+        const double Dh = A.T.Dhat[lw & 3][lw >> 4];
 #pragma unroll
-    for (int v = 0; v < NEQ - 1; v++) dvv[v] = R(0);
+        for (int v = 0; v < NEQ - 1; v++) {
+            double acc = __builtin_amdgcn_mfma_f64_4x4x4f64(Dh, (double)Fc[2][v], 0.0, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_4x4x4f64((double)sU[(0 * (NEQ - 1) + v) * EPB * NN + base + sx], Dh, acc,
+                                                     0, 0, 0);
+            const double dy = __builtin_amdgcn_mfma_f64_4x4x4f64(Dh, (double)sU[(1 * (NEQ - 1) + v) * EPB * NN + base + sy],
+                                                                0.0, 0, 0, 0);
+            dvv[v] = acc + __shfl(dy, sy, 64);
+        }
+    } else {
 #pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        const int st = dir_stride<NP>(d);
-        const int a = ai[d];
-        const int lb = tid - a * st;
+        for (int v = 0; v < NEQ - 1; v++) dvv[v] = R(0);
 #pragma unroll
-        for (int v = 0; v < NEQ - 1; v++)
+        for (int d = 0; d < DIM; d++) {
+            const int st = dir_stride<NP>(d);
+            const int a = ai[d];
+            const int lb = tid - a * st;
 #pragma unroll
-            for (int b = 0; b < NP; b++)
-                dvv[v] += (R)A.T.Dhat[a][b] * sU[(d * (NEQ - 1) + v) * EPB * NN + lb + b * st];
+            for (int v = 0; v < NEQ - 1; v++)
+#pragma unroll
+                for (int b = 0; b < NP; b++)
+                    dvv[v] += (R)A.T.Dhat[a][b] * sU[(d * (NEQ - 1) + v) * EPB * NN + lb + b * st];
+        }
     }
 #pragma unroll
     for (int v = 0; v < NEQ - 1; v++) {
