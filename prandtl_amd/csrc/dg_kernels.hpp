@@ -182,6 +182,14 @@ __device__ __forceinline__ int line_base_rt(int l, int dir) {
     }
 }
 
+// LDS slot of node n (0..63, n = i + 4 j + 16 k, 3D p = 3) of a one-wave element: k XORed into
+// the xi bits and into the eta bits, a permutation of each 16-node k-plane.  With 64 lanes reading
+// FP64 values, each of these patterns then hits distinct LDS banks within every 16 lanes (32 banks
+// of 4 bytes) and every 32 lanes (64 banks): the own node, a line partner in any direction, a
+// face's nodes (one index fixed at 0 or 3), the node with its xi and zeta (or eta and zeta)
+// indices exchanged.  In node order the last two are 2- to 4-way conflicts.
+__device__ __forceinline__ int node_swz(int n) { return n ^ (5 * ((n >> 4) & 3)); }
+
 template <int NP>
 __device__ __forceinline__ constexpr int dir_stride(int dir) {
     return dir == 0 ? 1 : (dir == 1 ? NP : NP * NP);
@@ -554,6 +562,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     // volume pairs: each symmetric pair evaluated once when the lines of an element lie in one
     // wave (the partner's value arrives by a cross-lane shuffle)
     constexpr bool ROT = !GEN && !BUG1 && (C::BLOCK == 64 || NN == 64);
+    // the node arrays (sS, sPd, sPv, sLam) stay in node order: swizzled by node_swz, the face
+    // items' own-side reads lose their bank conflicts (4.3 M -> 1.0 M conflict cycles per launch at
+    // TGV 32^3 p = 3), but the address arithmetic costs a 12-byte spill at 128 VGPRs, 6 % per launch
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -1179,9 +1190,13 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     double wref[NEQ];
 #pragma unroll
     for (int v = 0; v < NEQ; v++) wref[v] = (MIX && active) ? sWref[eb][v] : 0.0;
+    // the MFMA path reads sW only at bit-swapped nodes of the wave's element; it stores node n at
+    // n ^ 5 (n >> 4) (the zeta index XORed into the xi and eta bits), where the own, the
+    // xi-swapped and the eta-swapped patterns each touch every LDS bank once
+    const int lsw = LIFT_MFMA ? (tid & ~63) | node_swz(tid & 63) : tid;
     if (active) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) sW[v][tid] = (R)(w[v] - wref[v]);
+        for (int v = 0; v < NEQ; v++) sW[v][lsw] = (R)(w[v] - wref[v]);
     }
     for (int it = tid; it < nel * C::FITEMS; it += C::BLOCK) {
         const int fe = it / C::FITEMS, i = it - fe * C::FITEMS;
@@ -1248,12 +1263,13 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
             const int lw = tid & 63, base = tid - lw;
             const int sx = ((lw >> 4) & 3) | (lw & 12) | ((lw & 3) << 4);
             const int sy = (lw & 3) | (((lw >> 4) & 3) << 2) | (((lw >> 2) & 3) << 4);
+            const int ax = base + node_swz(sx), ay = base + node_swz(sy);
             const double Dc = A.T.D[lw & 3][lw >> 4];
 #pragma unroll
             for (int v = 0; v < NEQ; v++) {
-                gref[2][v] = __builtin_amdgcn_mfma_f64_4x4x4f64(Dc, (double)sW[v][tid], 0.0, 0, 0, 0);
-                gref[0][v] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)sW[v][base + sx], Dc, 0.0, 0, 0, 0);
-                const double gy = __builtin_amdgcn_mfma_f64_4x4x4f64(Dc, (double)sW[v][base + sy], 0.0, 0, 0, 0);
+                gref[2][v] = __builtin_amdgcn_mfma_f64_4x4x4f64(Dc, (double)wrel[v], 0.0, 0, 0, 0);
+                gref[0][v] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)sW[v][ax], Dc, 0.0, 0, 0, 0);
+                const double gy = __builtin_amdgcn_mfma_f64_4x4x4f64(Dc, (double)sW[v][ay], 0.0, 0, 0, 0);
                 gref[1][v] = __shfl(gy, sy, 64);
             }
         }
@@ -1362,10 +1378,12 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     }
     __syncthreads();  // sU (neighbour w) dead: reuse it for F~^d
     if (active) {
+        // the MFMA divergence reads the xi and eta fluxes at swizzled nodes, the zeta flux from
+        // the node's own register
 #pragma unroll
-        for (int d = 0; d < DIM; d++)
+        for (int d = 0; d < (LIFT_MFMA ? 2 : DIM); d++)
 #pragma unroll
-            for (int v = 0; v < NEQ - 1; v++) sU[(d * (NEQ - 1) + v) * EPB * NN + tid] = Fc[d][v];
+            for (int v = 0; v < NEQ - 1; v++) sU[(d * (NEQ - 1) + v) * EPB * NN + lsw] = Fc[d][v];
     }
     __syncthreads();
     if (!active) return;
@@ -1376,13 +1394,13 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
         const int lw = tid & 63, base = tid - lw;
         const int sx = ((lw >> 4) & 3) | (lw & 12) | ((lw & 3) << 4);
         const int sy = (lw & 3) | (((lw >> 4) & 3) << 2) | (((lw >> 2) & 3) << 4);
+        const int ax = base + node_swz(sx), ay = base + node_swz(sy);
         const double Dh = A.T.Dhat[lw & 3][lw >> 4];
 #pragma unroll
         for (int v = 0; v < NEQ - 1; v++) {
             double acc = __builtin_amdgcn_mfma_f64_4x4x4f64(Dh, (double)Fc[2][v], 0.0, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_4x4x4f64((double)sU[(0 * (NEQ - 1) + v) * EPB * NN + base + sx], Dh, acc,
-                                                     0, 0, 0);
-            const double dy = __builtin_amdgcn_mfma_f64_4x4x4f64(Dh, (double)sU[(1 * (NEQ - 1) + v) * EPB * NN + base + sy],
+            acc = __builtin_amdgcn_mfma_f64_4x4x4f64((double)sU[(0 * (NEQ - 1) + v) * EPB * NN + ax], Dh, acc, 0, 0, 0);
+            const double dy = __builtin_amdgcn_mfma_f64_4x4x4f64(Dh, (double)sU[(1 * (NEQ - 1) + v) * EPB * NN + ay],
                                                                 0.0, 0, 0, 0);
             dvv[v] = acc + __shfl(dy, sy, 64);
         }
