@@ -127,8 +127,11 @@ typedef struct {
 /* Stage hooks the steppers (prandtl_rk_step, prandtl_rk_run) apply after every RK stage:
  * the entropy filter (bounds from the state at the start of each step), then the
  * positivity limiter -- the Filter and Limiter hooks of src/Filters/Filter.hpp:11-30 and
- * src/Limiters/Limiter.hpp:10-16. */
-enum { PRANDTL_HOOK_FILTER = 1, PRANDTL_HOOK_LIMITER = 2 };
+ * src/Limiters/Limiter.hpp:10-16.  LSRK4(5) and SSP-RK3 detect the elements to filter or
+ * limit in the stage update itself (all shapes but 3D p = 4); PRANDTL_HOOK_UNFUSED asks for
+ * separate detection scans after every stage instead (the same elements and results, more
+ * launches: for checking and measurement). */
+enum { PRANDTL_HOOK_FILTER = 1, PRANDTL_HOOK_LIMITER = 2, PRANDTL_HOOK_UNFUSED = 4 };
 
 /* Subcell finite-volume blending (the reference built with SUBCELL_FV_BLENDING): every RHS
  * evaluation computes the per-element coefficient alpha from its input state (Persson-Peraire

@@ -22,7 +22,7 @@ SURF_CHANDRASHEKAR, SURF_EC, SURF_RUSANOV = 0, 1, 2
 FP64, MIXED = 0, 1
 BUG_A1, BUG_A2, BUG_A3 = 1, 2, 4
 SSPRK3, LSRK45, FORWARD_EULER, RK2, RK4 = 0, 1, 2, 3, 4
-HOOK_FILTER, HOOK_LIMITER = 1, 2
+HOOK_FILTER, HOOK_LIMITER, HOOK_UNFUSED = 1, 2, 4
 TAPE_SEGMENTS, TAPE_REVOLVE = 0, 1
 
 # every symbol the public header declares

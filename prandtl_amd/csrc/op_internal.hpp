@@ -305,9 +305,8 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
             return PRANDTL_OK;
         }
     }
-    // Navier-Stokes: one thread per node (the node's state stays in registers through the
-    // fused epilogue, no LDS accumulator: 4 waves/SIMD); Euler: one thread per GLL line
-    // (each symmetric pair flux evaluated once).  Chosen by measurement at TGV 32^3, p=3.
+    // Navier-Stokes and mixed precision: one thread per node (the node's state stays in registers
+    // through the fused epilogue, no LDS accumulator: 4 waves/SIMD at TGV 32^3 p=3).
     // Euler: the node kernel where its lines lie in one wave (each symmetric pair evaluated once
     // through cross-lane shuffles; KHI 256^2 p=3: 38.2 against 47.7 us per stage for the line
     // kernel), and on small meshes (fewer line-kernel workgroups than two per CU: the launch is

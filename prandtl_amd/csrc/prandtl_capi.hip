@@ -634,11 +634,11 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
         HIP_TRY(hipGetLastError());
     }
     double *cur = u;
-    const bool hooks = op->opts.stage_hooks != 0;
+    const bool hooks = (op->opts.stage_hooks & (PRANDTL_HOOK_FILTER | PRANDTL_HOOK_LIMITER)) != 0;
     // the fused steppers detect the hooks in the stage epilogue when the flux kernel holds each
-    // element in one wave (all but 3D p = 4)
+    // element in one wave (all but 3D p = 4), unless the caller asked for separate scans
     const bool fused = hooks && (scheme == PRANDTL_LSRK45 || scheme == PRANDTL_SSPRK3) &&
-                       !(op->dim == 3 && op->np == 5);
+                       !(op->dim == 3 && op->np == 5) && !(op->opts.stage_hooks & PRANDTL_HOOK_UNFUSED);
     const bool hf = (op->opts.stage_hooks & PRANDTL_HOOK_FILTER) != 0;
     const int hmode = !fused ? 0 : (hf ? HOOK_DETECT_FILTER : HOOK_DETECT_LIMITER);
     if (fused) {
@@ -868,7 +868,8 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
     if (opts) o = *opts;
     if (o.surface_flux < 0 || o.surface_flux > 2) return fail(PRANDTL_ERR_ARG, "unknown surface flux");
     if (o.precision != PRANDTL_FP64 && o.precision != PRANDTL_MIXED) return fail(PRANDTL_ERR_ARG, "unknown precision");
-    if (o.stage_hooks & ~(PRANDTL_HOOK_FILTER | PRANDTL_HOOK_LIMITER)) return fail(PRANDTL_ERR_ARG, "unknown stage hook");
+    if (o.stage_hooks & ~(PRANDTL_HOOK_FILTER | PRANDTL_HOOK_LIMITER | PRANDTL_HOOK_UNFUSED))
+        return fail(PRANDTL_ERR_ARG, "unknown stage hook");
     if (o.blending != 0 && o.blending != 1) return fail(PRANDTL_ERR_ARG, "blending must be 0 or 1");
     if (!(o.alpha_min >= 0.0 && o.alpha_min < 0.5) || !(o.alpha_max >= 0.0 && o.alpha_max <= 1.0))
         return fail(PRANDTL_ERR_ARG, "alpha_min must lie in [0, 0.5) and alpha_max in [0, 1]");
@@ -1271,7 +1272,7 @@ int prandtl_rk_run_tape_schedule(prandtl_op op, int scheme, double *u, double *t
     if (scheme != PRANDTL_LSRK45 && scheme != PRANDTL_SSPRK3) return fail(PRANDTL_ERR_ARG, "unknown scheme");
     if (schedule != PRANDTL_TAPE_SEGMENTS && schedule != PRANDTL_TAPE_REVOLVE)
         return fail(PRANDTL_ERR_ARG, "unknown checkpoint schedule");
-    if (op->opts.stage_hooks)
+    if (op->opts.stage_hooks & (PRANDTL_HOOK_FILTER | PRANDTL_HOOK_LIMITER))
         return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the filter or limiter hooks");
     if (op->opts.blending) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint does not differentiate the blending");
     if (op->nbc || op->d_ngeom) return fail(PRANDTL_ERR_UNSUPPORTED, "the adjoint covers periodic affine meshes");

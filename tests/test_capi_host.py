@@ -70,7 +70,7 @@ def test_rejects_nonconforming_faces_and_bad_jacobian():
 @pytest.mark.parametrize("kw,code", [({"gamma": 1.0}, -1), ({"surface_flux": 9}, -1), ({"precision": 7}, -1),
                                      ({"viscous": 1, "bug_compat": 1, "precision": 1}, -2),
                                      ({"viscous": 1, "bug_compat": 2, "precision": 1}, -2), ({"bug_compat": 8}, -1),
-                                     ({"viscous": 1, "Pr": 0.0}, -1), ({"stage_hooks": 4}, -1)])
+                                     ({"viscous": 1, "Pr": 0.0}, -1), ({"stage_hooks": 8}, -1)])
 def test_rejects_bad_options(kw, code):
     m = M.periodic_lattice(2, 3, (2, 2), lengths=(1, 1))
     rc, msg, _ = _create(m, **kw)

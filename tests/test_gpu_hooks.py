@@ -136,3 +136,58 @@ def test_hooked_stepper(device, scheme, nsteps, hooks):
     assert np.isfinite(out).all()
     assert err <= 1e-9
     op.close()
+
+
+def _rarefaction_3d(m):
+    X = m.node_coords()
+    rho = np.ones_like(X[0])
+    u = -2.0 * np.tanh((0.5 - X[0]) / 0.05)
+    p = np.full_like(X[0], 1e-6)
+    z = np.zeros_like(u)
+    return cases.prim_to_conserv(rho, [u, 0.01 * np.sin(2 * np.pi * X[1]), z], p)
+
+
+FUSED_CASES = [
+    ("KHI 32^2 p=3", lambda: cases.khi_mesh(n=32, order=3), cases.khi_state, 40),
+    ("double rarefaction 2D 16x4 p=4", lambda: M.periodic_lattice(2, 4, (16, 4), lengths=(1.0, 0.25)),
+     _double_rarefaction, 10),
+    ("double rarefaction 3D 8x2x2 p=2", lambda: M.periodic_lattice(3, 2, (8, 2, 2), lengths=(1.0, 0.25, 0.25)),
+     _rarefaction_3d, 10),
+    ("double rarefaction 3D 8x2x2 p=3", lambda: M.periodic_lattice(3, 3, (8, 2, 2), lengths=(1.0, 0.25, 0.25)),
+     _rarefaction_3d, 10),
+]
+
+
+@pytest.mark.parametrize("scheme", [capi.SSPRK3, capi.LSRK45])
+@pytest.mark.parametrize("hooks", [capi.HOOK_FILTER | capi.HOOK_LIMITER, capi.HOOK_LIMITER], ids=["filter+limiter", "limiter"])
+@pytest.mark.parametrize("name,make_mesh,make_state,nsteps", FUSED_CASES, ids=[c[0] for c in FUSED_CASES])
+def test_fused_detection_matches_scans(device, name, make_mesh, make_state, nsteps, hooks, scheme):
+    """The steppers' stage-fused detection (the flux kernel's epilogue flags the elements to
+    filter and limit, tracks each element's minimum entropy for the next step's bounds, and the
+    filter limits its own elements in place) against separate detection scans after every stage
+    (PRANDTL_HOOK_UNFUSED): the same numbers of elements filtered and limited and the same state,
+    bit for bit.  With the filter on, the filter restores positivity in these cases; the
+    limiter alone acts on the rarefactions."""
+    import torch
+
+    if hooks == capi.HOOK_LIMITER and name.startswith("KHI"):
+        pytest.skip("no inadmissible state in the KHI case: the limiter has nothing to do")
+    _say(f"fused vs unfused hooks: {name}, hooks {hooks}, scheme {scheme}, {nsteps} steps")
+    m = make_mesh()
+    u0 = make_state(m)
+    lam = cases.max_wave_speed(u0)
+    dt = cases.cfl_dt(m, lam, 0.5)
+    res = []
+    for flags in (hooks, hooks | capi.HOOK_UNFUSED):
+        op = capi.Operator(m, stage_hooks=flags)
+        op.hook_counts()
+        d = torch.from_numpy(u0).to(device)
+        op.rk_run(scheme, d, 0.0, dt, nsteps)
+        res.append((op.hook_counts(), d.cpu().numpy()))
+        op.close()
+    (cf, uf), (cu, uu) = res
+    print(f"[hooks] {name}: fused filtered/limited {cf}, unfused {cu}", flush=True)
+    assert cf[0 if hooks & capi.HOOK_FILTER else 1] > 0, "the case must exercise the hook"
+    assert cf == cu
+    assert np.isfinite(uf).all()
+    assert np.array_equal(uf, uu)
