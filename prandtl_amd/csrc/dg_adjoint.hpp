@@ -69,23 +69,28 @@ __host__ __device__ inline NodeQ<DIM> node_q(const double *s, double gm1) {
     return q;
 }
 
-// logarithmic mean of (x, y) and its partials, given 1/x, 1/y, ln x, ln y; the two branches
-// of the forward (series in u = ((y-x)/(y+x))^2 below 1e-4, closed form otherwise)
+// logarithmic mean L of (x, y), its reciprocal and its partials, given 1/x, 1/y, ln x, ln y;
+// the two branches of the forward (series in u = ((y-x)/(y+x))^2 below 1e-4, closed form
+// otherwise).  On the series branch L = (x+y)/(2P), P = 1 + u/3 + u^2/5 + u^3/7, so 1/L =
+// 2P/(x+y) needs no division, and 1/P is itself the series 1 - u/3 - 4u^2/45 - 44u^3/945 (the
+// omitted u^4 term is below 1e-17 relative for u < 1e-4); with u = f^2, f = (y-x)/(x+y):
+// dL/dx = 1/(2P) + c y, dL/dy = 1/(2P) - c x, c = 2 P'(u) f / (P^2 (x+y)).
 __host__ __device__ inline void log_mean_dq(double x, double y, double ix, double iy, double lx, double ly, double &L,
-                                            double &Lx, double &Ly) {
+                                            double &iL, double &Lx, double &Ly) {
     const double s = x + y, r = inv(s), f = (y - x) * r, u = f * f;
     if (u < 1e-4) {
         const double P = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
         const double dP = 1.0 / 3.0 + u * (2.0 / 5.0 + u * (3.0 / 7.0));
-        const double iP = inv(P);
-        const double A = 0.5 * s;
-        L = A * iP;
-        const double c = A * dP * iP * iP * 4.0 * f * r * r;
+        const double iP = 1.0 - u * (1.0 / 3.0 + u * (4.0 / 45.0 + u * (44.0 / 945.0)));
+        L = 0.5 * s * iP;
+        iL = 2.0 * r * P;
+        const double c = 2.0 * dP * iP * iP * f * r;
         Lx = 0.5 * iP + c * y;
         Ly = 0.5 * iP - c * x;
     } else {
         const double lg = ly - lx, ilg = inv(lg), d = y - x;
         L = d * ilg;
+        iL = lg * inv(d);
         Lx = (d * ix - lg) * ilg * ilg;
         Ly = (lg - d * iy) * ilg * ilg;
     }
@@ -106,31 +111,31 @@ __host__ __device__ inline void primb_zero(PrimB<DIM> &g) {
 // core of the entropy-conservative two-point flux
 //   F = (rho_ln Vn, rho_ln Vn vbar + p_hat m, rho_ln Vn h_hat),
 //   Vn = vbar . m, p_hat = (rho_1 + rho_2) / (2 (beta_1 + beta_2)),
-//   h_hat = 1/(2 beta_ln (gamma-1)) - |v_1|^2/4 - |v_2|^2/4 + |vbar|^2 + p_hat / rho_ln,
+//   h_hat = 1/(2 beta_ln (gamma-1)) - |v_1|^2/4 - |v_2|^2/4 + |vbar|^2 + p_hat / rho_ln
+//         = 1/(2 beta_ln (gamma-1)) + (v_1 . v_2)/2 + p_hat / rho_ln,
 // reversed: accumulates the adjoints of (rho, v, beta) of both nodes for the cotangent Fb.
 template <int DIM>
 __host__ __device__ inline void ec_core_back(const NodeQ<DIM> &a, const NodeQ<DIM> &b, const double *m,
                                              double gm1inv, const double *Fb, PrimB<DIM> &ga, PrimB<DIM> &gb) {
-    double Lr, Lr1, Lr2, Lb, Lb1, Lb2;
-    log_mean_dq(a.rho, b.rho, a.irho, b.irho, a.lrho, b.lrho, Lr, Lr1, Lr2);
-    log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, Lb1, Lb2);
-    const double iLr = inv(Lr), iLb = inv(Lb);
-    double vbar[DIM], Vn = 0.0, h = 0.0;
+    double Lr, iLr, Lr1, Lr2, Lb, iLb, Lb1, Lb2;
+    log_mean_dq(a.rho, b.rho, a.irho, b.irho, a.lrho, b.lrho, Lr, iLr, Lr1, Lr2);
+    log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, iLb, Lb1, Lb2);
+    double vs[DIM], Vn2 = 0.0, vdot = 0.0;   // vs = v_1 + v_2 = 2 vbar
     for (int d = 0; d < DIM; d++) {
-        vbar[d] = 0.5 * (a.v[d] + b.v[d]);
-        Vn += vbar[d] * m[d];
-        h += -0.25 * (a.v[d] * a.v[d] + b.v[d] * b.v[d]) + vbar[d] * vbar[d];
+        vs[d] = a.v[d] + b.v[d];
+        Vn2 += vs[d] * m[d];
+        vdot += a.v[d] * b.v[d];
     }
+    const double Vn = 0.5 * Vn2;
     const double ibs = inv(a.beta + b.beta);
     const double p_hat = 0.5 * (a.rho + b.rho) * ibs;
-    h += 0.5 * iLb * gm1inv + p_hat * iLr;
+    const double h = 0.5 * (vdot + iLb * gm1inv) + p_hat * iLr;
     const double mflux = Lr * Vn;
     // reverse
     double mflux_b = Fb[0] + Fb[DIM + 1] * h;
-    double vbar_b[DIM], p_hat_b = 0.0;
+    double p_hat_b = 0.0;
     for (int d = 0; d < DIM; d++) {
-        mflux_b += Fb[1 + d] * vbar[d];
-        vbar_b[d] = Fb[1 + d] * mflux;
+        mflux_b += 0.5 * Fb[1 + d] * vs[d];
         p_hat_b += Fb[1 + d] * m[d];
     }
     const double h_b = Fb[DIM + 1] * mflux;
@@ -138,10 +143,12 @@ __host__ __device__ inline void ec_core_back(const NodeQ<DIM> &a, const NodeQ<DI
     p_hat_b += h_b * iLr;
     const double Lr_b = h_b * (-p_hat * iLr * iLr) + mflux_b * Vn;
     const double Vn_b = mflux_b * Lr;
+    const double hh = 0.5 * h_b;
     for (int d = 0; d < DIM; d++) {
-        const double vb = h_b * 2.0 * vbar[d] + Vn_b * m[d] + vbar_b[d];
-        ga.v[d] += h_b * (-0.5 * a.v[d]) + 0.5 * vb;
-        gb.v[d] += h_b * (-0.5 * b.v[d]) + 0.5 * vb;
+        // through vbar (Vn and the momentum flux), and h_hat's (v_1 . v_2)/2
+        const double vb = 0.5 * (Vn_b * m[d] + Fb[1 + d] * mflux);
+        ga.v[d] += hh * b.v[d] + vb;
+        gb.v[d] += hh * a.v[d] + vb;
     }
     const double rb = p_hat_b * 0.5 * ibs, bb = -p_hat_b * p_hat * ibs;
     ga.rho += rb + Lr_b * Lr1;
@@ -199,9 +206,8 @@ __host__ __device__ inline void face_flux_vjp(int kind, const double *s1, const 
         for (int d = 0; d < DIM; d++) nmag += n[d] * n[d];
         nmag = sqrt(nmag);
         const double hl = 0.5 * nmag;
-        double Lb, Lb1, Lb2;
-        log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, Lb1, Lb2);
-        const double iLb = inv(Lb);
+        double Lb, iLb, Lb1, Lb2;
+        log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, iLb, Lb1, Lb2);
         const double rho_mean = 0.5 * (a.rho + b.rho), drho = b.rho - a.rho;
         double diss = 0.0, vsq1 = 0.0, vsq2 = 0.0;
         for (int d = 0; d < DIM; d++) {

@@ -562,6 +562,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     // volume pairs: each symmetric pair evaluated once when the lines of an element lie in one
     // wave (the partner's value arrives by a cross-lane shuffle)
     constexpr bool ROT = !GEN && !BUG1 && (C::BLOCK == 64 || NN == 64);
+    // even Np: the Np/2-apart pairs of directions 0 and 1 share one pass (half of each direction's
+    // lanes would otherwise evaluate their partner's pair again)
+    constexpr bool HALF2 = ROT && NP % 2 == 0;
     // the node arrays (sS, sPd, sPv, sLam) stay in node order: swizzled by node_swz, the face
     // items' own-side reads lose their bank conflicts (4.3 M -> 1.0 M conflict cycles per launch at
     // TGV 32^3 p = 3), but the address arithmetic costs a 12-byte spill at 128 VGPRs, 6 % per launch
@@ -936,10 +939,12 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             if constexpr (ROT) {
                 // each symmetric pair once: node a evaluates (a, a+k) for k <= Np/2 and receives
                 // (a-k, a) from the lane of node a-k (the same flux, F(u_i,u_j) = F(u_j,u_i));
-                // 2k = Np pairs are evaluated by both of their nodes
+                // 2k = Np pairs are evaluated by both of their nodes in the last direction, and in
+                // one shared pass after this loop in directions 0 and 1 (HALF2)
                 const int lane = tid & 63;
+                const int KMAX = (HALF2 && dir < 2) ? (NP - 1) / 2 : NP / 2;
 #pragma unroll 1
-                for (int k = 1; 2 * k <= NP; k++) {
+                for (int k = 1; k <= KMAX; k++) {
                     const int b = (a + k) % NP;
                     R F[NEQ];
                     pair_flux(b, lb + b * st, F);
@@ -979,6 +984,67 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
                 }
             }
+        }
+        if constexpr (HALF2) {
+            // the pairs (a, a + Np/2) of directions 0 and 1, each once: a lane with i < Np/2 takes
+            // its node's direction-0 pair (i, j) - (i + Np/2, j); each other lane one
+            // direction-1 pair (i', j') - (i', j' + Np/2) with j' < Np/2, numbered
+            // (i - Np/2) + (Np/2) j = i' + Np j'.  Both nodes of a pair receive it by shuffles.
+            constexpr int H = NP / 2;
+            const int lane = tid & 63;
+            const int i = ai[0], j = ai[1];
+            const int n00 = node - i - j * NP;   // node (0, 0, k)
+            int na, nb, dl;
+            if (i < H) {
+                na = node;
+                nb = node + H;
+                dl = 0;
+            } else {
+                const int idx = (i - H) + H * j;
+                na = n00 + (idx % NP) + (idx / NP) * NP;
+                nb = na + H * NP;
+                dl = 1;
+            }
+            R F[NEQ];
+            {
+                const int la = eb * NN + na, lb2 = eb * NN + nb;
+                const R *m = sM[eb][dl];
+                if constexpr (MIX) {
+                    NodePrimMx<DIM> qa, qb;
+                    qa.rho = sPd[0][la];
+                    qa.beta = sPd[1][la];
+                    qb.rho = sPd[0][lb2];
+                    qb.beta = sPd[1][lb2];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        qa.vel[d] = sPv[d][la];
+                        qb.vel[d] = sPv[d][lb2];
+                    }
+                    volume_flux_mx<DIM>(qa, qb, m, A.P.gammaM1Inverse, p_ref, h_ref, F);
+                } else {
+                    NodePrim<DIM> qa, qb;
+                    qa.rho = sPd[0][la];
+                    qa.beta = sPd[1][la];
+                    qb.rho = sPd[0][lb2];
+                    qb.beta = sPd[1][lb2];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        qa.vel[d] = sPv[d][la];
+                        qb.vel[d] = sPv[d][lb2];
+                    }
+                    volume_flux<DIM, true>(qa, qb, m, A.P.gammaM1Inverse, F);
+                }
+            }
+            // the direction-0 pair of this node was evaluated by the lane of (i mod H, j), the
+            // direction-1 pair by the lane that numbers (i, j mod H)
+            const int xl = lane - (i >= H ? H : 0);
+            const int jm = j < H ? j : j - H;
+            const int idy = i + NP * jm;
+            const int yl = lane - (node - n00) + (H + idy % H) + (idy / H) * NP;
+            const R dx = sT[i * NP + (i < H ? i + H : i - H)];
+            const R dy = sT[j * NP + (j < H ? j + H : j - H)];
+#pragma unroll
+            for (int v = 0; v < NEQ; v++) acc[v] += dx * __shfl(F[v], xl, 64) + dy * __shfl(F[v], yl, 64);
         }
     }
     if constexpr (BLEND) {

@@ -146,20 +146,23 @@ __device__ __forceinline__ void volume_flux(const NodePrim<DIM> &a, const NodePr
         rho_ln = rcp(il_rho);
     }
     const double il_b = inv_log_mean(a.beta, b.beta, r_b);
-    double mean[DIM];
-    double h_hat = 0.0, Vn = 0.0;
+    // with vs = v_a + v_b the mean velocity is vs/2, and the kinetic part of h-hat,
+    // |mean|^2 - (|v_a|^2 + |v_b|^2)/4, is (v_a . v_b)/2: one dot product, no cancellation
+    double vs[DIM];
+    double vdot = 0.0, Vn2 = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        mean[d] = 0.5 * (a.vel[d] + b.vel[d]);
-        h_hat += -0.25 * (a.vel[d] * a.vel[d] + b.vel[d] * b.vel[d]) + mean[d] * mean[d];
-        Vn += mean[d] * m[d];
+        vs[d] = a.vel[d] + b.vel[d];
+        vdot += a.vel[d] * b.vel[d];
+        Vn2 += vs[d] * m[d];
     }
     const double p_hat = 0.5 * rsum * r_b;                                // :71
-    h_hat += 0.5 * il_b * gammaM1Inverse + p_hat * il_rho;                 // :73
-    const double mflux = rho_ln * Vn;
+    const double h_hat = 0.5 * (vdot + il_b * gammaM1Inverse) + p_hat * il_rho;   // :73
+    const double mflux = 0.5 * rho_ln * Vn2;                              // rho_ln (mean . m)
+    const double hf = 0.5 * mflux;
     F[0] = mflux;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) F[1 + d] = mflux * mean[d] + p_hat * m[d];
+    for (int d = 0; d < DIM; d++) F[1 + d] = hf * vs[d] + p_hat * m[d];
     F[DIM + 1] = mflux * h_hat;
 }
 
@@ -195,24 +198,26 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
         const double il_b = inv_log_mean(q1.beta, q2.beta, r_b);
         const double rho_mean = 0.5 * rsum;
         const double drho = q2.rho - q1.rho;
+        // the kinetic parts through v1 . v2: h-hat's |mean|^2 - (|v1|^2 + |v2|^2)/4 = (v1 . v2)/2,
+        // and the energy jump's sum of drho v1_d v2_d / 2 is drho (v1 . v2)/2
         double vm[DIM];
-        double Vn = 0.0, h_hat = 0.0, diss = 0.0;
+        double Vn = 0.0, vdot = 0.0, dvm = 0.0, diss = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             vm[d] = 0.5 * (q1.vel[d] + q2.vel[d]);
             const double dv = q2.vel[d] - q1.vel[d];
             Vn += vm[d] * n[d];
-            h_hat += -0.25 * (q1.vel[d] * q1.vel[d] + q2.vel[d] * q2.vel[d]) + vm[d] * vm[d];
+            vdot += q1.vel[d] * q2.vel[d];
             if (DIM == 3 && d == 2 && bug3)
-                diss += 0.5 * drho * q1.vel[d] * q2.vel[d] + rho_mean * dv + vm[d];     // :131 as written
+                diss += rho_mean * dv + vm[d];                                          // :131 as written
             else
-                diss += 0.5 * drho * q1.vel[d] * q2.vel[d] + rho_mean * dv * vm[d];     // :105,:118,:131
+                dvm += dv * vm[d];                                                      // :105,:118,:131
         }
         const double lam = fmax(q1.lam, q2.lam);
         const double p_hat = rho_mean * r_b;
-        h_hat += 0.5 * il_b * P.gammaM1Inverse + p_hat * il_rho;
+        const double h_hat = 0.5 * (vdot + il_b * P.gammaM1Inverse) + p_hat * il_rho;
         // 1/beta = 2 p / rho
-        diss += 0.5 * drho * P.gammaM1Inverse * il_b +
+        diss += 0.5 * drho * (vdot + P.gammaM1Inverse * il_b) + rho_mean * dvm +
                 rho_mean * P.gammaM1Inverse * (q2.por - q1.por);                        // :151
         const double hl = 0.5 * lam * nmag;
         const double mflux = rho_ln * Vn;
@@ -285,20 +290,22 @@ __device__ __forceinline__ void volume_flux_mx(const NodePrimMx<DIM> &a, const N
     const double p_hat = 0.5 * rsum * r_b;
     const float ilr = (float)il_rho, ilb = (float)il_b;
     const float rho_ln = rcp(ilr);
-    float mean[DIM];
-    float h_hat = 0.f, Vn = 0.f;
+    // as volume_flux: vs = v_a + v_b, the kinetic part of h-hat is (v_a . v_b)/2
+    float vs[DIM];
+    float vdot = 0.f, Vn2 = 0.f;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        mean[d] = 0.5f * (a.vel[d] + b.vel[d]);
-        h_hat += -0.25f * (a.vel[d] * a.vel[d] + b.vel[d] * b.vel[d]) + mean[d] * mean[d];
-        Vn += mean[d] * m[d];
+        vs[d] = a.vel[d] + b.vel[d];
+        vdot += a.vel[d] * b.vel[d];
+        Vn2 += vs[d] * m[d];
     }
-    h_hat += (float)(0.5 * il_b * gammaM1Inverse + p_hat * il_rho - h_ref);
+    const float h_hat = 0.5f * vdot + (float)(0.5 * il_b * gammaM1Inverse + p_hat * il_rho - h_ref);
     const float pd = (float)(p_hat - p_ref);
-    const float mflux = rho_ln * Vn;
+    const float mflux = 0.5f * rho_ln * Vn2;
+    const float hf = 0.5f * mflux;
     F[0] = mflux;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) F[1 + d] = mflux * mean[d] + pd * m[d];
+    for (int d = 0; d < DIM; d++) F[1 + d] = hf * vs[d] + pd * m[d];
     F[DIM + 1] = mflux * h_hat;
 }
 
@@ -319,21 +326,22 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
         const float drho = (float)(q2.rho - q1.rho);
         const float g1 = (float)P.gammaM1Inverse;
         float vm[DIM];
-        float Vn = 0.f, h_hat = 0.f, diss = 0.f;
+        float Vn = 0.f, vdot = 0.f, dvm = 0.f, diss = 0.f;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             const float v1 = (float)q1.vel[d], v2 = (float)q2.vel[d];
             vm[d] = 0.5f * (v1 + v2);
             const float dv = (float)(q2.vel[d] - q1.vel[d]);
             Vn += vm[d] * n[d];
-            h_hat += -0.25f * (v1 * v1 + v2 * v2) + vm[d] * vm[d];
+            vdot += v1 * v2;
             if (DIM == 3 && d == 2 && bug3)
-                diss += 0.5f * drho * v1 * v2 + rho_mean * dv + vm[d];
+                diss += rho_mean * dv + vm[d];
             else
-                diss += 0.5f * drho * v1 * v2 + rho_mean * dv * vm[d];
+                dvm += dv * vm[d];
         }
+        diss += 0.5f * drho * vdot + rho_mean * dvm;
         const float lam = (float)fmax(q1.lam, q2.lam);
-        h_hat += (float)(0.5 * il_b * P.gammaM1Inverse + p_hat * il_rho - h_ref);
+        const float h_hat = 0.5f * vdot + (float)(0.5 * il_b * P.gammaM1Inverse + p_hat * il_rho - h_ref);
         // diss - h_ref drho: the energy jump measured from h_ref times the density jump
         const double drho64 = q2.rho - q1.rho;
         diss += (float)(drho64 * (0.5 * P.gammaM1Inverse * il_b - h_ref)) +
