@@ -234,7 +234,7 @@ int prandtl_op_vjp(prandtl_op op, const double *u, const double *lam, double *ou
  * call hipMalloc -- code under HIP graph capture, tight serving loops, harnesses that forbid it --
  * while prandtl_op_vjp allocates the handle's work arrays on its first call.  The work arrays live
  * in a fixed 64 MiB device array the library holds on every GPU it runs on until the process ends
- * (per node: 72 B for a 3D Euler operator; 3D Navier-Stokes at p = 3 adds 140 B); an operator that
+ * (per node: 56 B for a 3D Euler operator; 3D Navier-Stokes at p = 3 adds 140 B); an operator that
  * needs more is refused with PRANDTL_ERR_ARG.  Every call on one GPU shares that array: calls that
  * overlap on one GPU (other streams, other threads) race.  Same result as prandtl_op_vjp. */
 int prandtl_op_vjp_noalloc(prandtl_op op, const double *u, const double *lam, double *out, prandtl_stream stream);

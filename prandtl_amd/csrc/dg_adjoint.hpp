@@ -45,10 +45,10 @@ __host__ __device__ inline double inv(double x) {
 }
 
 // per-node quantities the volume adjoint reads for every pair: density, velocity, beta =
-// rho/(2p), their reciprocals and logarithms (computed once per node, not once per pair)
+// rho/(2p) and their reciprocals (computed once per node, not once per pair)
 template <int DIM>
 struct NodeQ {
-    double rho, v[DIM], beta, irho, ibeta, lrho, lbeta;
+    double rho, v[DIM], beta, irho, ibeta;
 };
 
 template <int DIM>
@@ -64,19 +64,17 @@ __host__ __device__ inline NodeQ<DIM> node_q(const double *s, double gm1) {
     const double p = gm1 * (s[DIM + 1] - 0.5 * q.rho * vsq);
     q.beta = 0.5 * q.rho * inv(p);
     q.ibeta = inv(q.beta);
-    q.lrho = log(q.rho);
-    q.lbeta = log(q.beta);
     return q;
 }
 
-// logarithmic mean L of (x, y), its reciprocal and its partials, given 1/x, 1/y, ln x, ln y;
-// the two branches of the forward (series in u = ((y-x)/(y+x))^2 below 1e-4, closed form
-// otherwise).  On the series branch L = (x+y)/(2P), P = 1 + u/3 + u^2/5 + u^3/7, so 1/L =
+// logarithmic mean L of (x, y), its reciprocal and its partials, given 1/x and 1/y; the two
+// branches of the forward (series in u = ((y-x)/(y+x))^2 below 1e-4, closed form otherwise, with
+// ln(y/x) taken only there).  On the series branch L = (x+y)/(2P), P = 1 + u/3 + u^2/5 + u^3/7, so 1/L =
 // 2P/(x+y) needs no division, and 1/P is itself the series 1 - u/3 - 4u^2/45 - 44u^3/945 (the
 // omitted u^4 term is below 1e-17 relative for u < 1e-4); with u = f^2, f = (y-x)/(x+y):
 // dL/dx = 1/(2P) + c y, dL/dy = 1/(2P) - c x, c = 2 P'(u) f / (P^2 (x+y)).
-__host__ __device__ inline void log_mean_dq(double x, double y, double ix, double iy, double lx, double ly, double &L,
-                                            double &iL, double &Lx, double &Ly) {
+__host__ __device__ inline void log_mean_dq(double x, double y, double ix, double iy, double &L, double &iL,
+                                            double &Lx, double &Ly) {
     const double s = x + y, r = inv(s), f = (y - x) * r, u = f * f;
     if (u < 1e-4) {
         const double P = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
@@ -88,7 +86,7 @@ __host__ __device__ inline void log_mean_dq(double x, double y, double ix, doubl
         Lx = 0.5 * iP + c * y;
         Ly = 0.5 * iP - c * x;
     } else {
-        const double lg = ly - lx, ilg = inv(lg), d = y - x;
+        const double lg = log(y * ix), ilg = inv(lg), d = y - x;
         L = d * ilg;
         iL = lg * inv(d);
         Lx = (d * ix - lg) * ilg * ilg;
@@ -118,8 +116,8 @@ template <int DIM>
 __host__ __device__ inline void ec_core_back(const NodeQ<DIM> &a, const NodeQ<DIM> &b, const double *m,
                                              double gm1inv, const double *Fb, PrimB<DIM> &ga, PrimB<DIM> &gb) {
     double Lr, iLr, Lr1, Lr2, Lb, iLb, Lb1, Lb2;
-    log_mean_dq(a.rho, b.rho, a.irho, b.irho, a.lrho, b.lrho, Lr, iLr, Lr1, Lr2);
-    log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, iLb, Lb1, Lb2);
+    log_mean_dq(a.rho, b.rho, a.irho, b.irho, Lr, iLr, Lr1, Lr2);
+    log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, Lb, iLb, Lb1, Lb2);
     double vs[DIM], Vn2 = 0.0, vdot = 0.0;   // vs = v_1 + v_2 = 2 vbar
     for (int d = 0; d < DIM; d++) {
         vs[d] = a.v[d] + b.v[d];
@@ -207,7 +205,7 @@ __host__ __device__ inline void face_flux_vjp(int kind, const double *s1, const 
         nmag = sqrt(nmag);
         const double hl = 0.5 * nmag;
         double Lb, iLb, Lb1, Lb2;
-        log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, a.lbeta, b.lbeta, Lb, iLb, Lb1, Lb2);
+        log_mean_dq(a.beta, b.beta, a.ibeta, b.ibeta, Lb, iLb, Lb1, Lb2);
         const double rho_mean = 0.5 * (a.rho + b.rho), drho = b.rho - a.rho;
         double diss = 0.0, vsq1 = 0.0, vsq2 = 0.0;
         for (int d = 0; d < DIM; d++) {
@@ -268,10 +266,14 @@ __host__ __device__ inline void face_flux_vjp(int kind, const double *s1, const 
         // two one-sided derivatives (what central differences of the forward measure)
         const double l1b = l1 > l2 ? lam_b : (l1 < l2 ? 0.0 : 0.5 * lam_b);
         const double l2b = lam_b - l1b;
-        if (vn1 > 0.0)
-            for (int d = 0; d < DIM; d++) ga.v[d] += l1b * a.v[d] / vn1;
-        if (vn2 > 0.0)
-            for (int d = 0; d < DIM; d++) gb.v[d] += l2b * b.v[d] / vn2;
+        if (vn1 > 0.0) {
+            const double g1 = l1b * inv(vn1);
+            for (int d = 0; d < DIM; d++) ga.v[d] += g1 * a.v[d];
+        }
+        if (vn2 > 0.0) {
+            const double g2 = l2b * inv(vn2);
+            for (int d = 0; d < DIM; d++) gb.v[d] += g2 * b.v[d];
+        }
         // dc/dbeta = -c / (2 beta)
         ga.beta += -l1b * c1 * 0.5 * a.ibeta;
         gb.beta += -l2b * c2 * 0.5 * b.ibeta;

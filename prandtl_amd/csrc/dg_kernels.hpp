@@ -1501,7 +1501,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
 // reverse, given the adjoints (ub, kb) of (u_s, k_s):
 //     lambda = kb + B_s ub,   ub_{s-1} = ub + dt (dL/du(u_{s-1}))^T lambda,   kb_{s-1} = A_s lambda.
 // k_vjp_prep evaluates, once per node, the quantities every pair reads (density, velocity,
-// beta, their reciprocals and logarithms; SoA, NQ = DIM + 6 arrays) and lambda.  k_vjp then
+// beta and their reciprocals; SoA, NQ = DIM + 4 arrays) and lambda.  k_vjp then
 // works as k_rhs (one thread per GLL line, each symmetric pair once): a pair's cotangent is
 // Fb = -(1/J)(Dhat2[a][b] lambda_a + Dhat2[b][a] lambda_b), its reverse accumulates the
 // adjoints of (rho, v, beta) of the line's nodes in registers, and those are mapped to
@@ -1547,8 +1547,6 @@ __global__ void __launch_bounds__(256) k_vjp_prep(const VjpArgs<NP> A) {
         A.q[(DIM + 1) * nodes + gi] = q.beta;
         A.q[(DIM + 2) * nodes + gi] = q.irho;
         A.q[(DIM + 3) * nodes + gi] = q.ibeta;
-        A.q[(DIM + 4) * nodes + gi] = q.lrho;
-        A.q[(DIM + 5) * nodes + gi] = q.lbeta;
         if (A.mode == VJP_LSRK) {
 #pragma unroll
             for (int v = 0; v < NEQ; v++) A.lamw[v * nodes + gi] = A.kb[v * nodes + gi] + A.cb * A.ub[v * nodes + gi];
@@ -1565,8 +1563,6 @@ __device__ __forceinline__ adj::NodeQ<DIM> load_q(const double *Q, size_t nodes,
     q.beta = Q[(DIM + 1) * nodes + gi];
     q.irho = Q[(DIM + 2) * nodes + gi];
     q.ibeta = Q[(DIM + 3) * nodes + gi];
-    q.lrho = Q[(DIM + 4) * nodes + gi];
-    q.lbeta = Q[(DIM + 5) * nodes + gi];
     return q;
 }
 
@@ -1590,7 +1586,7 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
     double invJ = 0.0, adjm[DIM][DIM];
     if (active) {
         const double *g = A.geom + (size_t)e * (1 + DIM * DIM);
-        invJ = 1.0 / g[0];
+        invJ = adj::inv(g[0]);
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
@@ -1669,7 +1665,7 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
                 const int nown = base + (side ? (NP - 1) * st : 0);
                 const int nnb = base + (side ? 0 : (NP - 1) * st);
                 const double swo = A.T.inv_w0 * invJ;
-                const double swn = A.T.inv_w0 / A.geom[(size_t)en * (1 + DIM * DIM)];
+                const double swn = A.T.inv_w0 * adj::inv(A.geom[(size_t)en * (1 + DIM * DIM)]);
                 double so[NEQ], sn[NEQ], fb[NEQ], b1[NEQ], b2[NEQ];
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) {

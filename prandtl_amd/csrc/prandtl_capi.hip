@@ -257,7 +257,7 @@ struct VjpSpec {
 
 // device work arrays of one operator-adjoint evaluation
 struct VjpBuffers {
-    double *q = nullptr;      // per-node quantities, (dim + 6) x nodes
+    double *q = nullptr;      // per-node quantities, (dim + 4) x nodes
     double *lamw = nullptr;   // VJP_LSRK: the stage cotangent (state size; unused by VJP_PLAIN, VJP_SSP)
     double *gf = nullptr;     // NS: lifting adjoint at face points, dim x 2 x neq x NE x Np^(dim-1)
     double *wb = nullptr;     // NS: partial adjoint of the entropy variables (state size)
@@ -271,7 +271,7 @@ struct VjpSizes {
 
 VjpSizes vjp_sizes(const prandtl_op_s *op) {
     const size_t nodes = (size_t)op->NE * op->nn;
-    VjpSizes z{(size_t)(op->dim + 6) * nodes, op->size, 0, 0, 0};
+    VjpSizes z{(size_t)(op->dim + 4) * nodes, op->size, 0, 0, 0};
     if (op->phys.viscous) {
         z.gf = (size_t)op->dim * 2 * op->neq * op->NE * (op->nn / op->np);
         z.wb = z.sb = op->size;
