@@ -74,9 +74,28 @@ enum {
     PRANDTL_BC_SUBSONIC_INFLOW_RV = 7, PRANDTL_BC_SUBSONIC_INFLOW_PTTT = 8, PRANDTL_BC_SUBSONIC_OUTFLOW_P = 9,
     PRANDTL_BC_NOSLIP_ADIABATIC = 10, PRANDTL_BC_NOSLIP_ISOTHERMAL = 11
 };
+/* Boundary states that vary over the boundary and in time -- the reference's function-valued
+ * conditions (ConditionFactory, src/Simulation/ConditionFactory.hpp:16-40, registered by a test
+ * case and evaluated at each face point's physical coordinates x and the operator's time t,
+ * Simulation.cpp:664-790).  For SUPERSONIC_INFLOW and SPECIFIED_STATE, whose value is a state:
+ *   PROFILE_CONSTANT  the state q (a zero-initialised profile);
+ *   PROFILE_FRONT     a planar front: q where x[i0] < c0 + (x[i1] + c1 t) / c2, else the state
+ *                     a[5 .. 4+neq] ahead of it; a[0] = i0, a[1] = i1, a[2] = c0, a[3] = c1,
+ *                     a[4] = c2 (DoubleMachReflectionTopBCFunction, TestCases/Euler/2D/
+ *                     DoubleMachReflection/DoubleMachReflection.hpp:32-52: i0 = 0, i1 = 1,
+ *                     c0 = 1/6, c1 = 20, c2 = sqrt 3);
+ *   PROFILE_RAMP      an inflow whose velocity rises over a time t_r:
+ *                     r = clamp((1 + tanh((t - t_r/2) / (t_r/10))) / 2, 0, 1) and the state
+ *                     (rho, rho v r, p / (gamma - 1) + rho |v r|^2 / 2) with a[0] = t_r,
+ *                     a[1] = rho, a[2 .. 1+dim] = v, a[2+dim] = p (NagashimaRampingInletBC,
+ *                     TestCases/Euler/2D/Nagashima_Ramjet/Nagashima_Ramjet.hpp:40-53).
+ * A non-constant profile needs the mesh's node coordinates (prandtl_mesh_desc.node_coords). */
+enum { PRANDTL_PROFILE_CONSTANT = 0, PRANDTL_PROFILE_FRONT = 1, PRANDTL_PROFILE_RAMP = 2 };
 typedef struct {
     int type;     /* PRANDTL_BC_* */
     double q[8];
+    int profile;  /* PRANDTL_PROFILE_* (SUPERSONIC_INFLOW, SPECIFIED_STATE; 0 otherwise) */
+    double a[16]; /* the profile's parameters */
 } prandtl_bc;
 
 /* Conforming mesh -- the reference's ParMesh and its finite-element space as the
@@ -99,6 +118,9 @@ typedef struct {
  *       directions, in increasing order) is the neighbour's point (u1, u2) of face f'
  *       with (a, b) = swap ? (t2, t1) : (t1, t2), u1 = flip1 ? Np-1-a : a,
  *       u2 = flip2 ? Np-1-b : b; NULL: the aligned-lattice matching (f' = f ^ 1).
+ * node_coords[(e * n + node) * dim]: the physical coordinates of every node (MFEM's
+ *     Tr.Transform of the face points, BdrFaceIntegrator.cpp:83); needed only when a
+ *     boundary condition has a non-constant profile (NULL otherwise).
  * All arrays are host pointers, copied at create time.  Boundary faces and general
  * meshes run in FP64 (mixed precision and the adjoint refuse them). */
 typedef struct {
@@ -111,6 +133,7 @@ typedef struct {
     const prandtl_bc *bc;
     const double *node_geometry;
     const int *face_orientation;
+    const double *node_coords;
 } prandtl_mesh_desc;
 
 /* ref: src/Physics/Physics.hpp:10-38 and NavierStokesFlux(dim, gamma, Pr, mu, ...),
@@ -171,6 +194,16 @@ int prandtl_op_mult(prandtl_op op, const double *u, double *dudt, prandtl_stream
  * grad[c * size + v * NE * n + e * n + node] = d(rho, vel, p)_v / dx_c.
  * Navier-Stokes operators only. */
 int prandtl_op_mult_grad(prandtl_op op, const double *u, double *dudt, double *grad, prandtl_stream stream);
+
+/* The time at which prandtl_op_mult and prandtl_op_mult_grad evaluate time-dependent boundary
+ * states (default 0) -- ref: TimeDependentOperator::SetTime / DGSEMOperator::GetTimeRef,
+ * DGSEMOperator.hpp:111-114 (the boundary integrators hold a reference to that time,
+ * Simulation.cpp:522).  The steppers take their stage times from their own t argument instead:
+ * t, t + dt, t + dt/2 for SSPRK3 (MFEM RK3SSPSolver), t + c_i dt for LSRK45 (the
+ * Carpenter-Kennedy c_i), t, t + a dt for RK2, t, t + dt/2, t + dt/2, t + dt for RK4, with t
+ * advanced by dt after each step. */
+int prandtl_op_set_time(prandtl_op op, double t);
+int prandtl_op_get_time(prandtl_op op, double *t);
 
 /* Max characteristic speed max(|V| + c) of the last Mult (Euler and NS) --
  * ref: DGSEMOperator::GetMaxCharSpeed, DGSEMOperator.hpp:106-109.  Waits for the stream the

@@ -97,6 +97,7 @@ struct Mesh {
     std::vector<prandtl_bc> bc;         // boundary conditions (empty for a periodic mesh)
     std::vector<double> node_geometry;  // general mesh: [e][node][1 + dim^2] (empty: affine elements)
     std::vector<int> face_orientation;  // general mesh: [e][2 dim] (empty: aligned lattice faces)
+    std::vector<double> node_coords;    // [e][node][dim]: needed when a boundary state has a front profile
 
     // Periodic lattice of n[0] x n[1] (x n[2]) axis-aligned boxes of the domain
     // [origin, origin + length] (the generated meshes of the benchmark cases).
@@ -176,7 +177,8 @@ public:
                              mesh.geometry.empty() ? nullptr : mesh.geometry.data(),
                              (int)mesh.bc.size(), mesh.bc.empty() ? nullptr : mesh.bc.data(),
                              mesh.node_geometry.empty() ? nullptr : mesh.node_geometry.data(),
-                             mesh.face_orientation.empty() ? nullptr : mesh.face_orientation.data()};
+                             mesh.face_orientation.empty() ? nullptr : mesh.face_orientation.data(),
+                             mesh.node_coords.empty() ? nullptr : mesh.node_coords.data()};
         prandtl_physics ph{phys.gamma, phys.Pr, phys.mu, phys.R_gas, phys.viscous ? 1 : 0, phys.sutherland ? 1 : 0};
         prandtl_opts o{(int)opts.surface_flux, (int)opts.precision, opts.bug_compat, opts.stage_hooks,
                        opts.blending ? 1 : 0, opts.alpha_max, opts.alpha_min};
@@ -187,10 +189,12 @@ public:
     DGSEMOperator &operator=(const DGSEMOperator &) = delete;
     ~DGSEMOperator() override { prandtl_op_destroy(op_); }
 
-    // dudt = L(u), fully overwritten (DGSEMOperator.cpp:537-591)
+    // dudt = L(u), fully overwritten (DGSEMOperator.cpp:537-591), with time-dependent boundary
+    // states at the operator's time (SetTime, or GetTimeRef as the reference's integrators hold it)
     void Mult(const Vector &u, Vector &dudt) const override {
         CheckState(u, "u");
         CheckState(dudt, "dudt");
+        check(prandtl_op_set_time(op_, t_));
         check(prandtl_op_mult(op_, u.Read(), dudt.Write(), stream_));
     }
     // a state vector of this operator: (dim + 2) x elements x nodes doubles (GetTrueVSize)

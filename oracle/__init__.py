@@ -31,15 +31,23 @@ BC_RIEMANN_INVARIANT, BC_SUBSONIC_INFLOW_RV, BC_SUBSONIC_INFLOW_PTTT, BC_SUBSONI
 BC_NOSLIP_ADIABATIC, BC_NOSLIP_ISOTHERMAL = 10, 11
 
 
+# boundary-state profiles (constant, planar front, velocity ramp), parameters as in include/prandtl_amd.h
+PROFILE_CONSTANT, PROFILE_FRONT, PROFILE_RAMP = 0, 1, 2
+
+
 class OrcBc(ctypes.Structure):
-    _fields_ = [("type", ctypes.c_int), ("q", ctypes.c_double * 8)]
+    _fields_ = [("type", ctypes.c_int), ("q", ctypes.c_double * 8), ("profile", ctypes.c_int),
+                ("a", ctypes.c_double * 16)]
 
 
-def bc(type_: int, *q) -> OrcBc:
+def bc(type_: int, *q, profile: int = PROFILE_CONSTANT, a=()) -> OrcBc:
     b = OrcBc()
     b.type = int(type_)
     for i, v in enumerate(q):
         b.q[i] = float(v)
+    b.profile = int(profile)
+    for i, v in enumerate(a):
+        b.a[i] = float(v)
     return b
 
 
@@ -141,6 +149,14 @@ def _configure(L):
         getattr(L, name).restype = None
     L.orc_bc_lifting_flux.argtypes = [ctypes.c_int, pp, bp, dp, dp, dp]
     L.orc_bc_lifting_flux.restype = None
+    L.orc_set_boundary_context.argtypes = [dp, ctypes.c_double]
+    L.orc_set_boundary_context.restype = None
+    L.orc_get_time.argtypes = []
+    L.orc_get_time.restype = ctypes.c_double
+    L.orc_set_time.argtypes = [ctypes.c_double]
+    L.orc_set_time.restype = None
+    L.orc_bc_state.argtypes = [ctypes.c_int, pp, bp, dp, ctypes.c_double, dp]
+    L.orc_bc_state.restype = ctypes.c_int
     L.orc_blend_coef.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, dp, dp]
     L.orc_blend_coef.restype = ctypes.c_int
@@ -457,8 +473,25 @@ def rk_run_blend(mesh, ph, u, scheme: int, nsteps: int, dt: float, hooks: int = 
 
 # ---------------------------------------------------------------------------
 # boundary conditions (prandtl_oracle.cpp)
-def rhs_ex(mesh, ph, u, bcs=None, alpha=None, want_grad=False, fma=False):
-    """The general RHS with boundary conditions (and blending): (dudt, lambda, mag[, grad, gmag])."""
+def _xyz(mesh, bcs):
+    """Node coordinates [e][node][dim] when a condition's state has a front (else None)."""
+    if not bcs or not any(b.profile == PROFILE_FRONT for b in bcs):
+        return None
+    return np.ascontiguousarray(np.transpose(mesh.node_coords(), (1, 2, 0)), dtype=np.float64)
+
+
+def rhs_ex(mesh, ph, u, bcs=None, alpha=None, want_grad=False, fma=False, time=0.0):
+    """The general RHS with boundary conditions (and blending) at `time` (the function-valued
+    boundary states): (dudt, lambda, mag[, grad, gmag])."""
+    xyz = _xyz(mesh, bcs)
+    lib(fma).orc_set_boundary_context(_dp(xyz), float(time))
+    try:
+        return _rhs_ex(mesh, ph, u, bcs, alpha, want_grad, fma)
+    finally:
+        lib(fma).orc_set_boundary_context(None, 0.0)
+
+
+def _rhs_ex(mesh, ph, u, bcs, alpha, want_grad, fma):
     u = np.ascontiguousarray(u, dtype=np.float64)
     dudt = np.zeros_like(u)
     mag = np.zeros_like(u)
@@ -479,17 +512,33 @@ def rhs_ex(mesh, ph, u, bcs=None, alpha=None, want_grad=False, fma=False):
 
 
 def rk_run_ex(mesh, ph, u, scheme: int, nsteps: int, dt: float, bcs=None, hooks: int = 0, blend=False,
-              alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN):
-    """The general stepper (boundary conditions, blending, stage hooks): (u, counts)."""
+              alpha_max=ALPHA_MAX, alpha_min=ALPHA_MIN, t0=0.0, return_time=False):
+    """The general stepper (boundary conditions, blending, stage hooks) from time t0, each stage at
+    its stage time: (u, counts[, t_end])."""
     u = np.array(u, dtype=np.float64, copy=True)
     counts = np.zeros(2, dtype=np.int32)
     nbc, arr = _bcs(bcs)
     ng, fo = _gen(mesh)
-    rc = lib().orc_rk_run_gen(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom), _dp(ng),
+    xyz = _xyz(mesh, bcs)
+    L = lib()
+    L.orc_set_boundary_context(_dp(xyz), float(t0))
+    try:
+        rc = L.orc_rk_run_gen(mesh.dim, mesh.order, mesh.num_elements, _ip(mesh.neighbors), _dp(mesh.geom), _dp(ng),
                               _ip(fo), ctypes.byref(ph), nbc, arr, int(scheme), int(nsteps), float(dt), int(hooks),
                               int(blend), float(alpha_max), float(alpha_min), _dp(u), _ip(counts))
+        t_end = L.orc_get_time()
+    finally:
+        L.orc_set_boundary_context(None, 0.0)
     assert rc == 0, rc
-    return u, counts
+    return (u, counts, t_end) if return_time else (u, counts)
+
+
+def bc_state(dim, ph, b, x, t):
+    """The state of condition b at the point x and time t (constant or profile)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s2 = np.zeros(dim + 2)
+    assert lib().orc_bc_state(dim, ctypes.byref(ph), ctypes.byref(b), _dp(x), float(t), _dp(s2)) == 0
+    return s2
 
 
 def bc_outer_state(dim, ph, b, s1, nor):

@@ -519,14 +519,59 @@ void RotateStateRef(int dim, real *state, const real *n) {
     m[1] = mv;
 }
 
-// ComputeOuterInviscidState of the BCs that define an outer state.
-void OuterState(const Phys &P, const orc_bc &bc, const real *s1, const real *nor, real *s2) {
+// The value of a state-valued condition at the face point x and time t: the constant vector, or
+// the function a test case registered with the ConditionFactory (src/Simulation/
+// ConditionFactory.hpp:16-40), which the reference evaluates per face point through a
+// VectorFunctionCoefficient / VectorTDFunctionCoefficient at the operator's time
+// (Simulation.cpp:664-790; SpecifiedStateBdrFaceIntegrator.cpp:26-40).  Double arithmetic, as
+// the registered functions compute.
+enum BcProfile { PROFILE_CONSTANT = 0, PROFILE_FRONT = 1, PROFILE_RAMP = 2 };
+void BcState(int dim, double gamma, const orc_bc &bc, const double *x, double t, double *y) {
+    const int neq = dim + 2;
+    if (bc.profile == PROFILE_FRONT) {
+        // DoubleMachReflectionTopBCFunction (TestCases/Euler/2D/DoubleMachReflection/
+        // DoubleMachReflection.hpp:35-51): if (x(0) < 1.0 / 6.0 + (x(1) + 20.0 * t) / sqrt(3)) the
+        // post-shock state, else the pre-shock state; here with the coordinates, offset, speed and
+        // divisor as parameters
+        const int i0 = (int)bc.a[0], i1 = (int)bc.a[1];
+        if (x[i0] < bc.a[2] + (x[i1] + bc.a[3] * t) / bc.a[4])
+            for (int v = 0; v < neq; v++) y[v] = bc.q[v];
+        else
+            for (int v = 0; v < neq; v++) y[v] = bc.a[5 + v];
+    } else if (bc.profile == PROFILE_RAMP) {
+        // NagashimaRampingInletBC (TestCases/Euler/2D/Nagashima_Ramjet/Nagashima_Ramjet.hpp:43-53):
+        // s = tanh((t - 0.5 t_ramp)/(0.1 t_ramp)), ramp = clamp(0.5 (1 + s), 0, 1), u = V ramp,
+        // y = (rho, rho u, pi/(gamma - 1) + 0.5 rho u u)
+        const double t_ramp = bc.a[0];
+        const double sv = std::tanh((t - 0.5 * t_ramp) / (0.1 * t_ramp));
+        const double ramp = std::clamp(0.5 * (1.0 + sv), 0.0, 1.0);
+        const double rho = bc.a[1];
+        y[0] = rho;
+        double e = bc.a[2 + dim] / (gamma - 1.0);
+        for (int d = 0; d < dim; d++) {
+            const double u = bc.a[2 + d] * ramp;
+            y[1 + d] = rho * u;
+            e += 0.5 * rho * u * u;
+        }
+        y[neq - 1] = e;
+    } else {
+        for (int v = 0; v < neq; v++) y[v] = bc.q[v];
+    }
+}
+
+// ComputeOuterInviscidState of the BCs that define an outer state; x, t: the face point's
+// coordinates and the operator's time (read by the function-valued states only).
+void OuterState(const Phys &P, const orc_bc &bc, const real *s1, const real *nor, real *s2, const double *x,
+                double t) {
     const int dim = P.dim, neq = P.neq;
     switch (bc.type) {
         case BC_SUPERSONIC_INFLOW:    // SupersonicInflowBdrFaceIntegrator.cpp:34-48
-        case BC_SPECIFIED_STATE:      // SpecifiedStateBdrFaceIntegrator.cpp:26-40
-            for (int v = 0; v < neq; v++) s2[v] = bc.q[v];
+        case BC_SPECIFIED_STATE: {    // SpecifiedStateBdrFaceIntegrator.cpp:26-40
+            double y[MAXEQ];
+            BcState(dim, (double)P.gamma, bc, x, t, y);
+            for (int v = 0; v < neq; v++) s2[v] = y[v];
             break;
+        }
         case BC_SUPERSONIC_OUTFLOW:   // SupersonicOutflowBdrFaceIntegrator.cpp:12-15
             for (int v = 0; v < neq; v++) s2[v] = s1[v];
             break;
@@ -612,7 +657,8 @@ void OuterState(const Phys &P, const orc_bc &bc, const real *s1, const real *nor
 // ComputeBdrFaceInviscidFlux: the slip wall's (SlipWallBdrFaceIntegrator.cpp:15-57, inherited by
 // the no-slip walls), the symmetry BC's (SymmetryBdrFaceIntegrator.cpp:10-32), otherwise the
 // surface flux with the outer state (BdrFaceIntegrator.cpp:245-253).
-real BdrInviscidFlux(const Phys &P, const orc_bc &bc, const real *s1, const real *nor, real *flux) {
+real BdrInviscidFlux(const Phys &P, const orc_bc &bc, const real *s1, const real *nor, real *flux, const double *x,
+                     double t) {
     const int dim = P.dim, neq = P.neq;
     real s2[MAXEQ];
     if (bc.type == BC_SLIP_WALL || bc.type == BC_NOSLIP_ADIABATIC || bc.type == BC_NOSLIP_ISOTHERMAL) {
@@ -651,13 +697,13 @@ real BdrInviscidFlux(const Phys &P, const orc_bc &bc, const real *s1, const real
         for (int d = 0; d < dim; d++) s2[1 + d] += -2.0 * mn * un[d];
         return FaceFlux(P, s1, s2, nor, flux);
     }
-    OuterState(P, bc, s1, nor, s2);
+    OuterState(P, bc, s1, nor, s2, x, t);
     return FaceFlux(P, s1, s2, nor, flux);
 }
 
 // ComputeBdrFaceViscousFlux: F_v . nor (row 0 zero); dq[c] the element's primitive gradients.
 void BdrViscousFlux(const Phys &P, const orc_bc &bc, const real *s1, const real (*dq)[MAXEQ], const real *nor,
-                    real *fn) {
+                    real *fn, const double *x, double t) {
     const int dim = P.dim, neq = P.neq;
     real s2[MAXEQ], zero[3][MAXEQ] = {}, flux[MAXEQ * 3];
     const real(*g)[MAXEQ] = dq;
@@ -668,11 +714,11 @@ void BdrViscousFlux(const Phys &P, const orc_bc &bc, const real *s1, const real 
         case BC_NOSLIP_ISOTHERMAL:
             break;
         case BC_SUBSONIC_INFLOW_PTTT:  // the outer state with the element's gradients (:75-99)
-            OuterState(P, bc, s1, nor, s2);
+            OuterState(P, bc, s1, nor, s2, x, t);
             sv = s2;
             break;
         default:                       // the outer state with zero gradients
-            OuterState(P, bc, s1, nor, s2);
+            OuterState(P, bc, s1, nor, s2, x, t);
             sv = s2;
             g = zero;
     }
@@ -693,7 +739,8 @@ void BdrViscousFlux(const Phys &P, const orc_bc &bc, const real *s1, const real 
 }
 
 // ComputeBdrFaceLiftingFlux: w1 the element's entropy variables at the face point.
-void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const real *w1, const real *nor, real *fn) {
+void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const real *w1, const real *nor, real *fn, const double *x,
+                    double t) {
     const int dim = P.dim, neq = P.neq;
     if (bc.type == BC_NOSLIP_ADIABATIC || bc.type == BC_NOSLIP_ISOTHERMAL) {
         // NoSlipAdiabWallBdrFaceIntegrator.cpp:74-95, NoSlipIsothWallBdrFaceIntegrator.cpp:41-66
@@ -707,7 +754,7 @@ void BdrLiftingFlux(const Phys &P, const orc_bc &bc, const real *w1, const real 
     // BdrFaceIntegrator.cpp:275-282
     real s1[MAXEQ], s2[MAXEQ], w2[MAXEQ];
     Entropy2Conserv(P, w1, s1);
-    OuterState(P, bc, s1, nor, s2);
+    OuterState(P, bc, s1, nor, s2, x, t);
     Conserv2Entropy(P, s2, w2);
     for (int k = 0; k < neq; k++) fn[k] = (w2[k] - w1[k]) * 0.5;
 }
@@ -733,6 +780,10 @@ struct Mesh {
     // coordinate, bit 4 the second, bit 5 swaps them); null: affine elements, aligned faces.
     const real *ngeom = nullptr;
     const int *forient = nullptr;
+    // function-valued boundary states: node coordinates [e][node][dim] and the operator's time
+    const double *xyz = nullptr;
+    double time = 0.0;
+    const double *xat(int e, int node) const { return xyz ? xyz + ((size_t)e * nn + node) * dim : nullptr; }
     real Jn(int e, int node) const {
         return ngeom ? ngeom[((size_t)e * nn + node) * (1 + dim * dim)] : J(e);
     }
@@ -1119,7 +1170,7 @@ void LiftingAndPrimGrad(const Mesh &M, const Phys &P, const real *u, real *grad,
                             for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adjn(e, n1, d, c);   // outward
                             real w1[MAXEQ], f[MAXEQ];
                             for (int v = 0; v < neq; v++) w1[v] = went[gidx(M, v, e, n1)];
-                            BdrLiftingFlux(P, bc, w1, nor, f);
+                            BdrLiftingFlux(P, bc, w1, nor, f, M.xat(e, n1), M.time);
                             for (int v = 0; v < neq; v++)
                                 for (int c = 0; c < dim; c++) {
                                     grad[(size_t)c * size + gidx(M, v, e, n1)] += 1.0 / (M.w[0] * J) * (f[v] * nor[c]);
@@ -1348,7 +1399,7 @@ real RhsEval(const Mesh &M, const Phys &P, const real *u, real *dudt, real *grad
                             for (int c = 0; c < dim; c++) nor[c] = (side ? 1.0 : -1.0) * M.adjn(e, n1, d, c);
                             real s1[MAXEQ], f[MAXEQ], dU[MAXEQ], aU[MAXEQ];
                             for (int v = 0; v < neq; v++) s1[v] = u[gidx(M, v, e, n1)];
-                            lam = std::max(lam, BdrInviscidFlux(P, bc, s1, nor, f));
+                            lam = std::max(lam, BdrInviscidFlux(P, bc, s1, nor, f, M.xat(e, n1), M.time));
                             for (int v = 0; v < neq; v++) {
                                 dU[v] = -f[v];
                                 aU[v] = std::fabs(f[v]);
@@ -1357,7 +1408,7 @@ real RhsEval(const Mesh &M, const Phys &P, const real *u, real *dudt, real *grad
                                 real dq[3][MAXEQ], fv[MAXEQ];
                                 for (int c = 0; c < dim; c++)
                                     for (int v = 0; v < neq; v++) dq[c][v] = grad[(size_t)c * size + gidx(M, v, e, n1)];
-                                BdrViscousFlux(P, bc, s1, dq, nor, fv);
+                                BdrViscousFlux(P, bc, s1, dq, nor, fv, M.xat(e, n1), M.time);
                                 for (int v = 0; v < neq; v++) {
                                     dU[v] += fv[v];
                                     if (mag) {
@@ -1525,14 +1576,14 @@ int orc_rhs_mag(int dim, int order, int NE, const int *nbr, const double *geom, 
 int orc_bc_outer_state(int dim, const orc_phys *p, const orc_bc *bc, const double *s1, const double *nor,
                        double *s2) {
     Phys P = phys_of(dim, p);
-    OuterState(P, *bc, s1, nor, s2);
+    OuterState(P, *bc, s1, nor, s2, nullptr, 0.0);
     return 0;
 }
 
 double orc_bc_inviscid_flux(int dim, const orc_phys *p, const orc_bc *bc, const double *s1, const double *nor,
                             double *f) {
     Phys P = phys_of(dim, p);
-    return BdrInviscidFlux(P, *bc, s1, nor, f);
+    return BdrInviscidFlux(P, *bc, s1, nor, f, nullptr, 0.0);
 }
 
 void orc_bc_viscous_flux(int dim, const orc_phys *p, const orc_bc *bc, const double *s1, const double *dq,
@@ -1541,13 +1592,31 @@ void orc_bc_viscous_flux(int dim, const orc_phys *p, const orc_bc *bc, const dou
     double g[3][MAXEQ] = {};
     for (int c = 0; c < dim; c++)
         for (int v = 0; v < P.neq; v++) g[c][v] = dq[c * P.neq + v];
-    BdrViscousFlux(P, *bc, s1, g, nor, fv);
+    BdrViscousFlux(P, *bc, s1, g, nor, fv, nullptr, 0.0);
 }
 
 void orc_bc_lifting_flux(int dim, const orc_phys *p, const orc_bc *bc, const double *w1, const double *nor,
                          double *fl) {
     Phys P = phys_of(dim, p);
-    BdrLiftingFlux(P, *bc, w1, nor, fl);
+    BdrLiftingFlux(P, *bc, w1, nor, fl, nullptr, 0.0);
+}
+
+// the boundary context of the following evaluations (orc_set_boundary_context)
+static const double *g_bxyz = nullptr;
+static double g_btime = 0.0;
+
+void orc_set_boundary_context(const double *xyz, double t) {
+    g_bxyz = xyz;
+    g_btime = t;
+}
+double orc_get_time(void) { return g_btime; }
+void orc_set_time(double t) { g_btime = t; }
+
+int orc_bc_state(int dim, const orc_phys *p, const orc_bc *bc, const double *x, double t, double *s2) {
+    if (dim < 2 || dim > 3 || !bc || !s2) return -1;
+    if (bc->profile == PROFILE_FRONT && !x) return -1;
+    BcState(dim, p->gamma, *bc, x, t, s2);
+    return 0;
 }
 
 // The general RHS: boundary conditions bcs[0..nbc-1] for the faces whose neighbour entry is
@@ -1581,6 +1650,10 @@ int orc_rhs_gen(int dim, int order, int NE, const int *nbr, const double *geom, 
         if (nbr[i] < 0 && (-nbr[i] - 1 >= nbc || !bcs)) return -4;   // a boundary face without its condition
     M.bc = nbc > 0 ? bcs : nullptr;
     M.nbc = nbc;
+    M.xyz = g_bxyz;
+    M.time = g_btime;
+    for (int k = 0; k < nbc; k++)
+        if (bcs[k].profile == PROFILE_FRONT && !M.xyz) return -5;   // a front without node coordinates
     if (gmag && !grad) return -3;
     double lam = RhsEval(M, P, u, dudt, grad, mag, gmag, alpha);
     if (lambda_max) *lambda_max = lam;

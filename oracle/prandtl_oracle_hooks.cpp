@@ -472,12 +472,31 @@ int orc_rk_run_gen(int dim, int order, int NE, const int *nbr, const double *geo
         if (hooks & 1) counts[0] += FilterApply(dim, order, NE, x, gamma, bound.data(), nullptr);
         if (hooks & 2) counts[1] += LimiterApply(dim, order, NE, x, gamma, 1e-8, ngeom);
     };
-    auto rhs = [&](const double *x) {
+    // the stage times of MFEM's steppers (ODESolver::Step sets f's time before each Mult), from
+    // the time of the boundary context; t += dt after each step
+    double t = orc_get_time();
+    auto rhs_at = [&](const double *x, double c) {
+        orc_set_time(t + c * dt);
         if (blend) BlendingCoefficient(dim, order, NE, x, gamma, alpha_max, alpha_min, alpha.data(), nullptr);
         return orc_rhs_gen(dim, order, NE, nbr, geom, ngeom, forient, p, nbc, bcs, x, blend ? alpha.data() : nullptr,
                            k.data(), nullptr, nullptr, nullptr, nullptr);
     };
-    for (int s = 0; s < nsteps; s++) {
+    int stage = 0;   // the stage of the current step (its time: the scheme's table below)
+    auto rhs = [&](const double *x) {
+        // SSP-RK3 t, t + dt, t + dt/2 (RK3SSPSolver); LSRK4(5) the Carpenter-Kennedy c_i; forward
+        // Euler t; RK2 t, t + a dt (a = 2/3); RK4 t, t + dt/2, t + dt/2, t + dt
+        static const double C0[3] = {0.0, 1.0, 0.5};
+        static const double C1[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                                     2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+        static const double C3[2] = {0.0, 2.0 / 3.0};
+        static const double C4[4] = {0.0, 0.5, 0.5, 1.0};
+        const double c = scheme == 0 ? C0[stage] : scheme == 1 ? C1[stage] : scheme == 3 ? C3[stage]
+                                                                   : scheme == 4 ? C4[stage] : 0.0;
+        stage++;
+        return rhs_at(x, c);
+    };
+    for (int s = 0; s < nsteps; s++, t += dt) {
+        stage = 0;
         if (hooks & 1) FilterConstraints(dim, order, NE, nbr, u, gamma, bound.data());
         if (scheme == 0) {
             // SSP-RK3, Shu-Osher form (as orc_rk_run)
@@ -554,6 +573,7 @@ int orc_rk_run_gen(int dim, int order, int NE, const int *nbr, const double *geo
             return -2;
         }
     }
+    orc_set_time(t);
     return 0;
 }
 

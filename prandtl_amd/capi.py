@@ -33,7 +33,8 @@ EXPORTED = [
     "prandtl_filter_constraints", "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
     "prandtl_device_alloc", "prandtl_device_free", "prandtl_memcpy", "prandtl_stream_synchronize",
     "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc", "prandtl_op_timing",
-    "prandtl_op_timing_read", "prandtl_rk_run_tape_schedule", "prandtl_rk_tape_info",
+    "prandtl_op_timing_read", "prandtl_rk_run_tape_schedule", "prandtl_rk_tape_info", "prandtl_op_set_time",
+    "prandtl_op_get_time",
 ]
 ALPHA_MAX, ALPHA_MIN = 0.5, 0.001   # the reference's blending defaults
 
@@ -44,23 +45,44 @@ BC_RIEMANN_INVARIANT, BC_SUBSONIC_INFLOW_RV, BC_SUBSONIC_INFLOW_PTTT, BC_SUBSONI
 BC_NOSLIP_ADIABATIC, BC_NOSLIP_ISOTHERMAL = 10, 11
 
 
+# boundary-state profiles (prandtl_bc.profile): the function-valued states of the reference's test cases
+PROFILE_CONSTANT, PROFILE_FRONT, PROFILE_RAMP = 0, 1, 2
+
+
 class Bc(ctypes.Structure):
-    _fields_ = [("type", ctypes.c_int), ("q", ctypes.c_double * 8)]
+    _fields_ = [("type", ctypes.c_int), ("q", ctypes.c_double * 8), ("profile", ctypes.c_int),
+                ("a", ctypes.c_double * 16)]
 
 
-def bc(type_: int, *q) -> Bc:
+def bc(type_: int, *q, profile: int = PROFILE_CONSTANT, a=()) -> Bc:
+    """A boundary condition: type, its constant values q, and (inflow / specified state) a profile
+    with its parameters a, as include/prandtl_amd.h lists them."""
     b = Bc()
     b.type = int(type_)
     for i, v in enumerate(q):
         b.q[i] = float(v)
+    b.profile = int(profile)
+    for i, v in enumerate(a):
+        b.a[i] = float(v)
     return b
+
+
+def bc_front(type_: int, behind, ahead, i0: int, i1: int, c0: float, c1: float, c2: float) -> Bc:
+    """A state with a planar front: `behind` where x[i0] < c0 + (x[i1] + c1 t) / c2, else `ahead`."""
+    return bc(type_, *behind, profile=PROFILE_FRONT, a=[i0, i1, c0, c1, c2, *ahead])
+
+
+def bc_ramp(type_: int, t_ramp: float, rho: float, vel, p: float) -> Bc:
+    """An inflow whose velocity rises as clamp((1 + tanh((t - t_r/2)/(t_r/10)))/2, 0, 1) times vel."""
+    return bc(type_, profile=PROFILE_RAMP, a=[t_ramp, rho, *vel, p])
 
 
 class MeshDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("num_elements", ctypes.c_int),
                 ("neighbors", ctypes.POINTER(ctypes.c_int)), ("geometry", ctypes.POINTER(ctypes.c_double)),
                 ("num_bc", ctypes.c_int), ("bc", ctypes.POINTER(Bc)),
-                ("node_geometry", ctypes.POINTER(ctypes.c_double)), ("face_orientation", ctypes.POINTER(ctypes.c_int))]
+                ("node_geometry", ctypes.POINTER(ctypes.c_double)), ("face_orientation", ctypes.POINTER(ctypes.c_int)),
+                ("node_coords", ctypes.POINTER(ctypes.c_double))]
 
 
 class Physics(ctypes.Structure):
@@ -119,6 +141,8 @@ def load():
         L.prandtl_limiter_apply.argtypes = [vp, vp, vp]
         L.prandtl_hook_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
         L.prandtl_blend_coefficients.argtypes = [vp, vp, vp, vp, vp]
+        L.prandtl_op_set_time.argtypes = [vp, ctypes.c_double]
+        L.prandtl_op_get_time.argtypes = [vp, dp]
         L.prandtl_last_error.argtypes = []
         L.prandtl_last_error.restype = ctypes.c_char_p
         for n in ("prandtl_op_mult", "prandtl_op_mult_grad", "prandtl_op_get_max_char_speed",
@@ -127,7 +151,7 @@ def load():
                   "prandtl_filter_apply", "prandtl_limiter_apply", "prandtl_hook_counts",
                   "prandtl_blend_coefficients", "prandtl_rk_tape_id", "prandtl_op_vjp_noalloc",
                   "prandtl_op_timing", "prandtl_op_timing_read", "prandtl_rk_run_tape_schedule",
-                  "prandtl_rk_tape_info"):
+                  "prandtl_rk_tape_info", "prandtl_op_set_time", "prandtl_op_get_time"):
             getattr(L, n).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -186,12 +210,17 @@ class Operator:
         fo = getattr(mesh, "face_orient", None)
         self._ngeom = None if ng is None else np.ascontiguousarray(ng, dtype=np.float64)
         self._forient = None if fo is None else np.ascontiguousarray(fo, dtype=np.int32)
+        # node coordinates [e][node][dim], for boundary states that vary in space
+        self._xyz = None
+        if bcs and any(b.profile == PROFILE_FRONT for b in bcs):
+            self._xyz = np.ascontiguousarray(np.transpose(mesh.node_coords(), (1, 2, 0)), dtype=np.float64)
         md = MeshDesc(mesh.dim, mesh.order, mesh.num_elements,
                       self._nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                       self._geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                       len(bcs) if bcs else 0, self._bcs,
                       None if self._ngeom is None else self._ngeom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                      None if self._forient is None else self._forient.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+                      None if self._forient is None else self._forient.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                      None if self._xyz is None else self._xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         ph = Physics(gamma, Pr, mu, R_gas, int(viscous), int(sutherland))
         op = Opts(int(surface_flux), int(precision), int(bug_compat), int(stage_hooks), int(blending),
                   float(alpha_max), float(alpha_min))
@@ -222,6 +251,15 @@ class Operator:
     def mult_grad(self, u, dudt, grad, stream=None):
         pu, pd, pg = _ptrs((u, self.size, "u"), (dudt, self.size, "dudt"), (grad, self.mesh.dim * self.size, "grad"))
         _check(load().prandtl_op_mult_grad(self._h, pu, pd, pg, _stream(stream)))
+
+    def set_time(self, t: float):
+        """The time of Mult's boundary states (TimeDependentOperator::SetTime)."""
+        _check(load().prandtl_op_set_time(self._h, float(t)))
+
+    def get_time(self) -> float:
+        v = ctypes.c_double(0.0)
+        _check(load().prandtl_op_get_time(self._h, ctypes.byref(v)))
+        return v.value
 
     def get_max_char_speed(self) -> float:
         v = ctypes.c_double(0.0)

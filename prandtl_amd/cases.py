@@ -170,6 +170,37 @@ def khi_state(mesh: Mesh, width=0.1, noise=1e-2, seed=1234, gamma=GAMMA):
     return prim_to_conserv(rho, [u, v], p, gamma)
 
 
+# ---------------------------------------------------------------------------
+# 2D double Mach reflection on [0,4] x [0,1] (the reference's TestCases/Euler/2D/
+# DoubleMachReflection: its mesh is the 384 x 96 lattice of this domain, refined once)
+# ---------------------------------------------------------------------------
+DMR_BOUNDARIES = ("left", "bottom1", "bottom2", "right", "top")   # the configuration's order
+
+
+def dmr_mesh(nx=384, ny=96, order=3) -> Mesh:
+    """nx x ny square-element lattice of [0,4] x [0,1] (nx = 4 ny, a multiple of 24, so that x = 1/6
+    falls on an element face).  Boundary faces carry the conditions in DMR_BOUNDARIES order: left
+    (x = 0), bottom1 (y = 0, x < 1/6), bottom2 (y = 0, x > 1/6), right (x = 4), top (y = 1)."""
+    assert nx % 24 == 0
+    m = periodic_lattice(2, order, (nx, ny), lengths=(4.0, 1.0), periodic=(False, False),
+                         boundary={(0, 0): 0, (0, 1): 3, (1, 0): 2, (1, 1): 4})
+    e = np.arange(m.num_elements)
+    first = (e // nx == 0) & (e % nx < nx // 24)   # bottom faces left of x = 1/6
+    m.neighbors[first, 2] = -(1 + 1)
+    return m
+
+
+def dmr_state(mesh: Mesh, gamma=GAMMA):
+    """DoubleMachReflectionIC: the Mach 10 post-shock state (rho 8, p 116.5, velocity 8.25 along
+    (cos 30, -sin 30)) where x < 1/6 + y/sqrt 3, the resting gas (rho 1.4, p 1) elsewhere."""
+    from .config import _dmr_states
+
+    post, pre = _dmr_states(1.0 / (gamma - 1.0))
+    X = mesh.node_coords()
+    behind = X[0] < 1.0 / 6.0 + X[1] / math.sqrt(3)
+    return np.stack([np.where(behind, post[v], pre[v]) for v in range(4)])
+
+
 def cfl_dt(mesh: Mesh, lam_max: float, cfl: float) -> float:
     """dt = cfl * h_min / (lambda_max (p+1)^2) -- ref: src/Simulation/Simulation.cpp:906."""
     return cfl * mesh.h_min() / (lam_max * (mesh.order + 1) ** 2)

@@ -11,11 +11,14 @@ visualization settings (:110-135; ``Case.paraview_collection`` and ``Case.save_s
 (index k <-> boundary name), the stepping settings.  Boundary values named by the configuration
 (the reference's ConditionFactory keys, C++ constants of each case header) come from
 ``REFERENCE_BC_CONSTANTS`` for the reference's constant-valued keys, or from ``bc_values``
-(key -> values); function-valued conditions (``"function"``) are not supported.
+(key -> values).  Function-valued state conditions (``"function"``) map to the library's state
+profiles (``REFERENCE_BC_FUNCTIONS``: the double-Mach-reflection shock front, the Nagashima inlet
+ramp); other function-valued conditions are refused.
 """
 from __future__ import annotations
 
 import json
+import math
 import os
 from dataclasses import dataclass, field
 from typing import Optional
@@ -44,6 +47,46 @@ REFERENCE_BC_CONSTANTS = {
     "LidDrivenCavityAdiaBCScalar": [0.0],
     "LidDrivenCavityWallVelBCVector": [0.0, 0.0],
     "LidDrivenCavityLidVelBCVector": [1.0, 0.0],
+}
+
+# boundary-state profiles of the library (prandtl_bc.profile)
+PROFILE_CONSTANT, PROFILE_FRONT, PROFILE_RAMP = 0, 1, 2
+
+
+def _dmr_states(gm1i):
+    """The double Mach reflection's post- and pre-shock states for 1/(gamma-1) = gm1i
+    (TestCases/Euler/2D/DoubleMachReflection/DoubleMachReflection.hpp:17-27, 40-50)."""
+    u, v = 7.144709581221619, 4.125
+    post = [8.0, 8.0 * u, -8.0 * v, 116.5 * gm1i + 0.5 * 8.0 * (u * u + v * v)]
+    pre = [1.4, 0.0, 0.0, 1.0 * gm1i]
+    return post, pre
+
+
+def _dmr_top(par):
+    """DoubleMachReflectionTopBCFunction(gammaM1Inverse = x1) (DoubleMachReflection.hpp:32-52): the
+    post-shock state where x < 1/6 + (y + 20 t)/sqrt 3, else the pre-shock state."""
+    post, pre = _dmr_states(float(par["x1"]))
+    return post, PROFILE_FRONT, [0, 1, 1.0 / 6.0, 20.0, math.sqrt(3), *pre]
+
+
+def _nagashima_ramp(par):
+    """NagashimaRampingInletBC(t_ramp = x1) (TestCases/Euler/2D/Nagashima_Ramjet/Nagashima_Ramjet.hpp:
+    8-19, 40-53): the free stream of Mach 2 from p0 = 303975 Pa, a0 = 360.63 m/s (gamma 1.4,
+    R 287.05), its velocity ramped over t_ramp."""
+    g, R, Ma, p0, a0 = 1.4, 287.05, 2.0, 303975.0, 360.63
+    C = 1 + 0.5 * (g - 1) * Ma * Ma
+    a = a0 / math.sqrt(C)
+    pa = p0 / math.pow(C, g / (g - 1))
+    Ta = (a * a) / (g * R)
+    rhoa = pa / (R * Ta)
+    return [], PROFILE_RAMP, [float(par["x1"]), rhoa, Ma * a, 0.0, pa]
+
+
+# The function-valued state conditions of the reference's cases (ConditionFactory keys registered by
+# the case headers) as the library's state profiles: key -> (parameters -> (q, profile, a))
+REFERENCE_BC_FUNCTIONS = {
+    "DoubleMachReflectionTopBCFunction": _dmr_top,
+    "NagashimaRampingInletBC": _nagashima_ramp,
 }
 
 # boundary-condition types by the configuration's "type" (Simulation.cpp:520-800)
@@ -79,12 +122,14 @@ class Case:
     checkpoint: dict = field(default_factory=dict)
     visualization: dict = field(default_factory=dict)
     ignored: dict = field(default_factory=dict)
+    bc_profiles: list = field(default_factory=list)   # (profile, a) per condition
 
     def bcs(self):
         """The condition table for prandtl_amd.capi.Operator (capi.Bc structs)."""
         from . import capi
 
-        return [capi.bc(t, *q) for t, q in self.bc_specs]
+        prof = self.bc_profiles or [(PROFILE_CONSTANT, [])] * len(self.bc_specs)
+        return [capi.bc(t, *q, profile=pr, a=a) for (t, q), (pr, a) in zip(self.bc_specs, prof)]
 
     def operator(self, **overrides):
         """prandtl_amd.capi.Operator of the case (FP64)."""
@@ -143,14 +188,24 @@ def _bc_spec(name, props, dim, bc_values):
         raise ConfigError(f"boundary {name!r}: unknown condition type {typ!r}")
     kind = BC_TYPES[typ]
     if typ in ("slip", "symmetry", "supersonic-outflow"):
-        return kind, []
+        return kind, [], PROFILE_CONSTANT, []
     if typ in ("supersonic-inflow", "specified-state"):
+        if "function" in props:
+            # Simulation.cpp:664-790: a VectorFunctionCoefficient (time_dependent: a
+            # VectorTDFunctionCoefficient) from the ConditionFactory key and its parameters
+            key = props["function"]
+            if key not in REFERENCE_BC_FUNCTIONS:
+                raise ConfigError(f"boundary {name!r}: the state function {key!r} has no profile in this library")
+            if dim != 2:
+                raise ConfigError(f"boundary {name!r}: {key!r} is a 2D condition")
+            q, profile, a = REFERENCE_BC_FUNCTIONS[key](props.get("params", {}))
+            return kind, q, profile, a
         if "vector" not in props:
-            raise ConfigError(f"boundary {name!r}: function-valued states are not supported")
+            raise ConfigError(f"boundary {name!r}: the condition names neither a vector nor a function")
         q = _bc_values(props["vector"], bc_values)
         if len(q) != dim + 2:
             raise ConfigError(f"boundary {name!r}: {len(q)} state values for {dim + 2} equations")
-        return kind, q
+        return kind, q, PROFILE_CONSTANT, []
     # no-slip walls: velocity vector, then the heat-flux (adiabatic) or temperature (isothermal) scalar
     vel = props.get("velocity", {})
     sec = props.get("heat" if typ == "no-slip-adiabatic" else "temperature", {})
@@ -160,13 +215,16 @@ def _bc_spec(name, props, dim, bc_values):
     s = _bc_values(sec["scalar"], bc_values)
     if len(v) != dim or len(s) != 1:
         raise ConfigError(f"boundary {name!r}: wall values of the wrong size")
-    return kind, v + s
+    return kind, v + s, PROFILE_CONSTANT, []
 
 
-def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: Optional[int] = None) -> Case:
+def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: Optional[int] = None,
+              mesh: Optional[M.Mesh] = None) -> Case:
     """Read a reference configuration file.  ``mesh_dir``: where to look for the mesh file (by its
     base name; default the configuration's directory); ``bc_values``: values of boundary keys
-    that REFERENCE_BC_CONSTANTS does not hold; ``order`` overrides the configuration's."""
+    that REFERENCE_BC_CONSTANTS does not hold; ``order`` overrides the configuration's; ``mesh``: a
+    mesh to use instead of the configuration's mesh file (its boundary faces numbered in the order
+    of the configuration's boundary_conditions)."""
     with open(path) as fh:
         cfg = json.load(fh)
     ct = cfg.get("compileTime", {})
@@ -194,16 +252,22 @@ def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: 
                    stage_hooks=HOOK_FILTER if ct.get("USE_FILTER", False) else 0)
     # boundary conditions in the configuration's order; k indexes the condition table
     bcs_cfg = rt.get("conditions", {}).get("boundary_conditions", {}) or {}
-    bc_names, bc_specs = [], []
+    bc_names, bc_specs, bc_profiles = [], [], []
     for name, props in bcs_cfg.items():
         bc_names.append(name)
-        bc_specs.append(_bc_spec(name, props, dim, bc_values))
+        kind, q, profile, a = _bc_spec(name, props, dim, bc_values)
+        bc_specs.append((kind, q))
+        bc_profiles.append((profile, a))
     bc_index = {n: k for k, n in enumerate(bc_names)}
-    mpath = _resolve_mesh_file(rt, os.path.dirname(os.path.abspath(path)), mesh_dir)
     # serial refinement, then the parallel mesh's uniform refinement (Simulation.cpp:329-336): both
     # split every element into 2^dim, so they fold into one count
     refine = int(rt.get("ser_ref_levels", 0) or 0) + int(rt.get("par_ref_levels", 0) or 0)
-    if mpath.endswith(".msh"):
+    mpath = None if mesh is not None else _resolve_mesh_file(rt, os.path.dirname(os.path.abspath(path)), mesh_dir)
+    if mesh is not None:
+        if mesh.dim != dim or mesh.order != p:
+            raise ConfigError(f"the given mesh is {mesh.dim}D of order {mesh.order}; the case is {dim}D of order {p}")
+        ignored["mesh_file"] = "replaced by the given mesh"
+    elif mpath.endswith(".msh"):
         mesh = M.gmsh_mesh(mpath, p, bc_index, refine=refine)
     else:
         # the reference's 2D MFEM meshes are periodic (no boundary); boundary names of MFEM meshes are
@@ -223,7 +287,7 @@ def load_case(path: str, mesh_dir: Optional[str] = None, bc_values=None, order: 
     if vis.get("visualize") and vis.get("visit") and not vis.get("paraview"):
         ignored["visit"] = "VisIt output is not written (ParaView output is)"
     return Case(path=path, dim=dim, order=p, mesh=mesh, physics=physics, options=options, bc_names=bc_names,
-                bc_specs=bc_specs, ode_solver=solver, scheme=ODE_SOLVERS.get(solver),
+                bc_specs=bc_specs, bc_profiles=bc_profiles, ode_solver=solver, scheme=ODE_SOLVERS.get(solver),
                 cfl=float(rt.get("cfl", 1.0)), dt=float(rt["dt"]) if "dt" in rt else None,
                 variable_dt=bool(rt.get("variable_dt", False)), final_time=float(rt.get("final_time", 0.0)),
                 initial_condition=dict(ic), checkpoint=ckpt, visualization=vis, ignored=ignored)
@@ -235,7 +299,10 @@ def reference_initial_state(case: Case):
     LidDrivenCavityIC(Ma = x1, gamma = x2): rho = 1 at rest, p = 1/(Ma^2 gamma)
     (TestCases/NavierStokes/2D/LidDrivenCavity/LidDrivenCavity.hpp:9-19);
     ForwardFacingStepIC(1/(gamma-1) = x1): rho = 1.4, u = 3, v = 0, p = 1
-    (TestCases/Euler/2D/ForwardFacingStep/ForwardFacingStep.hpp:9-20).  Other cases' initial
+    (TestCases/Euler/2D/ForwardFacingStep/ForwardFacingStep.hpp:9-20);
+    DoubleMachReflectionIC(1/(gamma-1) = x1): the post-shock state where x < 1/6 + y/sqrt 3, the
+    pre-shock state elsewhere, at the mesh's node coordinates
+    (TestCases/Euler/2D/DoubleMachReflection/DoubleMachReflection.hpp:9-29).  Other cases' initial
     conditions are functions of x the caller evaluates."""
     import numpy as np
 
@@ -253,5 +320,12 @@ def reference_initial_state(case: Case):
         u[0] = 1.4
         u[1] = 1.4 * 3.0
         u[case.dim + 1] = 1.0 * gm1i + 0.5 * 1.4 * 3.0 * 3.0
+        return u
+    if name == "DoubleMachReflectionIC":
+        post, pre = _dmr_states(float(par.get("x1", 0.0)))
+        X = case.mesh.node_coords()
+        behind = X[0] < 1.0 / 6.0 + X[1] / math.sqrt(3)
+        for v in range(case.dim + 2):
+            u[v] = np.where(behind, post[v], pre[v])
         return u
     raise ConfigError(f"initial condition {name!r} is a function of x: evaluate it at mesh.node_coords()")

@@ -3,7 +3,8 @@
 // The reference's BdrFaceIntegrator hierarchy (src/Integrators/BoundaryConditions/), one
 // condition per boundary attribute (Simulation.cpp:505-800), with constant boundary values.
 // A boundary face of element e is the neighbour entry -(k+1), k the index into the operator's
-// table of conditions.  Each condition supplies, at a face point with the element's trace s1
+// table of conditions.  The state of an inflow or specified-state condition may vary over the
+// boundary and in time (bc_state).  Each condition supplies, at a face point with the element's trace s1
 // and the element's outward normal n (the face area vector, CalcOrtho):
 //   the inviscid flux f(s1, n) and a wave speed   ComputeBdrFaceInviscidFlux
 //                                                   (BdrFaceIntegrator.cpp:245-253: the
@@ -23,7 +24,11 @@ namespace prandtl {
 struct BcDev {          // the layout of prandtl_bc
     int type;
     double q[8];
+    int profile;
+    double a[16];
 };
+
+enum BcProfile { BC_PROFILE_CONSTANT = 0, BC_PROFILE_FRONT = 1, BC_PROFILE_RAMP = 2 };
 
 enum BcKind {
     BC_SLIP_WALL = 1, BC_SYMMETRY = 2, BC_SUPERSONIC_INFLOW = 3, BC_SUPERSONIC_OUTFLOW = 4,
@@ -52,15 +57,52 @@ __device__ __forceinline__ void bc_unit(const double *n, double *un) {
     for (int d = 0; d < DIM; d++) un[d] = n[d] / a;
 }
 
-// ComputeOuterInviscidState (the conditions that define an outer state)
+// The state of a SUPERSONIC_INFLOW / SPECIFIED_STATE condition at the face point x and time t: the
+// constant q, or the test cases' function-valued states (the VectorFunctionCoefficient /
+// VectorTDFunctionCoefficient the reference evaluates per face point, Simulation.cpp:664-790):
+//   FRONT (DoubleMachReflectionTopBCFunction, DoubleMachReflection.hpp:32-52): q behind the
+//     planar front x[i0] < c0 + (x[i1] + c1 t) / c2, the state a[5..] ahead of it;
+//   RAMP (NagashimaRampingInletBC, Nagashima_Ramjet.hpp:40-53): the velocity a[2..1+dim] scaled by
+//     r = clamp((1 + tanh((t - t_r/2) / (t_r/10))) / 2, 0, 1), density a[1], pressure a[2+dim].
+// The operations are the reference functions' own, in their order.
 template <int DIM>
-__device__ void bc_outer_state(const BcDev &bc, const PhysConst &P, const double *s1, const double *n, double *s2) {
+__device__ __forceinline__ void bc_state(const BcDev &bc, const PhysConst &P, const double *x, double t, double *s2) {
+    constexpr int NEQ = DIM + 2;
+    if (bc.profile == BC_PROFILE_FRONT) {
+        const int i0 = (int)bc.a[0], i1 = (int)bc.a[1];
+        const bool behind = x[i0] < bc.a[2] + (x[i1] + bc.a[3] * t) / bc.a[4];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s2[v] = behind ? bc.q[v] : bc.a[5 + v];
+    } else if (bc.profile == BC_PROFILE_RAMP) {
+        const double tr = bc.a[0];
+        const double sh = tanh((t - 0.5 * tr) / (0.1 * tr));
+        const double r = fmin(fmax(0.5 * (1.0 + sh), 0.0), 1.0);
+        const double rho = bc.a[1];
+        double e = bc.a[2 + DIM] / P.gammaM1;
+        s2[0] = rho;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const double v = bc.a[2 + d] * r;
+            s2[1 + d] = rho * v;
+            e += 0.5 * rho * v * v;
+        }
+        s2[NEQ - 1] = e;
+    } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s2[v] = bc.q[v];
+    }
+}
+
+// ComputeOuterInviscidState (the conditions that define an outer state); x, t: the face point's
+// coordinates and the time (read by the function-valued states only)
+template <int DIM>
+__device__ void bc_outer_state(const BcDev &bc, const PhysConst &P, const double *s1, const double *n, double *s2,
+                               const double *x, double t) {
     constexpr int NEQ = DIM + 2;
     switch (bc.type) {
         case BC_SUPERSONIC_INFLOW:
         case BC_SPECIFIED_STATE:
-#pragma unroll
-            for (int v = 0; v < NEQ; v++) s2[v] = bc.q[v];
+            bc_state<DIM>(bc, P, x, t, s2);
             return;
         case BC_RIEMANN_INVARIANT: {   // RiemannInvariantBdrFaceIntegrator.cpp:44-117
             double un[DIM], Vi[DIM];
@@ -143,10 +185,10 @@ __device__ void bc_outer_state(const BcDev &bc, const PhysConst &P, const double
     }
 }
 
-// ComputeBdrFaceInviscidFlux; returns the wave speed.  q1 is make_side(s1).
+// ComputeBdrFaceInviscidFlux; returns the wave speed.  q1 is make_side(s1); x, t as bc_outer_state.
 template <int DIM, int SURF>
 __device__ double bc_inviscid_flux(const BcDev &bc, const PhysConst &P, bool bug3, const double *s1,
-                                   const SideState<DIM> &q1, const double *n, double *f) {
+                                   const SideState<DIM> &q1, const double *n, double *f, const double *x, double t) {
     constexpr int NEQ = DIM + 2;
     double un[DIM];
     bc_unit<DIM>(n, un);
@@ -215,7 +257,7 @@ __device__ double bc_inviscid_flux(const BcDev &bc, const PhysConst &P, bool bug
 #pragma unroll
         for (int d = 0; d < DIM; d++) s2[1 + d] += -2.0 * mn * un[d];
     } else {
-        bc_outer_state<DIM>(bc, P, s1, n, s2);
+        bc_outer_state<DIM>(bc, P, s1, n, s2, x, t);
     }
     const SideState<DIM> q2 = make_side<DIM>(s2, P);
     double nmag = 0.0;
@@ -245,10 +287,11 @@ __device__ __forceinline__ void bc_entropy(const double *s, const PhysConst &P, 
     w[DIM + 1] = -beta;
 }
 
-// ComputeBdrFaceLiftingFlux: w1 the element's entropy variables, s1 its conserved trace.
+// ComputeBdrFaceLiftingFlux: w1 the element's entropy variables, s1 its conserved trace; x, t as
+// bc_outer_state.
 template <int DIM>
 __device__ void bc_lifting_flux(const BcDev &bc, const PhysConst &P, const double *w1, const double *s1,
-                                const double *n, double *fl) {
+                                const double *n, double *fl, const double *x, double t) {
     constexpr int NEQ = DIM + 2;
     if (bc.type == BC_NOSLIP_ADIABATIC || bc.type == BC_NOSLIP_ISOTHERMAL) {
         // NoSlipAdiabWallBdrFaceIntegrator.cpp:74-95, NoSlipIsothWallBdrFaceIntegrator.cpp:41-66:
@@ -261,7 +304,7 @@ __device__ void bc_lifting_flux(const BcDev &bc, const PhysConst &P, const doubl
         return;
     }
     double s2[NEQ], w2[NEQ];
-    bc_outer_state<DIM>(bc, P, s1, n, s2);
+    bc_outer_state<DIM>(bc, P, s1, n, s2, x, t);
     bc_entropy<DIM>(s2, P, w2);
 #pragma unroll
     for (int k = 0; k < NEQ; k++) fl[k] = (w2[k] - w1[k]) * 0.5;
@@ -297,7 +340,7 @@ __device__ void bc_viscous_trace(const BcDev &bc, const PhysConst &P, const doub
             double n[DIM], s2[NEQ], flux[DIM + 1][DIM];
 #pragma unroll
             for (int c = 0; c < DIM; c++) n[c] = (side ? 1.0 : -1.0) * adj_d[c];
-            bc_outer_state<DIM>(bc, P, s, n, s2);
+            bc_outer_state<DIM>(bc, P, s, n, s2, nullptr, 0.0);   // a constant state
             viscous_flux<DIM, double>(s2, g, P, flux);
 #pragma unroll
             for (int v = 0; v < NEQ - 1; v++) {

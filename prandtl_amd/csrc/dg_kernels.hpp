@@ -118,6 +118,9 @@ struct RhsArgs {
     double *lam_part;      // want_lambda: each wave's max |V| + c, at its global wave index
     int *nan_flag;
     const BcDev *bc;       // boundary conditions (neighbour -(k+1) -> bc[k]; k_rhs_node<..., BND = true>)
+    const double *bxyz;    // coordinates of the face points [e][2 dim][Np^(dim-1)][dim] (function-valued
+                           // boundary states; null otherwise)
+    double time;           // the time of the boundary states
     double *alpha;         // k_rhs_node<..., BLEND = true>: the coefficients it computed (may be null)
     BlendParams<NP> blend; // the same: Persson-Peraire indicator and blending constants
     const double *ngeom;   // general mesh (k_rhs_node<..., GEN = true>): per-node (det J, adj J) [e][node][1+dim^2]
@@ -150,6 +153,8 @@ struct LiftArgs {
     float *Fff;
     double *grad;          // optional out: primitive gradients [c][v][e][node]
     const BcDev *bc;       // boundary conditions (neighbour -(k+1) -> bc[k])
+    const double *bxyz;    // face-point coordinates for function-valued boundary states (as RhsArgs)
+    double time;
     const double *ngeom;   // general mesh (k_lift<..., GEN = true>): per-node (det J, adj J)
     const int *forient;    // general mesh: face orientation codes
 };
@@ -683,7 +688,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 const R *m = GEN ? A.ngeom + ((size_t)ee * NN + nown) * G1 + 1 + dir * DIM : sM[ib][dir];
 #pragma unroll
                 for (int c = 0; c < DIM; c++) nor[c] = side ? m[c] : -m[c];
-                lam = fmax(lam, bc_inviscid_flux<DIM, SURF>(A.bc[-en - 1], A.P, A.bug3 != 0, so, qo, nor, fb));
+                const double *xb = A.bxyz ? A.bxyz + (((size_t)ee * 2 * DIM + 2 * dir + side) * L + t) * DIM : nullptr;
+                lam = fmax(lam, bc_inviscid_flux<DIM, SURF>(A.bc[-en - 1], A.P, A.bug3 != 0, so, qo, nor, fb, xb,
+                                                            A.time));
                 const double sg = side ? 1.0 : -1.0;
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) {
@@ -1283,7 +1290,10 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
                                    : A.geom + (size_t)(e0 + fe) * (1 + DIM * DIM) + 1 + dir * DIM;
 #pragma unroll
             for (int c = 0; c < DIM; c++) nor[c] = side ? gm[c] : -gm[c];
-            bc_lifting_flux<DIM>(A.bc[-en - 1], A.P, wo, so, nor, fl);
+            constexpr int LF = ipow(NP, DIM - 1);
+            const double *xb = A.bxyz ? A.bxyz + (((size_t)(e0 + fe) * 2 * DIM + 2 * dir + side) * LF + t) * DIM
+                                      : nullptr;
+            bc_lifting_flux<DIM>(A.bc[-en - 1], A.P, wo, so, nor, fl, xb, A.time);
 #pragma unroll
             for (int v = 0; v < NEQ; v++) wn[v] = wo[v] + 2.0 * fl[v];
         } else {

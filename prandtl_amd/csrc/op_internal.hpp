@@ -56,6 +56,9 @@ struct prandtl_op_s {
     double w[MAX_NP] = {};        // GLL weights on [0,1]
     BcDev *d_bc = nullptr;        // boundary conditions (neighbour -(k+1) -> d_bc[k])
     int nbc = 0;
+    double *d_bxyz = nullptr;     // face-point coordinates [e][2 dim][Np^(dim-1)][dim], only when a
+                                  // boundary state varies in space
+    double time = 0.0;            // prandtl_op_set_time: the time of Mult's boundary states
     double *d_alpha = nullptr;    // blending: per-element coefficient of the current RHS
     int *d_nbr = nullptr;
     double *d_geom = nullptr;
@@ -171,6 +174,7 @@ struct StageSpec {
     bool nan_check;
     int phases = 3;      // bit 0: k_lift (NS), bit 1: k_rhs
     int hook_mode = 0;   // HOOK_DETECT_* bits: stage hooks detected in the epilogue
+    double time = 0.0;   // the stage's time (function-valued boundary states)
 };
 
 template <int DIM, int NP, bool VISC, int SURF, bool MIX>
@@ -191,6 +195,8 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
         la.Fff = reinterpret_cast<float *>(op->d_Ff);
         la.grad = S.grad;
         la.bc = op->d_bc;
+        la.bxyz = op->d_bxyz;
+        la.time = S.time;
         la.ngeom = op->d_ngeom;
         la.forient = op->d_forient;
         const int nblk = (op->NE + NC::EPB - 1) / NC::EPB;
@@ -237,6 +243,8 @@ int launch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
     ra.nan_flag = S.nan_check ? op->d_nan : nullptr;
     ra.alpha = op->d_alpha;
     ra.bc = op->d_bc;
+    ra.bxyz = op->d_bxyz;
+    ra.time = S.time;
     ra.ngeom = op->d_ngeom;
     ra.forient = op->d_forient;
     ra.hook_mode = S.hook_mode;

@@ -7,6 +7,7 @@
 #include "dg_util.hpp"
 
 #include <atomic>
+#include <cmath>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -623,10 +624,15 @@ const double CK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0
 const double CK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
                         1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
                         2277821191437.0 / 14882151754819.0};
+// the stage times (as fractions of dt) of the same scheme
+const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                        2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
 
 // Advance u by nsteps; the state ping-pongs between u and the handle's alternate buffer
-// (a stage reads its neighbours' face nodes, so it cannot update in place).
-int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, int nan_check, hipStream_t st) {
+// (a stage reads its neighbours' face nodes, so it cannot update in place).  *t: the time at the
+// start, advanced by dt after every step (MFEM's ODESolver::Step); each stage runs at its stage time.
+int rk_advance(prandtl_op_s *op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
+               hipStream_t st) {
     int rc = ensure_rk_buffers(op, scheme);
     if (rc) return rc;
     if (nan_check > 0) {
@@ -682,8 +688,11 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
             }
             double *x = cur, *yin = cur;
             const unsigned gblk = (unsigned)std::min<size_t>((op->size + 255) / 256, 8192);
+            // MFEM's stage times: RK2 t, t + a dt; RK4 t + c dt, c = 0, 1/2, 1/2, 1
+            const double ct[4] = {0.0, scheme == PRANDTL_RK2 ? rk2a : 0.5, 0.5, 1.0};
             for (int stg = 0; stg < ns; stg++) {
                 StageSpec S{EPI_MULT, yin, op->d_k, nullptr, nullptr, 0.0, 0.0, dt, nullptr, false, false};
+                S.time = *t + ct[stg] * dt;
                 rc = run_stage(op, S, st);
                 if (rc) return rc;
                 double *ynext = (x == u) ? op->d_ub : u;
@@ -705,6 +714,7 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
                 StageSpec S{EPI_LSRK, cur, nxt, op->d_k, nullptr, CK_A[stg], CK_B[stg], dt, nullptr, false,
                             chk && stg == 4};
                 S.hook_mode = hmode | (hf && stg == 4 ? HOOK_DETECT_EMIN : 0);
+                S.time = *t + CK_C[stg] * dt;
                 rc = run_stage(op, S, st);
                 if (!rc && hooks) rc = fused ? fused_hooks(nxt, stg == 4) : stage_hooks(op, nxt, st);
                 if (rc) return rc;
@@ -714,22 +724,27 @@ int rk_advance(prandtl_op_s *op, int scheme, double *u, double dt, int nsteps, i
             // MFEM RK3SSPSolver (Shu-Osher): y = x + dt L(x); y2 = 3/4 x + 1/4 (y + dt L(y));
             // x = 1/3 x + 2/3 (y2 + dt L(y2)).  x stays in `cur`.
             double *y = (cur == u) ? op->d_ub : u;
+            // stage times t, t + dt, t + dt/2 (RK3SSPSolver::Step)
             StageSpec S1{EPI_SSP, cur, y, nullptr, cur, 0.0, 1.0, dt, nullptr, false, false};
             S1.hook_mode = hmode;
+            S1.time = *t;
             rc = run_stage(op, S1, st);
             if (!rc && hooks) rc = fused ? fused_hooks(y, false) : stage_hooks(op, y, st);
             if (rc) return rc;
             StageSpec S2{EPI_SSP, y, op->d_y2, nullptr, cur, 3.0 / 4.0, 1.0 / 4.0, dt, nullptr, false, false};
             S2.hook_mode = hmode;
+            S2.time = *t + dt;
             rc = run_stage(op, S2, st);
             if (!rc && hooks) rc = fused ? fused_hooks(op->d_y2, false) : stage_hooks(op, op->d_y2, st);
             if (rc) return rc;
             StageSpec S3{EPI_SSP, op->d_y2, cur, nullptr, cur, 1.0 / 3.0, 2.0 / 3.0, dt, nullptr, false, chk};
             S3.hook_mode = hmode | (hf ? HOOK_DETECT_EMIN : 0);
+            S3.time = *t + dt / 2;
             rc = run_stage(op, S3, st);
             if (!rc && hooks) rc = fused ? fused_hooks(cur, true) : stage_hooks(op, cur, st);
             if (rc) return rc;
         }
+        *t += dt;
         if (chk) {
             int flag = 0;
             HIP_TRY(hipMemcpyAsync(&flag, op->d_nan, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -889,6 +904,27 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         if (phys->viscous && (ty == PRANDTL_BC_SLIP_WALL || ty == PRANDTL_BC_SYMMETRY))
             return fail(PRANDTL_ERR_UNSUPPORTED, "slip and symmetry walls define no viscous or lifting flux "
                                                  "(the reference's base class aborts for them)");
+        const prandtl_bc &b = mesh->bc[k];
+        if (b.profile < PRANDTL_PROFILE_CONSTANT || b.profile > PRANDTL_PROFILE_RAMP)
+            return fail(PRANDTL_ERR_ARG, "boundary condition " + std::to_string(k) + ": unknown profile " +
+                                             std::to_string(b.profile));
+        if (b.profile != PRANDTL_PROFILE_CONSTANT && ty != PRANDTL_BC_SUPERSONIC_INFLOW &&
+            ty != PRANDTL_BC_SPECIFIED_STATE)
+            return fail(PRANDTL_ERR_ARG, "boundary condition " + std::to_string(k) +
+                                             ": profiles apply to supersonic-inflow and specified-state conditions");
+        if (b.profile == PRANDTL_PROFILE_FRONT) {
+            const double i0 = b.a[0], i1 = b.a[1];
+            if (!(i0 == 0.0 || i0 == 1.0 || (mesh->dim == 3 && i0 == 2.0)) ||
+                !(i1 == 0.0 || i1 == 1.0 || (mesh->dim == 3 && i1 == 2.0)) || !(b.a[4] != 0.0))
+                return fail(PRANDTL_ERR_ARG, "boundary condition " + std::to_string(k) +
+                                                 ": a front needs coordinate indices below dim and c2 != 0");
+            if (!mesh->node_coords)
+                return fail(PRANDTL_ERR_ARG, "boundary condition " + std::to_string(k) +
+                                                 ": a front needs the node coordinates (node_coords)");
+        }
+        if (b.profile == PRANDTL_PROFILE_RAMP && !(b.a[0] > 0.0 && b.a[1] > 0.0))
+            return fail(PRANDTL_ERR_ARG, "boundary condition " + std::to_string(k) +
+                                             ": a ramp needs t_r > 0 and a positive density");
     }
     if (mesh->num_bc > 0 && o.precision != PRANDTL_FP64)
         return fail(PRANDTL_ERR_UNSUPPORTED, "boundary faces run in FP64");
@@ -992,6 +1028,28 @@ int prandtl_op_create(const prandtl_mesh_desc *mesh, const prandtl_physics *phys
         if ((e = hipMalloc(&op->d_bc, (size_t)op->nbc * sizeof(BcDev))) != hipSuccess ||
             (e = hipMemcpy(op->d_bc, mesh->bc, (size_t)op->nbc * sizeof(BcDev), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+        bool spatial = false;
+        for (int k = 0; k < op->nbc; k++) spatial |= mesh->bc[k].profile == PRANDTL_PROFILE_FRONT;
+        if (spatial) {
+            // the coordinates of every element's face points, [e][2 dim][t][dim] (t the point's index on
+            // the face, as the face traces): read by the boundary faces' function-valued states
+            const int dim = op->dim, np = op->np, L = op->nn / np;
+            std::vector<double> hx((size_t)op->NE * 2 * dim * L * dim);
+            for (size_t el = 0; el < (size_t)op->NE; el++)
+                for (int d = 0; d < dim; d++) {
+                    const int st = d == 0 ? 1 : (d == 1 ? np : np * np);
+                    for (int side = 0; side < 2; side++)
+                        for (int t = 0; t < L; t++) {
+                            const int node = (t % st) + (t / st) * st * np + (side ? (np - 1) * st : 0);
+                            const double *src = mesh->node_coords + (el * op->nn + node) * dim;
+                            double *dst = &hx[((el * 2 * dim + 2 * d + side) * L + t) * dim];
+                            for (int c = 0; c < dim; c++) dst[c] = src[c];
+                        }
+                }
+            if ((e = hipMalloc(&op->d_bxyz, hx.size() * sizeof(double))) != hipSuccess ||
+                (e = hipMemcpy(op->d_bxyz, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+                return cleanup(fail(PRANDTL_ERR_HIP, std::string("device allocation: ") + hipGetErrorString(e)));
+        }
     }
     if (phys->viscous) {
         const size_t nface = op->nn / op->np;  // points per face
@@ -1032,6 +1090,7 @@ void prandtl_op_destroy(prandtl_op op) {
     (void)hipFree(op->d_nan);
     (void)hipFree(op->d_alpha);
     (void)hipFree(op->d_bc);
+    (void)hipFree(op->d_bxyz);
     for (hipEvent_t e : op->tev) (void)hipEventDestroy(e);
     for (double *p : op->tape_ckpt) (void)hipFree(p);
     release_adjoint_storage(op);
@@ -1044,6 +1103,7 @@ int prandtl_op_mult(prandtl_op op, const double *u, double *dudt, prandtl_stream
     if (!op || !u || !dudt) return fail(PRANDTL_ERR_ARG, "null argument");
     if (u == dudt) return fail(PRANDTL_ERR_ARG, "u and dudt must not alias");
     StageSpec S{EPI_MULT, u, dudt, nullptr, nullptr, 0.0, 0.0, 0.0, nullptr, true, false};
+    S.time = op->time;
     int rc = run_stage(op, S, (hipStream_t)stream);
     op->lam_valid = (rc == PRANDTL_OK);
     op->lam_stream = (hipStream_t)stream;
@@ -1054,6 +1114,7 @@ int prandtl_op_mult_grad(prandtl_op op, const double *u, double *dudt, double *g
     if (!op || !u || !dudt || !grad) return fail(PRANDTL_ERR_ARG, "null argument");
     if (!op->phys.viscous) return fail(PRANDTL_ERR_ARG, "gradients exist only for Navier-Stokes operators");
     StageSpec S{EPI_MULT, u, dudt, nullptr, nullptr, 0.0, 0.0, 0.0, grad, true, false};
+    S.time = op->time;
     int rc = run_stage(op, S, (hipStream_t)stream);
     op->lam_valid = (rc == PRANDTL_OK);
     op->lam_stream = (hipStream_t)stream;
@@ -1280,6 +1341,7 @@ int prandtl_rk_run_tape_schedule(prandtl_op op, int scheme, double *u, double *t
     if (schedule == PRANDTL_TAPE_REVOLVE && nsteps > kRevolveMaxSteps)
         return fail(PRANDTL_ERR_ARG, "the revolve schedule takes at most " + std::to_string(kRevolveMaxSteps) + " steps");
     hipStream_t st = (hipStream_t)stream;
+    double tt = t ? *t : 0.0;   // the stage times (the adjoint's operators are time-independent)
     // the previous tape is gone from here on: a run that fails leaves no tape
     op->tape_id = 0;
     op->tape_steps = 0;
@@ -1313,12 +1375,12 @@ int prandtl_rk_run_tape_schedule(prandtl_op op, int scheme, double *u, double *t
     }
     int cur = 0;
     for (size_t k = 0; k < marks.size(); k++) {
-        int rc = rk_advance(op, scheme, u, dt, marks[k] - cur, 0, st);
+        int rc = rk_advance(op, scheme, u, &tt, dt, marks[k] - cur, 0, st);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(op->tape_ckpt[k], u, op->size * sizeof(double), hipMemcpyDeviceToDevice, st));
         cur = marks[k];
     }
-    int rc = rk_advance(op, scheme, u, dt, nsteps - cur, 0, st);
+    int rc = rk_advance(op, scheme, u, &tt, dt, nsteps - cur, 0, st);
     if (rc) return rc;
     op->tape_marks = marks;
     op->tape_schedule = schedule;
@@ -1330,7 +1392,7 @@ int prandtl_rk_run_tape_schedule(prandtl_op op, int scheme, double *u, double *t
     op->tape_t0 = t ? *t : 0.0;
     op->tape_id = next_tape_id();
     op->tape_recomputed = -1;
-    if (t) *t += dt * nsteps;
+    if (t) *t = tt;
     return PRANDTL_OK;
 }
 
@@ -1449,7 +1511,8 @@ int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
                         r = reverse_block(xa, 1);
                     } else {
                         HIP_TRY(hipMemcpyAsync(op->adj_work, xa, bytes, hipMemcpyDeviceToDevice, st));
-                        r = rk_advance(op, op->tape_scheme, op->adj_work, dt, j - a, 0, st);
+                        double tj = op->tape_t0 + a * dt;   // (no time-dependent terms on the adjoint's meshes)
+                        r = rk_advance(op, op->tape_scheme, op->adj_work, &tj, dt, j - a, 0, st);
                         recomputed += j - a;
                         if (!r) r = reverse_block(op->adj_work, 1);
                     }
@@ -1466,7 +1529,8 @@ int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
                     if (taken[i] < 0) slot = (int)i;
                 if (slot < 0) return fail(PRANDTL_ERR_HIP, "revolve: no free snapshot buffer");
                 HIP_TRY(hipMemcpyAsync(pool[slot], xa, bytes, hipMemcpyDeviceToDevice, st));
-                const int r = rk_advance(op, op->tape_scheme, pool[slot], dt, m - a, 0, st);
+                double tm = op->tape_t0 + a * dt;
+                const int r = rk_advance(op, op->tape_scheme, pool[slot], &tm, dt, m - a, 0, st);
                 if (r) return r;
                 recomputed += m - a;
                 taken[slot] = m;
@@ -1514,9 +1578,23 @@ int prandtl_rk_run(prandtl_op op, int scheme, double *u, double *t, double dt, i
     if (!op || !u) return fail(PRANDTL_ERR_ARG, "null argument");
     if (scheme < PRANDTL_SSPRK3 || scheme > PRANDTL_RK4) return fail(PRANDTL_ERR_ARG, "unknown ODE solver");
     if (nsteps < 0 || !(dt >= 0.0)) return fail(PRANDTL_ERR_ARG, "bad step count or dt");
-    int rc = rk_advance(op, scheme, u, dt, nsteps, nan_check, (hipStream_t)stream);
-    if (rc == PRANDTL_OK && t) *t += dt * nsteps;
+    double tt = t ? *t : op->time;
+    int rc = rk_advance(op, scheme, u, &tt, dt, nsteps, nan_check, (hipStream_t)stream);
+    if (rc == PRANDTL_OK && t) *t = tt;
     return rc;
+}
+
+int prandtl_op_set_time(prandtl_op op, double t) {
+    if (!op) return fail(PRANDTL_ERR_ARG, "null argument");
+    if (!std::isfinite(t)) return fail(PRANDTL_ERR_ARG, "the time must be finite");
+    op->time = t;
+    return PRANDTL_OK;
+}
+
+int prandtl_op_get_time(prandtl_op op, double *t) {
+    if (!op || !t) return fail(PRANDTL_ERR_ARG, "null argument");
+    *t = op->time;
+    return PRANDTL_OK;
 }
 
 }  // extern "C"

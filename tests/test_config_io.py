@@ -100,8 +100,11 @@ def test_step_and_periodic_configurations():
 
 
 def test_refusals():
-    with pytest.raises(config.ConfigError, match="function-valued"):
-        _load("DoubleMachReflection")
+    with pytest.raises(config.ConfigError, match="not found"):
+        _load("DoubleMachReflection")      # its Gmsh lattice is not among the input meshes
+    with pytest.raises(config.ConfigError, match="no profile"):
+        _load_modified("DoubleMachReflection",
+                       lambda rt: rt["conditions"]["boundary_conditions"]["top"].update(function="SomeOtherBC"))
     with pytest.raises(config.ConfigError, match="axisymmetric"):
         _load("Ramp")
     with pytest.raises(config.ConfigError, match="not found"):
@@ -264,3 +267,30 @@ def test_reference_initial_states():
     assert np.allclose(u[:, 0, 0], s.bc_specs[0][1], rtol=1e-15)
     with pytest.raises(config.ConfigError, match="function of x"):
         config.reference_initial_state(_load("IsentropicVortex"))
+
+
+def test_double_mach_reflection_configuration():
+    """The double Mach reflection's configuration (TestCases/Euler/2D/DoubleMachReflection/config.json)
+    on the lattice its mesh file holds: the function-valued top condition becomes the moving front
+    (DoubleMachReflectionTopBCFunction with gammaM1Inverse = x1 = 2.5), the constant conditions
+    keep their vectors, and the initial state is the post-shock state exactly left of the incident
+    shock x = 1/6 + y/sqrt 3."""
+    m = cases.dmr_mesh(48, 12, 3)
+    c = config.load_case(os.path.join(CONFIGS, "DoubleMachReflection.json"), mesh=m)
+    assert c.bc_names == list(cases.DMR_BOUNDARIES)
+    assert [t for t, _ in c.bc_specs] == [5, 5, 1, 4, 5]
+    assert [pr for pr, _ in c.bc_profiles] == [0, 0, 0, 0, config.PROFILE_FRONT]
+    post = c.bc_specs[0][1]
+    assert c.bc_specs[4][1] == post                          # behind the front: the post-shock state
+    a = c.bc_profiles[4][1]
+    assert a[:5] == [0, 1, 1.0 / 6.0, 20.0, np.sqrt(3)] and a[5:] == [1.4, 0.0, 0.0, 2.5]
+    assert c.options["blending"] and c.scheme == config.SSPRK3 and c.dt == 1e-4 and c.final_time == 0.04
+    u = config.reference_initial_state(c)
+    X = m.node_coords()
+    behind = X[0] < 1.0 / 6.0 + X[1] / np.sqrt(3)
+    assert behind.any() and (~behind).any()
+    for v in range(4):
+        assert (u[v][behind] == post[v]).all() and (u[v][~behind] == a[5 + v]).all()
+    # the Operator's condition table carries the profile
+    bcs = c.bcs()
+    assert bcs[4].profile == config.PROFILE_FRONT and list(bcs[4].a)[:9] == a

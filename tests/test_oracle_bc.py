@@ -344,3 +344,157 @@ def test_wall_steady_states_converge(case):
         res.append(np.abs(du).max())
     # measured: second order (a factor 3.2-3.9 per halving of h), the rate of BR1 at these walls
     assert res[0] < 2e-2 and res[1] < res[0] / 3 and res[2] < res[1] / 3, res
+
+
+# ---------------------------------------------------------------------------
+# Function-valued boundary states (the ConditionFactory functions of the reference's cases)
+# ---------------------------------------------------------------------------
+def _dmr_bcs(gamma=G):
+    from prandtl_amd import config
+
+    post, pre = config._dmr_states(1.0 / (gamma - 1.0))
+    _, _, a = config._dmr_top({"x1": 1.0 / (gamma - 1.0)})
+    return [oracle.bc(oracle.BC_SPECIFIED_STATE, *post), oracle.bc(oracle.BC_SPECIFIED_STATE, *post),
+            oracle.bc(oracle.BC_SLIP_WALL), oracle.bc(oracle.BC_SUPERSONIC_OUTFLOW),
+            oracle.bc(oracle.BC_SPECIFIED_STATE, *post, profile=oracle.PROFILE_FRONT, a=a)], post, pre
+
+
+def test_dmr_states_are_a_mach10_shock():
+    """The double Mach reflection's two states (DoubleMachReflection.hpp:17-27) are the two sides of
+    a Mach 10 shock moving at speed 10 along n = (cos 30, -sin 30) into gas at rest with c = 1: in
+    the shock's frame the Rankine-Hugoniot jumps of mass, normal momentum and energy vanish and the
+    tangential velocity is continuous; and the front DoubleMachReflectionTopBCFunction places moves
+    at that speed (its x-speed 20/sqrt 3 is 10 / cos 30)."""
+    _, post, pre = _dmr_bcs()
+    n = np.array([np.sqrt(3) / 2, -0.5])
+    tang = np.array([0.5, np.sqrt(3) / 2])
+    r1, v1, p1 = _prim(np.array(pre))
+    r2, v2, p2 = _prim(np.array(post))
+    assert abs(np.sqrt(G * p1 / r1) - 1.0) < 1e-15
+    s = 10.0
+    w1, w2 = v1 @ n - s, v2 @ n - s
+    assert abs(r1 * w1 - r2 * w2) <= 1e-13 * abs(r1 * w1)
+    assert abs((r1 * w1 * w1 + p1) - (r2 * w2 * w2 + p2)) <= 1e-13 * (r1 * w1 * w1 + p1)
+    h1 = G / (G - 1) * p1 / r1 + 0.5 * w1 * w1
+    h2 = G / (G - 1) * p2 / r2 + 0.5 * w2 * w2
+    assert abs(h1 - h2) <= 1e-13 * h1
+    assert abs(v2 @ tang - v1 @ tang) <= 1e-13
+    assert abs(20.0 / np.sqrt(3) * n[0] - s) <= 1e-14
+
+
+def test_front_profile_position():
+    """PROFILE_FRONT switches from the state behind to the state ahead exactly on the line
+    x = 1/6 + (y + 20 t)/sqrt 3: located by bisection on x through the oracle at several heights
+    and times, and compared with that closed form."""
+    bcs, post, pre = _dmr_bcs()
+    b = bcs[4]
+    ph = oracle.phys()
+    for t in (0.0, 0.013, 0.04):
+        for y in (0.0, 0.37, 1.0):
+            lo, hi = -1.0, 5.0
+            assert np.allclose(oracle.bc_state(2, ph, b, [lo, y], t), post)
+            assert np.allclose(oracle.bc_state(2, ph, b, [hi, y], t), pre)
+            for _ in range(80):
+                mid = 0.5 * (lo + hi)
+                if np.array_equal(oracle.bc_state(2, ph, b, [mid, y], t), np.array(post)):
+                    lo = mid
+                else:
+                    hi = mid
+            assert abs(lo - (1.0 / 6.0 + (y + 20.0 * t) / np.sqrt(3))) <= 4e-16 * 5
+
+
+def test_ramp_profile():
+    """PROFILE_RAMP (NagashimaRampingInletBC): the velocity factor is 1/2 at t = t_r/2, rises
+    monotonically, is ~0 at t = 0 and exactly 1 (the clamp) well after t_r; density and pressure stay
+    the free stream's; the Nagashima free stream is Mach 2."""
+    from prandtl_amd import config
+
+    _, profile, a = config._nagashima_ramp({"x1": 0.02})
+    assert profile == config.PROFILE_RAMP
+    b = oracle.bc(oracle.BC_SUPERSONIC_INFLOW, profile=profile, a=a)
+    ph = oracle.phys()
+    tr, rho, V, p = a[0], a[1], a[2], a[4]
+    assert abs(V / np.sqrt(G * p / rho) - 2.0) <= 1e-13
+    prev = -1.0
+    for t in np.linspace(0.0, 3 * tr, 31):
+        s = oracle.bc_state(2, ph, b, [0.3, 0.1], t)
+        r_, v_, p_ = _prim(s)
+        assert abs(r_ - rho) == 0.0 and abs(p_ - p) <= 1e-12 * p and v_[1] == 0.0
+        f = v_[0] / V
+        assert f >= prev - 1e-15
+        prev = f
+    half = _prim(oracle.bc_state(2, ph, b, [0.0, 0.0], 0.5 * tr))[1][0] / V
+    assert abs(half - 0.5) <= 1e-15
+    assert _prim(oracle.bc_state(2, ph, b, [0.0, 0.0], 0.0))[1][0] / V < 1e-4
+    assert _prim(oracle.bc_state(2, ph, b, [0.0, 0.0], 10 * tr))[1][0] == V
+
+
+def test_front_rhs_changes_only_where_the_front_passed():
+    """The oracle's RHS with the moving front at two times differs exactly at the top-boundary face
+    nodes whose coordinates lie between the two front positions (the surface term of a face point
+    acts on its own node alone; Euler, no blending) -- found here from the node coordinates."""
+    m = cases.dmr_mesh(48, 12, 3)
+    bcs, _, _ = _dmr_bcs()
+    u = cases.dmr_state(m)
+    ph = oracle.phys()
+    t1, t2 = 0.01, 0.03
+    d1, _, _ = oracle.rhs_ex(m, ph, u, bcs, time=t1)
+    d2, _, _ = oracle.rhs_ex(m, ph, u, bcs, time=t2)
+    X = m.node_coords()
+    top = np.isclose(X[1], 1.0) & (m.neighbors[:, 3] < 0)[:, None]
+    f1 = X[0] < 1.0 / 6.0 + (X[1] + 20.0 * t1) / np.sqrt(3)
+    f2 = X[0] < 1.0 / 6.0 + (X[1] + 20.0 * t2) / np.sqrt(3)
+    flipped = top & (f1 != f2)
+    changed = (d1 != d2).any(axis=0)
+    assert flipped.sum() > 0
+    assert np.array_equal(changed, flipped)
+    # with the front far off, the profile is the constant state behind it (bit for bit)
+    far = list(bcs)
+    far[4] = oracle.bc(oracle.BC_SPECIFIED_STATE, *list(bcs[4].q)[:4], profile=oracle.PROFILE_FRONT,
+                       a=[0, 1, 100.0] + list(bcs[4].a)[3:9])
+    const = list(bcs)
+    const[4] = oracle.bc(oracle.BC_SPECIFIED_STATE, *list(bcs[4].q)[:4])
+    assert np.array_equal(oracle.rhs_ex(m, ph, u, far, time=t1)[0], oracle.rhs_ex(m, ph, u, const)[0])
+
+
+def test_stepper_stage_times():
+    """The oracle's steppers evaluate each stage at MFEM's stage time (ODESolver::Step's SetTime):
+    SSP-RK3 t, t + dt, t + dt/2 and LSRK4(5) t + c_i dt, checked by composing the steps from RHS
+    evaluations at those times; and the Carpenter-Kennedy c_i equal the row sums the 2N
+    coefficients A, B imply (c_1 = 0, K_1 = 1, K_i = A_i K_(i-1) + 1, c_(i+1) = c_i + B_i K_i)."""
+    A = [0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+         -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0]
+    B = [1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+         1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+         2277821191437.0 / 14882151754819.0]
+    C = [0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+         2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0]
+    c, K = [0.0], 0.0
+    for i in range(4):
+        K = A[i] * K + 1.0
+        c.append(c[-1] + B[i] * K)
+    assert np.allclose(c, C, rtol=0, atol=1e-12)
+    m = cases.dmr_mesh(24, 6, 2)
+    bcs, _, _ = _dmr_bcs()
+    u0 = cases.dmr_state(m)
+    ph = oracle.phys()
+    t0, dt = 0.011, 2e-4
+
+    def L(x, t):
+        return oracle.rhs_ex(m, ph, x, bcs, time=t)[0]
+
+    y1 = u0 + dt * L(u0, t0)
+    y2 = 0.75 * u0 + 0.25 * (y1 + dt * L(y1, t0 + dt))
+    ssp = u0 / 3.0 + 2.0 / 3.0 * (y2 + dt * L(y2, t0 + dt / 2))
+    got, _, t_end = oracle.rk_run_ex(m, ph, u0, 0, 1, dt, bcs, t0=t0, return_time=True)
+    assert t_end == t0 + dt
+    assert np.abs(got - ssp).max() <= 1e-13 * np.abs(ssp).max()
+    x, k = u0.copy(), np.zeros_like(u0)
+    for i in range(5):
+        k = A[i] * k + dt * L(x, t0 + C[i] * dt)
+        x = x + B[i] * k
+    got, _ = oracle.rk_run_ex(m, ph, u0, 1, 1, dt, bcs, t0=t0)
+    assert np.abs(got - x).max() <= 1e-13 * np.abs(x).max()
+    # a front that is already past: the stage times matter (the run at t0 differs from one at 0)
+    got0, _ = oracle.rk_run_ex(m, ph, u0, 0, 1, dt, bcs, t0=0.0)
+    assert not np.array_equal(got0, oracle.rk_run_ex(m, ph, u0, 0, 1, dt, bcs, t0=t0)[0])
