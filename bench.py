@@ -78,6 +78,11 @@ def warm_up(run_steps, steps):
             return done
 
 
+def set_warmup_min(seconds: float):
+    global WARMUP_MIN_S
+    WARMUP_MIN_S = max(0.0, float(seconds))
+
+
 def physical_cores():
     """Physical cores of the host (distinct (core, socket) pairs of `lscpu -p`), or None."""
     import subprocess
@@ -738,6 +743,8 @@ def main():
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--warmup-min-s", type=float, default=WARMUP_MIN_S,
+                    help="warm-up time before each timed leg (0: exactly --warmup steps; for profiler runs)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--compare-mixed", action="store_true",
                     help="run the FP32-mixed TGV comparison (FP64 and mixed side by side to --t-final) "
@@ -763,6 +770,7 @@ def main():
                     help="run the discrete-adjoint case (TGV Euler 16^3 p=4 + 64 Fourier modes, 200 RK stages, "
                          "checkpoints every 20 steps) instead of the default benchmark")
     args = ap.parse_args()
+    set_warmup_min(args.warmup_min_s)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (run with --gpus 1); "

@@ -45,13 +45,16 @@ def _jobs() -> int:
     return max(1, min(n or (os.cpu_count() or 4), 16))
 
 
-def build_library(extra_flags=None, verbose=False) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
+def build_library(extra_flags=None, verbose=False, lib_path=None, objdir=None) -> str:
+    """Compile and link the library (to `lib_path`, default prandtl_amd/libprandtl_amd.so; objects
+    in `objdir`, default prandtl_amd/_build/)."""
+    objdir = objdir or OBJDIR
+    os.makedirs(objdir, exist_ok=True)
     cc = hipcc()
     flags = HIPCC_FLAGS + list(extra_flags or [])
-    units = [(MAIN, os.path.join(OBJDIR, "prandtl_capi.o"), [])]
+    units = [(MAIN, os.path.join(objdir, "prandtl_capi.o"), [])]
     for d, n in STAGE_PAIRS:
-        units.append((STAGE, os.path.join(OBJDIR, f"stage_d{d}_np{n}.o"),
+        units.append((STAGE, os.path.join(objdir, f"stage_d{d}_np{n}.o"),
                       [f"-DPRANDTL_STAGE_DIM={d}", f"-DPRANDTL_STAGE_NP={n}"]))
     # largest units first (3D, high order), the C ABI last
     pending = units[1:][::-1] + units[:1]
@@ -74,15 +77,16 @@ def build_library(extra_flags=None, verbose=False) -> str:
     if failed:
         obj, rc, err = failed[0]
         raise RuntimeError(f"hipcc failed on {os.path.basename(obj)} ({rc}):\n{err[-4000:]}")
-    tmp = LIB + ".tmp"
+    lib = lib_path or LIB
+    tmp = lib + ".tmp"
     cmd = [cc] + flags + ["-shared", "-o", tmp] + [u[1] for u in units]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc link failed ({r.returncode}):\n{r.stderr[-4000:]}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

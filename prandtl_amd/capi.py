@@ -104,13 +104,15 @@ _lib = None
 
 
 def load():
-    """Load libprandtl_amd.so.  Raises if it has not been built: there is no fallback."""
+    """Load libprandtl_amd.so (or the build PRANDTL_AMD_LIB names).  Raises if it has not been built:
+    there is no fallback."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise PrandtlError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+        path = os.environ.get("PRANDTL_AMD_LIB") or LIB_PATH
+        if not os.path.exists(path):
+            raise PrandtlError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; "
                                f"g.build()'` (hipcc, gfx950)")
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
         L.prandtl_op_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(Physics), ctypes.POINTER(Opts),
                                         ctypes.POINTER(vp)]

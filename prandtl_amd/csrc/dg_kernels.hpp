@@ -418,7 +418,7 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
 #pragma unroll
                 for (int b = a + 1; b < NP; b++) {
                     double F[NEQ];
-                    volume_flux<DIM>(q[a], q[b], m, A.P.gammaM1Inverse, F);
+                    volume_flux<DIM>(q[a], q[b], m, A.P, F);
 #pragma unroll
                     for (int v = 0; v < NEQ; v++) {
                         acc[a][v] += A.T.Dhat2[a][b] * F[v];
@@ -872,7 +872,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
                     for (int c = 0; c < DIM; c++) mp[c] = (mo[c] + sMr[li][c]) * 0.5;
                     double F[NEQ];
-                    volume_flux<DIM, true>(qo, qb, mp, A.P.gammaM1Inverse, F);
+                    volume_flux<DIM, true>(qo, qb, mp, A.P, F);
                     const double dd = sT[a * NP + b];
 #pragma unroll
                     for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
@@ -909,7 +909,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         qo.vel[d] = sPv[d][tid];
                         qb.vel[d] = sPv[d][li];
                     }
-                    volume_flux_mx<DIM>(qo, qb, m, A.P.gammaM1Inverse, p_ref, h_ref, F);
+                    volume_flux_mx<DIM>(qo, qb, m, A.P, p_ref, h_ref, F);
                 } else if (BUG1 && dir == 2 && b < a) {
                     // the lower-zeta partner's slot holds its own last eta flux (:702 as written)
                     const int j = ai[1];
@@ -928,7 +928,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                             qo.vel[d] = sPv[d][li];
                             qb.vel[d] = sPv[d][lj];
                         }
-                        volume_flux<DIM, true>(qo, qb, sM[eb][1], A.P.gammaM1Inverse, F);
+                        volume_flux<DIM, true>(qo, qb, sM[eb][1], A.P, F);
                     }
                 } else {
                     NodePrim<DIM> qo, qb;
@@ -940,7 +940,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     qb.beta = sPd[1][li];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
-                    volume_flux<DIM, true>(qo, qb, m, A.P.gammaM1Inverse, F);
+                    volume_flux<DIM, true>(qo, qb, m, A.P, F);
                 }
             };
             if constexpr (ROT) {
@@ -950,7 +950,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 // one shared pass after this loop in directions 0 and 1 (HALF2)
                 const int lane = tid & 63;
                 const int KMAX = (HALF2 && dir < 2) ? (NP - 1) / 2 : NP / 2;
-#pragma unroll 1
+#pragma unroll
                 for (int k = 1; k <= KMAX; k++) {
                     const int b = (a + k) % NP;
                     R F[NEQ];
@@ -1027,7 +1027,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         qa.vel[d] = sPv[d][la];
                         qb.vel[d] = sPv[d][lb2];
                     }
-                    volume_flux_mx<DIM>(qa, qb, m, A.P.gammaM1Inverse, p_ref, h_ref, F);
+                    volume_flux_mx<DIM>(qa, qb, m, A.P, p_ref, h_ref, F);
                 } else {
                     NodePrim<DIM> qa, qb;
                     qa.rho = sPd[0][la];
@@ -1039,7 +1039,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         qa.vel[d] = sPv[d][la];
                         qb.vel[d] = sPv[d][lb2];
                     }
-                    volume_flux<DIM, true>(qa, qb, m, A.P.gammaM1Inverse, F);
+                    volume_flux<DIM, true>(qa, qb, m, A.P, F);
                 }
             }
             // the direction-0 pair of this node was evaluated by the lane of (i mod H, j), the

@@ -15,6 +15,9 @@ struct PhysConst {
     double mu, PrInverse, R_gas, mu_bulk;
     double mu0, T0, Ts;
     int sutherland;
+    // the logarithmic mean's series constants (lm_delta): kernel arguments, so that they enter the
+    // flux arithmetic as scalar operands instead of being rebuilt in vector registers at every use
+    double lm_k = 7.0 / 5.0, lm_c3 = 1.0 / 3.0, lm_c7 = 1.0 / 7.0;
 };
 
 // ---------------------------------------------------------------------------
@@ -94,18 +97,24 @@ __device__ __forceinline__ NodePrim<DIM> to_prim(const SideState<DIM> &q) {
     return r;
 }
 
+// delta = u/3 + u^2/5 + u^3/7 = u (1/3 + u (u + 7/5) / 7): one scalar constant per instruction
+__device__ __forceinline__ double lm_delta(double u, const PhysConst &P) {
+    return u * fma(u, (u + P.lm_k) * P.lm_c7, P.lm_c3);
+}
+
 // 1 / L(x, y), L the logarithmic mean, given r = 1/(x+y).
 // ref: ComputeLogMean, src/BasicOperations/BasicOperations.cpp:6-12.  With f = (y-x)/(y+x)
 // and u = f^2 (= the reference's (xi-1)^2/(xi+1)^2, xi = y/x), ln(y/x) = 2 artanh f, so
 // L = (x+y)/2 / (1 + u/3 + u^2/5 + u^3/7 + ...); the reference's series branch
 // (x+y) 52.5/(105 + u(35 + u(21 + 15u))) is that expansion to u^3.  Same branch point u < 1e-4;
 // the reciprocal form lets one 1/(x+y) serve the flux's other terms.
-__device__ __forceinline__ double inv_log_mean(double x, double y, double r) {
+__device__ __forceinline__ double inv_log_mean(double x, double y, double r, const PhysConst &P) {
     const double f = (y - x) * r;
     const double u = f * f;
     double il;
     if (u < 1e-4) {
-        il = 2.0 * r * (1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0))));
+        const double r2 = 2.0 * r;
+        il = fma(r2, lm_delta(u, P), r2);
     } else {
         il = log(y * rcp(x)) * rcp(y - x);
     }
@@ -113,15 +122,17 @@ __device__ __forceinline__ double inv_log_mean(double x, double y, double r) {
 }
 
 // 1 / L(x, y) and L(x, y) together, given r = 1/(x+y).  On the series branch
-// L = (x+y)/2 / S(u), S = 1 + u/3 + u^2/5 + u^3/7, and 1/S is itself a series,
-// 1 - u/3 - 4u^2/45 - 44u^3/945 + O(u^4) (the omitted term is below 1e-17 relative for
-// u < 1e-4), so L needs no reciprocal there.
-__device__ __forceinline__ void log_mean_both(double x, double y, double r, double &il, double &l) {
+// L = (x+y)/2 / (1 + delta), delta = u/3 + u^2/5 + u^3/7 < 3.4e-5, and 1/(1 + delta) =
+// 1 - delta (1 - delta (1 - delta)) + O(delta^4) (the omitted term is below 2e-18 relative),
+// so L needs no reciprocal there.
+__device__ __forceinline__ void log_mean_both(double x, double y, double r, const PhysConst &P, double &il,
+                                              double &l) {
     const double f = (y - x) * r;
     const double u = f * f;
     if (u < 1e-4) {
-        il = 2.0 * r * (1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0))));
-        l = 0.5 * (x + y) * (1.0 - u * (1.0 / 3.0 + u * (4.0 / 45.0 + u * (44.0 / 945.0))));
+        const double d = lm_delta(u, P), r2 = 2.0 * r;
+        il = fma(r2, d, r2);
+        l = 0.5 * (x + y) * fma(-d, fma(-d, 1.0 - d, 1.0), 1.0);
     } else {
         il = log(y * rcp(x)) * rcp(y - x);
         l = rcp(il);
@@ -134,18 +145,19 @@ __device__ __forceinline__ void log_mean_both(double x, double y, double r, doub
 // whose threads hold whole lines, keeps the reciprocal to stay within its registers).
 template <int DIM, bool LN_SERIES = false>
 __device__ __forceinline__ void volume_flux(const NodePrim<DIM> &a, const NodePrim<DIM> &b, const double *m,
-                                            double gammaM1Inverse, double *F) {
+                                            const PhysConst &P, double *F) {
+    const double gammaM1Inverse = P.gammaM1Inverse;
     const double rsum = a.rho + b.rho;
     const double r_rho = rcp(rsum);
     const double r_b = rcp(a.beta + b.beta);
     double il_rho, rho_ln;
     if constexpr (LN_SERIES) {
-        log_mean_both(a.rho, b.rho, r_rho, il_rho, rho_ln);
+        log_mean_both(a.rho, b.rho, r_rho, P, il_rho, rho_ln);
     } else {
-        il_rho = inv_log_mean(a.rho, b.rho, r_rho);
+        il_rho = inv_log_mean(a.rho, b.rho, r_rho, P);
         rho_ln = rcp(il_rho);
     }
-    const double il_b = inv_log_mean(a.beta, b.beta, r_b);
+    const double il_b = inv_log_mean(a.beta, b.beta, r_b, P);
     // with vs = v_a + v_b the mean velocity is vs/2, and the kinetic part of h-hat,
     // |mean|^2 - (|v_a|^2 + |v_b|^2)/4, is (v_a . v_b)/2: one dot product, no cancellation
     double vs[DIM];
@@ -194,8 +206,8 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
         const double r_rho = rcp(rsum);
         const double r_b = rcp(q1.beta + q2.beta);
         double il_rho, rho_ln;
-        log_mean_both(q1.rho, q2.rho, r_rho, il_rho, rho_ln);
-        const double il_b = inv_log_mean(q1.beta, q2.beta, r_b);
+        log_mean_both(q1.rho, q2.rho, r_rho, P, il_rho, rho_ln);
+        const double il_b = inv_log_mean(q1.beta, q2.beta, r_b, P);
         const double rho_mean = 0.5 * rsum;
         const double drho = q2.rho - q1.rho;
         // the kinetic parts through v1 . v2: h-hat's |mean|^2 - (|v1|^2 + |v2|^2)/4 = (v1 . v2)/2,
@@ -226,7 +238,7 @@ __device__ __forceinline__ void face_flux(const double *s1, const SideState<DIM>
         for (int d = 0; d < DIM; d++) flux[1 + d] = mflux * vm[d] + p_hat * n[d] - hl * (s2[1 + d] - s1[1 + d]);
         flux[DIM + 1] = mflux * h_hat - hl * diss;
     } else if constexpr (SURF == SURF_EC) {
-        volume_flux<DIM>(to_prim(q1), to_prim(q2), n, P.gammaM1Inverse, flux);
+        volume_flux<DIM>(to_prim(q1), to_prim(q2), n, P, flux);
     } else {
         double f1[DIM + 2], f2[DIM + 2];
         euler_flux_n<DIM>(s1, q1, n, f1);
@@ -281,12 +293,13 @@ struct NodePrimMx {
 
 template <int DIM>
 __device__ __forceinline__ void volume_flux_mx(const NodePrimMx<DIM> &a, const NodePrimMx<DIM> &b, const float *m,
-                                               double gammaM1Inverse, double p_ref, double h_ref, float *F) {
+                                               const PhysConst &P, double p_ref, double h_ref, float *F) {
+    const double gammaM1Inverse = P.gammaM1Inverse;
     const double rsum = a.rho + b.rho;
     const double r_rho = rcp(rsum);
     const double r_b = rcp(a.beta + b.beta);
-    const double il_rho = inv_log_mean(a.rho, b.rho, r_rho);
-    const double il_b = inv_log_mean(a.beta, b.beta, r_b);
+    const double il_rho = inv_log_mean(a.rho, b.rho, r_rho, P);
+    const double il_b = inv_log_mean(a.beta, b.beta, r_b, P);
     const double p_hat = 0.5 * rsum * r_b;
     const float ilr = (float)il_rho, ilb = (float)il_b;
     const float rho_ln = rcp(ilr);
@@ -317,8 +330,8 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
         const double rsum = q1.rho + q2.rho;
         const double r_rho = rcp(rsum);
         const double r_b = rcp(q1.beta + q2.beta);
-        const double il_rho = inv_log_mean(q1.rho, q2.rho, r_rho);
-        const double il_b = inv_log_mean(q1.beta, q2.beta, r_b);
+        const double il_rho = inv_log_mean(q1.rho, q2.rho, r_rho, P);
+        const double il_b = inv_log_mean(q1.beta, q2.beta, r_b, P);
         const double p_hat = 0.5 * rsum * r_b;
         const float ilr = (float)il_rho, ilb = (float)il_b;
         const float rho_ln = rcp(ilr);
@@ -364,7 +377,7 @@ __device__ __forceinline__ void face_flux_mx(const double *s1, const SideState<D
             a.vel[d] = (float)q1.vel[d];
             b.vel[d] = (float)q2.vel[d];
         }
-        volume_flux_mx<DIM>(a, b, n, P.gammaM1Inverse, p_ref, h_ref, flux);
+        volume_flux_mx<DIM>(a, b, n, P, p_ref, h_ref, flux);
     } else {
         double nd[DIM], f1[DIM + 2], f2[DIM + 2];
 #pragma unroll
