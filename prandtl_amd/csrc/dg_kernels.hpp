@@ -652,7 +652,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         }
         lam = qs.lam;
         if constexpr (LSIDE) sLam[tid] = qs.lam;
-        if constexpr (BLEND) qrp = rho_p<DIM>(s, A.P);
+        if constexpr (BLEND) qrp = qs.rho * qs.p;   // the indicator's rho p from the node's primitives
     }
     __syncthreads();
     if constexpr (BLEND) {
@@ -807,7 +807,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     // the viscous volume term) are loaded here, so their HBM latency overlaps the volume loop
     // instead of stalling the epilogue; the face terms are folded into acc (FOLD) to keep the
     // register budget.  Blending and general meshes keep their own registers for their work.
-    constexpr bool PRE = !BLEND && !GEN;
+    constexpr bool PRE = !GEN;
     double pre_x[NEQ];
     R pre_dv[NEQ - 1];
     if constexpr (PRE) {
@@ -828,8 +828,14 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             }
         }
     }
-    constexpr bool FOLD = PRE;
+    // affine elements fold the surface terms into the volume accumulator; blended ones weight the
+    // volume pairs by (1 - alpha) as they accumulate them, so that the fold holds for them too
+    constexpr bool FOLD = !GEN;
     __syncthreads();
+    R oma = R(1);   // 1 - alpha of the node's element (BLEND on affine elements)
+    if constexpr (BLEND && FOLD) {
+        if (active) oma = (R)(1.0 - sAlpha[eb][0]);
+    }
     int bad = 0;
     int ai[3];
     ai[0] = node % NP;
@@ -955,13 +961,13 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     const int b = (a + k) % NP;
                     R F[NEQ];
                     pair_flux(b, lb + b * st, F);
-                    const R dd = sT[a * NP + b];
+                    const R dd = BLEND ? sT[a * NP + b] * oma : sT[a * NP + b];
 #pragma unroll
                     for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
                     if (2 * k != NP) {
                         const int c = (a + NP - k) % NP;
                         const int src = lane + (c - a) * st;
-                        const R dc = sT[a * NP + c];
+                        const R dc = BLEND ? sT[a * NP + c] * oma : sT[a * NP + c];
 #pragma unroll
                         for (int v = 0; v < NEQ; v++) acc[v] += dc * __shfl(F[v], src, 64);
                     }
@@ -972,7 +978,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     const int b = (a + k) % NP;
                     R F[NEQ];
                     pair_flux(b, lb + b * st, F);
-                    const R dd = sT[a * NP + b];
+                    const R dd = BLEND ? sT[a * NP + b] * oma : sT[a * NP + b];
 #pragma unroll
                     for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
                 }
@@ -1048,21 +1054,26 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
             const int jm = j < H ? j : j - H;
             const int idy = i + NP * jm;
             const int yl = lane - (node - n00) + (H + idy % H) + (idy / H) * NP;
-            const R dx = sT[i * NP + (i < H ? i + H : i - H)];
-            const R dy = sT[j * NP + (j < H ? j + H : j - H)];
+            const R dx = sT[i * NP + (i < H ? i + H : i - H)] * oma;
+            const R dy = sT[j * NP + (j < H ? j + H : j - H)] * oma;
 #pragma unroll
             for (int v = 0; v < NEQ; v++) acc[v] += dx * __shfl(F[v], xl, 64) + dy * __shfl(F[v], yl, 64);
         }
     }
     if constexpr (BLEND) {
         if (active) {
-            // fold the DG terms now, (1 - alpha) volume + surface, to free registers for the FV
-            // items; acc then collects the RHS and the FV term is added into it
-            const double al = sAlpha[eb][0], sw = A.T.inv_w0 * invJ;
+            // the DG terms, (1 - alpha) volume + surface, as the RHS so far; acc then collects the
+            // RHS and the FV term is added into it
+            if constexpr (FOLD) {
 #pragma unroll
-            for (int v = 0; v < NEQ; v++) {
-                acc[v] = (-acc[v] * invJ) * (1.0 - al) + accs[v] * sw;
-                accs[v] = 0.0;
+                for (int v = 0; v < NEQ; v++) acc[v] = -acc[v] * invJ;
+            } else {
+                const double al = sAlpha[eb][0], sw = A.T.inv_w0 * invJ;
+#pragma unroll
+                for (int v = 0; v < NEQ; v++) {
+                    acc[v] = (-acc[v] * invJ) * (1.0 - al) + accs[v] * sw;
+                    accs[v] = 0.0;
+                }
             }
         }
         bool any = false;
