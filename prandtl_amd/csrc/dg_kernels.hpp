@@ -567,6 +567,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     // volume pairs: each symmetric pair evaluated once when the lines of an element lie in one
     // wave (the partner's value arrives by a cross-lane shuffle)
     constexpr bool ROT = !GEN && !BUG1 && (C::BLOCK == 64 || NN == 64);
+    constexpr bool ROTG = GEN && (C::BLOCK == 64 || NN == 64);   // the same on general meshes
     // even Np: the Np/2-apart pairs of directions 0 and 1 share one pass (half of each direction's
     // lanes would otherwise evaluate their partner's pair again)
     constexpr bool HALF2 = ROT && NP % 2 == 0;
@@ -861,10 +862,8 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 double mo[DIM];
 #pragma unroll
                 for (int c = 0; c < DIM; c++) mo[c] = sMr[tid][c];
-#pragma unroll 1
-                for (int k = 1; k < NP; k++) {
-                    const int b = (a + k) % NP;
-                    const int li = lb + b * st;
+                // the pair (a, b) with the averaged metric: symmetric in a and b like the flux
+                auto gen_pair = [&](int li, double *F) {
                     NodePrim<DIM> qo, qb;
                     qo.rho = sPd[0][tid];
                     qo.beta = sPd[1][tid];
@@ -877,11 +876,38 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     double mp[DIM];
 #pragma unroll
                     for (int c = 0; c < DIM; c++) mp[c] = (mo[c] + sMr[li][c]) * 0.5;
-                    double F[NEQ];
                     volume_flux<DIM, true>(qo, qb, mp, A.P, F);
-                    const double dd = sT[a * NP + b];
+                };
+                if constexpr (ROTG) {
+                    // each symmetric pair once where an element's lines lie in one wave: node a
+                    // evaluates (a, a+k), k <= Np/2, and receives (a-k, a) from the lane of a-k
+                    const int lane = tid & 63;
 #pragma unroll
-                    for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                    for (int k = 1; 2 * k <= NP; k++) {
+                        const int b = (a + k) % NP;
+                        double F[NEQ];
+                        gen_pair(lb + b * st, F);
+                        const double dd = sT[a * NP + b];
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                        if (2 * k != NP) {
+                            const int c = (a + NP - k) % NP;
+                            const int src = lane + (c - a) * st;
+                            const double dc = sT[a * NP + c];
+#pragma unroll
+                            for (int v = 0; v < NEQ; v++) acc[v] += dc * __shfl(F[v], src, 64);
+                        }
+                    }
+                } else {
+#pragma unroll 1
+                    for (int k = 1; k < NP; k++) {
+                        const int b = (a + k) % NP;
+                        double F[NEQ];
+                        gen_pair(lb + b * st, F);
+                        const double dd = sT[a * NP + b];
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                    }
                 }
                 if (a == 0 || a == NP - 1) {
                     const int side = (a == NP - 1) ? 1 : 0;
