@@ -46,22 +46,47 @@ __device__ __forceinline__ double rho_p(const double *s, const PhysConst &P) {
     return s[0] * (gammaM1 * (s[DIM + 1] - 0.5 * s[0] * vv));
 }
 
+// Orders the coefficient block's LDS accesses: a workgroup barrier, or, where every element's
+// nodes lie in one wave (each step reads and writes only its own element's slots), the wave's own
+// ordering of its LDS instructions.
+template <bool WAVE>
+__device__ __forceinline__ void coef_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+// Partial sums per modal energy: the largest divisor K of NN with 3 K <= NN, so that each of the
+// 3 K partial sums (NN / K squares) has a lane of its own.
+constexpr int energy_chunks(int nn) {
+    for (int k = nn / 3; k > 1; k--)
+        if (nn % k == 0) return k;
+    return 1;
+}
+
 // The workgroup's elements (EPB of NN = Np^dim nodes, thread tid = eb NN + node) from q = rho p
-// at the nodes: modes by three sum-factorised passes (double-buffered in sA / sB, one barrier
-// each), the three modal energies summed per element in a fixed order, eta and alpha into
-// sRes[eb] = {alpha, eta}.  Every thread of the workgroup calls it (it contains barriers);
-// sA, sB and sC hold EPB NN doubles each.  sRes is valid after the caller's next barrier.
-template <int DIM, int NP, int EPB>
+// at the nodes: modes by three sum-factorised passes (double-buffered in sA / sB), the three
+// modal energies as K partial sums over consecutive nodes then their sum in chunk order (a fixed
+// order, so the result is reproducible), eta and alpha into sRes[eb] = {alpha, eta}.  Every
+// thread of the workgroup calls it (it may contain barriers); sVi holds V1^-1 (Np x Np, row k =
+// degree k) in LDS; sA, sB and sC hold EPB NN doubles each.  WAVE: every element lies in one wave
+// (wave-local ordering in place of the barriers).  sRes is valid after the caller's next barrier.
+template <int DIM, int NP, int EPB, bool WAVE>
 __device__ __forceinline__ void blend_coef_block(double q, bool active, int tid, int eb, int node,
-                                                 const BlendParams<NP> &B, double *sA, double *sB, double *sC,
-                                                 double (*sSum)[3], double (*sRes)[2]) {
+                                                 const BlendParams<NP> &B, const double *sVi, double *sA, double *sB,
+                                                 double *sC, double (*sRes)[2]) {
     constexpr int NN = ipow(NP, DIM);
+    constexpr int K = energy_chunks(NN), CH = NN / K;
     int ai[3];
     ai[0] = node % NP;
     ai[1] = (node / NP) % NP;
     ai[2] = node / (NP * NP);
     if (active) sA[tid] = q;
-    __syncthreads();
+    coef_sync<WAVE>();
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         const double *src = (d & 1) ? sB : sA;
@@ -72,11 +97,11 @@ __device__ __forceinline__ void blend_coef_block(double q, bool active, int tid,
             const int lb = tid - a * st;
             double t = 0.0;
 #pragma unroll
-            for (int b = 0; b < NP; b++) t += B.Vi[a][b] * src[lb + b * st];
+            for (int b = 0; b < NP; b++) t += sVi[a * NP + b] * src[lb + b * st];
             q = t;
             if (d + 1 < DIM) dst[tid] = t;
         }
-        __syncthreads();   // the last pass's reads are done before sA / sB are reused below
+        coef_sync<WAVE>();   // the last pass's reads are done before sA / sB are reused below
     }
     // q: the mode of degrees (ai[0], ai[1], ai[2])
     if (active) {
@@ -88,16 +113,26 @@ __device__ __forceinline__ void blend_coef_block(double q, bool active, int tid,
         sB[tid] = md <= NP - 2 ? q2 : 0.0;   // modesM1: no degree above p - 1
         sA[tid] = md <= NP - 3 ? q2 : 0.0;   // modesM2: no degree above p - 2
     }
-    __syncthreads();
-    if (active && node < 3) {
-        const double *src = node == 0 ? sC : (node == 1 ? sB : sA);
-        double acc = 0.0;
-        for (int i = 0; i < NN; i++) acc += src[eb * NN + i];
-        sSum[eb][node] = acc;
+    coef_sync<WAVE>();
+    if (active && node < 3 * K) {
+        // partial sum c of energy set: its chunk's squares, into the chunk's first slot (no
+        // other lane reads that chunk)
+        const int set = node / K, c = node - set * K;
+        double *src = (set == 0 ? sC : (set == 1 ? sB : sA)) + eb * NN + c * CH;
+        double acc = src[0];
+#pragma unroll
+        for (int i = 1; i < CH; i++) acc += src[i];
+        src[0] = acc;
     }
-    __syncthreads();
+    coef_sync<WAVE>();
     if (active && node == 0) {
-        const double mm = sSum[eb][0], m1 = sSum[eb][1], m2 = sSum[eb][2];
+        double mm = 0.0, m1 = 0.0, m2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            mm += sC[eb * NN + c * CH];
+            m1 += sB[eb * NN + c * CH];
+            m2 += sA[eb * NN + c * CH];
+        }
         double ind = 1.0 - m1 / mm;
         const double ind2 = 1.0 - m2 / m1;
         ind = ind2 > ind ? ind2 : ind;                       // std::max(ind, ind2)
@@ -117,7 +152,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_blend_coef(const 
     using C = NodeCfg<DIM, NP>;
     constexpr int NEQ = DIM + 2, NN = C::NN, EPB = C::EPB;
     __shared__ double sA[EPB * NN], sB[EPB * NN], sC[EPB * NN];
-    __shared__ double sSum[EPB][3];
+    __shared__ double sVi[NP * NP];
     __shared__ double sRes[EPB][2];
 
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -127,6 +162,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_blend_coef(const 
     const bool active = (eb < EPB) && (e < A.NE);
     const size_t vs = (size_t)A.NE * NN;
 
+    if (tid < NP * NP) sVi[tid] = A.B.Vi[tid / NP][tid % NP];
     double q = 0.0;
     if (active) {
         double s[NEQ];
@@ -134,7 +170,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK)) k_blend_coef(const 
         for (int v = 0; v < NEQ; v++) s[v] = A.u[v * vs + (size_t)e * NN + node];
         q = rho_p<DIM>(s, A.P);
     }
-    blend_coef_block<DIM, NP, EPB>(q, active, tid, eb, node, A.B, sA, sB, sC, sSum, sRes);
+    __syncthreads();   // sVi
+    blend_coef_block<DIM, NP, EPB, C::BLOCK == 64 || 64 % NN == 0>(q, active, tid, eb, node, A.B, sVi, sA, sB, sC,
+                                                                   sRes);
     if (active && node == 0) {
         if (A.alpha) A.alpha[e] = sRes[eb][0];
         if (A.eta) A.eta[e] = sRes[eb][1];

@@ -557,7 +557,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     constexpr int SITEMS = (NP - 1) * L;  // subcell interfaces per element and direction
     static_assert(SITEMS <= FITEMS, "subcell fluxes are staged in the surface-flux buffer");
     static_assert(!BLEND || EPB * FITEMS * NEQ >= 3 * EPB * NN, "the coefficient scratch fits the surface-flux buffer");
-    __shared__ double sBSum[BLEND ? EPB : 1][3];
+    __shared__ double sVi[BLEND ? NP * NP : 1];     // V1^-1 of the blending indicator
     __shared__ double sAlpha[BLEND ? EPB : 1][2];   // alpha, eta
     // affine FP64 interior faces take the element's own side from LDS: the primitives above and
     // each node's max wave speed, and each face direction's |adj row|
@@ -583,6 +583,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
     const bool active = (eb < EPB) && (e < NE);
     const size_t vs = (size_t)NE * NN;
     if (tid < NP * NP) sT[tid] = (R)A.T.Dhat2[tid / NP][tid % NP];
+    if constexpr (BLEND) {
+        if (tid < NP * NP) sVi[tid] = A.blend.Vi[tid / NP][tid % NP];
+    }
     // the neighbour table rows and metric rows of the workgroup's elements, read once here
     // rather than by every face item (EPB (2 dim + dim^2) entries can exceed the block size)
     for (int i = tid; i < EPB * 2 * DIM; i += C::BLOCK) {
@@ -660,8 +663,8 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
         // the elements' blending coefficients (DGSEMOperator.cpp:547-549), the surface-flux
         // buffer as scratch before the face items fill it
         double *scr = &sF[0][0];
-        blend_coef_block<DIM, NP, EPB>(qrp, active, tid, eb, node, A.blend, scr, scr + EPB * NN, scr + 2 * EPB * NN,
-                                       sBSum, sAlpha);
+        blend_coef_block<DIM, NP, EPB, C::BLOCK == 64 || 64 % NN == 0>(qrp, active, tid, eb, node, A.blend, sVi, scr,
+                                                                       scr + EPB * NN, scr + 2 * EPB * NN, sAlpha);
         __syncthreads();
         if (active && node == 0 && A.alpha) A.alpha[e] = sAlpha[eb][0];
     }
@@ -804,96 +807,208 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
         for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
     }
-    // Affine elements without blending: the epilogue's operands (the RK register or base state,
-    // the viscous volume term) are loaded here, so their HBM latency overlaps the volume loop
-    // instead of stalling the epilogue; the face terms are folded into acc (FOLD) to keep the
-    // register budget.  Blending and general meshes keep their own registers for their work.
-    constexpr bool PRE = !GEN;
-    double pre_x[NEQ];
-    R pre_dv[NEQ - 1];
-    if constexpr (PRE) {
-        if (active) {
+    // The volume terms and the stage epilogue, one body instantiated for the blended path (BL) and
+    // for the unblended one: a blended workgroup whose elements all have alpha = 0 (the usual case
+    // away from shocks) takes the unblended body, which needs neither the (1 - alpha) weights nor
+    // the registers of the finite-volume work.
+    // (A by value: a copy of the kernel arguments that the body only reads, so that their loads are
+    // not repeated after every store the body makes.  The copy is indexed at fixed offsets only,
+    // else it would be placed in scratch memory: the two tables the finite-volume terms index per
+    // lane are read through pointers into the kernel arguments themselves.)
+    const double *const kinvw = A.T.inv_w;
+    const double(*const ksub)[NP] = A.T.sub;
+    int bad = 0;
+    auto body = [&, A](auto tag) __attribute__((always_inline)) {
+        constexpr bool BL = decltype(tag)::value;
+        // Affine elements: the epilogue's operands (the RK register or base state, the viscous
+        // volume term) are loaded here, so their HBM latency overlaps the volume loop instead of
+        // stalling the epilogue; the face terms are folded into acc (FOLD) to keep the register
+        // budget.  General meshes keep those registers for their per-node metrics.
+        constexpr bool PRE = !GEN;
+        double pre_x[NEQ];
+        R pre_dv[NEQ - 1];
+        if constexpr (PRE) {
+            if (active) {
 #pragma unroll
-            for (int v = 0; v < NEQ; v++) {
-                const size_t gi = v * vs + (size_t)e * NN + node;
-                pre_x[v] = A.mode == EPI_LSRK ? A.k[gi] : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
-            }
-            if constexpr (VISC) {
+                for (int v = 0; v < NEQ; v++) {
+                    const size_t gi = v * vs + (size_t)e * NN + node;
+                    pre_x[v] = A.mode == EPI_LSRK ? A.k[gi] : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
+                }
+                if constexpr (VISC) {
 #pragma unroll
-                for (int v = 0; v < NEQ - 1; v++) {
-                    if constexpr (MIX)
-                        pre_dv[v] = A.dvf[v * vs + (size_t)e * NN + node];
-                    else
-                        pre_dv[v] = A.dv[v * vs + (size_t)e * NN + node];
+                    for (int v = 0; v < NEQ - 1; v++) {
+                        if constexpr (MIX)
+                            pre_dv[v] = A.dvf[v * vs + (size_t)e * NN + node];
+                        else
+                            pre_dv[v] = A.dv[v * vs + (size_t)e * NN + node];
+                    }
                 }
             }
         }
-    }
-    // affine elements fold the surface terms into the volume accumulator; blended ones weight the
-    // volume pairs by (1 - alpha) as they accumulate them, so that the fold holds for them too
-    constexpr bool FOLD = !GEN;
-    __syncthreads();
-    R oma = R(1);   // 1 - alpha of the node's element (BLEND on affine elements)
-    if constexpr (BLEND && FOLD) {
-        if (active) oma = (R)(1.0 - sAlpha[eb][0]);
-    }
-    int bad = 0;
-    int ai[3];
-    ai[0] = node % NP;
-    ai[1] = (node / NP) % NP;
-    ai[2] = node / (NP * NP);
-    R acc[NEQ], accs[NEQ];
+        // affine elements fold the surface terms into the volume accumulator; blended ones weight the
+        // volume pairs by (1 - alpha) as they accumulate them, so that the fold holds for them too
+        constexpr bool FOLD = !GEN;
+        __syncthreads();
+        R oma = R(1);   // 1 - alpha of the node's element (BLEND on affine elements)
+        if constexpr (BL && FOLD) {
+            if (active) oma = (R)(1.0 - sAlpha[eb][0]);
+        }
+        int ai[3];
+        ai[0] = node % NP;
+        ai[1] = (node / NP) % NP;
+        ai[2] = node / (NP * NP);
+        R acc[NEQ], accs[NEQ];
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
-    double invJ = 0.0;
-    if constexpr (GEN) {
-        // per direction: the nodes' adj rows into sMr, then the pair fluxes with the averaged
-        // metric 1/2 (adj_d(i) + adj_d(j)) (ComputeMean, ChandrashekarFlux.cpp:15)
-        if (active) invJ = rcp(detJ);
+        for (int v = 0; v < NEQ; v++) acc[v] = accs[v] = R(0);
+        double invJ = 0.0;
+        if constexpr (GEN) {
+            // per direction: the nodes' adj rows into sMr, then the pair fluxes with the averaged
+            // metric 1/2 (adj_d(i) + adj_d(j)) (ComputeMean, ChandrashekarFlux.cpp:15)
+            if (active) invJ = rcp(detJ);
 #pragma unroll
-        for (int dir = 0; dir < DIM; dir++) {
-            if (dir > 0) __syncthreads();   // the previous direction's readers of sMr are done
-            load_rows(dir);
-            __syncthreads();
-            if (active) {
+            for (int dir = 0; dir < DIM; dir++) {
+                if (dir > 0) __syncthreads();   // the previous direction's readers of sMr are done
+                load_rows(dir);
+                __syncthreads();
+                if (active) {
+                    const int st = dir_stride<NP>(dir);
+                    const int a = ai[dir];
+                    const int lb = eb * NN + node - a * st;
+                    double mo[DIM];
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) mo[c] = sMr[tid][c];
+                    // the pair (a, b) with the averaged metric: symmetric in a and b like the flux
+                    auto gen_pair = [&](int li, double *F) {
+                        NodePrim<DIM> qo, qb;
+                        qo.rho = sPd[0][tid];
+                        qo.beta = sPd[1][tid];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) qo.vel[d] = sPv[d][tid];
+                        qb.rho = sPd[0][li];
+                        qb.beta = sPd[1][li];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
+                        double mp[DIM];
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) mp[c] = (mo[c] + sMr[li][c]) * 0.5;
+                        volume_flux<DIM, true>(qo, qb, mp, A.P, F);
+                    };
+                    if constexpr (ROTG) {
+                        // each symmetric pair once where an element's lines lie in one wave: node a
+                        // evaluates (a, a+k), k <= Np/2, and receives (a-k, a) from the lane of a-k
+                        const int lane = tid & 63;
+#pragma unroll
+                        for (int k = 1; 2 * k <= NP; k++) {
+                            const int b = (a + k) % NP;
+                            double F[NEQ];
+                            gen_pair(lb + b * st, F);
+                            const double dd = sT[a * NP + b];
+#pragma unroll
+                            for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                            if (2 * k != NP) {
+                                const int c = (a + NP - k) % NP;
+                                const int src = lane + (c - a) * st;
+                                const double dc = sT[a * NP + c];
+#pragma unroll
+                                for (int v = 0; v < NEQ; v++) acc[v] += dc * __shfl(F[v], src, 64);
+                            }
+                        }
+                    } else {
+#pragma unroll 1
+                        for (int k = 1; k < NP; k++) {
+                            const int b = (a + k) % NP;
+                            double F[NEQ];
+                            gen_pair(lb + b * st, F);
+                            const double dd = sT[a * NP + b];
+#pragma unroll
+                            for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
+                        }
+                    }
+                    if (a == 0 || a == NP - 1) {
+                        const int side = (a == NP - 1) ? 1 : 0;
+                        const int it = eb * FITEMS + (dir * 2 + side) * L + line_index<NP>(node, dir);
+                        const double sg = side ? -1.0 : 1.0;
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
+                    }
+                }
+            }
+        } else if (active) {
+            invJ = rcp(A.geom[(size_t)e * (1 + DIM * DIM)]);
+            const double p_ref = MIX ? sPref[eb] : 0.0;
+            const double h_ref = MIX ? sHref[eb] : 0.0;
+#pragma unroll
+            for (int dir = 0; dir < DIM; dir++) {
                 const int st = dir_stride<NP>(dir);
                 const int a = ai[dir];
                 const int lb = eb * NN + node - a * st;
-                double mo[DIM];
+                const R *m = sM[GEN ? 0 : eb][dir];   // (GEN takes the branch above)
+                // the two-point flux with the node at LDS index li (the line partner b)
+                auto pair_flux = [&](int b, int li, R *F) {
+                    if constexpr (MIX) {
+                        NodePrimMx<DIM> qo, qb;
+                        qo.rho = sPd[0][tid];
+                        qo.beta = sPd[1][tid];
+                        qb.rho = sPd[0][li];
+                        qb.beta = sPd[1][li];
 #pragma unroll
-                for (int c = 0; c < DIM; c++) mo[c] = sMr[tid][c];
-                // the pair (a, b) with the averaged metric: symmetric in a and b like the flux
-                auto gen_pair = [&](int li, double *F) {
-                    NodePrim<DIM> qo, qb;
-                    qo.rho = sPd[0][tid];
-                    qo.beta = sPd[1][tid];
+                        for (int d = 0; d < DIM; d++) {
+                            qo.vel[d] = sPv[d][tid];
+                            qb.vel[d] = sPv[d][li];
+                        }
+                        volume_flux_mx<DIM>(qo, qb, m, A.P, p_ref, h_ref, F);
+                    } else if (BUG1 && dir == 2 && b < a) {
+                        // the lower-zeta partner's slot holds its own last eta flux (:702 as written)
+                        const int j = ai[1];
+                        if (j == NP - 1) {
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) qo.vel[d] = sPv[d][tid];
-                    qb.rho = sPd[0][li];
-                    qb.beta = sPd[1][li];
+                            for (int v = 0; v < NEQ; v++) F[v] = R(0);
+                        } else {
+                            const int lj = li + (NP - 1 - j) * NP;   // node (i, Np-1, b)
+                            NodePrim<DIM> qo, qb;
+                            qo.rho = sPd[0][li];
+                            qo.beta = sPd[1][li];
+                            qb.rho = sPd[0][lj];
+                            qb.beta = sPd[1][lj];
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
-                    double mp[DIM];
+                            for (int d = 0; d < DIM; d++) {
+                                qo.vel[d] = sPv[d][li];
+                                qb.vel[d] = sPv[d][lj];
+                            }
+                            volume_flux<DIM, true>(qo, qb, sM[eb][1], A.P, F);
+                        }
+                    } else {
+                        NodePrim<DIM> qo, qb;
+                        qo.rho = sPd[0][tid];
+                        qo.beta = sPd[1][tid];
 #pragma unroll
-                    for (int c = 0; c < DIM; c++) mp[c] = (mo[c] + sMr[li][c]) * 0.5;
-                    volume_flux<DIM, true>(qo, qb, mp, A.P, F);
+                        for (int d = 0; d < DIM; d++) qo.vel[d] = sPv[d][tid];
+                        qb.rho = sPd[0][li];
+                        qb.beta = sPd[1][li];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
+                        volume_flux<DIM, true>(qo, qb, m, A.P, F);
+                    }
                 };
-                if constexpr (ROTG) {
-                    // each symmetric pair once where an element's lines lie in one wave: node a
-                    // evaluates (a, a+k), k <= Np/2, and receives (a-k, a) from the lane of a-k
+                if constexpr (ROT) {
+                    // each symmetric pair once: node a evaluates (a, a+k) for k <= Np/2 and receives
+                    // (a-k, a) from the lane of node a-k (the same flux, F(u_i,u_j) = F(u_j,u_i));
+                    // 2k = Np pairs are evaluated by both of their nodes in the last direction, and in
+                    // one shared pass after this loop in directions 0 and 1 (HALF2)
                     const int lane = tid & 63;
+                    const int KMAX = (HALF2 && dir < 2) ? (NP - 1) / 2 : NP / 2;
 #pragma unroll
-                    for (int k = 1; 2 * k <= NP; k++) {
+                    for (int k = 1; k <= KMAX; k++) {
                         const int b = (a + k) % NP;
-                        double F[NEQ];
-                        gen_pair(lb + b * st, F);
-                        const double dd = sT[a * NP + b];
+                        R F[NEQ];
+                        pair_flux(b, lb + b * st, F);
+                        const R dd = BL ? sT[a * NP + b] * oma : sT[a * NP + b];
 #pragma unroll
                         for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
                         if (2 * k != NP) {
                             const int c = (a + NP - k) % NP;
                             const int src = lane + (c - a) * st;
-                            const double dc = sT[a * NP + c];
+                            const R dc = BL ? sT[a * NP + c] * oma : sT[a * NP + c];
 #pragma unroll
                             for (int v = 0; v < NEQ; v++) acc[v] += dc * __shfl(F[v], src, 64);
                         }
@@ -902,9 +1017,9 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll 1
                     for (int k = 1; k < NP; k++) {
                         const int b = (a + k) % NP;
-                        double F[NEQ];
-                        gen_pair(lb + b * st, F);
-                        const double dd = sT[a * NP + b];
+                        R F[NEQ];
+                        pair_flux(b, lb + b * st, F);
+                        const R dd = BL ? sT[a * NP + b] * oma : sT[a * NP + b];
 #pragma unroll
                         for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
                     }
@@ -912,308 +1027,217 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                 if (a == 0 || a == NP - 1) {
                     const int side = (a == NP - 1) ? 1 : 0;
                     const int it = eb * FITEMS + (dir * 2 + side) * L + line_index<NP>(node, dir);
-                    const double sg = side ? -1.0 : 1.0;
+                    if constexpr (FOLD) {
+                        // du = -(1/J)(acc - accs/w0): the face term enters acc as -/+ f/w0
+                        const R sg = (R)(side ? A.T.inv_w0 : -A.T.inv_w0);
 #pragma unroll
-                    for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
+                        for (int v = 0; v < NEQ; v++) acc[v] += sg * sF[it][v];
+                    } else {
+                        const R sg = side ? R(-1) : R(1);
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
+                    }
                 }
             }
-        }
-    } else if (active) {
-        invJ = rcp(A.geom[(size_t)e * (1 + DIM * DIM)]);
-        const double p_ref = MIX ? sPref[eb] : 0.0;
-        const double h_ref = MIX ? sHref[eb] : 0.0;
-#pragma unroll
-        for (int dir = 0; dir < DIM; dir++) {
-            const int st = dir_stride<NP>(dir);
-            const int a = ai[dir];
-            const int lb = eb * NN + node - a * st;
-            const R *m = sM[GEN ? 0 : eb][dir];   // (GEN takes the branch above)
-            // the two-point flux with the node at LDS index li (the line partner b)
-            auto pair_flux = [&](int b, int li, R *F) {
-                if constexpr (MIX) {
-                    NodePrimMx<DIM> qo, qb;
-                    qo.rho = sPd[0][tid];
-                    qo.beta = sPd[1][tid];
-                    qb.rho = sPd[0][li];
-                    qb.beta = sPd[1][li];
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        qo.vel[d] = sPv[d][tid];
-                        qb.vel[d] = sPv[d][li];
-                    }
-                    volume_flux_mx<DIM>(qo, qb, m, A.P, p_ref, h_ref, F);
-                } else if (BUG1 && dir == 2 && b < a) {
-                    // the lower-zeta partner's slot holds its own last eta flux (:702 as written)
-                    const int j = ai[1];
-                    if (j == NP - 1) {
-#pragma unroll
-                        for (int v = 0; v < NEQ; v++) F[v] = R(0);
-                    } else {
-                        const int lj = li + (NP - 1 - j) * NP;   // node (i, Np-1, b)
-                        NodePrim<DIM> qo, qb;
-                        qo.rho = sPd[0][li];
-                        qo.beta = sPd[1][li];
-                        qb.rho = sPd[0][lj];
-                        qb.beta = sPd[1][lj];
+            if constexpr (HALF2) {
+                // the pairs (a, a + Np/2) of directions 0 and 1, each once: a lane with i < Np/2 takes
+                // its node's direction-0 pair (i, j) - (i + Np/2, j); each other lane one
+                // direction-1 pair (i', j') - (i', j' + Np/2) with j' < Np/2, numbered
+                // (i - Np/2) + (Np/2) j = i' + Np j'.  Both nodes of a pair receive it by shuffles.
+                constexpr int H = NP / 2;
+                const int lane = tid & 63;
+                const int i = ai[0], j = ai[1];
+                const int n00 = node - i - j * NP;   // node (0, 0, k)
+                int na, nb, dl;
+                if (i < H) {
+                    na = node;
+                    nb = node + H;
+                    dl = 0;
+                } else {
+                    const int idx = (i - H) + H * j;
+                    na = n00 + (idx % NP) + (idx / NP) * NP;
+                    nb = na + H * NP;
+                    dl = 1;
+                }
+                R F[NEQ];
+                {
+                    const int la = eb * NN + na, lb2 = eb * NN + nb;
+                    const R *m = sM[eb][dl];
+                    if constexpr (MIX) {
+                        NodePrimMx<DIM> qa, qb;
+                        qa.rho = sPd[0][la];
+                        qa.beta = sPd[1][la];
+                        qb.rho = sPd[0][lb2];
+                        qb.beta = sPd[1][lb2];
 #pragma unroll
                         for (int d = 0; d < DIM; d++) {
-                            qo.vel[d] = sPv[d][li];
-                            qb.vel[d] = sPv[d][lj];
+                            qa.vel[d] = sPv[d][la];
+                            qb.vel[d] = sPv[d][lb2];
                         }
-                        volume_flux<DIM, true>(qo, qb, sM[eb][1], A.P, F);
-                    }
-                } else {
-                    NodePrim<DIM> qo, qb;
-                    qo.rho = sPd[0][tid];
-                    qo.beta = sPd[1][tid];
+                        volume_flux_mx<DIM>(qa, qb, m, A.P, p_ref, h_ref, F);
+                    } else {
+                        NodePrim<DIM> qa, qb;
+                        qa.rho = sPd[0][la];
+                        qa.beta = sPd[1][la];
+                        qb.rho = sPd[0][lb2];
+                        qb.beta = sPd[1][lb2];
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) qo.vel[d] = sPv[d][tid];
-                    qb.rho = sPd[0][li];
-                    qb.beta = sPd[1][li];
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) qb.vel[d] = sPv[d][li];
-                    volume_flux<DIM, true>(qo, qb, m, A.P, F);
-                }
-            };
-            if constexpr (ROT) {
-                // each symmetric pair once: node a evaluates (a, a+k) for k <= Np/2 and receives
-                // (a-k, a) from the lane of node a-k (the same flux, F(u_i,u_j) = F(u_j,u_i));
-                // 2k = Np pairs are evaluated by both of their nodes in the last direction, and in
-                // one shared pass after this loop in directions 0 and 1 (HALF2)
-                const int lane = tid & 63;
-                const int KMAX = (HALF2 && dir < 2) ? (NP - 1) / 2 : NP / 2;
-#pragma unroll
-                for (int k = 1; k <= KMAX; k++) {
-                    const int b = (a + k) % NP;
-                    R F[NEQ];
-                    pair_flux(b, lb + b * st, F);
-                    const R dd = BLEND ? sT[a * NP + b] * oma : sT[a * NP + b];
-#pragma unroll
-                    for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
-                    if (2 * k != NP) {
-                        const int c = (a + NP - k) % NP;
-                        const int src = lane + (c - a) * st;
-                        const R dc = BLEND ? sT[a * NP + c] * oma : sT[a * NP + c];
-#pragma unroll
-                        for (int v = 0; v < NEQ; v++) acc[v] += dc * __shfl(F[v], src, 64);
+                        for (int d = 0; d < DIM; d++) {
+                            qa.vel[d] = sPv[d][la];
+                            qb.vel[d] = sPv[d][lb2];
+                        }
+                        volume_flux<DIM, true>(qa, qb, m, A.P, F);
                     }
                 }
-            } else {
-#pragma unroll 1
-                for (int k = 1; k < NP; k++) {
-                    const int b = (a + k) % NP;
-                    R F[NEQ];
-                    pair_flux(b, lb + b * st, F);
-                    const R dd = BLEND ? sT[a * NP + b] * oma : sT[a * NP + b];
+                // the direction-0 pair of this node was evaluated by the lane of (i mod H, j), the
+                // direction-1 pair by the lane that numbers (i, j mod H)
+                const int xl = lane - (i >= H ? H : 0);
+                const int jm = j < H ? j : j - H;
+                const int idy = i + NP * jm;
+                const int yl = lane - (node - n00) + (H + idy % H) + (idy / H) * NP;
+                const R dx = sT[i * NP + (i < H ? i + H : i - H)] * oma;
+                const R dy = sT[j * NP + (j < H ? j + H : j - H)] * oma;
 #pragma unroll
-                    for (int v = 0; v < NEQ; v++) acc[v] += dd * F[v];
-                }
+                for (int v = 0; v < NEQ; v++) acc[v] += dx * __shfl(F[v], xl, 64) + dy * __shfl(F[v], yl, 64);
             }
-            if (a == 0 || a == NP - 1) {
-                const int side = (a == NP - 1) ? 1 : 0;
-                const int it = eb * FITEMS + (dir * 2 + side) * L + line_index<NP>(node, dir);
+        }
+        if constexpr (BLEND) {
+            if (active) {
+                // the DG terms, (1 - alpha) volume + surface, as the RHS so far; acc then collects the
+                // RHS and the FV term is added into it
                 if constexpr (FOLD) {
-                    // du = -(1/J)(acc - accs/w0): the face term enters acc as -/+ f/w0
-                    const R sg = (R)(side ? A.T.inv_w0 : -A.T.inv_w0);
 #pragma unroll
-                    for (int v = 0; v < NEQ; v++) acc[v] += sg * sF[it][v];
+                    for (int v = 0; v < NEQ; v++) acc[v] = -acc[v] * invJ;
                 } else {
-                    const R sg = side ? R(-1) : R(1);
+                    const double al = BL ? sAlpha[eb][0] : 0.0, sw = A.T.inv_w0 * invJ;
 #pragma unroll
-                    for (int v = 0; v < NEQ; v++) accs[v] += sg * sF[it][v];
+                    for (int v = 0; v < NEQ; v++) {
+                        acc[v] = (-acc[v] * invJ) * (1.0 - al) + accs[v] * sw;
+                        accs[v] = 0.0;
+                    }
+                }
+            }
+            if constexpr (BL) {
+#pragma unroll 1
+                for (int dir = 0; dir < DIM; dir++) {
+                    __syncthreads();   // the previous readers of sF (and sMr) are done
+                    if constexpr (GEN) {
+                        load_rows(dir);
+                        __syncthreads();
+                    }
+                    const int st = dir_stride<NP>(dir);
+                    for (int it = tid; it < EPB * SITEMS; it += C::BLOCK) {
+                        const int ib = it / SITEMS, r = it - ib * SITEMS;
+                        if (blk * EPB + ib >= NE || sAlpha[ib][0] == 0.0) continue;
+                        const int i = r / L, t = r - i * L;
+                        const int nl = ib * NN + line_base<NP>(t, dir) + i * st;
+                        double sl[NEQ], sr[NEQ];
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) {
+                            sl[v] = sS[v][nl];
+                            sr[v] = sS[v][nl + st];
+                        }
+                        const SideState<DIM> ql = make_side<DIM>(sl, A.P);
+                        const SideState<DIM> qr = make_side<DIM>(sr, A.P);
+                        R mg[DIM];
+                        if constexpr (GEN) {
+                            // the subcell metric of interface i+1 from the line's per-node adj rows
+                            const int l0 = ib * NN + line_base<NP>(t, dir);
+#pragma unroll
+                            for (int c = 0; c < DIM; c++) mg[c] = 0.0;
+#pragma unroll
+                            for (int k = 0; k < NP; k++) {
+                                const double wk = ksub[i][k];
+#pragma unroll
+                                for (int c = 0; c < DIM; c++) mg[c] += wk * sMr[l0 + k * st][c];
+                            }
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < DIM; c++) mg[c] = sM[ib][dir][c];
+                        }
+                        const R *m = mg;
+                        R nmag = m[0] * m[0];
+#pragma unroll
+                        for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
+                        nmag = sqrt(nmag);
+                        R f[NEQ];
+                        face_flux<DIM, SURF>(sl, ql, sr, qr, m, nmag, A.P, A.bug3 != 0, f);
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
+                    }
+                    __syncthreads();
+                    if (active && sAlpha[eb][0] != 0.0) {
+                        const int a = (node / st) % NP;   // not ai[dir]: dir is not unrolled here
+                        const int it = eb * SITEMS + line_index<NP>(node, dir);
+                        const double sc = sAlpha[eb][0] * invJ * kinvw[a];   // alpha / (J w_a)
+#pragma unroll
+                        for (int v = 0; v < NEQ; v++) {
+                            double g = 0.0;
+                            if (a > 0) g += sF[it + (a - 1) * L][v];
+                            if (a < NP - 1) g -= sF[it + a * L][v];
+                            acc[v] += g * sc;
+                        }
+                    }
                 }
             }
         }
-        if constexpr (HALF2) {
-            // the pairs (a, a + Np/2) of directions 0 and 1, each once: a lane with i < Np/2 takes
-            // its node's direction-0 pair (i, j) - (i + Np/2, j); each other lane one
-            // direction-1 pair (i', j') - (i', j' + Np/2) with j' < Np/2, numbered
-            // (i - Np/2) + (Np/2) j = i' + Np j'.  Both nodes of a pair receive it by shuffles.
-            constexpr int H = NP / 2;
-            const int lane = tid & 63;
-            const int i = ai[0], j = ai[1];
-            const int n00 = node - i - j * NP;   // node (0, 0, k)
-            int na, nb, dl;
-            if (i < H) {
-                na = node;
-                nb = node + H;
-                dl = 0;
-            } else {
-                const int idx = (i - H) + H * j;
-                na = n00 + (idx % NP) + (idx / NP) * NP;
-                nb = na + H * NP;
-                dl = 1;
-            }
-            R F[NEQ];
-            {
-                const int la = eb * NN + na, lb2 = eb * NN + nb;
-                const R *m = sM[eb][dl];
-                if constexpr (MIX) {
-                    NodePrimMx<DIM> qa, qb;
-                    qa.rho = sPd[0][la];
-                    qa.beta = sPd[1][la];
-                    qb.rho = sPd[0][lb2];
-                    qb.beta = sPd[1][lb2];
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        qa.vel[d] = sPv[d][la];
-                        qb.vel[d] = sPv[d][lb2];
-                    }
-                    volume_flux_mx<DIM>(qa, qb, m, A.P, p_ref, h_ref, F);
-                } else {
-                    NodePrim<DIM> qa, qb;
-                    qa.rho = sPd[0][la];
-                    qa.beta = sPd[1][la];
-                    qb.rho = sPd[0][lb2];
-                    qb.beta = sPd[1][lb2];
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        qa.vel[d] = sPv[d][la];
-                        qb.vel[d] = sPv[d][lb2];
-                    }
-                    volume_flux<DIM, true>(qa, qb, m, A.P, F);
-                }
-            }
-            // the direction-0 pair of this node was evaluated by the lane of (i mod H, j), the
-            // direction-1 pair by the lane that numbers (i, j mod H)
-            const int xl = lane - (i >= H ? H : 0);
-            const int jm = j < H ? j : j - H;
-            const int idy = i + NP * jm;
-            const int yl = lane - (node - n00) + (H + idy % H) + (idy / H) * NP;
-            const R dx = sT[i * NP + (i < H ? i + H : i - H)] * oma;
-            const R dy = sT[j * NP + (j < H ? j + H : j - H)] * oma;
-#pragma unroll
-            for (int v = 0; v < NEQ; v++) acc[v] += dx * __shfl(F[v], xl, 64) + dy * __shfl(F[v], yl, 64);
-        }
-    }
-    if constexpr (BLEND) {
+        double snew[NEQ];   // the node's stage output (stage hooks)
         if (active) {
-            // the DG terms, (1 - alpha) volume + surface, as the RHS so far; acc then collects the
-            // RHS and the FV term is added into it
-            if constexpr (FOLD) {
+            const R sw = (R)(A.T.inv_w0 * invJ);
+            const R rinvJ = (R)invJ;
+            const double h_ref = MIX ? sHref[eb] : 0.0;
+            double du0 = 0.0;
 #pragma unroll
-                for (int v = 0; v < NEQ; v++) acc[v] = -acc[v] * invJ;
-            } else {
-                const double al = sAlpha[eb][0], sw = A.T.inv_w0 * invJ;
-#pragma unroll
-                for (int v = 0; v < NEQ; v++) {
-                    acc[v] = (-acc[v] * invJ) * (1.0 - al) + accs[v] * sw;
-                    accs[v] = 0.0;
+            for (int v = 0; v < NEQ; v++) {
+                R du;
+                if constexpr (BLEND) {
+                    du = acc[v];
+                } else if constexpr (FOLD) {
+                    du = -acc[v] * rinvJ;
+                } else {
+                    du = -acc[v] * rinvJ + accs[v] * sw;
                 }
+                double dud;
+                if constexpr (MIX) {
+                    // the energy sums carried F_E - h_ref F_rho: add h_ref times the inviscid mass RHS
+                    if (v == 0) du0 = (double)du;
+                    dud = (v == NEQ - 1) ? (double)du + h_ref * du0 : (double)du;
+                } else {
+                    dud = du;
+                }
+                if constexpr (VISC) {
+                    if (v > 0) {
+                        if constexpr (PRE)
+                            dud += pre_dv[v - 1];
+                        else if constexpr (MIX)
+                            dud += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
+                        else
+                            dud += A.dv[(v - 1) * vs + (size_t)e * NN + node];
+                    }
+                }
+                // the RK accumulation is FP64 in both modes
+                const size_t gi = v * vs + (size_t)e * NN + node;
+                if constexpr (PRE)
+                    snew[v] = stage_update_x<NP>(A, gi, own_state(v, eb, node), dud, pre_x[v], bad);
+                else
+                    snew[v] = stage_update<NP>(A, gi, own_state(v, eb, node), dud, bad);
             }
         }
+        if constexpr (C::BLOCK == 64 || NN == 64) {
+            // an element's nodes lie in one wave: the hooks' detection from the registers
+            if (A.hook_mode) stage_hook_detect<DIM, NP, NN>(A, active, e, node, snew);
+        }
+    };
+    if constexpr (BLEND) {
         bool any = false;
 #pragma unroll
         for (int ib = 0; ib < EPB; ib++) any |= (blk * EPB + ib < NE) && sAlpha[ib][0] != 0.0;
-        if (any) {   // the same value in every thread of the workgroup
-#pragma unroll 1
-            for (int dir = 0; dir < DIM; dir++) {
-                __syncthreads();   // the previous readers of sF (and sMr) are done
-                if constexpr (GEN) {
-                    load_rows(dir);
-                    __syncthreads();
-                }
-                const int st = dir_stride<NP>(dir);
-                for (int it = tid; it < EPB * SITEMS; it += C::BLOCK) {
-                    const int ib = it / SITEMS, r = it - ib * SITEMS;
-                    if (blk * EPB + ib >= NE || sAlpha[ib][0] == 0.0) continue;
-                    const int i = r / L, t = r - i * L;
-                    const int nl = ib * NN + line_base<NP>(t, dir) + i * st;
-                    double sl[NEQ], sr[NEQ];
-#pragma unroll
-                    for (int v = 0; v < NEQ; v++) {
-                        sl[v] = sS[v][nl];
-                        sr[v] = sS[v][nl + st];
-                    }
-                    const SideState<DIM> ql = make_side<DIM>(sl, A.P);
-                    const SideState<DIM> qr = make_side<DIM>(sr, A.P);
-                    R mg[DIM];
-                    if constexpr (GEN) {
-                        // the subcell metric of interface i+1 from the line's per-node adj rows
-                        const int l0 = ib * NN + line_base<NP>(t, dir);
-#pragma unroll
-                        for (int c = 0; c < DIM; c++) mg[c] = 0.0;
-#pragma unroll
-                        for (int k = 0; k < NP; k++) {
-                            const double wk = A.T.sub[i][k];
-#pragma unroll
-                            for (int c = 0; c < DIM; c++) mg[c] += wk * sMr[l0 + k * st][c];
-                        }
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < DIM; c++) mg[c] = sM[ib][dir][c];
-                    }
-                    const R *m = mg;
-                    R nmag = m[0] * m[0];
-#pragma unroll
-                    for (int d = 1; d < DIM; d++) nmag += m[d] * m[d];
-                    nmag = sqrt(nmag);
-                    R f[NEQ];
-                    face_flux<DIM, SURF>(sl, ql, sr, qr, m, nmag, A.P, A.bug3 != 0, f);
-#pragma unroll
-                    for (int v = 0; v < NEQ; v++) sF[it][v] = f[v];
-                }
-                __syncthreads();
-                if (active && sAlpha[eb][0] != 0.0) {
-                    const int a = (node / st) % NP;   // not ai[dir]: dir is not unrolled here
-                    const int it = eb * SITEMS + line_index<NP>(node, dir);
-                    const double sc = sAlpha[eb][0] * invJ * A.T.inv_w[a];   // alpha / (J w_a)
-#pragma unroll
-                    for (int v = 0; v < NEQ; v++) {
-                        double g = 0.0;
-                        if (a > 0) g += sF[it + (a - 1) * L][v];
-                        if (a < NP - 1) g -= sF[it + a * L][v];
-                        acc[v] += g * sc;
-                    }
-                }
-            }
-        }
-    }
-    double snew[NEQ];   // the node's stage output (stage hooks)
-    if (active) {
-        const R sw = (R)(A.T.inv_w0 * invJ);
-        const R rinvJ = (R)invJ;
-        const double h_ref = MIX ? sHref[eb] : 0.0;
-        double du0 = 0.0;
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-            R du;
-            if constexpr (BLEND) {
-                du = acc[v];
-            } else if constexpr (FOLD) {
-                du = -acc[v] * rinvJ;
-            } else {
-                du = -acc[v] * rinvJ + accs[v] * sw;
-            }
-            double dud;
-            if constexpr (MIX) {
-                // the energy sums carried F_E - h_ref F_rho: add h_ref times the inviscid mass RHS
-                if (v == 0) du0 = (double)du;
-                dud = (v == NEQ - 1) ? (double)du + h_ref * du0 : (double)du;
-            } else {
-                dud = du;
-            }
-            if constexpr (VISC) {
-                if (v > 0) {
-                    if constexpr (PRE)
-                        dud += pre_dv[v - 1];
-                    else if constexpr (MIX)
-                        dud += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
-                    else
-                        dud += A.dv[(v - 1) * vs + (size_t)e * NN + node];
-                }
-            }
-            // the RK accumulation is FP64 in both modes
-            const size_t gi = v * vs + (size_t)e * NN + node;
-            if constexpr (PRE)
-                snew[v] = stage_update_x<NP>(A, gi, own_state(v, eb, node), dud, pre_x[v], bad);
-            else
-                snew[v] = stage_update<NP>(A, gi, own_state(v, eb, node), dud, bad);
-        }
-    }
-    if constexpr (C::BLOCK == 64 || NN == 64) {
-        // an element's nodes lie in one wave: the hooks' detection from the registers
-        if (A.hook_mode) stage_hook_detect<DIM, NP, NN>(A, active, e, node, snew);
+        if (any)   // the same value in every thread of the workgroup
+            body(std::true_type{});
+        else
+            body(std::false_type{});
+    } else {
+        body(std::false_type{});
     }
     finish_block<NP>(A, A.want_lambda ? lam : 0.0, bad);
 }
