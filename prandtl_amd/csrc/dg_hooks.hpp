@@ -82,6 +82,7 @@ template <int DIM, int NP>
 struct HookCfg {
     static constexpr int NN = ipow(NP, DIM);
     static constexpr int EPB = 256 / NN > 0 ? 256 / NN : 1;   // elements per 256-thread block
+    static constexpr int FBLOCK = NN <= 64 ? 64 : 128;         // filter / limiter: one element per workgroup
 };
 
 // Per-element pass over the nodes, one thread per node, EPB elements per block:
@@ -161,16 +162,19 @@ __device__ __forceinline__ double filter_sigma(double zeta, int j) {
 // FilterSolution: one element per workgroup at a time, one thread per node; the workgroups
 // walk the list of detected elements
 template <int DIM, int NP>
-__global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
+__global__ void __launch_bounds__((HookCfg<DIM, NP>::FBLOCK)) k_filter(const HookArgs<NP> A) {
     constexpr int NEQ = DIM + 2, NN = ipow(NP, DIM);
-    static_assert(NN <= 128, "one element per workgroup of at most 128 threads");
+    constexpr int BL = HookCfg<DIM, NP>::FBLOCK, NW = BL / 64;
+    static_assert(NN <= BL && (BL == 64 || BL == 128), "one element per workgroup of one or two waves");
+    constexpr int tdepth = BL == 64 ? 6 : 7;   // steps per round: 2^tdepth - 1 lanes evaluate
     __shared__ double sU[NEQ][NN], sT[NEQ][NN];
     __shared__ double sH[NP][NEQ];      // the expansion of the node being root-found
-    __shared__ int sViol[128];          // a round's verdicts, heap-indexed
+    __shared__ double sP[NP][NP][NP];   // the 1D projections, read per lane
+    __shared__ unsigned long long sMask[NW];   // a round's verdicts, one bit per heap node
     __shared__ int s_first;
     const int i = threadIdx.x;
-    const int tdepth = 31 - __clz((int)blockDim.x);   // steps per round: 2^tdepth - 1 lanes evaluate
     const bool node_ok = i < NN;
+    for (int t = i; t < NP * NP * NP; t += BL) sP[t / (NP * NP)][(t / NP) % NP][t % NP] = A.T.P[t / (NP * NP)][(t / NP) % NP][t % NP];
     const size_t vs = (size_t)A.NE * NN;
     // the scan before this hook (on the same stream) started the counter at zero, so it holds at
     // most NE; more means the hooks of this operator ran out of order: report, change nothing
@@ -201,7 +205,7 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
             for (int v = 0; v < NEQ; v++) {
                 double a = 0.0;
 #pragma unroll
-                for (int k = 0; k < NP; k++) a += A.T.P[j][i0][k] * sU[v][i - i0 + k];
+                for (int k = 0; k < NP; k++) a += sP[j][i0][k] * sU[v][i - i0 + k];
                 t[v] = a;
             }
         }
@@ -216,7 +220,7 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
             for (int v = 0; v < NEQ; v++) {
                 double a = 0.0;
 #pragma unroll
-                for (int k = 0; k < NP; k++) a += A.T.P[j][i1][k] * sT[v][i + (k - i1) * NP];
+                for (int k = 0; k < NP; k++) a += sP[j][i1][k] * sT[v][i + (k - i1) * NP];
                 t[v] = a;
             }
         }
@@ -231,7 +235,7 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
                 for (int v = 0; v < NEQ; v++) {
                     double a = 0.0;
 #pragma unroll
-                    for (int k = 0; k < NP; k++) a += A.T.P[j][i2][k] * sT[v][i + (k - i2) * NP * NP];
+                    for (int k = 0; k < NP; k++) a += sP[j][i2][k] * sT[v][i + (k - i2) * NP * NP];
                     t[v] = a;
                 }
             }
@@ -275,6 +279,7 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
         double z1 = zeta, z2 = zmax;
         for (int done = 0; done < kFilterNIter;) {
             const int d = min(tdepth, kFilterNIter - done);
+            bool viol = false;
             if (i >= 1 && i < (1 << d)) {
                 // heap node i: follow its path from the root, halving as the bisection does
                 double a = z1, b = z2;
@@ -295,13 +300,23 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
                     for (int j = 1; j < NP; j++) acc += sH[j][v] * filter_sigma<NP>(z3, j);
                     st[v] = acc;
                 }
-                sViol[i] = filter_violates<DIM>(st, A.gamma, A.gammaM1, eb) ? 1 : 0;
+                viol = filter_violates<DIM>(st, A.gamma, A.gammaM1, eb);
             }
-            __syncthreads();
+            // the verdicts as bits (heap node n: bit n % 64 of wave n / 64), then the walk down
+            // the tree on registers
+            unsigned long long m[NW];
+            m[0] = __ballot(viol);
+            if constexpr (NW > 1) {
+                if ((i & 63) == 0) sMask[i >> 6] = m[0];
+                __syncthreads();
+#pragma unroll
+                for (int w = 0; w < NW; w++) m[w] = sMask[w];
+            }
             int nd = 1;
             for (int k = 0; k < d; k++) {
                 const double z3 = 0.5 * (z1 + z2);
-                if (sViol[nd]) {
+                const unsigned long long mw = (NW == 1 || nd < 64) ? m[0] : m[NW - 1];
+                if ((mw >> (nd & 63)) & 1ull) {
                     z1 = z3;
                     nd = 2 * nd + 1;
                 } else {
@@ -310,7 +325,7 @@ __global__ void __launch_bounds__(128) k_filter(const HookArgs<NP> A) {
                 }
             }
             done += d;
-            __syncthreads();   // sViol is rewritten by the next round
+            if constexpr (NW > 1) __syncthreads();   // sMask is rewritten by the next round
         }
         zeta = z2;
         start = first + 1;

@@ -541,7 +541,7 @@ int launch_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
     a.track_emin = 0;
     using C = HookCfg<DIM, NP>;
     const int sblk = (op->NE + C::EPB - 1) / C::EPB;
-    constexpr int ebl = C::NN <= 64 ? 64 : 128;   // one element per workgroup at a time
+    constexpr int ebl = C::FBLOCK;   // one element per workgroup at a time
     // the filter and limiter walk the scan's list of elements with a fixed grid (the limiter's
     // list is usually empty: a smaller grid costs less to launch)
     const int lblk = std::min(op->NE, 8192), lblk_lim = std::min(op->NE, 512);
