@@ -228,23 +228,26 @@ __device__ __forceinline__ int face_point_map(int code, int t) {
 }
 
 // Stage epilogue: writes the RHS (Mult) or applies the fused RK stage update to one value;
-// x is the RK register (LSRK) or the base state of the Shu-Osher combination (SSP).
+// x is the RK register (LSRK) or the base state of the Shu-Osher combination (SSP).  The outputs
+// are streamed to HBM (non-temporal): the next kernel (the lifting pass) reads the new state
+// once, and the RK register is read again only a stage later, so keeping their lines dirty in
+// L2 and the MALL would only delay the write-back into the next kernel's reads.
 template <int NP>
 __device__ __forceinline__ double stage_update_x(const RhsArgs<NP> &A, size_t gi, double u_in, double du, double x,
                                                  int &bad) {
     if (A.mode == EPI_MULT) {
-        A.out[gi] = du;
+        __builtin_nontemporal_store(du, &A.out[gi]);
         return du;
     } else if (A.mode == EPI_LSRK) {
         const double kk = A.ca * x + A.dt * du;
-        A.k[gi] = kk;
+        __builtin_nontemporal_store(kk, &A.k[gi]);
         const double un = u_in + A.cb * kk;
-        A.out[gi] = un;
+        __builtin_nontemporal_store(un, &A.out[gi]);
         bad |= !(un == un);
         return un;
     } else {
         const double un = A.ca * x + A.cb * (u_in + A.dt * du);
-        A.out[gi] = un;
+        __builtin_nontemporal_store(un, &A.out[gi]);
         bad |= !(un == un);
         return un;
     }
@@ -252,7 +255,7 @@ __device__ __forceinline__ double stage_update_x(const RhsArgs<NP> &A, size_t gi
 
 template <int NP>
 __device__ __forceinline__ double stage_update(const RhsArgs<NP> &A, size_t gi, double u_in, double du, int &bad) {
-    const double x = A.mode == EPI_LSRK ? A.k[gi] : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
+    const double x = A.mode == EPI_LSRK ? __builtin_nontemporal_load(&A.k[gi]) : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
     return stage_update_x<NP>(A, gi, u_in, du, x, bad);
 }
 
@@ -832,7 +835,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) {
                     const size_t gi = v * vs + (size_t)e * NN + node;
-                    pre_x[v] = A.mode == EPI_LSRK ? A.k[gi] : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
+                    pre_x[v] = A.mode == EPI_LSRK ? __builtin_nontemporal_load(&A.k[gi]) : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
                 }
                 if constexpr (VISC) {
 #pragma unroll
