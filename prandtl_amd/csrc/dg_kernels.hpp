@@ -842,8 +842,8 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                     for (int v = 0; v < NEQ - 1; v++) {
                         if constexpr (MIX)
                             pre_dv[v] = A.dvf[v * vs + (size_t)e * NN + node];
-                        else
-                            pre_dv[v] = A.dv[v * vs + (size_t)e * NN + node];
+                        else   // its last use (FP64; the FP32 array of the mixed mode stays cached)
+                            pre_dv[v] = __builtin_nontemporal_load(&A.dv[v * vs + (size_t)e * NN + node]);
                     }
                 }
             }
@@ -1215,7 +1215,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         else if constexpr (MIX)
                             dud += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
                         else
-                            dud += A.dv[(v - 1) * vs + (size_t)e * NN + node];
+                            dud += __builtin_nontemporal_load(&A.dv[(v - 1) * vs + (size_t)e * NN + node]);
                     }
                 }
                 // the RK accumulation is FP64 in both modes
