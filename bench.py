@@ -46,9 +46,9 @@ STAGE_BYTES_PER_NODE = 360.0
 
 # HBM bytes per launch of the dominant kernel (k_rhs_node FP64, LSRK stage, TGV 32^3 p=3),
 # measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (two runs; counter KiB x 1024, the
-# factor a plain copy of known size reads back): 214.6 MB + 167.8 MB.  Recorded in
+# factor a plain copy of known size reads back): 211.1 MB + 167.8 MB.  Recorded in
 # profiles/pmc_tgv32_p3_fp64_mixed.md; PMC counters cannot be read inside this process.
-RHS_LSRK_TRAFFIC_TGV32_P3 = 382.4e6
+RHS_LSRK_TRAFFIC_TGV32_P3 = 378.9e6
 
 
 def log(msg):

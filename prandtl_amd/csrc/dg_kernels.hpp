@@ -227,36 +227,57 @@ __device__ __forceinline__ int face_point_map(int code, int t) {
     return DIM == 2 ? u1 : u1 + NP * u2;
 }
 
+// Streamed (non-temporal) access for the arrays a kernel touches once: in the 3D kernels, whose
+// stage arrays (the state, the RK register, the viscous terms) together exceed the 256 MB MALL at
+// the sizes they run, streaming the outputs keeps their dirty lines from delaying the next
+// kernel's reads; in 2D the arrays of the cases here fit the MALL, and cached stores let the next
+// kernel find them there.  A compile-time choice: a runtime flag could let the compiler merge the
+// two forms of a store and drop the hint.
+template <bool STREAM>
+__device__ __forceinline__ void put(double *p, double v) {
+    if constexpr (STREAM)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+template <bool STREAM>
+__device__ __forceinline__ double take(const double *p) {
+    if constexpr (STREAM)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 // Stage epilogue: writes the RHS (Mult) or applies the fused RK stage update to one value;
 // x is the RK register (LSRK) or the base state of the Shu-Osher combination (SSP).  The outputs
-// are streamed to HBM (non-temporal): the next kernel (the lifting pass) reads the new state
-// once, and the RK register is read again only a stage later, so keeping their lines dirty in
-// L2 and the MALL would only delay the write-back into the next kernel's reads.
-template <int NP>
+// are streamed in 3D: the next kernel (the lifting pass) reads the new state once, and the RK
+// register is read again only a stage later.
+template <int DIM, int NP>
 __device__ __forceinline__ double stage_update_x(const RhsArgs<NP> &A, size_t gi, double u_in, double du, double x,
                                                  int &bad) {
+    constexpr bool S = DIM == 3;
     if (A.mode == EPI_MULT) {
-        __builtin_nontemporal_store(du, &A.out[gi]);
+        put<S>(&A.out[gi], du);
         return du;
     } else if (A.mode == EPI_LSRK) {
         const double kk = A.ca * x + A.dt * du;
-        __builtin_nontemporal_store(kk, &A.k[gi]);
+        put<S>(&A.k[gi], kk);
         const double un = u_in + A.cb * kk;
-        __builtin_nontemporal_store(un, &A.out[gi]);
+        put<S>(&A.out[gi], un);
         bad |= !(un == un);
         return un;
     } else {
         const double un = A.ca * x + A.cb * (u_in + A.dt * du);
-        __builtin_nontemporal_store(un, &A.out[gi]);
+        put<S>(&A.out[gi], un);
         bad |= !(un == un);
         return un;
     }
 }
 
-template <int NP>
+template <int DIM, int NP>
 __device__ __forceinline__ double stage_update(const RhsArgs<NP> &A, size_t gi, double u_in, double du, int &bad) {
-    const double x = A.mode == EPI_LSRK ? __builtin_nontemporal_load(&A.k[gi]) : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
-    return stage_update_x<NP>(A, gi, u_in, du, x, bad);
+    const double x = A.mode == EPI_LSRK ? take<DIM == 3>(&A.k[gi]) : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
+    return stage_update_x<DIM, NP>(A, gi, u_in, du, x, bad);
 }
 
 // Stage hooks detected on the stage output s (one node per lane, an element's NN nodes in
@@ -490,7 +511,7 @@ __global__ void __launch_bounds__((LineCfg<DIM, NP>::BLOCK), PRANDTL_LINE_WAVES)
             const size_t gi = v * vs + off0 + idx;
             double du = sD[v][idx];
             if (VISC && v > 0) du += A.dv[(v - 1) * vs + off0 + idx];
-            stage_update<NP>(A, gi, A.mode == EPI_MULT ? 0.0 : A.u[gi], du, bad);
+            stage_update<DIM, NP>(A, gi, A.mode == EPI_MULT ? 0.0 : A.u[gi], du, bad);
         }
     }
     finish_block<NP>(A, lam, bad);
@@ -835,7 +856,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
 #pragma unroll
                 for (int v = 0; v < NEQ; v++) {
                     const size_t gi = v * vs + (size_t)e * NN + node;
-                    pre_x[v] = A.mode == EPI_LSRK ? __builtin_nontemporal_load(&A.k[gi]) : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
+                    pre_x[v] = A.mode == EPI_LSRK ? take<DIM == 3>(&A.k[gi]) : (A.mode == EPI_SSP ? A.x[gi] : 0.0);
                 }
                 if constexpr (VISC) {
 #pragma unroll
@@ -843,7 +864,7 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         if constexpr (MIX)
                             pre_dv[v] = A.dvf[v * vs + (size_t)e * NN + node];
                         else   // its last use (FP64; the FP32 array of the mixed mode stays cached)
-                            pre_dv[v] = __builtin_nontemporal_load(&A.dv[v * vs + (size_t)e * NN + node]);
+                            pre_dv[v] = take<DIM == 3>(&A.dv[v * vs + (size_t)e * NN + node]);
                     }
                 }
             }
@@ -1215,15 +1236,15 @@ __global__ void __launch_bounds__((NodeCfg<DIM, NP>::BLOCK), PRANDTL_NODE_WAVES)
                         else if constexpr (MIX)
                             dud += A.dvf[(v - 1) * vs + (size_t)e * NN + node];
                         else
-                            dud += __builtin_nontemporal_load(&A.dv[(v - 1) * vs + (size_t)e * NN + node]);
+                            dud += take<DIM == 3>(&A.dv[(v - 1) * vs + (size_t)e * NN + node]);
                     }
                 }
                 // the RK accumulation is FP64 in both modes
                 const size_t gi = v * vs + (size_t)e * NN + node;
                 if constexpr (PRE)
-                    snew[v] = stage_update_x<NP>(A, gi, own_state(v, eb, node), dud, pre_x[v], bad);
+                    snew[v] = stage_update_x<DIM, NP>(A, gi, own_state(v, eb, node), dud, pre_x[v], bad);
                 else
-                    snew[v] = stage_update<NP>(A, gi, own_state(v, eb, node), dud, bad);
+                    snew[v] = stage_update<DIM, NP>(A, gi, own_state(v, eb, node), dud, bad);
             }
         }
         if constexpr (C::BLOCK == 64 || NN == 64) {
