@@ -167,6 +167,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// xcd_block with each XCD's range walked from its end.  The lifting pass runs between two flux
+// passes, which walk the ranges forward; walking backwards it starts on the state the previous
+// flux pass wrote last, and leaves the flux pass its freshest outputs (dv, Ff) to start on: the
+// arrays of a stage outgrow the 256 MB MALL at the sizes that matter, so what was touched last
+// is what is still there.
+__device__ __forceinline__ int xcd_block_rev(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, i = b >> 3;
+    const int qx = q + (x < r ? 1 : 0);   // element blocks of XCD x
+    return x * q + (x < r ? x : r) + (qx - 1 - i);
+}
+
 // node index of point a on line l of direction dir
 template <int NP>
 __device__ __forceinline__ constexpr int line_base(int l, int dir) {
@@ -1315,7 +1327,7 @@ __global__ void __launch_bounds__((LiftCfg<DIM, NP>::BLOCK), PRANDTL_LIFT_WAVES)
     __shared__ int sNbr[EPB][2 * DIM];    // face neighbours of the workgroup's elements
     __shared__ int sOr[GEN ? EPB : 1][2 * DIM];   // GEN: face orientation codes
 
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int blk = xcd_block_rev(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     const int eb = tid / NN, node = tid - (tid / NN) * NN;
     const int e0 = blk * EPB;
