@@ -272,7 +272,7 @@ void EntropyGrad2PrimGrad(const Phys &P, const real *state, real *g) {
     v_sq = 0.5 * v_sq;
     real p = P.gammaM1 * (state[neq - 1] - state[0] * v_sq);
     real KE = v_sq * state[0];
-    real gs[MAXEQ];
+    real gs[MAXEQ] = {};
     for (int v = 0; v < neq; v++) gs[v] = g[v];
     const real ge = gs[neq - 1];
     real mdot = 0.0;
