@@ -18,6 +18,9 @@ struct PhysConst {
     // the logarithmic mean's series constants (lm_delta): kernel arguments, so that they enter the
     // flux arithmetic as scalar operands instead of being rebuilt in vector registers at every use
     double lm_k = 7.0 / 5.0, lm_c3 = 1.0 / 3.0, lm_c7 = 1.0 / 7.0;
+    // the longer series' coefficients 1/(2k+1), k = 1..11 (lm_delta_long)
+    double lm_cl[11] = {1.0 / 3.0,  1.0 / 5.0,  1.0 / 7.0,  1.0 / 9.0,  1.0 / 11.0, 1.0 / 13.0,
+                        1.0 / 15.0, 1.0 / 17.0, 1.0 / 19.0, 1.0 / 21.0, 1.0 / 23.0};
 };
 
 // ---------------------------------------------------------------------------
@@ -102,12 +105,30 @@ __device__ __forceinline__ double lm_delta(double u, const PhysConst &P) {
     return u * fma(u, (u + P.lm_k) * P.lm_c7, P.lm_c3);
 }
 
+// The same series to u^N for 1e-4 <= u < kLmLongU: delta = sum_{k=1..N} u^k/(2k+1).  Its
+// tail sum_{k>N} u^k/(2k+1) < u^(N+1)/((2N+3)(1-u)) is below 1.1e-17 (a tenth of the
+// double's half-ulp at 1) for u < 0.05, N = 11.  In this range it is also the more accurate of
+// the two forms: ln(y/x) of a ratio near 1 turns the ratio's rounding error e into a relative
+// error e/ln(y/x), about 50 ulp at u = 1e-4.  A wave whose pairs all fall below kLmLongU skips
+// the logarithm, which a single lane with a ratio a little off 1 otherwise makes every lane wait
+// for (the density and temperature jumps of a shear layer put such pairs in most waves).
+constexpr double kLmLongU = 0.05;
+constexpr int kLmLongN = 11;
+__device__ __forceinline__ double lm_delta_long(double u, const PhysConst &P) {
+    static_assert(sizeof(P.lm_cl) / sizeof(double) == kLmLongN, "one coefficient per term");
+    double s = P.lm_cl[kLmLongN - 1];
+#pragma unroll
+    for (int k = kLmLongN - 2; k >= 0; k--) s = fma(s, u, P.lm_cl[k]);
+    return u * s;
+}
+
 // 1 / L(x, y), L the logarithmic mean, given r = 1/(x+y).
 // ref: ComputeLogMean, src/BasicOperations/BasicOperations.cpp:6-12.  With f = (y-x)/(y+x)
 // and u = f^2 (= the reference's (xi-1)^2/(xi+1)^2, xi = y/x), ln(y/x) = 2 artanh f, so
 // L = (x+y)/2 / (1 + u/3 + u^2/5 + u^3/7 + ...); the reference's series branch
-// (x+y) 52.5/(105 + u(35 + u(21 + 15u))) is that expansion to u^3.  Same branch point u < 1e-4;
-// the reciprocal form lets one 1/(x+y) serve the flux's other terms.
+// (x+y) 52.5/(105 + u(35 + u(21 + 15u))) is that expansion to u^3.  Same branch point u < 1e-4,
+// then the longer series to u < kLmLongU, the closed form beyond; the reciprocal form lets one
+// 1/(x+y) serve the flux's other terms.
 __device__ __forceinline__ double inv_log_mean(double x, double y, double r, const PhysConst &P) {
     const double f = (y - x) * r;
     const double u = f * f;
@@ -115,13 +136,16 @@ __device__ __forceinline__ double inv_log_mean(double x, double y, double r, con
     if (u < 1e-4) {
         const double r2 = 2.0 * r;
         il = fma(r2, lm_delta(u, P), r2);
+    } else if (u < kLmLongU) {
+        const double r2 = 2.0 * r;
+        il = fma(r2, lm_delta_long(u, P), r2);
     } else {
         il = log(y * rcp(x)) * rcp(y - x);
     }
     return il;
 }
 
-// 1 / L(x, y) and L(x, y) together, given r = 1/(x+y).  On the series branch
+// 1 / L(x, y) and L(x, y) together, given r = 1/(x+y).  On the short series branch
 // L = (x+y)/2 / (1 + delta), delta = u/3 + u^2/5 + u^3/7 < 3.4e-5, and 1/(1 + delta) =
 // 1 - delta (1 - delta (1 - delta)) + O(delta^4) (the omitted term is below 2e-18 relative),
 // so L needs no reciprocal there.
@@ -133,6 +157,10 @@ __device__ __forceinline__ void log_mean_both(double x, double y, double r, cons
         const double d = lm_delta(u, P), r2 = 2.0 * r;
         il = fma(r2, d, r2);
         l = 0.5 * (x + y) * fma(-d, fma(-d, 1.0 - d, 1.0), 1.0);
+    } else if (u < kLmLongU) {
+        const double r2 = 2.0 * r;
+        il = fma(r2, lm_delta_long(u, P), r2);
+        l = rcp(il);
     } else {
         il = log(y * rcp(x)) * rcp(y - x);
         l = rcp(il);

@@ -38,7 +38,8 @@ def _gpu_rhs(mesh, u, phys):
 
 
 def _errors(mesh, u, phys):
-    oph = oracle.phys(gamma=phys["gamma"], Pr=phys["Pr"], mu=phys["mu"], viscous=True)
+    oph = oracle.phys(gamma=phys.get("gamma", 1.4), Pr=phys.get("Pr", 0.72), mu=phys.get("mu", 0.0),
+                      viscous=phys.get("viscous", False))
     gpu = _gpu_rhs(mesh, u, phys)
     orc, _ = oracle.rhs(mesh, oph, u)
     orc_fma, _ = oracle.rhs(mesh, oph, u, fma=True)
@@ -68,3 +69,44 @@ def test_well_conditioned_full_size(device):
     print("[parity-summary] " + json.dumps(dict(case="TGV 32^3 p=3 NS Ma=0.3, 1% jitter (seed 5)", **e)), flush=True)
     assert e["gpu_vs_oracle"] <= 1e-12, e
     assert e["gpu_vs_truth"] <= 1e-12, e
+
+
+def _log_mean_branches(mesh, u):
+    """Fractions of the volume pairs' density logarithmic means, (a, a+k mod Np) along every line of
+    every element, with u = ((y-x)/(y+x))^2 below 1e-4, in [1e-4, 0.05) and from 0.05 up."""
+    n = mesh.order + 1
+    rho = np.asarray(u[0]).reshape((-1,) + (n,) * mesh.dim)
+    us = []
+    for ax in range(1, mesh.dim + 1):
+        for k in range(1, n):
+            b = np.roll(rho, -k, axis=ax)
+            us.append((((b - rho) / (b + rho)) ** 2).ravel())
+    us = np.concatenate(us)
+    return [float((us < 1e-4).mean()), float(((us >= 1e-4) & (us < 0.05)).mean()), float((us >= 0.05).mean())]
+
+
+@pytest.mark.parametrize("case", ["khi2d", "tgv3d"])
+def test_log_mean_ranges_against_the_exact_operator(device, case):
+    """States whose node pairs put the logarithmic means on all three of the GPU's branches (the
+    short series, the series to u^11 up to u = 0.05, the closed form ln(y/x)/(y-x) beyond), where
+    the oracle keeps the closed form from u = 1e-4 on: the GPU against the extended-precision
+    operator, at most twice the double oracle's error, and both within 1e-12."""
+    rng = np.random.default_rng(11)
+    if case == "khi2d":
+        _say("KHI 16^2 p=3 Euler, 10% jitter: every log-mean branch, against the extended-precision operator")
+        m = cases.khi_mesh(n=16, order=3)
+        u = cases.khi_state(m)
+        u = u * (1.0 + 0.1 * rng.uniform(-1, 1, u.shape))
+        phys = dict(gamma=1.4, viscous=False)
+    else:
+        _say("TGV 6^3 p=3 NS Ma=0.3, 30% jitter: every log-mean branch, against the extended-precision operator")
+        m = cases.tgv_mesh(n=6, order=3)
+        u = cases.tgv_state(m, mach=0.3)
+        u = u * (1.0 + 0.3 * rng.uniform(-1, 1, u.shape))
+        phys = cases.tgv_physics(mach=0.3)
+    fr = _log_mean_branches(m, u)
+    assert min(fr) > 0.01, fr
+    e = _errors(m, u, phys)
+    print("[parity-summary] " + json.dumps(dict(case=f"{case}: log-mean branch fractions {fr}", **e)), flush=True)
+    assert e["gpu_vs_truth"] <= 2.0 * e["oracle_vs_truth"], e
+    assert e["gpu_vs_truth"] <= 1e-12 and e["gpu_vs_oracle"] <= 1e-12, e
