@@ -79,7 +79,7 @@ def build_library(extra_flags=None, verbose=False, lib_path=None, objdir=None) -
         raise RuntimeError(f"hipcc failed on {os.path.basename(obj)} ({rc}):\n{err[-4000:]}")
     lib = lib_path or LIB
     tmp = lib + ".tmp"
-    cmd = [cc] + flags + ["-shared", "-o", tmp] + [u[1] for u in units]
+    cmd = [cc] + flags + ["-shared", "-o", tmp] + [u[1] for u in units] + ["-lrocprofiler-sdk-roctx"]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -6,8 +6,11 @@
 #include "op_internal.hpp"
 #include "dg_util.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -17,6 +20,29 @@ using namespace prandtl;
 namespace prandtl {
 
 thread_local std::string g_last_error;
+
+// roctx ranges around the library's phases (an RHS evaluation, the stage hooks, a run of RK
+// steps, a VJP, an adjoint sweep) when PRANDTL_ROCTX is set to a nonzero value; rocprofv3
+// --marker-trace records them.  They mark the host's enqueue intervals: the kernels a range
+// launched run on the stream after it.
+bool roctx_enabled() {
+    static const bool on = [] {
+        const char *v = std::getenv("PRANDTL_ROCTX");
+        return v && std::atoi(v) != 0;
+    }();
+    return on;
+}
+struct RoctxRange {
+    const bool on;
+    explicit RoctxRange(const char *name) : on(roctx_enabled()) {
+        if (on) roctxRangePushA(name);
+    }
+    ~RoctxRange() {
+        if (on) roctxRangePop();
+    }
+    RoctxRange(const RoctxRange &) = delete;
+    RoctxRange &operator=(const RoctxRange &) = delete;
+};
 
 // GLL rule on [0,1] from its closed form (0, 1 and the roots of P'_N(2x-1), weights
 // 1/(N(N+1) P_N^2)), Newton's method in long double; the reference takes it from
@@ -194,6 +220,7 @@ int dispatch_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
 }
 
 int run_stage(prandtl_op_s *op, const StageSpec &S, hipStream_t st) {
+    RoctxRange range("prandtl.rhs");
     int rc;
     if (op->timing && S.phases == 3) {
         // each pass between its own pair of events
@@ -390,6 +417,7 @@ int dispatch_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st, const VjpBu
 }
 
 int run_vjp(prandtl_op_s *op, const VjpSpec &S, hipStream_t st) {
+    RoctxRange range("prandtl.vjp");
     int rc = vjp_supported(op);
     if (rc) return rc;
     VjpBuffers B;
@@ -612,6 +640,7 @@ int run_hook(prandtl_op_s *op, int what, double *u, hipStream_t st) {
 
 // the hooks the stepper applies after a stage: filter, then limiter
 int stage_hooks(prandtl_op_s *op, double *u, hipStream_t st) {
+    RoctxRange range("prandtl.hooks");
     int rc = PRANDTL_OK;
     if (op->opts.stage_hooks & PRANDTL_HOOK_FILTER) rc = run_hook(op, HOOK_FILTER, u, st);
     if (!rc && (op->opts.stage_hooks & PRANDTL_HOOK_LIMITER)) rc = run_hook(op, HOOK_LIMITER, u, st);
@@ -633,6 +662,7 @@ const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 
 // start, advanced by dt after every step (MFEM's ODESolver::Step); each stage runs at its stage time.
 int rk_advance(prandtl_op_s *op, int scheme, double *u, double *t, double dt, int nsteps, int nan_check,
                hipStream_t st) {
+    RoctxRange range("prandtl.rk_run");
     int rc = ensure_rk_buffers(op, scheme);
     if (rc) return rc;
     if (nan_check > 0) {
@@ -1399,6 +1429,7 @@ int prandtl_rk_run_tape_schedule(prandtl_op op, int scheme, double *u, double *t
 int prandtl_rk_adjoint(prandtl_op op, double *ubar, prandtl_stream stream) {
     if (!op || !ubar) return fail(PRANDTL_ERR_ARG, "null argument");
     if (op->tape_ckpt.empty() || op->tape_id == 0) return fail(PRANDTL_ERR_ARG, "no taped run on this operator");
+    RoctxRange range("prandtl.adjoint");
     hipStream_t st = (hipStream_t)stream;
     const size_t bytes = op->size * sizeof(double);
     const int C = op->tape_every, nsteps = op->tape_steps;
