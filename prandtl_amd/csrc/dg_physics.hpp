@@ -38,20 +38,6 @@ __device__ __forceinline__ double rcp(double x) {
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// sqrt(x) for a finite x >= 0 well inside the normal range (a squared speed or sound speed): the
-// v_rsq_f64 estimate y, one Goldschmidt step on s = x y and h = y/2, one Newton correction of s on
-// the residual x - s^2 (within ~1 ulp), and 0 at x = 0.  It skips the range scaling and the second
-// correction of the library's correctly rounded square root, which these arguments do not need.
-__device__ __forceinline__ double sqrt_pos(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    double s = x * y, h = 0.5 * y;
-    const double r = fma(-s, h, 0.5);
-    s = fma(s, r, s);
-    h = fma(h, r, h);
-    s = fma(fma(-s, s, x), h, s);
-    return x == 0.0 ? 0.0 : s;
-}
-
 // Primitive state of one node as the two-point flux consumes it.
 template <int DIM>
 struct NodePrim {
@@ -82,7 +68,7 @@ __device__ __forceinline__ SideState<DIM> make_side(const double *s, const PhysC
     const double por = q.p * q.irho;
     q.por = por;
     q.beta = 0.5 * rcp(por);                                   // :67  rho / (2 p)
-    q.lam = sqrt_pos(q.vsq) + sqrt_pos(P.gamma * por);         // :61-65
+    q.lam = sqrt(q.vsq) + sqrt(P.gamma * por);                 // :61-65
     return q;
 }
 
