@@ -67,19 +67,35 @@ __host__ __device__ inline NodeQ<DIM> node_q(const double *s, double gm1) {
     return q;
 }
 
-// logarithmic mean L of (x, y), its reciprocal and its partials, given 1/x and 1/y; the two
-// branches of the forward (series in u = ((y-x)/(y+x))^2 below 1e-4, closed form otherwise, with
-// ln(y/x) taken only there).  On the series branch L = (x+y)/(2P), P = 1 + u/3 + u^2/5 + u^3/7, so 1/L =
-// 2P/(x+y) needs no division, and 1/P is itself the series 1 - u/3 - 4u^2/45 - 44u^3/945 (the
-// omitted u^4 term is below 1e-17 relative for u < 1e-4); with u = f^2, f = (y-x)/(x+y):
-// dL/dx = 1/(2P) + c y, dL/dy = 1/(2P) - c x, c = 2 P'(u) f / (P^2 (x+y)).
+// logarithmic mean L of (x, y), its reciprocal and its partials, given 1/x and 1/y; the three
+// branches of the forward, in u = ((y-x)/(y+x))^2: the series P = 1 + u/3 + u^2/5 + u^3/7 below
+// 1e-4, the series to u^11 below kLogMeanSeriesU = 0.05, the closed form with ln(y/x) beyond.  On
+// the series branches L = (x+y)/(2P), so 1/L = 2P/(x+y) needs no division; below 1e-4, 1/P is
+// itself the series 1 - u/3 - 4u^2/45 - 44u^3/945 (the omitted u^4 term is below 1e-17 relative).
+// With u = f^2, f = (y-x)/(x+y): dL/dx = 1/(2P) + c y, dL/dy = 1/(2P) - c x,
+// c = 2 P'(u) f / (P^2 (x+y)).  The closed form's partials (y-x)/x - ln(y/x) over ln(y/x)^2 cancel
+// to O(u) in the numerator, so in the series range they would lose digits that the series keeps.
+constexpr double kLogMeanSeriesU = 0.05;   // the forward's switch to the closed form
 __host__ __device__ inline void log_mean_dq(double x, double y, double ix, double iy, double &L, double &iL,
                                             double &Lx, double &Ly) {
     const double s = x + y, r = inv(s), f = (y - x) * r, u = f * f;
-    if (u < 1e-4) {
-        const double P = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
-        const double dP = 1.0 / 3.0 + u * (2.0 / 5.0 + u * (3.0 / 7.0));
-        const double iP = 1.0 - u * (1.0 / 3.0 + u * (4.0 / 45.0 + u * (44.0 / 945.0)));
+    if (u < kLogMeanSeriesU) {
+        double P, dP, iP;
+        if (u < 1e-4) {
+            P = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
+            dP = 1.0 / 3.0 + u * (2.0 / 5.0 + u * (3.0 / 7.0));
+            iP = 1.0 - u * (1.0 / 3.0 + u * (4.0 / 45.0 + u * (44.0 / 945.0)));
+        } else {
+            // P = 1 + sum_{k=1..11} u^k/(2k+1), P' = sum_{k=1..11} k u^(k-1)/(2k+1)
+            double p = 1.0 / 23.0, dp = 11.0 / 23.0;
+            for (int k = 10; k >= 1; k--) {
+                p = p * u + 1.0 / (2 * k + 1);
+                dp = dp * u + (double)k / (2 * k + 1);
+            }
+            P = 1.0 + u * p;
+            dP = dp;
+            iP = inv(P);
+        }
         L = 0.5 * s * iP;
         iL = 2.0 * r * P;
         const double c = 2.0 * dP * iP * iP * f * r;

@@ -22,6 +22,10 @@ void vjp_face(int dim, int kind, int bug3, double gamma, const double *s1, const
 }
 
 extern "C" {
+// the logarithmic mean's value, reciprocal and partials: out = (L, 1/L, dL/dx, dL/dy)
+void log_mean_partials(double x, double y, double *out) {
+    prandtl::adj::log_mean_dq(x, y, 1.0 / x, 1.0 / y, out[0], out[1], out[2], out[3]);
+}
 void vjp_entropy_vars(int dim, double gamma, const double *s, const double *w_b, double *s_b) {
     for (int i = 0; i < dim + 2; i++) s_b[i] = 0.0;
     if (dim == 2)

@@ -1,8 +1,9 @@
 """The flux vector-Jacobian products of the adjoint (prandtl_amd/csrc/dg_adjoint.hpp), built
 for the host, against central finite differences of the oracle's forward fluxes (the
 reference's ChandrashekarFlux restated): for random states, normals and cotangents F_b,
-s_b[k] = <F_b, (F(s + h e_k) - F(s - h e_k)) / 2h>.  Both log-mean branches are exercised
-(states with |(y-x)/(y+x)| above and below 1e-2)."""
+s_b[k] = <F_b, (F(s + h e_k) - F(s - h e_k)) / 2h>.  The flux tests take near and far state
+pairs (the log mean's short series and its closed form); test_log_mean_partials covers all three
+of its branches against 40-digit values."""
 import ctypes
 import os
 import subprocess
@@ -26,6 +27,7 @@ def vjp_lib(tmp_path_factory):
     L.vjp_volume.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp, dp, dp, dp, dp]
     L.vjp_face.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp, dp, dp, dp, dp]
     L.vjp_entropy_vars.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp, dp]
+    L.log_mean_partials.argtypes = [ctypes.c_double, ctypes.c_double, dp]
     L.vjp_entgrad.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp, dp, dp, dp]
     L.vjp_viscous_flux.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp, dp, dp,
                                    dp]
@@ -284,3 +286,31 @@ def test_operator_vjp_ns_host(op_vjp_lib, dim, kind):
     lhs = float((lam * (dp - dm)).sum() / (2 * eps))
     rhs = float((g * v).sum())
     assert abs(lhs - rhs) <= 1e-7 * max(abs(lhs), abs(rhs)), (lhs, rhs)
+
+
+def test_log_mean_partials(vjp_lib):
+    """The logarithmic mean L(x, y) = (y - x)/ln(y/x), its reciprocal and its partials
+    dL/dx = (L/x - 1) L/(y - x), dL/dy = (1 - L/y) L/(y - x) against 40-digit values, on all three
+    of the forward's branches in u = ((y-x)/(y+x))^2: the short series (u < 1e-4), the series to
+    u^11 (u < 0.05) and the closed form.  Relative errors within 64 ulp; the closed form alone
+    loses digits to cancellation in the partials where u is small."""
+    import mpmath
+
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(8)
+    out = np.zeros(4)
+    worst = {}
+    for target_u in (1e-7, 5e-5, 2e-4, 3e-3, 2e-2, 4.9e-2, 0.1, 0.5):
+        for _ in range(40):
+            x = float(10 ** rng.uniform(-3, 3))
+            f = float(np.sqrt(target_u) * rng.uniform(0.9, 1.0) * rng.choice([-1.0, 1.0]))
+            y = x * (1 + f) / (1 - f)
+            vjp_lib.log_mean_partials(x, y, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            X, Y = mpmath.mpf(x), mpmath.mpf(y)
+            Lm = (Y - X) / mpmath.log(Y / X)
+            ref = [Lm, 1 / Lm, (Lm / X - 1) * Lm / (Y - X), (1 - Lm / Y) * Lm / (Y - X)]
+            for k in range(4):
+                e = float(abs((mpmath.mpf(out[k]) - ref[k]) / ref[k])) / np.finfo(float).eps
+                worst[(target_u, k)] = max(worst.get((target_u, k), 0.0), e)
+    bad = {key: v for key, v in worst.items() if v > 64}
+    assert not bad, bad
