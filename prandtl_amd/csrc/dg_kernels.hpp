@@ -45,6 +45,9 @@
 
 namespace prandtl {
 
+// the adjoint differentiates the log mean on the branches the forward takes
+static_assert(adj::kLogMeanSeriesU == kLmLongU, "the forward and the adjoint switch to the closed form at one u");
+
 constexpr int ipow(int b, int e) {
     int r = 1;
     for (int i = 0; i < e; i++) r *= b;
